@@ -1,0 +1,189 @@
+"""ctypes binding of the C oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker. The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+REF_DRIVER = os.path.join(HERE, "_ref", "vcp_ref_driver")
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("n_elem", C.c_int * 3),
+        ("n_poly_grid", C.c_int),
+        ("n_poly_sol", C.c_int),
+        ("n_poly_int_rule", C.c_int),
+        ("dof_location", C.c_int),
+        ("periodic", C.c_int * 3),
+        ("mach", C.c_double), ("reynolds", C.c_double),
+        ("reynolds_length", C.c_double), ("temp_free_stream", C.c_double),
+        ("theta_sym", C.c_double), ("rel_penalty", C.c_double),
+        ("cfl", C.c_double), ("dt_synchr", C.c_double),
+        ("bug_compat", C.c_int),
+    ]
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE, "oracle"], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+        ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.POINTER(OrcParams), dp]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_sizes.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_reference.argtypes = [C.c_void_p, dp]
+        L.orc_set_restart.argtypes = [C.c_void_p, dp]
+        L.orc_set_modal.argtypes = [C.c_void_p, dp]
+        L.orc_get_modal.argtypes = [C.c_void_p, dp]
+        L.orc_get_residual.argtypes = [C.c_void_p, dp]
+        L.orc_get_nodal.argtypes = [C.c_void_p, dp]
+        L.orc_residual.argtypes = [C.c_void_p, C.c_int, dp]
+        L.orc_sync_step.argtypes = [C.c_void_p, dp]
+        L.orc_time_steps.argtypes = [C.c_void_p, dp]
+        L.orc_cg_iterations.argtypes = [C.c_void_p, ip]
+        L.orc_get_setup.argtypes = [C.c_void_p, C.c_int, dp]
+        L.orc_get_table.argtypes = [C.c_void_p, C.c_int, dp]
+        _lib = L
+    return _lib
+
+
+class Oracle:
+    """One case held by the C oracle."""
+
+    def __init__(self, case, bug_compat: bool = True):
+        L = lib()
+        p = OrcParams()
+        p.n_elem[:] = list(case.n_elem)
+        p.n_poly_grid = case.p
+        p.n_poly_sol = case.p
+        p.n_poly_int_rule = case.rule
+        p.dof_location = 2
+        p.periodic[:] = [1, 1, 1]
+        p.mach, p.reynolds, p.reynolds_length = case.mach, case.reynolds, 1.0
+        p.temp_free_stream = case.temp
+        p.theta_sym, p.rel_penalty = case.theta_sym, case.rel_penalty
+        p.cfl, p.dt_synchr = case.cfl, case.dt_synchr
+        p.bug_compat = 1 if bug_compat else 0
+        grid = np.ascontiguousarray(case.element_grid())
+        h = L.orc_create(C.byref(p), grid)
+        if not h:
+            raise RuntimeError(L.orc_last_error().decode())
+        self.h = h
+        ne, k, m = C.c_int(), C.c_int(), C.c_int()
+        L.orc_sizes(h, C.byref(ne), C.byref(k), C.byref(m))
+        self.n_elem, self.K, self.M = ne.value, k.value, m.value
+        self.n_dofs = self.K ** 3
+        self.case = case
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def _check(self, rc):
+        if rc < 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        return rc
+
+    def reference(self):
+        out = np.zeros(4)
+        lib().orc_reference(self.h, out)
+        return dict(u_ref=out[0], p_ref=out[1], rho_ref=out[2], mu=out[3])
+
+    def set_restart(self, nodal):
+        self._check(lib().orc_set_restart(self.h, np.ascontiguousarray(nodal, dtype=np.float64)))
+
+    def set_modal(self, modal):
+        lib().orc_set_modal(self.h, np.ascontiguousarray(modal, dtype=np.float64))
+
+    def modal(self):
+        out = np.zeros((self.n_elem, 5, self.n_dofs))
+        lib().orc_get_modal(self.h, out)
+        return out
+
+    def residual_array(self):
+        out = np.zeros((self.n_elem, 5, self.n_dofs))
+        lib().orc_get_residual(self.h, out)
+        return out
+
+    def nodal(self):
+        out = np.zeros((self.n_elem, self.n_dofs, 5))
+        lib().orc_get_nodal(self.h, out)
+        return out
+
+    def residual(self, monitoring=True):
+        md = np.zeros(8)
+        self._check(lib().orc_residual(self.h, 1 if monitoring else 0, md))
+        return md
+
+    def sync_step(self):
+        md = np.zeros(8)
+        self._check(lib().orc_sync_step(self.h, md))
+        return md
+
+    def time_steps(self):
+        out = np.zeros(self.n_elem)
+        self._check(lib().orc_time_steps(self.h, out))
+        return out
+
+    def cg_iterations(self):
+        out = np.zeros(self.n_elem, dtype=np.int32)
+        lib().orc_cg_iterations(self.h, out)
+        return out
+
+    def setup(self, which):
+        M, E = self.M, self.n_elem
+        shape = {0: (E, 13, M ** 3), 7: (E, 4), 8: (E, 6)}.get(which, (E, 16, M * M))
+        out = np.zeros(shape)
+        lib().orc_get_setup(self.h, which, out)
+        return out
+
+    def table(self, which):
+        K, M = self.K, self.M
+        shape = {0: (M,), 1: (M,), 2: (K, M), 3: (K, M), 4: (K,), 5: (self.case.p + 1,)}[which]
+        out = np.zeros(shape)
+        lib().orc_get_table(self.h, which, out)
+        return out
+
+
+def run_reference(param_path: str, n_steps: int, dumps=(), threads: int | None = None,
+                  timeout: float = 3600.0):
+    """Run the reference build (oracle/_ref) on a case directory; returns the
+    parsed JSON line it prints. Only available where the reference was built."""
+    import json
+    env = dict(os.environ)
+    if threads:
+        env["OMP_NUM_THREADS"] = str(threads)
+    cwd = os.path.dirname(os.path.abspath(param_path))
+    cmd = [REF_DRIVER, os.path.basename(param_path), str(n_steps)]
+    if dumps:
+        cmd.append(",".join(str(d) for d in dumps))
+    out = subprocess.run(cmd, cwd=cwd, env=env, capture_output=True, text=True,
+                         timeout=timeout, check=True).stdout
+    for line in out.splitlines():
+        if line.startswith("{\"ref_driver\""):
+            return json.loads(line)
+    raise RuntimeError("reference driver printed no result line")

@@ -1,0 +1,1117 @@
+/*
+ * vcp_oracle.c -- TEST INFRASTRUCTURE ONLY (the checker, never the product).
+ *
+ * Plain-C restatement of the VCP3D explicit DG residual and SSP-RK3
+ * synchronization step for entropy variables + Ismail-Roe, periodic blocks.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+ * this library. Every routine cites the reference file:line it follows
+ * (paths relative to VCP3D_CurviLinear/).
+ *
+ * Pinning (what fixes each function besides reading the reference):
+ *   tables      -- Gauss-Legendre vs numpy.leggauss, orthonormality, LGL
+ *                  closed forms, Lagrange partition of unity (tests).
+ *   tensor prod -- dense Kronecker products (tests).
+ *   metrics     -- affine grids: constant J = (h/2)^3, exact normals.
+ *   residual    -- reference golden vectors (oracle/_ref build), free-stream
+ *                  preservation, conservation, TGV symmetry (tests).
+ *   CG          -- numpy.linalg.solve of the assembled mass matrix (tests).
+ *   RK step     -- reference golden trajectories.
+ */
+#define _USE_MATH_DEFINES
+#include "vcp_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NVAR 5
+static const double GAM = 1.4;
+static const double PRANDTL_LAM = 0.72;              /* VCP3D_CurviLinear.hpp:175 */
+static const double RGAS = 287.058;                  /* VCP3D_CurviLinear.hpp:177 */
+static const double LAMBDA_OVER_MU = -2.0 / 3.0;     /* VCP3D_CurviLinear.hpp:186 */
+
+static char g_err[512];
+const char *orc_last_error(void) { return g_err; }
+static int fail(const char *msg) { snprintf(g_err, sizeof g_err, "%s", msg); return -1; }
+
+struct orc_case {
+  orc_params prm;
+  int ni, nj, nk, nElem;
+  int P, K, M, Kg, nDOFs, nInt, nInt2D, nGrid;
+  double pRef, uRef, rhoRef, mu;
+  /* 1D tables, operators stored row-major as (n_out x n_in) */
+  double *rInt, *wInt, *rDOF, *rGrid;
+  double *A, *D;          /* M x K : phi_dof(r_int), phi'_dof(r_int)  */
+  double *AT, *DT;        /* K x M                                  */
+  double *LD;             /* K x K : phi_mode(r_nodal)  (modal->nodal) */
+  double *VI;             /* K x K : inverse of LD       (nodal->modal) */
+  double *FMin, *FMax, *dFMin, *dFMax; /* 1 x K face vectors (also K x 1) */
+  double *w3, *w2, *basisCenter;
+  /* geometry */
+  double *vm;             /* [nElem][13][nInt] */
+  double *sm[6];          /* [nElem][16][nInt2D] iMin,iMax,jMin,jMax,kMin,kMax */
+  double *len;            /* [nElem][4]: I, J, K, volume */
+  double *aveNorm;        /* [nElem][3][3] */
+  int *nbr;               /* [nElem][6] */
+  /* state */
+  double *sol, *solOld, *res, *resMin[3], *dUdV;
+  int *cgIt;
+  int resValid, timeStep;
+  /* scratch */
+  double *work;
+};
+
+/* ------------------------------------------------------------------ utils */
+static double *dalloc(size_t n) { return (double *) calloc(n ? n : 1, sizeof(double)); }
+
+/* out[k][j][i] (+)= sum Ox[i][ii] Oy[j][jj] Oz[k][kk] in[kk][jj][ii]; the
+   sum-factorisation of TPSAGV/TPAVR/TPSAGI/TPRIF (include/TensorProduct*.hpp)
+   written as three 1D operator applications. */
+static void tp3(int nix, int niy, int niz, int nox, int noy, int noz,
+                const double *ox, const double *oy, const double *oz,
+                const double *in, double *out, int acc, double *w)
+{
+  double *t1 = w;                          /* [niz][niy][nox] */
+  double *t2 = w + (size_t) niz * niy * nox; /* [niz][noy][nox] */
+  for (int k = 0; k < niz; ++k)
+    for (int j = 0; j < niy; ++j)
+      for (int i = 0; i < nox; ++i) {
+        double s = 0.0;
+        for (int ii = 0; ii < nix; ++ii) s += ox[i * nix + ii] * in[(k * niy + j) * nix + ii];
+        t1[(k * niy + j) * nox + i] = s;
+      }
+  for (int k = 0; k < niz; ++k)
+    for (int j = 0; j < noy; ++j)
+      for (int i = 0; i < nox; ++i) {
+        double s = 0.0;
+        for (int jj = 0; jj < niy; ++jj) s += oy[j * niy + jj] * t1[(k * niy + jj) * nox + i];
+        t2[(k * noy + j) * nox + i] = s;
+      }
+  for (int k = 0; k < noz; ++k)
+    for (int j = 0; j < noy; ++j)
+      for (int i = 0; i < nox; ++i) {
+        double s = 0.0;
+        for (int kk = 0; kk < niz; ++kk) s += oz[k * niz + kk] * t2[(kk * noy + j) * nox + i];
+        if (acc) out[(k * noy + j) * nox + i] += s; else out[(k * noy + j) * nox + i] = s;
+      }
+}
+
+/* --------------------------------------------------------- 1D polynomials */
+/* Orthonormal Legendre phi_n = sqrt((2n+1)/2) P_n and its derivative; same
+   basis as NormJacobi(n,0,0,x) / GradNormJacobi (StandAloneFunctions.cpp:96-138,
+   :329-383). */
+static void legendre(int n, double x, double *p, double *dp)
+{
+  double p0 = 1.0, p1 = x, d0 = 0.0, d1 = 1.0;
+  if (n == 0) { p1 = 1.0; d1 = 0.0; }
+  for (int m = 1; m < n; ++m) {
+    const double p2 = ((2.0 * m + 1.0) * x * p1 - m * p0) / (m + 1.0);
+    const double d2 = d0 + (2.0 * m + 1.0) * p1;
+    p0 = p1; p1 = p2; d0 = d1; d1 = d2;
+  }
+  const double s = sqrt((2.0 * n + 1.0) / 2.0);
+  *p = s * p1; *dp = s * d1;
+}
+
+/* Gauss-Legendre nodes (ascending) and weights by Newton iteration; the
+   reference obtains the same rule from Golub-Welsch with alpha=beta=0
+   (StandardElementClass.cpp:662-689, GaussJacobiQuadrature.cpp). */
+static void gauss_legendre(int n, double *x, double *w)
+{
+  for (int i = 0; i < n; ++i) {
+    double z = cos(M_PI * (i + 0.75) / (n + 0.5));
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = z;
+      for (int m = 1; m < n; ++m) { double p2 = ((2.0 * m + 1.0) * z * p1 - m * p0) / (m + 1.0); p0 = p1; p1 = p2; }
+      const double pn = (n == 0) ? 1.0 : p1, pnm1 = p0;
+      const double dp = n * (z * pn - pnm1) / (z * z - 1.0);
+      const double dz = pn / dp;
+      z -= dz;
+      if (fabs(dz) < 1e-16) break;
+    }
+    double p0 = 1.0, p1 = z;
+    for (int m = 1; m < n; ++m) { double p2 = ((2.0 * m + 1.0) * z * p1 - m * p0) / (m + 1.0); p0 = p1; p1 = p2; }
+    const double dp = n * (z * p1 - p0) / (z * z - 1.0);
+    x[n - 1 - i] = z;
+    w[n - 1 - i] = 2.0 / ((1.0 - z * z) * dp * dp);
+  }
+  if (n % 2 == 1) x[n / 2] = 0.0;
+}
+
+/* DOF locations in the standard edge (StandAloneFunctions.cpp:182-323). */
+static int dof_locations(int n, int loc, double *r)
+{
+  if (loc == 1) {
+    const double dh = 2.0 / (n - 1);
+    for (int i = 0; i < n; ++i) r[i] = -1.0 + i * dh;
+    return 0;
+  }
+  switch (n) {
+    case 1: r[0] = 0.0; break;
+    case 2: r[0] = -1.0; r[1] = 1.0; break;
+    case 3: r[0] = -1.0; r[1] = 0.0; r[2] = 1.0; break;
+    case 4: r[0] = -1.0; r[1] = -sqrt(1.0 / 5.0); r[2] = sqrt(1.0 / 5.0); r[3] = 1.0; break;
+    case 5: r[0] = -1.0; r[1] = -sqrt(3.0 / 7.0); r[2] = 0.0; r[3] = sqrt(3.0 / 7.0); r[4] = 1.0; break;
+    case 6: { const double t1 = 2.0 * sqrt(7.0) / 21.0, t2 = sqrt(1.0 / 3.0 + t1), t3 = sqrt(1.0 / 3.0 - t1);
+              r[0] = -1.0; r[1] = -t2; r[2] = -t3; r[3] = t3; r[4] = t2; r[5] = 1.0; break; }
+    case 7: { const double t1 = 2.0 * sqrt(5.0 / 3.0) / 11.0, t2 = sqrt(5.0 / 11.0 + t1), t3 = sqrt(5.0 / 11.0 - t1);
+              r[0] = -1.0; r[1] = -t2; r[2] = -t3; r[3] = 0.0; r[4] = t3; r[5] = t2; r[6] = 1.0; break; }
+    default: {
+      /* roots of P'_{n-1} by Newton (LGL), symmetric */
+      r[0] = -1.0; r[n - 1] = 1.0;
+      for (int i = 1; i < n - 1; ++i) {
+        double z = -cos(M_PI * i / (n - 1));
+        for (int it = 0; it < 100; ++it) {
+          double p, dp, pm, dpm;
+          legendre(n - 1, z, &p, &dp);
+          legendre(n - 2, z, &pm, &dpm);
+          /* (1-z^2) P'' = 2 z P' - N(N+1) P for P = P_{N}, N = n-1 */
+          const double s = sqrt((2.0 * (n - 1) + 1.0) / 2.0);
+          const double P1 = dp / s, P0 = p / s;
+          const double P2 = (2.0 * z * P1 - (n - 1.0) * n * P0) / (1.0 - z * z);
+          const double dz = P1 / P2;
+          z -= dz;
+          if (fabs(dz) < 1e-16) break;
+        }
+        r[i] = z;
+      }
+      for (int i = 1; i < n / 2; ++i) { const double a = 0.5 * (r[n - 1 - i] - r[i]); r[i] = -a; r[n - 1 - i] = a; }
+      if (n % 2 == 1) r[n / 2] = 0.0;
+    }
+  }
+  return 0;
+}
+
+/* Lagrange basis l_j of nodes xn at x (value and derivative). */
+static void lagrange(int n, const double *xn, double x, double *l, double *dl)
+{
+  for (int j = 0; j < n; ++j) {
+    double v = 1.0, d = 0.0;
+    for (int m = 0; m < n; ++m) {
+      if (m == j) continue;
+      const double f = 1.0 / (xn[j] - xn[m]);
+      d = d * (x - xn[m]) * f + v * f;
+      v *= (x - xn[m]) * f;
+    }
+    if (l) l[j] = v;
+    if (dl) dl[j] = d;
+  }
+}
+
+/* Small dense inverse (Gauss-Jordan, partial pivoting); the reference uses
+   LAPACK dgetrf/dgetri (StandAloneFunctions.cpp:144-177). */
+static int invert(int n, double *a)
+{
+  int *piv = (int *) malloc(n * sizeof(int));
+  double *b = dalloc((size_t) n * n);
+  for (int i = 0; i < n; ++i) b[i * n + i] = 1.0;
+  for (int c = 0; c < n; ++c) {
+    int p = c;
+    for (int r = c + 1; r < n; ++r) if (fabs(a[r * n + c]) > fabs(a[p * n + c])) p = r;
+    if (a[p * n + c] == 0.0) { free(piv); free(b); return -1; }
+    for (int k = 0; k < n; ++k) {
+      double t = a[c * n + k]; a[c * n + k] = a[p * n + k]; a[p * n + k] = t;
+      t = b[c * n + k]; b[c * n + k] = b[p * n + k]; b[p * n + k] = t;
+    }
+    const double d = 1.0 / a[c * n + c];
+    for (int k = 0; k < n; ++k) { a[c * n + k] *= d; b[c * n + k] *= d; }
+    for (int r = 0; r < n; ++r) {
+      if (r == c) continue;
+      const double f = a[r * n + c];
+      if (f == 0.0) continue;
+      for (int k = 0; k < n; ++k) { a[r * n + k] -= f * a[c * n + k]; b[r * n + k] -= f * b[c * n + k]; }
+    }
+  }
+  memcpy(a, b, sizeof(double) * n * n);
+  free(piv); free(b);
+  return 0;
+}
+
+/* ------------------------------------------------------ standard element */
+/* StandardElementClass::DataStandardElement (StandardElementClass.cpp:105-175)
+   and its helpers :329-911. */
+static void standard_element(orc_case *c)
+{
+  const int K = c->K, M = c->M;
+  c->rInt = dalloc(M); c->wInt = dalloc(M); c->rDOF = dalloc(K); c->rGrid = dalloc(c->Kg);
+  gauss_legendre(M, c->rInt, c->wInt);
+  dof_locations(K, c->prm.dof_location, c->rDOF);
+  c->A = dalloc(M * K); c->D = dalloc(M * K); c->AT = dalloc(M * K); c->DT = dalloc(M * K);
+  for (int p = 0; p < M; ++p)
+    for (int q = 0; q < K; ++q) {
+      double v, d; legendre(q, c->rInt[p], &v, &d);
+      c->A[p * K + q] = v; c->D[p * K + q] = d;
+      c->AT[q * M + p] = v; c->DT[q * M + p] = d;
+    }
+  c->LD = dalloc(K * K); c->VI = dalloc(K * K);
+  for (int p = 0; p < K; ++p)
+    for (int q = 0; q < K; ++q) { double v, d; legendre(q, c->rDOF[p], &v, &d); c->LD[p * K + q] = v; c->VI[p * K + q] = v; }
+  invert(K, c->VI);
+  c->FMin = dalloc(K); c->FMax = dalloc(K); c->dFMin = dalloc(K); c->dFMax = dalloc(K);
+  for (int q = 0; q < K; ++q) {
+    legendre(q, -1.0, &c->FMin[q], &c->dFMin[q]);
+    legendre(q, 1.0, &c->FMax[q], &c->dFMax[q]);
+  }
+  c->w3 = dalloc(c->nInt); c->w2 = dalloc(c->nInt2D);
+  for (int k = 0; k < M; ++k) for (int j = 0; j < M; ++j) {
+    c->w2[k * M + j] = c->wInt[j] * c->wInt[k];
+    for (int i = 0; i < M; ++i) c->w3[(k * M + j) * M + i] = c->wInt[i] * c->wInt[j] * c->wInt[k];
+  }
+  c->basisCenter = dalloc(c->nDOFs);
+  double *v0 = dalloc(K);
+  for (int q = 0; q < K; ++q) { double d; legendre(q, 0.0, &v0[q], &d); }
+  for (int k = 0; k < K; ++k) for (int j = 0; j < K; ++j) for (int i = 0; i < K; ++i)
+    c->basisCenter[(k * K + j) * K + i] = v0[i] * v0[j] * v0[k];
+  free(v0);
+}
+
+/* ---------------------------------------------------------------- metrics */
+/* ElementClass::LGLDistribution (ElementClass.cpp:3406-3623): classify the
+   grid DOF spacing of one element; returns 1 LGL, 0 equidistant, -1 unclear. */
+static int lgl_distribution(const orc_case *c, const double *g)
+{
+  const int Pg = c->prm.n_poly_grid, Kg = Pg + 1;
+  double req[16], rlgl[16], ld[16];
+  dof_locations(Kg, 1, req); dof_locations(Kg, 2, rlgl);
+  int nEq = 0, nLgl = 0, nBad = 0;
+  for (int dir = 0; dir < 3; ++dir)
+    for (int a = 0; a < Kg; ++a)
+      for (int b = 0; b < Kg; ++b) {
+        ld[0] = 0.0;
+        for (int t = 1; t < Kg; ++t) {
+          int i1, i0;
+          if (dir == 0) { i1 = (a * Kg + b) * Kg + t; i0 = i1 - 1; }
+          else if (dir == 1) { i1 = (a * Kg + t) * Kg + b; i0 = i1 - Kg; }
+          else { i1 = (t * Kg + a) * Kg + b; i0 = i1 - Kg * Kg; }
+          const double dx = g[i1] - g[i0], dy = g[c->nGrid + i1] - g[c->nGrid + i0],
+                       dz = g[2 * c->nGrid + i1] - g[2 * c->nGrid + i0];
+          ld[t] = ld[t - 1] + sqrt(dx * dx + dy * dy + dz * dz);
+        }
+        const double s = 2.0 / ld[Pg];
+        double mEq = 0.0, mL = 0.0;
+        for (int t = 1; t < Pg; ++t) {
+          const double v = -1.0 + s * ld[t];
+          mEq = fmax(mEq, fabs(v - req[t])); mL = fmax(mL, fabs(v - rlgl[t]));
+        }
+        if (mEq <= mL) { ++nEq; if (mEq > 0.05) ++nBad; }
+        else { ++nLgl; if (mL > 0.05) ++nBad; }
+      }
+  if (nEq > nLgl) { if (nEq < 10 * nLgl || nEq < 10 * nBad) return -1; }
+  else { if (nLgl < 10 * nEq || nLgl < 10 * nBad) return -1; }
+  return nLgl > nEq;
+}
+
+/* FinalMetric{I,J,K}Face (ElementClass.cpp:1907-2088): unit normal, area
+   factor, inverse metric from dx/dr, dx/ds, dx/dt at one face point. */
+static void face_metric(int dir, const double d[9], double out[13])
+{
+  const double dxdr = d[0], dydr = d[1], dzdr = d[2], dxds = d[3], dyds = d[4], dzds = d[5],
+               dxdt = d[6], dydt = d[7], dzdt = d[8];
+  double nx, ny, nz;
+  if (dir == 0) { nx = dyds * dzdt - dydt * dzds; ny = dxdt * dzds - dxds * dzdt; nz = dxds * dydt - dxdt * dyds; }
+  else if (dir == 1) { nx = dydt * dzdr - dydr * dzdt; ny = dxdr * dzdt - dxdt * dzdr; nz = dxdt * dydr - dxdr * dydt; }
+  else { nx = dydr * dzds - dyds * dzdr; ny = dxds * dzdr - dxdr * dzds; nz = dxdr * dyds - dxds * dydr; }
+  const double len = sqrt(nx * nx + ny * ny + nz * nz);
+  out[0] = nx / len; out[1] = ny / len; out[2] = nz / len; out[3] = len;
+  const double Jinv = 1.0 / (dxdr * (dyds * dzdt - dzds * dydt) - dxds * (dydr * dzdt - dzdr * dydt)
+                           + dxdt * (dydr * dzds - dzdr * dyds));
+  out[4] = (dyds * dzdt - dzds * dydt) * Jinv; out[5] = (dzds * dxdt - dxds * dzdt) * Jinv;
+  out[6] = (dxds * dydt - dyds * dxdt) * Jinv; out[7] = (dzdr * dydt - dydr * dzdt) * Jinv;
+  out[8] = (dxdr * dzdt - dzdr * dxdt) * Jinv; out[9] = (dydr * dxdt - dxdr * dydt) * Jinv;
+  out[10] = (dydr * dzds - dzdr * dyds) * Jinv; out[11] = (dzdr * dxds - dxdr * dzds) * Jinv;
+  out[12] = (dxdr * dyds - dydr * dxds) * Jinv;
+}
+
+/* SolverClass::ComputeMetricTerms (SolverClass.cpp:829-1016) with
+   ElementClass::ComputeMetricTerms (ElementClass.cpp:1027-1407) and
+   ComputeLengthScales (ElementClass.cpp:927-1022). */
+static int metrics(orc_case *c, const double *grid)
+{
+  const int M = c->M, Kg = c->Kg, nInt = c->nInt, n2 = c->nInt2D, nG = c->nGrid;
+  int nLgl = 0, nEq = 0;
+  for (int e = 0; e < c->nElem; ++e) {
+    const int r = lgl_distribution(c, grid + (size_t) e * 3 * nG);
+    if (r < 0) return fail("Not able to determine the point distribution");
+    if (r) ++nLgl; else ++nEq;
+  }
+  if (nLgl && nEq) return fail("Elements with both equidistant and LGL spacing encountered");
+  dof_locations(Kg, nLgl ? 2 : 1, c->rGrid);
+  /* Lagrange tables of the grid DOFs */
+  double *Lg = dalloc(M * Kg), *dLg = dalloc(M * Kg);
+  double lf[2][16], dlf[2][16];
+  for (int p = 0; p < M; ++p) lagrange(Kg, c->rGrid, c->rInt[p], Lg + p * Kg, dLg + p * Kg);
+  for (int q = 0; q < Kg; ++q) { lf[0][q] = (q == 0); lf[1][q] = (q == Kg - 1); }
+  lagrange(Kg, c->rGrid, -1.0, NULL, dlf[0]);
+  lagrange(Kg, c->rGrid, 1.0, NULL, dlf[1]);
+
+  c->vm = dalloc((size_t) c->nElem * 13 * nInt);
+  for (int f = 0; f < 6; ++f) c->sm[f] = dalloc((size_t) c->nElem * 16 * n2);
+  c->len = dalloc((size_t) c->nElem * 4);
+  c->aveNorm = dalloc((size_t) c->nElem * 9);
+  double *xyz[4][3];
+  for (int a = 0; a < 4; ++a) for (int b = 0; b < 3; ++b) xyz[a][b] = dalloc(nInt);
+  double *w = c->work;
+  for (int e = 0; e < c->nElem; ++e) {
+    const double *g = grid + (size_t) e * 3 * nG;
+    double *vm = c->vm + (size_t) e * 13 * nInt;
+    for (int b = 0; b < 3; ++b) {
+      tp3(Kg, Kg, Kg, M, M, M, Lg, Lg, Lg, g + b * nG, xyz[0][b], 0, w);
+      tp3(Kg, Kg, Kg, M, M, M, dLg, Lg, Lg, g + b * nG, xyz[1][b], 0, w);
+      tp3(Kg, Kg, Kg, M, M, M, Lg, dLg, Lg, g + b * nG, xyz[2][b], 0, w);
+      tp3(Kg, Kg, Kg, M, M, M, Lg, Lg, dLg, g + b * nG, xyz[3][b], 0, w);
+    }
+    for (int l = 0; l < nInt; ++l) {
+      const double dxdr = xyz[1][0][l], dydr = xyz[1][1][l], dzdr = xyz[1][2][l];
+      const double dxds = xyz[2][0][l], dyds = xyz[2][1][l], dzds = xyz[2][2][l];
+      const double dxdt = xyz[3][0][l], dydt = xyz[3][1][l], dzdt = xyz[3][2][l];
+      const double J = dxdr * (dyds * dzdt - dzds * dydt) - dxds * (dydr * dzdt - dzdr * dydt)
+                     + dxdt * (dydr * dzds - dzdr * dyds);
+      if (J <= 0.0) return fail("Negative or zero Jacobians encountered");
+      const double Ji = 1.0 / J;
+      vm[0 * nInt + l] = J;
+      vm[1 * nInt + l] = Ji * (dyds * dzdt - dzds * dydt);
+      vm[2 * nInt + l] = Ji * (dzds * dxdt - dxds * dzdt);
+      vm[3 * nInt + l] = Ji * (dxds * dydt - dyds * dxdt);
+      vm[4 * nInt + l] = Ji * (dzdr * dydt - dydr * dzdt);
+      vm[5 * nInt + l] = Ji * (dxdr * dzdt - dzdr * dxdt);
+      vm[6 * nInt + l] = Ji * (dydr * dxdt - dxdr * dydt);
+      vm[7 * nInt + l] = Ji * (dydr * dzds - dzdr * dyds);
+      vm[8 * nInt + l] = Ji * (dzdr * dxds - dxdr * dzds);
+      vm[9 * nInt + l] = Ji * (dxdr * dyds - dydr * dxds);
+      vm[10 * nInt + l] = xyz[0][0][l]; vm[11 * nInt + l] = xyz[0][1][l]; vm[12 * nInt + l] = xyz[0][2][l];
+    }
+    /* faces: f = 2*dir + side */
+    for (int f = 0; f < 6; ++f) {
+      const int dir = f / 2, side = f % 2;
+      const double *fv = lf[side], *dfv = dlf[side];
+      double *sm = c->sm[f] + (size_t) e * 16 * n2;
+      double *cc = w + 4 * (size_t) nInt * 3; /* after tp3 scratch */
+      double *C[4][3];
+      for (int a = 0; a < 4; ++a) for (int b = 0; b < 3; ++b) C[a][b] = cc + (size_t) (a * 3 + b) * n2;
+      for (int b = 0; b < 3; ++b) {
+        const double *gb = g + b * nG;
+        if (dir == 0) {
+          tp3(Kg, Kg, Kg, 1, M, M, fv, Lg, Lg, gb, C[0][b], 0, w);
+          tp3(Kg, Kg, Kg, 1, M, M, dfv, Lg, Lg, gb, C[1][b], 0, w);
+          tp3(Kg, Kg, Kg, 1, M, M, fv, dLg, Lg, gb, C[2][b], 0, w);
+          tp3(Kg, Kg, Kg, 1, M, M, fv, Lg, dLg, gb, C[3][b], 0, w);
+        } else if (dir == 1) {
+          tp3(Kg, Kg, Kg, M, 1, M, Lg, fv, Lg, gb, C[0][b], 0, w);
+          tp3(Kg, Kg, Kg, M, 1, M, dLg, fv, Lg, gb, C[1][b], 0, w);
+          tp3(Kg, Kg, Kg, M, 1, M, Lg, dfv, Lg, gb, C[2][b], 0, w);
+          tp3(Kg, Kg, Kg, M, 1, M, Lg, fv, dLg, gb, C[3][b], 0, w);
+        } else {
+          tp3(Kg, Kg, Kg, M, M, 1, Lg, Lg, fv, gb, C[0][b], 0, w);
+          tp3(Kg, Kg, Kg, M, M, 1, dLg, Lg, fv, gb, C[1][b], 0, w);
+          tp3(Kg, Kg, Kg, M, M, 1, Lg, dLg, fv, gb, C[2][b], 0, w);
+          tp3(Kg, Kg, Kg, M, M, 1, Lg, Lg, dfv, gb, C[3][b], 0, w);
+        }
+      }
+      for (int l = 0; l < n2; ++l) {
+        double d[9], out[13];
+        for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) d[a * 3 + b] = C[1 + a][b][l];
+        face_metric(dir, d, out);
+        for (int q = 0; q < 13; ++q) sm[q * n2 + l] = out[q];
+        for (int b = 0; b < 3; ++b) sm[(13 + b) * n2 + l] = C[0][b][l];
+      }
+    }
+    /* length scales and averaged normals */
+    double vol = 0.0;
+    for (int l = 0; l < nInt; ++l) vol += c->w3[l] * vm[l];
+    double *ln = c->len + (size_t) e * 4;
+    ln[3] = vol;
+    for (int dir = 0; dir < 3; ++dir) {
+      const double *s0 = c->sm[2 * dir] + (size_t) e * 16 * n2, *s1 = c->sm[2 * dir + 1] + (size_t) e * 16 * n2;
+      double an[3] = {0, 0, 0}, area = 0.0;
+      for (int l = 0; l < n2; ++l) {
+        const double wt = c->w2[l];
+        for (int b = 0; b < 3; ++b) an[b] += wt * (s0[b * n2 + l] * s0[3 * n2 + l] + s1[b * n2 + l] * s1[3 * n2 + l]);
+        area += wt * (s0[3 * n2 + l] + s1[3 * n2 + l]);
+      }
+      ln[dir] = 2.0 * vol / area;
+      for (int b = 0; b < 3; ++b) c->aveNorm[(size_t) e * 9 + dir * 3 + b] = an[b] / area;
+    }
+  }
+  for (int a = 0; a < 4; ++a) for (int b = 0; b < 3; ++b) free(xyz[a][b]);
+  free(Lg); free(dLg);
+  return 0;
+}
+
+/* Structured neighbours with periodic wrap; the reference aliases the halo
+   ring to the opposite owned element (SolverClass.cpp:1689-1693, :1776-1780,
+   :1869-1873, :1956-1960, :2048-2052, :2135-2139). */
+static void connectivity(orc_case *c)
+{
+  c->nbr = (int *) malloc(sizeof(int) * 6 * c->nElem);
+  const int n[3] = {c->ni, c->nj, c->nk};
+  for (int k = 0; k < c->nk; ++k) for (int j = 0; j < c->nj; ++j) for (int i = 0; i < c->ni; ++i) {
+    const int e = i + c->ni * (j + c->nj * k);
+    const int idx[3] = {i, j, k};
+    for (int d = 0; d < 3; ++d)
+      for (int s = 0; s < 2; ++s) {
+        int t[3] = {idx[0], idx[1], idx[2]};
+        t[d] += s ? 1 : -1;
+        int ok = 1;
+        if (t[d] < 0 || t[d] >= n[d]) {
+          if (c->prm.periodic[d]) t[d] = (t[d] + n[d]) % n[d]; else ok = 0;
+        }
+        c->nbr[e * 6 + 2 * d + s] = ok ? t[0] + c->ni * (t[1] + c->nj * t[2]) : -1;
+      }
+  }
+}
+
+/* --------------------------------------------------------------- physics */
+/* Entropy variables -> rho, u, v, w, p, rE (ElementClass.cpp:4195-4206). */
+static void ent2prim(const double *V, double *rho, double *u, double *v, double *w,
+                     double *p, double *eKin)
+{
+  const double gm1 = GAM - 1.0, ov1mg = -1.0 / gm1;
+  const double V4Inv = 1.0 / V[4];
+  *u = -V4Inv * V[1]; *v = -V4Inv * V[2]; *w = -V4Inv * V[3];
+  *eKin = 0.5 * ((*u) * (*u) + (*v) * (*v) + (*w) * (*w));
+  const double s = GAM - gm1 * (V[0] - V[4] * (*eKin));
+  const double tmp = -V[4] * exp(s);
+  *rho = pow(tmp, ov1mg);
+  *p = -(*rho) * V4Inv;
+}
+
+/* ElementClass::VolumeResidual, steps 1, 3, 5, 6, 7 (ElementClass.cpp:3809-4443)
+   for entropy variables without SGS model, CML, sponge or body force. */
+static void volume_residual(orc_case *c, int e, int mon, double *mach2max)
+{
+  const int K = c->K, M = c->M, nInt = c->nInt, nD = c->nDOFs;
+  const double *sol = c->sol + (size_t) e * NVAR * nD;
+  const double *vm = c->vm + (size_t) e * 13 * nInt;
+  double *res = c->res + (size_t) e * NVAR * nD;
+  double *dUdV = c->dUdV + (size_t) e * 15 * nInt;
+  double *W = c->work + 8 * (size_t) nInt;
+  double *Var = W, *Dr = W + NVAR * nInt, *Ds = W + 2 * NVAR * nInt, *Dt = W + 3 * NVAR * nInt;
+  double *w = c->work;
+  for (int l = 0; l < NVAR; ++l) {
+    tp3(K, K, K, M, M, M, c->A, c->A, c->A, sol + l * nD, Var + l * nInt, 0, w);
+    tp3(K, K, K, M, M, M, c->D, c->A, c->A, sol + l * nD, Dr + l * nInt, 0, w);
+    tp3(K, K, K, M, M, M, c->A, c->D, c->A, sol + l * nD, Ds + l * nInt, 0, w);
+    tp3(K, K, K, M, M, M, c->A, c->A, c->D, sol + l * nD, Dt + l * nInt, 0, w);
+  }
+  /* parametric -> Cartesian gradients; bug_compat uses the metric of
+     integration point l (the variable index), ElementClass.cpp:3856-3875 */
+  for (int l = 0; l < NVAR; ++l)
+    for (int m = 0; m < nInt; ++m) {
+      const int q = c->prm.bug_compat ? l : m;
+      const double dr = Dr[l * nInt + m], ds = Ds[l * nInt + m], dt = Dt[l * nInt + m];
+      Dr[l * nInt + m] = dr * vm[1 * nInt + q] + ds * vm[4 * nInt + q] + dt * vm[7 * nInt + q];
+      Ds[l * nInt + m] = dr * vm[2 * nInt + q] + ds * vm[5 * nInt + q] + dt * vm[8 * nInt + q];
+      Dt[l * nInt + m] = dr * vm[3 * nInt + q] + ds * vm[6 * nInt + q] + dt * vm[9 * nInt + q];
+    }
+  for (int i = 0; i < NVAR * nD; ++i) res[i] = 0.0;
+  const double gm1 = GAM - 1.0;
+  const double factHF = c->mu * GAM / PRANDTL_LAM;
+  for (int m = 0; m < nInt; ++m) {
+    double V[5], rho, u, v, ww, p, eKin;
+    for (int l = 0; l < 5; ++l) V[l] = Var[l * nInt + m];
+    ent2prim(V, &rho, &u, &v, &ww, &p, &eKin);
+    const double V4Inv = 1.0 / V[4];
+    const double rE = p / gm1 + rho * eKin;
+    double gx[5], gy[5], gz[5];
+    for (int l = 0; l < 5; ++l) { gx[l] = Dr[l * nInt + m]; gy[l] = Ds[l * nInt + m]; gz[l] = Dt[l * nInt + m]; }
+    const double dudx = -V4Inv * (gx[1] + u * gx[4]), dudy = -V4Inv * (gy[1] + u * gy[4]), dudz = -V4Inv * (gz[1] + u * gz[4]);
+    const double dvdx = -V4Inv * (gx[2] + v * gx[4]), dvdy = -V4Inv * (gy[2] + v * gy[4]), dvdz = -V4Inv * (gz[2] + v * gz[4]);
+    const double dwdx = -V4Inv * (gx[3] + ww * gx[4]), dwdy = -V4Inv * (gy[3] + ww * gy[4]), dwdz = -V4Inv * (gz[3] + ww * gz[4]);
+    const double fact = V4Inv * V4Inv / gm1;
+    const double dedx = fact * gx[4], dedy = fact * gy[4], dedz = fact * gz[4];
+    /* dU/dV (ElementClass.cpp:4228-4253) */
+    const double rH = rE + p, ru = rho * u, rv = rho * v, rw = rho * ww;
+    double *du = dUdV;
+    du[0 * nInt + m] = rho; du[1 * nInt + m] = ru; du[2 * nInt + m] = rv; du[3 * nInt + m] = rw;
+    du[4 * nInt + m] = rE; du[5 * nInt + m] = ru * u + p; du[6 * nInt + m] = ru * v; du[7 * nInt + m] = ru * ww;
+    du[8 * nInt + m] = rH * u; du[9 * nInt + m] = rv * v + p; du[10 * nInt + m] = rv * ww;
+    du[11 * nInt + m] = rH * v; du[12 * nInt + m] = rw * ww + p; du[13 * nInt + m] = rH * ww;
+    du[14 * nInt + m] = rE * rH / rho + p * eKin;
+    if (mon) {
+      const double M2 = rho * (u * u + v * v + ww * ww) / (GAM * p);
+      if (M2 > *mach2max) *mach2max = M2;
+    }
+    /* fluxes (ElementClass.cpp:4314-4416) */
+    const double weight = -vm[m] * c->w3[m];
+    const double muT = c->mu;
+    const double divV = LAMBDA_OVER_MU * (dudx + dvdy + dwdz);
+    const double txx = muT * (2.0 * dudx + divV), tyy = muT * (2.0 * dvdy + divV), tzz = muT * (2.0 * dwdz + divV);
+    const double txy = muT * (dudy + dvdx), txz = muT * (dudz + dwdx), tyz = muT * (dvdz + dwdy);
+    const double qx = -factHF * dedx, qy = -factHF * dedy, qz = -factHF * dedz;
+    const double fx[5] = {ru, ru * u - txx + p, rv * u - txy, rw * u - txz, (rE + p) * u - (u * txx + v * txy + ww * txz - qx)};
+    const double fy[5] = {rv, ru * v - txy, rv * v - tyy + p, rw * v - tyz, (rE + p) * v - (u * txy + v * tyy + ww * tyz - qy)};
+    const double fz[5] = {rw, ru * ww - txz, rv * ww - tyz, rw * ww - tzz + p, (rE + p) * ww - (u * txz + v * tyz + ww * tzz - qz)};
+    for (int d = 0; d < 3; ++d) {
+      const double mx = weight * vm[(1 + 3 * d) * nInt + m], my = weight * vm[(2 + 3 * d) * nInt + m],
+                   mz = weight * vm[(3 + 3 * d) * nInt + m];
+      double *F = (d == 0) ? Dr : (d == 1) ? Ds : Dt;
+      for (int l = 0; l < 5; ++l) F[l * nInt + m] = fx[l] * mx + fy[l] * my + fz[l] * mz;
+    }
+  }
+  for (int l = 0; l < NVAR; ++l) {
+    tp3(M, M, M, K, K, K, c->DT, c->AT, c->AT, Dr + l * nInt, res + l * nD, 1, w);
+    tp3(M, M, M, K, K, K, c->AT, c->DT, c->AT, Ds + l * nInt, res + l * nD, 1, w);
+    tp3(M, M, M, K, K, K, c->AT, c->AT, c->DT, Dt + l * nInt, res + l * nD, 1, w);
+  }
+}
+
+/* Trace of sol (5 x K^3) and its parametric gradients on face f of an
+   element: TensorProductSolAndGrad{I,J,K}Face (include/TensorProductSolAndGrad*Face.hpp). */
+static void face_trace(const orc_case *c, int f, const double *sol, double *S, double *Gr, double *Gs, double *Gt)
+{
+  const int K = c->K, M = c->M, n2 = c->nInt2D, nD = c->nDOFs, dir = f / 2, side = f % 2;
+  const double *F = side ? c->FMax : c->FMin, *dF = side ? c->dFMax : c->dFMin;
+  double *w = c->work;
+  for (int l = 0; l < NVAR; ++l) {
+    const double *b = sol + l * nD;
+    if (dir == 0) {
+      tp3(K, K, K, 1, M, M, F, c->A, c->A, b, S + l * n2, 0, w);
+      tp3(K, K, K, 1, M, M, dF, c->A, c->A, b, Gr + l * n2, 0, w);
+      tp3(K, K, K, 1, M, M, F, c->D, c->A, b, Gs + l * n2, 0, w);
+      tp3(K, K, K, 1, M, M, F, c->A, c->D, b, Gt + l * n2, 0, w);
+    } else if (dir == 1) {
+      tp3(K, K, K, M, 1, M, c->A, F, c->A, b, S + l * n2, 0, w);
+      tp3(K, K, K, M, 1, M, c->D, F, c->A, b, Gr + l * n2, 0, w);
+      tp3(K, K, K, M, 1, M, c->A, dF, c->A, b, Gs + l * n2, 0, w);
+      tp3(K, K, K, M, 1, M, c->A, F, c->D, b, Gt + l * n2, 0, w);
+    } else {
+      tp3(K, K, K, M, M, 1, c->A, c->A, F, b, S + l * n2, 0, w);
+      tp3(K, K, K, M, M, 1, c->D, c->A, F, b, Gr + l * n2, 0, w);
+      tp3(K, K, K, M, M, 1, c->A, c->D, F, b, Gs + l * n2, 0, w);
+      tp3(K, K, K, M, M, 1, c->A, c->A, dF, b, Gt + l * n2, 0, w);
+    }
+  }
+}
+
+/* Lift face data into 5 x K^3 residual: TensorProductRes{I,J,K}Face
+   (include/TensorProductRes*Face.hpp), normal flux with the face vector and
+   the three symmetrising fluxes with the derivative in their slot. */
+static void face_lift(const orc_case *c, int f, const double *flux, const double *sx,
+                      const double *sy, const double *sz, double *res)
+{
+  const int K = c->K, M = c->M, n2 = c->nInt2D, nD = c->nDOFs, dir = f / 2, side = f % 2;
+  const double *F = side ? c->FMax : c->FMin, *dF = side ? c->dFMax : c->dFMin;
+  double *w = c->work;
+  const double *AT = c->AT, *DT = c->DT;
+  for (int l = 0; l < NVAR; ++l) {
+    double *r = res + l * nD;
+    if (dir == 0) {
+      tp3(1, M, M, K, K, K, F, AT, AT, flux + l * n2, r, 1, w);
+      tp3(1, M, M, K, K, K, dF, AT, AT, sx + l * n2, r, 1, w);
+      tp3(1, M, M, K, K, K, F, DT, AT, sy + l * n2, r, 1, w);
+      tp3(1, M, M, K, K, K, F, AT, DT, sz + l * n2, r, 1, w);
+    } else if (dir == 1) {
+      tp3(M, 1, M, K, K, K, AT, F, AT, flux + l * n2, r, 1, w);
+      tp3(M, 1, M, K, K, K, DT, F, AT, sx + l * n2, r, 1, w);
+      tp3(M, 1, M, K, K, K, AT, dF, AT, sy + l * n2, r, 1, w);
+      tp3(M, 1, M, K, K, K, AT, F, DT, sz + l * n2, r, 1, w);
+    } else {
+      tp3(M, M, 1, K, K, K, AT, AT, F, flux + l * n2, r, 1, w);
+      tp3(M, M, 1, K, K, K, DT, AT, F, sx + l * n2, r, 1, w);
+      tp3(M, M, 1, K, K, K, AT, DT, F, sy + l * n2, r, 1, w);
+      tp3(M, M, 1, K, K, K, AT, AT, dF, sz + l * n2, r, 1, w);
+    }
+  }
+}
+
+/* Ismail-Roe entropy-variable flux (FluxesFace.cpp:813-975) at one point;
+   n is the unit normal from L to R, area = weight * |n| factor. */
+static void ismail_roe(const double *VL, const double *VR, const double n[3], double area, double *flux)
+{
+  const double beta = 1.0 / 6.0, alphaMax = 0.0;
+  const double gam = GAM, gm1 = GAM - 1.0, ovgm1 = 1.0 / gm1;
+  const double gp1Ovg = (gam + 1.0) / gam, gm1Ovg = gm1 / gam;
+  double rL, uL, vL, wL, pL, eL, rR, uR, vR, wR, pR, eR;
+  ent2prim(VL, &rL, &uL, &vL, &wL, &pL, &eL);
+  ent2prim(VR, &rR, &uR, &vR, &wR, &pR, &eR);
+  const double dV0 = VR[0] - VL[0], dV1 = VR[1] - VL[1], dV2 = VR[2] - VL[2], dV3 = VR[3] - VL[3], dV4 = VR[4] - VL[4];
+  const double z0L = sqrt(rL / pL), z0R = sqrt(rR / pR);
+  const double z1L = uL * z0L, z1R = uR * z0R, z2L = vL * z0L, z2R = vR * z0R, z3L = wL * z0L, z3R = wR * z0R;
+  const double z4L = sqrt(rL * pL), z4R = sqrt(rR * pR);
+  const double z0A = 0.5 * (z0L + z0R), z1A = 0.5 * (z1L + z1R), z2A = 0.5 * (z2L + z2R),
+               z3A = 0.5 * (z3L + z3R), z4A = 0.5 * (z4L + z4R);
+  /* logarithmic means (FluxesFace.cpp:876-897) */
+  double zeta = z0L / z0R, f = (zeta - 1.0) / (zeta + 1.0), uu = f * f, F;
+  F = (uu < 0.01) ? 1.0 + uu / 3.0 + uu * uu / 5.0 + uu * uu * uu / 7.0 + uu * uu * uu * uu / 9.0 : log(zeta) / (2.0 * f);
+  const double z0Log = z0A / F;
+  zeta = z4L / z4R; f = (zeta - 1.0) / (zeta + 1.0); uu = f * f;
+  F = (uu < 0.01) ? 1.0 + uu / 3.0 + uu * uu / 5.0 + uu * uu * uu / 7.0 + uu * uu * uu * uu / 9.0 : log(zeta) / (2.0 * f);
+  const double z4Log = z4A / F;
+  const double oz0 = 1.0 / z0A;
+  const double rA = z0A * z4Log, p1 = oz0 * z4A, p2 = 0.5 * (gp1Ovg * z4Log / z0Log + gm1Ovg * p1);
+  const double uA = oz0 * z1A, vA = oz0 * z2A, wA = oz0 * z3A;
+  const double vn = uA * n[0] + vA * n[1] + wA * n[2];
+  const double kin = 0.5 * (uA * uA + vA * vA + wA * wA);
+  const double a2 = gam * p2 / rA, a = sqrt(a2);
+  const double H = a2 * ovgm1 + kin, E = H - p2 / rA;
+  const double ova = 1.0 / a, ova2 = 1.0 / a2;
+  double abv1 = rA * (uA * dV1 + vA * dV2 + wA * dV3);
+  double abv2 = abv1 + rA * (dV0 + H * dV4);
+  const double dr = abv1 + rA * (dV0 + E * dV4);
+  const double dru = uA * abv2 + p2 * dV1, drv = vA * abv2 + p2 * dV2, drw = wA * abv2 + p2 * dV3;
+  const double drE = H * abv1 + rA * E * (dV0 + H * dV4) + p2 * kin * dV4;
+  double lam1 = fabs(vn + a), lam2 = fabs(vn - a), lam3 = fabs(vn);
+  lam1 *= 0.5 * (1.0 + beta + alphaMax); lam2 *= 0.5 * (1.0 + beta + alphaMax); lam3 *= 0.5;
+  abv1 = 0.5 * (lam1 + lam2); abv2 = 0.5 * (lam1 - lam2);
+  const double abv3 = abv1 - lam3;
+  const double abv4 = gm1 * (kin * dr - uA * dru - vA * drv - wA * drw + drE);
+  const double abv5 = n[0] * dru + n[1] * drv + n[2] * drw - vn * dr;
+  const double abv6 = abv3 * abv4 * ova2 + abv2 * abv5 * ova;
+  const double abv7 = abv2 * abv4 * ova + abv3 * abv5;
+  flux[0] = rA * vn - (lam3 * dr + abv6);
+  flux[1] = rA * vn * uA + p1 * n[0] - (lam3 * dru + uA * abv6 + n[0] * abv7);
+  flux[2] = rA * vn * vA + p1 * n[1] - (lam3 * drv + vA * abv6 + n[1] * abv7);
+  flux[3] = rA * vn * wA + p1 * n[2] - (lam3 * drw + wA * abv6 + n[2] * abv7);
+  flux[4] = rA * vn * H - (lam3 * drE + H * abv6 + vn * abv7);
+  for (int l = 0; l < 5; ++l) flux[l] *= area;
+}
+
+/* FluxesInternalFace (FluxesFace.cpp:220-343) with ViscousFluxesFace
+   (:1290-1651) and PenAndSymFluxesSymmVar (:1152-1284). Inputs are traces
+   with parametric gradients; outputs: flux (normal flux, L->R normal) and the
+   parametric symmetrising fluxes for L and R. */
+static void internal_face_flux(const orc_case *c, double lenScale,
+                               const double *SL, double *GrL, double *GsL, double *GtL, const double *mL,
+                               const double *SR, double *GrR, double *GsR, double *GtR, const double *mR,
+                               double *flux, double *sxL, double *syL, double *szL,
+                               double *sxR, double *syR, double *szR)
+{
+  const int n2 = c->nInt2D;
+  const int bug = c->prm.bug_compat;
+  /* Cartesian gradients, averaged (FluxesFace.cpp:263-289) */
+  double *gx = c->work + 64 * (size_t) n2, *gy = gx + 5 * n2, *gz = gy + 5 * n2;
+  for (int l = 0; l < 5; ++l)
+    for (int m = 0; m < n2; ++m) {
+      const int q = bug ? (l < n2 ? l : 0) : m;
+      const int i = l * n2 + m;
+      const double xL = GrL[i] * mL[4 * n2 + q] + GsL[i] * mL[7 * n2 + q] + GtL[i] * mL[10 * n2 + q];
+      const double yL = GrL[i] * mL[5 * n2 + q] + GsL[i] * mL[8 * n2 + q] + GtL[i] * mL[11 * n2 + q];
+      const double zL = GrL[i] * mL[6 * n2 + q] + GsL[i] * mL[9 * n2 + q] + GtL[i] * mL[12 * n2 + q];
+      const double xR = GrR[i] * mR[4 * n2 + q] + GsR[i] * mR[7 * n2 + q] + GtR[i] * mR[10 * n2 + q];
+      const double yR = GrR[i] * mR[5 * n2 + q] + GsR[i] * mR[8 * n2 + q] + GtR[i] * mR[11 * n2 + q];
+      const double zR = GrR[i] * mR[6 * n2 + q] + GsR[i] * mR[9 * n2 + q] + GtR[i] * mR[12 * n2 + q];
+      gx[i] = 0.5 * (xL + xR); gy[i] = 0.5 * (yL + yR); gz[i] = 0.5 * (zL + zR);
+    }
+  const double gm1 = GAM - 1.0, ovgm1 = 1.0 / gm1;
+  const double mu = c->mu, factHF = GAM / PRANDTL_LAM;
+  const double tau = (c->P + 1.0) * (c->P + 1.0) * c->prm.rel_penalty / lenScale;
+  const double lam = LAMBDA_OVER_MU, lamp1 = lam + 1.0, lamp2 = lam + 2.0;
+  for (int m = 0; m < n2; ++m) {
+    double VL[5], VR[5], fl[5];
+    for (int l = 0; l < 5; ++l) { VL[l] = SL[l * n2 + m]; VR[l] = SR[l * n2 + m]; }
+    const double nn[3] = {mL[0 * n2 + m], mL[1 * n2 + m], mL[2 * n2 + m]};
+    const double wA = c->w2[m] * mL[3 * n2 + m];
+    ismail_roe(VL, VR, nn, wA, fl);
+    /* viscous flux on the averaged state */
+    double Va[5];
+    for (int l = 0; l < 5; ++l) Va[l] = 0.5 * (VL[l] + VR[l]);
+    const double V4Inv = 1.0 / Va[4];
+    const double u = -V4Inv * Va[1], v = -V4Inv * Va[2], w = -V4Inv * Va[3];
+    double dx[5], dy[5], dz[5];
+    for (int l = 0; l < 5; ++l) { dx[l] = gx[l * n2 + m]; dy[l] = gy[l * n2 + m]; dz[l] = gz[l * n2 + m]; }
+    const double dudx = -V4Inv * (dx[1] + u * dx[4]), dvdx = -V4Inv * (dx[2] + v * dx[4]), dwdx = -V4Inv * (dx[3] + w * dx[4]);
+    const double dudy = -V4Inv * (dy[1] + u * dy[4]), dvdy = -V4Inv * (dy[2] + v * dy[4]), dwdy = -V4Inv * (dy[3] + w * dy[4]);
+    const double dudz = -V4Inv * (dz[1] + u * dz[4]), dvdz = -V4Inv * (dz[2] + v * dz[4]), dwdz = -V4Inv * (dz[3] + w * dz[4]);
+    const double fe = V4Inv * V4Inv * ovgm1;
+    const double dedx = fe * dx[4], dedy = fe * dy[4], dedz = fe * dz[4];
+    const double nx = wA * nn[0], ny = wA * nn[1], nz = wA * nn[2];
+    const double lamDiv = LAMBDA_OVER_MU * (dudx + dvdy + dwdz);
+    const double txx = mu * (2.0 * dudx + lamDiv), tyy = mu * (2.0 * dvdy + lamDiv), tzz = mu * (2.0 * dwdz + lamDiv);
+    const double txy = mu * (dudy + dvdx), txz = mu * (dudz + dwdx), tyz = mu * (dvdz + dwdy);
+    const double fh = mu * factHF;
+    const double qx = -fh * dedx, qy = -fh * dedy, qz = -fh * dedz;
+    fl[1] -= txx * nx + txy * ny + txz * nz;
+    fl[2] -= txy * nx + tyy * ny + tyz * nz;
+    fl[3] -= txz * nx + tyz * ny + tzz * nz;
+    fl[4] -= (u * txx + v * txy + w * txz - qx) * nx + (u * txy + v * tyy + w * tyz - qy) * ny
+           + (u * txz + v * tyz + w * tzz - qz) * nz;
+    /* penalty and symmetrising fluxes (FluxesFace.cpp:1152-1284) */
+    const double pOvRho = -V4Inv, e = pOvRho * ovgm1;
+    const double dV1 = VL[1] - VR[1], dV2 = VL[2] - VR[2], dV3 = VL[3] - VR[3], dV4 = VL[4] - VR[4];
+    const double factFlux = pOvRho * c->w2[m] * mL[3 * n2 + m] * mu;
+    double sx_ = factFlux * nn[0], sy_ = factFlux * nn[1], sz_ = factFlux * nn[2];
+    const double abvE = u * u + v * v + w * w + factHF * e;
+    const double K11_11 = lamp2, K11_14 = u * lamp2, K11_22 = 1.0, K11_24 = v, K11_33 = 1.0, K11_34 = w, K11_44 = abvE + u * u * lamp1;
+    const double K22_11 = 1.0, K22_14 = u, K22_22 = lamp2, K22_24 = v * lamp2, K22_33 = 1.0, K22_34 = w, K22_44 = abvE + v * v * lamp1;
+    const double K33_11 = 1.0, K33_14 = u, K33_22 = 1.0, K33_24 = v, K33_33 = lamp2, K33_34 = w * lamp2, K33_44 = abvE + w * w * lamp1;
+    const double K12_12 = lam, K12_14 = v * lam, K12_21 = 1.0, K12_24 = u, K12_41 = v, K12_42 = u * lam, K12_44 = u * v * lamp1;
+    const double K13_13 = lam, K13_14 = w * lam, K13_31 = 1.0, K13_34 = u, K13_41 = w, K13_43 = u * lam, K13_44 = u * w * lamp1;
+    const double K23_23 = lam, K23_24 = w * lam, K23_32 = 1.0, K23_34 = v, K23_42 = w, K23_43 = v * lam, K23_44 = v * w * lamp1;
+    const double un = u * nn[0] + v * nn[1] + w * nn[2];
+    const double dVn = dV1 * nn[0] + dV2 * nn[1] + dV3 * nn[2];
+    const double ttau = tau * factFlux, abvMom = lamp1 * (dVn + un * dV4);
+    fl[1] += ttau * (abvMom * nn[0] + dV1 + u * dV4);
+    fl[2] += ttau * (abvMom * nn[1] + dV2 + v * dV4);
+    fl[3] += ttau * (abvMom * nn[2] + dV3 + w * dV4);
+    fl[4] += ttau * (lamp1 * un * dVn + u * dV1 + v * dV2 + w * dV3 + (abvE + lamp1 * un * un) * dV4);
+    const double th = 0.5 * c->prm.theta_sym;
+    sx_ *= th; sy_ *= th; sz_ *= th;
+    double FX[5], FY[5], FZ[5];
+    FX[0] = 0.0;
+    FX[1] = -(K11_11 * dV1 + K11_14 * dV4) * sx_ - (K12_12 * dV2 + K12_14 * dV4) * sy_ - (K13_13 * dV3 + K13_14 * dV4) * sz_;
+    FX[2] = -(K11_22 * dV2 + K11_24 * dV4) * sx_ - (K12_21 * dV1 + K12_24 * dV4) * sy_;
+    FX[3] = -(K11_33 * dV3 + K11_34 * dV4) * sx_ - (K13_31 * dV1 + K13_34 * dV4) * sz_;
+    FX[4] = -(K11_14 * dV1 + K11_24 * dV2 + K11_34 * dV3 + K11_44 * dV4) * sx_
+            - (K12_41 * dV1 + K12_42 * dV2 + K12_44 * dV4) * sy_ - (K13_41 * dV1 + K13_43 * dV3 + K13_44 * dV4) * sz_;
+    FY[0] = 0.0;
+    FY[1] = -(K12_21 * dV2 + K12_41 * dV4) * sx_ - (K22_11 * dV1 + K22_14 * dV4) * sy_;
+    FY[2] = -(K12_12 * dV1 + K12_42 * dV4) * sx_ - (K22_22 * dV2 + K22_24 * dV4) * sy_ - (K23_23 * dV3 + K23_24 * dV4) * sz_;
+    FY[3] = -(K22_33 * dV3 + K22_34 * dV4) * sy_ - (K23_32 * dV2 + K23_34 * dV4) * sz_;
+    FY[4] = -(K12_14 * dV1 + K12_24 * dV2 + K12_44 * dV4) * sx_
+            - (K22_14 * dV1 + K22_24 * dV2 + K22_34 * dV3 + K22_44 * dV4) * sy_ - (K23_42 * dV2 + K23_43 * dV3 + K23_44 * dV4) * sz_;
+    FZ[0] = 0.0;
+    FZ[1] = -(K13_31 * dV3 + K13_41 * dV4) * sx_ - (K33_11 * dV1 + K33_14 * dV4) * sz_;
+    FZ[2] = -(K23_32 * dV3 + K23_42 * dV4) * sy_ - (K33_22 * dV2 + K33_24 * dV4) * sz_;
+    FZ[3] = -(K13_13 * dV1 + K13_43 * dV4) * sx_ - (K23_23 * dV2 + K23_43 * dV4) * sy_ - (K33_33 * dV3 + K33_34 * dV4) * sz_;
+    FZ[4] = -(K13_14 * dV1 + K13_34 * dV3 + K13_44 * dV4) * sx_
+            - (K23_24 * dV2 + K23_34 * dV3 + K23_44 * dV4) * sy_ - (K33_14 * dV1 + K33_24 * dV2 + K33_34 * dV3 + K33_44 * dV4) * sz_;
+    for (int l = 0; l < 5; ++l) {
+      flux[l * n2 + m] = fl[l];
+      const int q = bug ? (l < n2 ? l : 0) : m;
+      /* Cartesian symmetrising flux -> parametric, L and R metrics
+         (FluxesFace.cpp:316-334) */
+      sxL[l * n2 + m] = FX[l] * mL[4 * n2 + q] + FY[l] * mL[5 * n2 + q] + FZ[l] * mL[6 * n2 + q];
+      syL[l * n2 + m] = FX[l] * mL[7 * n2 + q] + FY[l] * mL[8 * n2 + q] + FZ[l] * mL[9 * n2 + q];
+      szL[l * n2 + m] = FX[l] * mL[10 * n2 + q] + FY[l] * mL[11 * n2 + q] + FZ[l] * mL[12 * n2 + q];
+      sxR[l * n2 + m] = FX[l] * mR[4 * n2 + q] + FY[l] * mR[5 * n2 + q] + FZ[l] * mR[6 * n2 + q];
+      syR[l * n2 + m] = FX[l] * mR[7 * n2 + q] + FY[l] * mR[8 * n2 + q] + FZ[l] * mR[9 * n2 + q];
+      szR[l * n2 + m] = FX[l] * mR[10 * n2 + q] + FY[l] * mR[11 * n2 + q] + FZ[l] * mR[12 * n2 + q];
+    }
+  }
+}
+
+/* ElementClass::{I,J,K}FaceResiduals for an interior (incl. periodic) min face
+   of element e (ElementClass.cpp:2582-2852 and the J/K analogues). */
+static int face_residual(orc_case *c, int dir, int e)
+{
+  const int n2 = c->nInt2D, nD = c->nDOFs;
+  const int L = c->nbr[e * 6 + 2 * dir];
+  if (L < 0) return fail("physical boundary faces are not part of this oracle");
+  double *buf = c->work + 80 * (size_t) n2 + 64;
+  double *SL = buf, *GrL = SL + 5 * n2, *GsL = GrL + 5 * n2, *GtL = GsL + 5 * n2;
+  double *SR = GtL + 5 * n2, *GrR = SR + 5 * n2, *GsR = GrR + 5 * n2, *GtR = GsR + 5 * n2;
+  double *flux = GtR + 5 * n2, *sxL = flux + 5 * n2, *syL = sxL + 5 * n2, *szL = syL + 5 * n2;
+  double *sxR = szL + 5 * n2, *syR = sxR + 5 * n2, *szR = syR + 5 * n2;
+  face_trace(c, 2 * dir, c->sol + (size_t) e * NVAR * nD, SR, GrR, GsR, GtR);
+  face_trace(c, 2 * dir + 1, c->sol + (size_t) L * NVAR * nD, SL, GrL, GsL, GtL);
+  const double lenScale = fmin(c->len[(size_t) e * 4 + dir], c->len[(size_t) L * 4 + dir]);
+  const double *mL = c->sm[2 * dir + 1] + (size_t) L * 16 * n2;
+  const double *mR = c->sm[2 * dir] + (size_t) e * 16 * n2;
+  internal_face_flux(c, lenScale, SL, GrL, GsL, GtL, mL, SR, GrR, GsR, GtR, mR, flux, sxL, syL, szL, sxR, syR, szR);
+  double *rmin = c->resMin[dir] + (size_t) e * NVAR * nD;
+  for (int i = 0; i < NVAR * nD; ++i) rmin[i] = 0.0;
+  face_lift(c, 2 * dir + 1, flux, sxL, syL, szL, rmin);
+  for (int i = 0; i < 5 * n2; ++i) flux[i] = -flux[i];
+  face_lift(c, 2 * dir, flux, sxR, syR, szR, c->res + (size_t) e * NVAR * nD);
+  return 0;
+}
+
+/* CG_EntropyVarMassMatrix (CG_MassMatrix.cpp:250-504). */
+static int cg_entropy(orc_case *c, int e)
+{
+  const int K = c->K, M = c->M, nD = c->nDOFs, nInt = c->nInt;
+  const double *sol = c->sol + (size_t) e * NVAR * nD;
+  const double *Jac = c->vm + (size_t) e * 13 * nInt;
+  const double *dUdV = c->dUdV + (size_t) e * 15 * nInt;
+  double *rv = c->res + (size_t) e * NVAR * nD;
+  double *z = c->work + 8 * (size_t) nInt, *p = z + 5 * nD, *Ap = p + 5 * nD, *x = Ap + 5 * nD, *pI = x + 5 * nD;
+  double *w = c->work;
+  const double JacInv = 8.0 / c->len[(size_t) e * 4 + 3];
+  double r0 = 0.0;
+  for (int i = 0; i < 5 * nD; ++i) { r0 = fmax(r0, fabs(rv[i])); x[i] = 0.0; }
+  c->cgIt[e] = 0;
+  if (r0 * JacInv > 1e-10) {
+    double V[5] = {0, 0, 0, 0, 0};
+    for (int l = 0; l < 5; ++l) for (int i = 0; i < nD; ++i) V[l] += c->basisCenter[i] * sol[l * nD + i];
+    double rho, u, v, ww, pr, eKin;
+    ent2prim(V, &rho, &u, &v, &ww, &pr, &eKin);
+    const double gm1 = GAM - 1.0, govgm1 = GAM / gm1, pInv = 1.0 / pr;
+    const double a1 = gm1 * rho * pInv * pInv, a2 = eKin * a1;
+    double dv[15];
+    dv[0] = JacInv * (govgm1 / rho + a2 * eKin); dv[1] = -JacInv * u * a2; dv[2] = -JacInv * v * a2;
+    dv[3] = -JacInv * ww * a2; dv[4] = JacInv * (a2 - pInv); dv[5] = JacInv * (a1 * u * u + pInv);
+    dv[6] = JacInv * a1 * u * v; dv[7] = JacInv * a1 * u * ww; dv[8] = -JacInv * a1 * u;
+    dv[9] = JacInv * (a1 * v * v + pInv); dv[10] = JacInv * a1 * v * ww; dv[11] = -JacInv * a1 * v;
+    dv[12] = JacInv * (a1 * ww * ww + pInv); dv[13] = -JacInv * a1 * ww; dv[14] = JacInv * a1;
+    static const int S[5][5] = {{0, 1, 2, 3, 4}, {1, 5, 6, 7, 8}, {2, 6, 9, 10, 11}, {3, 7, 10, 12, 13}, {4, 8, 11, 13, 14}};
+#define PRECOND()                                                                  \
+    for (int i = 0; i < nD; ++i)                                                   \
+      for (int a = 0; a < 5; ++a) {                                                \
+        double s = 0.0;                                                            \
+        for (int b = 0; b < 5; ++b) s += dv[S[a][b]] * rv[b * nD + i];             \
+        z[a * nD + i] = s;                                                         \
+      }
+    PRECOND();
+    for (int i = 0; i < 5 * nD; ++i) p[i] = z[i];
+    for (int it = 0;; ++it) {
+      if (it == 100) return fail("Convergence not reached for CG algorithm");
+      double rz = 0.0;
+      for (int i = 0; i < 5 * nD; ++i) rz += rv[i] * z[i];
+      for (int l = 0; l < 5; ++l) tp3(K, K, K, M, M, M, c->A, c->A, c->A, p + l * nD, pI + l * nInt, 0, w);
+      for (int m = 0; m < nInt; ++m) {
+        const double wt = c->w3[m] * Jac[m];
+        double q[5], o[5];
+        for (int l = 0; l < 5; ++l) q[l] = wt * pI[l * nInt + m];
+        for (int a = 0; a < 5; ++a) {
+          double s = 0.0;
+          for (int b = 0; b < 5; ++b) s += dUdV[S[a][b] * nInt + m] * q[b];
+          o[a] = s;
+        }
+        for (int l = 0; l < 5; ++l) pI[l * nInt + m] = o[l];
+      }
+      for (int l = 0; l < 5; ++l) tp3(M, M, M, K, K, K, c->AT, c->AT, c->AT, pI + l * nInt, Ap + l * nD, 0, w);
+      double pAp = 0.0;
+      for (int i = 0; i < 5 * nD; ++i) pAp += p[i] * Ap[i];
+      const double alpha = rz / pAp;
+      double rmax = 0.0;
+      for (int i = 0; i < 5 * nD; ++i) { x[i] += alpha * p[i]; rv[i] -= alpha * Ap[i]; rmax = fmax(rmax, fabs(rv[i])); }
+      c->cgIt[e] = it + 1;
+      if (rmax < 1e-10 * r0) break;
+      PRECOND();
+      double zAp = 0.0;
+      for (int i = 0; i < 5 * nD; ++i) zAp += z[i] * Ap[i];
+      const double beta = -alpha * zAp / rz;
+      for (int i = 0; i < 5 * nD; ++i) p[i] = beta * p[i] + z[i];
+    }
+#undef PRECOND
+  }
+  for (int i = 0; i < 5 * nD; ++i) rv[i] = x[i];
+  return 0;
+}
+
+/* ----------------------------------------------------------------- API */
+orc_case *orc_create(const orc_params *prm, const double *elem_grid)
+{
+  if (prm->n_poly_int_rule < 2 * prm->n_poly_sol) { fail("integration rule not accurate enough"); return NULL; }
+  orc_case *c = (orc_case *) calloc(1, sizeof(orc_case));
+  c->prm = *prm;
+  c->ni = prm->n_elem[0]; c->nj = prm->n_elem[1]; c->nk = prm->n_elem[2];
+  c->nElem = c->ni * c->nj * c->nk;
+  c->P = prm->n_poly_sol; c->K = c->P + 1; c->M = prm->n_poly_int_rule / 2 + 1;
+  c->Kg = prm->n_poly_grid + 1; c->nGrid = c->Kg * c->Kg * c->Kg;
+  c->nDOFs = c->K * c->K * c->K; c->nInt = c->M * c->M * c->M; c->nInt2D = c->M * c->M;
+  /* reference conditions for an all-periodic block (InputParamClass.cpp:605-758) */
+  const double muDim = 1.716e-5;
+  const double vInf = prm->mach * sqrt(GAM * RGAS * prm->temp_free_stream);
+  c->rhoRef = prm->reynolds * muDim / (prm->reynolds_length * vInf);
+  c->pRef = c->rhoRef * RGAS * prm->temp_free_stream;
+  c->uRef = sqrt(c->pRef / c->rhoRef);
+  c->mu = muDim * c->uRef / c->pRef;
+  size_t big = (size_t) c->nInt; if ((size_t) c->nGrid > big) big = c->nGrid;
+  c->work = dalloc(200 * big + 400 * (size_t) c->nInt2D + 4096);
+  standard_element(c);
+  if (metrics(c, elem_grid)) { orc_destroy(c); return NULL; }
+  connectivity(c);
+  const size_t ns = (size_t) c->nElem * NVAR * c->nDOFs;
+  c->sol = dalloc(ns); c->solOld = dalloc(ns); c->res = dalloc(ns);
+  for (int d = 0; d < 3; ++d) c->resMin[d] = dalloc(ns);
+  c->dUdV = dalloc((size_t) c->nElem * 15 * c->nInt);
+  c->cgIt = (int *) calloc(c->nElem, sizeof(int));
+  return c;
+}
+
+void orc_destroy(orc_case *c)
+{
+  if (!c) return;
+  double *ptrs[] = {c->rInt, c->wInt, c->rDOF, c->rGrid, c->A, c->D, c->AT, c->DT, c->LD, c->VI,
+                    c->FMin, c->FMax, c->dFMin, c->dFMax, c->w3, c->w2, c->basisCenter, c->vm,
+                    c->sm[0], c->sm[1], c->sm[2], c->sm[3], c->sm[4], c->sm[5], c->len, c->aveNorm,
+                    c->sol, c->solOld, c->res, c->resMin[0], c->resMin[1], c->resMin[2], c->dUdV, c->work};
+  for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
+  free(c->nbr); free(c->cgIt); free(c);
+}
+
+void orc_sizes(const orc_case *c, int *n_elem, int *n_dofs1d, int *n_int1d)
+{
+  *n_elem = c->nElem; *n_dofs1d = c->K; *n_int1d = c->M;
+}
+
+void orc_reference(const orc_case *c, double out[4])
+{
+  out[0] = c->uRef; out[1] = c->pRef; out[2] = c->rhoRef; out[3] = c->mu;
+}
+
+/* SolverClass::ReadRestartSolution, entropy branch (SolverClass.cpp:3300-3372). */
+int orc_set_restart(orc_case *c, const double *nodal)
+{
+  const int K = c->K, nD = c->nDOFs;
+  const double gm1 = GAM - 1.0, ovgm1 = 1.0 / gm1;
+  double *node = dalloc(5 * (size_t) nD);
+  for (int e = 0; e < c->nElem; ++e) {
+    for (int d = 0; d < nD; ++d) {
+      const double *s = nodal + ((size_t) e * nD + d) * 5;
+      double rho = s[0];
+      const double ri = 1.0 / rho;
+      double u = ri * s[1], v = ri * s[2], w = ri * s[3];
+      double p = gm1 * (s[4] - 0.5 * (u * s[1] + v * s[2] + w * s[3]));
+      rho /= c->rhoRef; u /= c->uRef; v /= c->uRef; w /= c->uRef; p /= c->pRef;
+      const double sEnt = log(p / pow(rho, GAM)), pInv = 1.0 / p;
+      node[0 * nD + d] = (GAM - sEnt) * ovgm1 - 0.5 * rho * (u * u + v * v + w * w) * pInv;
+      node[1 * nD + d] = rho * u * pInv; node[2 * nD + d] = rho * v * pInv;
+      node[3 * nD + d] = rho * w * pInv; node[4 * nD + d] = -rho * pInv;
+    }
+    for (int l = 0; l < 5; ++l)
+      tp3(K, K, K, K, K, K, c->VI, c->VI, c->VI, node + l * nD, c->sol + ((size_t) e * 5 + l) * nD, 0, c->work);
+  }
+  free(node);
+  c->resValid = 0;
+  return 0;
+}
+
+void orc_set_modal(orc_case *c, const double *modal)
+{
+  memcpy(c->sol, modal, sizeof(double) * (size_t) c->nElem * 5 * c->nDOFs);
+  c->resValid = 0;
+}
+
+void orc_get_modal(const orc_case *c, double *modal)
+{
+  memcpy(modal, c->sol, sizeof(double) * (size_t) c->nElem * 5 * c->nDOFs);
+}
+
+void orc_get_residual(const orc_case *c, double *res)
+{
+  memcpy(res, c->res, sizeof(double) * (size_t) c->nElem * 5 * c->nDOFs);
+}
+
+/* SolverClass::WriteRestartSolution solution branch (SolverClass.cpp:4476-4560). */
+void orc_get_nodal(const orc_case *c, double *nodal)
+{
+  const int K = c->K, nD = c->nDOFs;
+  double *node = dalloc(5 * (size_t) nD);
+  const double momRef = c->rhoRef * c->uRef;
+  for (int e = 0; e < c->nElem; ++e) {
+    for (int l = 0; l < 5; ++l)
+      tp3(K, K, K, K, K, K, c->LD, c->LD, c->LD, c->sol + ((size_t) e * 5 + l) * nD, node + l * nD, 0, c->work);
+    for (int d = 0; d < nD; ++d) {
+      double V[5], rho, u, v, w, p, eKin;
+      for (int l = 0; l < 5; ++l) V[l] = node[l * nD + d];
+      ent2prim(V, &rho, &u, &v, &w, &p, &eKin);
+      double *o = nodal + ((size_t) e * nD + d) * 5;
+      o[0] = c->rhoRef * rho; o[1] = momRef * rho * u; o[2] = momRef * rho * v; o[3] = momRef * rho * w;
+      o[4] = c->pRef * (rho * eKin + p / (GAM - 1.0));
+    }
+  }
+  free(node);
+}
+
+/* SolverClass::Residual (SolverClass.cpp:3381-3795). */
+int orc_residual(orc_case *c, int mon, double md[8])
+{
+  double mach2 = 0.0;
+  for (int e = 0; e < c->nElem; ++e) volume_residual(c, e, mon, &mach2);
+  for (int d = 0; d < 3; ++d)
+    for (int e = 0; e < c->nElem; ++e)
+      if (face_residual(c, d, e)) return -1;
+  const size_t nv = (size_t) NVAR * c->nDOFs;
+  for (int d = 0; d < 3; ++d)
+    for (int e = 0; e < c->nElem; ++e) {
+      const int R = c->nbr[e * 6 + 2 * d + 1];
+      if (R < 0) continue;
+      double *r = c->res + e * nv;
+      const double *s = c->resMin[d] + R * nv;
+      for (size_t i = 0; i < nv; ++i) r[i] += s[i];
+    }
+  for (int e = 0; e < c->nElem; ++e)
+    if (cg_entropy(c, e)) return -1;
+  for (int i = 0; i < 6; ++i) md[i] = 0.0;
+  md[6] = mach2; md[7] = mon ? 0.0 : -1.0;
+  c->resValid = 1;
+  return 0;
+}
+
+/* ElementClass::ComputeTimeStep (ElementClass.cpp:1522-1644). */
+int orc_time_steps(orc_case *c, double *dt)
+{
+  static const double f1[] = {2.0, 6.0, 12.0, 20.0, 28.0, 38.0, 50.0, 64.0, 80.0, 98.0};
+  static const double f2[] = {6.0, 36.0, 150.0, 420.0, 980.0, 1975.0, 3575.0, 7000.0, 14000.0, 28000.0};
+  const int K = c->K, nD = c->nDOFs;
+  const double muDim = c->mu * c->pRef / c->uRef;
+  double *node = dalloc(5 * (size_t) nD);
+  double rvl = fmax(fmax(1.0, 2.0 + LAMBDA_OVER_MU), GAM / PRANDTL_LAM);
+  for (int e = 0; e < c->nElem; ++e) {
+    const double *ln = c->len + (size_t) e * 4, *an = c->aveNorm + (size_t) e * 9;
+    const double di = 1.0 / ln[0], dj = 1.0 / ln[1], dk = 1.0 / ln[2];
+    const double dl2 = di * di + dj * dj + dk * dk;
+    const double radVisOverNu = rvl * dl2;
+    for (int l = 0; l < 5; ++l)
+      tp3(K, K, K, K, K, K, c->LD, c->LD, c->LD, c->sol + ((size_t) e * 5 + l) * nD, node + l * nD, 0, c->work);
+    double radInvMax = 0.0, radVisMax = 0.0, a2Min = 1e10;
+    for (int d = 0; d < nD; ++d) {
+      double V[5], rho, u, v, w, p, eKin;
+      for (int l = 0; l < 5; ++l) V[l] = node[l * nD + d];
+      ent2prim(V, &rho, &u, &v, &w, &p, &eKin);
+      rho *= c->rhoRef; u *= c->uRef; v *= c->uRef; w *= c->uRef; p *= c->pRef;
+      const double ri = 1.0 / rho, a2 = GAM * p * ri;
+      const double vI = u * an[0] + v * an[1] + w * an[2];
+      const double vJ = u * an[3] + v * an[4] + w * an[5];
+      const double vK = u * an[6] + v * an[7] + w * an[8];
+      const double a = sqrt(fabs(a2));
+      const double ri_ = (fabs(vI) + a) * di + (fabs(vJ) + a) * dj + (fabs(vK) + a) * dk;
+      const double rv_ = ri * (muDim * radVisOverNu);
+      radInvMax = fmax(radInvMax, ri_); radVisMax = fmax(radVisMax, rv_); a2Min = fmin(a2Min, a2);
+    }
+    if (a2Min < 0.0) { free(node); return fail("Negative square speed of sound"); }
+    const double dtInv = f1[c->P] * radInvMax + f2[c->P] * radVisMax;
+    dt[e] = c->prm.cfl * c->uRef / dtInv;
+  }
+  free(node);
+  return 0;
+}
+
+/* SolverClass::ExecuteTimeSynchronizationStep (SolverClass.cpp:2177-2422). */
+int orc_sync_step(orc_case *c, double md[8])
+{
+  static const double alphaRK[] = {1.0, 3.0 / 4.0, 1.0 / 3.0};
+  static const double betaRK[] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
+  if (!c->resValid && orc_residual(c, 1, md)) return -1;
+  const size_t ns = (size_t) c->nElem * NVAR * c->nDOFs;
+  double *dte = dalloc(c->nElem);
+  int nSteps = 0, reached = 0;
+  double elapsed = 0.0;
+  while (!reached) {
+    ++c->timeStep; ++nSteps;
+    if (orc_time_steps(c, dte)) { free(dte); return -1; }
+    double dt = 1e10;
+    for (int e = 0; e < c->nElem; ++e) dt = fmin(dt, dte[e]);
+    memcpy(c->solOld, c->sol, sizeof(double) * ns);
+    const double dtSync = c->prm.dt_synchr * c->uRef;
+    if (elapsed + dt >= 0.99 * dtSync) { dt = dtSync - elapsed; reached = 1; }
+    elapsed += dt;
+    for (int s = 0; s < 3; ++s) {
+      const double a = alphaRK[s], b = betaRK[s] * dt;
+      for (size_t i = 0; i < ns; ++i) c->sol[i] = (1.0 - a) * c->sol[i] + a * c->solOld[i] - b * c->res[i];
+      if (orc_residual(c, reached && s == 2, md)) { free(dte); return -1; }
+    }
+  }
+  free(dte);
+  return nSteps;
+}
+
+void orc_cg_iterations(const orc_case *c, int *iters)
+{
+  memcpy(iters, c->cgIt, sizeof(int) * c->nElem);
+}
+
+void orc_get_setup(const orc_case *c, int which, double *out)
+{
+  const size_t E = c->nElem;
+  if (which == 0) memcpy(out, c->vm, sizeof(double) * E * 13 * c->nInt);
+  else if (which >= 1 && which <= 6) memcpy(out, c->sm[which - 1], sizeof(double) * E * 16 * c->nInt2D);
+  else if (which == 7) memcpy(out, c->len, sizeof(double) * E * 4);
+  else if (which == 8) for (size_t i = 0; i < 6 * E; ++i) out[i] = c->nbr[i];
+}
+
+void orc_get_table(const orc_case *c, int which, double *out)
+{
+  const int K = c->K, M = c->M;
+  switch (which) {
+    case 0: memcpy(out, c->rInt, sizeof(double) * M); break;
+    case 1: memcpy(out, c->wInt, sizeof(double) * M); break;
+    case 2: for (int q = 0; q < K; ++q) for (int p = 0; p < M; ++p) out[q * M + p] = c->A[p * K + q]; break;
+    case 3: for (int q = 0; q < K; ++q) for (int p = 0; p < M; ++p) out[q * M + p] = c->D[p * K + q]; break;
+    case 4: memcpy(out, c->rDOF, sizeof(double) * K); break;
+    case 5: memcpy(out, c->rGrid, sizeof(double) * c->Kg); break;
+  }
+}

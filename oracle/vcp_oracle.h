@@ -1,0 +1,78 @@
+/*
+ * vcp_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU restatement (plain C) of the VCP3D explicit DG residual and SSP-RK3
+ * synchronization step, used by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py as the checker. The product (vcp3d_open_amd)
+ * never links, loads or calls it.
+ *
+ * Scope: entropy working variables, Ismail-Roe flux, no SGS model, no wall
+ * model, structured single block with periodic (translation) faces.
+ * Each function cites the reference file:line it restates; paths are
+ * relative to VCP3D_CurviLinear/.
+ */
+#ifndef VCP_ORACLE_H
+#define VCP_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int    n_elem[3];        /* elements in i, j, k */
+  int    n_poly_grid;      /* "Polynomial degree grid" */
+  int    n_poly_sol;       /* "Polynomial degree solution" */
+  int    n_poly_int_rule;  /* "Polynomial degree integration rule" */
+  int    dof_location;     /* 1 equidistant, 2 LGL ("Location of the DOFs") */
+  int    periodic[3];      /* 1: periodic translation in i/j/k */
+  double mach, reynolds, reynolds_length, temp_free_stream;
+  double theta_sym, rel_penalty, cfl, dt_synchr;
+  int    bug_compat;       /* 1: metric indexed by variable index, as the
+                              reference does (ElementClass.cpp:3865-3873,
+                              FluxesFace.cpp:272-282, :325-332) */
+} orc_params;
+
+typedef struct orc_case orc_case;
+
+/* elem_grid: [nElem][3][(n_poly_grid+1)^3], element i fastest, then j, k;
+   grid DOF ii fastest (the reference's mCoorNodalGridDOFs). */
+orc_case *orc_create(const orc_params *prm, const double *elem_grid);
+void      orc_destroy(orc_case *c);
+const char *orc_last_error(void);
+
+/* Sizes: nElem, K (=P+1), M (integration points 1D). */
+void orc_sizes(const orc_case *c, int *n_elem, int *n_dofs1d, int *n_int1d);
+/* uRef, pRef, rhoRef, mu (non-dimensional) as ComputeReferenceConditions. */
+void orc_reference(const orc_case *c, double out[4]);
+
+/* nodal: [nElem][nDOFs][5] dimensional conservative (restart layout). */
+int  orc_set_restart(orc_case *c, const double *nodal);
+/* modal: [nElem][5][nDOFs] entropy variables. */
+void orc_set_modal(orc_case *c, const double *modal);
+void orc_get_modal(const orc_case *c, double *modal);
+/* residual after the mass-matrix solve, [nElem][5][nDOFs]. */
+void orc_get_residual(const orc_case *c, double *res);
+/* nodal dimensional conservative [nElem][nDOFs][5] (restart writer). */
+void orc_get_nodal(const orc_case *c, double *nodal);
+
+/* SolverClass::Residual; returns 0 or <0 on error. */
+int  orc_residual(orc_case *c, int compute_monitoring, double monitoring[8]);
+/* SolverClass::ExecuteTimeSynchronizationStep; returns number of RK steps. */
+int  orc_sync_step(orc_case *c, double monitoring[8]);
+/* Per-element stable time step (ElementClass::ComputeTimeStep). */
+int  orc_time_steps(orc_case *c, double *dt_elem);
+/* CG iteration count of the last mass-matrix solve, per element. */
+void orc_cg_iterations(const orc_case *c, int *iters);
+
+/* Setup data for tests. which: 0 volume metric [nElem][13][nInt],
+   1..6 face metric (iMin,iMax,jMin,jMax,kMin,kMax) [nElem][16][nInt2D],
+   7 length scales+volume [nElem][4] (I,J,K,vol), 8 neighbours [nElem][6] (as double). */
+void orc_get_setup(const orc_case *c, int which, double *out);
+/* 1D tables: which 0 Gauss points, 1 weights, 2 legendre at int [K][M],
+   3 derivative [K][M], 4 nodal DOF locations [K], 5 grid DOF locations. */
+void orc_get_table(const orc_case *c, int which, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,224 @@
+"""Case builders: grids, initial conditions and parameter files.
+
+The reference takes its grid from a Plot3D file and non-uniform initial states
+from an SU2 restart (the SINE_AND_COSINE option aborts, ElementClass.cpp:526-532),
+so every case here is expressed in those two inputs plus a parameter file.
+Reference conditions follow InputParamClass::ComputeReferenceConditions
+(VCP3D_CurviLinear/src/InputParamClass.cpp:605-758) for an all-periodic block:
+mu = 1.716e-5 (constant), vInf = M sqrt(gamma R T), rhoRef = Re mu /(L vInf),
+pRef = rhoRef R T, uRef = sqrt(pRef/rhoRef).
+"""
+from __future__ import annotations
+
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import io as vio
+
+GAMMA = 1.4
+R_GAS = 287.058
+MU_DIM = 1.716e-5
+
+
+def lgl_nodes(n: int) -> np.ndarray:
+    """Legendre-Gauss-Lobatto nodes on [-1, 1] (n points), endpoints exact."""
+    if n == 1:
+        return np.zeros(1)
+    if n == 2:
+        return np.array([-1.0, 1.0])
+    # interior nodes are the roots of P'_{n-1}
+    c = np.zeros(n)
+    c[-1] = 1.0
+    dc = np.polynomial.legendre.legder(c)
+    x = np.sort(np.polynomial.legendre.legroots(dc))
+    # polish with Newton on P'_{n-1}
+    d2c = np.polynomial.legendre.legder(dc)
+    for _ in range(3):
+        x = x - np.polynomial.legendre.legval(x, dc) / np.polynomial.legendre.legval(x, d2c)
+    x = 0.5 * (x - x[::-1])  # enforce symmetry
+    return np.concatenate([[-1.0], x, [1.0]])
+
+
+def reference_conditions(mach: float, reynolds: float, length: float, temp: float):
+    v_inf = mach * math.sqrt(GAMMA * R_GAS * temp)
+    rho_ref = reynolds * MU_DIM / (length * v_inf)
+    p_ref = rho_ref * R_GAS * temp
+    u_ref = math.sqrt(p_ref / rho_ref)
+    return dict(v_inf=v_inf, rho_ref=rho_ref, p_ref=p_ref, u_ref=u_ref,
+                mu_nondim=MU_DIM * u_ref / p_ref)
+
+
+def periodic_subfaces(n_elem, lengths):
+    """Six all-periodic subface lines (translation = donor minus own face)."""
+    ni, nj, nk = n_elem
+    lx, ly, lz = lengths
+    I, J, K = ni + 1, nj + 1, nk + 1
+    fmt = lambda v: repr(float(v))
+    return [
+        [f"Periodic1to1Trans 1 1 1 {J} 1 {K}   {I} {I} 1 {J} 1 {K}   1 2 3   {fmt(lx)} 0.0 0.0"],
+        [f"Periodic1to1Trans {I} {I} 1 {J} 1 {K}   1 1 1 {J} 1 {K}   1 2 3   {fmt(-lx)} 0.0 0.0"],
+        [f"Periodic1to1Trans 1 {I} 1 1 1 {K}   1 {I} {J} {J} 1 {K}   1 2 3   0.0 {fmt(ly)} 0.0"],
+        [f"Periodic1to1Trans 1 {I} {J} {J} 1 {K}   1 {I} 1 1 1 {K}   1 2 3   0.0 {fmt(-ly)} 0.0"],
+        [f"Periodic1to1Trans 1 {I} 1 {J} 1 1   1 {I} 1 {J} {K} {K}   1 2 3   0.0 0.0 {fmt(lz)}"],
+        [f"Periodic1to1Trans 1 {I} 1 {J} {K} {K}   1 {I} 1 {J} 1 1   1 2 3   0.0 0.0 {fmt(-lz)}"],
+    ]
+
+
+@dataclass
+class TGVCase:
+    """Taylor-Green vortex on the periodic cube [0, 2 pi]^3.
+
+    u = V0 sin x cos y cos z, v = -V0 cos x sin y cos z, w = 0,
+    p = p0 + rho0 V0^2/16 (cos 2x + cos 2y)(cos 2z + 2), rho = p/(R T0).
+    With curvilinear=True the grid nodes are moved by
+    x_i += 0.1 h sin(x) sin(y) sin(z) (evaluated at the unperturbed nodes),
+    which leaves the periodic faces untouched.
+    Optional phase shifts move the vortex: x -> x + phase[0], etc.
+    """
+    n: int = 8
+    p: int = 3
+    rule: int | None = None
+    mach: float = 0.1
+    reynolds: float = 1600.0
+    temp: float = 300.0
+    curvilinear: bool = False
+    perturb: float = 0.1
+    phase: tuple = (0.0, 0.0, 0.0)
+    n_steps: int = 1
+    dt_synchr: float | None = None
+    cfl: float = 2.0
+    theta_sym: float = 1.0
+    rel_penalty: float = 1.0
+    name: str = "tgv"
+
+    def __post_init__(self):
+        if self.rule is None:
+            self.rule = 3 * self.p if self.p <= 4 else {5: 15, 6: 16}.get(self.p, 3 * self.p)
+        self.length = 2.0 * math.pi
+        self.ref = reference_conditions(self.mach, self.reynolds, 1.0, self.temp)
+        if self.dt_synchr is None:
+            self.dt_synchr = 0.5 * self.stable_dt_estimate()
+
+    @property
+    def n_elem(self):
+        return (self.n, self.n, self.n)
+
+    @property
+    def n_elems(self):
+        return self.n ** 3
+
+    @property
+    def n_dofs(self):
+        return (self.p + 1) ** 3
+
+    def stable_dt_estimate(self) -> float:
+        """Conservative dimensional time step for CFL = self.cfl, following the
+        spectral-radius form of ElementClass::ComputeTimeStep
+        (ElementClass.cpp:1522-1644) with bounds on |u| and a."""
+        f1 = [2.0, 6.0, 12.0, 20.0, 28.0, 38.0, 50.0, 64.0, 80.0, 98.0]
+        f2 = [6.0, 36.0, 150.0, 420.0, 980.0, 1975.0, 3575.0, 7000.0, 14000.0, 28000.0]
+        h = self.length / self.n * (1.0 - 2.0 * self.perturb if self.curvilinear else 1.0)
+        a = math.sqrt(GAMMA * R_GAS * self.temp) * 1.1
+        v = self.ref["v_inf"] * 1.5
+        rad_inv = 3.0 * (v + a) / h
+        rho_min = 0.8 * self.ref["rho_ref"]
+        rad_vis = MU_DIM / rho_min * max(2.0 - 2.0 / 3.0, GAMMA / 0.72) * 3.0 / h ** 2
+        return self.cfl / (f1[self.p] * rad_inv + f2[self.p] * rad_vis)
+
+    # -------------------------------------------------------------- geometry
+    def grid_block(self) -> np.ndarray:
+        """Plot3D block (3, kl, jl, il) with nodes at LGL positions."""
+        r = lgl_nodes(self.p + 1)
+        h = self.length / self.n
+        pts = [0.0]
+        for e in range(self.n):
+            for q in range(1, self.p + 1):
+                pts.append(e * h + 0.5 * (r[q] + 1.0) * h)
+        x1 = np.array(pts)
+        x1[-1] = self.length
+        z, y, x = np.meshgrid(x1, x1, x1, indexing="ij")
+        xyz = np.stack([x, y, z])
+        if self.curvilinear:
+            d = self.perturb * h * np.sin(x) * np.sin(y) * np.sin(z)
+            xyz = xyz + d[None]
+        return xyz
+
+    def element_grid(self) -> np.ndarray:
+        return vio.block_to_elements(self.grid_block(), self.n_elem, self.p)
+
+    def nodal_coords(self) -> np.ndarray:
+        """(nElem, nDOFs, 3): with equal grid and solution degree on LGL
+        points the nodal solution DOFs coincide with the grid DOFs."""
+        return np.transpose(self.element_grid(), (0, 2, 1)).copy()
+
+    # ----------------------------------------------------- initial condition
+    def initial_conservative(self, coords: np.ndarray | None = None) -> np.ndarray:
+        """Dimensional conservative state (nElem, nDOFs, 5) at the nodes."""
+        if coords is None:
+            coords = self.nodal_coords()
+        x = coords[..., 0] + self.phase[0]
+        y = coords[..., 1] + self.phase[1]
+        z = coords[..., 2] + self.phase[2]
+        v0 = self.ref["v_inf"]
+        rho0 = self.ref["rho_ref"]
+        p0 = self.ref["p_ref"]
+        u = v0 * np.sin(x) * np.cos(y) * np.cos(z)
+        v = -v0 * np.cos(x) * np.sin(y) * np.cos(z)
+        w = np.zeros_like(u)
+        p = p0 + rho0 * v0 * v0 / 16.0 * (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2.0)
+        rho = p / (R_GAS * self.temp)
+        rhoE = p / (GAMMA - 1.0) + 0.5 * rho * (u * u + v * v + w * w)
+        return np.stack([rho, rho * u, rho * v, rho * w, rhoE], axis=-1)
+
+    # ------------------------------------------------------ parameter file
+    def params(self, grid_file="grid.xyz", restart_file="restart.dat",
+               sol_file="out/Solution") -> dict:
+        return {
+            "Polynomial degree grid": self.p,
+            "Polynomial degree solution": self.p,
+            "Polynomial degree integration rule": self.rule,
+            "Location of the DOFs": "LGL",
+            "Grid file name": grid_file,
+            "Solution file name": sol_file,
+            "Restart file name": restart_file,
+            "Restart": "yes",
+            "Paraview visualization file name": "out/Solution.vtk",
+            "Space average data file name": "out/SpaceAverage.vtk",
+            "SU2 grid file name": "out/grid.su2",
+            "Write SU2 grid file": "no",
+            "Save solution every": 100000000,
+            "Write time history info every": 1,
+            "FEM working variables": "Entropy",
+            "Riemann solver": "IsmailRoe",
+            "Subgrid scale model": "None",
+            "Wall modeling": "No wall model",
+            "Theta parameter symmetrizing terms": self.theta_sym,
+            "Relative penalty parameter": self.rel_penalty,
+            "Mach number": self.mach,
+            "Reynolds number": self.reynolds,
+            "Reynolds length": 1.0,
+            "Free stream temperature": self.temp,
+            "Free stream velocity direction": [1.0, 0.0, 0.0],
+            "Fluctuations to add to initial solution": "No fluctuations",
+            "Monitor forces": "no",
+            "Number of synchronization time steps": self.n_steps,
+            "Synchronization time step": self.dt_synchr,
+            "CFL number": self.cfl,
+            "Compute average solution": "no",
+        }
+
+    def subfaces(self):
+        return periodic_subfaces(self.n_elem, (self.length,) * 3)
+
+    def write_inputs(self, directory: str) -> str:
+        """Write grid, restart and parameter file; returns the parameter path."""
+        os.makedirs(os.path.join(directory, "out"), exist_ok=True)
+        vio.write_plot3d(os.path.join(directory, "grid.xyz"), self.grid_block())
+        vio.write_su2_restart(os.path.join(directory, "restart.dat"),
+                              self.nodal_coords(), self.initial_conservative())
+        path = os.path.join(directory, "param.cfg")
+        vio.write_param_file(path, self.params(), self.subfaces())
+        return path

@@ -1,0 +1,106 @@
+/*
+ * vcp3d_amd.h -- C ABI of the MI355X (gfx950) explicit DG residual and
+ * SSP-RK3 synchronization step of VCP3D.
+ *
+ * The reference has no plugin layer: the seam is SolverClass
+ * (VCP3D_CurviLinear/include/SolverClass.hpp:16-124). Each entry point below
+ * names the reference member it replaces. All pointers are host pointers to
+ * caller-owned memory, copied in or out; device memory, streams and kernels
+ * stay inside the context. Every call returns 0 on success and a negative
+ * code on failure (vcp_last_error() gives the text); nothing calls exit().
+ *
+ * Array layouts (FP64):
+ *   elem_grid  [nElem][3][(n_poly_grid+1)^3]  the per-element grid DOFs
+ *              (ElementClass::mCoorNodalGridDOFs, filled by
+ *              SolverClass::ReadGrid, SolverClass.cpp:2988-3034)
+ *   modal      [nElem][5][(P+1)^3]            entropy variables per element
+ *              (ElementClass::mSol / mRes, unpadded)
+ *   nodal      [nElem][(P+1)^3][5]            dimensional conservative
+ *              variables at the nodal DOFs (SU2 restart payload,
+ *              SolverClass.cpp:4375-4804)
+ * Element index: i fastest, then j, then k.
+ */
+#ifndef VCP3D_AMD_H
+#define VCP3D_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Input parameters of the path; names and meaning follow InputParamClass
+   (VCP3D_CurviLinear/include/InputParamClass.hpp:18-301). */
+typedef struct vcp_case_desc {
+  int    n_elem[3];          /* elements in i, j, k (from the subface ranges)   */
+  int    n_poly_grid;        /* "Polynomial degree grid"                        */
+  int    n_poly_sol;         /* "Polynomial degree solution"                    */
+  int    n_poly_int_rule;    /* "Polynomial degree integration rule"            */
+  int    dof_location;       /* 1 equidistant, 2 LGL ("Location of the DOFs")   */
+  int    periodic[3];        /* 1: Periodic1to1Trans pair on the i/j/k faces    */
+  double mach;               /* "Mach number"                                   */
+  double reynolds;           /* "Reynolds number"                               */
+  double reynolds_length;    /* "Reynolds length"                               */
+  double temp_free_stream;   /* "Free stream temperature"                       */
+  double theta_sym;          /* "Theta parameter symmetrizing terms"            */
+  double rel_penalty;        /* "Relative penalty parameter"                    */
+  double cfl;                /* "CFL number"                                    */
+  double dt_synchr;          /* "Synchronization time step" (dimensional)       */
+  int    bug_compat;         /* 1: reproduce the reference's metric indexing in
+                                the gradient conversion (metric of integration
+                                point l used for variable l,
+                                ElementClass.cpp:3865-3873, FluxesFace.cpp:
+                                272-282, :325-332); 0: use the point's metric  */
+} vcp_case_desc;
+
+typedef struct vcp_ctx vcp_ctx;
+
+/* Setup: SolverClass::SolverClass + ReadGrid + ComputeMetricTerms
+   (SolverClass.cpp:12-61, :2787-3034, :829-1016). */
+int vcp_create(const vcp_case_desc *desc, const double *elem_grid, vcp_ctx **out);
+int vcp_destroy(vcp_ctx *ctx);
+
+/* Sizes: number of elements, nDOFs1D (K), nInt1D (M). */
+int vcp_sizes(const vcp_ctx *ctx, int *n_elem, int *n_dofs_1d, int *n_int_1d);
+/* Reference conditions: uRef, pRef, rhoRef, non-dimensional mu
+   (InputParamClass::ComputeReferenceConditions, InputParamClass.cpp:605-758). */
+int vcp_reference(const vcp_ctx *ctx, double out[4]);
+
+/* SolverClass::ReadRestartSolution (SolverClass.cpp:3040-3376): nodal
+   dimensional conservative -> modal entropy variables. */
+int vcp_set_restart(vcp_ctx *ctx, const double *nodal);
+/* Direct access to the modal working variables (ElementClass::mSol). */
+int vcp_set_solution(vcp_ctx *ctx, const double *modal);
+int vcp_get_solution(vcp_ctx *ctx, double *modal);
+/* Solution payload of SolverClass::WriteRestartSolution (SolverClass.cpp:4375-4560). */
+int vcp_get_nodal(vcp_ctx *ctx, double *nodal);
+
+/* SolverClass::Residual (SolverClass.cpp:3381-3795; SolverClass.hpp:72-73):
+   residual multiplied by the inverse (entropy) mass matrix. monitoring[8]
+   as the reference: 6 force coefficients, max Mach^2, max eddy viscosity. */
+int vcp_residual(vcp_ctx *ctx, int compute_monitoring, double monitoring[8]);
+/* ElementClass::mRes of every element after vcp_residual / vcp_sync_step. */
+int vcp_get_residual(vcp_ctx *ctx, double *modal);
+
+/* SolverClass::ExecuteTimeSynchronizationStep (SolverClass.cpp:2177-2422;
+   SolverClass.hpp:50): SSP-RK3 steps until the synchronization time is
+   reached, dt = min over elements, last step clipped by the 0.99 rule; the
+   residual of the previous call is reused for the first stage. Returns the
+   number of RK steps taken (>= 1) or a negative code. */
+int vcp_sync_step(vcp_ctx *ctx, double monitoring[8]);
+
+/* ElementClass::ComputeTimeStep of every element (ElementClass.cpp:1522-1644). */
+int vcp_time_steps(vcp_ctx *ctx, double *dt_elem);
+/* Iterations of the last per-element mass-matrix solve
+   (CG_EntropyVarMassMatrix, CG_MassMatrix.cpp:250-504). */
+int vcp_cg_iterations(vcp_ctx *ctx, int *iters);
+
+/* Measurement: times `reps` residual evaluations with HIP events recorded on
+   the context's stream around each kernel; writes the mean milliseconds per
+   launch of [face kernel, volume+mass-matrix kernel] into ms[2]. */
+int vcp_profile_residual(vcp_ctx *ctx, int reps, double ms[2]);
+
+const char *vcp_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

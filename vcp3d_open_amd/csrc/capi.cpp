@@ -1,0 +1,376 @@
+// C ABI of the GPU path (include/vcp3d_amd.h): context set-up, device
+// buffers and the host sequencing of SolverClass::Residual and
+// SolverClass::ExecuteTimeSynchronizationStep.
+#include "vcp3d_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "setup.hpp"
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(const std::string &msg, int code = -1)
+{
+  g_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return fail(std::string(#expr) + ": " + hipGetErrorString(e_), -3);         \
+  } while (0)
+
+}  // namespace
+
+struct vcp_ctx {
+  vcp::CaseParams prm;
+  vcp::Tables tab;
+  vcp::RefConditions rc{};
+  vcp::DeviceCase dc;
+  hipStream_t stream = nullptr;
+  std::vector<void *> allocs;
+  bool res_valid = false;
+  int time_step = 0;
+
+  template <class T>
+  hipError_t alloc(T **p, size_t n)
+  {
+    void *v = nullptr;
+    hipError_t e = hipMalloc(&v, n * sizeof(T) > 0 ? n * sizeof(T) : 16);
+    if (e == hipSuccess) { allocs.push_back(v); *p = static_cast<T *>(v); }
+    return e;
+  }
+  ~vcp_ctx()
+  {
+    for (void *p : allocs) (void) hipFree(p);
+    if (stream) (void) hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+size_t state_size(const vcp_ctx *c) { return (size_t) c->dc.nElem * 5 * c->dc.nDOF; }
+
+int check_flags(vcp_ctx *c)
+{
+  int flags = 0;
+  HIP_TRY(hipMemcpyAsync(&flags, c->dc.flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (flags & vcp::kFlagNegativeA2) return fail("Negative square speed of sound", -4);
+  if (flags & vcp::kFlagCGNotConverged) return fail("Convergence not reached for CG algorithm", -5);
+  if (flags & vcp::kFlagNonFinite) return fail("non-finite residual", -6);
+  return 0;
+}
+
+int set_scalar(vcp_ctx *c, int slot, double v)
+{
+  HIP_TRY(hipMemcpyAsync(c->dc.scal + slot, &v, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  return 0;
+}
+
+// faces + volume/mass matrix, with the given stage control
+int residual_pass(vcp_ctx *c, const vcp::StageCtl &ctl)
+{
+  HIP_TRY(vcp::launch_faces(c->dc, c->stream));
+  HIP_TRY(vcp::launch_volume(c->dc, ctl, c->stream));
+  return 0;
+}
+
+void fill_monitoring(double md[8], bool mon, double mach2)
+{
+  for (int i = 0; i < 6; ++i) md[i] = 0.0;   // no walls: force coefficients vanish
+  md[6] = mon ? mach2 : 0.0;
+  md[7] = mon ? 0.0 : -1.0;                  // no SGS model: eddy viscosity 0
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *vcp_last_error(void) { return g_error.c_str(); }
+
+int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
+{
+  if (!d || !elem_grid || !out) return fail("null argument");
+  *out = nullptr;
+  vcp::CaseParams p;
+  for (int i = 0; i < 3; ++i) { p.n_elem[i] = d->n_elem[i]; p.periodic[i] = d->periodic[i]; }
+  p.n_poly_grid = d->n_poly_grid; p.n_poly_sol = d->n_poly_sol; p.n_poly_int_rule = d->n_poly_int_rule;
+  p.dof_location = d->dof_location; p.mach = d->mach; p.reynolds = d->reynolds;
+  p.reynolds_length = d->reynolds_length; p.temp_free_stream = d->temp_free_stream;
+  p.theta_sym = d->theta_sym; p.rel_penalty = d->rel_penalty; p.cfl = d->cfl; p.dt_synchr = d->dt_synchr;
+  p.bug_compat = d->bug_compat;
+  if (p.n_elem[0] <= 0 || p.n_elem[1] <= 0 || p.n_elem[2] <= 0) return fail("invalid element counts");
+  if (p.n_poly_sol <= 0 || p.n_poly_sol > 9 || p.n_poly_grid <= 0 || p.n_poly_grid > 9)
+    return fail("polynomial degrees must be in 1..9");
+  if (p.n_poly_int_rule < 2 * p.n_poly_sol)
+    return fail("\"Polynomial degree integration rule\" not accurate enough");
+  if (!p.periodic[0] || !p.periodic[1] || !p.periodic[2])
+    return fail("non-periodic block faces are not supported by this build");
+  const int K = p.n_poly_sol + 1, M = vcp::n_int1d(p.n_poly_int_rule);
+  if (!vcp::supported_pair(K, M)) return fail("unsupported (nDOFs1D, nInt1D) pair");
+
+  vcp_ctx *c = new vcp_ctx();
+  c->prm = p;
+  vcp::build_tables(p, c->tab);
+  vcp::Geometry g;
+  const std::string err = vcp::build_geometry(p, c->tab, elem_grid, g);
+  if (!err.empty()) { delete c; return fail(err); }
+  c->rc = vcp::reference_conditions(p);
+
+  vcp::DeviceCase &dc = c->dc;
+  const int nElem = p.n_elem[0] * p.n_elem[1] * p.n_elem[2];
+  dc.nElem = nElem; dc.K = K; dc.M = M; dc.P = p.n_poly_sol;
+  dc.nDOF = K * K * K; dc.nInt = M * M * M; dc.nInt2D = M * M;
+  dc.bug_compat = p.bug_compat; dc.mu = c->rc.mu; dc.theta_sym = p.theta_sym; dc.rel_penalty = p.rel_penalty;
+  dc.cfl = p.cfl; dc.u_ref = c->rc.u_ref; dc.p_ref = c->rc.p_ref; dc.rho_ref = c->rc.rho_ref;
+
+  // table block
+  const vcp::Tables &t = c->tab;
+  std::vector<double> tb;
+  auto put = [&](const std::vector<double> &v) { const int off = (int) tb.size(); tb.insert(tb.end(), v.begin(), v.end()); return off; };
+  dc.to.A = put(t.A); dc.to.D = put(t.D); dc.to.AT = put(t.AT); dc.to.DT = put(t.DT); dc.to.LD = put(t.LD);
+  dc.to.fmin = put(t.f_min); dc.to.fmax = put(t.f_max); dc.to.dfmin = put(t.df_min); dc.to.dfmax = put(t.df_max);
+  dc.to.w3 = put(t.w3); dc.to.w2 = put(t.w2); dc.to.bc = put(t.basis_center); dc.to.total = (int) tb.size();
+
+  // device layouts of the geometry
+  const int nInt = dc.nInt, n2 = dc.nInt2D;
+  std::vector<double> vgeo((size_t) nElem * vcp::kVolGeo * nInt);
+  std::vector<double> fgeo((size_t) nElem * 3 * vcp::kFaceGeo * n2);
+  std::vector<double> econ((size_t) nElem * vcp::kElemConst, 0.0);
+  for (int e = 0; e < nElem; ++e) {
+    std::memcpy(&vgeo[(size_t) e * vcp::kVolGeo * nInt], &g.vol[(size_t) e * 13 * nInt],
+                sizeof(double) * vcp::kVolGeo * nInt);
+    for (int q = 0; q < 4; ++q) econ[(size_t) e * vcp::kElemConst + q] = g.lens[(size_t) e * 4 + q];
+    for (int q = 0; q < 9; ++q) econ[(size_t) e * vcp::kElemConst + 4 + q] = g.ave_norm[(size_t) e * 9 + q];
+    for (int dir = 0; dir < 3; ++dir) {
+      const int L = g.nbr[(size_t) e * 6 + 2 * dir];
+      double *dst = &fgeo[((size_t) e * 3 + dir) * vcp::kFaceGeo * n2];
+      const double *mR = &g.face[2 * dir][(size_t) e * 16 * n2];
+      const double *mL = L >= 0 ? &g.face[2 * dir + 1][(size_t) L * 16 * n2] : mR;
+      for (int l = 0; l < n2; ++l) {
+        for (int q = 0; q < 4; ++q) dst[q * n2 + l] = mL[q * n2 + l];
+        for (int q = 0; q < 9; ++q) {
+          dst[(4 + q) * n2 + l] = mL[(4 + q) * n2 + l];
+          dst[(13 + q) * n2 + l] = mR[(4 + q) * n2 + l];
+        }
+      }
+    }
+  }
+  hipError_t he = hipStreamCreate(&c->stream);
+  const size_t ns = (size_t) nElem * 5 * dc.nDOF;
+  if (he == hipSuccess) he = c->alloc(&dc.tables, tb.size());
+  if (he == hipSuccess) he = c->alloc(&dc.vgeo, vgeo.size());
+  if (he == hipSuccess) he = c->alloc(&dc.fgeo, fgeo.size());
+  if (he == hipSuccess) he = c->alloc(&dc.econst, econ.size());
+  if (he == hipSuccess) he = c->alloc(&dc.nbr, g.nbr.size());
+  if (he == hipSuccess) he = c->alloc(&dc.U, ns);
+  if (he == hipSuccess) he = c->alloc(&dc.U0, ns);
+  if (he == hipSuccess) he = c->alloc(&dc.R, ns);
+  if (he == hipSuccess) he = c->alloc(&dc.faceRes, 6 * ns);
+  if (he == hipSuccess) he = c->alloc(&dc.scal, 8);
+  if (he == hipSuccess) he = c->alloc(&dc.flags, 4);
+  if (he == hipSuccess) he = c->alloc(&dc.cgIter, (size_t) nElem);
+  if (he == hipSuccess) he = c->alloc(&dc.dtElem, (size_t) nElem);
+  if (he == hipSuccess) he = hipMemcpy(dc.tables, tb.data(), tb.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (he == hipSuccess) he = hipMemcpy(dc.vgeo, vgeo.data(), vgeo.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (he == hipSuccess) he = hipMemcpy(dc.fgeo, fgeo.data(), fgeo.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (he == hipSuccess) he = hipMemcpy(dc.econst, econ.data(), econ.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (he == hipSuccess) he = hipMemcpy(dc.nbr, g.nbr.data(), g.nbr.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (he == hipSuccess) he = hipMemset(dc.U, 0, ns * sizeof(double));
+  if (he == hipSuccess) he = hipMemset(dc.R, 0, ns * sizeof(double));
+  if (he == hipSuccess) he = hipMemset(dc.faceRes, 0, 6 * ns * sizeof(double));
+  if (he == hipSuccess) he = hipMemset(dc.flags, 0, 4 * sizeof(int));
+  if (he != hipSuccess) {
+    const std::string msg = std::string("device set-up failed: ") + hipGetErrorString(he);
+    delete c;
+    return fail(msg, -3);
+  }
+  *out = c;
+  return 0;
+}
+
+int vcp_destroy(vcp_ctx *c)
+{
+  if (c) { (void) hipStreamSynchronize(c->stream); delete c; }
+  return 0;
+}
+
+int vcp_sizes(const vcp_ctx *c, int *n_elem, int *k, int *m)
+{
+  if (!c) return fail("null context");
+  if (n_elem) *n_elem = c->dc.nElem;
+  if (k) *k = c->dc.K;
+  if (m) *m = c->dc.M;
+  return 0;
+}
+
+int vcp_reference(const vcp_ctx *c, double out[4])
+{
+  if (!c || !out) return fail("null argument");
+  out[0] = c->rc.u_ref; out[1] = c->rc.p_ref; out[2] = c->rc.rho_ref; out[3] = c->rc.mu;
+  return 0;
+}
+
+int vcp_set_restart(vcp_ctx *c, const double *nodal)
+{
+  if (!c || !nodal) return fail("null argument");
+  const int nD = c->dc.nDOF;
+  std::vector<double> modal(state_size(c));
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < c->dc.nElem; ++e)
+    vcp::nodal_cons_to_modal_entropy(c->tab, c->rc, nodal + (size_t) e * nD * 5, &modal[(size_t) e * 5 * nD]);
+  return vcp_set_solution(c, modal.data());
+}
+
+int vcp_set_solution(vcp_ctx *c, const double *modal)
+{
+  if (!c || !modal) return fail("null argument");
+  HIP_TRY(hipMemcpyAsync(c->dc.U, modal, state_size(c) * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->res_valid = false;
+  return 0;
+}
+
+int vcp_get_solution(vcp_ctx *c, double *modal)
+{
+  if (!c || !modal) return fail("null argument");
+  HIP_TRY(hipMemcpyAsync(modal, c->dc.U, state_size(c) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int vcp_get_nodal(vcp_ctx *c, double *nodal)
+{
+  if (!c || !nodal) return fail("null argument");
+  std::vector<double> modal(state_size(c));
+  const int rc = vcp_get_solution(c, modal.data());
+  if (rc) return rc;
+  const int nD = c->dc.nDOF;
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < c->dc.nElem; ++e)
+    vcp::modal_entropy_to_nodal_cons(c->tab, c->rc, &modal[(size_t) e * 5 * nD], nodal + (size_t) e * nD * 5);
+  return 0;
+}
+
+int vcp_get_residual(vcp_ctx *c, double *modal)
+{
+  if (!c || !modal) return fail("null argument");
+  HIP_TRY(hipMemcpyAsync(modal, c->dc.R, state_size(c) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int vcp_residual(vcp_ctx *c, int mon, double md[8])
+{
+  if (!c) return fail("null context");
+  int rc = set_scalar(c, 1, 0.0);
+  if (rc) return rc;
+  vcp::StageCtl ctl{1, 0, 0.0, 0.0, mon ? 1 : 0};
+  if ((rc = residual_pass(c, ctl))) return rc;
+  double mach2 = 0.0;
+  HIP_TRY(hipMemcpyAsync(&mach2, c->dc.scal + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if ((rc = check_flags(c))) return rc;
+  if (md) fill_monitoring(md, mon != 0, mach2);
+  c->res_valid = true;
+  return 0;
+}
+
+int vcp_time_steps(vcp_ctx *c, double *dt_elem)
+{
+  if (!c) return fail("null context");
+  int rc = set_scalar(c, 0, 1e10);
+  if (rc) return rc;
+  HIP_TRY(vcp::launch_time_step(c->dc, c->stream));
+  if (dt_elem)
+    HIP_TRY(hipMemcpyAsync(dt_elem, c->dc.dtElem, c->dc.nElem * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  return check_flags(c);
+}
+
+int vcp_sync_step(vcp_ctx *c, double md[8])
+{
+  static const double alphaRK[] = {1.0, 3.0 / 4.0, 1.0 / 3.0};
+  static const double betaRK[] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
+  if (!c) return fail("null context");
+  double dummy[8];
+  if (!md) md = dummy;
+  int rc = 0;
+  if (!c->res_valid && (rc = vcp_residual(c, 1, md))) return rc;
+  const double dtSync = c->prm.dt_synchr * c->rc.u_ref;
+  double elapsed = 0.0;
+  bool reached = false;
+  int nSteps = 0;
+  while (!reached) {
+    ++c->time_step; ++nSteps;
+    if ((rc = vcp_time_steps(c, nullptr))) return rc;
+    double dt = 0.0;
+    HIP_TRY(hipMemcpyAsync(&dt, c->dc.scal, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (elapsed + dt >= 0.99 * dtSync) { dt = dtSync - elapsed; reached = true; }
+    elapsed += dt;
+    HIP_TRY(vcp::launch_stage0(c->dc, dt, c->stream));
+    vcp::StageCtl s1{0, 1, alphaRK[1], betaRK[1] * dt, 0};
+    vcp::StageCtl s2{0, 1, alphaRK[2], betaRK[2] * dt, 0};
+    vcp::StageCtl s3{1, 0, 0.0, 0.0, reached ? 1 : 0};
+    if (reached && (rc = set_scalar(c, 1, 0.0))) return rc;
+    if ((rc = residual_pass(c, s1))) return rc;
+    if ((rc = residual_pass(c, s2))) return rc;
+    if ((rc = residual_pass(c, s3))) return rc;
+    if (reached) {
+      double mach2 = 0.0;
+      HIP_TRY(hipMemcpyAsync(&mach2, c->dc.scal + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      if ((rc = check_flags(c))) return rc;
+      fill_monitoring(md, true, mach2);
+    }
+  }
+  if ((rc = check_flags(c))) return rc;
+  c->res_valid = true;
+  return nSteps;
+}
+
+int vcp_cg_iterations(vcp_ctx *c, int *iters)
+{
+  if (!c || !iters) return fail("null argument");
+  HIP_TRY(hipMemcpyAsync(iters, c->dc.cgIter, c->dc.nElem * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int vcp_profile_residual(vcp_ctx *c, int reps, double ms[2])
+{
+  if (!c || !ms || reps <= 0) return fail("invalid argument");
+  std::vector<hipEvent_t> ev(3 * reps);
+  for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+  vcp::StageCtl ctl{1, 0, 0.0, 0.0, 0};
+  for (int r = 0; r < reps; ++r) {
+    HIP_TRY(hipEventRecord(ev[3 * r], c->stream));
+    HIP_TRY(vcp::launch_faces(c->dc, c->stream));
+    HIP_TRY(hipEventRecord(ev[3 * r + 1], c->stream));
+    HIP_TRY(vcp::launch_volume(c->dc, ctl, c->stream));
+    HIP_TRY(hipEventRecord(ev[3 * r + 2], c->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  double f = 0.0, v = 0.0;
+  for (int r = 0; r < reps; ++r) {
+    float a = 0.f, b = 0.f;
+    HIP_TRY(hipEventElapsedTime(&a, ev[3 * r], ev[3 * r + 1]));
+    HIP_TRY(hipEventElapsedTime(&b, ev[3 * r + 1], ev[3 * r + 2]));
+    f += a; v += b;
+  }
+  for (auto &e : ev) (void) hipEventDestroy(e);
+  ms[0] = f / reps; ms[1] = v / reps;
+  c->res_valid = true;
+  return check_flags(c);
+}
+
+}  // extern "C"

@@ -1,0 +1,915 @@
+// gfx950 kernels of the VCP3D explicit DG residual and SSP-RK3 stage (FP64).
+//
+//   face_kernel          interior (incl. periodic) faces: traces of both
+//                        elements, Ismail-Roe + viscous + penalty +
+//                        symmetrising fluxes, lifted to both elements into
+//                        per-element face slots (no atomics)
+//   volume_kernel        volume residual, gather of the six face slots,
+//                        per-element entropy mass-matrix PCG, optional
+//                        in-place RK stage update
+//   time_step_kernel     per-element stable time step and global minimum
+//   stage0_kernel        U0 <- U, U <- U0 - dt R
+//
+// Reference behaviour followed (VCP3D_CurviLinear/src/):
+//   ElementClass::VolumeResidual          ElementClass.cpp:3809-4443
+//   ElementClass::{I,J,K}FaceResiduals    ElementClass.cpp:2582-3400
+//   FluxesInternalFace / InviscidFluxesFace / ViscousFluxesFace /
+//   PenAndSymFluxesSymmVar                FluxesFace.cpp:220-343, :813-975,
+//                                         :1290-1651, :1152-1284
+//   CG_EntropyVarMassMatrix               CG_MassMatrix.cpp:250-504
+//   ElementClass::ComputeTimeStep         ElementClass.cpp:1522-1644
+//   SolverClass::ExecuteTimeSynchronizationStep  SolverClass.cpp:2177-2422
+#include "kernels.hpp"
+
+#include <cstdint>
+
+namespace vcp {
+namespace {
+
+constexpr double kGam = 1.4;
+constexpr double kGm1 = 0.4;
+constexpr double kPrLam = 0.72;
+constexpr double kLamOverMu = -2.0 / 3.0;
+
+// ------------------------------------------------------------ utilities
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ inline double wave_max(double v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Block reduction of NV values (sum for the first NS, max for the rest).
+template <int NT, int NV, int NS>
+__device__ inline void block_reduce(double (&v)[NV], double *red)
+{
+#pragma unroll
+  for (int q = 0; q < NV; ++q) v[q] = (q < NS) ? wave_sum(v[q]) : wave_max(v[q]);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) red[wid * NV + q] = v[q];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double s = red[q];
+#pragma unroll
+    for (int w = 1; w < NT / 64; ++w) s = (q < NS) ? s + red[w * NV + q] : fmax(s, red[w * NV + q]);
+    v[q] = s;
+  }
+}
+
+// out[o][n][i] = sum_q op[n*NIN+q] * in[o][q][i]   (i < INNER)
+// in row stride IR, out row stride OR (in units of doubles).
+template <int NIN, int NOUT, int OUTER, int INNER, int IR, int OR, int NT>
+__device__ inline void contract(const double *__restrict__ in, double *__restrict__ out,
+                                const double *__restrict__ op)
+{
+  constexpr int TOT = OUTER * NOUT * INNER;
+  for (int idx = threadIdx.x; idx < TOT; idx += NT) {
+    const int o = idx / (NOUT * INNER);
+    const int r = idx - o * (NOUT * INNER);
+    const int n = r / INNER;
+    const int i = r - n * INNER;
+    const double *src = in + o * NIN * IR + i;
+    const double *w = op + n * NIN;
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < NIN; ++q) s += w[q] * src[q * IR];
+    out[(o * NOUT + n) * OR + i] = s;
+  }
+}
+
+// Same with two inputs and two operators summed.
+template <int NIN, int NOUT, int OUTER, int INNER, int IR, int OR, int NT>
+__device__ inline void contract2(const double *__restrict__ in1, const double *__restrict__ op1,
+                                 const double *__restrict__ in2, const double *__restrict__ op2,
+                                 double *__restrict__ out)
+{
+  constexpr int TOT = OUTER * NOUT * INNER;
+  for (int idx = threadIdx.x; idx < TOT; idx += NT) {
+    const int o = idx / (NOUT * INNER);
+    const int r = idx - o * (NOUT * INNER);
+    const int n = r / INNER;
+    const int i = r - n * INNER;
+    const double *s1 = in1 + o * NIN * IR + i, *s2 = in2 + o * NIN * IR + i;
+    const double *w1 = op1 + n * NIN, *w2 = op2 + n * NIN;
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < NIN; ++q) s += w1[q] * s1[q * IR] + w2[q] * s2[q * IR];
+    out[(o * NOUT + n) * OR + i] = s;
+  }
+}
+
+// In-place contraction of NROWS contiguous rows of length NIN -> NOUT.
+template <int NIN, int NOUT, int NROWS, int NT>
+__device__ inline void rows_inplace(double *buf, const double *__restrict__ op)
+{
+  for (int r = threadIdx.x; r < NROWS; r += NT) {
+    double x[NIN];
+#pragma unroll
+    for (int i = 0; i < NIN; ++i) x[i] = buf[r * NIN + i];
+#pragma unroll
+    for (int q = 0; q < NOUT; ++q) {
+      double s = 0.0;
+#pragma unroll
+      for (int i = 0; i < NIN; ++i) s += op[q * NIN + i] * x[i];
+      buf[r * NIN + q] = s;
+    }
+  }
+}
+
+__device__ inline void ent2prim(const double V[5], double &rho, double &u, double &v, double &w,
+                                double &p, double &eKin)
+{
+  const double V4Inv = 1.0 / V[4];
+  u = -V4Inv * V[1]; v = -V4Inv * V[2]; w = -V4Inv * V[3];
+  eKin = 0.5 * (u * u + v * v + w * w);
+  const double s = kGam - kGm1 * (V[0] - V[4] * eKin);
+  rho = pow(-V[4] * exp(s), -1.0 / kGm1);
+  p = -rho * V4Inv;
+}
+
+__device__ inline void atomic_max_pos(double *addr, double v)
+{
+  atomicMax(reinterpret_cast<unsigned long long *>(addr), (unsigned long long) __double_as_longlong(v));
+}
+__device__ inline void atomic_min_pos(double *addr, double v)
+{
+  atomicMin(reinterpret_cast<unsigned long long *>(addr), (unsigned long long) __double_as_longlong(v));
+}
+
+struct FaceArgs {
+  const double *U, *tables, *fgeo, *econst;
+  const int *nbr;
+  double *faceRes;
+  TableOffsets to;
+  int nElem, bug;
+  double mu, tauFact, thetaHalf;
+};
+
+// ============================================================ face kernel
+// One wave64 per element min face (blockIdx.x = dir * nElem + e).
+template <int K, int M>
+__global__ __launch_bounds__(64) void face_kernel(FaceArgs a)
+{
+  constexpr int ND = K * K * K, N2 = M * M;
+  __shared__ double sA[M * K], sD[M * K], sF[4][K], sW2[N2];
+  __shared__ double sU[2][5 * ND];
+  __shared__ double sB[2][2][5 * K * K];      // side, (value|normal-derivative)
+  __shared__ double sT[2][3][5 * K * M];      // side, (A, D, N)
+  __shared__ double sFlux[5 * N2], sSym[2][3][5 * N2];
+  __shared__ double sCm[2][5][9];
+  const int dir = blockIdx.x / a.nElem;
+  const int e = blockIdx.x - dir * a.nElem;
+  const int L = a.nbr[e * 6 + 2 * dir];
+  const int tid = threadIdx.x;
+  if (L < 0) return;  // physical boundary: handled by boundary kernels
+  for (int i = tid; i < M * K; i += 64) { sA[i] = a.tables[a.to.A + i]; sD[i] = a.tables[a.to.D + i]; }
+  for (int i = tid; i < K; i += 64) {
+    sF[0][i] = a.tables[a.to.fmin + i]; sF[1][i] = a.tables[a.to.fmax + i];
+    sF[2][i] = a.tables[a.to.dfmin + i]; sF[3][i] = a.tables[a.to.dfmax + i];
+  }
+  for (int i = tid; i < N2; i += 64) sW2[i] = a.tables[a.to.w2 + i];
+  // side 0 = right element (own min face), side 1 = left element (its max face)
+  const double *UR = a.U + (size_t) e * 5 * ND, *UL = a.U + (size_t) L * 5 * ND;
+  for (int i = tid; i < 5 * ND; i += 64) { sU[0][i] = UR[i]; sU[1][i] = UL[i]; }
+  const double *g = a.fgeo + ((size_t) e * 3 + dir) * kFaceGeo * N2;
+  if (a.bug)
+    for (int i = tid; i < 2 * 5 * 9; i += 64) {
+      const int side = i / 45, l = (i / 9) % 5, q = i % 9;
+      const int pt = l < N2 ? l : 0;
+      sCm[side][l][q] = g[(side ? 4 + q : 13 + q) * N2 + pt];
+    }
+  __syncthreads();
+  // strides of (normal, fast, slow) axes in the [k][j][i] DOF layout
+  const int sn = dir == 0 ? 1 : dir == 1 ? K : K * K;
+  const int sf = dir == 0 ? K : 1;
+  const int ss = dir == 2 ? K : K * K;
+  // normal contraction: b[side][0|1][v][s][f]
+  for (int idx = tid; idx < 2 * 5 * K * K; idx += 64) {
+    const int side = idx / (5 * K * K);
+    const int r = idx - side * 5 * K * K;
+    const int v = r / (K * K), s = (r / K) % K, f = r % K;
+    const double *fv = side ? sF[1] : sF[0];
+    const double *dfv = side ? sF[3] : sF[2];
+    const double *u = &sU[side][v * ND + s * ss + f * sf];
+    double b0 = 0.0, b1 = 0.0;
+#pragma unroll
+    for (int t = 0; t < K; ++t) { b0 += fv[t] * u[t * sn]; b1 += dfv[t] * u[t * sn]; }
+    sB[side][0][r] = b0; sB[side][1][r] = b1;
+  }
+  __syncthreads();
+  // fast tangential contraction: T[side][A|D|N][v][s][f'], f' < M
+  for (int idx = tid; idx < 2 * 5 * K * M; idx += 64) {
+    const int side = idx / (5 * K * M);
+    const int r = idx - side * 5 * K * M;
+    const int vs = r / M, fp = r % M;
+    const double *b0 = &sB[side][0][vs * K], *b1 = &sB[side][1][vs * K];
+    double tA = 0.0, tD = 0.0, tN = 0.0;
+#pragma unroll
+    for (int f = 0; f < K; ++f) {
+      tA += sA[fp * K + f] * b0[f]; tD += sD[fp * K + f] * b0[f]; tN += sA[fp * K + f] * b1[f];
+    }
+    sT[side][0][r] = tA; sT[side][1][r] = tD; sT[side][2][r] = tN;
+  }
+  __syncthreads();
+  const double lenScale = fmin(a.econst[(size_t) e * kElemConst + dir], a.econst[(size_t) L * kElemConst + dir]);
+  const double tau = a.tauFact / lenScale;
+  const double mu = a.mu, factHF = kGam / kPrLam;
+  const double lam = kLamOverMu, lamp1 = lam + 1.0, lamp2 = lam + 2.0;
+  for (int m = tid; m < N2; m += 64) {
+    const int sp = m / M, fp = m - sp * M;
+    // traces and parametric gradients of both sides
+    double V[2][5], Gr[2][5], Gs[2][5], Gt[2][5];
+#pragma unroll
+    for (int side = 0; side < 2; ++side)
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        const double *tA = &sT[side][0][v * K * M + fp], *tD = &sT[side][1][v * K * M + fp],
+                     *tN = &sT[side][2][v * K * M + fp];
+        double val = 0.0, gN = 0.0, gF = 0.0, gS = 0.0;
+#pragma unroll
+        for (int s = 0; s < K; ++s) {
+          const double as = sA[sp * K + s], ds = sD[sp * K + s];
+          val += as * tA[s * M]; gN += as * tN[s * M]; gF += as * tD[s * M]; gS += ds * tA[s * M];
+        }
+        V[side][v] = val;
+        if (dir == 0) { Gr[side][v] = gN; Gs[side][v] = gF; Gt[side][v] = gS; }
+        else if (dir == 1) { Gr[side][v] = gF; Gs[side][v] = gN; Gt[side][v] = gS; }
+        else { Gr[side][v] = gF; Gs[side][v] = gS; Gt[side][v] = gN; }
+      }
+    const double nn[3] = {g[0 * N2 + m], g[1 * N2 + m], g[2 * N2 + m]};
+    const double wA = sW2[m] * g[3 * N2 + m];
+    double mL[9], mR[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) { mL[q] = g[(4 + q) * N2 + m]; mR[q] = g[(13 + q) * N2 + m]; }
+    // averaged Cartesian gradients of the entropy variables
+    double gx[5], gy[5], gz[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      const double *cL = a.bug ? sCm[1][v] : mL, *cR = a.bug ? sCm[0][v] : mR;
+      const double xL = Gr[1][v] * cL[0] + Gs[1][v] * cL[3] + Gt[1][v] * cL[6];
+      const double yL = Gr[1][v] * cL[1] + Gs[1][v] * cL[4] + Gt[1][v] * cL[7];
+      const double zL = Gr[1][v] * cL[2] + Gs[1][v] * cL[5] + Gt[1][v] * cL[8];
+      const double xR = Gr[0][v] * cR[0] + Gs[0][v] * cR[3] + Gt[0][v] * cR[6];
+      const double yR = Gr[0][v] * cR[1] + Gs[0][v] * cR[4] + Gt[0][v] * cR[7];
+      const double zR = Gr[0][v] * cR[2] + Gs[0][v] * cR[5] + Gt[0][v] * cR[8];
+      gx[v] = 0.5 * (xL + xR); gy[v] = 0.5 * (yL + yR); gz[v] = 0.5 * (zL + zR);
+    }
+    // ---- Ismail-Roe (entropy form, beta = 1/6, alphaMax = 0)
+    double fl[5];
+    {
+      const double *VL = V[1], *VR = V[0];
+      double rL, uL, vL, wL, pL, eL, rR, uR, vR, wR, pR, eR;
+      ent2prim(VL, rL, uL, vL, wL, pL, eL);
+      ent2prim(VR, rR, uR, vR, wR, pR, eR);
+      const double dV0 = VR[0] - VL[0], dV1 = VR[1] - VL[1], dV2 = VR[2] - VL[2],
+                   dV3 = VR[3] - VL[3], dV4 = VR[4] - VL[4];
+      const double z0L = sqrt(rL / pL), z0R = sqrt(rR / pR);
+      const double z4L = sqrt(rL * pL), z4R = sqrt(rR * pR);
+      const double z0A = 0.5 * (z0L + z0R), z4A = 0.5 * (z4L + z4R);
+      const double z1A = 0.5 * (uL * z0L + uR * z0R), z2A = 0.5 * (vL * z0L + vR * z0R),
+                   z3A = 0.5 * (wL * z0L + wR * z0R);
+      double zeta = z0L / z0R, f = (zeta - 1.0) / (zeta + 1.0), uu = f * f, F;
+      F = (uu < 0.01) ? 1.0 + uu / 3.0 + uu * uu / 5.0 + uu * uu * uu / 7.0 + uu * uu * uu * uu / 9.0
+                      : log(zeta) / (2.0 * f);
+      const double z0Log = z0A / F;
+      zeta = z4L / z4R; f = (zeta - 1.0) / (zeta + 1.0); uu = f * f;
+      F = (uu < 0.01) ? 1.0 + uu / 3.0 + uu * uu / 5.0 + uu * uu * uu / 7.0 + uu * uu * uu * uu / 9.0
+                      : log(zeta) / (2.0 * f);
+      const double z4Log = z4A / F;
+      const double oz0 = 1.0 / z0A;
+      const double rA = z0A * z4Log, p1 = oz0 * z4A;
+      const double p2 = 0.5 * ((kGam + 1.0) / kGam * z4Log / z0Log + kGm1 / kGam * p1);
+      const double uA = oz0 * z1A, vA = oz0 * z2A, wAv = oz0 * z3A;
+      const double vn = uA * nn[0] + vA * nn[1] + wAv * nn[2];
+      const double kin = 0.5 * (uA * uA + vA * vA + wAv * wAv);
+      const double a2 = kGam * p2 / rA, aa = sqrt(a2);
+      const double H = a2 / kGm1 + kin, E = H - p2 / rA;
+      double abv1 = rA * (uA * dV1 + vA * dV2 + wAv * dV3);
+      double abv2 = abv1 + rA * (dV0 + H * dV4);
+      const double dr = abv1 + rA * (dV0 + E * dV4);
+      const double dru = uA * abv2 + p2 * dV1, drv = vA * abv2 + p2 * dV2, drw = wAv * abv2 + p2 * dV3;
+      const double drE = H * abv1 + rA * E * (dV0 + H * dV4) + p2 * kin * dV4;
+      const double fac = 0.5 * (1.0 + 1.0 / 6.0);
+      const double lam1 = fabs(vn + aa) * fac, lam2 = fabs(vn - aa) * fac, lam3 = 0.5 * fabs(vn);
+      abv1 = 0.5 * (lam1 + lam2); abv2 = 0.5 * (lam1 - lam2);
+      const double abv3 = abv1 - lam3;
+      const double abv4 = kGm1 * (kin * dr - uA * dru - vA * drv - wAv * drw + drE);
+      const double abv5 = nn[0] * dru + nn[1] * drv + nn[2] * drw - vn * dr;
+      const double abv6 = abv3 * abv4 / a2 + abv2 * abv5 / aa;
+      const double abv7 = abv2 * abv4 / aa + abv3 * abv5;
+      fl[0] = wA * (rA * vn - (lam3 * dr + abv6));
+      fl[1] = wA * (rA * vn * uA + p1 * nn[0] - (lam3 * dru + uA * abv6 + nn[0] * abv7));
+      fl[2] = wA * (rA * vn * vA + p1 * nn[1] - (lam3 * drv + vA * abv6 + nn[1] * abv7));
+      fl[3] = wA * (rA * vn * wAv + p1 * nn[2] - (lam3 * drw + wAv * abv6 + nn[2] * abv7));
+      fl[4] = wA * (rA * vn * H - (lam3 * drE + H * abv6 + vn * abv7));
+    }
+    // ---- viscous normal flux, penalty and symmetrising fluxes on the average
+    double Va[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) Va[v] = 0.5 * (V[0][v] + V[1][v]);
+    const double V4Inv = 1.0 / Va[4];
+    const double u = -V4Inv * Va[1], vv = -V4Inv * Va[2], w = -V4Inv * Va[3];
+    const double dudx = -V4Inv * (gx[1] + u * gx[4]), dvdx = -V4Inv * (gx[2] + vv * gx[4]), dwdx = -V4Inv * (gx[3] + w * gx[4]);
+    const double dudy = -V4Inv * (gy[1] + u * gy[4]), dvdy = -V4Inv * (gy[2] + vv * gy[4]), dwdy = -V4Inv * (gy[3] + w * gy[4]);
+    const double dudz = -V4Inv * (gz[1] + u * gz[4]), dvdz = -V4Inv * (gz[2] + vv * gz[4]), dwdz = -V4Inv * (gz[3] + w * gz[4]);
+    const double fe = V4Inv * V4Inv / kGm1;
+    const double dedx = fe * gx[4], dedy = fe * gy[4], dedz = fe * gz[4];
+    {
+      const double nx = wA * nn[0], ny = wA * nn[1], nz = wA * nn[2];
+      const double lamDiv = kLamOverMu * (dudx + dvdy + dwdz);
+      const double txx = mu * (2.0 * dudx + lamDiv), tyy = mu * (2.0 * dvdy + lamDiv), tzz = mu * (2.0 * dwdz + lamDiv);
+      const double txy = mu * (dudy + dvdx), txz = mu * (dudz + dwdx), tyz = mu * (dvdz + dwdy);
+      const double fh = mu * factHF;
+      const double qx = -fh * dedx, qy = -fh * dedy, qz = -fh * dedz;
+      fl[1] -= txx * nx + txy * ny + txz * nz;
+      fl[2] -= txy * nx + tyy * ny + tyz * nz;
+      fl[3] -= txz * nx + tyz * ny + tzz * nz;
+      fl[4] -= (u * txx + vv * txy + w * txz - qx) * nx + (u * txy + vv * tyy + w * tyz - qy) * ny
+             + (u * txz + vv * tyz + w * tzz - qz) * nz;
+    }
+    const double pOvRho = -V4Inv, eInt = pOvRho / kGm1;
+    const double dV1 = V[1][1] - V[0][1], dV2 = V[1][2] - V[0][2], dV3 = V[1][3] - V[0][3], dV4 = V[1][4] - V[0][4];
+    const double factFlux = pOvRho * wA * mu;
+    const double abvE = u * u + vv * vv + w * w + factHF * eInt;
+    const double un = u * nn[0] + vv * nn[1] + w * nn[2];
+    const double dVn = dV1 * nn[0] + dV2 * nn[1] + dV3 * nn[2];
+    const double ttau = tau * factFlux, abvMom = lamp1 * (dVn + un * dV4);
+    fl[1] += ttau * (abvMom * nn[0] + dV1 + u * dV4);
+    fl[2] += ttau * (abvMom * nn[1] + dV2 + vv * dV4);
+    fl[3] += ttau * (abvMom * nn[2] + dV3 + w * dV4);
+    fl[4] += ttau * (lamp1 * un * dVn + u * dV1 + vv * dV2 + w * dV3 + (abvE + lamp1 * un * un) * dV4);
+    const double sx = a.thetaHalf * factFlux * nn[0], sy = a.thetaHalf * factFlux * nn[1],
+                 sz = a.thetaHalf * factFlux * nn[2];
+    // -(K_x1 sx + K_x2 sy + K_x3 sz) dV, K symmetric viscous Jacobians
+    double FX[5], FY[5], FZ[5];
+    const double K11_14 = u * lamp2, K11_44 = abvE + u * u * lamp1;
+    const double K22_24 = vv * lamp2, K22_44 = abvE + vv * vv * lamp1;
+    const double K33_34 = w * lamp2, K33_44 = abvE + w * w * lamp1;
+    const double K12_14 = vv * lam, K12_42 = u * lam, K12_44 = u * vv * lamp1;
+    const double K13_14 = w * lam, K13_43 = u * lam, K13_44 = u * w * lamp1;
+    const double K23_24 = w * lam, K23_43 = vv * lam, K23_44 = vv * w * lamp1;
+    FX[0] = 0.0;
+    FX[1] = -(lamp2 * dV1 + K11_14 * dV4) * sx - (lam * dV2 + K12_14 * dV4) * sy - (lam * dV3 + K13_14 * dV4) * sz;
+    FX[2] = -(dV2 + vv * dV4) * sx - (dV1 + u * dV4) * sy;
+    FX[3] = -(dV3 + w * dV4) * sx - (dV1 + u * dV4) * sz;
+    FX[4] = -(K11_14 * dV1 + vv * dV2 + w * dV3 + K11_44 * dV4) * sx
+            - (vv * dV1 + K12_42 * dV2 + K12_44 * dV4) * sy - (w * dV1 + K13_43 * dV3 + K13_44 * dV4) * sz;
+    FY[0] = 0.0;
+    FY[1] = -(dV2 + vv * dV4) * sx - (dV1 + u * dV4) * sy;
+    FY[2] = -(lam * dV1 + K12_42 * dV4) * sx - (lamp2 * dV2 + K22_24 * dV4) * sy - (lam * dV3 + K23_24 * dV4) * sz;
+    FY[3] = -(dV3 + w * dV4) * sy - (dV2 + vv * dV4) * sz;
+    FY[4] = -(K12_14 * dV1 + u * dV2 + K12_44 * dV4) * sx
+            - (u * dV1 + K22_24 * dV2 + w * dV3 + K22_44 * dV4) * sy - (w * dV2 + K23_43 * dV3 + K23_44 * dV4) * sz;
+    FZ[0] = 0.0;
+    FZ[1] = -(dV3 + w * dV4) * sx - (dV1 + u * dV4) * sz;
+    FZ[2] = -(dV3 + w * dV4) * sy - (dV2 + vv * dV4) * sz;
+    FZ[3] = -(lam * dV1 + K13_43 * dV4) * sx - (lam * dV2 + K23_43 * dV4) * sy - (lamp2 * dV3 + K33_34 * dV4) * sz;
+    FZ[4] = -(K13_14 * dV1 + u * dV3 + K13_44 * dV4) * sx
+            - (K23_24 * dV2 + vv * dV3 + K23_44 * dV4) * sy - (u * dV1 + vv * dV2 + K33_34 * dV3 + K33_44 * dV4) * sz;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      sFlux[v * N2 + m] = fl[v];
+      const double *cL = a.bug ? sCm[1][v] : mL, *cR = a.bug ? sCm[0][v] : mR;
+      // parametric symmetrising fluxes: side 0 right (own), side 1 left
+      sSym[1][0][v * N2 + m] = FX[v] * cL[0] + FY[v] * cL[1] + FZ[v] * cL[2];
+      sSym[1][1][v * N2 + m] = FX[v] * cL[3] + FY[v] * cL[4] + FZ[v] * cL[5];
+      sSym[1][2][v * N2 + m] = FX[v] * cL[6] + FY[v] * cL[7] + FZ[v] * cL[8];
+      sSym[0][0][v * N2 + m] = FX[v] * cR[0] + FY[v] * cR[1] + FZ[v] * cR[2];
+      sSym[0][1][v * N2 + m] = FX[v] * cR[3] + FY[v] * cR[4] + FZ[v] * cR[5];
+      sSym[0][2][v * N2 + m] = FX[v] * cR[6] + FY[v] * cR[7] + FZ[v] * cR[8];
+    }
+  }
+  __syncthreads();
+  // ---- lift to both elements. Parametric sym slots (r,s,t) -> (normal, fast, slow)
+  const int iN = dir, iF = dir == 0 ? 1 : 0, iS = dir == 2 ? 1 : 2;
+  // P[side][0..2][v][s'][f] reusing sT
+  for (int idx = tid; idx < 2 * 5 * M * K; idx += 64) {
+    const int side = idx / (5 * M * K);
+    const int r = idx - side * 5 * M * K;
+    const int v = r / (M * K), s = (r / K) % M, f = r % K;
+    const double sign = side ? 1.0 : -1.0;
+    const double *fx = &sFlux[v * N2 + s * M], *yF = &sSym[side][iF][v * N2 + s * M],
+                 *yS = &sSym[side][iS][v * N2 + s * M], *yN = &sSym[side][iN][v * N2 + s * M];
+    double p1 = 0.0, p2 = 0.0, p3 = 0.0;
+#pragma unroll
+    for (int fp = 0; fp < M; ++fp) {
+      const double af = sA[fp * K + f], df = sD[fp * K + f];
+      p1 += af * sign * fx[fp] + df * yF[fp];
+      p2 += af * yS[fp];
+      p3 += af * yN[fp];
+    }
+    sT[side][0][r] = p1; sT[side][1][r] = p2; sT[side][2][r] = p3;
+  }
+  __syncthreads();
+  // G[side][0|1][v][s][f] reusing sB
+  for (int idx = tid; idx < 2 * 5 * K * K; idx += 64) {
+    const int side = idx / (5 * K * K);
+    const int r = idx - side * 5 * K * K;
+    const int v = r / (K * K), s = (r / K) % K, f = r % K;
+    double g1 = 0.0, g2 = 0.0;
+#pragma unroll
+    for (int sp = 0; sp < M; ++sp) {
+      const int j = (v * M + sp) * K + f;
+      g1 += sA[sp * K + s] * sT[side][0][j] + sD[sp * K + s] * sT[side][1][j];
+      g2 += sA[sp * K + s] * sT[side][2][j];
+    }
+    sB[side][0][r] = g1; sB[side][1][r] = g2;
+  }
+  __syncthreads();
+  double *outR = a.faceRes + ((size_t) e * 6 + 2 * dir) * 5 * ND;
+  double *outL = a.faceRes + ((size_t) L * 6 + 2 * dir + 1) * 5 * ND;
+  for (int idx = tid; idx < 2 * 5 * ND; idx += 64) {
+    const int side = idx / (5 * ND);
+    const int r = idx - side * 5 * ND;
+    const int v = r / ND, d = r - v * ND;
+    const int i = d % K, j = (d / K) % K, k = d / (K * K);
+    const int t = dir == 0 ? i : dir == 1 ? j : k;
+    const int f = dir == 0 ? j : i;
+    const int s = dir == 2 ? j : k;
+    const double *fv = side ? sF[1] : sF[0], *dfv = side ? sF[3] : sF[2];
+    const int gi = (v * K + s) * K + f;
+    const double val = fv[t] * sB[side][0][gi] + dfv[t] * sB[side][1][gi];
+    (side ? outL : outR)[r] = val;
+  }
+}
+
+struct VolArgs {
+  double *U;
+  const double *U0;
+  double *R;
+  const double *faceRes, *tables, *vgeo, *econst;
+  double *scal;
+  int *flags, *cgIter;
+  TableOffsets to;
+  int nElem, bug;
+  double mu;
+  StageCtl ctl;
+};
+
+// ========================================================== volume kernel
+template <int K, int M, int NT>
+__global__ __launch_bounds__(NT) void volume_kernel(VolArgs a)
+{
+  constexpr int ND = K * K * K, NI = M * M * M;
+  constexpr int PPT = (NI + NT - 1) / NT;          // integration points per thread
+  constexpr int DPT = (5 * ND + NT - 1) / NT;      // modal entries per thread
+  constexpr int FWD = 2 * 5 * M * K * K + 3 * 5 * M * M * K;
+  constexpr int BWD = 15 * NI + 10 * M * K * K;
+  constexpr int CGS = 5 * M * K * K + 5 * M * M * K + 5 * NI + 5 * M * K * K;
+  constexpr int BIG = FWD > BWD ? (FWD > CGS ? FWD : CGS) : (BWD > CGS ? BWD : CGS);
+  __shared__ double sA[M * K], sD[M * K], sAT[K * M], sDT[K * M], sBC[ND];
+  __shared__ double sU[5 * ND];
+  __shared__ double sBig[BIG];
+  __shared__ double sCm[5][9];
+  __shared__ double sRed[(NT / 64) * 3];
+  const int e = blockIdx.x;
+  const int tid = threadIdx.x;
+  const double *Ue = a.U + (size_t) e * 5 * ND;
+  const double *vg = a.vgeo + (size_t) e * kVolGeo * NI;
+  for (int i = tid; i < M * K; i += NT) {
+    sA[i] = a.tables[a.to.A + i]; sD[i] = a.tables[a.to.D + i];
+    sAT[i] = a.tables[a.to.AT + i]; sDT[i] = a.tables[a.to.DT + i];
+  }
+  for (int i = tid; i < ND; i += NT) sBC[i] = a.tables[a.to.bc + i];
+  for (int i = tid; i < 5 * ND; i += NT) sU[i] = Ue[i];
+  if (a.bug)
+    for (int i = tid; i < 45; i += NT) sCm[i / 9][i % 9] = vg[(1 + i % 9) * NI + i / 9];
+  __syncthreads();
+
+  // ---------------- step 1: solution and parametric gradients at the points
+  double *zA = sBig, *zD = zA + 5 * M * K * K;
+  double *yAA = zD + 5 * M * K * K, *yDA = yAA + 5 * M * M * K, *yAD = yDA + 5 * M * M * K;
+  contract<K, M, 5, K * K, K * K, K * K, NT>(sU, zA, sA);
+  contract<K, M, 5, K * K, K * K, K * K, NT>(sU, zD, sD);
+  __syncthreads();
+  contract<K, M, 5 * M, K, K, K, NT>(zA, yAA, sA);
+  contract<K, M, 5 * M, K, K, K, NT>(zA, yDA, sD);
+  contract<K, M, 5 * M, K, K, K, NT>(zD, yAD, sA);
+  __syncthreads();
+  double Vp[PPT][5], Gr[PPT][5], Gs[PPT][5], Gt[PPT][5];
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) {
+    const int m = tid + q * NT;
+    if (m < NI) {
+      const int i = m % M, jk = m / M;  // jk = k*M + j
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        const double *ra = &yAA[(v * M * M + jk) * K], *rd = &yDA[(v * M * M + jk) * K],
+                     *rz = &yAD[(v * M * M + jk) * K];
+        double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+#pragma unroll
+        for (int ii = 0; ii < K; ++ii) {
+          s0 += sA[i * K + ii] * ra[ii]; s1 += sD[i * K + ii] * ra[ii];
+          s2 += sA[i * K + ii] * rd[ii]; s3 += sA[i * K + ii] * rz[ii];
+        }
+        Vp[q][v] = s0; Gr[q][v] = s1; Gs[q][v] = s2; Gt[q][v] = s3;
+      }
+    }
+  }
+  // element-centre entropy variables for the CG preconditioner
+  double Vc[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    double s = 0.0;
+    for (int i = 0; i < ND; ++i) s += sBC[i] * sU[v * ND + i];
+    Vc[v] = s;
+  }
+  __syncthreads();
+
+  // ---------------- steps 3, 5, 6: pointwise fluxes, dU/dV, monitoring
+  double dUdV[PPT][15], wJ[PPT];
+  double *F0 = sBig, *F1 = F0 + 5 * NI, *F2 = F1 + 5 * NI;
+  const double factHF = a.mu * kGam / kPrLam;
+  double mach2 = 0.0;
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) {
+    const int m = tid + q * NT;
+    if (m >= NI) continue;
+    const double J = vg[m];
+    const double w3 = a.tables[a.to.w3 + m];
+    wJ[q] = w3 * J;
+    double gx[5], gy[5], gz[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      double c[9];
+      if (a.bug) {
+#pragma unroll
+        for (int t = 0; t < 9; ++t) c[t] = sCm[v][t];
+      } else {
+#pragma unroll
+        for (int t = 0; t < 9; ++t) c[t] = vg[(1 + t) * NI + m];
+      }
+      gx[v] = Gr[q][v] * c[0] + Gs[q][v] * c[3] + Gt[q][v] * c[6];
+      gy[v] = Gr[q][v] * c[1] + Gs[q][v] * c[4] + Gt[q][v] * c[7];
+      gz[v] = Gr[q][v] * c[2] + Gs[q][v] * c[5] + Gt[q][v] * c[8];
+    }
+    double rho, u, v, w, p, eKin;
+    ent2prim(Vp[q], rho, u, v, w, p, eKin);
+    const double V4Inv = 1.0 / Vp[q][4];
+    const double rE = p / kGm1 + rho * eKin;
+    const double dudx = -V4Inv * (gx[1] + u * gx[4]), dudy = -V4Inv * (gy[1] + u * gy[4]), dudz = -V4Inv * (gz[1] + u * gz[4]);
+    const double dvdx = -V4Inv * (gx[2] + v * gx[4]), dvdy = -V4Inv * (gy[2] + v * gy[4]), dvdz = -V4Inv * (gz[2] + v * gz[4]);
+    const double dwdx = -V4Inv * (gx[3] + w * gx[4]), dwdy = -V4Inv * (gy[3] + w * gy[4]), dwdz = -V4Inv * (gz[3] + w * gz[4]);
+    const double fe = V4Inv * V4Inv / kGm1;
+    const double dedx = fe * gx[4], dedy = fe * gy[4], dedz = fe * gz[4];
+    const double rH = rE + p, ru = rho * u, rv = rho * v, rw = rho * w;
+    dUdV[q][0] = rho; dUdV[q][1] = ru; dUdV[q][2] = rv; dUdV[q][3] = rw; dUdV[q][4] = rE;
+    dUdV[q][5] = ru * u + p; dUdV[q][6] = ru * v; dUdV[q][7] = ru * w; dUdV[q][8] = rH * u;
+    dUdV[q][9] = rv * v + p; dUdV[q][10] = rv * w; dUdV[q][11] = rH * v;
+    dUdV[q][12] = rw * w + p; dUdV[q][13] = rH * w; dUdV[q][14] = rE * rH / rho + p * eKin;
+    if (a.ctl.monitor) mach2 = fmax(mach2, rho * (u * u + v * v + w * w) / (kGam * p));
+    const double weight = -wJ[q];
+    const double mu = a.mu;
+    const double divV = kLamOverMu * (dudx + dvdy + dwdz);
+    const double txx = mu * (2.0 * dudx + divV), tyy = mu * (2.0 * dvdy + divV), tzz = mu * (2.0 * dwdz + divV);
+    const double txy = mu * (dudy + dvdx), txz = mu * (dudz + dwdx), tyz = mu * (dvdz + dwdy);
+    const double qx = -factHF * dedx, qy = -factHF * dedy, qz = -factHF * dedz;
+    const double fx[5] = {ru, ru * u - txx + p, rv * u - txy, rw * u - txz, (rE + p) * u - (u * txx + v * txy + w * txz - qx)};
+    const double fy[5] = {rv, ru * v - txy, rv * v - tyy + p, rw * v - tyz, (rE + p) * v - (u * txy + v * tyy + w * tyz - qy)};
+    const double fz[5] = {rw, ru * w - txz, rv * w - tyz, rw * w - tzz + p, (rE + p) * w - (u * txz + v * tyz + w * tzz - qz)};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double mx = weight * vg[(1 + 3 * d) * NI + m], my = weight * vg[(2 + 3 * d) * NI + m],
+                   mz = weight * vg[(3 + 3 * d) * NI + m];
+      double *Fd = d == 0 ? F0 : d == 1 ? F1 : F2;
+#pragma unroll
+      for (int l = 0; l < 5; ++l) Fd[l * NI + m] = fx[l] * mx + fy[l] * my + fz[l] * mz;
+    }
+  }
+  if (a.ctl.monitor) {
+    mach2 = wave_max(mach2);
+    if ((tid & 63) == 0) atomic_max_pos(&a.scal[1], mach2);
+  }
+  __syncthreads();
+
+  // ---------------- step 7: volume residual (three transposed contractions)
+  rows_inplace<M, K, 5 * M * M, NT>(F0, sDT);
+  rows_inplace<M, K, 5 * M * M, NT>(F1, sAT);
+  rows_inplace<M, K, 5 * M * M, NT>(F2, sAT);
+  __syncthreads();
+  double *Y01 = F2 + 5 * NI, *Y2 = Y01 + 5 * M * K * K;
+  contract2<M, K, 5 * M, K, M, K, NT>(F0, sAT, F1, sDT, Y01);
+  contract<M, K, 5 * M, K, M, K, NT>(F2, Y2, sAT);
+  __syncthreads();
+  contract2<M, K, 5, K * K, K * K, K * K, NT>(Y01, sAT, Y2, sDT, sU);
+  __syncthreads();
+
+  // ---------------- gather the six face contributions
+  double r[DPT];
+  const double *fr = a.faceRes + (size_t) e * 6 * 5 * ND;
+#pragma unroll
+  for (int q = 0; q < DPT; ++q) {
+    const int i = tid + q * NT;
+    r[q] = 0.0;
+    if (i < 5 * ND) {
+      double s = sU[i];
+#pragma unroll
+      for (int f = 0; f < 6; ++f) s += fr[f * 5 * ND + i];
+      r[q] = s;
+    }
+  }
+
+  // ---------------- entropy-variable mass matrix: flexible PCG
+  const double JacInv = 8.0 / a.econst[(size_t) e * kElemConst + 3];
+  double x[DPT], z[DPT];
+  double r0 = 0.0;
+#pragma unroll
+  for (int q = 0; q < DPT; ++q) { x[q] = 0.0; r0 = fmax(r0, fabs(r[q])); }
+  {
+    double red[1] = {r0};
+    block_reduce<NT, 1, 0>(red, sRed);
+    r0 = red[0];
+  }
+  int iters = 0;
+  if (r0 * JacInv > 1e-10) {
+    double rho, u, v, w, p, eKin;
+    ent2prim(Vc, rho, u, v, w, p, eKin);
+    const double pInv = 1.0 / p, a1 = kGm1 * rho * pInv * pInv, a2 = eKin * a1;
+    double dv[15];
+    dv[0] = JacInv * (kGam / kGm1 / rho + a2 * eKin); dv[1] = -JacInv * u * a2; dv[2] = -JacInv * v * a2;
+    dv[3] = -JacInv * w * a2; dv[4] = JacInv * (a2 - pInv); dv[5] = JacInv * (a1 * u * u + pInv);
+    dv[6] = JacInv * a1 * u * v; dv[7] = JacInv * a1 * u * w; dv[8] = -JacInv * a1 * u;
+    dv[9] = JacInv * (a1 * v * v + pInv); dv[10] = JacInv * a1 * v * w; dv[11] = -JacInv * a1 * v;
+    dv[12] = JacInv * (a1 * w * w + pInv); dv[13] = -JacInv * a1 * w; dv[14] = JacInv * a1;
+    // preconditioner z = dV/dU(centre) r: needs the five variables of one DOF,
+    // exchanged through LDS
+    // r (preconditioner exchange) and p (forward contraction input) share sU;
+    // every reuse is separated by a barrier
+    double *sR = sU, *sP = sU;
+    auto precond = [&]() {
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < DPT; ++q) { const int i = tid + q * NT; if (i < 5 * ND) sR[i] = r[q]; }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < DPT; ++q) {
+        const int i = tid + q * NT;
+        if (i < 5 * ND) {
+          const int l = i / ND, d = i - l * ND;
+          const double r0_ = sR[d], r1 = sR[ND + d], r2 = sR[2 * ND + d], r3 = sR[3 * ND + d], r4 = sR[4 * ND + d];
+          double s;
+          if (l == 0) s = dv[0] * r0_ + dv[1] * r1 + dv[2] * r2 + dv[3] * r3 + dv[4] * r4;
+          else if (l == 1) s = dv[1] * r0_ + dv[5] * r1 + dv[6] * r2 + dv[7] * r3 + dv[8] * r4;
+          else if (l == 2) s = dv[2] * r0_ + dv[6] * r1 + dv[9] * r2 + dv[10] * r3 + dv[11] * r4;
+          else if (l == 3) s = dv[3] * r0_ + dv[7] * r1 + dv[10] * r2 + dv[12] * r3 + dv[13] * r4;
+          else s = dv[4] * r0_ + dv[8] * r1 + dv[11] * r2 + dv[13] * r3 + dv[14] * r4;
+          z[q] = s;
+        }
+      }
+    };
+    precond();
+    double pv[DPT];
+    double rz = 0.0;
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) { pv[q] = z[q]; rz += r[q] * z[q]; }
+    { double red[1] = {rz}; block_reduce<NT, 1, 1>(red, sRed); rz = red[0]; }
+    double *cz = sBig, *cy = cz + 5 * M * K * K, *pI = cy + 5 * M * M * K, *by = pI + 5 * NI;
+    for (int it = 0;; ++it) {
+      if (it == 100) { atomicOr(a.flags, kFlagCGNotConverged); break; }
+      // Ap = M p: forward to the points, dU/dV * w J, transposed back
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < DPT; ++q) { const int i = tid + q * NT; if (i < 5 * ND) sP[i] = pv[q]; }
+      __syncthreads();
+      contract<K, M, 5, K * K, K * K, K * K, NT>(sP, cz, sA);
+      __syncthreads();
+      contract<K, M, 5 * M, K, K, K, NT>(cz, cy, sA);
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < PPT; ++q) {
+        const int m = tid + q * NT;
+        if (m < NI) {
+          const int i = m % M, jk = m / M;
+          double pp[5];
+#pragma unroll
+          for (int l = 0; l < 5; ++l) {
+            const double *ra = &cy[(l * M * M + jk) * K];
+            double s = 0.0;
+#pragma unroll
+            for (int ii = 0; ii < K; ++ii) s += sA[i * K + ii] * ra[ii];
+            pp[l] = wJ[q] * s;
+          }
+          const double *D = dUdV[q];
+          pI[0 * NI + m] = D[0] * pp[0] + D[1] * pp[1] + D[2] * pp[2] + D[3] * pp[3] + D[4] * pp[4];
+          pI[1 * NI + m] = D[1] * pp[0] + D[5] * pp[1] + D[6] * pp[2] + D[7] * pp[3] + D[8] * pp[4];
+          pI[2 * NI + m] = D[2] * pp[0] + D[6] * pp[1] + D[9] * pp[2] + D[10] * pp[3] + D[11] * pp[4];
+          pI[3 * NI + m] = D[3] * pp[0] + D[7] * pp[1] + D[10] * pp[2] + D[12] * pp[3] + D[13] * pp[4];
+          pI[4 * NI + m] = D[4] * pp[0] + D[8] * pp[1] + D[11] * pp[2] + D[13] * pp[3] + D[14] * pp[4];
+        }
+      }
+      __syncthreads();
+      rows_inplace<M, K, 5 * M * M, NT>(pI, sAT);
+      __syncthreads();
+      contract<M, K, 5 * M, K, M, K, NT>(pI, by, sAT);
+      __syncthreads();
+      contract<M, K, 5, K * K, K * K, K * K, NT>(by, cz, sAT);   // Ap in cz
+      __syncthreads();
+      double pAp = 0.0;
+#pragma unroll
+      for (int q = 0; q < DPT; ++q) { const int i = tid + q * NT; if (i < 5 * ND) pAp += pv[q] * cz[i]; }
+      { double red[1] = {pAp}; block_reduce<NT, 1, 1>(red, sRed); pAp = red[0]; }
+      const double alpha = rz / pAp;
+      double ap[DPT];
+      double rmax = 0.0;
+#pragma unroll
+      for (int q = 0; q < DPT; ++q) {
+        const int i = tid + q * NT;
+        ap[q] = (i < 5 * ND) ? cz[i] : 0.0;
+        x[q] += alpha * pv[q];
+        r[q] -= alpha * ap[q];
+        rmax = fmax(rmax, fabs(r[q]));
+      }
+      precond();
+      double zAp = 0.0, rzn = 0.0;
+#pragma unroll
+      for (int q = 0; q < DPT; ++q) { zAp += z[q] * ap[q]; rzn += r[q] * z[q]; }
+      {
+        double red[3] = {zAp, rzn, rmax};
+        block_reduce<NT, 3, 2>(red, sRed);
+        zAp = red[0]; rzn = red[1]; rmax = red[2];
+      }
+      iters = it + 1;
+      if (rmax < 1e-10 * r0) break;
+      const double beta = -alpha * zAp / rz;
+#pragma unroll
+      for (int q = 0; q < DPT; ++q) pv[q] = beta * pv[q] + z[q];
+      rz = rzn;
+    }
+  }
+  if (tid == 0) a.cgIter[e] = iters;
+  // ---------------- outputs: R and/or the in-place RK stage update
+  double *Re = a.R + (size_t) e * 5 * ND;
+  double *Uw = a.U + (size_t) e * 5 * ND;
+  const double *U0e = a.U0 + (size_t) e * 5 * ND;
+  bool bad = false;
+#pragma unroll
+  for (int q = 0; q < DPT; ++q) {
+    const int i = tid + q * NT;
+    if (i < 5 * ND) {
+      if (!isfinite(x[q])) bad = true;
+      if (a.ctl.store_res) Re[i] = x[q];
+      if (a.ctl.update) Uw[i] = (1.0 - a.ctl.a) * Uw[i] + a.ctl.a * U0e[i] - a.ctl.bdt * x[q];
+    }
+  }
+  if (bad) atomicOr(a.flags, kFlagNonFinite);
+}
+
+// ======================================================= time-step kernel
+struct DtArgs {
+  const double *U, *tables, *econst;
+  double *dtElem, *scal;
+  int *flags;
+  TableOffsets to;
+  int nElem, P;
+  double cfl, u_ref, p_ref, rho_ref, muDim;
+};
+
+template <int K>
+__global__ __launch_bounds__(64) void time_step_kernel(DtArgs a)
+{
+  constexpr int ND = K * K * K;
+  __shared__ double sL[K * K], sU[5 * ND], sT1[5 * ND], sT2[5 * ND];
+  __shared__ double sRed[3];
+  const int e = blockIdx.x, tid = threadIdx.x;
+  for (int i = tid; i < K * K; i += 64) sL[i] = a.tables[a.to.LD + i];
+  for (int i = tid; i < 5 * ND; i += 64) sU[i] = a.U[(size_t) e * 5 * ND + i];
+  __syncthreads();
+  contract<K, K, 5 * K * K, 1, 1, 1, 64>(sU, sT1, sL);
+  __syncthreads();
+  contract<K, K, 5 * K, K, K, K, 64>(sT1, sT2, sL);
+  __syncthreads();
+  contract<K, K, 5, K * K, K * K, K * K, 64>(sT2, sT1, sL);
+  __syncthreads();
+  const double *ec = a.econst + (size_t) e * kElemConst;
+  const double di = 1.0 / ec[0], dj = 1.0 / ec[1], dk = 1.0 / ec[2];
+  const double dl2 = di * di + dj * dj + dk * dk;
+  const double rvn = fmax(fmax(1.0, 2.0 + kLamOverMu), kGam / kPrLam) * dl2;
+  double radInv = 0.0, radVis = 0.0, a2min = 1e10;
+  for (int d = tid; d < ND; d += 64) {
+    double V[5];
+#pragma unroll
+    for (int l = 0; l < 5; ++l) V[l] = sT1[l * ND + d];
+    double rho, u, v, w, p, ek;
+    ent2prim(V, rho, u, v, w, p, ek);
+    rho *= a.rho_ref; u *= a.u_ref; v *= a.u_ref; w *= a.u_ref; p *= a.p_ref;
+    const double ri = 1.0 / rho, a2 = kGam * p * ri;
+    const double vI = u * ec[4] + v * ec[5] + w * ec[6];
+    const double vJ = u * ec[7] + v * ec[8] + w * ec[9];
+    const double vK = u * ec[10] + v * ec[11] + w * ec[12];
+    const double c = sqrt(fabs(a2));
+    radInv = fmax(radInv, (fabs(vI) + c) * di + (fabs(vJ) + c) * dj + (fabs(vK) + c) * dk);
+    radVis = fmax(radVis, ri * a.muDim * rvn);
+    a2min = fmin(a2min, a2);
+  }
+  radInv = wave_max(radInv); radVis = wave_max(radVis); a2min = -wave_max(-a2min);
+  if (tid == 0) {
+    const double f1[] = {2.0, 6.0, 12.0, 20.0, 28.0, 38.0, 50.0, 64.0, 80.0, 98.0};
+    const double f2[] = {6.0, 36.0, 150.0, 420.0, 980.0, 1975.0, 3575.0, 7000.0, 14000.0, 28000.0};
+    if (a2min < 0.0) atomicOr(a.flags, kFlagNegativeA2);
+    const double dt = a.cfl * a.u_ref / (f1[a.P] * radInv + f2[a.P] * radVis);
+    a.dtElem[e] = dt;
+    atomic_min_pos(&a.scal[0], dt);
+  }
+}
+
+__global__ void stage0_kernel(double *U, double *U0, const double *R, size_t n, double dt)
+{
+  for (size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x; i < n; i += (size_t) gridDim.x * blockDim.x) {
+    const double u = U[i];
+    U0[i] = u;
+    U[i] = u - dt * R[i];
+  }
+}
+
+template <int K, int M>
+hipError_t faces_impl(const DeviceCase &dc, hipStream_t s)
+{
+  FaceArgs a;
+  a.U = dc.U; a.tables = dc.tables; a.fgeo = dc.fgeo; a.econst = dc.econst; a.nbr = dc.nbr;
+  a.faceRes = dc.faceRes; a.to = dc.to; a.nElem = dc.nElem; a.bug = dc.bug_compat;
+  a.mu = dc.mu; a.tauFact = (dc.P + 1.0) * (dc.P + 1.0) * dc.rel_penalty; a.thetaHalf = 0.5 * dc.theta_sym;
+  hipLaunchKernelGGL((face_kernel<K, M>), dim3(3 * dc.nElem), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+template <int K, int M>
+hipError_t volume_impl(const DeviceCase &dc, const StageCtl &ctl, hipStream_t s)
+{
+  VolArgs a;
+  a.U = dc.U; a.U0 = dc.U0; a.R = dc.R; a.faceRes = dc.faceRes; a.tables = dc.tables;
+  a.vgeo = dc.vgeo; a.econst = dc.econst; a.scal = dc.scal; a.flags = dc.flags; a.cgIter = dc.cgIter;
+  a.to = dc.to; a.nElem = dc.nElem; a.bug = dc.bug_compat; a.mu = dc.mu; a.ctl = ctl;
+  hipLaunchKernelGGL((volume_kernel<K, M, 256>), dim3(dc.nElem), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+template <int K>
+hipError_t dt_impl(const DeviceCase &dc, hipStream_t s)
+{
+  DtArgs a;
+  a.U = dc.U; a.tables = dc.tables; a.econst = dc.econst; a.dtElem = dc.dtElem; a.scal = dc.scal;
+  a.flags = dc.flags; a.to = dc.to; a.nElem = dc.nElem; a.P = dc.P; a.cfl = dc.cfl;
+  a.u_ref = dc.u_ref; a.p_ref = dc.p_ref; a.rho_ref = dc.rho_ref; a.muDim = dc.mu * dc.p_ref / dc.u_ref;
+  hipLaunchKernelGGL((time_step_kernel<K>), dim3(dc.nElem), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+#define VCP_PAIRS(X) X(3, 4) X(4, 5) X(4, 6) X(5, 7) X(6, 8) X(7, 9)
+
+bool supported_pair(int K, int M)
+{
+#define X(k, m) if (K == k && M == m) return true;
+  VCP_PAIRS(X)
+#undef X
+  return false;
+}
+
+hipError_t launch_faces(const DeviceCase &dc, hipStream_t s)
+{
+#define X(k, m) if (dc.K == k && dc.M == m) return faces_impl<k, m>(dc, s);
+  VCP_PAIRS(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_volume(const DeviceCase &dc, const StageCtl &ctl, hipStream_t s)
+{
+#define X(k, m) if (dc.K == k && dc.M == m) return volume_impl<k, m>(dc, ctl, s);
+  VCP_PAIRS(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_time_step(const DeviceCase &dc, hipStream_t s)
+{
+  switch (dc.K) {
+    case 3: return dt_impl<3>(dc, s);
+    case 4: return dt_impl<4>(dc, s);
+    case 5: return dt_impl<5>(dc, s);
+    case 6: return dt_impl<6>(dc, s);
+    case 7: return dt_impl<7>(dc, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_stage0(const DeviceCase &dc, double dt, hipStream_t s)
+{
+  const size_t n = (size_t) dc.nElem * 5 * dc.nDOF;
+  const int blocks = (int) ((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536);
+  hipLaunchKernelGGL(stage0_kernel, dim3(blocks), dim3(256), 0, s, dc.U, dc.U0, dc.R, n, dt);
+  return hipGetLastError();
+}
+
+}  // namespace vcp

@@ -1,0 +1,57 @@
+// Device-side data of one case and the kernel launchers (gfx950, FP64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace vcp {
+
+// Offsets (in doubles) of the 1D tables in the device table block.
+struct TableOffsets {
+  int A, D, AT, DT, LD, fmin, fmax, dfmin, dfmax, w3, w2, bc, total;
+};
+
+// Per-face geometry slot layout, [nElem][3][kFaceGeo][nInt2D]:
+//   0..2 unit normal (L -> R), 3 area factor, 4..12 inverse metric of the
+//   left (max-face) element, 13..21 inverse metric of the right (min-face)
+//   element.
+constexpr int kFaceGeo = 22;
+// Per-element volume geometry, [nElem][kVolGeo][nInt]: J, dr/dx .. dt/dz.
+constexpr int kVolGeo = 10;
+// Per-element constants, [nElem][kElemConst]: len I,J,K, volume, aveNorm[9].
+constexpr int kElemConst = 16;
+
+// Device status flags.
+enum : int { kFlagCGNotConverged = 1, kFlagNegativeA2 = 2, kFlagNonFinite = 4 };
+
+struct DeviceCase {
+  int nElem = 0, K = 0, M = 0, nDOF = 0, nInt = 0, nInt2D = 0, P = 0;
+  int bug_compat = 1;
+  double mu = 0, theta_sym = 1, rel_penalty = 1;
+  double cfl = 1, u_ref = 1, p_ref = 1, rho_ref = 1;
+  TableOffsets to{};
+  double *tables = nullptr;
+  double *vgeo = nullptr, *fgeo = nullptr, *econst = nullptr;
+  int *nbr = nullptr;
+  double *U = nullptr, *U0 = nullptr, *R = nullptr;
+  double *faceRes = nullptr;      // [nElem][6][5][nDOF]
+  double *scal = nullptr;         // device scalars: [0] dt min (as bits), [1] Mach^2 max (bits)
+  int *flags = nullptr;
+  int *cgIter = nullptr;          // [nElem]
+  double *dtElem = nullptr;       // [nElem]
+};
+
+// Stage control for the fused volume/mass-matrix kernel.
+struct StageCtl {
+  int store_res;      // write R = M^-1 res
+  int update;         // U = (1-a) U + a U0 - bdt R in place
+  double a, bdt;
+  int monitor;        // Mach^2 max
+};
+
+bool supported_pair(int K, int M);
+hipError_t launch_faces(const DeviceCase &dc, hipStream_t s);
+hipError_t launch_volume(const DeviceCase &dc, const StageCtl &ctl, hipStream_t s);
+hipError_t launch_time_step(const DeviceCase &dc, hipStream_t s);
+hipError_t launch_stage0(const DeviceCase &dc, double dt, hipStream_t s);
+
+}  // namespace vcp

@@ -1,0 +1,67 @@
+// Host-side setup of a VCP3D block for the GPU path: 1D operator tables,
+// metric terms at volume and face integration points, neighbour maps and
+// reference conditions. Runs once per case, in FP64, on the host.
+#pragma once
+
+#include <string>
+#include <vector>
+
+namespace vcp {
+
+constexpr int kNVar = 5;
+constexpr double kGamma = 1.4;            // VCP3D_CurviLinear.hpp:174
+constexpr double kPrandtlLam = 0.72;      // VCP3D_CurviLinear.hpp:175
+constexpr double kRGas = 287.058;         // VCP3D_CurviLinear.hpp:177
+constexpr double kLambdaOverMu = -2.0 / 3.0;
+constexpr double kMuDim = 1.716e-5;       // InputParamClass.cpp:607
+
+// Input parameters of the path; field meaning follows InputParamClass.
+struct CaseParams {
+  int n_elem[3] = {0, 0, 0};
+  int n_poly_grid = 0, n_poly_sol = 0, n_poly_int_rule = 0;
+  int dof_location = 2;        // 1 equidistant, 2 LGL
+  int periodic[3] = {1, 1, 1};
+  double mach = 0, reynolds = 0, reynolds_length = 1, temp_free_stream = 0;
+  double theta_sym = 1, rel_penalty = 1, cfl = 1, dt_synchr = 0;
+  int bug_compat = 1;
+};
+
+struct Tables {
+  int P = 0, K = 0, M = 0, Kg = 0;
+  std::vector<double> r_int, w_int, r_dof, r_grid;
+  // row-major (n_out x n_in) operators
+  std::vector<double> A, D;       // M x K: phi_q(r_p), phi'_q(r_p)
+  std::vector<double> AT, DT;     // K x M
+  std::vector<double> LD;         // K x K: phi_q(r_dof_p)
+  std::vector<double> VI;         // inverse of LD
+  std::vector<double> f_min, f_max, df_min, df_max;  // K
+  std::vector<double> w3, w2, basis_center;
+};
+
+struct Geometry {
+  // per element, as the reference stores it
+  std::vector<double> vol;        // [nElem][13][nInt]   J, dr/dx..dt/dz, x,y,z
+  std::vector<double> face[6];    // [nElem][16][nInt2D] iMin,iMax,jMin,jMax,kMin,kMax
+  std::vector<double> lens;       // [nElem][4] len I, J, K, volume
+  std::vector<double> ave_norm;   // [nElem][9]
+  std::vector<int> nbr;           // [nElem][6], -1 physical boundary
+};
+
+struct RefConditions { double p_ref, rho_ref, u_ref, mu; };
+
+int n_int1d(int rule);
+void build_tables(const CaseParams &p, Tables &t);
+// Computes metrics from per-element grid DOFs [nElem][3][Kg^3]; returns an
+// error message or "" on success.
+std::string build_geometry(const CaseParams &p, Tables &t, const double *elem_grid, Geometry &g);
+RefConditions reference_conditions(const CaseParams &p);
+
+// Nodal dimensional conservative [nDOFs][5] of one element -> modal entropy
+// variables [5][nDOFs] (SolverClass::ReadRestartSolution).
+void nodal_cons_to_modal_entropy(const Tables &t, const RefConditions &rc,
+                                 const double *nodal, double *modal);
+// Modal entropy [5][nDOFs] -> nodal dimensional conservative [nDOFs][5].
+void modal_entropy_to_nodal_cons(const Tables &t, const RefConditions &rc,
+                                 const double *modal, double *nodal);
+
+}  // namespace vcp

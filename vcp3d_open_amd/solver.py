@@ -1,0 +1,183 @@
+"""Python mirror of the reference SolverClass hot path over the C ABI
+(include/vcp3d_amd.h), loading the in-tree gfx950 library.
+
+Method names follow SolverClass (VCP3D_CurviLinear/include/SolverClass.hpp:
+40-124): Residual, ExecuteTimeSynchronizationStep, ReadRestartSolution,
+WriteRestartSolution. There is no CPU fallback: if the library is missing
+the import fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvcp3d_amd.so")
+
+
+class CaseDesc(C.Structure):
+    """vcp_case_desc (include/vcp3d_amd.h)."""
+    _fields_ = [
+        ("n_elem", C.c_int * 3),
+        ("n_poly_grid", C.c_int),
+        ("n_poly_sol", C.c_int),
+        ("n_poly_int_rule", C.c_int),
+        ("dof_location", C.c_int),
+        ("periodic", C.c_int * 3),
+        ("mach", C.c_double), ("reynolds", C.c_double),
+        ("reynolds_length", C.c_double), ("temp_free_stream", C.c_double),
+        ("theta_sym", C.c_double), ("rel_penalty", C.c_double),
+        ("cfl", C.c_double), ("dt_synchr", C.c_double),
+        ("bug_compat", C.c_int),
+    ]
+
+
+EXPORTS = ["vcp_create", "vcp_destroy", "vcp_sizes", "vcp_reference", "vcp_set_restart",
+           "vcp_set_solution", "vcp_get_solution", "vcp_get_nodal", "vcp_residual",
+           "vcp_get_residual", "vcp_sync_step", "vcp_time_steps", "vcp_cg_iterations",
+           "vcp_profile_residual", "vcp_last_error"]
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libvcp3d_amd.so and declares the C ABI signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with `python vcp3d_open_amd/build.py`")
+    L = C.CDLL(path)
+    dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+    ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+    vp = C.c_void_p
+    L.vcp_create.argtypes = [C.POINTER(CaseDesc), dp, C.POINTER(vp)]
+    L.vcp_destroy.argtypes = [vp]
+    L.vcp_sizes.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.vcp_reference.argtypes = [vp, dp]
+    L.vcp_set_restart.argtypes = [vp, dp]
+    L.vcp_set_solution.argtypes = [vp, dp]
+    L.vcp_get_solution.argtypes = [vp, dp]
+    L.vcp_get_nodal.argtypes = [vp, dp]
+    L.vcp_residual.argtypes = [vp, C.c_int, dp]
+    L.vcp_get_residual.argtypes = [vp, dp]
+    L.vcp_sync_step.argtypes = [vp, dp]
+    L.vcp_time_steps.argtypes = [vp, dp]
+    L.vcp_cg_iterations.argtypes = [vp, ip]
+    L.vcp_profile_residual.argtypes = [vp, C.c_int, dp]
+    L.vcp_last_error.restype = C.c_char_p
+    for name in EXPORTS:
+        if name != "vcp_last_error":
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+class VCPError(RuntimeError):
+    pass
+
+
+def desc_from_case(case, bug_compat: bool = True) -> CaseDesc:
+    d = CaseDesc()
+    d.n_elem[:] = list(case.n_elem)
+    d.n_poly_grid = case.p
+    d.n_poly_sol = case.p
+    d.n_poly_int_rule = case.rule
+    d.dof_location = 2
+    d.periodic[:] = [1, 1, 1]
+    d.mach, d.reynolds, d.reynolds_length = case.mach, case.reynolds, 1.0
+    d.temp_free_stream = case.temp
+    d.theta_sym, d.rel_penalty = case.theta_sym, case.rel_penalty
+    d.cfl, d.dt_synchr = case.cfl, case.dt_synchr
+    d.bug_compat = 1 if bug_compat else 0
+    return d
+
+
+class SolverClass:
+    """One case on the GPU. Mirrors the reference SolverClass hot-path API."""
+
+    def __init__(self, case, bug_compat: bool = True, elem_grid: np.ndarray | None = None):
+        self._L = load_library()
+        self.case = case
+        desc = desc_from_case(case, bug_compat)
+        grid = np.ascontiguousarray(case.element_grid() if elem_grid is None else elem_grid,
+                                    dtype=np.float64)
+        h = C.c_void_p()
+        self._check(self._L.vcp_create(C.byref(desc), grid, C.byref(h)))
+        self.h = h
+        ne, k, m = C.c_int(), C.c_int(), C.c_int()
+        self._L.vcp_sizes(h, C.byref(ne), C.byref(k), C.byref(m))
+        self.n_elem, self.K, self.M = ne.value, k.value, m.value
+        self.n_dofs = self.K ** 3
+
+    def _check(self, rc: int) -> int:
+        if rc < 0:
+            raise VCPError(self._L.vcp_last_error().decode())
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.vcp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- reference-named API ------------------------------------------------
+    def ReadRestartSolution(self, nodal: np.ndarray) -> None:
+        self._check(self._L.vcp_set_restart(self.h, np.ascontiguousarray(nodal, dtype=np.float64)))
+
+    def Residual(self, compute_monitoring: bool = True) -> np.ndarray:
+        md = np.zeros(8)
+        self._check(self._L.vcp_residual(self.h, 1 if compute_monitoring else 0, md))
+        return md
+
+    def ExecuteTimeSynchronizationStep(self) -> np.ndarray:
+        md = np.zeros(8)
+        self._check(self._L.vcp_sync_step(self.h, md))
+        return md
+
+    def WriteRestartSolution(self) -> np.ndarray:
+        """Nodal dimensional conservative payload of the restart file."""
+        out = np.zeros((self.n_elem, self.n_dofs, 5))
+        self._check(self._L.vcp_get_nodal(self.h, out))
+        return out
+
+    # -- direct access ------------------------------------------------------
+    def reference(self) -> dict:
+        out = np.zeros(4)
+        self._check(self._L.vcp_reference(self.h, out))
+        return dict(u_ref=out[0], p_ref=out[1], rho_ref=out[2], mu=out[3])
+
+    def set_solution(self, modal: np.ndarray) -> None:
+        self._check(self._L.vcp_set_solution(self.h, np.ascontiguousarray(modal, dtype=np.float64)))
+
+    def solution(self) -> np.ndarray:
+        out = np.zeros((self.n_elem, 5, self.n_dofs))
+        self._check(self._L.vcp_get_solution(self.h, out))
+        return out
+
+    def residual_array(self) -> np.ndarray:
+        out = np.zeros((self.n_elem, 5, self.n_dofs))
+        self._check(self._L.vcp_get_residual(self.h, out))
+        return out
+
+    def time_steps(self) -> np.ndarray:
+        out = np.zeros(self.n_elem)
+        self._check(self._L.vcp_time_steps(self.h, out))
+        return out
+
+    def cg_iterations(self) -> np.ndarray:
+        out = np.zeros(self.n_elem, dtype=np.int32)
+        self._check(self._L.vcp_cg_iterations(self.h, out))
+        return out
+
+    def profile_residual(self, reps: int = 10) -> tuple:
+        ms = np.zeros(2)
+        self._check(self._L.vcp_profile_residual(self.h, reps, ms))
+        return float(ms[0]), float(ms[1])

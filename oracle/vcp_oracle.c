@@ -24,6 +24,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <omp.h>
 
 #define NVAR 5
 static const double GAM = 1.4;
@@ -58,9 +59,13 @@ struct orc_case {
   double *sol, *solOld, *res, *resMin[3], *dUdV;
   int *cgIt;
   int resValid, timeStep;
-  /* scratch */
-  double *work;
+  /* per-thread scratch */
+  double **works;
+  int nthreads;
+  size_t work_size;
 };
+
+static double *W(const orc_case *c) { return c->works[omp_get_thread_num()]; }
 
 /* ------------------------------------------------------------------ utils */
 static double *dalloc(size_t n) { return (double *) calloc(n ? n : 1, sizeof(double)); }
@@ -351,7 +356,7 @@ static int metrics(orc_case *c, const double *grid)
   c->aveNorm = dalloc((size_t) c->nElem * 9);
   double *xyz[4][3];
   for (int a = 0; a < 4; ++a) for (int b = 0; b < 3; ++b) xyz[a][b] = dalloc(nInt);
-  double *w = c->work;
+  double *w = W(c);
   for (int e = 0; e < c->nElem; ++e) {
     const double *g = grid + (size_t) e * 3 * nG;
     double *vm = c->vm + (size_t) e * 13 * nInt;
@@ -485,9 +490,9 @@ static void volume_residual(orc_case *c, int e, int mon, double *mach2max)
   const double *vm = c->vm + (size_t) e * 13 * nInt;
   double *res = c->res + (size_t) e * NVAR * nD;
   double *dUdV = c->dUdV + (size_t) e * 15 * nInt;
-  double *W = c->work + 8 * (size_t) nInt;
-  double *Var = W, *Dr = W + NVAR * nInt, *Ds = W + 2 * NVAR * nInt, *Dt = W + 3 * NVAR * nInt;
-  double *w = c->work;
+  double *Wb = W(c) + 8 * (size_t) nInt;
+  double *Var = Wb, *Dr = Wb + NVAR * nInt, *Ds = Wb + 2 * NVAR * nInt, *Dt = Wb + 3 * NVAR * nInt;
+  double *w = W(c);
   for (int l = 0; l < NVAR; ++l) {
     tp3(K, K, K, M, M, M, c->A, c->A, c->A, sol + l * nD, Var + l * nInt, 0, w);
     tp3(K, K, K, M, M, M, c->D, c->A, c->A, sol + l * nD, Dr + l * nInt, 0, w);
@@ -562,7 +567,7 @@ static void face_trace(const orc_case *c, int f, const double *sol, double *S, d
 {
   const int K = c->K, M = c->M, n2 = c->nInt2D, nD = c->nDOFs, dir = f / 2, side = f % 2;
   const double *F = side ? c->FMax : c->FMin, *dF = side ? c->dFMax : c->dFMin;
-  double *w = c->work;
+  double *w = W(c);
   for (int l = 0; l < NVAR; ++l) {
     const double *b = sol + l * nD;
     if (dir == 0) {
@@ -592,7 +597,7 @@ static void face_lift(const orc_case *c, int f, const double *flux, const double
 {
   const int K = c->K, M = c->M, n2 = c->nInt2D, nD = c->nDOFs, dir = f / 2, side = f % 2;
   const double *F = side ? c->FMax : c->FMin, *dF = side ? c->dFMax : c->dFMin;
-  double *w = c->work;
+  double *w = W(c);
   const double *AT = c->AT, *DT = c->DT;
   for (int l = 0; l < NVAR; ++l) {
     double *r = res + l * nD;
@@ -680,7 +685,7 @@ static void internal_face_flux(const orc_case *c, double lenScale,
   const int n2 = c->nInt2D;
   const int bug = c->prm.bug_compat;
   /* Cartesian gradients, averaged (FluxesFace.cpp:263-289) */
-  double *gx = c->work + 64 * (size_t) n2, *gy = gx + 5 * n2, *gz = gy + 5 * n2;
+  double *gx = W(c) + 64 * (size_t) n2, *gy = gx + 5 * n2, *gz = gy + 5 * n2;
   for (int l = 0; l < 5; ++l)
     for (int m = 0; m < n2; ++m) {
       const int q = bug ? (l < n2 ? l : 0) : m;
@@ -788,7 +793,7 @@ static int face_residual(orc_case *c, int dir, int e)
   const int n2 = c->nInt2D, nD = c->nDOFs;
   const int L = c->nbr[e * 6 + 2 * dir];
   if (L < 0) return fail("physical boundary faces are not part of this oracle");
-  double *buf = c->work + 80 * (size_t) n2 + 64;
+  double *buf = W(c) + 80 * (size_t) n2 + 64;
   double *SL = buf, *GrL = SL + 5 * n2, *GsL = GrL + 5 * n2, *GtL = GsL + 5 * n2;
   double *SR = GtL + 5 * n2, *GrR = SR + 5 * n2, *GsR = GrR + 5 * n2, *GtR = GsR + 5 * n2;
   double *flux = GtR + 5 * n2, *sxL = flux + 5 * n2, *syL = sxL + 5 * n2, *szL = syL + 5 * n2;
@@ -815,8 +820,8 @@ static int cg_entropy(orc_case *c, int e)
   const double *Jac = c->vm + (size_t) e * 13 * nInt;
   const double *dUdV = c->dUdV + (size_t) e * 15 * nInt;
   double *rv = c->res + (size_t) e * NVAR * nD;
-  double *z = c->work + 8 * (size_t) nInt, *p = z + 5 * nD, *Ap = p + 5 * nD, *x = Ap + 5 * nD, *pI = x + 5 * nD;
-  double *w = c->work;
+  double *z = W(c) + 8 * (size_t) nInt, *p = z + 5 * nD, *Ap = p + 5 * nD, *x = Ap + 5 * nD, *pI = x + 5 * nD;
+  double *w = W(c);
   const double JacInv = 8.0 / c->len[(size_t) e * 4 + 3];
   double r0 = 0.0;
   for (int i = 0; i < 5 * nD; ++i) { r0 = fmax(r0, fabs(rv[i])); x[i] = 0.0; }
@@ -899,7 +904,10 @@ orc_case *orc_create(const orc_params *prm, const double *elem_grid)
   c->uRef = sqrt(c->pRef / c->rhoRef);
   c->mu = muDim * c->uRef / c->pRef;
   size_t big = (size_t) c->nInt; if ((size_t) c->nGrid > big) big = c->nGrid;
-  c->work = dalloc(200 * big + 400 * (size_t) c->nInt2D + 4096);
+  c->work_size = 200 * big + 400 * (size_t) c->nInt2D + 4096;
+  c->nthreads = omp_get_max_threads();
+  c->works = (double **) calloc(c->nthreads, sizeof(double *));
+  for (int t = 0; t < c->nthreads; ++t) c->works[t] = dalloc(c->work_size);
   standard_element(c);
   if (metrics(c, elem_grid)) { orc_destroy(c); return NULL; }
   connectivity(c);
@@ -917,8 +925,10 @@ void orc_destroy(orc_case *c)
   double *ptrs[] = {c->rInt, c->wInt, c->rDOF, c->rGrid, c->A, c->D, c->AT, c->DT, c->LD, c->VI,
                     c->FMin, c->FMax, c->dFMin, c->dFMax, c->w3, c->w2, c->basisCenter, c->vm,
                     c->sm[0], c->sm[1], c->sm[2], c->sm[3], c->sm[4], c->sm[5], c->len, c->aveNorm,
-                    c->sol, c->solOld, c->res, c->resMin[0], c->resMin[1], c->resMin[2], c->dUdV, c->work};
+                    c->sol, c->solOld, c->res, c->resMin[0], c->resMin[1], c->resMin[2], c->dUdV};
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
+  if (c->works) for (int t = 0; t < c->nthreads; ++t) free(c->works[t]);
+  free(c->works);
   free(c->nbr); free(c->cgIt); free(c);
 }
 
@@ -952,7 +962,7 @@ int orc_set_restart(orc_case *c, const double *nodal)
       node[3 * nD + d] = rho * w * pInv; node[4 * nD + d] = -rho * pInv;
     }
     for (int l = 0; l < 5; ++l)
-      tp3(K, K, K, K, K, K, c->VI, c->VI, c->VI, node + l * nD, c->sol + ((size_t) e * 5 + l) * nD, 0, c->work);
+      tp3(K, K, K, K, K, K, c->VI, c->VI, c->VI, node + l * nD, c->sol + ((size_t) e * 5 + l) * nD, 0, W(c));
   }
   free(node);
   c->resValid = 0;
@@ -983,7 +993,7 @@ void orc_get_nodal(const orc_case *c, double *nodal)
   const double momRef = c->rhoRef * c->uRef;
   for (int e = 0; e < c->nElem; ++e) {
     for (int l = 0; l < 5; ++l)
-      tp3(K, K, K, K, K, K, c->LD, c->LD, c->LD, c->sol + ((size_t) e * 5 + l) * nD, node + l * nD, 0, c->work);
+      tp3(K, K, K, K, K, K, c->LD, c->LD, c->LD, c->sol + ((size_t) e * 5 + l) * nD, node + l * nD, 0, W(c));
     for (int d = 0; d < nD; ++d) {
       double V[5], rho, u, v, w, p, eKin;
       for (int l = 0; l < 5; ++l) V[l] = node[l * nD + d];
@@ -1000,12 +1010,22 @@ void orc_get_nodal(const orc_case *c, double *nodal)
 int orc_residual(orc_case *c, int mon, double md[8])
 {
   double mach2 = 0.0;
-  for (int e = 0; e < c->nElem; ++e) volume_residual(c, e, mon, &mach2);
-  for (int d = 0; d < 3; ++d)
+  int err = 0;
+#pragma omp parallel for schedule(static) reduction(max : mach2)
+  for (int e = 0; e < c->nElem; ++e) {
+    double m2 = 0.0;
+    volume_residual(c, e, mon, &m2);
+    mach2 = fmax(mach2, m2);
+  }
+  for (int d = 0; d < 3; ++d) {
+#pragma omp parallel for schedule(static) reduction(| : err)
     for (int e = 0; e < c->nElem; ++e)
-      if (face_residual(c, d, e)) return -1;
+      if (face_residual(c, d, e)) err |= 1;
+  }
+  if (err) return -1;
   const size_t nv = (size_t) NVAR * c->nDOFs;
   for (int d = 0; d < 3; ++d)
+#pragma omp parallel for schedule(static)
     for (int e = 0; e < c->nElem; ++e) {
       const int R = c->nbr[e * 6 + 2 * d + 1];
       if (R < 0) continue;
@@ -1013,8 +1033,10 @@ int orc_residual(orc_case *c, int mon, double md[8])
       const double *s = c->resMin[d] + R * nv;
       for (size_t i = 0; i < nv; ++i) r[i] += s[i];
     }
+#pragma omp parallel for schedule(dynamic, 4) reduction(| : err)
   for (int e = 0; e < c->nElem; ++e)
-    if (cg_entropy(c, e)) return -1;
+    if (cg_entropy(c, e)) err |= 1;
+  if (err) return -1;
   for (int i = 0; i < 6; ++i) md[i] = 0.0;
   md[6] = mach2; md[7] = mon ? 0.0 : -1.0;
   c->resValid = 1;
@@ -1028,35 +1050,41 @@ int orc_time_steps(orc_case *c, double *dt)
   static const double f2[] = {6.0, 36.0, 150.0, 420.0, 980.0, 1975.0, 3575.0, 7000.0, 14000.0, 28000.0};
   const int K = c->K, nD = c->nDOFs;
   const double muDim = c->mu * c->pRef / c->uRef;
-  double *node = dalloc(5 * (size_t) nD);
-  double rvl = fmax(fmax(1.0, 2.0 + LAMBDA_OVER_MU), GAM / PRANDTL_LAM);
-  for (int e = 0; e < c->nElem; ++e) {
-    const double *ln = c->len + (size_t) e * 4, *an = c->aveNorm + (size_t) e * 9;
-    const double di = 1.0 / ln[0], dj = 1.0 / ln[1], dk = 1.0 / ln[2];
-    const double dl2 = di * di + dj * dj + dk * dk;
-    const double radVisOverNu = rvl * dl2;
-    for (int l = 0; l < 5; ++l)
-      tp3(K, K, K, K, K, K, c->LD, c->LD, c->LD, c->sol + ((size_t) e * 5 + l) * nD, node + l * nD, 0, c->work);
-    double radInvMax = 0.0, radVisMax = 0.0, a2Min = 1e10;
-    for (int d = 0; d < nD; ++d) {
-      double V[5], rho, u, v, w, p, eKin;
-      for (int l = 0; l < 5; ++l) V[l] = node[l * nD + d];
-      ent2prim(V, &rho, &u, &v, &w, &p, &eKin);
-      rho *= c->rhoRef; u *= c->uRef; v *= c->uRef; w *= c->uRef; p *= c->pRef;
-      const double ri = 1.0 / rho, a2 = GAM * p * ri;
-      const double vI = u * an[0] + v * an[1] + w * an[2];
-      const double vJ = u * an[3] + v * an[4] + w * an[5];
-      const double vK = u * an[6] + v * an[7] + w * an[8];
-      const double a = sqrt(fabs(a2));
-      const double ri_ = (fabs(vI) + a) * di + (fabs(vJ) + a) * dj + (fabs(vK) + a) * dk;
-      const double rv_ = ri * (muDim * radVisOverNu);
-      radInvMax = fmax(radInvMax, ri_); radVisMax = fmax(radVisMax, rv_); a2Min = fmin(a2Min, a2);
+  const double rvl = fmax(fmax(1.0, 2.0 + LAMBDA_OVER_MU), GAM / PRANDTL_LAM);
+  int bad = 0;
+#pragma omp parallel reduction(| : bad)
+  {
+    double *node = dalloc(5 * (size_t) nD);
+#pragma omp for schedule(static)
+    for (int e = 0; e < c->nElem; ++e) {
+      const double *ln = c->len + (size_t) e * 4, *an = c->aveNorm + (size_t) e * 9;
+      const double di = 1.0 / ln[0], dj = 1.0 / ln[1], dk = 1.0 / ln[2];
+      const double dl2 = di * di + dj * dj + dk * dk;
+      const double radVisOverNu = rvl * dl2;
+      for (int l = 0; l < 5; ++l)
+        tp3(K, K, K, K, K, K, c->LD, c->LD, c->LD, c->sol + ((size_t) e * 5 + l) * nD, node + l * nD, 0, W(c));
+      double radInvMax = 0.0, radVisMax = 0.0, a2Min = 1e10;
+      for (int d = 0; d < nD; ++d) {
+        double V[5], rho, u, v, w, p, eKin;
+        for (int l = 0; l < 5; ++l) V[l] = node[l * nD + d];
+        ent2prim(V, &rho, &u, &v, &w, &p, &eKin);
+        rho *= c->rhoRef; u *= c->uRef; v *= c->uRef; w *= c->uRef; p *= c->pRef;
+        const double ri = 1.0 / rho, a2 = GAM * p * ri;
+        const double vI = u * an[0] + v * an[1] + w * an[2];
+        const double vJ = u * an[3] + v * an[4] + w * an[5];
+        const double vK = u * an[6] + v * an[7] + w * an[8];
+        const double a = sqrt(fabs(a2));
+        const double ri_ = (fabs(vI) + a) * di + (fabs(vJ) + a) * dj + (fabs(vK) + a) * dk;
+        const double rv_ = ri * (muDim * radVisOverNu);
+        radInvMax = fmax(radInvMax, ri_); radVisMax = fmax(radVisMax, rv_); a2Min = fmin(a2Min, a2);
+      }
+      if (a2Min < 0.0) bad = 1;
+      const double dtInv = f1[c->P] * radInvMax + f2[c->P] * radVisMax;
+      dt[e] = c->prm.cfl * c->uRef / dtInv;
     }
-    if (a2Min < 0.0) { free(node); return fail("Negative square speed of sound"); }
-    const double dtInv = f1[c->P] * radInvMax + f2[c->P] * radVisMax;
-    dt[e] = c->prm.cfl * c->uRef / dtInv;
+    free(node);
   }
-  free(node);
+  if (bad) return fail("Negative square speed of sound");
   return 0;
 }
 

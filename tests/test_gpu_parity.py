@@ -19,7 +19,7 @@ CASES = [
     ("curv_n4_p4", dict(n=4, p=4, curvilinear=True)),
     ("cart_n3_p2", dict(n=3, p=2)),
     ("curv_n4_p5", dict(n=4, p=5, curvilinear=True)),
-    ("cart_n2_p6", dict(n=2, p=6, rule=16)),
+    ("cart_n2_p5", dict(n=2, p=5, rule=14)),
     ("cart_n3_p3_rule10", dict(n=3, p=3, rule=10)),
 ]
 

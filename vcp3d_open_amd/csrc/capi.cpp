@@ -141,6 +141,7 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
   auto put = [&](const std::vector<double> &v) { const int off = (int) tb.size(); tb.insert(tb.end(), v.begin(), v.end()); return off; };
   dc.to.A = put(t.A); dc.to.D = put(t.D); dc.to.AT = put(t.AT); dc.to.DT = put(t.DT); dc.to.LD = put(t.LD);
   dc.to.fmin = put(t.f_min); dc.to.fmax = put(t.f_max); dc.to.dfmin = put(t.df_min); dc.to.dfmax = put(t.df_max);
+  for (int i = 0; i < M * K; ++i) { dc.opA[i] = t.A[i]; dc.opD[i] = t.D[i]; }
   dc.to.w3 = put(t.w3); dc.to.w2 = put(t.w2); dc.to.bc = put(t.basis_center); dc.to.total = (int) tb.size();
 
   // device layouts of the geometry

@@ -443,6 +443,15 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs a)
   }
 }
 
+// 1D operators of the standard element as kernel arguments: indexed with
+// compile-time constants after unrolling, they are read into scalar registers.
+template <int K, int M>
+struct Ops {
+  double A[M * K];   // phi_q(r_p), row-major (p, q)
+  double D[M * K];   // phi'_q(r_p)
+};
+
+template <int K, int M>
 struct VolArgs {
   double *U;
   const double *U0;
@@ -454,54 +463,110 @@ struct VolArgs {
   int nElem, bug;
   double mu;
   StageCtl ctl;
+  Ops<K, M> op;
 };
 
 // ========================================================== volume kernel
+// One workgroup of NT threads per element. LDS holds the modal state, the
+// sum-factorisation intermediates and the primitive state at the points;
+// the 1D operators come from the kernel arguments (scalar registers) in the
+// pencil stages and from LDS rows in the per-point stages.
 template <int K, int M, int NT>
-__global__ __launch_bounds__(NT) void volume_kernel(VolArgs a)
+__global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
 {
   constexpr int ND = K * K * K, NI = M * M * M;
-  constexpr int PPT = (NI + NT - 1) / NT;          // integration points per thread
   constexpr int DPT = (5 * ND + NT - 1) / NT;      // modal entries per thread
-  constexpr int FWD = 2 * 5 * M * K * K + 3 * 5 * M * M * K;
-  constexpr int BWD = 15 * NI + 10 * M * K * K;
-  constexpr int CGS = 5 * M * K * K + 5 * M * M * K + 5 * NI + 5 * M * K * K;
-  constexpr int BIG = FWD > BWD ? (FWD > CGS ? FWD : CGS) : (BWD > CGS ? BWD : CGS);
-  __shared__ double sA[M * K], sD[M * K], sAT[K * M], sDT[K * M], sBC[ND];
+  constexpr int SZ = 5 * M * K * K;                // z-stage buffer
+  constexpr int SY = 5 * M * M * K;                // y-stage buffer
+  constexpr int SF = 15 * NI;                      // weighted parametric fluxes
+  constexpr int SB = SF + 10 * M * K * K;          // fluxes + backward y-stage
+  constexpr int BIG = (2 * SZ + 3 * SY) > SB ? (2 * SZ + 3 * SY) : SB;
+  __shared__ double sA[M * K], sD[M * K];
   __shared__ double sU[5 * ND];
   __shared__ double sBig[BIG];
+  __shared__ double sPrim[6 * NI];                 // rho, u, v, w, p, w*J at the points
   __shared__ double sCm[5][9];
   __shared__ double sRed[(NT / 64) * 3];
+  __shared__ double sDv[15];
   const int e = blockIdx.x;
   const int tid = threadIdx.x;
+  const double *A = a.op.A, *D = a.op.D;
   const double *Ue = a.U + (size_t) e * 5 * ND;
   const double *vg = a.vgeo + (size_t) e * kVolGeo * NI;
-  for (int i = tid; i < M * K; i += NT) {
-    sA[i] = a.tables[a.to.A + i]; sD[i] = a.tables[a.to.D + i];
-    sAT[i] = a.tables[a.to.AT + i]; sDT[i] = a.tables[a.to.DT + i];
-  }
-  for (int i = tid; i < ND; i += NT) sBC[i] = a.tables[a.to.bc + i];
+  for (int i = tid; i < M * K; i += NT) { sA[i] = A[i]; sD[i] = D[i]; }
   for (int i = tid; i < 5 * ND; i += NT) sU[i] = Ue[i];
   if (a.bug)
     for (int i = tid; i < 45; i += NT) sCm[i / 9][i % 9] = vg[(1 + i % 9) * NI + i / 9];
   __syncthreads();
 
+  // element-centre entropy variables -> preconditioner dV/dU / Jbar
+  if (tid < 64) {
+    double Vc[5] = {0, 0, 0, 0, 0};
+    for (int i = tid; i < ND; i += 64) {
+      const double b = a.tables[a.to.bc + i];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) Vc[v] += b * sU[v * ND + i];
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) Vc[v] = wave_sum(Vc[v]);
+    if (tid == 0) {
+      double rho, u, v, w, p, eKin;
+      ent2prim(Vc, rho, u, v, w, p, eKin);
+      const double JacInv = 8.0 / a.econst[(size_t) e * kElemConst + 3];
+      const double pInv = 1.0 / p, a1 = kGm1 * rho * pInv * pInv, a2 = eKin * a1;
+      sDv[0] = JacInv * (kGam / kGm1 / rho + a2 * eKin); sDv[1] = -JacInv * u * a2; sDv[2] = -JacInv * v * a2;
+      sDv[3] = -JacInv * w * a2; sDv[4] = JacInv * (a2 - pInv); sDv[5] = JacInv * (a1 * u * u + pInv);
+      sDv[6] = JacInv * a1 * u * v; sDv[7] = JacInv * a1 * u * w; sDv[8] = -JacInv * a1 * u;
+      sDv[9] = JacInv * (a1 * v * v + pInv); sDv[10] = JacInv * a1 * v * w; sDv[11] = -JacInv * a1 * v;
+      sDv[12] = JacInv * (a1 * w * w + pInv); sDv[13] = -JacInv * a1 * w; sDv[14] = JacInv * a1;
+    }
+  }
+
   // ---------------- step 1: solution and parametric gradients at the points
-  double *zA = sBig, *zD = zA + 5 * M * K * K;
-  double *yAA = zD + 5 * M * K * K, *yDA = yAA + 5 * M * M * K, *yAD = yDA + 5 * M * M * K;
-  contract<K, M, 5, K * K, K * K, K * K, NT>(sU, zA, sA);
-  contract<K, M, 5, K * K, K * K, K * K, NT>(sU, zD, sD);
+  double *zA = sBig, *zD = zA + SZ, *yAA = zD + SZ, *yDA = yAA + SY, *yAD = yDA + SY;
+  for (int pc = tid; pc < 5 * K * K; pc += NT) {      // pencil (v, j, i) along k
+    const int v = pc / (K * K), ji = pc - v * K * K;
+    double x[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) x[q] = sU[(v * K + q) * K * K + ji];
+#pragma unroll
+    for (int n = 0; n < M; ++n) {
+      double sa = 0.0, sd = 0.0;
+#pragma unroll
+      for (int q = 0; q < K; ++q) { sa += A[n * K + q] * x[q]; sd += D[n * K + q] * x[q]; }
+      zA[(v * M + n) * K * K + ji] = sa;
+      zD[(v * M + n) * K * K + ji] = sd;
+    }
+  }
   __syncthreads();
-  contract<K, M, 5 * M, K, K, K, NT>(zA, yAA, sA);
-  contract<K, M, 5 * M, K, K, K, NT>(zA, yDA, sD);
-  contract<K, M, 5 * M, K, K, K, NT>(zD, yAD, sA);
+  for (int pc = tid; pc < 5 * M * K; pc += NT) {      // pencil (v, k', i) along j
+    const int vk = pc / K, i = pc - vk * K;
+    double xa[K], xd[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) { xa[q] = zA[(vk * K + q) * K + i]; xd[q] = zD[(vk * K + q) * K + i]; }
+#pragma unroll
+    for (int n = 0; n < M; ++n) {
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+      for (int q = 0; q < K; ++q) { s0 += A[n * K + q] * xa[q]; s1 += D[n * K + q] * xa[q]; s2 += A[n * K + q] * xd[q]; }
+      yAA[(vk * M + n) * K + i] = s0;
+      yDA[(vk * M + n) * K + i] = s1;
+      yAD[(vk * M + n) * K + i] = s2;
+    }
+  }
   __syncthreads();
+  // per-point x stage; results held until the flux buffer may overwrite the
+  // y-stage buffers
+  constexpr int PPT = (NI + NT - 1) / NT;
   double Vp[PPT][5], Gr[PPT][5], Gs[PPT][5], Gt[PPT][5];
 #pragma unroll
   for (int q = 0; q < PPT; ++q) {
     const int m = tid + q * NT;
     if (m < NI) {
-      const int i = m % M, jk = m / M;  // jk = k*M + j
+      const int i = m % M, jk = m / M;
+      double ai[K], di[K];
+#pragma unroll
+      for (int ii = 0; ii < K; ++ii) { ai[ii] = sA[i * K + ii]; di[ii] = sD[i * K + ii]; }
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
         const double *ra = &yAA[(v * M * M + jk) * K], *rd = &yDA[(v * M * M + jk) * K],
@@ -509,25 +574,16 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs a)
         double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
 #pragma unroll
         for (int ii = 0; ii < K; ++ii) {
-          s0 += sA[i * K + ii] * ra[ii]; s1 += sD[i * K + ii] * ra[ii];
-          s2 += sA[i * K + ii] * rd[ii]; s3 += sA[i * K + ii] * rz[ii];
+          const double x = ra[ii];
+          s0 += ai[ii] * x; s1 += di[ii] * x; s2 += ai[ii] * rd[ii]; s3 += ai[ii] * rz[ii];
         }
         Vp[q][v] = s0; Gr[q][v] = s1; Gs[q][v] = s2; Gt[q][v] = s3;
       }
     }
   }
-  // element-centre entropy variables for the CG preconditioner
-  double Vc[5];
-#pragma unroll
-  for (int v = 0; v < 5; ++v) {
-    double s = 0.0;
-    for (int i = 0; i < ND; ++i) s += sBC[i] * sU[v * ND + i];
-    Vc[v] = s;
-  }
   __syncthreads();
 
-  // ---------------- steps 3, 5, 6: pointwise fluxes, dU/dV, monitoring
-  double dUdV[PPT][15], wJ[PPT];
+  // ---------------- steps 3, 5, 6: pointwise fluxes, primitive state, monitoring
   double *F0 = sBig, *F1 = F0 + 5 * NI, *F2 = F1 + 5 * NI;
   const double factHF = a.mu * kGam / kPrLam;
   double mach2 = 0.0;
@@ -536,8 +592,7 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs a)
     const int m = tid + q * NT;
     if (m >= NI) continue;
     const double J = vg[m];
-    const double w3 = a.tables[a.to.w3 + m];
-    wJ[q] = w3 * J;
+    const double wJ = a.tables[a.to.w3 + m] * J;
     double gx[5], gy[5], gz[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
@@ -555,6 +610,8 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs a)
     }
     double rho, u, v, w, p, eKin;
     ent2prim(Vp[q], rho, u, v, w, p, eKin);
+    sPrim[0 * NI + m] = rho; sPrim[1 * NI + m] = u; sPrim[2 * NI + m] = v;
+    sPrim[3 * NI + m] = w; sPrim[4 * NI + m] = p; sPrim[5 * NI + m] = wJ;
     const double V4Inv = 1.0 / Vp[q][4];
     const double rE = p / kGm1 + rho * eKin;
     const double dudx = -V4Inv * (gx[1] + u * gx[4]), dudy = -V4Inv * (gy[1] + u * gy[4]), dudz = -V4Inv * (gz[1] + u * gz[4]);
@@ -562,13 +619,8 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs a)
     const double dwdx = -V4Inv * (gx[3] + w * gx[4]), dwdy = -V4Inv * (gy[3] + w * gy[4]), dwdz = -V4Inv * (gz[3] + w * gz[4]);
     const double fe = V4Inv * V4Inv / kGm1;
     const double dedx = fe * gx[4], dedy = fe * gy[4], dedz = fe * gz[4];
-    const double rH = rE + p, ru = rho * u, rv = rho * v, rw = rho * w;
-    dUdV[q][0] = rho; dUdV[q][1] = ru; dUdV[q][2] = rv; dUdV[q][3] = rw; dUdV[q][4] = rE;
-    dUdV[q][5] = ru * u + p; dUdV[q][6] = ru * v; dUdV[q][7] = ru * w; dUdV[q][8] = rH * u;
-    dUdV[q][9] = rv * v + p; dUdV[q][10] = rv * w; dUdV[q][11] = rH * v;
-    dUdV[q][12] = rw * w + p; dUdV[q][13] = rH * w; dUdV[q][14] = rE * rH / rho + p * eKin;
     if (a.ctl.monitor) mach2 = fmax(mach2, rho * (u * u + v * v + w * w) / (kGam * p));
-    const double weight = -wJ[q];
+    const double ru = rho * u, rv = rho * v, rw = rho * w;
     const double mu = a.mu;
     const double divV = kLamOverMu * (dudx + dvdy + dwdz);
     const double txx = mu * (2.0 * dudx + divV), tyy = mu * (2.0 * dvdy + divV), tzz = mu * (2.0 * dwdz + divV);
@@ -577,6 +629,7 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs a)
     const double fx[5] = {ru, ru * u - txx + p, rv * u - txy, rw * u - txz, (rE + p) * u - (u * txx + v * txy + w * txz - qx)};
     const double fy[5] = {rv, ru * v - txy, rv * v - tyy + p, rw * v - tyz, (rE + p) * v - (u * txy + v * tyy + w * tyz - qy)};
     const double fz[5] = {rw, ru * w - txz, rv * w - tyz, rw * w - tzz + p, (rE + p) * w - (u * txz + v * tyz + w * tzz - qz)};
+    const double weight = -wJ;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const double mx = weight * vg[(1 + 3 * d) * NI + m], my = weight * vg[(2 + 3 * d) * NI + m],
@@ -592,16 +645,57 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs a)
   }
   __syncthreads();
 
-  // ---------------- step 7: volume residual (three transposed contractions)
-  rows_inplace<M, K, 5 * M * M, NT>(F0, sDT);
-  rows_inplace<M, K, 5 * M * M, NT>(F1, sAT);
-  rows_inplace<M, K, 5 * M * M, NT>(F2, sAT);
+  // ---------------- step 7: transposed contractions of the three fluxes
+  for (int r = tid; r < 15 * M * M; r += NT) {        // rows along i, in place
+    const int d = r / (5 * M * M);
+    double x[M];
+#pragma unroll
+    for (int q = 0; q < M; ++q) x[q] = sBig[r * M + q];
+#pragma unroll
+    for (int n = 0; n < K; ++n) {
+      double s = 0.0;
+      if (d == 0) {
+#pragma unroll
+        for (int q = 0; q < M; ++q) s += D[q * K + n] * x[q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < M; ++q) s += A[q * K + n] * x[q];
+      }
+      sBig[r * M + n] = s;
+    }
+  }
   __syncthreads();
-  double *Y01 = F2 + 5 * NI, *Y2 = Y01 + 5 * M * K * K;
-  contract2<M, K, 5 * M, K, M, K, NT>(F0, sAT, F1, sDT, Y01);
-  contract<M, K, 5 * M, K, M, K, NT>(F2, Y2, sAT);
+  double *Y01 = sBig + SF, *Y2 = Y01 + 5 * M * K * K;   // after the flux block
+  for (int pc = tid; pc < 5 * M * K; pc += NT) {      // pencil (v, k, i') along j
+    const int vk = pc / K, i = pc - vk * K;
+    double x0[M], x1[M], x2[M];
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+      x0[q] = F0[(vk * M + q) * M + i]; x1[q] = F1[(vk * M + q) * M + i]; x2[q] = F2[(vk * M + q) * M + i];
+    }
+#pragma unroll
+    for (int n = 0; n < K; ++n) {
+      double s01 = 0.0, s2 = 0.0;
+#pragma unroll
+      for (int q = 0; q < M; ++q) { s01 += A[q * K + n] * x0[q] + D[q * K + n] * x1[q]; s2 += A[q * K + n] * x2[q]; }
+      Y01[(vk * K + n) * K + i] = s01;
+      Y2[(vk * K + n) * K + i] = s2;
+    }
+  }
   __syncthreads();
-  contract2<M, K, 5, K * K, K * K, K * K, NT>(Y01, sAT, Y2, sDT, sU);
+  for (int pc = tid; pc < 5 * K * K; pc += NT) {      // pencil (v, j', i') along k
+    const int v = pc / (K * K), ji = pc - v * K * K;
+    double y0[M], y2[M];
+#pragma unroll
+    for (int q = 0; q < M; ++q) { y0[q] = Y01[(v * M + q) * K * K + ji]; y2[q] = Y2[(v * M + q) * K * K + ji]; }
+#pragma unroll
+    for (int n = 0; n < K; ++n) {
+      double s = 0.0;
+#pragma unroll
+      for (int q = 0; q < M; ++q) s += A[q * K + n] * y0[q] + D[q * K + n] * y2[q];
+      sU[(v * K + n) * K * K + ji] = s;
+    }
+  }
   __syncthreads();
 
   // ---------------- gather the six face contributions
@@ -621,110 +715,159 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs a)
 
   // ---------------- entropy-variable mass matrix: flexible PCG
   const double JacInv = 8.0 / a.econst[(size_t) e * kElemConst + 3];
-  double x[DPT], z[DPT];
+  double x[DPT], z[DPT], pv[DPT];
   double r0 = 0.0;
 #pragma unroll
   for (int q = 0; q < DPT; ++q) { x[q] = 0.0; r0 = fmax(r0, fabs(r[q])); }
-  {
-    double red[1] = {r0};
-    block_reduce<NT, 1, 0>(red, sRed);
-    r0 = red[0];
-  }
+  { double red[1] = {r0}; block_reduce<NT, 1, 0>(red, sRed); r0 = red[0]; }
   int iters = 0;
   if (r0 * JacInv > 1e-10) {
-    double rho, u, v, w, p, eKin;
-    ent2prim(Vc, rho, u, v, w, p, eKin);
-    const double pInv = 1.0 / p, a1 = kGm1 * rho * pInv * pInv, a2 = eKin * a1;
-    double dv[15];
-    dv[0] = JacInv * (kGam / kGm1 / rho + a2 * eKin); dv[1] = -JacInv * u * a2; dv[2] = -JacInv * v * a2;
-    dv[3] = -JacInv * w * a2; dv[4] = JacInv * (a2 - pInv); dv[5] = JacInv * (a1 * u * u + pInv);
-    dv[6] = JacInv * a1 * u * v; dv[7] = JacInv * a1 * u * w; dv[8] = -JacInv * a1 * u;
-    dv[9] = JacInv * (a1 * v * v + pInv); dv[10] = JacInv * a1 * v * w; dv[11] = -JacInv * a1 * v;
-    dv[12] = JacInv * (a1 * w * w + pInv); dv[13] = -JacInv * a1 * w; dv[14] = JacInv * a1;
-    // preconditioner z = dV/dU(centre) r: needs the five variables of one DOF,
-    // exchanged through LDS
-    // r (preconditioner exchange) and p (forward contraction input) share sU;
-    // every reuse is separated by a barrier
-    double *sR = sU, *sP = sU;
+    // z = dV/dU(centre) r; the five variables of a DOF are exchanged through LDS
     auto precond = [&]() {
       __syncthreads();
 #pragma unroll
-      for (int q = 0; q < DPT; ++q) { const int i = tid + q * NT; if (i < 5 * ND) sR[i] = r[q]; }
+      for (int q = 0; q < DPT; ++q) { const int i = tid + q * NT; if (i < 5 * ND) sU[i] = r[q]; }
       __syncthreads();
 #pragma unroll
       for (int q = 0; q < DPT; ++q) {
         const int i = tid + q * NT;
         if (i < 5 * ND) {
           const int l = i / ND, d = i - l * ND;
-          const double r0_ = sR[d], r1 = sR[ND + d], r2 = sR[2 * ND + d], r3 = sR[3 * ND + d], r4 = sR[4 * ND + d];
-          double s;
-          if (l == 0) s = dv[0] * r0_ + dv[1] * r1 + dv[2] * r2 + dv[3] * r3 + dv[4] * r4;
-          else if (l == 1) s = dv[1] * r0_ + dv[5] * r1 + dv[6] * r2 + dv[7] * r3 + dv[8] * r4;
-          else if (l == 2) s = dv[2] * r0_ + dv[6] * r1 + dv[9] * r2 + dv[10] * r3 + dv[11] * r4;
-          else if (l == 3) s = dv[3] * r0_ + dv[7] * r1 + dv[10] * r2 + dv[12] * r3 + dv[13] * r4;
-          else s = dv[4] * r0_ + dv[8] * r1 + dv[11] * r2 + dv[13] * r3 + dv[14] * r4;
-          z[q] = s;
+          const double r0_ = sU[d], r1 = sU[ND + d], r2 = sU[2 * ND + d], r3 = sU[3 * ND + d], r4 = sU[4 * ND + d];
+          const int c0 = l == 0 ? 0 : l == 1 ? 1 : l == 2 ? 2 : l == 3 ? 3 : 4;
+          const int c1 = l == 0 ? 1 : l == 1 ? 5 : l == 2 ? 6 : l == 3 ? 7 : 8;
+          const int c2 = l == 0 ? 2 : l == 1 ? 6 : l == 2 ? 9 : l == 3 ? 10 : 11;
+          const int c3 = l == 0 ? 3 : l == 1 ? 7 : l == 2 ? 10 : l == 3 ? 12 : 13;
+          const int c4 = l == 0 ? 4 : l == 1 ? 8 : l == 2 ? 11 : l == 3 ? 13 : 14;
+          z[q] = sDv[c0] * r0_ + sDv[c1] * r1 + sDv[c2] * r2 + sDv[c3] * r3 + sDv[c4] * r4;
+        } else {
+          z[q] = 0.0;
         }
       }
     };
     precond();
-    double pv[DPT];
     double rz = 0.0;
 #pragma unroll
     for (int q = 0; q < DPT; ++q) { pv[q] = z[q]; rz += r[q] * z[q]; }
     { double red[1] = {rz}; block_reduce<NT, 1, 1>(red, sRed); rz = red[0]; }
-    double *cz = sBig, *cy = cz + 5 * M * K * K, *pI = cy + 5 * M * M * K, *by = pI + 5 * NI;
+    double *cz = sBig, *cy = cz + SZ, *pI = cy + SY, *by = pI + 5 * NI;
     for (int it = 0;; ++it) {
       if (it == 100) { atomicOr(a.flags, kFlagCGNotConverged); break; }
-      // Ap = M p: forward to the points, dU/dV * w J, transposed back
       __syncthreads();
 #pragma unroll
-      for (int q = 0; q < DPT; ++q) { const int i = tid + q * NT; if (i < 5 * ND) sP[i] = pv[q]; }
+      for (int q = 0; q < DPT; ++q) { const int i = tid + q * NT; if (i < 5 * ND) sU[i] = pv[q]; }
       __syncthreads();
-      contract<K, M, 5, K * K, K * K, K * K, NT>(sP, cz, sA);
-      __syncthreads();
-      contract<K, M, 5 * M, K, K, K, NT>(cz, cy, sA);
-      __syncthreads();
+      for (int pc = tid; pc < 5 * K * K; pc += NT) {
+        const int v = pc / (K * K), ji = pc - v * K * K;
+        double xx[K];
 #pragma unroll
-      for (int q = 0; q < PPT; ++q) {
-        const int m = tid + q * NT;
-        if (m < NI) {
-          const int i = m % M, jk = m / M;
-          double pp[5];
+        for (int q = 0; q < K; ++q) xx[q] = sU[(v * K + q) * K * K + ji];
 #pragma unroll
-          for (int l = 0; l < 5; ++l) {
-            const double *ra = &cy[(l * M * M + jk) * K];
-            double s = 0.0;
+        for (int n = 0; n < M; ++n) {
+          double s = 0.0;
 #pragma unroll
-            for (int ii = 0; ii < K; ++ii) s += sA[i * K + ii] * ra[ii];
-            pp[l] = wJ[q] * s;
-          }
-          const double *D = dUdV[q];
-          pI[0 * NI + m] = D[0] * pp[0] + D[1] * pp[1] + D[2] * pp[2] + D[3] * pp[3] + D[4] * pp[4];
-          pI[1 * NI + m] = D[1] * pp[0] + D[5] * pp[1] + D[6] * pp[2] + D[7] * pp[3] + D[8] * pp[4];
-          pI[2 * NI + m] = D[2] * pp[0] + D[6] * pp[1] + D[9] * pp[2] + D[10] * pp[3] + D[11] * pp[4];
-          pI[3 * NI + m] = D[3] * pp[0] + D[7] * pp[1] + D[10] * pp[2] + D[12] * pp[3] + D[13] * pp[4];
-          pI[4 * NI + m] = D[4] * pp[0] + D[8] * pp[1] + D[11] * pp[2] + D[13] * pp[3] + D[14] * pp[4];
+          for (int q = 0; q < K; ++q) s += A[n * K + q] * xx[q];
+          cz[(v * M + n) * K * K + ji] = s;
         }
       }
       __syncthreads();
-      rows_inplace<M, K, 5 * M * M, NT>(pI, sAT);
-      __syncthreads();
-      contract<M, K, 5 * M, K, M, K, NT>(pI, by, sAT);
-      __syncthreads();
-      contract<M, K, 5, K * K, K * K, K * K, NT>(by, cz, sAT);   // Ap in cz
-      __syncthreads();
-      double pAp = 0.0;
+      for (int pc = tid; pc < 5 * M * K; pc += NT) {
+        const int vk = pc / K, i = pc - vk * K;
+        double xx[K];
 #pragma unroll
-      for (int q = 0; q < DPT; ++q) { const int i = tid + q * NT; if (i < 5 * ND) pAp += pv[q] * cz[i]; }
-      { double red[1] = {pAp}; block_reduce<NT, 1, 1>(red, sRed); pAp = red[0]; }
-      const double alpha = rz / pAp;
+        for (int q = 0; q < K; ++q) xx[q] = cz[(vk * K + q) * K + i];
+#pragma unroll
+        for (int n = 0; n < M; ++n) {
+          double s = 0.0;
+#pragma unroll
+          for (int q = 0; q < K; ++q) s += A[n * K + q] * xx[q];
+          cy[(vk * M + n) * K + i] = s;
+        }
+      }
+      __syncthreads();
+      for (int m = tid; m < NI; m += NT) {
+        const int i = m % M, jk = m / M;
+        double ai[K];
+#pragma unroll
+        for (int ii = 0; ii < K; ++ii) ai[ii] = sA[i * K + ii];
+        double pp[5];
+        const double wJ = sPrim[5 * NI + m];
+#pragma unroll
+        for (int l = 0; l < 5; ++l) {
+          const double *ra = &cy[(l * M * M + jk) * K];
+          double s = 0.0;
+#pragma unroll
+          for (int ii = 0; ii < K; ++ii) s += ai[ii] * ra[ii];
+          pp[l] = wJ * s;
+        }
+        // dU/dV at the point from the primitive state (ElementClass.cpp:4228-4253)
+        const double rho = sPrim[m], u = sPrim[NI + m], v = sPrim[2 * NI + m], w = sPrim[3 * NI + m],
+                     p = sPrim[4 * NI + m];
+        const double eKin = 0.5 * (u * u + v * v + w * w);
+        const double rE = p / kGm1 + rho * eKin, rH = rE + p;
+        const double ru = rho * u, rv = rho * v, rw = rho * w;
+        const double d44 = rE * rH / rho + p * eKin;
+        pI[0 * NI + m] = rho * pp[0] + ru * pp[1] + rv * pp[2] + rw * pp[3] + rE * pp[4];
+        pI[1 * NI + m] = ru * pp[0] + (ru * u + p) * pp[1] + ru * v * pp[2] + ru * w * pp[3] + rH * u * pp[4];
+        pI[2 * NI + m] = rv * pp[0] + ru * v * pp[1] + (rv * v + p) * pp[2] + rv * w * pp[3] + rH * v * pp[4];
+        pI[3 * NI + m] = rw * pp[0] + ru * w * pp[1] + rv * w * pp[2] + (rw * w + p) * pp[3] + rH * w * pp[4];
+        pI[4 * NI + m] = rE * pp[0] + rH * u * pp[1] + rH * v * pp[2] + rH * w * pp[3] + d44 * pp[4];
+      }
+      __syncthreads();
+      for (int rr = tid; rr < 5 * M * M; rr += NT) {
+        double xx[M];
+#pragma unroll
+        for (int q = 0; q < M; ++q) xx[q] = pI[rr * M + q];
+#pragma unroll
+        for (int n = 0; n < K; ++n) {
+          double s = 0.0;
+#pragma unroll
+          for (int q = 0; q < M; ++q) s += A[q * K + n] * xx[q];
+          pI[rr * M + n] = s;
+        }
+      }
+      __syncthreads();
+      for (int pc = tid; pc < 5 * M * K; pc += NT) {
+        const int vk = pc / K, i = pc - vk * K;
+        double xx[M];
+#pragma unroll
+        for (int q = 0; q < M; ++q) xx[q] = pI[(vk * M + q) * M + i];
+#pragma unroll
+        for (int n = 0; n < K; ++n) {
+          double s = 0.0;
+#pragma unroll
+          for (int q = 0; q < M; ++q) s += A[q * K + n] * xx[q];
+          by[(vk * K + n) * K + i] = s;
+        }
+      }
+      __syncthreads();
+      for (int pc = tid; pc < 5 * K * K; pc += NT) {
+        const int v = pc / (K * K), ji = pc - v * K * K;
+        double xx[M];
+#pragma unroll
+        for (int q = 0; q < M; ++q) xx[q] = by[(v * M + q) * K * K + ji];
+#pragma unroll
+        for (int n = 0; n < K; ++n) {
+          double s = 0.0;
+#pragma unroll
+          for (int q = 0; q < M; ++q) s += A[q * K + n] * xx[q];
+          cz[(v * K + n) * K * K + ji] = s;                 // Ap
+        }
+      }
+      __syncthreads();
       double ap[DPT];
-      double rmax = 0.0;
+      double pAp = 0.0;
 #pragma unroll
       for (int q = 0; q < DPT; ++q) {
         const int i = tid + q * NT;
         ap[q] = (i < 5 * ND) ? cz[i] : 0.0;
+        pAp += pv[q] * ap[q];
+      }
+      { double red[1] = {pAp}; block_reduce<NT, 1, 1>(red, sRed); pAp = red[0]; }
+      const double alpha = rz / pAp;
+      double rmax = 0.0;
+#pragma unroll
+      for (int q = 0; q < DPT; ++q) {
         x[q] += alpha * pv[q];
         r[q] -= alpha * ap[q];
         rmax = fmax(rmax, fabs(r[q]));
@@ -845,7 +988,8 @@ hipError_t faces_impl(const DeviceCase &dc, hipStream_t s)
 template <int K, int M>
 hipError_t volume_impl(const DeviceCase &dc, const StageCtl &ctl, hipStream_t s)
 {
-  VolArgs a;
+  VolArgs<K, M> a;
+  for (int i = 0; i < M * K; ++i) { a.op.A[i] = dc.opA[i]; a.op.D[i] = dc.opD[i]; }
   a.U = dc.U; a.U0 = dc.U0; a.R = dc.R; a.faceRes = dc.faceRes; a.tables = dc.tables;
   a.vgeo = dc.vgeo; a.econst = dc.econst; a.scal = dc.scal; a.flags = dc.flags; a.cgIter = dc.cgIter;
   a.to = dc.to; a.nElem = dc.nElem; a.bug = dc.bug_compat; a.mu = dc.mu; a.ctl = ctl;
@@ -866,7 +1010,7 @@ hipError_t dt_impl(const DeviceCase &dc, hipStream_t s)
 
 }  // namespace
 
-#define VCP_PAIRS(X) X(3, 4) X(4, 5) X(4, 6) X(5, 7) X(6, 8) X(7, 9)
+#define VCP_PAIRS(X) X(3, 4) X(4, 5) X(4, 6) X(5, 7) X(6, 8)
 
 bool supported_pair(int K, int M)
 {

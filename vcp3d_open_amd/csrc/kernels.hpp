@@ -29,6 +29,7 @@ struct DeviceCase {
   double mu = 0, theta_sym = 1, rel_penalty = 1;
   double cfl = 1, u_ref = 1, p_ref = 1, rho_ref = 1;
   TableOffsets to{};
+  double opA[128] = {}, opD[128] = {};   // host copies of the M x K operators
   double *tables = nullptr;
   double *vgeo = nullptr, *fgeo = nullptr, *econst = nullptr;
   int *nbr = nullptr;

@@ -142,6 +142,10 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
   dc.to.A = put(t.A); dc.to.D = put(t.D); dc.to.AT = put(t.AT); dc.to.DT = put(t.DT); dc.to.LD = put(t.LD);
   dc.to.fmin = put(t.f_min); dc.to.fmax = put(t.f_max); dc.to.dfmin = put(t.df_min); dc.to.dfmax = put(t.df_max);
   for (int i = 0; i < M * K; ++i) { dc.opA[i] = t.A[i]; dc.opD[i] = t.D[i]; }
+  for (int q = 0; q < K; ++q) {
+    dc.opF[0][q] = t.f_min[q]; dc.opF[1][q] = t.f_max[q]; dc.opF[2][q] = t.df_min[q]; dc.opF[3][q] = t.df_max[q];
+  }
+  for (int q = 0; q < M; ++q) dc.opW[q] = t.w_int[q];
   dc.to.w3 = put(t.w3); dc.to.w2 = put(t.w2); dc.to.bc = put(t.basis_center); dc.to.total = (int) tb.size();
 
   // device layouts of the geometry

@@ -146,124 +146,142 @@ __device__ inline void atomic_min_pos(double *addr, double v)
   atomicMin(reinterpret_cast<unsigned long long *>(addr), (unsigned long long) __double_as_longlong(v));
 }
 
+template <int K, int M>
+struct FaceOps {
+  double A[M * K], D[M * K];   // phi_q(r_p), phi'_q(r_p), row-major (p, q)
+  double F[4][K];              // phi(-1), phi(+1), phi'(-1), phi'(+1)
+  double w[M];                 // 1D Gauss weights
+};
+
+template <int K, int M>
 struct FaceArgs {
-  const double *U, *tables, *fgeo, *econst;
+  const double *U, *fgeo, *econst;
   const int *nbr;
   double *faceRes;
-  TableOffsets to;
   int nElem, bug;
   double mu, tauFact, thetaHalf;
+  FaceOps<K, M> op;
 };
 
 // ============================================================ face kernel
-// One wave64 per element min face (blockIdx.x = dir * nElem + e).
+// One wave64 per element min face in direction DIR. Side 0 is
+// the element itself (its min face), side 1 the neighbour L (its max face);
+// the normal points from L to the element.
 template <int K, int M>
-__global__ __launch_bounds__(64) void face_kernel(FaceArgs a)
+struct FaceSmem {
+  double A[M * K], D[M * K], W2[M * M];
+  double B[2][2][5 * K * K];      // side, (value | normal derivative)
+  double T[2][3][5 * K * M];      // side, (A, D, N) after the fast contraction
+  double Flux[5 * M * M], Sym[2][3][5 * M * M];   // sym slots (normal, fast, slow)
+  double Cm[2][5][9];
+};
+
+template <int K, int M, int DIR>
+__global__ __launch_bounds__(64) void face_kernel(FaceArgs<K, M> a)
 {
+  const int e = blockIdx.x;
+  const int L = a.nbr[e * 6 + 2 * DIR];
+  if (L < 0) return;  // physical boundary face
+  __shared__ FaceSmem<K, M> sm;
   constexpr int ND = K * K * K, N2 = M * M;
-  __shared__ double sA[M * K], sD[M * K], sF[4][K], sW2[N2];
-  __shared__ double sU[2][5 * ND];
-  __shared__ double sB[2][2][5 * K * K];      // side, (value|normal-derivative)
-  __shared__ double sT[2][3][5 * K * M];      // side, (A, D, N)
-  __shared__ double sFlux[5 * N2], sSym[2][3][5 * N2];
-  __shared__ double sCm[2][5][9];
-  const int dir = blockIdx.x / a.nElem;
-  const int e = blockIdx.x - dir * a.nElem;
-  const int L = a.nbr[e * 6 + 2 * dir];
+  // strides of the (normal, fast, slow) axes in the [k][j][i] DOF layout
+  constexpr int SN = DIR == 0 ? 1 : DIR == 1 ? K : K * K;
+  constexpr int SFA = DIR == 0 ? K : 1;
+  constexpr int SSL = DIR == 2 ? K : K * K;
+  double *sA = sm.A, *sD = sm.D, *sW2 = sm.W2, *sFlux = sm.Flux;
+  auto &sB = sm.B;
+  auto &sT = sm.T;
+  auto &sSym = sm.Sym;
+  auto &sCm = sm.Cm;
   const int tid = threadIdx.x;
-  if (L < 0) return;  // physical boundary: handled by boundary kernels
-  for (int i = tid; i < M * K; i += 64) { sA[i] = a.tables[a.to.A + i]; sD[i] = a.tables[a.to.D + i]; }
-  for (int i = tid; i < K; i += 64) {
-    sF[0][i] = a.tables[a.to.fmin + i]; sF[1][i] = a.tables[a.to.fmax + i];
-    sF[2][i] = a.tables[a.to.dfmin + i]; sF[3][i] = a.tables[a.to.dfmax + i];
-  }
-  for (int i = tid; i < N2; i += 64) sW2[i] = a.tables[a.to.w2 + i];
-  // side 0 = right element (own min face), side 1 = left element (its max face)
-  const double *UR = a.U + (size_t) e * 5 * ND, *UL = a.U + (size_t) L * 5 * ND;
-  for (int i = tid; i < 5 * ND; i += 64) { sU[0][i] = UR[i]; sU[1][i] = UL[i]; }
-  const double *g = a.fgeo + ((size_t) e * 3 + dir) * kFaceGeo * N2;
+  const double *A = a.op.A, *D = a.op.D;
+  for (int i = tid; i < M * K; i += 64) { sA[i] = A[i]; sD[i] = D[i]; }
+  for (int i = tid; i < N2; i += 64) sW2[i] = a.op.w[i % M] * a.op.w[i / M];
+  const double *g = a.fgeo + ((size_t) e * 3 + DIR) * kFaceGeo * N2;
   if (a.bug)
     for (int i = tid; i < 2 * 5 * 9; i += 64) {
       const int side = i / 45, l = (i / 9) % 5, q = i % 9;
       const int pt = l < N2 ? l : 0;
       sCm[side][l][q] = g[(side ? 4 + q : 13 + q) * N2 + pt];
     }
-  __syncthreads();
-  // strides of (normal, fast, slow) axes in the [k][j][i] DOF layout
-  const int sn = dir == 0 ? 1 : dir == 1 ? K : K * K;
-  const int sf = dir == 0 ? K : 1;
-  const int ss = dir == 2 ? K : K * K;
-  // normal contraction: b[side][0|1][v][s][f]
-  for (int idx = tid; idx < 2 * 5 * K * K; idx += 64) {
-    const int side = idx / (5 * K * K);
-    const int r = idx - side * 5 * K * K;
-    const int v = r / (K * K), s = (r / K) % K, f = r % K;
-    const double *fv = side ? sF[1] : sF[0];
-    const double *dfv = side ? sF[3] : sF[2];
-    const double *u = &sU[side][v * ND + s * ss + f * sf];
-    double b0 = 0.0, b1 = 0.0;
+  // normal contraction straight from HBM: b[side][0|1][v][s][f]
 #pragma unroll
-    for (int t = 0; t < K; ++t) { b0 += fv[t] * u[t * sn]; b1 += dfv[t] * u[t * sn]; }
-    sB[side][0][r] = b0; sB[side][1][r] = b1;
+  for (int side = 0; side < 2; ++side) {
+    const double *Us = a.U + (size_t) (side ? L : e) * 5 * ND;
+    const double *Fv = a.op.F[side], *dFv = a.op.F[2 + side];
+    for (int pc = tid; pc < 5 * K * K; pc += 64) {
+      const int v = pc / (K * K), s = (pc / K) % K, f = pc % K;
+      const double *u = Us + v * ND + s * SSL + f * SFA;
+      double b0 = 0.0, b1 = 0.0;
+#pragma unroll
+      for (int t = 0; t < K; ++t) { const double x = u[t * SN]; b0 += Fv[t] * x; b1 += dFv[t] * x; }
+      sB[side][0][pc] = b0; sB[side][1][pc] = b1;
+    }
   }
   __syncthreads();
   // fast tangential contraction: T[side][A|D|N][v][s][f'], f' < M
-  for (int idx = tid; idx < 2 * 5 * K * M; idx += 64) {
-    const int side = idx / (5 * K * M);
-    const int r = idx - side * 5 * K * M;
-    const int vs = r / M, fp = r % M;
-    const double *b0 = &sB[side][0][vs * K], *b1 = &sB[side][1][vs * K];
-    double tA = 0.0, tD = 0.0, tN = 0.0;
+  for (int pc = tid; pc < 2 * 5 * K; pc += 64) {
+    const int side = pc / (5 * K), vs = pc - side * 5 * K;
+    double x0[K], x1[K];
 #pragma unroll
-    for (int f = 0; f < K; ++f) {
-      tA += sA[fp * K + f] * b0[f]; tD += sD[fp * K + f] * b0[f]; tN += sA[fp * K + f] * b1[f];
+    for (int f = 0; f < K; ++f) { x0[f] = sB[side][0][vs * K + f]; x1[f] = sB[side][1][vs * K + f]; }
+#pragma unroll
+    for (int n = 0; n < M; ++n) {
+      double tA = 0.0, tD = 0.0, tN = 0.0;
+#pragma unroll
+      for (int f = 0; f < K; ++f) { tA += A[n * K + f] * x0[f]; tD += D[n * K + f] * x0[f]; tN += A[n * K + f] * x1[f]; }
+      sT[side][0][vs * M + n] = tA; sT[side][1][vs * M + n] = tD; sT[side][2][vs * M + n] = tN;
     }
-    sT[side][0][r] = tA; sT[side][1][r] = tD; sT[side][2][r] = tN;
   }
   __syncthreads();
-  const double lenScale = fmin(a.econst[(size_t) e * kElemConst + dir], a.econst[(size_t) L * kElemConst + dir]);
+  const double lenScale = fmin(a.econst[(size_t) e * kElemConst + DIR], a.econst[(size_t) L * kElemConst + DIR]);
   const double tau = a.tauFact / lenScale;
   const double mu = a.mu, factHF = kGam / kPrLam;
   const double lam = kLamOverMu, lamp1 = lam + 1.0, lamp2 = lam + 2.0;
   for (int m = tid; m < N2; m += 64) {
     const int sp = m / M, fp = m - sp * M;
-    // traces and parametric gradients of both sides
-    double V[2][5], Gr[2][5], Gs[2][5], Gt[2][5];
+    double as[K], ds[K];
 #pragma unroll
-    for (int side = 0; side < 2; ++side)
+    for (int s = 0; s < K; ++s) { as[s] = sA[sp * K + s]; ds[s] = sD[sp * K + s]; }
+    double mL[9], mR[9];
+    if (!a.bug) {
 #pragma unroll
-      for (int v = 0; v < 5; ++v) {
+      for (int q = 0; q < 9; ++q) { mL[q] = g[(4 + q) * N2 + m]; mR[q] = g[(13 + q) * N2 + m]; }
+    }
+    // traces and averaged Cartesian gradients of the entropy variables
+    double V[2][5], gx[5], gy[5], gz[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      gx[v] = gy[v] = gz[v] = 0.0;
+#pragma unroll
+      for (int side = 0; side < 2; ++side) {
         const double *tA = &sT[side][0][v * K * M + fp], *tD = &sT[side][1][v * K * M + fp],
                      *tN = &sT[side][2][v * K * M + fp];
         double val = 0.0, gN = 0.0, gF = 0.0, gS = 0.0;
 #pragma unroll
         for (int s = 0; s < K; ++s) {
-          const double as = sA[sp * K + s], ds = sD[sp * K + s];
-          val += as * tA[s * M]; gN += as * tN[s * M]; gF += as * tD[s * M]; gS += ds * tA[s * M];
+          const double xa = tA[s * M];
+          val += as[s] * xa; gS += ds[s] * xa; gN += as[s] * tN[s * M]; gF += as[s] * tD[s * M];
         }
         V[side][v] = val;
-        if (dir == 0) { Gr[side][v] = gN; Gs[side][v] = gF; Gt[side][v] = gS; }
-        else if (dir == 1) { Gr[side][v] = gF; Gs[side][v] = gN; Gt[side][v] = gS; }
-        else { Gr[side][v] = gF; Gs[side][v] = gS; Gt[side][v] = gN; }
+        const double gr = DIR == 0 ? gN : gF;
+        const double gs = DIR == 0 ? gF : DIR == 1 ? gN : gS;
+        const double gt = DIR == 2 ? gN : gS;
+        double c[9];
+        if (a.bug) {
+#pragma unroll
+          for (int q = 0; q < 9; ++q) c[q] = sCm[side][v][q];
+        } else {
+#pragma unroll
+          for (int q = 0; q < 9; ++q) c[q] = side ? mL[q] : mR[q];
+        }
+        gx[v] += 0.5 * (gr * c[0] + gs * c[3] + gt * c[6]);
+        gy[v] += 0.5 * (gr * c[1] + gs * c[4] + gt * c[7]);
+        gz[v] += 0.5 * (gr * c[2] + gs * c[5] + gt * c[8]);
       }
+    }
     const double nn[3] = {g[0 * N2 + m], g[1 * N2 + m], g[2 * N2 + m]};
     const double wA = sW2[m] * g[3 * N2 + m];
-    double mL[9], mR[9];
-#pragma unroll
-    for (int q = 0; q < 9; ++q) { mL[q] = g[(4 + q) * N2 + m]; mR[q] = g[(13 + q) * N2 + m]; }
-    // averaged Cartesian gradients of the entropy variables
-    double gx[5], gy[5], gz[5];
-#pragma unroll
-    for (int v = 0; v < 5; ++v) {
-      const double *cL = a.bug ? sCm[1][v] : mL, *cR = a.bug ? sCm[0][v] : mR;
-      const double xL = Gr[1][v] * cL[0] + Gs[1][v] * cL[3] + Gt[1][v] * cL[6];
-      const double yL = Gr[1][v] * cL[1] + Gs[1][v] * cL[4] + Gt[1][v] * cL[7];
-      const double zL = Gr[1][v] * cL[2] + Gs[1][v] * cL[5] + Gt[1][v] * cL[8];
-      const double xR = Gr[0][v] * cR[0] + Gs[0][v] * cR[3] + Gt[0][v] * cR[6];
-      const double yR = Gr[0][v] * cR[1] + Gs[0][v] * cR[4] + Gt[0][v] * cR[7];
-      const double zR = Gr[0][v] * cR[2] + Gs[0][v] * cR[5] + Gt[0][v] * cR[8];
-      gx[v] = 0.5 * (xL + xR); gy[v] = 0.5 * (yL + yR); gz[v] = 0.5 * (zL + zR);
-    }
     // ---- Ismail-Roe (entropy form, beta = 1/6, alphaMax = 0)
     double fl[5];
     {
@@ -314,17 +332,15 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs a)
       fl[4] = wA * (rA * vn * H - (lam3 * drE + H * abv6 + vn * abv7));
     }
     // ---- viscous normal flux, penalty and symmetrising fluxes on the average
-    double Va[5];
-#pragma unroll
-    for (int v = 0; v < 5; ++v) Va[v] = 0.5 * (V[0][v] + V[1][v]);
-    const double V4Inv = 1.0 / Va[4];
-    const double u = -V4Inv * Va[1], vv = -V4Inv * Va[2], w = -V4Inv * Va[3];
-    const double dudx = -V4Inv * (gx[1] + u * gx[4]), dvdx = -V4Inv * (gx[2] + vv * gx[4]), dwdx = -V4Inv * (gx[3] + w * gx[4]);
-    const double dudy = -V4Inv * (gy[1] + u * gy[4]), dvdy = -V4Inv * (gy[2] + vv * gy[4]), dwdy = -V4Inv * (gy[3] + w * gy[4]);
-    const double dudz = -V4Inv * (gz[1] + u * gz[4]), dvdz = -V4Inv * (gz[2] + vv * gz[4]), dwdz = -V4Inv * (gz[3] + w * gz[4]);
-    const double fe = V4Inv * V4Inv / kGm1;
-    const double dedx = fe * gx[4], dedy = fe * gy[4], dedz = fe * gz[4];
+    const double V4Inv = 2.0 / (V[0][4] + V[1][4]);
+    const double u = -V4Inv * 0.5 * (V[0][1] + V[1][1]), vv = -V4Inv * 0.5 * (V[0][2] + V[1][2]),
+                 w = -V4Inv * 0.5 * (V[0][3] + V[1][3]);
     {
+      const double dudx = -V4Inv * (gx[1] + u * gx[4]), dvdx = -V4Inv * (gx[2] + vv * gx[4]), dwdx = -V4Inv * (gx[3] + w * gx[4]);
+      const double dudy = -V4Inv * (gy[1] + u * gy[4]), dvdy = -V4Inv * (gy[2] + vv * gy[4]), dwdy = -V4Inv * (gy[3] + w * gy[4]);
+      const double dudz = -V4Inv * (gz[1] + u * gz[4]), dvdz = -V4Inv * (gz[2] + vv * gz[4]), dwdz = -V4Inv * (gz[3] + w * gz[4]);
+      const double fe = V4Inv * V4Inv / kGm1;
+      const double dedx = fe * gx[4], dedy = fe * gy[4], dedz = fe * gz[4];
       const double nx = wA * nn[0], ny = wA * nn[1], nz = wA * nn[2];
       const double lamDiv = kLamOverMu * (dudx + dvdy + dwdz);
       const double txx = mu * (2.0 * dudx + lamDiv), tyy = mu * (2.0 * dvdy + lamDiv), tzz = mu * (2.0 * dwdz + lamDiv);
@@ -350,7 +366,6 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs a)
     fl[4] += ttau * (lamp1 * un * dVn + u * dV1 + vv * dV2 + w * dV3 + (abvE + lamp1 * un * un) * dV4);
     const double sx = a.thetaHalf * factFlux * nn[0], sy = a.thetaHalf * factFlux * nn[1],
                  sz = a.thetaHalf * factFlux * nn[2];
-    // -(K_x1 sx + K_x2 sy + K_x3 sz) dV, K symmetric viscous Jacobians
     double FX[5], FY[5], FZ[5];
     const double K11_14 = u * lamp2, K11_44 = abvE + u * u * lamp1;
     const double K22_24 = vv * lamp2, K22_44 = abvE + vv * vv * lamp1;
@@ -376,70 +391,92 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs a)
     FZ[3] = -(lam * dV1 + K13_43 * dV4) * sx - (lam * dV2 + K23_43 * dV4) * sy - (lamp2 * dV3 + K33_34 * dV4) * sz;
     FZ[4] = -(K13_14 * dV1 + u * dV3 + K13_44 * dV4) * sx
             - (K23_24 * dV2 + vv * dV3 + K23_44 * dV4) * sy - (u * dV1 + vv * dV2 + K33_34 * dV3 + K33_44 * dV4) * sz;
+    // parametric slot of each axis: r = 0, s = 1, t = 2 -> (normal, fast, slow)
+    constexpr int PN = DIR, PF = DIR == 0 ? 1 : 0, PS = DIR == 2 ? 1 : 2;
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
       sFlux[v * N2 + m] = fl[v];
-      const double *cL = a.bug ? sCm[1][v] : mL, *cR = a.bug ? sCm[0][v] : mR;
-      // parametric symmetrising fluxes: side 0 right (own), side 1 left
-      sSym[1][0][v * N2 + m] = FX[v] * cL[0] + FY[v] * cL[1] + FZ[v] * cL[2];
-      sSym[1][1][v * N2 + m] = FX[v] * cL[3] + FY[v] * cL[4] + FZ[v] * cL[5];
-      sSym[1][2][v * N2 + m] = FX[v] * cL[6] + FY[v] * cL[7] + FZ[v] * cL[8];
-      sSym[0][0][v * N2 + m] = FX[v] * cR[0] + FY[v] * cR[1] + FZ[v] * cR[2];
-      sSym[0][1][v * N2 + m] = FX[v] * cR[3] + FY[v] * cR[4] + FZ[v] * cR[5];
-      sSym[0][2][v * N2 + m] = FX[v] * cR[6] + FY[v] * cR[7] + FZ[v] * cR[8];
+#pragma unroll
+      for (int side = 0; side < 2; ++side) {
+        double c[9];
+        if (a.bug) {
+#pragma unroll
+          for (int q = 0; q < 9; ++q) c[q] = sCm[side][v][q];
+        } else {
+#pragma unroll
+          for (int q = 0; q < 9; ++q) c[q] = side ? mL[q] : mR[q];
+        }
+        double par[3];
+        par[0] = FX[v] * c[0] + FY[v] * c[1] + FZ[v] * c[2];
+        par[1] = FX[v] * c[3] + FY[v] * c[4] + FZ[v] * c[5];
+        par[2] = FX[v] * c[6] + FY[v] * c[7] + FZ[v] * c[8];
+        sSym[side][0][v * N2 + m] = par[PN];
+        sSym[side][1][v * N2 + m] = par[PF];
+        sSym[side][2][v * N2 + m] = par[PS];
+      }
     }
   }
   __syncthreads();
-  // ---- lift to both elements. Parametric sym slots (r,s,t) -> (normal, fast, slow)
-  const int iN = dir, iF = dir == 0 ? 1 : 0, iS = dir == 2 ? 1 : 2;
-  // P[side][0..2][v][s'][f] reusing sT
-  for (int idx = tid; idx < 2 * 5 * M * K; idx += 64) {
-    const int side = idx / (5 * M * K);
-    const int r = idx - side * 5 * M * K;
-    const int v = r / (M * K), s = (r / K) % M, f = r % K;
+  // ---- lift: fast-axis contraction P[side][0..2][v][s'][f]
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {
     const double sign = side ? 1.0 : -1.0;
-    const double *fx = &sFlux[v * N2 + s * M], *yF = &sSym[side][iF][v * N2 + s * M],
-                 *yS = &sSym[side][iS][v * N2 + s * M], *yN = &sSym[side][iN][v * N2 + s * M];
-    double p1 = 0.0, p2 = 0.0, p3 = 0.0;
+    for (int pc = tid; pc < 5 * M; pc += 64) {
+      const double *fx = &sFlux[pc * M], *yN = &sSym[side][0][pc * M], *yF = &sSym[side][1][pc * M],
+                   *yS = &sSym[side][2][pc * M];
+      double xf[M], xF[M], xS[M], xN[M];
 #pragma unroll
-    for (int fp = 0; fp < M; ++fp) {
-      const double af = sA[fp * K + f], df = sD[fp * K + f];
-      p1 += af * sign * fx[fp] + df * yF[fp];
-      p2 += af * yS[fp];
-      p3 += af * yN[fp];
+      for (int q = 0; q < M; ++q) { xf[q] = sign * fx[q]; xF[q] = yF[q]; xS[q] = yS[q]; xN[q] = yN[q]; }
+#pragma unroll
+      for (int n = 0; n < K; ++n) {
+        double p1 = 0.0, p2 = 0.0, p3 = 0.0;
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+          p1 += A[q * K + n] * xf[q] + D[q * K + n] * xF[q];
+          p2 += A[q * K + n] * xS[q];
+          p3 += A[q * K + n] * xN[q];
+        }
+        sT[side][0][pc * K + n] = p1; sT[side][1][pc * K + n] = p2; sT[side][2][pc * K + n] = p3;
+      }
     }
-    sT[side][0][r] = p1; sT[side][1][r] = p2; sT[side][2][r] = p3;
   }
   __syncthreads();
-  // G[side][0|1][v][s][f] reusing sB
-  for (int idx = tid; idx < 2 * 5 * K * K; idx += 64) {
-    const int side = idx / (5 * K * K);
-    const int r = idx - side * 5 * K * K;
-    const int v = r / (K * K), s = (r / K) % K, f = r % K;
-    double g1 = 0.0, g2 = 0.0;
+  // slow-axis contraction G[side][0|1][v][s][f]
+  for (int pc = tid; pc < 2 * 5 * K; pc += 64) {
+    const int side = pc / (5 * K), r = pc - side * 5 * K;
+    const int v = r / K, f = r - v * K;
+    double x1[M], x2[M], x3[M];
 #pragma unroll
-    for (int sp = 0; sp < M; ++sp) {
-      const int j = (v * M + sp) * K + f;
-      g1 += sA[sp * K + s] * sT[side][0][j] + sD[sp * K + s] * sT[side][1][j];
-      g2 += sA[sp * K + s] * sT[side][2][j];
+    for (int q = 0; q < M; ++q) {
+      const int j = (v * M + q) * K + f;
+      x1[q] = sT[side][0][j]; x2[q] = sT[side][1][j]; x3[q] = sT[side][2][j];
     }
-    sB[side][0][r] = g1; sB[side][1][r] = g2;
+#pragma unroll
+    for (int n = 0; n < K; ++n) {
+      double g1 = 0.0, g2 = 0.0;
+#pragma unroll
+      for (int q = 0; q < M; ++q) { g1 += A[q * K + n] * x1[q] + D[q * K + n] * x2[q]; g2 += A[q * K + n] * x3[q]; }
+      sB[side][0][(v * K + n) * K + f] = g1; sB[side][1][(v * K + n) * K + f] = g2;
+    }
   }
   __syncthreads();
-  double *outR = a.faceRes + ((size_t) e * 6 + 2 * dir) * 5 * ND;
-  double *outL = a.faceRes + ((size_t) L * 6 + 2 * dir + 1) * 5 * ND;
-  for (int idx = tid; idx < 2 * 5 * ND; idx += 64) {
-    const int side = idx / (5 * ND);
-    const int r = idx - side * 5 * ND;
-    const int v = r / ND, d = r - v * ND;
-    const int i = d % K, j = (d / K) % K, k = d / (K * K);
-    const int t = dir == 0 ? i : dir == 1 ? j : k;
-    const int f = dir == 0 ? j : i;
-    const int s = dir == 2 ? j : k;
-    const double *fv = side ? sF[1] : sF[0], *dfv = side ? sF[3] : sF[2];
-    const int gi = (v * K + s) * K + f;
-    const double val = fv[t] * sB[side][0][gi] + dfv[t] * sB[side][1][gi];
-    (side ? outL : outR)[r] = val;
+  // normal direction: outer product with the face vectors, into the slots
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {
+    double *out = a.faceRes + ((size_t) (side ? L : e) * 6 + 2 * DIR + side) * 5 * ND;
+    const double *Fv = a.op.F[side], *dFv = a.op.F[2 + side];
+    for (int idx = tid; idx < 5 * ND; idx += 64) {
+      const int v = idx / ND, d = idx - v * ND;
+      const int i = d % K, j = (d / K) % K, k = d / (K * K);
+      const int t = DIR == 0 ? i : DIR == 1 ? j : k;
+      const int f = DIR == 0 ? j : i;
+      const int s = DIR == 2 ? j : k;
+      const int gi = (v * K + s) * K + f;
+      double fv = 0.0, dfv = 0.0;
+#pragma unroll
+      for (int q = 0; q < K; ++q) if (q == t) { fv = Fv[q]; dfv = dFv[q]; }
+      out[idx] = fv * sB[side][0][gi] + dfv * sB[side][1][gi];
+    }
   }
 }
 
@@ -977,11 +1014,19 @@ __global__ void stage0_kernel(double *U, double *U0, const double *R, size_t n, 
 template <int K, int M>
 hipError_t faces_impl(const DeviceCase &dc, hipStream_t s)
 {
-  FaceArgs a;
-  a.U = dc.U; a.tables = dc.tables; a.fgeo = dc.fgeo; a.econst = dc.econst; a.nbr = dc.nbr;
-  a.faceRes = dc.faceRes; a.to = dc.to; a.nElem = dc.nElem; a.bug = dc.bug_compat;
+  FaceArgs<K, M> a;
+  a.U = dc.U; a.fgeo = dc.fgeo; a.econst = dc.econst; a.nbr = dc.nbr;
+  a.faceRes = dc.faceRes; a.nElem = dc.nElem; a.bug = dc.bug_compat;
   a.mu = dc.mu; a.tauFact = (dc.P + 1.0) * (dc.P + 1.0) * dc.rel_penalty; a.thetaHalf = 0.5 * dc.theta_sym;
-  hipLaunchKernelGGL((face_kernel<K, M>), dim3(3 * dc.nElem), dim3(64), 0, s, a);
+  for (int i = 0; i < M * K; ++i) { a.op.A[i] = dc.opA[i]; a.op.D[i] = dc.opD[i]; }
+  for (int q = 0; q < K; ++q) {
+    a.op.F[0][q] = dc.opF[0][q]; a.op.F[1][q] = dc.opF[1][q];
+    a.op.F[2][q] = dc.opF[2][q]; a.op.F[3][q] = dc.opF[3][q];
+  }
+  for (int q = 0; q < M; ++q) a.op.w[q] = dc.opW[q];
+  hipLaunchKernelGGL((face_kernel<K, M, 0>), dim3(dc.nElem), dim3(64), 0, s, a);
+  hipLaunchKernelGGL((face_kernel<K, M, 1>), dim3(dc.nElem), dim3(64), 0, s, a);
+  hipLaunchKernelGGL((face_kernel<K, M, 2>), dim3(dc.nElem), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 

@@ -30,6 +30,8 @@ struct DeviceCase {
   double cfl = 1, u_ref = 1, p_ref = 1, rho_ref = 1;
   TableOffsets to{};
   double opA[128] = {}, opD[128] = {};   // host copies of the M x K operators
+  double opF[4][16] = {};                // phi(-1), phi(+1), phi'(-1), phi'(+1)
+  double opW[16] = {};                   // 1D Gauss weights
   double *tables = nullptr;
   double *vgeo = nullptr, *fgeo = nullptr, *econst = nullptr;
   int *nbr = nullptr;

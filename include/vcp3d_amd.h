@@ -98,6 +98,15 @@ int vcp_cg_iterations(vcp_ctx *ctx, int *iters);
    launch of [face kernel, volume+mass-matrix kernel] into ms[2]. */
 int vcp_profile_residual(vcp_ctx *ctx, int reps, double ms[2]);
 
+/* Host-only set-up (no device calls), for checking the tables and metrics
+   without a GPU: SolverClass::ComputeMetricTerms (SolverClass.cpp:829-1016).
+   vol [nElem][13][M^3] (J, dr/dx..dt/dz, x, y, z), faces [6][nElem][16][M^2]
+   (iMin..kMax: unit normal, area factor, dr/dx..dt/dz, x, y, z), lens
+   [nElem][4] (length scales I, J, K, volume), nbr [nElem][6] (-1 boundary).
+   Any output pointer may be NULL. */
+int vcp_host_setup(const vcp_case_desc *desc, const double *elem_grid, double *vol,
+                   double *faces, double *lens, int *nbr);
+
 const char *vcp_last_error(void);
 
 #ifdef __cplusplus

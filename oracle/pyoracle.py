@@ -66,6 +66,9 @@ def lib():
         L.orc_cg_iterations.argtypes = [C.c_void_p, ip]
         L.orc_get_setup.argtypes = [C.c_void_p, C.c_int, dp]
         L.orc_get_table.argtypes = [C.c_void_p, C.c_int, dp]
+        L.orc_tensor_apply.argtypes = [C.c_int] * 6 + [dp] * 5
+        L.orc_cg_solve.argtypes = [C.c_void_p, C.c_int, dp, dp]
+        L.orc_get_dudv.argtypes = [C.c_void_p, C.c_int, dp]
         _lib = L
     return _lib
 
@@ -161,12 +164,32 @@ class Oracle:
         lib().orc_get_setup(self.h, which, out)
         return out
 
+    def cg_solve(self, e, rhs):
+        x = np.zeros((5, self.n_dofs))
+        it = self._check(lib().orc_cg_solve(self.h, e, np.ascontiguousarray(rhs, dtype=np.float64), x))
+        return x, it
+
+    def dudv(self, e):
+        out = np.zeros((15, self.M ** 3))
+        lib().orc_get_dudv(self.h, e, out)
+        return out
+
     def table(self, which):
         K, M = self.K, self.M
         shape = {0: (M,), 1: (M,), 2: (K, M), 3: (K, M), 4: (K,), 5: (self.case.p + 1,)}[which]
         out = np.zeros(shape)
         lib().orc_get_table(self.h, which, out)
         return out
+
+
+def tensor_apply(ox, oy, oz, tin):
+    """out[k][j][i] = sum Ox[i][ii] Oy[j][jj] Oz[k][kk] in[kk][jj][ii]."""
+    ox, oy, oz = (np.ascontiguousarray(o, dtype=np.float64) for o in (ox, oy, oz))
+    tin = np.ascontiguousarray(tin, dtype=np.float64)
+    niz, niy, nix = tin.shape
+    out = np.zeros((oz.shape[0], oy.shape[0], ox.shape[0]))
+    lib().orc_tensor_apply(nix, niy, niz, ox.shape[0], oy.shape[0], oz.shape[0], ox, oy, oz, tin, out)
+    return out
 
 
 def run_reference(param_path: str, n_steps: int, dumps=(), threads: int | None = None,

@@ -1143,3 +1143,37 @@ void orc_get_table(const orc_case *c, int which, double *out)
     case 5: memcpy(out, c->rGrid, sizeof(double) * c->Kg); break;
   }
 }
+
+/* ------------------------------------------------------------ test hooks */
+void orc_tensor_apply(int nix, int niy, int niz, int nox, int noy, int noz,
+                      const double *ox, const double *oy, const double *oz,
+                      const double *in, double *out)
+{
+  size_t n = (size_t) niz * niy * nox + (size_t) niz * noy * nox + 16;
+  double *w = dalloc(n);
+  tp3(nix, niy, niz, nox, noy, noz, ox, oy, oz, in, out, 0, w);
+  free(w);
+}
+
+/* Solves the element's entropy mass-matrix system for rhs [5][nDOFs] with
+   the flexible PCG of CG_EntropyVarMassMatrix; dU/dV is the one stored by
+   the last orc_residual call. Returns the iteration count or <0. */
+int orc_cg_solve(orc_case *c, int e, const double *rhs, double *x)
+{
+  const size_t nv = (size_t) NVAR * c->nDOFs;
+  double *r = c->res + (size_t) e * nv;
+  double *save = dalloc(nv);
+  memcpy(save, r, sizeof(double) * nv);
+  memcpy(r, rhs, sizeof(double) * nv);
+  const int rc = cg_entropy(c, e);
+  memcpy(x, r, sizeof(double) * nv);
+  memcpy(r, save, sizeof(double) * nv);
+  free(save);
+  return rc < 0 ? rc : c->cgIt[e];
+}
+
+/* dU/dV at the integration points of element e, [15][nInt]. */
+void orc_get_dudv(const orc_case *c, int e, double *out)
+{
+  memcpy(out, c->dUdV + (size_t) e * 15 * c->nInt, sizeof(double) * 15 * c->nInt);
+}

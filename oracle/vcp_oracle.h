@@ -72,6 +72,14 @@ void orc_get_setup(const orc_case *c, int which, double *out);
    3 derivative [K][M], 4 nodal DOF locations [K], 5 grid DOF locations. */
 void orc_get_table(const orc_case *c, int which, double *out);
 
+/* Test hooks: one sum-factorised tensor application (out = Oz Oy Ox in),
+   one element mass-matrix solve, dU/dV at the points of an element. */
+void orc_tensor_apply(int nix, int niy, int niz, int nox, int noy, int noz,
+                      const double *ox, const double *oy, const double *oz,
+                      const double *in, double *out);
+int  orc_cg_solve(orc_case *c, int e, const double *rhs, double *x);
+void orc_get_dudv(const orc_case *c, int e, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,72 @@
+"""Host-side pieces of the library and the package, no GPU: the set-up the
+GPU path runs on (tables, metrics, neighbour maps) against the oracle, and
+the file formats."""
+import os
+
+import numpy as np
+import pytest
+
+from vcp3d_open_amd import io as vio
+from vcp3d_open_amd.cases import TGVCase
+from vcp3d_open_amd.solver import host_setup
+
+
+@pytest.mark.parametrize("kw", [dict(n=3, p=2), dict(n=4, p=4, curvilinear=True), dict(n=4, p=5, curvilinear=True)])
+def test_host_setup_matches_oracle(oracle_lib, kw):
+    case = TGVCase(**kw)
+    h = host_setup(case)
+    o = oracle_lib.Oracle(case)
+    vol = o.setup(0)
+    scale = np.abs(vol).max(axis=(0, 2), keepdims=True)
+    assert np.all(np.abs(h["vol"] - vol) <= 1e-12 * scale)
+    for f in range(6):
+        fm = o.setup(1 + f)
+        scale = np.abs(fm).max(axis=(0, 2), keepdims=True) + 1e-300
+        assert np.all(np.abs(h["faces"][f] - fm) <= 1e-12 * scale), f
+    np.testing.assert_allclose(h["lens"], o.setup(7), rtol=1e-12)
+    assert np.array_equal(h["nbr"], o.setup(8).astype(np.int32))     # bit-exact maps
+
+
+def test_unsupported_grid_is_rejected():
+    from vcp3d_open_amd.solver import VCPError
+    case = TGVCase(n=3, p=5, curvilinear=True)   # perturbation too strong for LGL detection
+    with pytest.raises(VCPError, match="point distribution"):
+        host_setup(case)
+
+
+def test_plot3d_round_trip(tmp_path):
+    case = TGVCase(n=3, p=3, curvilinear=True)
+    xyz = case.grid_block()
+    path = os.path.join(tmp_path, "g.xyz")
+    vio.write_plot3d(path, xyz)
+    assert np.array_equal(vio.read_plot3d(path), xyz)
+    # header layout of the reference reader: 4 | 1 | 4 | 12 | il jl kl | 12 | 8*3*n
+    raw = np.fromfile(path, dtype=np.int32, count=9)
+    il = 3 * 3 + 1
+    assert list(raw) == [4, 1, 4, 12, il, il, il, 12, 8 * 3 * il ** 3]
+
+
+def test_su2_restart_round_trip(tmp_path):
+    case = TGVCase(n=2, p=3)
+    cons = case.initial_conservative()
+    path = os.path.join(tmp_path, "r.dat")
+    vio.write_su2_restart(path, case.nodal_coords(), cons, time_step=7)
+    names, data, step = vio.read_su2_restart(path, case.n_dofs)
+    assert names[:4] == ["x", "y", "z", "Density"] and step == 7
+    assert np.array_equal(data[:, :, 3:], cons)
+    hdr = np.fromfile(path, dtype=np.int32, count=5)
+    assert list(hdr) == [535532, 8, case.n_elems * case.n_dofs, 1, 8]
+
+
+def test_taylor_green_initial_state():
+    case = TGVCase(n=4, p=3)
+    x = case.nodal_coords()
+    cons = case.initial_conservative()
+    rho = cons[..., 0]
+    u = cons[..., 1] / rho
+    v0 = case.ref["v_inf"]
+    np.testing.assert_allclose(u, v0 * np.sin(x[..., 0]) * np.cos(x[..., 1]) * np.cos(x[..., 2]), atol=1e-12 * v0)
+    assert np.abs(cons[..., 3]).max() == 0.0
+    # periodic grid: opposite faces coincide after the translation
+    g = case.grid_block()
+    np.testing.assert_allclose(g[0, :, :, -1] - g[0, :, :, 0], 2 * np.pi, rtol=1e-15)

@@ -205,6 +205,31 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
   return 0;
 }
 
+int vcp_host_setup(const vcp_case_desc *d, const double *elem_grid, double *vol, double *faces,
+                   double *lens, int *nbr)
+{
+  if (!d || !elem_grid) return fail("null argument");
+  vcp::CaseParams p;
+  for (int i = 0; i < 3; ++i) { p.n_elem[i] = d->n_elem[i]; p.periodic[i] = d->periodic[i]; }
+  p.n_poly_grid = d->n_poly_grid; p.n_poly_sol = d->n_poly_sol; p.n_poly_int_rule = d->n_poly_int_rule;
+  p.dof_location = d->dof_location;
+  if (p.n_elem[0] <= 0 || p.n_elem[1] <= 0 || p.n_elem[2] <= 0) return fail("invalid element counts");
+  if (p.n_poly_sol <= 0 || p.n_poly_sol > 9 || p.n_poly_grid <= 0 || p.n_poly_grid > 9)
+    return fail("polynomial degrees must be in 1..9");
+  vcp::Tables t;
+  vcp::build_tables(p, t);
+  vcp::Geometry g;
+  const std::string err = vcp::build_geometry(p, t, elem_grid, g);
+  if (!err.empty()) return fail(err);
+  if (vol) std::memcpy(vol, g.vol.data(), g.vol.size() * sizeof(double));
+  if (faces)
+    for (int f = 0; f < 6; ++f)
+      std::memcpy(faces + f * g.face[f].size(), g.face[f].data(), g.face[f].size() * sizeof(double));
+  if (lens) std::memcpy(lens, g.lens.data(), g.lens.size() * sizeof(double));
+  if (nbr) std::memcpy(nbr, g.nbr.data(), g.nbr.size() * sizeof(int));
+  return 0;
+}
+
 int vcp_destroy(vcp_ctx *c)
 {
   if (c) { (void) hipStreamSynchronize(c->stream); delete c; }

@@ -37,7 +37,7 @@ class CaseDesc(C.Structure):
 EXPORTS = ["vcp_create", "vcp_destroy", "vcp_sizes", "vcp_reference", "vcp_set_restart",
            "vcp_set_solution", "vcp_get_solution", "vcp_get_nodal", "vcp_residual",
            "vcp_get_residual", "vcp_sync_step", "vcp_time_steps", "vcp_cg_iterations",
-           "vcp_profile_residual", "vcp_last_error"]
+           "vcp_profile_residual", "vcp_host_setup", "vcp_last_error"]
 
 _lib = None
 
@@ -67,6 +67,7 @@ def load_library(path: str = LIB_PATH):
     L.vcp_time_steps.argtypes = [vp, dp]
     L.vcp_cg_iterations.argtypes = [vp, ip]
     L.vcp_profile_residual.argtypes = [vp, C.c_int, dp]
+    L.vcp_host_setup.argtypes = [C.POINTER(CaseDesc), dp, dp, dp, dp, ip]
     L.vcp_last_error.restype = C.c_char_p
     for name in EXPORTS:
         if name != "vcp_last_error":
@@ -93,6 +94,22 @@ def desc_from_case(case, bug_compat: bool = True) -> CaseDesc:
     d.cfl, d.dt_synchr = case.cfl, case.dt_synchr
     d.bug_compat = 1 if bug_compat else 0
     return d
+
+
+def host_setup(case, bug_compat: bool = True) -> dict:
+    """Host-only tables/metrics of the library (no GPU needed)."""
+    L = load_library()
+    desc = desc_from_case(case, bug_compat)
+    grid = np.ascontiguousarray(case.element_grid(), dtype=np.float64)
+    E, M = case.n_elems, case.rule // 2 + 1
+    vol = np.zeros((E, 13, M ** 3))
+    faces = np.zeros((6, E, 16, M * M))
+    lens = np.zeros((E, 4))
+    nbr = np.zeros((E, 6), dtype=np.int32)
+    rc = L.vcp_host_setup(C.byref(desc), grid, vol, faces, lens, nbr)
+    if rc < 0:
+        raise VCPError(L.vcp_last_error().decode())
+    return dict(vol=vol, faces=faces, lens=lens, nbr=nbr)
 
 
 class SolverClass:

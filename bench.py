@@ -60,6 +60,25 @@ def face_kernel_bytes(K: int, M: int) -> float:
     return 8.0 * (2 * 5 * K**3 + 22 * M**2 + 2 * 5 * K**3 + 2)
 
 
+def calib_copy(nbytes: int) -> int:
+    """One device-to-device hipMemcpy of nbytes (the runtime's copy kernel):
+    the reference point that scales the HBM byte counters."""
+    import ctypes as C
+    hip = C.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so"))
+    src, dst = C.c_void_p(), C.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, C.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    if hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) != 0:   # hipMemcpyDeviceToDevice
+        raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    return nbytes
+
+
 def cpu_baseline(case, steps: int = 1) -> dict:
     """C oracle (OpenMP over the host cores) on the same case: one RK step."""
     from oracle.pyoracle import Oracle, build
@@ -86,6 +105,8 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=5)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -141,6 +162,8 @@ def main() -> int:
         "roofline": roofline,
         "mach2_max": float(md[6]),
     }
+    if args.calib_copy_mib:
+        out["calib_copy_bytes"] = calib_copy(args.calib_copy_mib << 20)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(case)
     print(json.dumps(out))

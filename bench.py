@@ -43,9 +43,22 @@ def volume_kernel_flops(K: int, M: int, cg_iters: float) -> float:
 
 
 def volume_kernel_bytes(K: int, M: int) -> float:
-    """Algorithmic HBM bytes per element: U, U0, geometry (J + 9 metric terms
-    per point), 16 element constants, six face slots read; U written."""
-    return 8.0 * (5 * K**3 * 3 + 10 * M**3 + 16 + 30 * K**3)
+    """Algorithmic HBM bytes per element of a residual launch: U, geometry
+    (J + 9 metric terms per point), 16 element constants and the six face
+    slots read; R written."""
+    return 8.0 * (5 * K**3 + 30 * K**3 + 10 * M**3 + 16 + 5 * K**3)
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch from the calibrated FETCH_SIZE/WRITE_SIZE passes
+    recorded in profiles/ for this kernel, or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_tgv32_p4_fp64.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+        return data["kernels"][kernel]["bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def face_kernel_flops(K: int, M: int) -> float:
@@ -137,16 +150,16 @@ def main() -> int:
     ms_face, ms_vol = solver.profile_residual(args.profile_reps)
     K, M = case.p + 1, 7
     if ms_vol >= ms_face:
-        name, ms = "volume_kernel<5,7>", ms_vol
+        name, ms = "volume_kernel<5, 7, 256>", ms_vol
         flops = volume_kernel_flops(K, M, cg_mean) * case.n_elems
         bytes_ = volume_kernel_bytes(K, M) * case.n_elems
     else:
-        name, ms = "face_kernel<5,7>", ms_face
+        name, ms = "face_kernel<5, 7, DIR> x3", ms_face
         flops = face_kernel_flops(K, M) * 3 * case.n_elems
         bytes_ = face_kernel_bytes(K, M) * 3 * case.n_elems
     achieved = flops / (ms * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None, "kernel": name,
+                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(name), "kernel": name,
                 "ms_per_launch": ms, "algorithmic_flops": flops, "algorithmic_bytes": bytes_,
                 "hbm_gbs_algorithmic": bytes_ / (ms * 1e-3) / 1e9,
                 "ms_face_kernel": ms_face, "ms_volume_kernel": ms_vol, "cg_iters_mean": cg_mean}

@@ -29,7 +29,37 @@ class OrcParams(C.Structure):
         ("theta_sym", C.c_double), ("rel_penalty", C.c_double),
         ("cfl", C.c_double), ("dt_synchr", C.c_double),
         ("bug_compat", C.c_int),
+        ("bc", C.c_int * 6),
+        ("bc_data", (C.c_double * 5) * 6),
+        ("outlet_len", C.c_double), ("outlet_sigma", C.c_double),
+        ("outlet_beta_l", C.c_double), ("outlet_beta_t", C.c_double),
+        ("outlet_one_minus_beta", C.c_int), ("outlet_average_local", C.c_int),
+        ("inlet_len", C.c_double), ("inlet_sigma", C.c_double),
+        ("inlet_incoming", C.c_int),
+        ("sponge_layer", C.c_int), ("cml", C.c_int),
+        ("sponge_elems", C.c_int * 6),
+        ("damping_constant", C.c_double * 6), ("damping_exponent", C.c_double * 6),
     ]
+
+
+def fill_struct(st, desc: dict):
+    """Copies a case description dict into a ctypes structure with the same
+    field names (arrays element-wise, nested for bc_data)."""
+    for name, _ in st._fields_:
+        if name not in desc:
+            continue
+        val = desc[name]
+        attr = getattr(st, name)
+        if hasattr(attr, "__len__"):
+            for i, v in enumerate(val):
+                if hasattr(attr[i], "__len__"):
+                    for j, w in enumerate(v):
+                        attr[i][j] = w
+                else:
+                    attr[i] = v
+        else:
+            setattr(st, name, val)
+    return st
 
 
 def build(force: bool = False) -> str:
@@ -69,6 +99,8 @@ def lib():
         L.orc_tensor_apply.argtypes = [C.c_int] * 6 + [dp] * 5
         L.orc_cg_solve.argtypes = [C.c_void_p, C.c_int, dp, dp]
         L.orc_get_dudv.argtypes = [C.c_void_p, C.c_int, dp]
+        L.orc_set_damping_state.argtypes = [C.c_void_p, dp]
+        L.orc_nscbc_average.argtypes = [C.c_void_p, dp]
         _lib = L
     return _lib
 
@@ -78,17 +110,7 @@ class Oracle:
 
     def __init__(self, case, bug_compat: bool = True):
         L = lib()
-        p = OrcParams()
-        p.n_elem[:] = list(case.n_elem)
-        p.n_poly_grid = case.p
-        p.n_poly_sol = case.p
-        p.n_poly_int_rule = case.rule
-        p.dof_location = 2
-        p.periodic[:] = [1, 1, 1]
-        p.mach, p.reynolds, p.reynolds_length = case.mach, case.reynolds, 1.0
-        p.temp_free_stream = case.temp
-        p.theta_sym, p.rel_penalty = case.theta_sym, case.rel_penalty
-        p.cfl, p.dt_synchr = case.cfl, case.dt_synchr
+        p = fill_struct(OrcParams(), case.description())
         p.bug_compat = 1 if bug_compat else 0
         grid = np.ascontiguousarray(case.element_grid())
         h = L.orc_create(C.byref(p), grid)
@@ -118,6 +140,14 @@ class Oracle:
 
     def set_restart(self, nodal):
         self._check(lib().orc_set_restart(self.h, np.ascontiguousarray(nodal, dtype=np.float64)))
+
+    def set_damping_state(self, nodal_prim):
+        self._check(lib().orc_set_damping_state(self.h, np.ascontiguousarray(nodal_prim, dtype=np.float64)))
+
+    def nscbc_average(self):
+        out = np.zeros(6)
+        lib().orc_nscbc_average(self.h, out)
+        return out
 
     def set_modal(self, modal):
         lib().orc_set_modal(self.h, np.ascontiguousarray(modal, dtype=np.float64))
@@ -159,7 +189,8 @@ class Oracle:
 
     def setup(self, which):
         M, E = self.M, self.n_elem
-        shape = {0: (E, 13, M ** 3), 7: (E, 4), 8: (E, 6)}.get(which, (E, 16, M * M))
+        shape = {0: (E, 13, M ** 3), 7: (E, 4), 8: (E, 6), 9: (E, 3, M ** 3),
+                 10: (E, 5, M ** 3)}.get(which, (E, 16, M * M))
         out = np.zeros(shape)
         lib().orc_get_setup(self.h, which, out)
         return out
@@ -176,7 +207,8 @@ class Oracle:
 
     def table(self, which):
         K, M = self.K, self.M
-        shape = {0: (M,), 1: (M,), 2: (K, M), 3: (K, M), 4: (K,), 5: (self.case.p + 1,)}[which]
+        shape = {0: (M,), 1: (M,), 2: (K, M), 3: (K, M), 4: (K,), 5: (self.case.p + 1,),
+                 6: (M * M, M * M), 7: (M, M), 8: (1,)}[which]
         out = np.zeros(shape)
         lib().orc_get_table(self.h, which, out)
         return out

@@ -49,6 +49,23 @@ struct orc_case {
   double *VI;             /* K x K : inverse of LD       (nodal->modal) */
   double *FMin, *FMax, *dFMin, *dFMax; /* 1 x K face vectors (also K x 1) */
   double *w3, *w2, *basisCenter;
+  double *Lq;             /* M x K : nodal DOF Lagrange basis at the Gauss points */
+  double *dLq;            /* M x M : Gauss-point Lagrange derivative, [point][basis] */
+  double *PC;             /* nInt2D x nInt2D polynomial-correction matrix */
+  double dell0;           /* -l'_0(r_0) of the nodal DOF basis */
+  /* physical boundaries (PC-NSCBC) */
+  int nscbc;              /* any NSCBC face */
+  int ig[6][3];           /* gradient slots (normal, first, second transverse) */
+  double bcData[6][5];    /* non-dimensional prescribed data */
+  double faceArea[6], machAvg[6];
+  /* sponge layer / characteristic matching layer */
+  int *spongeIdx;         /* [nElem] -> sponge slot or -1 */
+  int nSponge;
+  unsigned char *spongeFlag; /* bit 2*dir+side: element lies in that layer */
+  double *dampFn;         /* [nSponge][3][nInt] */
+  double *dampState;      /* [nSponge][5][nInt] conservative, non-dimensional */
+  double *matchScale;     /* [nSponge][3] normalisation of the damping function */
+  int dampStateSet;
   /* geometry */
   double *vm;             /* [nElem][13][nInt] */
   double *sm[6];          /* [nElem][16][nInt2D] iMin,iMax,jMin,jMax,kMin,kMax */
@@ -66,6 +83,8 @@ struct orc_case {
 };
 
 static double *W(const orc_case *c) { return c->works[omp_get_thread_num()]; }
+static int is_nscbc(int bc);
+static void nscbc_average(orc_case *c);
 
 /* ------------------------------------------------------------------ utils */
 static double *dalloc(size_t n) { return (double *) calloc(n ? n : 1, sizeof(double)); }
@@ -269,6 +288,48 @@ static void standard_element(orc_case *c)
   for (int k = 0; k < K; ++k) for (int j = 0; j < K; ++j) for (int i = 0; i < K; ++i)
     c->basisCenter[(k * K + j) * K + i] = v0[i] * v0[j] * v0[k];
   free(v0);
+  /* Lagrange tables (StandardElementClass::ValuesLagrangeBasis1D,
+     StandardElementClass.cpp:327-414) */
+  c->Lq = dalloc(M * K); c->dLq = dalloc(M * M);
+  for (int p = 0; p < M; ++p) {
+    lagrange(K, c->rDOF, c->rInt[p], c->Lq + p * K, NULL);
+    lagrange(M, c->rInt, c->rInt[p], NULL, c->dLq + p * M);
+  }
+  double *dl = dalloc(K);
+  lagrange(K, c->rDOF, c->rDOF[0], NULL, dl);
+  c->dell0 = -dl[0];
+  free(dl);
+  /* polynomial-correction matrix (StandardElementClass::
+     AssemblePolynomialCorrectionOperators, StandardElementClass.cpp:176-291):
+     PC = L (dL^T dL)^-1 dL^T with L the 2D nodal->Gauss interpolation and
+     dL = dell0 * L. */
+  const int n2 = M * M, k2 = K * K;
+  double *L2 = dalloc((size_t) n2 * k2), *LS = dalloc((size_t) k2 * k2), *T = dalloc((size_t) k2 * n2);
+  for (int i1 = 0; i1 < M; ++i1) for (int i2 = 0; i2 < M; ++i2)
+    for (int j1 = 0; j1 < K; ++j1) for (int j2 = 0; j2 < K; ++j2)
+      L2[(i1 * M + i2) * k2 + j1 * K + j2] = c->Lq[i1 * K + j1] * c->Lq[i2 * K + j2];
+  const double c0 = c->dell0;
+  for (int i = 0; i < k2; ++i)
+    for (int j = 0; j < k2; ++j) {
+      double t = 0.0;
+      for (int q = 0; q < n2; ++q) t += (c0 * L2[q * k2 + i]) * (c0 * L2[q * k2 + j]);
+      LS[i * k2 + j] = t;
+    }
+  invert(k2, LS);
+  for (int i = 0; i < k2; ++i)
+    for (int j = 0; j < n2; ++j) {
+      double t = 0.0;
+      for (int q = 0; q < k2; ++q) t += LS[i * k2 + q] * (c0 * L2[j * k2 + q]);
+      T[i * n2 + j] = t;
+    }
+  c->PC = dalloc((size_t) n2 * n2);
+  for (int i = 0; i < n2; ++i)
+    for (int j = 0; j < n2; ++j) {
+      double t = 0.0;
+      for (int q = 0; q < k2; ++q) t += L2[i * k2 + q] * T[q * n2 + j];
+      c->PC[i * n2 + j] = t;
+    }
+  free(L2); free(LS); free(T);
 }
 
 /* ---------------------------------------------------------------- metrics */
@@ -481,8 +542,69 @@ static void ent2prim(const double *V, double *rho, double *u, double *v, double 
   *p = -(*rho) * V4Inv;
 }
 
-/* ElementClass::VolumeResidual, steps 1, 3, 5, 6, 7 (ElementClass.cpp:3809-4443)
-   for entropy variables without SGS model, CML, sponge or body force. */
+/* Characteristic-matching-layer source at one integration point
+   (ElementClass::VolumeResidual step 2, ElementClass.cpp:3887-4117): wave
+   amplitudes of the Cartesian primitive gradients in x, y and z with the
+   outgoing eigenvalues of the layer orientation removed, P = S L, mapped to
+   conservative variables and weighted with the normalised damping
+   functions. V: entropy variables, gx/gy/gz: their Cartesian gradients,
+   wgt[3]: -J w sigma_d / sigma_max, nsg[3]: layer orientation signs. */
+static void cml_source(const double *V, const double *gx, const double *gy, const double *gz,
+                       const double wgt[3], const double nsg[3], double *out)
+{
+  const double gm1 = GAM - 1.0, ov1mg = -1.0 / gm1, ovgm1 = -ov1mg;
+  const double V4Inv = 1.0 / V[4];
+  const double u = -V4Inv * V[1], v = -V4Inv * V[2], w = -V4Inv * V[3];
+  const double ek = 0.5 * (u * u + v * v + w * w);
+  const double s = GAM - gm1 * (V[0] - V[4] * ek);
+  const double rho = pow(-V[4] * exp(s), ov1mg);
+  const double p = -rho * V4Inv, rE = -p * ov1mg + rho * ek;
+  const double H = rE / rho - V4Inv, a2 = -GAM * V4Inv, a = sqrt(a2);
+  const double rhoa = rho * a, ovra = 1.0 / rhoa, ova2 = 1.0 / a2, rova = rho / a;
+  const double rhou = rho * u, rhov = rho * v, rhow = rho * w;
+  /* entropy -> primitive Jacobian rows */
+  const double J11 = rho, J12 = rhou, J13 = rhov, J14 = rhow, J15 = rE;
+  const double J22 = -V4Inv, J25 = -V4Inv * u, J33 = -V4Inv, J35 = -V4Inv * v, J44 = -V4Inv, J45 = -V4Inv * w;
+  const double J51 = p, J52 = p * u, J53 = p * v, J54 = p * w, J55 = p * H;
+  const double *g[3] = {gx, gy, gz};
+  double dq[3][5];
+  for (int d = 0; d < 3; ++d) {
+    const double *G = g[d];
+    dq[d][0] = J11 * G[0] + J12 * G[1] + J13 * G[2] + J14 * G[3] + J15 * G[4];
+    dq[d][1] = J22 * G[1] + J25 * G[4];
+    dq[d][2] = J33 * G[2] + J35 * G[4];
+    dq[d][3] = J44 * G[3] + J45 * G[4];
+    dq[d][4] = J51 * G[0] + J52 * G[1] + J53 * G[2] + J54 * G[3] + J55 * G[4];
+  }
+  const double vel[3] = {u, v, w};
+  for (int l = 0; l < 5; ++l) out[l] = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    double l1 = vel[d] - a, l2 = vel[d], l3 = vel[d] + a;
+    if (nsg[d] * l1 > 0.0) l1 = 0.0;
+    if (nsg[d] * l2 > 0.0) l2 = 0.0;
+    if (nsg[d] * l3 > 0.0) l3 = 0.0;
+    const double dr = dq[d][0], dp = dq[d][4];
+    const double dun = dq[d][1 + d];              /* normal velocity derivative */
+    const double La = 0.5 * l1 * (ovra * dp - dun);
+    const double Lb = 0.5 * l3 * (ovra * dp + dun);
+    const double Le = l2 * (dr - ova2 * dp);      /* entropy wave */
+    /* P = S L per direction: continuity, momentum (normal: acoustic, the two
+       tangential ones: vorticity), energy (ElementClass.cpp:4038-4062) */
+    double P[5];
+    P[0] = rova * (Lb + La) + Le;
+    P[4] = rhoa * (Lb + La);
+    for (int t = 0; t < 3; ++t) P[1 + t] = (t == d) ? (Lb - La) : l2 * dq[d][1 + t];
+    const double wgt_d = wgt[d];
+    out[0] += wgt_d * P[0];
+    out[1] += wgt_d * (u * P[0] + rho * P[1]);
+    out[2] += wgt_d * (v * P[0] + rho * P[2]);
+    out[3] += wgt_d * (w * P[0] + rho * P[3]);
+    out[4] += wgt_d * (ek * P[0] + rhou * P[1] + rhov * P[2] + rhow * P[3] + ovgm1 * P[4]);
+  }
+}
+
+/* ElementClass::VolumeResidual, steps 1-3, 5-8 (ElementClass.cpp:3809-4489)
+   for entropy variables without SGS model or body force. */
 static void volume_residual(orc_case *c, int e, int mon, double *mach2max)
 {
   const int K = c->K, M = c->M, nInt = c->nInt, nD = c->nDOFs;
@@ -512,9 +634,31 @@ static void volume_residual(orc_case *c, int e, int mon, double *mach2max)
   for (int i = 0; i < NVAR * nD; ++i) res[i] = 0.0;
   const double gm1 = GAM - 1.0;
   const double factHF = c->mu * GAM / PRANDTL_LAM;
+  /* sponge element: CML and damping sources (ElementClass.cpp:3887-4117,
+     :4450-4489) */
+  const int sp = c->spongeIdx ? c->spongeIdx[e] : -1;
+  const int useCml = sp >= 0 && c->prm.cml;
+  double *Scml = Wb + 20 * (size_t) nInt, *Ssp = Wb + 25 * (size_t) nInt;
+  const double *dfn = sp >= 0 ? c->dampFn + (size_t) sp * 3 * nInt : NULL;
+  const double *dst = sp >= 0 ? c->dampState + (size_t) sp * 5 * nInt : NULL;
+  double ovs[3] = {0.0, 0.0, 0.0}, nsg[3] = {1.0, 1.0, 1.0};
+  if (sp >= 0)
+    for (int d = 0; d < 3; ++d) {
+      const int fl = c->spongeFlag[sp] >> (2 * d);
+      if (fl & 1) nsg[d] = -1.0;
+      if (fl & 3) ovs[d] = c->matchScale[(size_t) sp * 3 + d];
+    }
   for (int m = 0; m < nInt; ++m) {
     double V[5], rho, u, v, ww, p, eKin;
     for (int l = 0; l < 5; ++l) V[l] = Var[l * nInt + m];
+    if (useCml) {
+      double gx[5], gy[5], gz[5], o[5];
+      for (int l = 0; l < 5; ++l) { gx[l] = Dr[l * nInt + m]; gy[l] = Ds[l * nInt + m]; gz[l] = Dt[l * nInt + m]; }
+      const double weight = -vm[m] * c->w3[m];
+      const double wgt[3] = {weight * ovs[0] * dfn[m], weight * ovs[1] * dfn[nInt + m], weight * ovs[2] * dfn[2 * nInt + m]};
+      cml_source(V, gx, gy, gz, wgt, nsg, o);
+      for (int l = 0; l < 5; ++l) Scml[l * nInt + m] = o[l];
+    }
     ent2prim(V, &rho, &u, &v, &ww, &p, &eKin);
     const double V4Inv = 1.0 / V[4];
     const double rE = p / gm1 + rho * eKin;
@@ -537,6 +681,15 @@ static void volume_residual(orc_case *c, int e, int mon, double *mach2max)
       const double M2 = rho * (u * u + v * v + ww * ww) / (GAM * p);
       if (M2 > *mach2max) *mach2max = M2;
     }
+    if (sp >= 0) {
+      const double weight = vm[m] * c->w3[m];
+      const double sigma = weight * (dfn[m] + dfn[nInt + m] + dfn[2 * nInt + m]);
+      Ssp[m] = sigma * (rho - dst[m]);
+      Ssp[nInt + m] = sigma * (rho * u - dst[nInt + m]);
+      Ssp[2 * nInt + m] = sigma * (rho * v - dst[2 * nInt + m]);
+      Ssp[3 * nInt + m] = sigma * (rho * ww - dst[3 * nInt + m]);
+      Ssp[4 * nInt + m] = sigma * (rE - dst[4 * nInt + m]);
+    }
     /* fluxes (ElementClass.cpp:4314-4416) */
     const double weight = -vm[m] * c->w3[m];
     const double muT = c->mu;
@@ -554,11 +707,15 @@ static void volume_residual(orc_case *c, int e, int mon, double *mach2max)
       for (int l = 0; l < 5; ++l) F[l * nInt + m] = fx[l] * mx + fy[l] * my + fz[l] * mz;
     }
   }
+  if (useCml)
+    for (int l = 0; l < NVAR; ++l) tp3(M, M, M, K, K, K, c->AT, c->AT, c->AT, Scml + l * nInt, res + l * nD, 1, w);
   for (int l = 0; l < NVAR; ++l) {
     tp3(M, M, M, K, K, K, c->DT, c->AT, c->AT, Dr + l * nInt, res + l * nD, 1, w);
     tp3(M, M, M, K, K, K, c->AT, c->DT, c->AT, Ds + l * nInt, res + l * nD, 1, w);
     tp3(M, M, M, K, K, K, c->AT, c->AT, c->DT, Dt + l * nInt, res + l * nD, 1, w);
   }
+  if (sp >= 0)
+    for (int l = 0; l < NVAR; ++l) tp3(M, M, M, K, K, K, c->AT, c->AT, c->AT, Ssp + l * nInt, res + l * nD, 1, w);
 }
 
 /* Trace of sol (5 x K^3) and its parametric gradients on face f of an
@@ -675,8 +832,11 @@ static void ismail_roe(const double *VL, const double *VR, const double n[3], do
 /* FluxesInternalFace (FluxesFace.cpp:220-343) with ViscousFluxesFace
    (:1290-1651) and PenAndSymFluxesSymmVar (:1152-1284). Inputs are traces
    with parametric gradients; outputs: flux (normal flux, L->R normal) and the
-   parametric symmetrising fluxes for L and R. */
-static void internal_face_flux(const orc_case *c, double lenScale,
+   parametric symmetrising fluxes for L and R. fn multiplies the unit normal
+   of mL (factNorm: +1 interior and max faces, -1 boundary min faces,
+   FluxesFace.cpp:105-106, :168-174); a boundary face passes its own metric
+   as mL and mR (FluxesFace.cpp:112-147, :191-210). */
+static void internal_face_flux(const orc_case *c, double fn, double lenScale,
                                const double *SL, double *GrL, double *GsL, double *GtL, const double *mL,
                                const double *SR, double *GrR, double *GsR, double *GtR, const double *mR,
                                double *flux, double *sxL, double *syL, double *szL,
@@ -705,7 +865,7 @@ static void internal_face_flux(const orc_case *c, double lenScale,
   for (int m = 0; m < n2; ++m) {
     double VL[5], VR[5], fl[5];
     for (int l = 0; l < 5; ++l) { VL[l] = SL[l * n2 + m]; VR[l] = SR[l * n2 + m]; }
-    const double nn[3] = {mL[0 * n2 + m], mL[1 * n2 + m], mL[2 * n2 + m]};
+    const double nn[3] = {fn * mL[0 * n2 + m], fn * mL[1 * n2 + m], fn * mL[2 * n2 + m]};
     const double wA = c->w2[m] * mL[3 * n2 + m];
     ismail_roe(VL, VR, nn, wA, fl);
     /* viscous flux on the averaged state */
@@ -786,14 +946,280 @@ static void internal_face_flux(const orc_case *c, double lenScale,
   }
 }
 
+/* Boundary state of a PC-NSCBC face (BCStandardCharacteristicSubfaceClass
+   and its outflow / static-inflow / total-inflow ComputeBoundaryState,
+   BoundaryState.cpp:996-1291, :1300-1568, :1579-1837, :1848-2241).
+   In: the interior trace SL [5][n2] and its parametric gradients
+   GL[0..2] (d/dxi, d/deta, d/dzeta, each [5][n2]).
+   Out: boundary entropy state SR and its parametric gradients GR[0..2]. */
+static void nscbc_state(const orc_case *c, int f, int e, const double *SL, double *const GL[3],
+                        double *SR, double *const GR[3])
+{
+  const int n2 = c->nInt2D, M = c->M, bc = c->prm.bc[f];
+  const double fn = (f % 2) ? 1.0 : -1.0;
+  const double gm1 = GAM - 1.0, ovgm1 = 1.0 / gm1, govgm1 = GAM * ovgm1;
+  const double *sm = c->sm[f] + (size_t) e * 16 * n2;
+  const int *ig = c->ig[f];
+  const double *dL[3] = {GL[ig[0]], GL[ig[1]], GL[ig[2]]};
+  double *dR[3] = {GR[ig[0]], GR[ig[1]], GR[ig[2]]};
+  const double *mI[3], *mJ[3], *mK[3];
+  for (int q = 0; q < 3; ++q) {
+    mI[q] = sm + (size_t) (4 + 3 * ig[0] + q) * n2;
+    mJ[q] = sm + (size_t) (4 + 3 * ig[1] + q) * n2;
+    mK[q] = sm + (size_t) (4 + 3 * ig[2] + q) * n2;
+  }
+  /* ConvertEntropyToPrimitive (BoundaryState.cpp:1002-1108): SR <- (rho,u,v,w,p),
+     dR[i] <- dQ/dV dL[i] */
+  for (int l = 0; l < n2; ++l) {
+    const double V0 = SL[l], V1 = SL[n2 + l], V2 = SL[2 * n2 + l], V3 = SL[3 * n2 + l], V4 = SL[4 * n2 + l];
+    const double V4Inv = 1.0 / V4;
+    const double u = -V4Inv * V1, v = -V4Inv * V2, w = -V4Inv * V3;
+    const double ek = 0.5 * (u * u + v * v + w * w);
+    const double sEnt = GAM - gm1 * (V0 - V4 * ek);
+    const double a2 = -GAM * V4Inv;
+    const double rho = pow(-V4 * exp(sEnt), -ovgm1);
+    const double p = a2 * rho / GAM;
+    const double povrho = -V4Inv;
+    const double E = ovgm1 * povrho + ek, H = E + povrho;
+    for (int i = 0; i < 3; ++i) {
+      const double *dV = dL[i];
+      double *dQ = dR[i];
+      const double d0 = dV[l], d1 = dV[n2 + l], d2 = dV[2 * n2 + l], d3 = dV[3 * n2 + l], d4 = dV[4 * n2 + l];
+      dQ[l] = rho * d0 + rho * u * d1 + rho * v * d2 + rho * w * d3 + rho * E * d4;
+      dQ[n2 + l] = povrho * d1 + povrho * u * d4;
+      dQ[2 * n2 + l] = povrho * d2 + povrho * v * d4;
+      dQ[3 * n2 + l] = povrho * d3 + povrho * w * d4;
+      dQ[4 * n2 + l] = p * d0 + p * u * d1 + p * v * d2 + p * w * d3 + p * H * d4;
+    }
+    SR[l] = rho; SR[n2 + l] = u; SR[2 * n2 + l] = v; SR[3 * n2 + l] = w; SR[4 * n2 + l] = p;
+  }
+  /* wave amplitudes, incoming ones replaced; reconstructed normal derivative
+     in dR[0], B = target - (dQ/dn - Q dell) in dR[1] */
+  const double dell = fn * c->dell0;
+  const int iPhi = fn > 0.0 ? 0 : 4, iPsi = iPhi == 0 ? 4 : 0;
+  const double Mach = c->machAvg[f];
+  for (int l = 0; l < n2; ++l) {
+    double LL[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const double rr[3] = {mI[0][l], mI[1][l], mI[2][l]};
+    const double magrr = sqrt(rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2]);
+    const double ovmagrr = 1.0 / magrr;
+    const double rn[3] = {rr[0] * ovmagrr, rr[1] * ovmagrr, rr[2] * ovmagrr};
+    const double rho = SR[l], u = SR[n2 + l], v = SR[2 * n2 + l], w = SR[3 * n2 + l], p = SR[4 * n2 + l];
+    const double drhodr = dR[0][l], dudr = dR[0][n2 + l], dvdr = dR[0][2 * n2 + l], dwdr = dR[0][3 * n2 + l],
+                 dpdr = dR[0][4 * n2 + l];
+    const double ovrho = 1.0 / rho, a2 = GAM * p * ovrho, a = sqrt(a2);
+    const double ova2 = 1.0 / a2, rhoa = rho * a, ovrhoa = 1.0 / rhoa, rhova = rho / a;
+    const double dsdr = drhodr - ova2 * dpdr;
+    const double dw1dr = rn[1] * dwdr - rn[2] * dvdr;
+    const double dw2dr = rn[2] * dudr - rn[0] * dwdr;
+    const double dw3dr = rn[0] * dvdr - rn[1] * dudr;
+    const double durdr = rn[0] * dudr + rn[1] * dvdr + rn[2] * dwdr;
+    double lmb1, lmb3;
+    if (bc == ORC_BC_OUTFLOW_NSCBC) {
+      /* BoundaryState.cpp:1318-1519 */
+      const double sign = iPhi == 0 ? -1.0 : 1.0;
+      const double Mavg2 = Mach * Mach;
+      const double kk = 0.5 * c->prm.outlet_sigma * (1.0 - Mavg2) / c->prm.outlet_len;
+      const double bl = c->prm.outlet_beta_l < 0.0 ? Mach : c->prm.outlet_beta_l;
+      const double bt = c->prm.outlet_beta_t < 0.0 ? Mach : c->prm.outlet_beta_t;
+      const double cbl = c->prm.outlet_one_minus_beta ? fmin(1.0, 1.0 - bl) : fmin(1.0, bl);
+      const double cbt = c->prm.outlet_one_minus_beta ? fmin(1.0, 1.0 - bt) : fmin(1.0, bt);
+      const double pInf = c->bcData[f][0];
+      const double ss[3] = {mJ[0][l], mJ[1][l], mJ[2][l]};
+      const double tt[3] = {mK[0][l], mK[1][l], mK[2][l]};
+      const double duds = dR[1][n2 + l], dudt = dR[2][n2 + l];
+      const double dvds = dR[1][2 * n2 + l], dvdt = dR[2][2 * n2 + l];
+      const double dwds = dR[1][3 * n2 + l], dwdt = dR[2][3 * n2 + l];
+      const double dpds = dR[1][4 * n2 + l], dpdt = dR[2][4 * n2 + l];
+      const double ur = u * rr[0] + v * rr[1] + w * rr[2];
+      const double us = u * ss[0] + v * ss[1] + w * ss[2];
+      const double ut = u * tt[0] + v * tt[1] + w * tt[2];
+      const double arr = a * magrr;
+      const double ars = a * (rn[0] * ss[0] + rn[1] * ss[1] + rn[2] * ss[2]);
+      const double art = a * (rn[0] * tt[0] + rn[1] * tt[1] + rn[2] * tt[2]);
+      lmb1 = ur - arr; lmb3 = ur + arr;
+      const double durds = rn[0] * duds + rn[1] * dvds + rn[2] * dwds;
+      const double durdt = rn[0] * dudt + rn[1] * dvdt + rn[2] * dwdt;
+      LL[0] = 0.5 * lmb1 * (ovrhoa * dpdr - durdr);
+      LL[1] = (rn[0] * dsdr - dw1dr);
+      LL[2] = (rn[1] * dsdr - dw2dr);
+      LL[3] = (rn[2] * dsdr - dw3dr);
+      LL[4] = 0.5 * lmb3 * (ovrhoa * dpdr + durdr);
+      const double LT = 0.5 * ovrhoa * (us + sign * ars) * dpds + sign * 0.5 * us * durds
+                      + 0.5 * ovrhoa * (ut + sign * art) * dpdt + sign * 0.5 * ut * durdt;
+      const double TT = -0.5 * a * (ss[0] * duds + ss[1] * dvds + ss[2] * dwds
+                                  + tt[0] * dudt + tt[1] * dvdt + tt[2] * dwdt);
+      LL[iPhi] = kk * ovrho * magrr * (p - pInf) - cbl * LT + cbt * TT;
+    } else if (bc == ORC_BC_INFLOW_STATIC_NSCBC) {
+      /* BoundaryState.cpp:1596-1783 */
+      const double sign = fn > 0.0 ? -1.0 : 1.0;
+      const double Mavg2 = Mach * Mach;
+      const double ka = sign * 0.5 * c->prm.inlet_sigma * (1.0 - Mavg2) / c->prm.inlet_len;
+      const double ku = c->prm.inlet_sigma / c->prm.inlet_len;
+      const double rhoInf = c->bcData[f][0], uInf = c->bcData[f][1], vInf = c->bcData[f][2], wInf = c->bcData[f][3];
+      const double ur = u * rr[0] + v * rr[1] + w * rr[2];
+      const double arr = a * magrr;
+      lmb1 = ur - arr; lmb3 = ur + arr;
+      const double lmb2 = ur, ovlmb2 = 1.0 / lmb2;
+      if (c->prm.inlet_incoming) {
+        LL[0] = 0.5 * lmb1 * (ovrhoa * dpdr - durdr);
+        LL[4] = 0.5 * lmb3 * (ovrhoa * dpdr + durdr);
+      }
+      const double drho = rho - rhoInf, du = u - uInf, dv = v - vInf, dw = w - wInf;
+      const double kaol = ku * a * ovlmb2;
+      LL[iPhi] = ka * a * (rr[0] * du + rr[1] * dv + rr[2] * dw);
+      LL[1] = kaol * (rr[2] * dv - rr[1] * dw + rr[0] * drho);
+      LL[2] = kaol * (rr[0] * dw - rr[2] * du + rr[1] * drho);
+      LL[3] = kaol * (rr[1] * du - rr[0] * dv + rr[2] * drho);
+    } else {
+      /* total-condition inflow, BoundaryState.cpp:1866-2184 */
+      const double ovrg = 1.0 / RGAS, ovcp = 1.0 / (GAM * RGAS / gm1);
+      const double kk = -c->prm.inlet_sigma / c->prm.inlet_len;
+      const double PtInf = c->bcData[f][0], TtInf = c->bcData[f][1];
+      const double ZtInf = c->bcData[f][3], ZpInf = c->bcData[f][4];
+      const double ova = 1.0 / a;
+      const double T = p * ovrho * ovrg;
+      const double ur = u * rn[0] + v * rn[1] + w * rn[2];
+      const double w1 = rn[1] * w - rn[2] * v, w2 = rn[2] * u - rn[0] * w, w3 = rn[0] * v - rn[1] * u;
+      lmb1 = ur - a; lmb3 = ur + a;
+      const double lmb2 = ur, ovlmb2 = 1.0 / lmb2;
+      if (c->prm.inlet_incoming) {
+        LL[0] = 0.5 * lmb1 * (ovrhoa * dpdr - durdr);
+        LL[4] = 0.5 * lmb3 * (ovrhoa * dpdr + durdr);
+      }
+      const double ek = 0.5 * (u * u + v * v + w * w);
+      const double magU2 = 2.0 * ek, magU = sqrt(magU2);
+      const double M2 = magU2 / a2;
+      const double ff = 1.0 + 0.5 * gm1 * M2;
+      const double Pt = p * pow(ff, govgm1), Tt = T * ff;
+      const double Zt = v / magU, Zp = w / magU;
+      const double ka = kk * a;
+      const double X1 = ka * (Pt - PtInf), X2 = ka * (Tt - TtInf), X3 = ka * (Zt - ZtInf), X4 = ka * (Zp - ZpInf);
+      const double PtovTt = Pt / Tt, ovrPtovTt = ovrg * PtovTt, Ttovrho = Tt * ovrho;
+      const double govaPt = GAM * ova * Pt, ekovrho = ek * ovrho, gm1ekova = gm1 * ek * ova;
+      const double gm1ovaTt = gm1 * ova * Tt, ovmagU2 = 1.0 / magU2;
+      const double gp = gm1ekova + ur, gmn = gm1ekova - ur;
+      const double f1 = w1 - rn[0] * ekovrho, f2 = w2 - rn[1] * ekovrho, f3 = w3 - rn[2] * ekovrho;
+      const double A1 = gp * ovrPtovTt - govaPt, A2 = f1 * ovrPtovTt, A3 = f2 * ovrPtovTt, A4 = f3 * ovrPtovTt;
+      const double A5 = gmn * ovrPtovTt - govaPt;
+      const double B1 = gp * ovcp - gm1ovaTt, B2 = f1 * ovcp + rn[0] * Ttovrho, B3 = f2 * ovcp + rn[1] * Ttovrho;
+      const double B4 = f3 * ovcp + rn[2] * Ttovrho, B5 = gmn * ovcp - gm1ovaTt;
+      const double C1 = (rn[1] * magU - ur * Zt) * ovmagU2, C2 = -(rn[2] * magU + w1 * Zt) * ovmagU2;
+      const double C3 = (-w2 * Zt) * ovmagU2, C4 = (rn[0] * magU - w3 * Zt) * ovmagU2, C5 = -C1;
+      const double D1 = (rn[2] * magU - ur * Zp) * ovmagU2, D2 = (rn[1] * magU - w1 * Zp) * ovmagU2;
+      const double D3 = -(rn[0] * magU + w2 * Zp) * ovmagU2, D4 = (-w3 * Zp) * ovmagU2, D5 = -D1;
+      const double Y1 = LL[iPsi] * A1, Y2 = LL[iPsi] * B1, Y3 = LL[iPsi] * C1, Y4 = LL[iPsi] * D1;
+      const double ovA2 = 1.0 / A2;
+      const double E0 = ovA2 * X1, E3 = -ovA2 * A3, E4 = -ovA2 * A4, E5 = -ovA2 * A5, P2 = -ovA2 * Y1;
+      const double ovdF = 1.0 / (D3 + D2 * E3);
+      const double F0 = ovdF * (X4 - D2 * E0), F4 = -ovdF * (D4 + D2 * E4), F5 = -ovdF * (D5 + D2 * E5);
+      const double P3 = -ovdF * (Y4 + D2 * P2);
+      const double ovdG = 1.0 / (C4 + C2 * E4 + C3 * F4 + C2 * E3 * F4);
+      const double G0 = ovdG * (X3 - C2 * E0 - C3 * F0 - C2 * E3 * F0);
+      const double G5 = -ovdG * (C5 + C2 * E5 + C3 * F5 + C2 * E3 * F5);
+      const double P4 = -ovdG * (Y3 + C2 * P2 + C3 * P3 + C2 * E3 * P3);
+      const double ovdH = 1.0 / (B5 + B2 * E5 + B3 * F5 + B4 * G5 + B2 * E3 * F5 + B2 * E4 * G5 + B3 * F4 * G5
+                                 + B2 * E3 * F4 * G5);
+      const double H0 = ovdH * (X2 - B2 * E0 - B3 * F0 - B4 * G0 - B2 * E3 * F0 - B2 * E4 * G0 - B3 * F4 * G0
+                                - B2 * E3 * F4 * G0);
+      const double P5 = -ovdH * (Y2 + B2 * P2 + B3 * P3 + B4 * P4 + B2 * E3 * P3 + B2 * E4 * P4 + B3 * F4 * P4
+                                 + B2 * E3 * F4 * P4);
+      LL[iPhi] = H0 + P5;
+      LL[3] = G0 + G5 * LL[iPhi] + P4;
+      LL[2] = F0 + F4 * LL[3] + F5 * LL[iPhi] + P3;
+      LL[1] = E0 + E3 * LL[2] + E4 * LL[3] + E5 * LL[iPhi] + P2;
+      LL[1] *= ovlmb2; LL[2] *= ovlmb2; LL[3] *= ovlmb2;
+    }
+    const double ovlmb1 = 1.0 / lmb1, ovlmb3 = 1.0 / lmb3;
+    const double t0 = ovlmb1 * rhova * LL[0] + rn[0] * LL[1] + rn[1] * LL[2] + rn[2] * LL[3] + ovlmb3 * rhova * LL[4];
+    const double t1 = -ovlmb1 * rn[0] * LL[0] - rn[2] * LL[2] + rn[1] * LL[3] + ovlmb3 * rn[0] * LL[4];
+    const double t2 = -ovlmb1 * rn[1] * LL[0] + rn[2] * LL[1] - rn[0] * LL[3] + ovlmb3 * rn[1] * LL[4];
+    const double t3 = -ovlmb1 * rn[2] * LL[0] - rn[1] * LL[1] + rn[0] * LL[2] + ovlmb3 * rn[2] * LL[4];
+    const double t4 = ovlmb1 * rhoa * LL[0] + ovlmb3 * rhoa * LL[4];
+    dR[0][l] = t0; dR[0][n2 + l] = t1; dR[0][2 * n2 + l] = t2; dR[0][3 * n2 + l] = t3; dR[0][4 * n2 + l] = t4;
+    dR[1][l] = t0 - (drhodr - rho * dell);
+    dR[1][n2 + l] = t1 - (dudr - u * dell);
+    dR[1][2 * n2 + l] = t2 - (dvdr - v * dell);
+    dR[1][3 * n2 + l] = t3 - (dwdr - w * dell);
+    dR[1][4 * n2 + l] = t4 - (dpdr - p * dell);
+  }
+  /* ReconstructBoundaryStateAndGradient (BoundaryState.cpp:1114-1291) */
+  for (int k = 0; k < 5; ++k)
+    for (int i = 0; i < n2; ++i) {
+      double t = 0.0;
+      for (int j = 0; j < n2; ++j) t += c->PC[i * n2 + j] * dR[1][k * n2 + j];
+      SR[k * n2 + i] = t;
+    }
+  if (fn < 0.0) for (int i = 0; i < 5 * n2; ++i) SR[i] = -SR[i];
+  for (int l = 0; l < n2; ++l) {
+    const double rho = SR[l], u = SR[n2 + l], v = SR[2 * n2 + l], w = SR[3 * n2 + l], p = SR[4 * n2 + l];
+    const double sEnt = log(p / pow(rho, GAM));
+    const double dr = dR[0][l], du = dR[0][n2 + l], dv = dR[0][2 * n2 + l], dw = dR[0][3 * n2 + l], dp = dR[0][4 * n2 + l];
+    const double ovrho = 1.0 / rho, ovp = 1.0 / p, rovp = rho * ovp;
+    const double ek = 0.5 * (u * u + v * v + w * w);
+    const double W0 = -rovp * ek + (GAM - sEnt) * ovgm1, W1 = rovp * u, W2 = rovp * v, W3 = rovp * w, W4 = -rovp;
+    SR[l] = W0; SR[n2 + l] = W1; SR[2 * n2 + l] = W2; SR[3 * n2 + l] = W3; SR[4 * n2 + l] = W4;
+    const double q0 = govgm1 * ovrho - ovp * ek, q4 = ovp * (rovp * ek - ovgm1);
+    dR[0][l] = q0 * dr + (-W1) * du + (-W2) * dv + (-W3) * dw + q4 * dp;
+    dR[0][n2 + l] = u * ovp * dr + rovp * du + (-ovp * W1) * dp;
+    dR[0][2 * n2 + l] = v * ovp * dr + rovp * dv + (-ovp * W2) * dp;
+    dR[0][3 * n2 + l] = w * ovp * dr + rovp * dw + (-ovp * W3) * dp;
+    dR[0][4 * n2 + l] = -ovp * dr + ovp * rovp * dp;
+  }
+  /* transverse derivatives of the reconstructed state: along the slow and the
+     fast face index; the reference stores the slow one in the first
+     transverse slot (bug_compat), the correct slot is the second */
+  double *dSlow = c->prm.bug_compat ? dR[1] : dR[2];
+  double *dFast = c->prm.bug_compat ? dR[2] : dR[1];
+  for (int v = 0; v < 5; ++v)
+    for (int i = 0; i < M; ++i)
+      for (int j = 0; j < M; ++j) {
+        double ds = 0.0, dt = 0.0;
+        for (int k = 0; k < M; ++k) {
+          ds += c->dLq[i * M + k] * SR[v * n2 + k * M + j];
+          dt += c->dLq[j * M + k] * SR[v * n2 + i * M + k];
+        }
+        dSlow[v * n2 + i * M + j] = ds;
+        dFast[v * n2 + i * M + j] = dt;
+      }
+}
+
+/* Physical (NSCBC) face f of element e: FluxesBoundaryFace
+   (FluxesFace.cpp:16-214) and the lift of the boundary branch of
+   ElementClass::{I,J,K}FaceResiduals (ElementClass.cpp:2751-2850). */
+static int boundary_face(orc_case *c, int f, int e, double *buf)
+{
+  const int n2 = c->nInt2D, nD = c->nDOFs;
+  if (!is_nscbc(c->prm.bc[f])) return fail("unsupported physical boundary");
+  double *SL = buf, *G0 = SL + 5 * n2, *G1 = G0 + 5 * n2, *G2 = G1 + 5 * n2;
+  double *SR = G2 + 5 * n2, *H0 = SR + 5 * n2, *H1 = H0 + 5 * n2, *H2 = H1 + 5 * n2;
+  double *flux = H2 + 5 * n2, *sxL = flux + 5 * n2, *syL = sxL + 5 * n2, *szL = syL + 5 * n2;
+  double *sxR = szL + 5 * n2, *syR = sxR + 5 * n2, *szR = syR + 5 * n2;
+  face_trace(c, f, c->sol + (size_t) e * NVAR * nD, SL, G0, G1, G2);
+  double *GL[3] = {G0, G1, G2}, *GR[3] = {H0, H1, H2};
+  nscbc_state(c, f, e, SL, GL, SR, GR);
+  const double *m = c->sm[f] + (size_t) e * 16 * n2;
+  const double fn = (f % 2) ? 1.0 : -1.0;
+  internal_face_flux(c, fn, c->len[(size_t) e * 4 + f / 2], SL, G0, G1, G2, m, SR, H0, H1, H2, m,
+                     flux, sxL, syL, szL, sxR, syR, szR);
+  face_lift(c, f, flux, sxL, syL, szL, c->res + (size_t) e * NVAR * nD);
+  return 0;
+}
+
 /* ElementClass::{I,J,K}FaceResiduals for an interior (incl. periodic) min face
    of element e (ElementClass.cpp:2582-2852 and the J/K analogues). */
 static int face_residual(orc_case *c, int dir, int e)
 {
   const int n2 = c->nInt2D, nD = c->nDOFs;
   const int L = c->nbr[e * 6 + 2 * dir];
-  if (L < 0) return fail("physical boundary faces are not part of this oracle");
   double *buf = W(c) + 80 * (size_t) n2 + 64;
+  if (L < 0) {
+    double *rmin = c->resMin[dir] + (size_t) e * NVAR * nD;
+    for (int i = 0; i < NVAR * nD; ++i) rmin[i] = 0.0;
+    if (boundary_face(c, 2 * dir, e, buf)) return -1;
+    if (c->nbr[e * 6 + 2 * dir + 1] < 0 && boundary_face(c, 2 * dir + 1, e, buf)) return -1;
+    return 0;
+  }
   double *SL = buf, *GrL = SL + 5 * n2, *GsL = GrL + 5 * n2, *GtL = GsL + 5 * n2;
   double *SR = GtL + 5 * n2, *GrR = SR + 5 * n2, *GsR = GrR + 5 * n2, *GtR = GsR + 5 * n2;
   double *flux = GtR + 5 * n2, *sxL = flux + 5 * n2, *syL = sxL + 5 * n2, *szL = syL + 5 * n2;
@@ -803,12 +1229,13 @@ static int face_residual(orc_case *c, int dir, int e)
   const double lenScale = fmin(c->len[(size_t) e * 4 + dir], c->len[(size_t) L * 4 + dir]);
   const double *mL = c->sm[2 * dir + 1] + (size_t) L * 16 * n2;
   const double *mR = c->sm[2 * dir] + (size_t) e * 16 * n2;
-  internal_face_flux(c, lenScale, SL, GrL, GsL, GtL, mL, SR, GrR, GsR, GtR, mR, flux, sxL, syL, szL, sxR, syR, szR);
+  internal_face_flux(c, 1.0, lenScale, SL, GrL, GsL, GtL, mL, SR, GrR, GsR, GtR, mR, flux, sxL, syL, szL, sxR, syR, szR);
   double *rmin = c->resMin[dir] + (size_t) e * NVAR * nD;
   for (int i = 0; i < NVAR * nD; ++i) rmin[i] = 0.0;
   face_lift(c, 2 * dir + 1, flux, sxL, syL, szL, rmin);
   for (int i = 0; i < 5 * n2; ++i) flux[i] = -flux[i];
   face_lift(c, 2 * dir, flux, sxR, syR, szR, c->res + (size_t) e * NVAR * nD);
+  if (c->nbr[e * 6 + 2 * dir + 1] < 0 && boundary_face(c, 2 * dir + 1, e, buf)) return -1;
   return 0;
 }
 
@@ -885,6 +1312,177 @@ static int cg_entropy(orc_case *c, int e)
   return 0;
 }
 
+/* ------------------------------------------------------- case set-up */
+static int is_nscbc(int bc)
+{
+  return bc == ORC_BC_OUTFLOW_NSCBC || bc == ORC_BC_INFLOW_STATIC_NSCBC || bc == ORC_BC_INFLOW_TOTAL_NSCBC;
+}
+
+static int check_params(const orc_case *c)
+{
+  const orc_params *p = &c->prm;
+  for (int f = 0; f < 6; ++f) {
+    const int per = p->periodic[f / 2];
+    if (per && p->bc[f] != ORC_BC_PERIODIC) return fail("a periodic direction cannot carry a physical boundary");
+    if (!per && !is_nscbc(p->bc[f])) return fail("unsupported boundary condition on a non-periodic block face");
+  }
+  if (p->outlet_average_local)
+    return fail("\"Outlet use local (element) averaged Mach number\" is not supported");
+  if (p->cml && !p->sponge_layer) return fail("a characteristic matching layer needs the sponge layer");
+  if (p->sponge_layer) {
+    if (p->n_poly_grid != p->n_poly_sol) return fail("sponge layer needs equal grid and solution degree");
+    int any = 0;
+    for (int f = 0; f < 6; ++f) {
+      if (p->sponge_elems[f] < 0) return fail("Sponge layer cannot have negative number of elements");
+      any |= p->sponge_elems[f] > 0;
+    }
+    if (!any) return fail("No sponge layer elements have been detected in a sponge layer simulation.");
+  }
+  return 0;
+}
+
+/* InputParamClass::ComputeReferenceConditions (InputParamClass.cpp:605-758)
+   and the non-dimensionalisation of the prescribed data
+   (SubfaceClasses.cpp:1304-1313, :1385-1400, :1453-1475). */
+static void reference_conditions(orc_case *c)
+{
+  const orc_params *q = &c->prm;
+  const double muDim = 1.716e-5;
+  double p = -1.0, pTot = -1.0, rho = -1.0, TTot = -1.0;
+  for (int f = 0; f < 6; ++f) {
+    if (q->periodic[f / 2]) continue;
+    if (q->bc[f] == ORC_BC_OUTFLOW_NSCBC) p = q->bc_data[f][0];
+    if (q->bc[f] == ORC_BC_INFLOW_STATIC_NSCBC) rho = q->bc_data[f][0];
+    if (q->bc[f] == ORC_BC_INFLOW_TOTAL_NSCBC) { pTot = q->bc_data[f][0]; TTot = q->bc_data[f][1]; }
+  }
+  c->pRef = -1.0;
+  if (p > 0.0) c->pRef = p; else if (pTot > 0.0) c->pRef = pTot;
+  c->rhoRef = -1.0;
+  if (rho > 0.0) c->rhoRef = rho;
+  if (c->rhoRef < 0.0 && c->pRef > 0.0 && TTot > 0.0) c->rhoRef = c->pRef / (RGAS * TTot);
+  if (c->pRef < 0.0 || c->rhoRef < 0.0) {
+    const double vInf = q->mach * sqrt(GAM * RGAS * q->temp_free_stream);
+    c->rhoRef = q->reynolds * muDim / (q->reynolds_length * vInf);
+    c->pRef = c->rhoRef * RGAS * q->temp_free_stream;
+  }
+  c->uRef = sqrt(c->pRef / c->rhoRef);
+  c->mu = muDim * c->uRef / c->pRef;
+  const double TRef = c->pRef / c->rhoRef;
+  for (int f = 0; f < 6; ++f) {
+    const double *d = q->bc_data[f];
+    double *o = c->bcData[f];
+    for (int l = 0; l < 5; ++l) o[l] = 0.0;
+    if (q->periodic[f / 2]) continue;
+    if (q->bc[f] == ORC_BC_OUTFLOW_NSCBC) o[0] = d[0] / c->pRef;
+    else if (q->bc[f] == ORC_BC_INFLOW_STATIC_NSCBC) {
+      o[0] = d[0] / c->rhoRef; o[1] = d[1] / c->uRef; o[2] = d[2] / c->uRef; o[3] = d[3] / c->uRef;
+    } else if (q->bc[f] == ORC_BC_INFLOW_TOTAL_NSCBC) {
+      const double inv = 1.0 / sqrt(d[2] * d[2] + d[3] * d[3] + d[4] * d[4]);
+      o[0] = d[0] / c->pRef; o[1] = d[1] / TRef; o[2] = d[2] * inv; o[3] = d[3] * inv; o[4] = d[4] * inv;
+    }
+  }
+}
+
+/* SolverClass::InitNSCBC (SolverClass.cpp:2447-2497): boundary areas
+   (ElementClass::ComputeSurfaceAreaBoundary, ElementClass.cpp:303-327) and
+   the face orientation of every NSCBC face (ConfigureParamNSCBC,
+   SubfaceClasses.cpp:1325-1349, :1363-1384, with IdentifyBoundaryIndices,
+   BoundaryState.cpp:829-925, evaluated at integration point 0). */
+static int setup_nscbc(orc_case *c)
+{
+  const int n2 = c->nInt2D;
+  c->nscbc = 0;
+  for (int f = 0; f < 6; ++f) {
+    c->faceArea[f] = 0.0; c->machAvg[f] = 0.0;
+    if (c->prm.periodic[f / 2]) continue;
+    c->nscbc = 1;
+    int first = -1;
+    for (int e = 0; e < c->nElem; ++e) {
+      if (c->nbr[e * 6 + f] >= 0) continue;
+      if (first < 0) first = e;
+      const double *sm = c->sm[f] + (size_t) e * 16 * n2;
+      for (int l = 0; l < n2; ++l) c->faceArea[f] += c->w2[l] * sm[3 * n2 + l];
+    }
+    const double *sm = c->sm[f] + (size_t) first * 16 * n2;
+    int found = -1;
+    for (int i = 0; i < 3; ++i) {
+      const double *g = sm + (size_t) (4 + 3 * i) * n2;
+      const double len = sqrt(g[0] * g[0] + g[n2] * g[n2] + g[2 * n2] * g[2 * n2]);
+      const double ct = (sm[0] * g[0] + sm[n2] * g[n2] + sm[2 * n2] * g[2 * n2]) / len;
+      if (fabs(ct - 1.0) < 1e-10) found = i;
+    }
+    if (found < 0) return fail("Could not identify relevant boundary indices!");
+    c->ig[f][0] = found;
+    c->ig[f][1] = found == 0 ? 1 : 0;
+    c->ig[f][2] = found == 2 ? 1 : 2;
+  }
+  return 0;
+}
+
+/* SolverClass::InitSpongeLayer and SetUpSpongeLayer{I,J,K}{MIN,MAX}
+   (SpongeLayer.cpp:13-662) with ElementClass::ConfigureSpongeLayer
+   (SpongeLayer.cpp:844-935): layer membership by global element index,
+   damping function sigma_f * (|x_d - x_0| / width)^alpha_f at the
+   integration points, CML scale 1/sigma_f. */
+static int setup_sponge(orc_case *c, const double *grid)
+{
+  if (!c->prm.sponge_layer) return 0;
+  const int K = c->K, nInt = c->nInt, nG = c->nGrid;
+  const int n[3] = {c->ni, c->nj, c->nk};
+  c->spongeIdx = (int *) malloc(sizeof(int) * c->nElem);
+  c->nSponge = 0;
+  for (int e = 0; e < c->nElem; ++e) {
+    const int idx[3] = {e % c->ni, (e / c->ni) % c->nj, e / (c->ni * c->nj)};
+    int flag = 0;
+    for (int d = 0; d < 3; ++d) {
+      if (idx[d] < c->prm.sponge_elems[2 * d]) flag |= 1 << (2 * d);
+      if (idx[d] >= n[d] - c->prm.sponge_elems[2 * d + 1]) flag |= 1 << (2 * d + 1);
+    }
+    c->spongeIdx[e] = flag ? c->nSponge++ : -1;
+  }
+  c->spongeFlag = (unsigned char *) calloc(c->nSponge ? c->nSponge : 1, 1);
+  c->dampFn = dalloc((size_t) c->nSponge * 3 * nInt);
+  c->dampState = dalloc((size_t) c->nSponge * 5 * nInt);
+  c->matchScale = dalloc((size_t) c->nSponge * 3);
+  double *xyz = dalloc((size_t) 3 * nInt);
+  for (int e = 0; e < c->nElem; ++e) {
+    const int s = c->spongeIdx[e];
+    if (s < 0) continue;
+    const int idx[3] = {e % c->ni, (e / c->ni) % c->nj, e / (c->ni * c->nj)};
+    for (int d = 0; d < 3; ++d) {
+      if (idx[d] < c->prm.sponge_elems[2 * d]) c->spongeFlag[s] |= 1 << (2 * d);
+      if (idx[d] >= n[d] - c->prm.sponge_elems[2 * d + 1]) c->spongeFlag[s] |= 1 << (2 * d + 1);
+    }
+  }
+  for (int f = 0; f < 6; ++f) {
+    if (c->prm.sponge_elems[f] <= 0) continue;
+    const int d = f / 2, bit = 1 << f;
+    const int stride = d == 0 ? 1 : d == 1 ? K : K * K;
+    double lo = 1e300, hi = -1e300;
+    for (int e = 0; e < c->nElem; ++e) {
+      const int s = c->spongeIdx[e];
+      if (s < 0 || !(c->spongeFlag[s] & bit)) continue;
+      const double *g = grid + ((size_t) e * 3 + d) * nG;
+      lo = fmin(lo, g[0]); hi = fmax(hi, g[c->P * stride]);
+    }
+    const double ov = 1.0 / (hi - lo), x0 = (f % 2) ? lo : hi;
+    const double ss = c->prm.damping_constant[f], aa = c->prm.damping_exponent[f];
+    for (int e = 0; e < c->nElem; ++e) {
+      const int s = c->spongeIdx[e];
+      if (s < 0 || !(c->spongeFlag[s] & bit)) continue;
+      tp3(K, K, K, c->M, c->M, c->M, c->Lq, c->Lq, c->Lq, grid + ((size_t) e * 3 + d) * nG, xyz, 0, W(c));
+      double *fn = c->dampFn + ((size_t) s * 3 + d) * nInt;
+      for (int l = 0; l < nInt; ++l) {
+        if ((f % 2) ? xyz[l] < x0 : xyz[l] > x0) { free(xyz); return fail("Inconsistent sponge layer location."); }
+        fn[l] = ss * pow(ov * fabs(xyz[l] - x0), aa);
+      }
+      if (c->prm.cml) c->matchScale[(size_t) s * 3 + d] = fabs(ss) < 1.0e-4 ? 1.0 : 1.0 / ss;
+    }
+  }
+  free(xyz);
+  return 0;
+}
+
 /* ----------------------------------------------------------------- API */
 orc_case *orc_create(const orc_params *prm, const double *elem_grid)
 {
@@ -896,13 +1494,8 @@ orc_case *orc_create(const orc_params *prm, const double *elem_grid)
   c->P = prm->n_poly_sol; c->K = c->P + 1; c->M = prm->n_poly_int_rule / 2 + 1;
   c->Kg = prm->n_poly_grid + 1; c->nGrid = c->Kg * c->Kg * c->Kg;
   c->nDOFs = c->K * c->K * c->K; c->nInt = c->M * c->M * c->M; c->nInt2D = c->M * c->M;
-  /* reference conditions for an all-periodic block (InputParamClass.cpp:605-758) */
-  const double muDim = 1.716e-5;
-  const double vInf = prm->mach * sqrt(GAM * RGAS * prm->temp_free_stream);
-  c->rhoRef = prm->reynolds * muDim / (prm->reynolds_length * vInf);
-  c->pRef = c->rhoRef * RGAS * prm->temp_free_stream;
-  c->uRef = sqrt(c->pRef / c->rhoRef);
-  c->mu = muDim * c->uRef / c->pRef;
+  if (check_params(c)) { free(c); return NULL; }
+  reference_conditions(c);
   size_t big = (size_t) c->nInt; if ((size_t) c->nGrid > big) big = c->nGrid;
   c->work_size = 200 * big + 400 * (size_t) c->nInt2D + 4096;
   c->nthreads = omp_get_max_threads();
@@ -911,6 +1504,7 @@ orc_case *orc_create(const orc_params *prm, const double *elem_grid)
   standard_element(c);
   if (metrics(c, elem_grid)) { orc_destroy(c); return NULL; }
   connectivity(c);
+  if (setup_nscbc(c) || setup_sponge(c, elem_grid)) { orc_destroy(c); return NULL; }
   const size_t ns = (size_t) c->nElem * NVAR * c->nDOFs;
   c->sol = dalloc(ns); c->solOld = dalloc(ns); c->res = dalloc(ns);
   for (int d = 0; d < 3; ++d) c->resMin[d] = dalloc(ns);
@@ -925,11 +1519,12 @@ void orc_destroy(orc_case *c)
   double *ptrs[] = {c->rInt, c->wInt, c->rDOF, c->rGrid, c->A, c->D, c->AT, c->DT, c->LD, c->VI,
                     c->FMin, c->FMax, c->dFMin, c->dFMax, c->w3, c->w2, c->basisCenter, c->vm,
                     c->sm[0], c->sm[1], c->sm[2], c->sm[3], c->sm[4], c->sm[5], c->len, c->aveNorm,
-                    c->sol, c->solOld, c->res, c->resMin[0], c->resMin[1], c->resMin[2], c->dUdV};
+                    c->sol, c->solOld, c->res, c->resMin[0], c->resMin[1], c->resMin[2], c->dUdV,
+                    c->Lq, c->dLq, c->PC, c->dampFn, c->dampState, c->matchScale};
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
   if (c->works) for (int t = 0; t < c->nthreads; ++t) free(c->works[t]);
   free(c->works);
-  free(c->nbr); free(c->cgIt); free(c);
+  free(c->nbr); free(c->cgIt); free(c->spongeIdx); free(c->spongeFlag); free(c);
 }
 
 void orc_sizes(const orc_case *c, int *n_elem, int *n_dofs1d, int *n_int1d)
@@ -1011,6 +1606,10 @@ int orc_residual(orc_case *c, int mon, double md[8])
 {
   double mach2 = 0.0;
   int err = 0;
+  if (c->prm.sponge_layer && !c->dampStateSet) return fail("sponge layer damping state not set");
+  /* the reference calls AverageDataNSCBC before every Residual
+     (VCP3D_CurviLinear.cpp:89, SolverClass.cpp:2405) */
+  if (c->nscbc) nscbc_average(c);
 #pragma omp parallel for schedule(static) reduction(max : mach2)
   for (int e = 0; e < c->nElem; ++e) {
     double m2 = 0.0;
@@ -1129,6 +1728,14 @@ void orc_get_setup(const orc_case *c, int which, double *out)
   else if (which >= 1 && which <= 6) memcpy(out, c->sm[which - 1], sizeof(double) * E * 16 * c->nInt2D);
   else if (which == 7) memcpy(out, c->len, sizeof(double) * E * 4);
   else if (which == 8) for (size_t i = 0; i < 6 * E; ++i) out[i] = c->nbr[i];
+  else if (which == 9 || which == 10) {
+    const size_t nv = (which == 9 ? 3 : 5) * (size_t) c->nInt;
+    for (size_t e = 0; e < E; ++e) {
+      const int s = c->spongeIdx ? c->spongeIdx[e] : -1;
+      for (size_t i = 0; i < nv; ++i)
+        out[e * nv + i] = s < 0 ? 0.0 : (which == 9 ? c->dampFn : c->dampState)[(size_t) s * nv + i];
+    }
+  }
 }
 
 void orc_get_table(const orc_case *c, int which, double *out)
@@ -1141,7 +1748,83 @@ void orc_get_table(const orc_case *c, int which, double *out)
     case 3: for (int q = 0; q < K; ++q) for (int p = 0; p < M; ++p) out[q * M + p] = c->D[p * K + q]; break;
     case 4: memcpy(out, c->rDOF, sizeof(double) * K); break;
     case 5: memcpy(out, c->rGrid, sizeof(double) * c->Kg); break;
+    case 6: memcpy(out, c->PC, sizeof(double) * M * M * M * M); break;
+    case 7: memcpy(out, c->dLq, sizeof(double) * M * M); break;
+    case 8: out[0] = c->dell0; break;
   }
+}
+
+/* SolverClass::ReadAveragedSolutionSpongeLayer + ElementClass::
+   InitializeDampingState (SpongeLayer.cpp:665-838, :940-997): averaged
+   nodal primitive state interpolated to the integration points, made
+   non-dimensional and converted to conservative variables. */
+int orc_set_damping_state(orc_case *c, const double *nodal_prim)
+{
+  if (!c->prm.sponge_layer) return fail("no sponge layer in this case");
+  const int K = c->K, M = c->M, nD = c->nDOFs, nInt = c->nInt;
+  double *node = dalloc(5 * (size_t) nD), *q = dalloc(5 * (size_t) nInt);
+  const double ovgm1 = 1.0 / (GAM - 1.0);
+  for (int e = 0; e < c->nElem; ++e) {
+    const int s = c->spongeIdx[e];
+    if (s < 0) continue;
+    for (int d = 0; d < nD; ++d)
+      for (int l = 0; l < 5; ++l) node[l * nD + d] = nodal_prim[((size_t) e * nD + d) * 5 + l];
+    for (int l = 0; l < 5; ++l) tp3(K, K, K, M, M, M, c->Lq, c->Lq, c->Lq, node + l * nD, q + l * nInt, 0, W(c));
+    double *ds = c->dampState + (size_t) s * 5 * nInt;
+    for (int m = 0; m < nInt; ++m) {
+      const double rho = q[m] / c->rhoRef, u = q[nInt + m] / c->uRef, v = q[2 * nInt + m] / c->uRef,
+                   w = q[3 * nInt + m] / c->uRef, p = q[4 * nInt + m] / c->pRef;
+      ds[m] = rho; ds[nInt + m] = rho * u; ds[2 * nInt + m] = rho * v; ds[3 * nInt + m] = rho * w;
+      ds[4 * nInt + m] = p * ovgm1 + 0.5 * rho * (u * u + v * v + w * w);
+    }
+  }
+  free(node); free(q);
+  c->dampStateSet = 1;
+  c->resValid = 0;
+  return 0;
+}
+
+/* SolverClass::AverageDataNSCBC (SolverClass.cpp:2502-2581) with
+   ElementClass::AverageMachNSCBC (ElementClass.cpp:365-479) and
+   WeightedMachElement (BoundaryState.cpp:935-994): area-weighted mean of
+   the face-normal Mach number u.grad(xi_n)/(|grad(xi_n)| a) over each NSCBC
+   block face. */
+static void nscbc_average(orc_case *c)
+{
+  const int n2 = c->nInt2D, nD = c->nDOFs;
+  for (int f = 0; f < 6; ++f) {
+    double sum = 0.0;
+    if (!c->prm.periodic[f / 2]) {
+#pragma omp parallel reduction(+ : sum)
+      {
+        double *S = dalloc(5 * (size_t) n2), *G = dalloc(15 * (size_t) n2);
+#pragma omp for schedule(static)
+        for (int e = 0; e < c->nElem; ++e) {
+          if (c->nbr[e * 6 + f] >= 0) continue;
+          face_trace(c, f, c->sol + (size_t) e * NVAR * nD, S, G, G + 5 * n2, G + 10 * n2);
+          const double *sm = c->sm[f] + (size_t) e * 16 * n2;
+          const double *rr = sm + (size_t) (4 + 3 * c->ig[f][0]) * n2;
+          for (int l = 0; l < n2; ++l) {
+            const double rx = rr[l], ry = rr[n2 + l], rz = rr[2 * n2 + l];
+            const double mag = sqrt(rx * rx + ry * ry + rz * rz);
+            const double V4Inv = 1.0 / S[4 * n2 + l];
+            const double u = -V4Inv * S[n2 + l], v = -V4Inv * S[2 * n2 + l], w = -V4Inv * S[3 * n2 + l];
+            const double a = sqrt(-GAM * V4Inv);
+            const double urr = (u * rx + v * ry + w * rz) / mag;
+            sum += c->w2[l] * sm[3 * n2 + l] * (urr / a);
+          }
+        }
+        free(S); free(G);
+      }
+    }
+    c->machAvg[f] = fabs(c->faceArea[f]) > 1e-30 ? sum / c->faceArea[f] : sum;
+  }
+}
+
+void orc_nscbc_average(orc_case *c, double mach_avg[6])
+{
+  nscbc_average(c);
+  for (int f = 0; f < 6; ++f) mach_avg[f] = c->machAvg[f];
 }
 
 /* ------------------------------------------------------------ test hooks */

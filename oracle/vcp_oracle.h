@@ -7,7 +7,9 @@
  * never links, loads or calls it.
  *
  * Scope: entropy working variables, Ismail-Roe flux, no SGS model, no wall
- * model, structured single block with periodic (translation) faces.
+ * model, structured single block whose faces are periodic (translation) or
+ * PC-NSCBC in/outflow, with optional sponge layer and characteristic
+ * matching layer.
  * Each function cites the reference file:line it restates; paths are
  * relative to VCP3D_CurviLinear/.
  */
@@ -29,8 +31,30 @@ typedef struct {
   double theta_sym, rel_penalty, cfl, dt_synchr;
   int    bug_compat;       /* 1: metric indexed by variable index, as the
                               reference does (ElementClass.cpp:3865-3873,
-                              FluxesFace.cpp:272-282, :325-332) */
+                              FluxesFace.cpp:272-282, :325-332), and the
+                              transverse derivatives of the reconstructed
+                              NSCBC state stored in swapped slots
+                              (BoundaryState.cpp:1261-1287) */
+  /* Physical block faces, used where periodic[face/2] == 0; faces are
+     iMin, iMax, jMin, jMax, kMin, kMax (one subface per block face). */
+  int    bc[6];            /* ORC_BC_* */
+  double bc_data[6][5];    /* dimensional prescribed data of the face:
+                              outflow {p}; static inflow {rho, u, v, w};
+                              total inflow {p_t, T_t, dir_x, dir_y, dir_z} */
+  /* "NSCBC tuning parameters" (InputParamClass.cpp:406-424) */
+  double outlet_len, outlet_sigma, outlet_beta_l, outlet_beta_t;
+  int    outlet_one_minus_beta, outlet_average_local;
+  double inlet_len, inlet_sigma;
+  int    inlet_incoming;
+  /* "Sponge layer", "Characteristic matching layer" and the "Sponge layer
+     information" block (InputParamClass.cpp:1196-1330) */
+  int    sponge_layer, cml;
+  int    sponge_elems[6];
+  double damping_constant[6], damping_exponent[6];
 } orc_params;
+
+enum { ORC_BC_PERIODIC = 0, ORC_BC_OUTFLOW_NSCBC = 1,
+       ORC_BC_INFLOW_STATIC_NSCBC = 2, ORC_BC_INFLOW_TOTAL_NSCBC = 3 };
 
 typedef struct orc_case orc_case;
 
@@ -64,12 +88,26 @@ int  orc_time_steps(orc_case *c, double *dt_elem);
 /* CG iteration count of the last mass-matrix solve, per element. */
 void orc_cg_iterations(const orc_case *c, int *iters);
 
+/* Sponge-layer target: the averaged nodal primitive state (rho, u, v, w, p),
+   dimensional, [nElem][nDOFs][5] (SolverClass::ReadAveragedSolutionSpongeLayer,
+   SpongeLayer.cpp:665-838). Required before the first residual when
+   sponge_layer is set. */
+int  orc_set_damping_state(orc_case *c, const double *nodal_prim);
+/* SolverClass::AverageDataNSCBC: boundary-averaged Mach number of the six
+   block faces for the current state (0 where no NSCBC face). */
+void orc_nscbc_average(orc_case *c, double mach_avg[6]);
+
 /* Setup data for tests. which: 0 volume metric [nElem][13][nInt],
    1..6 face metric (iMin,iMax,jMin,jMax,kMin,kMax) [nElem][16][nInt2D],
-   7 length scales+volume [nElem][4] (I,J,K,vol), 8 neighbours [nElem][6] (as double). */
+   7 length scales+volume [nElem][4] (I,J,K,vol), 8 neighbours [nElem][6] (as double),
+   9 damping functions [nElem][3][nInt] (0 outside the sponge),
+   10 damping state [nElem][5][nInt] (0 outside the sponge). */
 void orc_get_setup(const orc_case *c, int which, double *out);
 /* 1D tables: which 0 Gauss points, 1 weights, 2 legendre at int [K][M],
-   3 derivative [K][M], 4 nodal DOF locations [K], 5 grid DOF locations. */
+   3 derivative [K][M], 4 nodal DOF locations [K], 5 grid DOF locations,
+   6 PC-NSCBC matrix [M^2][M^2], 7 Lagrange derivative on the Gauss points
+   [M][M] (row: point, column: basis), 8 the NSCBC coefficient
+   -l'_0(r_0) of the nodal DOF basis (1 value). */
 void orc_get_table(const orc_case *c, int which, double *out);
 
 /* Test hooks: one sum-factorised tensor application (out = Oz Oy Ox in),

@@ -1,0 +1,154 @@
+"""The C oracle's open-boundary terms (PC-NSCBC in/outflow, boundary Mach
+averaging, sponge layer, characteristic matching layer) against the
+reference's own outputs (tests/golden/ob_*.npz, written by
+tests/golden/make_golden.py from the reference build) and against what the
+mathematics fixes: the polynomial-correction matrix as a scaled orthogonal
+projector, the Lagrange derivative on the Gauss points, the LGL end
+coefficient, the damping function's closed form and free-stream
+preservation of the whole open-box residual.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from tests.golden.make_golden import load
+from vcp3d_open_amd.cases import GAMMA, OpenBoxCase, lgl_nodes
+
+SOL_TOL = 1e-12
+RES_TOL = 1e-9
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.fixture(scope="module")
+def orc(oracle_lib):
+    return oracle_lib
+
+
+def start(orc, case):
+    o = orc.Oracle(case)
+    o.set_restart(case.initial_conservative())
+    if case.sponge_elems > 0:
+        o.set_damping_state(case.damping_primitive())
+    return o
+
+
+@pytest.mark.parametrize("name", ["ob_static_p3", "ob_total_p3", "ob_total_p2", "ob_static_p4", "ob_static_p5"])
+def test_open_box_against_reference(orc, name):
+    case, g = load(name)
+    o = start(orc, case)
+    np.testing.assert_allclose(list(o.reference().values()), g["reference"], rtol=1e-14)
+    o.residual()
+    R = o.residual_array()
+    assert rel(R, g["res0"]) < RES_TOL
+    # the boundary elements on their own (inflow i = 0, outflow i = n-1)
+    i = np.arange(case.n_elems) % case.n[0]
+    for sel in (i == 0, i == case.n[0] - 1):
+        assert rel(R[sel], g["res0"][sel]) < RES_TOL
+    steps = max([int(k.split("_")[1]) for k in g if k.startswith("sol_")] or [0])
+    for _ in range(steps):
+        o.sync_step()
+    if steps:
+        assert rel(o.nodal(), g[f"sol_{steps}"]) < SOL_TOL
+        assert rel(o.residual_array(), g[f"res_{steps}"]) < RES_TOL
+
+
+@pytest.mark.slow
+def test_open_box_full_size_subset(orc):
+    """48 x 24 x 24, P = 4, sponge + CML: every 97th element and the
+    per-variable norms of R(U_0)."""
+    case, g = load("ob_n48_p4")
+    o = start(orc, case)
+    o.residual()
+    R = o.residual_array()
+    assert rel(R[g["elements"]], g["res0"]) < RES_TOL
+    np.testing.assert_allclose(np.sqrt((R ** 2).sum(axis=(0, 2))), g["res0_norms"], rtol=RES_TOL)
+
+
+@pytest.mark.parametrize("p,rule", [(2, 6), (3, 9), (4, 12), (5, 15)])
+def test_pc_matrix_is_scaled_projector(orc, p, rule):
+    """PC = (1/c) Q Q^T, Q an orthonormal basis of the degree-P polynomials
+    on the M x M Gauss points (any basis spans the same space; Legendre is
+    used here), c = -l'_0(r_0) = P(P+1)/4 for LGL nodes."""
+    case = OpenBoxCase(n=(3, 2, 2), p=p, rule=rule, sponge_elems=0)
+    o = orc.Oracle(case)
+    M = rule // 2 + 1
+    x, _ = np.polynomial.legendre.leggauss(M)
+    V = np.polynomial.legendre.legvander(x, p)                   # [M][P+1]
+    V2 = np.einsum("ia,jb->ijab", V, V).reshape(M * M, (p + 1) ** 2)
+    Q, _ = np.linalg.qr(V2)
+    c = o.table(8)[0]
+    assert abs(c - p * (p + 1) / 4.0) < 1e-12 * c
+    np.testing.assert_allclose(o.table(6), (Q @ Q.T) / c, atol=1e-12)
+
+
+@pytest.mark.parametrize("rule", [6, 9, 12, 15])
+def test_gauss_lagrange_derivative(orc, rule):
+    case = OpenBoxCase(n=(3, 2, 2), p=2, rule=rule, sponge_elems=0)
+    o = orc.Oracle(case)
+    M = rule // 2 + 1
+    x, _ = np.polynomial.legendre.leggauss(M)
+    rng = np.random.default_rng(rule)
+    coef = rng.standard_normal(M)                                # degree M-1
+    f = np.polynomial.polynomial.polyval(x, coef)
+    df = np.polynomial.polynomial.polyval(x, np.polynomial.polynomial.polyder(coef))
+    np.testing.assert_allclose(o.table(7) @ f, df, atol=1e-11 * np.abs(df).max())
+
+
+def test_damping_function_closed_form(orc):
+    """sigma ((x - x0)/width)^alpha at the integration points of the outlet
+    layer (x0 = start of the layer), zero elsewhere."""
+    case = OpenBoxCase(n=(6, 2, 2), p=3, sponge_elems=2, damping_constant=7.0, damping_exponent=1.5)
+    o = orc.Oracle(case)
+    M = o.M
+    xg, _ = np.polynomial.legendre.leggauss(M)
+    h = case.lengths[0] / case.n[0]
+    x0, width = case.lengths[0] - 2 * h, 2 * h
+    fn = o.setup(9)
+    for e in range(case.n_elems):
+        i = e % case.n[0]
+        xq = np.tile(i * h + 0.5 * (xg + 1.0) * h, M * M)         # i fastest
+        expect = 7.0 * (np.abs(xq - x0) / width) ** 1.5 if i >= 4 else np.zeros_like(xq)
+        np.testing.assert_allclose(fn[e, 0], expect, rtol=1e-13, atol=1e-13)
+        assert not fn[e, 1:].any()
+
+
+def test_uniform_state_mach_average_and_damping_state(orc):
+    case = OpenBoxCase(n=(6, 2, 2), p=3, sponge_elems=2)
+    o = orc.Oracle(case)
+    q = case.damping_primitive()
+    cons = np.stack([q[..., 0], q[..., 0] * q[..., 1], 0 * q[..., 0], 0 * q[..., 0],
+                     q[..., 4] / (GAMMA - 1) + 0.5 * q[..., 0] * q[..., 1] ** 2], axis=-1)
+    o.set_restart(cons)
+    o.set_damping_state(q)
+    np.testing.assert_allclose(o.nscbc_average(), [case.mach, case.mach, 0, 0, 0, 0], rtol=1e-12, atol=0)
+    ds = o.setup(10)
+    ref = o.reference()
+    sp = (np.arange(case.n_elems) % case.n[0]) >= 4
+    rho = case.rho_inf / ref["rho_ref"]
+    u = case.u_inf / ref["u_ref"]
+    rE = case.p_inf / ref["p_ref"] / (GAMMA - 1) + 0.5 * rho * u * u
+    np.testing.assert_allclose(ds[sp][:, 0], rho, rtol=1e-13)
+    np.testing.assert_allclose(ds[sp][:, 1], rho * u, rtol=1e-13)
+    np.testing.assert_allclose(ds[sp][:, 4], rE, rtol=1e-13)
+    assert not ds[~sp].any()
+
+
+@pytest.mark.parametrize("inflow", ["static", "total"])
+def test_open_box_free_stream_preservation(orc, inflow):
+    """The free stream that matches the prescribed data and the sponge target
+    is a steady state: every relaxation term vanishes and the reconstructed
+    boundary state equals the interior trace (the PC matrix reproduces
+    constants), so R = 0 to roundoff."""
+    case = OpenBoxCase(n=(6, 2, 2), p=3, inflow=inflow, sponge_elems=2)
+    o = orc.Oracle(case)
+    q = case.damping_primitive()
+    cons = np.stack([q[..., 0], q[..., 0] * q[..., 1], 0 * q[..., 0], 0 * q[..., 0],
+                     q[..., 4] / (GAMMA - 1) + 0.5 * q[..., 0] * q[..., 1] ** 2], axis=-1)
+    o.set_restart(cons)
+    o.set_damping_state(q)
+    o.residual()
+    assert np.abs(o.residual_array()).max() < 1e-10

@@ -213,6 +213,14 @@ class TGVCase:
     def subfaces(self):
         return periodic_subfaces(self.n_elem, (self.length,) * 3)
 
+    def description(self) -> dict:
+        """Fields of the case description (vcp_case_desc); omitted ones are 0."""
+        return dict(n_elem=self.n_elem, n_poly_grid=self.p, n_poly_sol=self.p,
+                    n_poly_int_rule=self.rule, dof_location=2, periodic=(1, 1, 1),
+                    mach=self.mach, reynolds=self.reynolds, reynolds_length=1.0,
+                    temp_free_stream=self.temp, theta_sym=self.theta_sym,
+                    rel_penalty=self.rel_penalty, cfl=self.cfl, dt_synchr=self.dt_synchr)
+
     def write_inputs(self, directory: str) -> str:
         """Write grid, restart and parameter file; returns the parameter path."""
         os.makedirs(os.path.join(directory, "out"), exist_ok=True)
@@ -221,4 +229,276 @@ class TGVCase:
                               self.nodal_coords(), self.initial_conservative())
         path = os.path.join(directory, "param.cfg")
         vio.write_param_file(path, self.params(), self.subfaces())
+        return path
+
+
+# ------------------------------------------------------------------------
+# Open box with characteristic in- and outflow
+# ------------------------------------------------------------------------
+BC_PERIODIC = 0
+BC_OUTFLOW_NSCBC = 1
+BC_INFLOW_STATIC_NSCBC = 2
+BC_INFLOW_TOTAL_NSCBC = 3
+_BC_KEYWORD = {BC_OUTFLOW_NSCBC: "BCOutflowNSCBC", BC_INFLOW_STATIC_NSCBC: "BCInflowStaticNSCBC",
+               BC_INFLOW_TOTAL_NSCBC: "BCInflowTotalNSCBC"}
+
+
+@dataclass
+class OpenBoxCase:
+    """Box [0, Lx] x [0, Ly] x [0, Lz] of uniform hexahedra, PC-NSCBC inflow
+    at iMin and outflow at iMax, periodic in j and k, optional sponge layer
+    with characteristic matching layer in front of the outlet.
+
+    Free stream: Mach `mach` along +x at `temp`, `p_inf`. Initial state: an
+    isentropic vortex (axis along z, strength eps = peak swirl speed / U,
+    radius R) convected by the free stream plus a Gaussian pressure pulse
+    p' = amp p_inf exp(-ln 2 |x - x_p|^2 / w^2) (rho' = p'/a^2), centres drawn
+    from `seed`. The sponge target is the free stream.
+    Reference conditions follow the prescribed boundary data: pRef = outlet
+    pressure; rhoRef = inlet density, or pRef / (R T_t) for a total-condition
+    inlet (InputParamClass.cpp:605-758).
+    """
+    n: tuple = (12, 6, 6)
+    lengths: tuple = (2.0, 1.0, 1.0)
+    p: int = 3
+    rule: int | None = None
+    mach: float = 0.5
+    temp: float = 300.0
+    p_inf: float = 101325.0
+    inflow: str = "static"           # "static" or "total"
+    sponge_elems: int = 2            # layers at the outlet (0: no sponge)
+    cml: bool = True
+    damping_constant: float = 20.0
+    damping_exponent: float = 2.0
+    outlet_len: float = 1.0
+    outlet_sigma: float = 0.25
+    outlet_beta_l: float = -1.0      # < 0: the boundary-averaged Mach number
+    outlet_beta_t: float = -1.0
+    outlet_one_minus_beta: bool = True
+    inlet_len: float = 1.0
+    inlet_sigma: float = 0.25
+    inlet_incoming: bool = True
+    vortex_strength: float = 0.5
+    vortex_radius: float = 0.05      # fraction of Ly
+    pulse_amp: float = 1.0e-3
+    pulse_width: float = 0.05        # fraction of Ly
+    seed: int = 0
+    n_steps: int = 1
+    dt_synchr: float | None = None
+    cfl: float = 2.0
+    theta_sym: float = 1.0
+    rel_penalty: float = 1.0
+    reynolds: float = 1.0e6          # written to the parameter file only
+    name: str = "openbox"
+
+    def __post_init__(self):
+        if self.rule is None:
+            self.rule = 3 * self.p if self.p <= 4 else {5: 15, 6: 16}.get(self.p, 3 * self.p)
+        self.n = tuple(int(v) for v in self.n)
+        a_inf = math.sqrt(GAMMA * R_GAS * self.temp)
+        self.u_inf = self.mach * a_inf
+        self.rho_inf = self.p_inf / (R_GAS * self.temp)
+        f = 1.0 + 0.5 * (GAMMA - 1.0) * self.mach ** 2
+        self.pt_inf = self.p_inf * f ** (GAMMA / (GAMMA - 1.0))
+        self.tt_inf = self.temp * f
+        rng = np.random.default_rng(self.seed)
+        lx, ly, lz = self.lengths
+        self.vortex_centre = (lx * rng.uniform(0.25, 0.45), ly * rng.uniform(0.4, 0.6))
+        self.pulse_centre = (lx * rng.uniform(0.3, 0.6), ly * rng.uniform(0.3, 0.7), lz * rng.uniform(0.3, 0.7))
+        p_ref = self.p_inf
+        rho_ref = self.rho_inf if self.inflow == "static" else p_ref / (R_GAS * self.tt_inf)
+        u_ref = math.sqrt(p_ref / rho_ref)
+        self.ref = dict(p_ref=p_ref, rho_ref=rho_ref, u_ref=u_ref, mu_nondim=MU_DIM * u_ref / p_ref,
+                        v_inf=self.u_inf)
+        if self.dt_synchr is None:
+            self.dt_synchr = 0.5 * self.stable_dt_estimate()
+
+    @property
+    def n_elem(self):
+        return self.n
+
+    @property
+    def n_elems(self):
+        return self.n[0] * self.n[1] * self.n[2]
+
+    @property
+    def n_dofs(self):
+        return (self.p + 1) ** 3
+
+    def stable_dt_estimate(self) -> float:
+        f1 = [2.0, 6.0, 12.0, 20.0, 28.0, 38.0, 50.0, 64.0, 80.0, 98.0]
+        h = min(L / n for L, n in zip(self.lengths, self.n))
+        a = math.sqrt(GAMMA * R_GAS * self.temp) * 1.1
+        v = self.u_inf * (1.0 + 1.5 * self.vortex_strength)
+        rad_inv = 3.0 * (v + a) / h
+        return self.cfl / (f1[self.p] * rad_inv)
+
+    # -------------------------------------------------------------- geometry
+    def grid_block(self) -> np.ndarray:
+        r = lgl_nodes(self.p + 1)
+        axes = []
+        for L, n in zip(self.lengths, self.n):
+            h = L / n
+            pts = [0.0]
+            for e in range(n):
+                for q in range(1, self.p + 1):
+                    pts.append(e * h + 0.5 * (r[q] + 1.0) * h)
+            x1 = np.array(pts)
+            x1[-1] = L
+            axes.append(x1)
+        z, y, x = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+        return np.stack([x, y, z])
+
+    def element_grid(self) -> np.ndarray:
+        return vio.block_to_elements(self.grid_block(), self.n_elem, self.p)
+
+    def nodal_coords(self) -> np.ndarray:
+        return np.transpose(self.element_grid(), (0, 2, 1)).copy()
+
+    # ----------------------------------------------------- initial condition
+    def initial_primitive(self, coords: np.ndarray | None = None) -> np.ndarray:
+        """Dimensional (rho, u, v, w, p) at the nodes, (nElem, nDOFs, 5)."""
+        if coords is None:
+            coords = self.nodal_coords()
+        lx, ly, lz = self.lengths
+        x, y, z = coords[..., 0], coords[..., 1], coords[..., 2]
+        xc, yc = self.vortex_centre
+        R = self.vortex_radius * ly
+        dx = x - xc
+        dy = (y - yc + 0.5 * ly) % ly - 0.5 * ly          # nearest periodic image
+        r2 = (dx * dx + dy * dy) / (R * R)
+        f = np.exp(0.5 * (1.0 - r2))
+        eps = self.vortex_strength
+        u = self.u_inf * (1.0 - eps * dy / R * f)
+        v = self.u_inf * eps * dx / R * f
+        w = np.zeros_like(u)
+        tr = 1.0 - 0.5 * (GAMMA - 1.0) * (self.mach * eps) ** 2 * f * f
+        rho = self.rho_inf * tr ** (1.0 / (GAMMA - 1.0))
+        p = self.p_inf * tr ** (GAMMA / (GAMMA - 1.0))
+        xp, yp, zp = self.pulse_centre
+        wd = self.pulse_width * ly
+        d2 = (x - xp) ** 2 + ((y - yp + 0.5 * ly) % ly - 0.5 * ly) ** 2 + ((z - zp + 0.5 * lz) % lz - 0.5 * lz) ** 2
+        dp = self.pulse_amp * self.p_inf * np.exp(-math.log(2.0) * d2 / (wd * wd))
+        a2 = GAMMA * self.p_inf / self.rho_inf
+        p = p + dp
+        rho = rho + dp / a2
+        return np.stack([rho, u, v, w, p], axis=-1)
+
+    def initial_conservative(self, coords: np.ndarray | None = None) -> np.ndarray:
+        q = self.initial_primitive(coords)
+        rho, u, v, w, p = (q[..., i] for i in range(5))
+        rhoE = p / (GAMMA - 1.0) + 0.5 * rho * (u * u + v * v + w * w)
+        return np.stack([rho, rho * u, rho * v, rho * w, rhoE], axis=-1)
+
+    def damping_primitive(self) -> np.ndarray:
+        """Sponge target (the averaged primitive state): the free stream."""
+        q = np.zeros((self.n_elems, self.n_dofs, 5))
+        q[..., 0] = self.rho_inf
+        q[..., 1] = self.u_inf
+        q[..., 4] = self.p_inf
+        return q
+
+    # ------------------------------------------------------ boundary data
+    def bc(self) -> list:
+        inflow = BC_INFLOW_STATIC_NSCBC if self.inflow == "static" else BC_INFLOW_TOTAL_NSCBC
+        return [inflow, BC_OUTFLOW_NSCBC, 0, 0, 0, 0]
+
+    def bc_data(self) -> list:
+        data = [[0.0] * 5 for _ in range(6)]
+        if self.inflow == "static":
+            data[0][:4] = [self.rho_inf, self.u_inf, 0.0, 0.0]
+        else:
+            data[0][:] = [self.pt_inf, self.tt_inf, 1.0, 0.0, 0.0]
+        data[1][0] = self.p_inf
+        return data
+
+    def description(self) -> dict:
+        d = dict(n_elem=self.n_elem, n_poly_grid=self.p, n_poly_sol=self.p,
+                 n_poly_int_rule=self.rule, dof_location=2, periodic=(0, 1, 1),
+                 mach=self.mach, reynolds=self.reynolds, reynolds_length=1.0,
+                 temp_free_stream=self.temp, theta_sym=self.theta_sym,
+                 rel_penalty=self.rel_penalty, cfl=self.cfl, dt_synchr=self.dt_synchr,
+                 bc=self.bc(), bc_data=self.bc_data(),
+                 outlet_len=self.outlet_len, outlet_sigma=self.outlet_sigma,
+                 outlet_beta_l=self.outlet_beta_l, outlet_beta_t=self.outlet_beta_t,
+                 outlet_one_minus_beta=int(self.outlet_one_minus_beta), outlet_average_local=0,
+                 inlet_len=self.inlet_len, inlet_sigma=self.inlet_sigma,
+                 inlet_incoming=int(self.inlet_incoming))
+        if self.sponge_elems > 0:
+            d.update(sponge_layer=1, cml=int(self.cml),
+                     sponge_elems=[0, self.sponge_elems, 0, 0, 0, 0],
+                     damping_constant=[self.damping_constant] * 6,
+                     damping_exponent=[self.damping_exponent] * 6)
+        return d
+
+    # ------------------------------------------------------ parameter file
+    def params(self, grid_file="grid.xyz", restart_file="restart.dat",
+               sol_file="out/Solution") -> dict:
+        prm = TGVCase.params(self, grid_file, restart_file, sol_file)
+        prm.update({
+            "Reynolds number": self.reynolds,
+            "Outlet characteristic length": self.outlet_len,
+            "Outlet normal relaxation (sigma)": self.outlet_sigma,
+            "Outlet use one minus beta transverse relaxation": "yes" if self.outlet_one_minus_beta else "no",
+            "Outlet use local (element) averaged Mach number": "no",
+            "Outlet coupled   transverse relaxation (beta_l)": self.outlet_beta_l,
+            "Outlet uncoupled transverse relaxation (beta_t)": self.outlet_beta_t,
+            "Inlet characteristic length": self.inlet_len,
+            "Inlet normal relaxation (sigma)": self.inlet_sigma,
+            "Inlet use outgoing wave amplitude": "yes" if self.inlet_incoming else "no",
+        })
+        if self.sponge_elems > 0:
+            prm["Averaged solution file name"] = "averaged.dat"
+            prm["Sponge layer"] = "yes"
+            prm["Characteristic matching layer"] = "yes" if self.cml else "no"
+        return prm
+
+    def subfaces(self):
+        ni, nj, nk = self.n
+        lx, ly, lz = self.lengths
+        I, J, K = ni + 1, nj + 1, nk + 1
+        fmt = lambda v: repr(float(v))
+        bc = self.bc()
+        return [
+            [f"{_BC_KEYWORD[bc[0]]} 1 1 1 {J} 1 {K}"],
+            [f"{_BC_KEYWORD[bc[1]]} {I} {I} 1 {J} 1 {K}"],
+            [f"Periodic1to1Trans 1 {I} 1 1 1 {K}   1 {I} {J} {J} 1 {K}   1 2 3   0.0 {fmt(ly)} 0.0"],
+            [f"Periodic1to1Trans 1 {I} {J} {J} 1 {K}   1 {I} 1 1 1 {K}   1 2 3   0.0 {fmt(-ly)} 0.0"],
+            [f"Periodic1to1Trans 1 {I} 1 {J} 1 1   1 {I} 1 {J} {K} {K}   1 2 3   0.0 0.0 {fmt(lz)}"],
+            [f"Periodic1to1Trans 1 {I} 1 {J} {K} {K}   1 {I} 1 {J} 1 1   1 2 3   0.0 0.0 {fmt(-lz)}"],
+        ]
+
+    def prescribed(self) -> list:
+        if self.inflow == "static":
+            inlet = ["1 1 1 Density VelocityX VelocityY VelocityZ",
+                     f"    {self.rho_inf!r} {self.u_inf!r} 0.0 0.0"]
+        else:
+            inlet = ["1 1 1 TotalPressure TotalTemperature VelocityDirX VelocityDirY VelocityDirZ",
+                     f"    {self.pt_inf!r} {self.tt_inf!r} 1.0 0.0 0.0"]
+        return inlet + ["2 1 1 Pressure", f"    {self.p_inf!r}"]
+
+    def sponge_block(self) -> str | None:
+        if self.sponge_elems <= 0:
+            return None
+        lines = ["Sponge layer information"]
+        for f in range(6):
+            nb = self.sponge_elems if f == 1 else 0
+            lines.append(f"{f + 1} {nb} {self.damping_constant!r} {self.damping_exponent!r}")
+        return "\n".join(lines)
+
+    def write_inputs(self, directory: str) -> str:
+        os.makedirs(os.path.join(directory, "out"), exist_ok=True)
+        coords = self.nodal_coords()
+        vio.write_plot3d(os.path.join(directory, "grid.xyz"), self.grid_block())
+        vio.write_su2_restart(os.path.join(directory, "restart.dat"), coords, self.initial_conservative(coords))
+        if self.sponge_elems > 0:
+            q = self.damping_primitive()
+            cons = np.zeros_like(q)
+            extra = np.concatenate([q, np.zeros(q.shape[:2] + (7,))], axis=2)
+            vio.write_su2_restart(os.path.join(directory, "averaged.dat"), coords, cons, extra=extra,
+                                  extra_names=["AvDensity", "AvVelocity_x", "AvVelocity_y", "AvVelocity_z",
+                                               "AvPressure", "AvVxVx", "AvVyVy", "AvVzVz", "AvVxVy",
+                                               "AvVxVz", "AvVyVz", "AvEddyVis"])
+        path = os.path.join(directory, "param.cfg")
+        vio.write_param_file(path, self.params(), self.subfaces(), self.prescribed(), self.sponge_block())
         return path

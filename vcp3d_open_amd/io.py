@@ -79,18 +79,26 @@ def block_to_elements(xyz: np.ndarray, n_elem, n_poly_grid: int) -> np.ndarray:
 
 # ------------------------------------------------------------ SU2 restart
 def write_su2_restart(path: str, coords: np.ndarray, cons: np.ndarray,
-                      time_step: int = 0) -> None:
+                      time_step: int = 0, extra: np.ndarray | None = None,
+                      extra_names: list | None = None) -> None:
     """Write a restart file.
 
     coords: (nElem, nDOFs, 3) nodal coordinates; cons: (nElem, nDOFs, 5)
     dimensional conservative variables; element order i fastest, then j, k.
+    extra: optional further columns (nElem, nDOFs, n), e.g. the averaged
+    primitive variables, velocity products and eddy viscosity of an averaged
+    solution file (SolverClass.hpp:748-809 reads columns 8..12).
     """
     n_elem, n_dofs, _ = cons.shape
     names = ["x", "y", "z", "Density", "Momentum_x", "Momentum_y",
              "Momentum_z", "Energy"]
-    buf = np.concatenate([coords, cons], axis=2).astype(np.float64)
+    cols = [coords, cons]
+    if extra is not None:
+        cols.append(extra)
+        names += list(extra_names) if extra_names else [f"Extra_{i}" for i in range(extra.shape[2])]
+    buf = np.concatenate(cols, axis=2).astype(np.float64)
     with open(path, "wb") as f:
-        f.write(struct.pack("<5i", SU2_MAGIC, 8, n_elem * n_dofs, 1, 8))
+        f.write(struct.pack("<5i", SU2_MAGIC, len(names), n_elem * n_dofs, 1, 8))
         for nm in names:
             f.write(nm.encode().ljust(NAME_LEN, b"\0"))
         f.write(buf.tobytes(order="C"))

@@ -48,8 +48,46 @@ typedef struct vcp_case_desc {
                                 the gradient conversion (metric of integration
                                 point l used for variable l,
                                 ElementClass.cpp:3865-3873, FluxesFace.cpp:
-                                272-282, :325-332); 0: use the point's metric  */
+                                272-282, :325-332) and its slot order of the
+                                transverse derivatives of a reconstructed
+                                NSCBC state (BoundaryState.cpp:1261-1287);
+                                0: use the point's metric and the matching
+                                slots                                          */
+  /* Physical block faces iMin, iMax, jMin, jMax, kMin, kMax (used where the
+     direction is not periodic; one subface per block face), from the
+     "Subface information" and "Prescribed boundary data" blocks
+     (InputParamClass.cpp:1441-1576, :1340-1398). */
+  int    bc[6];              /* VCP_BC_* */
+  double bc_data[6][5];      /* dimensional prescribed data of the face:
+                                outflow {Pressure}; static inflow {Density,
+                                VelocityX, VelocityY, VelocityZ}; total inflow
+                                {TotalPressure, TotalTemperature, VelocityDirX,
+                                VelocityDirY, VelocityDirZ}                      */
+  /* "NSCBC tuning parameters" (InputParamClass.cpp:406-424) */
+  double outlet_len;         /* "Outlet characteristic length"                  */
+  double outlet_sigma;       /* "Outlet normal relaxation (sigma)"              */
+  double outlet_beta_l;      /* "Outlet coupled transverse relaxation (beta_l)",
+                                < 0: the boundary-averaged Mach number         */
+  double outlet_beta_t;      /* "Outlet uncoupled transverse relaxation (beta_t)" */
+  int    outlet_one_minus_beta;  /* "Outlet use one minus beta transverse relaxation" */
+  int    outlet_average_local;   /* "Outlet use local (element) averaged Mach
+                                    number": must be 0 (the reference keeps one
+                                    value per subface object, written by every
+                                    element, BoundaryState.cpp:985-990)         */
+  double inlet_len;          /* "Inlet characteristic length"                   */
+  double inlet_sigma;        /* "Inlet normal relaxation (sigma)"               */
+  int    inlet_incoming;     /* "Inlet use outgoing wave amplitude"             */
+  /* "Sponge layer", "Characteristic matching layer" and the "Sponge layer
+     information" block (InputParamClass.cpp:428-432, :1196-1330) */
+  int    sponge_layer;
+  int    cml;
+  int    sponge_elems[6];    /* layer thickness in elements per block face      */
+  double damping_constant[6];
+  double damping_exponent[6];
 } vcp_case_desc;
+
+enum { VCP_BC_PERIODIC = 0, VCP_BC_OUTFLOW_NSCBC = 1, VCP_BC_INFLOW_STATIC_NSCBC = 2,
+       VCP_BC_INFLOW_TOTAL_NSCBC = 3 };
 
 typedef struct vcp_ctx vcp_ctx;
 
@@ -67,6 +105,17 @@ int vcp_reference(const vcp_ctx *ctx, double out[4]);
 /* SolverClass::ReadRestartSolution (SolverClass.cpp:3040-3376): nodal
    dimensional conservative -> modal entropy variables. */
 int vcp_set_restart(vcp_ctx *ctx, const double *nodal);
+/* Sponge-layer target state: the averaged nodal primitive variables
+   (rho, u, v, w, p), dimensional, [nElem][(P+1)^3][5]
+   (SolverClass::ReadAveragedSolutionSpongeLayer, SpongeLayer.cpp:665-838,
+   and ElementClass::InitializeDampingState, SpongeLayer.cpp:940-997).
+   Needed before the first residual of a case with a sponge layer. */
+int vcp_set_damping_state(vcp_ctx *ctx, const double *nodal_prim);
+/* SolverClass::AverageDataNSCBC (SolverClass.cpp:2502-2581): boundary-averaged
+   face-normal Mach number of the six block faces for the current state
+   (0 where the face has no NSCBC). vcp_residual and vcp_sync_step evaluate it
+   before every residual, as every reference call site of Residual does. */
+int vcp_nscbc_average(vcp_ctx *ctx, double mach_avg[6]);
 /* Direct access to the modal working variables (ElementClass::mSol). */
 int vcp_set_solution(vcp_ctx *ctx, const double *modal);
 int vcp_get_solution(vcp_ctx *ctx, double *modal);
@@ -95,7 +144,7 @@ int vcp_cg_iterations(vcp_ctx *ctx, int *iters);
 
 /* Measurement: times `reps` residual evaluations with HIP events recorded on
    the context's stream around each kernel; writes the mean milliseconds per
-   launch of [face kernel, volume+mass-matrix kernel] into ms[2]. */
+   launch of [boundary + face kernels, volume+mass-matrix kernel] into ms[2]. */
 int vcp_profile_residual(vcp_ctx *ctx, int reps, double ms[2]);
 
 /* Host-only set-up (no device calls), for checking the tables and metrics

@@ -70,3 +70,29 @@ def test_taylor_green_initial_state():
     # periodic grid: opposite faces coincide after the translation
     g = case.grid_block()
     np.testing.assert_allclose(g[0, :, :, -1] - g[0, :, :, 0], 2 * np.pi, rtol=1e-15)
+
+
+def test_open_box_host_setup_and_rejections():
+    """Open box: physical faces have no neighbour; invalid boundary and
+    sponge settings are refused by vcp_create before any device call."""
+    import ctypes as C
+    from vcp3d_open_amd.cases import OpenBoxCase
+    from vcp3d_open_amd.solver import desc_from_case, host_setup, load_library
+    case = OpenBoxCase(n=(4, 2, 3), p=3, sponge_elems=1)
+    nbr = host_setup(case)["nbr"]
+    i = np.arange(case.n_elems) % 4
+    assert np.all(nbr[i == 0, 0] == -1) and np.all(nbr[i == 3, 1] == -1)
+    assert np.all(nbr[:, 2:] >= 0) and np.all(nbr[i > 0, 0] >= 0)
+    L = load_library()
+    grid = np.ascontiguousarray(case.element_grid())
+    for field, value, msg in [("outlet_average_local", 1, "averaged Mach"),
+                              ("sponge_layer", 0, "sponge"),
+                              ("periodic", (1, 1, 1), "periodic direction")]:
+        d = desc_from_case(case)
+        if isinstance(value, tuple):
+            getattr(d, field)[:] = list(value)
+        else:
+            setattr(d, field, value)
+        h = C.c_void_p()
+        assert L.vcp_create(C.byref(d), grid, C.byref(h)) < 0
+        assert msg in L.vcp_last_error().decode()

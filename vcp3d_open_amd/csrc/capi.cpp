@@ -37,9 +37,11 @@ struct vcp_ctx {
   vcp::Tables tab;
   vcp::RefConditions rc{};
   vcp::DeviceCase dc;
+  vcp::Sponge sponge;
   hipStream_t stream = nullptr;
   std::vector<void *> allocs;
   bool res_valid = false;
+  bool damping_set = false;
   int time_step = 0;
 
   template <class T>
@@ -78,12 +80,38 @@ int set_scalar(vcp_ctx *c, int slot, double v)
   return 0;
 }
 
-// faces + volume/mass matrix, with the given stage control
+// open boundaries + faces + volume/mass matrix, with the given stage control
 int residual_pass(vcp_ctx *c, const vcp::StageCtl &ctl)
 {
+  if (c->prm.sponge_layer && !c->damping_set)
+    return fail("sponge layer damping state not set (vcp_set_damping_state)");
+  HIP_TRY(vcp::launch_boundary(c->dc, c->stream));
   HIP_TRY(vcp::launch_faces(c->dc, c->stream));
   HIP_TRY(vcp::launch_volume(c->dc, ctl, c->stream));
   return 0;
+}
+
+// vcp_case_desc -> CaseParams (field by field, same meaning).
+void params_from_desc(const vcp_case_desc *d, vcp::CaseParams &p)
+{
+  for (int i = 0; i < 3; ++i) { p.n_elem[i] = d->n_elem[i]; p.periodic[i] = d->periodic[i]; }
+  p.n_poly_grid = d->n_poly_grid; p.n_poly_sol = d->n_poly_sol; p.n_poly_int_rule = d->n_poly_int_rule;
+  p.dof_location = d->dof_location; p.mach = d->mach; p.reynolds = d->reynolds;
+  p.reynolds_length = d->reynolds_length; p.temp_free_stream = d->temp_free_stream;
+  p.theta_sym = d->theta_sym; p.rel_penalty = d->rel_penalty; p.cfl = d->cfl; p.dt_synchr = d->dt_synchr;
+  p.bug_compat = d->bug_compat;
+  for (int f = 0; f < 6; ++f) {
+    p.bc[f] = d->bc[f];
+    for (int q = 0; q < 5; ++q) p.bc_data[f][q] = d->bc_data[f][q];
+    p.sponge_elems[f] = d->sponge_elems[f];
+    p.damping_constant[f] = d->damping_constant[f];
+    p.damping_exponent[f] = d->damping_exponent[f];
+  }
+  p.outlet_len = d->outlet_len; p.outlet_sigma = d->outlet_sigma;
+  p.outlet_beta_l = d->outlet_beta_l; p.outlet_beta_t = d->outlet_beta_t;
+  p.outlet_one_minus_beta = d->outlet_one_minus_beta; p.outlet_average_local = d->outlet_average_local;
+  p.inlet_len = d->inlet_len; p.inlet_sigma = d->inlet_sigma; p.inlet_incoming = d->inlet_incoming;
+  p.sponge_layer = d->sponge_layer; p.cml = d->cml;
 }
 
 void fill_monitoring(double md[8], bool mon, double mach2)
@@ -104,19 +132,16 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
   if (!d || !elem_grid || !out) return fail("null argument");
   *out = nullptr;
   vcp::CaseParams p;
-  for (int i = 0; i < 3; ++i) { p.n_elem[i] = d->n_elem[i]; p.periodic[i] = d->periodic[i]; }
-  p.n_poly_grid = d->n_poly_grid; p.n_poly_sol = d->n_poly_sol; p.n_poly_int_rule = d->n_poly_int_rule;
-  p.dof_location = d->dof_location; p.mach = d->mach; p.reynolds = d->reynolds;
-  p.reynolds_length = d->reynolds_length; p.temp_free_stream = d->temp_free_stream;
-  p.theta_sym = d->theta_sym; p.rel_penalty = d->rel_penalty; p.cfl = d->cfl; p.dt_synchr = d->dt_synchr;
-  p.bug_compat = d->bug_compat;
+  params_from_desc(d, p);
   if (p.n_elem[0] <= 0 || p.n_elem[1] <= 0 || p.n_elem[2] <= 0) return fail("invalid element counts");
   if (p.n_poly_sol <= 0 || p.n_poly_sol > 9 || p.n_poly_grid <= 0 || p.n_poly_grid > 9)
     return fail("polynomial degrees must be in 1..9");
   if (p.n_poly_int_rule < 2 * p.n_poly_sol)
     return fail("\"Polynomial degree integration rule\" not accurate enough");
-  if (!p.periodic[0] || !p.periodic[1] || !p.periodic[2])
-    return fail("non-periodic block faces are not supported by this build");
+  {
+    const std::string why = vcp::check_case(p);
+    if (!why.empty()) return fail(why);
+  }
   const int K = p.n_poly_sol + 1, M = vcp::n_int1d(p.n_poly_int_rule);
   if (!vcp::supported_pair(K, M)) return fail("unsupported (nDOFs1D, nInt1D) pair");
 
@@ -127,6 +152,10 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
   const std::string err = vcp::build_geometry(p, c->tab, elem_grid, g);
   if (!err.empty()) { delete c; return fail(err); }
   c->rc = vcp::reference_conditions(p);
+  vcp::OpenBoundary ob;
+  std::string serr = vcp::setup_open_boundary(p, c->tab, g, c->rc, ob);
+  if (serr.empty()) serr = vcp::setup_sponge(p, c->tab, g, elem_grid, c->sponge);
+  if (!serr.empty()) { delete c; return fail(serr); }
 
   vcp::DeviceCase &dc = c->dc;
   const int nElem = p.n_elem[0] * p.n_elem[1] * p.n_elem[2];
@@ -146,7 +175,27 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
     dc.opF[0][q] = t.f_min[q]; dc.opF[1][q] = t.f_max[q]; dc.opF[2][q] = t.df_min[q]; dc.opF[3][q] = t.df_max[q];
   }
   for (int q = 0; q < M; ++q) dc.opW[q] = t.w_int[q];
-  dc.to.w3 = put(t.w3); dc.to.w2 = put(t.w2); dc.to.bc = put(t.basis_center); dc.to.total = (int) tb.size();
+  dc.to.w3 = put(t.w3); dc.to.w2 = put(t.w2); dc.to.bc = put(t.basis_center);
+  dc.to.P1 = put(t.P1); dc.to.dLq = put(t.dLq); dc.to.total = (int) tb.size();
+  // open boundaries: face list, own face metric, parameters
+  dc.nBFace = (int) ob.faces.size() / 2;
+  std::vector<double> bgeo((size_t) dc.nBFace * 13 * M * M);
+  for (int b = 0; b < dc.nBFace; ++b) {
+    const int e = ob.faces[2 * b], f = ob.faces[2 * b + 1];
+    std::memcpy(&bgeo[(size_t) b * 13 * M * M], &g.face[f][(size_t) e * 16 * M * M], sizeof(double) * 13 * M * M);
+  }
+  vcp::NscbcParams &nb = dc.nscbc;
+  nb.outlet_len = p.outlet_len; nb.outlet_sigma = p.outlet_sigma;
+  nb.outlet_beta_l = p.outlet_beta_l; nb.outlet_beta_t = p.outlet_beta_t;
+  nb.inlet_len = p.inlet_len; nb.inlet_sigma = p.inlet_sigma; nb.dell0 = t.dell0;
+  nb.outlet_one_minus_beta = p.outlet_one_minus_beta; nb.inlet_incoming = p.inlet_incoming;
+  for (int f = 0; f < 6; ++f) {
+    nb.bc[f] = p.periodic[f / 2] ? vcp::kBCPeriodic : p.bc[f];
+    for (int q = 0; q < 5; ++q) nb.data[f][q] = ob.data[f][q];
+    nb.invArea[f] = ob.area[f] > 0.0 ? 1.0 / ob.area[f] : 0.0;
+  }
+  for (int f = 0; f < 7; ++f) nb.begin[f] = ob.begin[f];
+  dc.nSponge = c->sponge.n; dc.cml = p.cml;
 
   // device layouts of the geometry
   const int nInt = dc.nInt, n2 = dc.nInt2D;
@@ -187,6 +236,26 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
   if (he == hipSuccess) he = c->alloc(&dc.flags, 4);
   if (he == hipSuccess) he = c->alloc(&dc.cgIter, (size_t) nElem);
   if (he == hipSuccess) he = c->alloc(&dc.dtElem, (size_t) nElem);
+  if (he == hipSuccess && dc.nBFace) {
+    he = c->alloc(&dc.bface, ob.faces.size());
+    if (he == hipSuccess) he = c->alloc(&dc.bgeo, bgeo.size());
+    if (he == hipSuccess) he = c->alloc(&dc.machPart, (size_t) dc.nBFace);
+    if (he == hipSuccess) he = hipMemcpy(dc.bface, ob.faces.data(), ob.faces.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(dc.bgeo, bgeo.data(), bgeo.size() * sizeof(double), hipMemcpyHostToDevice);
+  }
+  if (he == hipSuccess && dc.nSponge) {
+    const vcp::Sponge &sp = c->sponge;
+    const size_t ns5 = (size_t) dc.nSponge * 5 * nInt;
+    he = c->alloc(&dc.spIdx, sp.idx.size());
+    if (he == hipSuccess) he = c->alloc(&dc.spCoef, sp.coef.size());
+    if (he == hipSuccess) he = c->alloc(&dc.dampFn, sp.fn.size());
+    if (he == hipSuccess) he = c->alloc(&dc.dampState, ns5);
+    if (he == hipSuccess) he = c->alloc(&dc.srcBuf, ns5);
+    if (he == hipSuccess) he = hipMemcpy(dc.spIdx, sp.idx.data(), sp.idx.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(dc.spCoef, sp.coef.data(), sp.coef.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(dc.dampFn, sp.fn.data(), sp.fn.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemset(dc.dampState, 0, ns5 * sizeof(double));
+  }
   if (he == hipSuccess) he = hipMemcpy(dc.tables, tb.data(), tb.size() * sizeof(double), hipMemcpyHostToDevice);
   if (he == hipSuccess) he = hipMemcpy(dc.vgeo, vgeo.data(), vgeo.size() * sizeof(double), hipMemcpyHostToDevice);
   if (he == hipSuccess) he = hipMemcpy(dc.fgeo, fgeo.data(), fgeo.size() * sizeof(double), hipMemcpyHostToDevice);
@@ -210,9 +279,7 @@ int vcp_host_setup(const vcp_case_desc *d, const double *elem_grid, double *vol,
 {
   if (!d || !elem_grid) return fail("null argument");
   vcp::CaseParams p;
-  for (int i = 0; i < 3; ++i) { p.n_elem[i] = d->n_elem[i]; p.periodic[i] = d->periodic[i]; }
-  p.n_poly_grid = d->n_poly_grid; p.n_poly_sol = d->n_poly_sol; p.n_poly_int_rule = d->n_poly_int_rule;
-  p.dof_location = d->dof_location;
+  params_from_desc(d, p);
   if (p.n_elem[0] <= 0 || p.n_elem[1] <= 0 || p.n_elem[2] <= 0) return fail("invalid element counts");
   if (p.n_poly_sol <= 0 || p.n_poly_sol > 9 || p.n_poly_grid <= 0 || p.n_poly_grid > 9)
     return fail("polynomial degrees must be in 1..9");
@@ -261,6 +328,41 @@ int vcp_set_restart(vcp_ctx *c, const double *nodal)
   for (int e = 0; e < c->dc.nElem; ++e)
     vcp::nodal_cons_to_modal_entropy(c->tab, c->rc, nodal + (size_t) e * nD * 5, &modal[(size_t) e * 5 * nD]);
   return vcp_set_solution(c, modal.data());
+}
+
+int vcp_set_damping_state(vcp_ctx *c, const double *nodal_prim)
+{
+  if (!c || !nodal_prim) return fail("null argument");
+  if (!c->prm.sponge_layer) return fail("no sponge layer in this case");
+  const vcp::Sponge &sp = c->sponge;
+  const int nD = c->dc.nDOF, nInt = c->dc.nInt;
+  std::vector<double> ds((size_t) sp.n * 5 * nInt);
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < c->dc.nElem; ++e)
+    if (sp.idx[e] >= 0)
+      vcp::damping_state(c->tab, c->rc, nodal_prim + (size_t) e * nD * 5, &ds[(size_t) sp.idx[e] * 5 * nInt]);
+  HIP_TRY(hipMemcpyAsync(c->dc.dampState, ds.data(), ds.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->damping_set = true;
+  c->res_valid = false;
+  return 0;
+}
+
+int vcp_nscbc_average(vcp_ctx *c, double mach_avg[6])
+{
+  if (!c || !mach_avg) return fail("null argument");
+  for (int f = 0; f < 6; ++f) mach_avg[f] = 0.0;
+  if (!c->dc.nBFace) return 0;
+  HIP_TRY(vcp::launch_boundary(c->dc, c->stream));
+  std::vector<double> part(c->dc.nBFace);
+  HIP_TRY(hipMemcpyAsync(part.data(), c->dc.machPart, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int f = 0; f < 6; ++f) {
+    double sum = 0.0;
+    for (int b = c->dc.nscbc.begin[f]; b < c->dc.nscbc.begin[f + 1]; ++b) sum += part[b];
+    mach_avg[f] = sum * c->dc.nscbc.invArea[f];
+  }
+  return 0;
 }
 
 int vcp_set_solution(vcp_ctx *c, const double *modal)
@@ -384,6 +486,7 @@ int vcp_profile_residual(vcp_ctx *c, int reps, double ms[2])
   vcp::StageCtl ctl{1, 0, 0.0, 0.0, 0};
   for (int r = 0; r < reps; ++r) {
     HIP_TRY(hipEventRecord(ev[3 * r], c->stream));
+    HIP_TRY(vcp::launch_boundary(c->dc, c->stream));
     HIP_TRY(vcp::launch_faces(c->dc, c->stream));
     HIP_TRY(hipEventRecord(ev[3 * r + 1], c->stream));
     HIP_TRY(vcp::launch_volume(c->dc, ctl, c->stream));

@@ -7,12 +7,21 @@
 //   volume_kernel        volume residual, gather of the six face slots,
 //                        per-element entropy mass-matrix PCG, optional
 //                        in-place RK stage update
+//   mach_kernel          boundary-averaged Mach number partial sums (NSCBC)
+//   nscbc_kernel         PC-NSCBC boundary faces: characteristic boundary
+//                        state, polynomial-correction reconstruction, flux
+//                        and lift into the element's face slot
 //   time_step_kernel     per-element stable time step and global minimum
 //   stage0_kernel        U0 <- U, U <- U0 - dt R
 //
 // Reference behaviour followed (VCP3D_CurviLinear/src/):
 //   ElementClass::VolumeResidual          ElementClass.cpp:3809-4443
 //   ElementClass::{I,J,K}FaceResiduals    ElementClass.cpp:2582-3400
+//   FluxesBoundaryFace + NSCBC states     FluxesFace.cpp:16-214,
+//                                         BoundaryState.cpp:829-2241
+//   AverageDataNSCBC / AverageMachNSCBC   SolverClass.cpp:2502-2581,
+//                                         ElementClass.cpp:365-479
+//   sponge layer / CML sources            ElementClass.cpp:3887-4117, :4450-4489
 //   FluxesInternalFace / InviscidFluxesFace / ViscousFluxesFace /
 //   PenAndSymFluxesSymmVar                FluxesFace.cpp:220-343, :813-975,
 //                                         :1290-1651, :1152-1284
@@ -26,6 +35,7 @@
 namespace vcp {
 namespace {
 
+enum : int { kBCOutflowNSCBC = 1, kBCInflowStaticNSCBC = 2, kBCInflowTotalNSCBC = 3 };
 constexpr double kGam = 1.4;
 constexpr double kGm1 = 0.4;
 constexpr double kPrLam = 0.72;
@@ -496,12 +506,66 @@ struct VolArgs {
   const double *faceRes, *tables, *vgeo, *econst;
   double *scal;
   int *flags, *cgIter;
+  const int *spIdx;                       // sponge slot per element (null: no sponge)
+  const double *spCoef, *dampFn, *dampState;
+  double *srcBuf;
   TableOffsets to;
-  int nElem, bug;
+  int nElem, bug, cml;
   double mu;
   StageCtl ctl;
   Ops<K, M> op;
 };
+
+// Characteristic-matching-layer source at one point (ElementClass.cpp:
+// 3887-4117): wave amplitudes of the Cartesian primitive gradients per
+// direction with the outgoing eigenvalues of the layer orientation removed,
+// P = S L mapped to the conservative variables. V: entropy variables,
+// gx/gy/gz their Cartesian gradients, wgt[d] = -J w sigma_d scale_d,
+// nsg[d] the layer orientation. Accumulates into out.
+__device__ __forceinline__ void cml_source(const double V[5], const double gx[5], const double gy[5],
+                                           const double gz[5], const double wgt[3], const double nsg[3],
+                                           double out[5])
+{
+  const double V4Inv = 1.0 / V[4];
+  const double u = -V4Inv * V[1], v = -V4Inv * V[2], w = -V4Inv * V[3];
+  const double ek = 0.5 * (u * u + v * v + w * w);
+  const double s = kGam - kGm1 * (V[0] - V[4] * ek);
+  const double rho = pow(-V[4] * exp(s), -1.0 / kGm1);
+  const double p = -rho * V4Inv, rE = p / kGm1 + rho * ek;
+  const double H = rE / rho - V4Inv, a2 = -kGam * V4Inv, a = sqrt(a2);
+  const double rhoa = rho * a, ovra = 1.0 / rhoa, ova2 = 1.0 / a2, rova = rho / a;
+  const double rhou = rho * u, rhov = rho * v, rhow = rho * w;
+  const double vel[3] = {u, v, w};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double *G = d == 0 ? gx : d == 1 ? gy : gz;
+    double dq[5];
+    dq[0] = rho * G[0] + rhou * G[1] + rhov * G[2] + rhow * G[3] + rE * G[4];
+    dq[1] = -V4Inv * G[1] + (-V4Inv * u) * G[4];
+    dq[2] = -V4Inv * G[2] + (-V4Inv * v) * G[4];
+    dq[3] = -V4Inv * G[3] + (-V4Inv * w) * G[4];
+    dq[4] = p * G[0] + (p * u) * G[1] + (p * v) * G[2] + (p * w) * G[3] + (p * H) * G[4];
+    double l1 = vel[d] - a, l2 = vel[d], l3 = vel[d] + a;
+    if (nsg[d] * l1 > 0.0) l1 = 0.0;
+    if (nsg[d] * l2 > 0.0) l2 = 0.0;
+    if (nsg[d] * l3 > 0.0) l3 = 0.0;
+    const double dun = dq[1 + d];
+    const double La = 0.5 * l1 * (ovra * dq[4] - dun);
+    const double Lb = 0.5 * l3 * (ovra * dq[4] + dun);
+    const double Le = l2 * (dq[0] - ova2 * dq[4]);
+    double P[5];
+    P[0] = rova * (Lb + La) + Le;
+    P[4] = rhoa * (Lb + La);
+#pragma unroll
+    for (int t = 0; t < 3; ++t) P[1 + t] = (t == d) ? (Lb - La) : l2 * dq[1 + t];
+    const double wd = wgt[d];
+    out[0] += wd * P[0];
+    out[1] += wd * (u * P[0] + rho * P[1]);
+    out[2] += wd * (v * P[0] + rho * P[2]);
+    out[3] += wd * (w * P[0] + rho * P[3]);
+    out[4] += wd * (ek * P[0] + rhou * P[1] + rhov * P[2] + rhow * P[3] + P[4] / kGm1);
+  }
+}
 
 // ========================================================== volume kernel
 // One workgroup of NT threads per element. LDS holds the modal state, the
@@ -530,6 +594,7 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
   const double *A = a.op.A, *D = a.op.D;
   const double *Ue = a.U + (size_t) e * 5 * ND;
   const double *vg = a.vgeo + (size_t) e * kVolGeo * NI;
+  const int sp = a.spIdx ? a.spIdx[e] : -1;
   for (int i = tid; i < M * K; i += NT) { sA[i] = A[i]; sD[i] = D[i]; }
   for (int i = tid; i < 5 * ND; i += NT) sU[i] = Ue[i];
   if (a.bug)
@@ -675,6 +740,27 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
 #pragma unroll
       for (int l = 0; l < 5; ++l) Fd[l * NI + m] = fx[l] * mx + fy[l] * my + fz[l] * mz;
     }
+    if (sp >= 0) {
+      // sponge damping sigma J w (U - U_target) and CML sources
+      // (ElementClass.cpp:4450-4489, :3887-4117)
+      const double *fn = a.dampFn + (size_t) sp * 3 * NI, *ds = a.dampState + (size_t) sp * 5 * NI;
+      const double sigma = wJ * (fn[m] + fn[NI + m] + fn[2 * NI + m]);
+      double src[5];
+      src[0] = sigma * (rho - ds[m]);
+      src[1] = sigma * (ru - ds[NI + m]);
+      src[2] = sigma * (rv - ds[2 * NI + m]);
+      src[3] = sigma * (rw - ds[3 * NI + m]);
+      src[4] = sigma * (rE - ds[4 * NI + m]);
+      if (a.cml) {
+        const double *cf = a.spCoef + (size_t) sp * 8;
+        const double wgt[3] = {weight * cf[0] * fn[m], weight * cf[1] * fn[NI + m], weight * cf[2] * fn[2 * NI + m]};
+        const double nsg[3] = {cf[3], cf[4], cf[5]};
+        cml_source(Vp[q], gx, gy, gz, wgt, nsg, src);
+      }
+      double *sb = a.srcBuf + (size_t) sp * 5 * NI;
+#pragma unroll
+      for (int l = 0; l < 5; ++l) sb[l * NI + m] = src[l];
+    }
   }
   if (a.ctl.monitor) {
     mach2 = wave_max(mach2);
@@ -734,6 +820,52 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
     }
   }
   __syncthreads();
+  if (sp >= 0) {
+    // sponge element: source terms projected with the basis functions
+    const double *sb = a.srcBuf + (size_t) sp * 5 * NI;
+    for (int r = tid; r < 5 * M * M; r += NT) {       // rows along i
+      double xx[M];
+#pragma unroll
+      for (int q = 0; q < M; ++q) xx[q] = sb[r * M + q];
+#pragma unroll
+      for (int n = 0; n < K; ++n) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < M; ++q) s += A[q * K + n] * xx[q];
+        sBig[r * K + n] = s;
+      }
+    }
+    __syncthreads();
+    double *Ys = sBig + 5 * M * M * K;
+    for (int pc = tid; pc < 5 * M * K; pc += NT) {    // pencils along j
+      const int vk = pc / K, i = pc - vk * K;
+      double xx[M];
+#pragma unroll
+      for (int q = 0; q < M; ++q) xx[q] = sBig[(vk * M + q) * K + i];
+#pragma unroll
+      for (int n = 0; n < K; ++n) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < M; ++q) s += A[q * K + n] * xx[q];
+        Ys[(vk * K + n) * K + i] = s;
+      }
+    }
+    __syncthreads();
+    for (int pc = tid; pc < 5 * K * K; pc += NT) {    // pencils along k, accumulated
+      const int v = pc / (K * K), ji = pc - v * K * K;
+      double xx[M];
+#pragma unroll
+      for (int q = 0; q < M; ++q) xx[q] = Ys[(v * M + q) * K * K + ji];
+#pragma unroll
+      for (int n = 0; n < K; ++n) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < M; ++q) s += A[q * K + n] * xx[q];
+        sU[(v * K + n) * K * K + ji] += s;
+      }
+    }
+    __syncthreads();
+  }
 
   // ---------------- gather the six face contributions
   double r[DPT];
@@ -944,6 +1076,570 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
   if (bad) atomicOr(a.flags, kFlagNonFinite);
 }
 
+// ================================================= open-boundary kernels
+// Ismail-Roe + viscous + interior-penalty + symmetrising fluxes at one face
+// point: VL/VR entropy states (L -> R along nn), gx/gy/gz the averaged
+// Cartesian gradients, wA = weight * area factor. fl: normal flux, FX/FY/FZ:
+// Cartesian symmetrising fluxes (FluxesFace.cpp:813-975, :1290-1651,
+// :1152-1284).
+__device__ __forceinline__ void point_flux(const double VL[5], const double VR[5], const double gx[5],
+                                           const double gy[5], const double gz[5], const double nn[3],
+                                           double wA, double tau, double mu, double thetaHalf,
+                                           double fl[5], double FX[5], double FY[5], double FZ[5])
+{
+  const double factHF = kGam / kPrLam;
+  const double lam = kLamOverMu, lamp1 = lam + 1.0, lamp2 = lam + 2.0;
+  {
+    double rL, uL, vL, wL, pL, eL, rR, uR, vR, wR, pR, eR;
+    ent2prim(VL, rL, uL, vL, wL, pL, eL);
+    ent2prim(VR, rR, uR, vR, wR, pR, eR);
+    const double dV0 = VR[0] - VL[0], dV1 = VR[1] - VL[1], dV2 = VR[2] - VL[2],
+                 dV3 = VR[3] - VL[3], dV4 = VR[4] - VL[4];
+    const double z0L = sqrt(rL / pL), z0R = sqrt(rR / pR);
+    const double z4L = sqrt(rL * pL), z4R = sqrt(rR * pR);
+    const double z0A = 0.5 * (z0L + z0R), z4A = 0.5 * (z4L + z4R);
+    const double z1A = 0.5 * (uL * z0L + uR * z0R), z2A = 0.5 * (vL * z0L + vR * z0R),
+                 z3A = 0.5 * (wL * z0L + wR * z0R);
+    double zeta = z0L / z0R, f = (zeta - 1.0) / (zeta + 1.0), uu = f * f, F;
+    F = (uu < 0.01) ? 1.0 + uu / 3.0 + uu * uu / 5.0 + uu * uu * uu / 7.0 + uu * uu * uu * uu / 9.0
+                    : log(zeta) / (2.0 * f);
+    const double z0Log = z0A / F;
+    zeta = z4L / z4R; f = (zeta - 1.0) / (zeta + 1.0); uu = f * f;
+    F = (uu < 0.01) ? 1.0 + uu / 3.0 + uu * uu / 5.0 + uu * uu * uu / 7.0 + uu * uu * uu * uu / 9.0
+                    : log(zeta) / (2.0 * f);
+    const double z4Log = z4A / F;
+    const double oz0 = 1.0 / z0A;
+    const double rA = z0A * z4Log, p1 = oz0 * z4A;
+    const double p2 = 0.5 * ((kGam + 1.0) / kGam * z4Log / z0Log + kGm1 / kGam * p1);
+    const double uA = oz0 * z1A, vA = oz0 * z2A, wAv = oz0 * z3A;
+    const double vn = uA * nn[0] + vA * nn[1] + wAv * nn[2];
+    const double kin = 0.5 * (uA * uA + vA * vA + wAv * wAv);
+    const double a2 = kGam * p2 / rA, aa = sqrt(a2);
+    const double H = a2 / kGm1 + kin, E = H - p2 / rA;
+    double abv1 = rA * (uA * dV1 + vA * dV2 + wAv * dV3);
+    double abv2 = abv1 + rA * (dV0 + H * dV4);
+    const double dr = abv1 + rA * (dV0 + E * dV4);
+    const double dru = uA * abv2 + p2 * dV1, drv = vA * abv2 + p2 * dV2, drw = wAv * abv2 + p2 * dV3;
+    const double drE = H * abv1 + rA * E * (dV0 + H * dV4) + p2 * kin * dV4;
+    const double fac = 0.5 * (1.0 + 1.0 / 6.0);
+    const double lam1 = fabs(vn + aa) * fac, lam2 = fabs(vn - aa) * fac, lam3 = 0.5 * fabs(vn);
+    abv1 = 0.5 * (lam1 + lam2); abv2 = 0.5 * (lam1 - lam2);
+    const double abv3 = abv1 - lam3;
+    const double abv4 = kGm1 * (kin * dr - uA * dru - vA * drv - wAv * drw + drE);
+    const double abv5 = nn[0] * dru + nn[1] * drv + nn[2] * drw - vn * dr;
+    const double abv6 = abv3 * abv4 / a2 + abv2 * abv5 / aa;
+    const double abv7 = abv2 * abv4 / aa + abv3 * abv5;
+    fl[0] = wA * (rA * vn - (lam3 * dr + abv6));
+    fl[1] = wA * (rA * vn * uA + p1 * nn[0] - (lam3 * dru + uA * abv6 + nn[0] * abv7));
+    fl[2] = wA * (rA * vn * vA + p1 * nn[1] - (lam3 * drv + vA * abv6 + nn[1] * abv7));
+    fl[3] = wA * (rA * vn * wAv + p1 * nn[2] - (lam3 * drw + wAv * abv6 + nn[2] * abv7));
+    fl[4] = wA * (rA * vn * H - (lam3 * drE + H * abv6 + vn * abv7));
+  }
+  const double V4Inv = 2.0 / (VL[4] + VR[4]);
+  const double u = -V4Inv * 0.5 * (VL[1] + VR[1]), vv = -V4Inv * 0.5 * (VL[2] + VR[2]),
+               w = -V4Inv * 0.5 * (VL[3] + VR[3]);
+  {
+    const double dudx = -V4Inv * (gx[1] + u * gx[4]), dvdx = -V4Inv * (gx[2] + vv * gx[4]), dwdx = -V4Inv * (gx[3] + w * gx[4]);
+    const double dudy = -V4Inv * (gy[1] + u * gy[4]), dvdy = -V4Inv * (gy[2] + vv * gy[4]), dwdy = -V4Inv * (gy[3] + w * gy[4]);
+    const double dudz = -V4Inv * (gz[1] + u * gz[4]), dvdz = -V4Inv * (gz[2] + vv * gz[4]), dwdz = -V4Inv * (gz[3] + w * gz[4]);
+    const double fe = V4Inv * V4Inv / kGm1;
+    const double dedx = fe * gx[4], dedy = fe * gy[4], dedz = fe * gz[4];
+    const double nx = wA * nn[0], ny = wA * nn[1], nz = wA * nn[2];
+    const double lamDiv = kLamOverMu * (dudx + dvdy + dwdz);
+    const double txx = mu * (2.0 * dudx + lamDiv), tyy = mu * (2.0 * dvdy + lamDiv), tzz = mu * (2.0 * dwdz + lamDiv);
+    const double txy = mu * (dudy + dvdx), txz = mu * (dudz + dwdx), tyz = mu * (dvdz + dwdy);
+    const double fh = mu * factHF;
+    const double qx = -fh * dedx, qy = -fh * dedy, qz = -fh * dedz;
+    fl[1] -= txx * nx + txy * ny + txz * nz;
+    fl[2] -= txy * nx + tyy * ny + tyz * nz;
+    fl[3] -= txz * nx + tyz * ny + tzz * nz;
+    fl[4] -= (u * txx + vv * txy + w * txz - qx) * nx + (u * txy + vv * tyy + w * tyz - qy) * ny
+           + (u * txz + vv * tyz + w * tzz - qz) * nz;
+  }
+  const double pOvRho = -V4Inv, eInt = pOvRho / kGm1;
+  const double dV1 = VL[1] - VR[1], dV2 = VL[2] - VR[2], dV3 = VL[3] - VR[3], dV4 = VL[4] - VR[4];
+  const double factFlux = pOvRho * wA * mu;
+  const double abvE = u * u + vv * vv + w * w + factHF * eInt;
+  const double un = u * nn[0] + vv * nn[1] + w * nn[2];
+  const double dVn = dV1 * nn[0] + dV2 * nn[1] + dV3 * nn[2];
+  const double ttau = tau * factFlux, abvMom = lamp1 * (dVn + un * dV4);
+  fl[1] += ttau * (abvMom * nn[0] + dV1 + u * dV4);
+  fl[2] += ttau * (abvMom * nn[1] + dV2 + vv * dV4);
+  fl[3] += ttau * (abvMom * nn[2] + dV3 + w * dV4);
+  fl[4] += ttau * (lamp1 * un * dVn + u * dV1 + vv * dV2 + w * dV3 + (abvE + lamp1 * un * un) * dV4);
+  const double sx = thetaHalf * factFlux * nn[0], sy = thetaHalf * factFlux * nn[1], sz = thetaHalf * factFlux * nn[2];
+  const double K11_14 = u * lamp2, K11_44 = abvE + u * u * lamp1;
+  const double K22_24 = vv * lamp2, K22_44 = abvE + vv * vv * lamp1;
+  const double K33_34 = w * lamp2, K33_44 = abvE + w * w * lamp1;
+  const double K12_14 = vv * lam, K12_42 = u * lam, K12_44 = u * vv * lamp1;
+  const double K13_14 = w * lam, K13_43 = u * lam, K13_44 = u * w * lamp1;
+  const double K23_24 = w * lam, K23_43 = vv * lam, K23_44 = vv * w * lamp1;
+  FX[0] = 0.0;
+  FX[1] = -(lamp2 * dV1 + K11_14 * dV4) * sx - (lam * dV2 + K12_14 * dV4) * sy - (lam * dV3 + K13_14 * dV4) * sz;
+  FX[2] = -(dV2 + vv * dV4) * sx - (dV1 + u * dV4) * sy;
+  FX[3] = -(dV3 + w * dV4) * sx - (dV1 + u * dV4) * sz;
+  FX[4] = -(K11_14 * dV1 + vv * dV2 + w * dV3 + K11_44 * dV4) * sx
+          - (vv * dV1 + K12_42 * dV2 + K12_44 * dV4) * sy - (w * dV1 + K13_43 * dV3 + K13_44 * dV4) * sz;
+  FY[0] = 0.0;
+  FY[1] = -(dV2 + vv * dV4) * sx - (dV1 + u * dV4) * sy;
+  FY[2] = -(lam * dV1 + K12_42 * dV4) * sx - (lamp2 * dV2 + K22_24 * dV4) * sy - (lam * dV3 + K23_24 * dV4) * sz;
+  FY[3] = -(dV3 + w * dV4) * sy - (dV2 + vv * dV4) * sz;
+  FY[4] = -(K12_14 * dV1 + u * dV2 + K12_44 * dV4) * sx
+          - (u * dV1 + K22_24 * dV2 + w * dV3 + K22_44 * dV4) * sy - (w * dV2 + K23_43 * dV3 + K23_44 * dV4) * sz;
+  FZ[0] = 0.0;
+  FZ[1] = -(dV3 + w * dV4) * sx - (dV1 + u * dV4) * sz;
+  FZ[2] = -(dV3 + w * dV4) * sy - (dV2 + vv * dV4) * sz;
+  FZ[3] = -(lam * dV1 + K13_43 * dV4) * sx - (lam * dV2 + K23_43 * dV4) * sy - (lamp2 * dV3 + K33_34 * dV4) * sz;
+  FZ[4] = -(K13_14 * dV1 + u * dV3 + K13_44 * dV4) * sx
+          - (K23_24 * dV2 + vv * dV3 + K23_44 * dV4) * sy - (u * dV1 + vv * dV2 + K33_34 * dV3 + K33_44 * dV4) * sz;
+}
+
+template <int K, int M>
+struct BndArgs {
+  const double *U, *tables, *econst, *bgeo;   // bgeo: [nBFace][13][nInt2D] own face metric
+  const int *bface;
+  double *machPart, *faceRes;
+  TableOffsets to;
+  int bug;
+  double mu, tauFact, thetaHalf;
+  NscbcParams nb;
+};
+
+// Operators and trace of one element face in LDS (runtime direction).
+template <int K, int M>
+struct BndSmem {
+  double A[M * K], D[M * K], F[2][K], P1[M * M], dL[M * M], W2[M * M];
+  double B[2][5 * K * K];          // normal contraction: value, normal derivative
+  double T[3][5 * K * M];          // fast contraction: A, D (of value), A (of normal derivative)
+  double S[4][5 * M * M];          // interior trace: value, d/d(normal), d/d(fast), d/d(slow)
+  double Q[4][5 * M * M];          // boundary state: value, d/d(normal), d/d(fast), d/d(slow)
+  double X[2][5 * M * M];          // B matrix / PC stage; later flux + spare
+  double Sym[3][5 * M * M];        // symmetrising fluxes (normal, fast, slow)
+  double red[2];
+};
+
+// Trace of element e's face (dir, side): values and parametric derivatives
+// in the face-local (normal, fast, slow) slots, [5][slow][fast] layout
+// (TensorProductSolAndGrad{I,J,K}Face).
+template <int K, int M>
+__device__ void bnd_trace(BndSmem<K, M> &sm, const double *Ue, int dir, int side, bool grads)
+{
+  constexpr int ND = K * K * K, N2 = M * M;
+  const int tid = threadIdx.x;
+  const int SN = dir == 0 ? 1 : dir == 1 ? K : K * K;
+  const int SFA = dir == 0 ? K : 1;
+  const int SSL = dir == 2 ? K : K * K;
+  const double *Fv = sm.F[0], *dFv = sm.F[1];
+  for (int pc = tid; pc < 5 * K * K; pc += 64) {
+    const int v = pc / (K * K), sl = (pc / K) % K, f = pc % K;
+    const double *u = Ue + v * ND + sl * SSL + f * SFA;
+    double b0 = 0.0, b1 = 0.0;
+#pragma unroll
+    for (int t = 0; t < K; ++t) { const double x = u[t * SN]; b0 += Fv[t] * x; b1 += dFv[t] * x; }
+    sm.B[0][pc] = b0; sm.B[1][pc] = b1;
+  }
+  __syncthreads();
+  for (int pc = tid; pc < 5 * K; pc += 64) {
+    double x0[K], x1[K];
+#pragma unroll
+    for (int f = 0; f < K; ++f) { x0[f] = sm.B[0][pc * K + f]; x1[f] = sm.B[1][pc * K + f]; }
+    for (int n = 0; n < M; ++n) {
+      double tA = 0.0, tD = 0.0, tN = 0.0;
+#pragma unroll
+      for (int f = 0; f < K; ++f) { tA += sm.A[n * K + f] * x0[f]; tD += sm.D[n * K + f] * x0[f]; tN += sm.A[n * K + f] * x1[f]; }
+      sm.T[0][pc * M + n] = tA; sm.T[1][pc * M + n] = tD; sm.T[2][pc * M + n] = tN;
+    }
+  }
+  __syncthreads();
+  for (int idx = tid; idx < 5 * N2; idx += 64) {
+    const int v = idx / N2, m = idx - v * N2, sp = m / M, fp = m - sp * M;
+    double val = 0.0, gN = 0.0, gF = 0.0, gS = 0.0;
+#pragma unroll
+    for (int sl = 0; sl < K; ++sl) {
+      const double a = sm.A[sp * K + sl], xa = sm.T[0][(v * K + sl) * M + fp];
+      val += a * xa;
+      if (grads) { gS += sm.D[sp * K + sl] * xa; gN += a * sm.T[2][(v * K + sl) * M + fp]; gF += a * sm.T[1][(v * K + sl) * M + fp]; }
+    }
+    sm.S[0][idx] = val;
+    if (grads) { sm.S[1][idx] = gN; sm.S[2][idx] = gF; sm.S[3][idx] = gS; }
+  }
+  __syncthreads();
+}
+
+template <int K, int M>
+__device__ void bnd_load_ops(BndSmem<K, M> &sm, const BndArgs<K, M> &a, int side)
+{
+  const int tid = threadIdx.x;
+  const double *T = a.tables;
+  for (int i = tid; i < M * K; i += 64) { sm.A[i] = T[a.to.A + i]; sm.D[i] = T[a.to.D + i]; }
+  for (int i = tid; i < K; i += 64) {
+    sm.F[0][i] = T[(side ? a.to.fmax : a.to.fmin) + i];
+    sm.F[1][i] = T[(side ? a.to.dfmax : a.to.dfmin) + i];
+  }
+  for (int i = tid; i < M * M; i += 64) {
+    sm.P1[i] = T[a.to.P1 + i]; sm.dL[i] = T[a.to.dLq + i]; sm.W2[i] = T[a.to.w2 + i];
+  }
+  __syncthreads();
+}
+
+// Weighted face-normal Mach number of one boundary face
+// (WeightedMachElement, BoundaryState.cpp:935-994).
+template <int K, int M>
+__global__ __launch_bounds__(64) void mach_kernel(BndArgs<K, M> a)
+{
+  constexpr int ND = K * K * K, N2 = M * M;
+  __shared__ BndSmem<K, M> sm;
+  const int b = blockIdx.x;
+  const int e = a.bface[2 * b], f = a.bface[2 * b + 1];
+  const int dir = f / 2, side = f % 2;
+  bnd_load_ops(sm, a, side);
+  bnd_trace(sm, a.U + (size_t) e * 5 * ND, dir, side, false);
+  const double *g = a.bgeo + (size_t) b * 13 * N2;
+  double sum = 0.0;
+  for (int m = threadIdx.x; m < N2; m += 64) {
+    const double *rr = g + (size_t) (4 + 3 * dir) * N2;
+    const double rx = rr[m], ry = rr[N2 + m], rz = rr[2 * N2 + m];
+    const double mag = sqrt(rx * rx + ry * ry + rz * rz);
+    const double V4Inv = 1.0 / sm.S[0][4 * N2 + m];
+    const double u = -V4Inv * sm.S[0][N2 + m], v = -V4Inv * sm.S[0][2 * N2 + m], w = -V4Inv * sm.S[0][3 * N2 + m];
+    const double c = sqrt(-kGam * V4Inv);
+    const double urr = (u * rx + v * ry + w * rz) / mag;
+    sum += sm.W2[m] * g[3 * N2 + m] * (urr / c);
+  }
+  sum = wave_sum(sum);
+  if (threadIdx.x == 0) a.machPart[b] = sum;
+}
+
+// One wave per PC-NSCBC boundary face: characteristic boundary state
+// (BoundaryState.cpp:996-2241), its polynomial-correction reconstruction,
+// the boundary flux (FluxesBoundaryFace, FluxesFace.cpp:16-214) and its lift
+// into the element's face slot.
+template <int K, int M>
+__global__ __launch_bounds__(64) void nscbc_kernel(BndArgs<K, M> a)
+{
+  constexpr int ND = K * K * K, N2 = M * M;
+  __shared__ BndSmem<K, M> sm;
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int e = a.bface[2 * b], f = a.bface[2 * b + 1];
+  const int dir = f / 2, side = f % 2;
+  const double fn = side ? 1.0 : -1.0;
+  const int bc = a.nb.bc[f];
+  bnd_load_ops(sm, a, side);
+  bnd_trace(sm, a.U + (size_t) e * 5 * ND, dir, side, true);
+  const double *g = a.bgeo + (size_t) b * 13 * N2;
+  // parametric slot (0 r, 1 s, 2 t) of the normal, fast and slow face axes
+  const int pN = dir, pF = dir == 0 ? 1 : 0, pS = dir == 2 ? 1 : 2;
+  // boundary-averaged Mach number (fixed summation order)
+  double mach = 0.0;
+  for (int i = a.nb.begin[f] + tid; i < a.nb.begin[f + 1]; i += 64) mach += a.machPart[i];
+  mach = wave_sum(mach) * a.nb.invArea[f];
+  const double dell = fn * a.nb.dell0;
+  const int iPhi = fn > 0.0 ? 0 : 4, iPsi = iPhi == 0 ? 4 : 0;
+
+  for (int m = tid; m < N2; m += 64) {
+    // entropy -> primitive state and derivatives (BoundaryState.cpp:1002-1108)
+    double V[5], Q[5], dQ[3][5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) V[v] = sm.S[0][v * N2 + m];
+    const double V4Inv = 1.0 / V[4];
+    const double u = -V4Inv * V[1], v = -V4Inv * V[2], w = -V4Inv * V[3];
+    const double ek = 0.5 * (u * u + v * v + w * w);
+    const double sEnt = kGam - kGm1 * (V[0] - V[4] * ek);
+    const double a2 = -kGam * V4Inv;
+    const double rho = pow(-V[4] * exp(sEnt), -1.0 / kGm1);
+    const double p = a2 * rho / kGam;
+    const double povrho = -V4Inv;
+    const double E = povrho / kGm1 + ek, H = E + povrho;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double *dV = sm.S[1 + c];
+      const double d0 = dV[m], d1 = dV[N2 + m], d2 = dV[2 * N2 + m], d3 = dV[3 * N2 + m], d4 = dV[4 * N2 + m];
+      dQ[c][0] = rho * d0 + rho * u * d1 + rho * v * d2 + rho * w * d3 + rho * E * d4;
+      dQ[c][1] = povrho * d1 + povrho * u * d4;
+      dQ[c][2] = povrho * d2 + povrho * v * d4;
+      dQ[c][3] = povrho * d3 + povrho * w * d4;
+      dQ[c][4] = p * d0 + p * u * d1 + p * v * d2 + p * w * d3 + p * H * d4;
+    }
+    Q[0] = rho; Q[1] = u; Q[2] = v; Q[3] = w; Q[4] = p;
+    // normal wave amplitudes with the incoming ones replaced
+    const double rr[3] = {g[(4 + 3 * pN) * N2 + m], g[(5 + 3 * pN) * N2 + m], g[(6 + 3 * pN) * N2 + m]};
+    const double magrr = sqrt(rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2]);
+    const double ovmagrr = 1.0 / magrr;
+    const double rn[3] = {rr[0] * ovmagrr, rr[1] * ovmagrr, rr[2] * ovmagrr};
+    const double drhodr = dQ[0][0], dudr = dQ[0][1], dvdr = dQ[0][2], dwdr = dQ[0][3], dpdr = dQ[0][4];
+    const double ovrho = 1.0 / rho, c2 = kGam * p * ovrho, c = sqrt(c2);
+    const double ova2 = 1.0 / c2, rhoa = rho * c, ovrhoa = 1.0 / rhoa, rhova = rho / c;
+    const double dsdr = drhodr - ova2 * dpdr;
+    const double dw1dr = rn[1] * dwdr - rn[2] * dvdr;
+    const double dw2dr = rn[2] * dudr - rn[0] * dwdr;
+    const double dw3dr = rn[0] * dvdr - rn[1] * dudr;
+    const double durdr = rn[0] * dudr + rn[1] * dvdr + rn[2] * dwdr;
+    double LL[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double lmb1, lmb3;
+    if (bc == kBCOutflowNSCBC) {
+      // BoundaryState.cpp:1318-1519
+      const double sign = iPhi == 0 ? -1.0 : 1.0;
+      const double kk = 0.5 * a.nb.outlet_sigma * (1.0 - mach * mach) / a.nb.outlet_len;
+      const double bl = a.nb.outlet_beta_l < 0.0 ? mach : a.nb.outlet_beta_l;
+      const double bt = a.nb.outlet_beta_t < 0.0 ? mach : a.nb.outlet_beta_t;
+      const double cbl = a.nb.outlet_one_minus_beta ? fmin(1.0, 1.0 - bl) : fmin(1.0, bl);
+      const double cbt = a.nb.outlet_one_minus_beta ? fmin(1.0, 1.0 - bt) : fmin(1.0, bt);
+      const double ss[3] = {g[(4 + 3 * pF) * N2 + m], g[(5 + 3 * pF) * N2 + m], g[(6 + 3 * pF) * N2 + m]};
+      const double tt[3] = {g[(4 + 3 * pS) * N2 + m], g[(5 + 3 * pS) * N2 + m], g[(6 + 3 * pS) * N2 + m]};
+      const double duds = dQ[1][1], dudt = dQ[2][1], dvds = dQ[1][2], dvdt = dQ[2][2];
+      const double dwds = dQ[1][3], dwdt = dQ[2][3], dpds = dQ[1][4], dpdt = dQ[2][4];
+      const double ur = u * rr[0] + v * rr[1] + w * rr[2];
+      const double us = u * ss[0] + v * ss[1] + w * ss[2];
+      const double ut = u * tt[0] + v * tt[1] + w * tt[2];
+      const double arr = c * magrr;
+      const double ars = c * (rn[0] * ss[0] + rn[1] * ss[1] + rn[2] * ss[2]);
+      const double art = c * (rn[0] * tt[0] + rn[1] * tt[1] + rn[2] * tt[2]);
+      lmb1 = ur - arr; lmb3 = ur + arr;
+      const double durds = rn[0] * duds + rn[1] * dvds + rn[2] * dwds;
+      const double durdt = rn[0] * dudt + rn[1] * dvdt + rn[2] * dwdt;
+      LL[0] = 0.5 * lmb1 * (ovrhoa * dpdr - durdr);
+      LL[1] = rn[0] * dsdr - dw1dr;
+      LL[2] = rn[1] * dsdr - dw2dr;
+      LL[3] = rn[2] * dsdr - dw3dr;
+      LL[4] = 0.5 * lmb3 * (ovrhoa * dpdr + durdr);
+      const double LT = 0.5 * ovrhoa * (us + sign * ars) * dpds + sign * 0.5 * us * durds
+                      + 0.5 * ovrhoa * (ut + sign * art) * dpdt + sign * 0.5 * ut * durdt;
+      const double TT = -0.5 * c * (ss[0] * duds + ss[1] * dvds + ss[2] * dwds
+                                  + tt[0] * dudt + tt[1] * dvdt + tt[2] * dwdt);
+      LL[iPhi] = kk * ovrho * magrr * (p - a.nb.data[f][0]) - cbl * LT + cbt * TT;
+    } else if (bc == kBCInflowStaticNSCBC) {
+      // BoundaryState.cpp:1596-1783
+      const double sign = fn > 0.0 ? -1.0 : 1.0;
+      const double ka = sign * 0.5 * a.nb.inlet_sigma * (1.0 - mach * mach) / a.nb.inlet_len;
+      const double ku = a.nb.inlet_sigma / a.nb.inlet_len;
+      const double ur = u * rr[0] + v * rr[1] + w * rr[2];
+      const double arr = c * magrr;
+      lmb1 = ur - arr; lmb3 = ur + arr;
+      if (a.nb.inlet_incoming) {
+        LL[0] = 0.5 * lmb1 * (ovrhoa * dpdr - durdr);
+        LL[4] = 0.5 * lmb3 * (ovrhoa * dpdr + durdr);
+      }
+      const double drho = rho - a.nb.data[f][0], du = u - a.nb.data[f][1], dv = v - a.nb.data[f][2],
+                   dw = w - a.nb.data[f][3];
+      const double kaol = ku * c * (1.0 / ur);
+      LL[iPhi] = ka * c * (rr[0] * du + rr[1] * dv + rr[2] * dw);
+      LL[1] = kaol * (rr[2] * dv - rr[1] * dw + rr[0] * drho);
+      LL[2] = kaol * (rr[0] * dw - rr[2] * du + rr[1] * drho);
+      LL[3] = kaol * (rr[1] * du - rr[0] * dv + rr[2] * drho);
+    } else {
+      // total-condition inflow, BoundaryState.cpp:1866-2184
+      const double ovrg = 1.0 / 287.058, ovcp = kGm1 / (kGam * 287.058);
+      const double kk = -a.nb.inlet_sigma / a.nb.inlet_len;
+      const double ova = 1.0 / c, T = p * ovrho * ovrg;
+      const double ur = u * rn[0] + v * rn[1] + w * rn[2];
+      const double w1 = rn[1] * w - rn[2] * v, w2 = rn[2] * u - rn[0] * w, w3 = rn[0] * v - rn[1] * u;
+      lmb1 = ur - c; lmb3 = ur + c;
+      const double ovlmb2 = 1.0 / ur;
+      if (a.nb.inlet_incoming) {
+        LL[0] = 0.5 * lmb1 * (ovrhoa * dpdr - durdr);
+        LL[4] = 0.5 * lmb3 * (ovrhoa * dpdr + durdr);
+      }
+      const double magU2 = 2.0 * ek, magU = sqrt(magU2);
+      const double ff = 1.0 + 0.5 * kGm1 * (magU2 / c2);
+      const double Pt = p * pow(ff, kGam / kGm1), Tt = T * ff;
+      const double Zt = v / magU, Zp = w / magU;
+      const double ka = kk * c;
+      const double X1 = ka * (Pt - a.nb.data[f][0]), X2 = ka * (Tt - a.nb.data[f][1]);
+      const double X3 = ka * (Zt - a.nb.data[f][3]), X4 = ka * (Zp - a.nb.data[f][4]);
+      const double ovrPtovTt = ovrg * (Pt / Tt), Ttovrho = Tt * ovrho;
+      const double govaPt = kGam * ova * Pt, ekovrho = ek * ovrho, gm1ekova = kGm1 * ek * ova;
+      const double gm1ovaTt = kGm1 * ova * Tt, ovmagU2 = 1.0 / magU2;
+      const double gp = gm1ekova + ur, gmn = gm1ekova - ur;
+      const double f1 = w1 - rn[0] * ekovrho, f2 = w2 - rn[1] * ekovrho, f3 = w3 - rn[2] * ekovrho;
+      const double A1 = gp * ovrPtovTt - govaPt, A2 = f1 * ovrPtovTt, A3 = f2 * ovrPtovTt, A4 = f3 * ovrPtovTt;
+      const double A5 = gmn * ovrPtovTt - govaPt;
+      const double B1 = gp * ovcp - gm1ovaTt, B2 = f1 * ovcp + rn[0] * Ttovrho, B3 = f2 * ovcp + rn[1] * Ttovrho;
+      const double B4 = f3 * ovcp + rn[2] * Ttovrho, B5 = gmn * ovcp - gm1ovaTt;
+      const double C1 = (rn[1] * magU - ur * Zt) * ovmagU2, C2 = -(rn[2] * magU + w1 * Zt) * ovmagU2;
+      const double C3 = (-w2 * Zt) * ovmagU2, C4 = (rn[0] * magU - w3 * Zt) * ovmagU2, C5 = -C1;
+      const double D1 = (rn[2] * magU - ur * Zp) * ovmagU2, D2 = (rn[1] * magU - w1 * Zp) * ovmagU2;
+      const double D3 = -(rn[0] * magU + w2 * Zp) * ovmagU2, D4 = (-w3 * Zp) * ovmagU2, D5 = -D1;
+      const double Y1 = LL[iPsi] * A1, Y2 = LL[iPsi] * B1, Y3 = LL[iPsi] * C1, Y4 = LL[iPsi] * D1;
+      // elimination of the 4 x 4 system for the incoming amplitudes
+      const double ovA2 = 1.0 / A2;
+      const double E0 = ovA2 * X1, E3 = -ovA2 * A3, E4 = -ovA2 * A4, E5 = -ovA2 * A5, P2 = -ovA2 * Y1;
+      const double ovdF = 1.0 / (D3 + D2 * E3);
+      const double F0 = ovdF * (X4 - D2 * E0), F4 = -ovdF * (D4 + D2 * E4), F5 = -ovdF * (D5 + D2 * E5);
+      const double P3 = -ovdF * (Y4 + D2 * P2);
+      const double ovdG = 1.0 / (C4 + C2 * E4 + C3 * F4 + C2 * E3 * F4);
+      const double G0 = ovdG * (X3 - C2 * E0 - C3 * F0 - C2 * E3 * F0);
+      const double G5 = -ovdG * (C5 + C2 * E5 + C3 * F5 + C2 * E3 * F5);
+      const double P4 = -ovdG * (Y3 + C2 * P2 + C3 * P3 + C2 * E3 * P3);
+      const double ovdH = 1.0 / (B5 + B2 * E5 + B3 * F5 + B4 * G5 + B2 * E3 * F5 + B2 * E4 * G5 + B3 * F4 * G5
+                                 + B2 * E3 * F4 * G5);
+      const double H0 = ovdH * (X2 - B2 * E0 - B3 * F0 - B4 * G0 - B2 * E3 * F0 - B2 * E4 * G0 - B3 * F4 * G0
+                                - B2 * E3 * F4 * G0);
+      const double P5 = -ovdH * (Y2 + B2 * P2 + B3 * P3 + B4 * P4 + B2 * E3 * P3 + B2 * E4 * P4 + B3 * F4 * P4
+                                 + B2 * E3 * F4 * P4);
+      LL[iPhi] = H0 + P5;
+      LL[3] = G0 + G5 * LL[iPhi] + P4;
+      LL[2] = F0 + F4 * LL[3] + F5 * LL[iPhi] + P3;
+      LL[1] = E0 + E3 * LL[2] + E4 * LL[3] + E5 * LL[iPhi] + P2;
+      LL[1] *= ovlmb2; LL[2] *= ovlmb2; LL[3] *= ovlmb2;
+    }
+    // target normal derivative of the primitive state and the B matrix
+    const double ovlmb1 = 1.0 / lmb1, ovlmb3 = 1.0 / lmb3;
+    double t[5];
+    t[0] = ovlmb1 * rhova * LL[0] + rn[0] * LL[1] + rn[1] * LL[2] + rn[2] * LL[3] + ovlmb3 * rhova * LL[4];
+    t[1] = -ovlmb1 * rn[0] * LL[0] - rn[2] * LL[2] + rn[1] * LL[3] + ovlmb3 * rn[0] * LL[4];
+    t[2] = -ovlmb1 * rn[1] * LL[0] + rn[2] * LL[1] - rn[0] * LL[3] + ovlmb3 * rn[1] * LL[4];
+    t[3] = -ovlmb1 * rn[2] * LL[0] - rn[1] * LL[1] + rn[0] * LL[2] + ovlmb3 * rn[2] * LL[4];
+    t[4] = ovlmb1 * rhoa * LL[0] + ovlmb3 * rhoa * LL[4];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      sm.Q[1][q * N2 + m] = t[q];
+      sm.X[0][q * N2 + m] = t[q] - (dQ[0][q] - Q[q] * dell);
+    }
+  }
+  __syncthreads();
+  // polynomial correction Q_b = (P1 x P1) B / dell (sign of the face):
+  // fast axis, then slow axis
+  for (int idx = tid; idx < 5 * N2; idx += 64) {
+    const int row = idx / M, j = idx - row * M;
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < M; ++q) s += sm.P1[j * M + q] * sm.X[0][row * M + q];
+    sm.X[1][idx] = s;
+  }
+  __syncthreads();
+  const double scale = 1.0 / dell;
+  for (int idx = tid; idx < 5 * N2; idx += 64) {
+    const int v = idx / N2, m = idx - v * N2, i = m / M, j = m - i * M;
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < M; ++q) s += sm.P1[i * M + q] * sm.X[1][(v * M + q) * M + j];
+    sm.Q[0][idx] = s * scale;
+  }
+  __syncthreads();
+  // primitive -> entropy variables for the state and its normal derivative
+  for (int m = tid; m < N2; m += 64) {
+    const double rho = sm.Q[0][m], u = sm.Q[0][N2 + m], v = sm.Q[0][2 * N2 + m], w = sm.Q[0][3 * N2 + m],
+                 p = sm.Q[0][4 * N2 + m];
+    const double dr = sm.Q[1][m], du = sm.Q[1][N2 + m], dv = sm.Q[1][2 * N2 + m], dw = sm.Q[1][3 * N2 + m],
+                 dp = sm.Q[1][4 * N2 + m];
+    const double sEnt = log(p / pow(rho, kGam));
+    const double ovrho = 1.0 / rho, ovp = 1.0 / p, rovp = rho * ovp;
+    const double ek = 0.5 * (u * u + v * v + w * w);
+    const double W1 = rovp * u, W2 = rovp * v, W3 = rovp * w;
+    sm.Q[0][m] = -rovp * ek + (kGam - sEnt) / kGm1;
+    sm.Q[0][N2 + m] = W1; sm.Q[0][2 * N2 + m] = W2; sm.Q[0][3 * N2 + m] = W3; sm.Q[0][4 * N2 + m] = -rovp;
+    const double q0 = kGam / kGm1 * ovrho - ovp * ek, q4 = ovp * (rovp * ek - 1.0 / kGm1);
+    sm.Q[1][m] = q0 * dr - W1 * du - W2 * dv - W3 * dw + q4 * dp;
+    sm.Q[1][N2 + m] = u * ovp * dr + rovp * du - ovp * W1 * dp;
+    sm.Q[1][2 * N2 + m] = v * ovp * dr + rovp * dv - ovp * W2 * dp;
+    sm.Q[1][3 * N2 + m] = w * ovp * dr + rovp * dw - ovp * W3 * dp;
+    sm.Q[1][4 * N2 + m] = -ovp * dr + ovp * rovp * dp;
+  }
+  __syncthreads();
+  // transverse derivatives of the reconstructed state on the Gauss points;
+  // the reference stores the slow-axis derivative in the fast-axis slot
+  // (BoundaryState.cpp:1261-1287)
+  double *dSlow = a.bug ? sm.Q[2] : sm.Q[3];
+  double *dFast = a.bug ? sm.Q[3] : sm.Q[2];
+  for (int idx = tid; idx < 5 * N2; idx += 64) {
+    const int v = idx / N2, m = idx - v * N2, i = m / M, j = m - i * M;
+    const double *q0 = sm.Q[0] + v * N2;
+    double ds = 0.0, dt = 0.0;
+#pragma unroll
+    for (int k = 0; k < M; ++k) { ds += sm.dL[i * M + k] * q0[k * M + j]; dt += sm.dL[j * M + k] * q0[i * M + k]; }
+    dSlow[idx] = ds; dFast[idx] = dt;
+  }
+  __syncthreads();
+  // fluxes with the face normal oriented out of the element
+  const double lenScale = a.econst[(size_t) e * kElemConst + dir];
+  const double tau = a.tauFact / lenScale;
+  for (int m = tid; m < N2; m += 64) {
+    double VL[5], VR[5], gx[5], gy[5], gz[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      VL[v] = sm.S[0][v * N2 + m];
+      VR[v] = sm.Q[0][v * N2 + m];
+      const int pt = a.bug ? (v < N2 ? v : 0) : m;
+      double c[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) c[q] = g[(4 + q) * N2 + pt];
+      double pl[3], pr[3];
+      pl[pN] = sm.S[1][v * N2 + m]; pl[pF] = sm.S[2][v * N2 + m]; pl[pS] = sm.S[3][v * N2 + m];
+      pr[pN] = sm.Q[1][v * N2 + m]; pr[pF] = sm.Q[2][v * N2 + m]; pr[pS] = sm.Q[3][v * N2 + m];
+      const double xL = pl[0] * c[0] + pl[1] * c[3] + pl[2] * c[6], xR = pr[0] * c[0] + pr[1] * c[3] + pr[2] * c[6];
+      const double yL = pl[0] * c[1] + pl[1] * c[4] + pl[2] * c[7], yR = pr[0] * c[1] + pr[1] * c[4] + pr[2] * c[7];
+      const double zL = pl[0] * c[2] + pl[1] * c[5] + pl[2] * c[8], zR = pr[0] * c[2] + pr[1] * c[5] + pr[2] * c[8];
+      gx[v] = 0.5 * (xL + xR); gy[v] = 0.5 * (yL + yR); gz[v] = 0.5 * (zL + zR);
+    }
+    const double nn[3] = {fn * g[m], fn * g[N2 + m], fn * g[2 * N2 + m]};
+    const double wA = sm.W2[m] * g[3 * N2 + m];
+    double fl[5], FX[5], FY[5], FZ[5];
+    point_flux(VL, VR, gx, gy, gz, nn, wA, tau, a.mu, a.thetaHalf, fl, FX, FY, FZ);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      const int pt = a.bug ? (v < N2 ? v : 0) : m;
+      double c[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) c[q] = g[(4 + q) * N2 + pt];
+      double par[3];
+      par[0] = FX[v] * c[0] + FY[v] * c[1] + FZ[v] * c[2];
+      par[1] = FX[v] * c[3] + FY[v] * c[4] + FZ[v] * c[5];
+      par[2] = FX[v] * c[6] + FY[v] * c[7] + FZ[v] * c[8];
+      sm.X[0][v * N2 + m] = fl[v];
+      sm.Sym[0][v * N2 + m] = par[pN];
+      sm.Sym[1][v * N2 + m] = par[pF];
+      sm.Sym[2][v * N2 + m] = par[pS];
+    }
+  }
+  __syncthreads();
+  // lift: fast axis, slow axis, then the face vectors along the normal
+  double *T0 = sm.Q[0], *T1 = sm.Q[1], *T2 = sm.Q[2];
+  for (int pc = tid; pc < 5 * M; pc += 64) {
+    double xf[M], xF[M], xS[M], xN[M];
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+      xf[q] = sm.X[0][pc * M + q]; xF[q] = sm.Sym[1][pc * M + q]; xS[q] = sm.Sym[2][pc * M + q]; xN[q] = sm.Sym[0][pc * M + q];
+    }
+    for (int n = 0; n < K; ++n) {
+      double p1 = 0.0, p2 = 0.0, p3 = 0.0;
+#pragma unroll
+      for (int q = 0; q < M; ++q) {
+        p1 += sm.A[q * K + n] * xf[q] + sm.D[q * K + n] * xF[q];
+        p2 += sm.A[q * K + n] * xS[q];
+        p3 += sm.A[q * K + n] * xN[q];
+      }
+      T0[pc * K + n] = p1; T1[pc * K + n] = p2; T2[pc * K + n] = p3;
+    }
+  }
+  __syncthreads();
+  for (int pc = tid; pc < 5 * K; pc += 64) {
+    const int v = pc / K, fi = pc - v * K;
+    double x1[M], x2[M], x3[M];
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+      const int j = (v * M + q) * K + fi;
+      x1[q] = T0[j]; x2[q] = T1[j]; x3[q] = T2[j];
+    }
+    for (int n = 0; n < K; ++n) {
+      double g1 = 0.0, g2 = 0.0;
+#pragma unroll
+      for (int q = 0; q < M; ++q) { g1 += sm.A[q * K + n] * x1[q] + sm.D[q * K + n] * x2[q]; g2 += sm.A[q * K + n] * x3[q]; }
+      sm.B[0][(v * K + n) * K + fi] = g1; sm.B[1][(v * K + n) * K + fi] = g2;
+    }
+  }
+  __syncthreads();
+  double *out = a.faceRes + ((size_t) e * 6 + f) * 5 * ND;
+  for (int idx = tid; idx < 5 * ND; idx += 64) {
+    const int v = idx / ND, d = idx - v * ND;
+    const int i = d % K, j = (d / K) % K, k = d / (K * K);
+    const int tn = dir == 0 ? i : dir == 1 ? j : k;
+    const int fa = dir == 0 ? j : i;
+    const int sl = dir == 2 ? j : k;
+    const int gi = (v * K + sl) * K + fa;
+    out[idx] = sm.F[0][tn] * sm.B[0][gi] + sm.F[1][tn] * sm.B[1][gi];
+  }
+}
+
 // ======================================================= time-step kernel
 struct DtArgs {
   const double *U, *tables, *econst;
@@ -1031,6 +1727,20 @@ hipError_t faces_impl(const DeviceCase &dc, hipStream_t s)
 }
 
 template <int K, int M>
+hipError_t boundary_impl(const DeviceCase &dc, hipStream_t s)
+{
+  if (dc.nBFace == 0) return hipSuccess;
+  BndArgs<K, M> a;
+  a.U = dc.U; a.tables = dc.tables; a.econst = dc.econst; a.bgeo = dc.bgeo; a.bface = dc.bface;
+  a.machPart = dc.machPart; a.faceRes = dc.faceRes; a.to = dc.to; a.bug = dc.bug_compat;
+  a.mu = dc.mu; a.tauFact = (dc.P + 1.0) * (dc.P + 1.0) * dc.rel_penalty; a.thetaHalf = 0.5 * dc.theta_sym;
+  a.nb = dc.nscbc;
+  hipLaunchKernelGGL((mach_kernel<K, M>), dim3(dc.nBFace), dim3(64), 0, s, a);
+  hipLaunchKernelGGL((nscbc_kernel<K, M>), dim3(dc.nBFace), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+template <int K, int M>
 hipError_t volume_impl(const DeviceCase &dc, const StageCtl &ctl, hipStream_t s)
 {
   VolArgs<K, M> a;
@@ -1038,6 +1748,8 @@ hipError_t volume_impl(const DeviceCase &dc, const StageCtl &ctl, hipStream_t s)
   a.U = dc.U; a.U0 = dc.U0; a.R = dc.R; a.faceRes = dc.faceRes; a.tables = dc.tables;
   a.vgeo = dc.vgeo; a.econst = dc.econst; a.scal = dc.scal; a.flags = dc.flags; a.cgIter = dc.cgIter;
   a.to = dc.to; a.nElem = dc.nElem; a.bug = dc.bug_compat; a.mu = dc.mu; a.ctl = ctl;
+  a.spIdx = dc.nSponge ? dc.spIdx : nullptr; a.spCoef = dc.spCoef; a.dampFn = dc.dampFn;
+  a.dampState = dc.dampState; a.srcBuf = dc.srcBuf; a.cml = dc.cml;
   hipLaunchKernelGGL((volume_kernel<K, M, 256>), dim3(dc.nElem), dim3(256), 0, s, a);
   return hipGetLastError();
 }
@@ -1068,6 +1780,14 @@ bool supported_pair(int K, int M)
 hipError_t launch_faces(const DeviceCase &dc, hipStream_t s)
 {
 #define X(k, m) if (dc.K == k && dc.M == m) return faces_impl<k, m>(dc, s);
+  VCP_PAIRS(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_boundary(const DeviceCase &dc, hipStream_t s)
+{
+#define X(k, m) if (dc.K == k && dc.M == m) return boundary_impl<k, m>(dc, s);
   VCP_PAIRS(X)
 #undef X
   return hipErrorInvalidValue;

@@ -7,7 +7,19 @@ namespace vcp {
 
 // Offsets (in doubles) of the 1D tables in the device table block.
 struct TableOffsets {
-  int A, D, AT, DT, LD, fmin, fmax, dfmin, dfmax, w3, w2, bc, total;
+  int A, D, AT, DT, LD, fmin, fmax, dfmin, dfmax, w3, w2, bc, P1, dLq, total;
+};
+
+// PC-NSCBC parameters of the block faces (InputParamClass "NSCBC tuning
+// parameters" and the prescribed data, non-dimensional).
+struct NscbcParams {
+  double outlet_len, outlet_sigma, outlet_beta_l, outlet_beta_t;
+  double inlet_len, inlet_sigma, dell0;
+  int outlet_one_minus_beta, inlet_incoming;
+  int bc[6];
+  int begin[7];                 // boundary faces of block face f: [begin[f], begin[f+1])
+  double data[6][5];
+  double invArea[6];
 };
 
 // Per-face geometry slot layout, [nElem][3][kFaceGeo][nInt2D]:
@@ -41,6 +53,19 @@ struct DeviceCase {
   int *flags = nullptr;
   int *cgIter = nullptr;          // [nElem]
   double *dtElem = nullptr;       // [nElem]
+  // open boundaries
+  int nBFace = 0;
+  int *bface = nullptr;           // [nBFace][2]: element, block face
+  double *bgeo = nullptr;         // [nBFace][13][nInt2D]: the face's normal, area factor, inverse metric
+  double *machPart = nullptr;     // [nBFace] weighted face-normal Mach number
+  NscbcParams nscbc{};
+  // sponge layer / characteristic matching layer
+  int nSponge = 0, cml = 0;
+  int *spIdx = nullptr;           // [nElem] slot or -1
+  double *spCoef = nullptr;       // [nSponge][8]
+  double *dampFn = nullptr;       // [nSponge][3][nInt]
+  double *dampState = nullptr;    // [nSponge][5][nInt]
+  double *srcBuf = nullptr;       // [nSponge][5][nInt] source terms of the current residual
 };
 
 // Stage control for the fused volume/mass-matrix kernel.
@@ -53,6 +78,8 @@ struct StageCtl {
 
 bool supported_pair(int K, int M);
 hipError_t launch_faces(const DeviceCase &dc, hipStream_t s);
+// boundary-averaged Mach number partial sums, then the PC-NSCBC faces
+hipError_t launch_boundary(const DeviceCase &dc, hipStream_t s);
 hipError_t launch_volume(const DeviceCase &dc, const StageCtl &ctl, hipStream_t s);
 hipError_t launch_time_step(const DeviceCase &dc, hipStream_t s);
 hipError_t launch_stage0(const DeviceCase &dc, double dt, hipStream_t s);

@@ -6,6 +6,12 @@
 //   ElementClass::ComputeLengthScales           ElementClass.cpp:927-1022
 //   ElementClass::LGLDistribution               ElementClass.cpp:3406-3623
 //   InputParamClass::ComputeReferenceConditions InputParamClass.cpp:605-758
+//   StandardElementClass::AssemblePolynomialCorrectionOperators
+//                                               StandardElementClass.cpp:176-322
+//   SolverClass::InitNSCBC, ConfigureParamNSCBC SolverClass.cpp:2447-2497,
+//                                               SubfaceClasses.cpp:1325-1384,
+//                                               BoundaryState.cpp:829-925
+//   SolverClass::InitSpongeLayer and friends    SpongeLayer.cpp:13-997
 // (paths relative to VCP3D_CurviLinear/src/).
 #include "setup.hpp"
 
@@ -266,6 +272,63 @@ void build_tables(const CaseParams &p, Tables &t)
   for (int k = 0; k < K; ++k)
     for (int j = 0; j < K; ++j)
       for (int i = 0; i < K; ++i) t.basis_center[(k * K + j) * K + i] = c0[i] * c0[j] * c0[k];
+  // Lagrange tables for the open-boundary and sponge terms
+  t.Lq.assign(M * K, 0.0); t.dLq.assign(M * M, 0.0);
+  for (int pt = 0; pt < M; ++pt) {
+    lagrange_basis(t.r_dof, t.r_int[pt], &t.Lq[pt * K], nullptr);
+    lagrange_basis(t.r_int, t.r_int[pt], nullptr, &t.dLq[pt * M]);
+  }
+  std::vector<double> dl(K);
+  lagrange_basis(t.r_dof, t.r_dof[0], nullptr, dl.data());
+  t.dell0 = -dl[0];
+  // projector onto the polynomials of degree <= P sampled at the Gauss
+  // points: Q Q^T with Q the (twice) Gram-Schmidt orthonormalised columns of
+  // the Legendre table A. The 2D polynomial-correction matrix of the
+  // reference is (P1 x P1) / dell0 (a Kronecker product of this projector).
+  std::vector<double> Q(M * K);
+  for (int q = 0; q < K; ++q) {
+    for (int pt = 0; pt < M; ++pt) Q[pt * K + q] = t.A[pt * K + q];
+    for (int pass = 0; pass < 2; ++pass)
+      for (int r = 0; r < q; ++r) {
+        double d = 0.0;
+        for (int pt = 0; pt < M; ++pt) d += Q[pt * K + r] * Q[pt * K + q];
+        for (int pt = 0; pt < M; ++pt) Q[pt * K + q] -= d * Q[pt * K + r];
+      }
+    double nrm = 0.0;
+    for (int pt = 0; pt < M; ++pt) nrm += Q[pt * K + q] * Q[pt * K + q];
+    nrm = 1.0 / std::sqrt(nrm);
+    for (int pt = 0; pt < M; ++pt) Q[pt * K + q] *= nrm;
+  }
+  t.P1.assign(M * M, 0.0);
+  for (int a = 0; a < M; ++a)
+    for (int b = 0; b < M; ++b) {
+      double v = 0.0;
+      for (int q = 0; q < K; ++q) v += Q[a * K + q] * Q[b * K + q];
+      t.P1[a * M + b] = v;
+    }
+}
+
+std::string check_case(const CaseParams &p)
+{
+  for (int f = 0; f < 6; ++f) {
+    const bool per = p.periodic[f / 2] != 0;
+    if (per && p.bc[f] != kBCPeriodic) return "a periodic direction cannot carry a physical boundary";
+    if (!per && (p.bc[f] < kBCOutflowNSCBC || p.bc[f] > kBCInflowTotalNSCBC))
+      return "unsupported boundary condition on a non-periodic block face";
+  }
+  if (p.outlet_average_local)
+    return "\"Outlet use local (element) averaged Mach number\" is not supported";
+  if (p.cml && !p.sponge_layer) return "a characteristic matching layer needs the sponge layer";
+  if (p.sponge_layer) {
+    if (p.n_poly_grid != p.n_poly_sol) return "sponge layer needs equal grid and solution degree";
+    bool any = false;
+    for (int f = 0; f < 6; ++f) {
+      if (p.sponge_elems[f] < 0) return "Sponge layer cannot have negative number of elements";
+      any |= p.sponge_elems[f] > 0;
+    }
+    if (!any) return "No sponge layer elements have been detected in a sponge layer simulation.";
+  }
+  return "";
 }
 
 std::string build_geometry(const CaseParams &p, Tables &t, const double *grid, Geometry &g)
@@ -383,15 +446,150 @@ std::string build_geometry(const CaseParams &p, Tables &t, const double *grid, G
   return "";
 }
 
+// Pressure from an outlet, density from a static inlet (or the total
+// temperature of a total-condition inlet) fix the reference state; a block
+// without such data uses the free stream (Mach, Reynolds, temperature).
 RefConditions reference_conditions(const CaseParams &p)
 {
   RefConditions r{};
-  const double vInf = p.mach * std::sqrt(kGamma * kRGas * p.temp_free_stream);
-  r.rho_ref = p.reynolds * kMuDim / (p.reynolds_length * vInf);
-  r.p_ref = r.rho_ref * kRGas * p.temp_free_stream;
+  double pOut = -1.0, pTot = -1.0, rhoIn = -1.0, TTot = -1.0;
+  for (int f = 0; f < 6; ++f) {
+    if (p.periodic[f / 2]) continue;
+    if (p.bc[f] == kBCOutflowNSCBC) pOut = p.bc_data[f][0];
+    if (p.bc[f] == kBCInflowStaticNSCBC) rhoIn = p.bc_data[f][0];
+    if (p.bc[f] == kBCInflowTotalNSCBC) { pTot = p.bc_data[f][0]; TTot = p.bc_data[f][1]; }
+  }
+  r.p_ref = pOut > 0.0 ? pOut : pTot > 0.0 ? pTot : -1.0;
+  r.rho_ref = rhoIn > 0.0 ? rhoIn : (r.p_ref > 0.0 && TTot > 0.0) ? r.p_ref / (kRGas * TTot) : -1.0;
+  if (r.p_ref <= 0.0 || r.rho_ref <= 0.0) {
+    const double vInf = p.mach * std::sqrt(kGamma * kRGas * p.temp_free_stream);
+    r.rho_ref = p.reynolds * kMuDim / (p.reynolds_length * vInf);
+    r.p_ref = r.rho_ref * kRGas * p.temp_free_stream;
+  }
   r.u_ref = std::sqrt(r.p_ref / r.rho_ref);
   r.mu = kMuDim * r.u_ref / r.p_ref;
   return r;
+}
+
+std::string setup_open_boundary(const CaseParams &p, const Tables &t, const Geometry &g,
+                                const RefConditions &rc, OpenBoundary &ob)
+{
+  const int nElem = p.n_elem[0] * p.n_elem[1] * p.n_elem[2], n2 = t.M * t.M;
+  ob = OpenBoundary();
+  const double TRef = rc.p_ref / rc.rho_ref;
+  for (int f = 0; f < 6; ++f) {
+    ob.begin[f] = (int) ob.faces.size() / 2;
+    if (p.periodic[f / 2]) continue;
+    ob.any = true;
+    const double *d = p.bc_data[f];
+    double *o = ob.data[f];
+    if (p.bc[f] == kBCOutflowNSCBC) {
+      o[0] = d[0] / rc.p_ref;
+    } else if (p.bc[f] == kBCInflowStaticNSCBC) {
+      o[0] = d[0] / rc.rho_ref;
+      for (int c = 1; c < 4; ++c) o[c] = d[c] / rc.u_ref;
+    } else {
+      const double len = std::sqrt(d[2] * d[2] + d[3] * d[3] + d[4] * d[4]);
+      if (!(len > 0.0)) return "total-condition inflow needs a velocity direction";
+      o[0] = d[0] / rc.p_ref; o[1] = d[1] / TRef;
+      for (int c = 2; c < 5; ++c) o[c] = d[c] / len;
+    }
+    int first = -1;
+    for (int e = 0; e < nElem; ++e) {
+      if (g.nbr[(size_t) e * 6 + f] >= 0) continue;
+      if (first < 0) first = e;
+      ob.faces.push_back(e);
+      ob.faces.push_back(f);
+      const double *sm = &g.face[f][(size_t) e * 16 * n2];
+      for (int l = 0; l < n2; ++l) ob.area[f] += t.w2[l] * sm[3 * n2 + l];
+    }
+    // orientation: the parametric direction whose gradient is parallel to
+    // the face normal (integration point 0 of the first boundary element)
+    const double *sm = &g.face[f][(size_t) first * 16 * n2];
+    int normal = -1;
+    for (int i = 0; i < 3; ++i) {
+      const double *gr = sm + (size_t) (4 + 3 * i) * n2;
+      const double len = std::sqrt(gr[0] * gr[0] + gr[n2] * gr[n2] + gr[2 * n2] * gr[2 * n2]);
+      const double ct = (sm[0] * gr[0] + sm[n2] * gr[n2] + sm[2 * n2] * gr[2 * n2]) / len;
+      if (std::fabs(ct - 1.0) < 1e-10) normal = i;
+    }
+    if (normal < 0) return "Could not identify relevant boundary indices!";
+    if (normal != f / 2) return "NSCBC face whose normal is not along its own parametric direction";
+    ob.ig[f][0] = normal;
+    ob.ig[f][1] = normal == 0 ? 1 : 0;
+    ob.ig[f][2] = normal == 2 ? 1 : 2;
+  }
+  ob.begin[6] = (int) ob.faces.size() / 2;
+  return "";
+}
+
+std::string setup_sponge(const CaseParams &p, const Tables &t, const Geometry &g, const double *grid,
+                         Sponge &sp)
+{
+  sp = Sponge();
+  if (!p.sponge_layer) return "";
+  const int ni = p.n_elem[0], nj = p.n_elem[1], nk = p.n_elem[2], nElem = ni * nj * nk;
+  const int n[3] = {ni, nj, nk};
+  const int K = t.K, nInt = t.M * t.M * t.M, nG = t.Kg * t.Kg * t.Kg;
+  std::vector<int> flag(nElem, 0);
+  sp.idx.assign(nElem, -1);
+  for (int e = 0; e < nElem; ++e) {
+    const int idx[3] = {e % ni, (e / ni) % nj, e / (ni * nj)};
+    for (int d = 0; d < 3; ++d) {
+      if (idx[d] < p.sponge_elems[2 * d]) flag[e] |= 1 << (2 * d);
+      if (idx[d] >= n[d] - p.sponge_elems[2 * d + 1]) flag[e] |= 1 << (2 * d + 1);
+    }
+    if (flag[e]) sp.idx[e] = sp.n++;
+  }
+  sp.fn.assign((size_t) sp.n * 3 * nInt, 0.0);
+  sp.coef.assign((size_t) sp.n * 8, 0.0);
+  for (int e = 0; e < nElem; ++e)
+    if (sp.idx[e] >= 0)
+      for (int d = 0; d < 3; ++d) sp.coef[(size_t) sp.idx[e] * 8 + 3 + d] = (flag[e] >> (2 * d)) & 1 ? -1.0 : 1.0;
+  for (int f = 0; f < 6; ++f) {
+    if (p.sponge_elems[f] <= 0) continue;
+    const int d = f / 2, bit = 1 << f;
+    const int last = p.n_poly_sol * (d == 0 ? 1 : d == 1 ? K : K * K);   // node index P along d
+    double lo = 1e300, hi = -1e300;
+    for (int e = 0; e < nElem; ++e) {
+      if (!(flag[e] & bit)) continue;
+      const double *ge = grid + ((size_t) e * 3 + d) * nG;
+      lo = std::min(lo, ge[0]);
+      hi = std::max(hi, ge[last]);
+    }
+    const double inv = 1.0 / (hi - lo), x0 = (f % 2) ? lo : hi;
+    const double sig = p.damping_constant[f], alpha = p.damping_exponent[f];
+    for (int e = 0; e < nElem; ++e) {
+      if (!(flag[e] & bit)) continue;
+      const double *x = &g.vol[((size_t) e * 13 + 10 + d) * nInt];
+      double *fn = &sp.fn[((size_t) sp.idx[e] * 3 + d) * nInt];
+      for (int l = 0; l < nInt; ++l) {
+        if ((f % 2) ? x[l] < x0 : x[l] > x0) return "Inconsistent sponge layer location.";
+        fn[l] = sig * std::pow(inv * std::fabs(x[l] - x0), alpha);
+      }
+      if (p.cml) sp.coef[(size_t) sp.idx[e] * 8 + d] = std::fabs(sig) < 1.0e-4 ? 1.0 : 1.0 / sig;
+    }
+  }
+  return "";
+}
+
+void damping_state(const Tables &t, const RefConditions &rc, const double *nodal_prim, double *out)
+{
+  const int K = t.K, M = t.M, nD = K * K * K, nInt = M * M * M;
+  std::vector<double> node((size_t) kNVar * nD), q((size_t) kNVar * nInt);
+  for (int d = 0; d < nD; ++d)
+    for (int l = 0; l < kNVar; ++l) node[(size_t) l * nD + d] = nodal_prim[(size_t) d * kNVar + l];
+  for (int l = 0; l < kNVar; ++l)
+    apply3(K, K, K, M, M, M, t.Lq.data(), t.Lq.data(), t.Lq.data(), &node[(size_t) l * nD], &q[(size_t) l * nInt]);
+  for (int m = 0; m < nInt; ++m) {
+    const double rho = q[m] / rc.rho_ref, u = q[nInt + m] / rc.u_ref, v = q[2 * nInt + m] / rc.u_ref,
+                 w = q[3 * nInt + m] / rc.u_ref, pr = q[4 * nInt + m] / rc.p_ref;
+    out[m] = rho;
+    out[nInt + m] = rho * u;
+    out[2 * nInt + m] = rho * v;
+    out[3 * nInt + m] = rho * w;
+    out[4 * nInt + m] = pr / (kGamma - 1.0) + 0.5 * rho * (u * u + v * v + w * w);
+  }
 }
 
 void nodal_cons_to_modal_entropy(const Tables &t, const RefConditions &rc, const double *nodal, double *modal)

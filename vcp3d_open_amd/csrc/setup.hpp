@@ -15,6 +15,9 @@ constexpr double kRGas = 287.058;         // VCP3D_CurviLinear.hpp:177
 constexpr double kLambdaOverMu = -2.0 / 3.0;
 constexpr double kMuDim = 1.716e-5;       // InputParamClass.cpp:607
 
+// Physical block-face kinds (one subface per block face).
+enum : int { kBCPeriodic = 0, kBCOutflowNSCBC = 1, kBCInflowStaticNSCBC = 2, kBCInflowTotalNSCBC = 3 };
+
 // Input parameters of the path; field meaning follows InputParamClass.
 struct CaseParams {
   int n_elem[3] = {0, 0, 0};
@@ -24,6 +27,18 @@ struct CaseParams {
   double mach = 0, reynolds = 0, reynolds_length = 1, temp_free_stream = 0;
   double theta_sym = 1, rel_penalty = 1, cfl = 1, dt_synchr = 0;
   int bug_compat = 1;
+  // physical faces iMin..kMax (used where periodic[f/2] == 0)
+  int bc[6] = {0, 0, 0, 0, 0, 0};
+  double bc_data[6][5] = {};   // dimensional prescribed data
+  // "NSCBC tuning parameters"
+  double outlet_len = 1, outlet_sigma = 0, outlet_beta_l = 1, outlet_beta_t = 1;
+  int outlet_one_minus_beta = 1, outlet_average_local = 0;
+  double inlet_len = 1, inlet_sigma = 1;
+  int inlet_incoming = 1;
+  // "Sponge layer", "Characteristic matching layer", sponge block
+  int sponge_layer = 0, cml = 0;
+  int sponge_elems[6] = {0, 0, 0, 0, 0, 0};
+  double damping_constant[6] = {}, damping_exponent[6] = {};
 };
 
 struct Tables {
@@ -36,6 +51,11 @@ struct Tables {
   std::vector<double> VI;         // inverse of LD
   std::vector<double> f_min, f_max, df_min, df_max;  // K
   std::vector<double> w3, w2, basis_center;
+  // open boundaries / sponge
+  std::vector<double> Lq;         // M x K: nodal DOF Lagrange basis at the Gauss points
+  std::vector<double> dLq;        // M x M: Gauss-point Lagrange derivative, [point][basis]
+  std::vector<double> P1;         // M x M: orthogonal projector onto degree <= P on the Gauss points
+  double dell0 = 0;               // -l'_0(r_0) of the nodal DOF basis
 };
 
 struct Geometry {
@@ -49,12 +69,39 @@ struct Geometry {
 
 struct RefConditions { double p_ref, rho_ref, u_ref, mu; };
 
+// PC-NSCBC faces of the block.
+struct OpenBoundary {
+  bool any = false;
+  int ig[6][3] = {};              // parametric slots: normal, first, second transverse
+  double data[6][5] = {};         // non-dimensional prescribed data
+  double area[6] = {};            // boundary surface area
+  std::vector<int> faces;         // [nFaces][2]: element, block face; grouped by face
+  int begin[7] = {};              // faces of block face f: [begin[f], begin[f+1])
+};
+
+// Sponge layer / characteristic matching layer elements.
+struct Sponge {
+  int n = 0;
+  std::vector<int> idx;           // [nElem]: slot or -1
+  std::vector<double> fn;         // [n][3][nInt] damping functions
+  std::vector<double> coef;       // [n][8]: CML scale per direction, layer sign per direction
+};
+
 int n_int1d(int rule);
 void build_tables(const CaseParams &p, Tables &t);
+// Consistency of the boundary, NSCBC and sponge parameters ("" if valid).
+std::string check_case(const CaseParams &p);
 // Computes metrics from per-element grid DOFs [nElem][3][Kg^3]; returns an
 // error message or "" on success.
 std::string build_geometry(const CaseParams &p, Tables &t, const double *elem_grid, Geometry &g);
 RefConditions reference_conditions(const CaseParams &p);
+std::string setup_open_boundary(const CaseParams &p, const Tables &t, const Geometry &g,
+                                const RefConditions &rc, OpenBoundary &ob);
+std::string setup_sponge(const CaseParams &p, const Tables &t, const Geometry &g, const double *elem_grid,
+                         Sponge &sp);
+// Averaged nodal primitive state of one element (dimensional, [nDOFs][5])
+// -> conservative non-dimensional damping target at the points, [5][nInt].
+void damping_state(const Tables &t, const RefConditions &rc, const double *nodal_prim, double *out);
 
 // Nodal dimensional conservative [nDOFs][5] of one element -> modal entropy
 // variables [5][nDOFs] (SolverClass::ReadRestartSolution).

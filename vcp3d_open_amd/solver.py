@@ -31,10 +31,21 @@ class CaseDesc(C.Structure):
         ("theta_sym", C.c_double), ("rel_penalty", C.c_double),
         ("cfl", C.c_double), ("dt_synchr", C.c_double),
         ("bug_compat", C.c_int),
+        ("bc", C.c_int * 6),
+        ("bc_data", (C.c_double * 5) * 6),
+        ("outlet_len", C.c_double), ("outlet_sigma", C.c_double),
+        ("outlet_beta_l", C.c_double), ("outlet_beta_t", C.c_double),
+        ("outlet_one_minus_beta", C.c_int), ("outlet_average_local", C.c_int),
+        ("inlet_len", C.c_double), ("inlet_sigma", C.c_double),
+        ("inlet_incoming", C.c_int),
+        ("sponge_layer", C.c_int), ("cml", C.c_int),
+        ("sponge_elems", C.c_int * 6),
+        ("damping_constant", C.c_double * 6), ("damping_exponent", C.c_double * 6),
     ]
 
 
 EXPORTS = ["vcp_create", "vcp_destroy", "vcp_sizes", "vcp_reference", "vcp_set_restart",
+           "vcp_set_damping_state", "vcp_nscbc_average",
            "vcp_set_solution", "vcp_get_solution", "vcp_get_nodal", "vcp_residual",
            "vcp_get_residual", "vcp_sync_step", "vcp_time_steps", "vcp_cg_iterations",
            "vcp_profile_residual", "vcp_host_setup", "vcp_last_error"]
@@ -58,6 +69,8 @@ def load_library(path: str = LIB_PATH):
     L.vcp_sizes.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.vcp_reference.argtypes = [vp, dp]
     L.vcp_set_restart.argtypes = [vp, dp]
+    L.vcp_set_damping_state.argtypes = [vp, dp]
+    L.vcp_nscbc_average.argtypes = [vp, dp]
     L.vcp_set_solution.argtypes = [vp, dp]
     L.vcp_get_solution.argtypes = [vp, dp]
     L.vcp_get_nodal.argtypes = [vp, dp]
@@ -81,17 +94,23 @@ class VCPError(RuntimeError):
 
 
 def desc_from_case(case, bug_compat: bool = True) -> CaseDesc:
+    """vcp_case_desc from a case's description() dict (fields not named there
+    stay 0)."""
     d = CaseDesc()
-    d.n_elem[:] = list(case.n_elem)
-    d.n_poly_grid = case.p
-    d.n_poly_sol = case.p
-    d.n_poly_int_rule = case.rule
-    d.dof_location = 2
-    d.periodic[:] = [1, 1, 1]
-    d.mach, d.reynolds, d.reynolds_length = case.mach, case.reynolds, 1.0
-    d.temp_free_stream = case.temp
-    d.theta_sym, d.rel_penalty = case.theta_sym, case.rel_penalty
-    d.cfl, d.dt_synchr = case.cfl, case.dt_synchr
+    for name, _ in CaseDesc._fields_:
+        val = case.description().get(name) if name != "bug_compat" else None
+        if val is None:
+            continue
+        attr = getattr(d, name)
+        if hasattr(attr, "__len__"):
+            for i, v in enumerate(val):
+                if hasattr(attr[i], "__len__"):
+                    for j, w in enumerate(v):
+                        attr[i][j] = w
+                else:
+                    attr[i] = v
+        else:
+            setattr(d, name, val)
     d.bug_compat = 1 if bug_compat else 0
     return d
 
@@ -163,6 +182,17 @@ class SolverClass:
         """Nodal dimensional conservative payload of the restart file."""
         out = np.zeros((self.n_elem, self.n_dofs, 5))
         self._check(self._L.vcp_get_nodal(self.h, out))
+        return out
+
+    def SetDampingState(self, nodal_prim: np.ndarray) -> None:
+        """Averaged nodal primitive state of the sponge layer
+        (SolverClass::ReadAveragedSolutionSpongeLayer)."""
+        self._check(self._L.vcp_set_damping_state(self.h, np.ascontiguousarray(nodal_prim, dtype=np.float64)))
+
+    def AverageDataNSCBC(self) -> np.ndarray:
+        """Boundary-averaged face-normal Mach number of the six block faces."""
+        out = np.zeros(6)
+        self._check(self._L.vcp_nscbc_average(self.h, out))
         return out
 
     # -- direct access ------------------------------------------------------

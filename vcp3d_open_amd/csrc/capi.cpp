@@ -175,6 +175,7 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
     dc.opF[0][q] = t.f_min[q]; dc.opF[1][q] = t.f_max[q]; dc.opF[2][q] = t.df_min[q]; dc.opF[3][q] = t.df_max[q];
   }
   for (int q = 0; q < M; ++q) dc.opW[q] = t.w_int[q];
+  for (int q = 0; q < K; ++q) dc.opC0[q] = t.c0[q];
   dc.to.w3 = put(t.w3); dc.to.w2 = put(t.w2); dc.to.bc = put(t.basis_center);
   dc.to.P1 = put(t.P1); dc.to.dLq = put(t.dLq); dc.to.total = (int) tb.size();
   // open boundaries: face list, own face metric, parameters

@@ -118,6 +118,30 @@ __device__ inline void contract2(const double *__restrict__ in1, const double *_
   }
 }
 
+// Block reduction with one barrier: partial results go to alternating
+// halves of red (2 * NT/64 * NV doubles), so a half is rewritten only after
+// every thread has passed the barrier of the following reduction.
+template <int NT, int NV, int NS>
+__device__ inline void block_reduce1(double (&v)[NV], double *red, int &half)
+{
+#pragma unroll
+  for (int q = 0; q < NV; ++q) v[q] = (q < NS) ? wave_sum(v[q]) : wave_max(v[q]);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double *buf = red + half * (NT / 64) * NV;
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) buf[wid * NV + q] = v[q];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double s = buf[q];
+#pragma unroll
+    for (int w = 1; w < NT / 64; ++w) s = (q < NS) ? s + buf[w * NV + q] : fmax(s, buf[w * NV + q]);
+    v[q] = s;
+  }
+  half ^= 1;
+}
+
 // In-place contraction of NROWS contiguous rows of length NIN -> NOUT.
 template <int NIN, int NOUT, int NROWS, int NT>
 __device__ inline void rows_inplace(double *buf, const double *__restrict__ op)
@@ -136,6 +160,9 @@ __device__ inline void rows_inplace(double *buf, const double *__restrict__ op)
   }
 }
 
+// t^(-1/(gamma-1)) = t^(-5/2) for gamma = 1.4, as 1 / (t^2 sqrt(t)).
+__device__ inline double pow_m2p5(double t) { return 1.0 / (t * t * sqrt(t)); }
+
 __device__ inline void ent2prim(const double V[5], double &rho, double &u, double &v, double &w,
                                 double &p, double &eKin)
 {
@@ -143,7 +170,7 @@ __device__ inline void ent2prim(const double V[5], double &rho, double &u, doubl
   u = -V4Inv * V[1]; v = -V4Inv * V[2]; w = -V4Inv * V[3];
   eKin = 0.5 * (u * u + v * v + w * w);
   const double s = kGam - kGm1 * (V[0] - V[4] * eKin);
-  rho = pow(-V[4] * exp(s), -1.0 / kGm1);
+  rho = pow_m2p5(-V[4] * exp(s));
   p = -rho * V4Inv;
 }
 
@@ -154,6 +181,123 @@ __device__ inline void atomic_max_pos(double *addr, double v)
 __device__ inline void atomic_min_pos(double *addr, double v)
 {
   atomicMin(reinterpret_cast<unsigned long long *>(addr), (unsigned long long) __double_as_longlong(v));
+}
+
+// Ismail-Roe + viscous + interior-penalty + symmetrising fluxes at one face
+// point: VL/VR entropy states (L -> R along nn), gx/gy/gz the averaged
+// Cartesian gradients, wA = weight * area factor. fl: normal flux, FX/FY/FZ:
+// Cartesian symmetrising fluxes (FluxesFace.cpp:813-975, :1290-1651,
+// :1152-1284).
+__device__ __forceinline__ void point_flux(const double VL[5], const double VR[5], const double gx[5],
+                                           const double gy[5], const double gz[5], const double nn[3],
+                                           double wA, double tau, double mu, double thetaHalf,
+                                           double fl[5], double FX[5], double FY[5], double FZ[5])
+{
+  const double factHF = kGam / kPrLam;
+  const double lam = kLamOverMu, lamp1 = lam + 1.0, lamp2 = lam + 2.0;
+  {
+    double rL, uL, vL, wL, pL, eL, rR, uR, vR, wR, pR, eR;
+    ent2prim(VL, rL, uL, vL, wL, pL, eL);
+    ent2prim(VR, rR, uR, vR, wR, pR, eR);
+    const double dV0 = VR[0] - VL[0], dV1 = VR[1] - VL[1], dV2 = VR[2] - VL[2],
+                 dV3 = VR[3] - VL[3], dV4 = VR[4] - VL[4];
+    const double z0L = sqrt(rL / pL), z0R = sqrt(rR / pR);
+    const double z4L = sqrt(rL * pL), z4R = sqrt(rR * pR);
+    const double z0A = 0.5 * (z0L + z0R), z4A = 0.5 * (z4L + z4R);
+    const double z1A = 0.5 * (uL * z0L + uR * z0R), z2A = 0.5 * (vL * z0L + vR * z0R),
+                 z3A = 0.5 * (wL * z0L + wR * z0R);
+    double zeta = z0L / z0R, f = (zeta - 1.0) / (zeta + 1.0), uu = f * f, F;
+    F = (uu < 0.01) ? 1.0 + uu / 3.0 + uu * uu / 5.0 + uu * uu * uu / 7.0 + uu * uu * uu * uu / 9.0
+                    : log(zeta) / (2.0 * f);
+    const double z0Log = z0A / F;
+    zeta = z4L / z4R; f = (zeta - 1.0) / (zeta + 1.0); uu = f * f;
+    F = (uu < 0.01) ? 1.0 + uu / 3.0 + uu * uu / 5.0 + uu * uu * uu / 7.0 + uu * uu * uu * uu / 9.0
+                    : log(zeta) / (2.0 * f);
+    const double z4Log = z4A / F;
+    const double oz0 = 1.0 / z0A;
+    const double rA = z0A * z4Log, p1 = oz0 * z4A;
+    const double p2 = 0.5 * ((kGam + 1.0) / kGam * z4Log / z0Log + kGm1 / kGam * p1);
+    const double uA = oz0 * z1A, vA = oz0 * z2A, wAv = oz0 * z3A;
+    const double vn = uA * nn[0] + vA * nn[1] + wAv * nn[2];
+    const double kin = 0.5 * (uA * uA + vA * vA + wAv * wAv);
+    const double a2 = kGam * p2 / rA, aa = sqrt(a2);
+    const double H = a2 / kGm1 + kin, E = H - p2 / rA;
+    double abv1 = rA * (uA * dV1 + vA * dV2 + wAv * dV3);
+    double abv2 = abv1 + rA * (dV0 + H * dV4);
+    const double dr = abv1 + rA * (dV0 + E * dV4);
+    const double dru = uA * abv2 + p2 * dV1, drv = vA * abv2 + p2 * dV2, drw = wAv * abv2 + p2 * dV3;
+    const double drE = H * abv1 + rA * E * (dV0 + H * dV4) + p2 * kin * dV4;
+    const double fac = 0.5 * (1.0 + 1.0 / 6.0);
+    const double lam1 = fabs(vn + aa) * fac, lam2 = fabs(vn - aa) * fac, lam3 = 0.5 * fabs(vn);
+    abv1 = 0.5 * (lam1 + lam2); abv2 = 0.5 * (lam1 - lam2);
+    const double abv3 = abv1 - lam3;
+    const double abv4 = kGm1 * (kin * dr - uA * dru - vA * drv - wAv * drw + drE);
+    const double abv5 = nn[0] * dru + nn[1] * drv + nn[2] * drw - vn * dr;
+    const double abv6 = abv3 * abv4 / a2 + abv2 * abv5 / aa;
+    const double abv7 = abv2 * abv4 / aa + abv3 * abv5;
+    fl[0] = wA * (rA * vn - (lam3 * dr + abv6));
+    fl[1] = wA * (rA * vn * uA + p1 * nn[0] - (lam3 * dru + uA * abv6 + nn[0] * abv7));
+    fl[2] = wA * (rA * vn * vA + p1 * nn[1] - (lam3 * drv + vA * abv6 + nn[1] * abv7));
+    fl[3] = wA * (rA * vn * wAv + p1 * nn[2] - (lam3 * drw + wAv * abv6 + nn[2] * abv7));
+    fl[4] = wA * (rA * vn * H - (lam3 * drE + H * abv6 + vn * abv7));
+  }
+  const double V4Inv = 2.0 / (VL[4] + VR[4]);
+  const double u = -V4Inv * 0.5 * (VL[1] + VR[1]), vv = -V4Inv * 0.5 * (VL[2] + VR[2]),
+               w = -V4Inv * 0.5 * (VL[3] + VR[3]);
+  {
+    const double dudx = -V4Inv * (gx[1] + u * gx[4]), dvdx = -V4Inv * (gx[2] + vv * gx[4]), dwdx = -V4Inv * (gx[3] + w * gx[4]);
+    const double dudy = -V4Inv * (gy[1] + u * gy[4]), dvdy = -V4Inv * (gy[2] + vv * gy[4]), dwdy = -V4Inv * (gy[3] + w * gy[4]);
+    const double dudz = -V4Inv * (gz[1] + u * gz[4]), dvdz = -V4Inv * (gz[2] + vv * gz[4]), dwdz = -V4Inv * (gz[3] + w * gz[4]);
+    const double fe = V4Inv * V4Inv / kGm1;
+    const double dedx = fe * gx[4], dedy = fe * gy[4], dedz = fe * gz[4];
+    const double nx = wA * nn[0], ny = wA * nn[1], nz = wA * nn[2];
+    const double lamDiv = kLamOverMu * (dudx + dvdy + dwdz);
+    const double txx = mu * (2.0 * dudx + lamDiv), tyy = mu * (2.0 * dvdy + lamDiv), tzz = mu * (2.0 * dwdz + lamDiv);
+    const double txy = mu * (dudy + dvdx), txz = mu * (dudz + dwdx), tyz = mu * (dvdz + dwdy);
+    const double fh = mu * factHF;
+    const double qx = -fh * dedx, qy = -fh * dedy, qz = -fh * dedz;
+    fl[1] -= txx * nx + txy * ny + txz * nz;
+    fl[2] -= txy * nx + tyy * ny + tyz * nz;
+    fl[3] -= txz * nx + tyz * ny + tzz * nz;
+    fl[4] -= (u * txx + vv * txy + w * txz - qx) * nx + (u * txy + vv * tyy + w * tyz - qy) * ny
+           + (u * txz + vv * tyz + w * tzz - qz) * nz;
+  }
+  const double pOvRho = -V4Inv, eInt = pOvRho / kGm1;
+  const double dV1 = VL[1] - VR[1], dV2 = VL[2] - VR[2], dV3 = VL[3] - VR[3], dV4 = VL[4] - VR[4];
+  const double factFlux = pOvRho * wA * mu;
+  const double abvE = u * u + vv * vv + w * w + factHF * eInt;
+  const double un = u * nn[0] + vv * nn[1] + w * nn[2];
+  const double dVn = dV1 * nn[0] + dV2 * nn[1] + dV3 * nn[2];
+  const double ttau = tau * factFlux, abvMom = lamp1 * (dVn + un * dV4);
+  fl[1] += ttau * (abvMom * nn[0] + dV1 + u * dV4);
+  fl[2] += ttau * (abvMom * nn[1] + dV2 + vv * dV4);
+  fl[3] += ttau * (abvMom * nn[2] + dV3 + w * dV4);
+  fl[4] += ttau * (lamp1 * un * dVn + u * dV1 + vv * dV2 + w * dV3 + (abvE + lamp1 * un * un) * dV4);
+  const double sx = thetaHalf * factFlux * nn[0], sy = thetaHalf * factFlux * nn[1], sz = thetaHalf * factFlux * nn[2];
+  const double K11_14 = u * lamp2, K11_44 = abvE + u * u * lamp1;
+  const double K22_24 = vv * lamp2, K22_44 = abvE + vv * vv * lamp1;
+  const double K33_34 = w * lamp2, K33_44 = abvE + w * w * lamp1;
+  const double K12_14 = vv * lam, K12_42 = u * lam, K12_44 = u * vv * lamp1;
+  const double K13_14 = w * lam, K13_43 = u * lam, K13_44 = u * w * lamp1;
+  const double K23_24 = w * lam, K23_43 = vv * lam, K23_44 = vv * w * lamp1;
+  FX[0] = 0.0;
+  FX[1] = -(lamp2 * dV1 + K11_14 * dV4) * sx - (lam * dV2 + K12_14 * dV4) * sy - (lam * dV3 + K13_14 * dV4) * sz;
+  FX[2] = -(dV2 + vv * dV4) * sx - (dV1 + u * dV4) * sy;
+  FX[3] = -(dV3 + w * dV4) * sx - (dV1 + u * dV4) * sz;
+  FX[4] = -(K11_14 * dV1 + vv * dV2 + w * dV3 + K11_44 * dV4) * sx
+          - (vv * dV1 + K12_42 * dV2 + K12_44 * dV4) * sy - (w * dV1 + K13_43 * dV3 + K13_44 * dV4) * sz;
+  FY[0] = 0.0;
+  FY[1] = -(dV2 + vv * dV4) * sx - (dV1 + u * dV4) * sy;
+  FY[2] = -(lam * dV1 + K12_42 * dV4) * sx - (lamp2 * dV2 + K22_24 * dV4) * sy - (lam * dV3 + K23_24 * dV4) * sz;
+  FY[3] = -(dV3 + w * dV4) * sy - (dV2 + vv * dV4) * sz;
+  FY[4] = -(K12_14 * dV1 + u * dV2 + K12_44 * dV4) * sx
+          - (u * dV1 + K22_24 * dV2 + w * dV3 + K22_44 * dV4) * sy - (w * dV2 + K23_43 * dV3 + K23_44 * dV4) * sz;
+  FZ[0] = 0.0;
+  FZ[1] = -(dV3 + w * dV4) * sx - (dV1 + u * dV4) * sz;
+  FZ[2] = -(dV3 + w * dV4) * sy - (dV2 + vv * dV4) * sz;
+  FZ[3] = -(lam * dV1 + K13_43 * dV4) * sx - (lam * dV2 + K23_43 * dV4) * sy - (lamp2 * dV3 + K33_34 * dV4) * sz;
+  FZ[4] = -(K13_14 * dV1 + u * dV3 + K13_44 * dV4) * sx
+          - (K23_24 * dV2 + vv * dV3 + K23_44 * dV4) * sy - (u * dV1 + vv * dV2 + K33_34 * dV3 + K33_44 * dV4) * sz;
 }
 
 template <int K, int M>
@@ -174,16 +318,17 @@ struct FaceArgs {
 };
 
 // ============================================================ face kernel
-// One wave64 per element min face in direction DIR. Side 0 is
-// the element itself (its min face), side 1 the neighbour L (its max face);
-// the normal points from L to the element.
+// One wave64 per element min face in direction DIR. Side 0 is the element
+// itself (its min face), side 1 the neighbour L (its max face); the normal
+// points from L to the element. LDS regions are reused across the phases:
+//   R1: normal contraction B -> traces/gradients at the points -> lift stage 1
+//   R2: fast contraction T -> fluxes at the points -> lift stage 2
 template <int K, int M>
 struct FaceSmem {
   double A[M * K], D[M * K], W2[M * M];
-  double B[2][2][5 * K * K];      // side, (value | normal derivative)
-  double T[2][3][5 * K * M];      // side, (A, D, N) after the fast contraction
-  double Flux[5 * M * M], Sym[2][3][5 * M * M];   // sym slots (normal, fast, slow)
   double Cm[2][5][9];
+  double R1[(2 * 5 * M * M + 3 * 5 * M * M) > (2 * 3 * 5 * M * K) ? (5 * 5 * M * M) : (2 * 3 * 5 * M * K)];
+  double R2[(4 * 5 * M * M + 3 * 5 * M * M) > (2 * 3 * 5 * K * M) ? (7 * 5 * M * M) : (2 * 3 * 5 * K * M)];
 };
 
 template <int K, int M, int DIR>
@@ -194,15 +339,23 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<K, M> a)
   if (L < 0) return;  // physical boundary face
   __shared__ FaceSmem<K, M> sm;
   constexpr int ND = K * K * K, N2 = M * M;
+  static_assert(2 * 2 * 5 * K * K <= sizeof(sm.R1) / 8, "R1 holds the normal contraction");
+  static_assert(2 * 2 * 5 * K * K <= sizeof(sm.R2) / 8, "R2 holds the second lift stage");
   // strides of the (normal, fast, slow) axes in the [k][j][i] DOF layout
   constexpr int SN = DIR == 0 ? 1 : DIR == 1 ? K : K * K;
   constexpr int SFA = DIR == 0 ? K : 1;
   constexpr int SSL = DIR == 2 ? K : K * K;
-  double *sA = sm.A, *sD = sm.D, *sW2 = sm.W2, *sFlux = sm.Flux;
-  auto &sB = sm.B;
-  auto &sT = sm.T;
-  auto &sSym = sm.Sym;
-  auto &sCm = sm.Cm;
+  // parametric slot of each axis: r = 0, s = 1, t = 2 -> (normal, fast, slow)
+  constexpr int PN = DIR, PF = DIR == 0 ? 1 : 0, PS = DIR == 2 ? 1 : 2;
+  double *sA = sm.A, *sD = sm.D, *sW2 = sm.W2;
+  // R1 views
+  double *B = sm.R1;                          // [side][value|deriv][5][K][K]
+  double *sV = sm.R1, *sG = sm.R1 + 2 * 5 * N2;  // [side][5][N2], [x|y|z][5][N2]
+  double *P1 = sm.R1;                         // lift stage 1: [side][3][5][M][K]
+  // R2 views
+  double *T = sm.R2;                          // [side][A|D|N][5][K][M]
+  double *sFlux = sm.R2, *sSym = sm.R2 + 5 * N2;  // flux [5][N2], sym [side][N|F|S][5][N2]
+  double *G2 = sm.R2;                         // lift stage 2: [side][0|1][5][K][K]
   const int tid = threadIdx.x;
   const double *A = a.op.A, *D = a.op.D;
   for (int i = tid; i < M * K; i += 64) { sA[i] = A[i]; sD[i] = D[i]; }
@@ -212,20 +365,21 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<K, M> a)
     for (int i = tid; i < 2 * 5 * 9; i += 64) {
       const int side = i / 45, l = (i / 9) % 5, q = i % 9;
       const int pt = l < N2 ? l : 0;
-      sCm[side][l][q] = g[(side ? 4 + q : 13 + q) * N2 + pt];
+      sm.Cm[side][l][q] = g[(side ? 4 + q : 13 + q) * N2 + pt];
     }
-  // normal contraction straight from HBM: b[side][0|1][v][s][f]
+  // normal contraction straight from HBM: B[side][0|1][v][s][f]
 #pragma unroll
   for (int side = 0; side < 2; ++side) {
     const double *Us = a.U + (size_t) (side ? L : e) * 5 * ND;
     const double *Fv = a.op.F[side], *dFv = a.op.F[2 + side];
     for (int pc = tid; pc < 5 * K * K; pc += 64) {
-      const int v = pc / (K * K), s = (pc / K) % K, f = pc % K;
-      const double *u = Us + v * ND + s * SSL + f * SFA;
+      const int v = pc / (K * K), sl = (pc / K) % K, f = pc % K;
+      const double *u = Us + v * ND + sl * SSL + f * SFA;
       double b0 = 0.0, b1 = 0.0;
 #pragma unroll
       for (int t = 0; t < K; ++t) { const double x = u[t * SN]; b0 += Fv[t] * x; b1 += dFv[t] * x; }
-      sB[side][0][pc] = b0; sB[side][1][pc] = b1;
+      B[(side * 2 + 0) * 5 * K * K + pc] = b0;
+      B[(side * 2 + 1) * 5 * K * K + pc] = b1;
     }
   }
   __syncthreads();
@@ -234,239 +388,145 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<K, M> a)
     const int side = pc / (5 * K), vs = pc - side * 5 * K;
     double x0[K], x1[K];
 #pragma unroll
-    for (int f = 0; f < K; ++f) { x0[f] = sB[side][0][vs * K + f]; x1[f] = sB[side][1][vs * K + f]; }
+    for (int f = 0; f < K; ++f) {
+      x0[f] = B[(side * 2 + 0) * 5 * K * K + vs * K + f];
+      x1[f] = B[(side * 2 + 1) * 5 * K * K + vs * K + f];
+    }
+    double *Ts = T + side * 3 * 5 * K * M;
 #pragma unroll
     for (int n = 0; n < M; ++n) {
       double tA = 0.0, tD = 0.0, tN = 0.0;
 #pragma unroll
       for (int f = 0; f < K; ++f) { tA += A[n * K + f] * x0[f]; tD += D[n * K + f] * x0[f]; tN += A[n * K + f] * x1[f]; }
-      sT[side][0][vs * M + n] = tA; sT[side][1][vs * M + n] = tD; sT[side][2][vs * M + n] = tN;
+      Ts[vs * M + n] = tA; Ts[5 * K * M + vs * M + n] = tD; Ts[2 * 5 * K * M + vs * M + n] = tN;
     }
   }
   __syncthreads();
+  // traces and averaged Cartesian gradients of the entropy variables at the
+  // points, one (point, variable) pair per thread iteration
+  for (int idx = tid; idx < 5 * N2; idx += 64) {
+    const int v = idx / N2, m = idx - v * N2;
+    const int sp = m / M, fp = m - sp * M;
+    double gxa = 0.0, gya = 0.0, gza = 0.0;
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const double *tA = T + (side * 3 + 0) * 5 * K * M + v * K * M + fp;
+      const double *tD = T + (side * 3 + 1) * 5 * K * M + v * K * M + fp;
+      const double *tN = T + (side * 3 + 2) * 5 * K * M + v * K * M + fp;
+      double val = 0.0, gN = 0.0, gF = 0.0, gS = 0.0;
+#pragma unroll
+      for (int sl = 0; sl < K; ++sl) {
+        const double as = sA[sp * K + sl], xa = tA[sl * M];
+        val += as * xa; gS += sD[sp * K + sl] * xa; gN += as * tN[sl * M]; gF += as * tD[sl * M];
+      }
+      sV[(side * 5 + v) * N2 + m] = val;
+      const double gr = DIR == 0 ? gN : gF;
+      const double gs = DIR == 0 ? gF : DIR == 1 ? gN : gS;
+      const double gt = DIR == 2 ? gN : gS;
+      double c[9];
+      if (a.bug) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) c[q] = sm.Cm[side][v][q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) c[q] = g[(side ? 4 + q : 13 + q) * N2 + m];
+      }
+      gxa += 0.5 * (gr * c[0] + gs * c[3] + gt * c[6]);
+      gya += 0.5 * (gr * c[1] + gs * c[4] + gt * c[7]);
+      gza += 0.5 * (gr * c[2] + gs * c[5] + gt * c[8]);
+    }
+    sG[(0 * 5 + v) * N2 + m] = gxa;
+    sG[(1 * 5 + v) * N2 + m] = gya;
+    sG[(2 * 5 + v) * N2 + m] = gza;
+  }
+  __syncthreads();
+  // fluxes at the points (Ismail-Roe, viscous, penalty, symmetrising)
   const double lenScale = fmin(a.econst[(size_t) e * kElemConst + DIR], a.econst[(size_t) L * kElemConst + DIR]);
   const double tau = a.tauFact / lenScale;
-  const double mu = a.mu, factHF = kGam / kPrLam;
-  const double lam = kLamOverMu, lamp1 = lam + 1.0, lamp2 = lam + 2.0;
   for (int m = tid; m < N2; m += 64) {
-    const int sp = m / M, fp = m - sp * M;
-    double as[K], ds[K];
-#pragma unroll
-    for (int s = 0; s < K; ++s) { as[s] = sA[sp * K + s]; ds[s] = sD[sp * K + s]; }
-    double mL[9], mR[9];
-    if (!a.bug) {
-#pragma unroll
-      for (int q = 0; q < 9; ++q) { mL[q] = g[(4 + q) * N2 + m]; mR[q] = g[(13 + q) * N2 + m]; }
-    }
-    // traces and averaged Cartesian gradients of the entropy variables
-    double V[2][5], gx[5], gy[5], gz[5];
+    double VL[5], VR[5], gx[5], gy[5], gz[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-      gx[v] = gy[v] = gz[v] = 0.0;
-#pragma unroll
-      for (int side = 0; side < 2; ++side) {
-        const double *tA = &sT[side][0][v * K * M + fp], *tD = &sT[side][1][v * K * M + fp],
-                     *tN = &sT[side][2][v * K * M + fp];
-        double val = 0.0, gN = 0.0, gF = 0.0, gS = 0.0;
-#pragma unroll
-        for (int s = 0; s < K; ++s) {
-          const double xa = tA[s * M];
-          val += as[s] * xa; gS += ds[s] * xa; gN += as[s] * tN[s * M]; gF += as[s] * tD[s * M];
-        }
-        V[side][v] = val;
-        const double gr = DIR == 0 ? gN : gF;
-        const double gs = DIR == 0 ? gF : DIR == 1 ? gN : gS;
-        const double gt = DIR == 2 ? gN : gS;
-        double c[9];
-        if (a.bug) {
-#pragma unroll
-          for (int q = 0; q < 9; ++q) c[q] = sCm[side][v][q];
-        } else {
-#pragma unroll
-          for (int q = 0; q < 9; ++q) c[q] = side ? mL[q] : mR[q];
-        }
-        gx[v] += 0.5 * (gr * c[0] + gs * c[3] + gt * c[6]);
-        gy[v] += 0.5 * (gr * c[1] + gs * c[4] + gt * c[7]);
-        gz[v] += 0.5 * (gr * c[2] + gs * c[5] + gt * c[8]);
-      }
+      VL[v] = sV[(5 + v) * N2 + m]; VR[v] = sV[v * N2 + m];
+      gx[v] = sG[v * N2 + m]; gy[v] = sG[(5 + v) * N2 + m]; gz[v] = sG[(10 + v) * N2 + m];
     }
     const double nn[3] = {g[0 * N2 + m], g[1 * N2 + m], g[2 * N2 + m]};
     const double wA = sW2[m] * g[3 * N2 + m];
-    // ---- Ismail-Roe (entropy form, beta = 1/6, alphaMax = 0)
-    double fl[5];
-    {
-      const double *VL = V[1], *VR = V[0];
-      double rL, uL, vL, wL, pL, eL, rR, uR, vR, wR, pR, eR;
-      ent2prim(VL, rL, uL, vL, wL, pL, eL);
-      ent2prim(VR, rR, uR, vR, wR, pR, eR);
-      const double dV0 = VR[0] - VL[0], dV1 = VR[1] - VL[1], dV2 = VR[2] - VL[2],
-                   dV3 = VR[3] - VL[3], dV4 = VR[4] - VL[4];
-      const double z0L = sqrt(rL / pL), z0R = sqrt(rR / pR);
-      const double z4L = sqrt(rL * pL), z4R = sqrt(rR * pR);
-      const double z0A = 0.5 * (z0L + z0R), z4A = 0.5 * (z4L + z4R);
-      const double z1A = 0.5 * (uL * z0L + uR * z0R), z2A = 0.5 * (vL * z0L + vR * z0R),
-                   z3A = 0.5 * (wL * z0L + wR * z0R);
-      double zeta = z0L / z0R, f = (zeta - 1.0) / (zeta + 1.0), uu = f * f, F;
-      F = (uu < 0.01) ? 1.0 + uu / 3.0 + uu * uu / 5.0 + uu * uu * uu / 7.0 + uu * uu * uu * uu / 9.0
-                      : log(zeta) / (2.0 * f);
-      const double z0Log = z0A / F;
-      zeta = z4L / z4R; f = (zeta - 1.0) / (zeta + 1.0); uu = f * f;
-      F = (uu < 0.01) ? 1.0 + uu / 3.0 + uu * uu / 5.0 + uu * uu * uu / 7.0 + uu * uu * uu * uu / 9.0
-                      : log(zeta) / (2.0 * f);
-      const double z4Log = z4A / F;
-      const double oz0 = 1.0 / z0A;
-      const double rA = z0A * z4Log, p1 = oz0 * z4A;
-      const double p2 = 0.5 * ((kGam + 1.0) / kGam * z4Log / z0Log + kGm1 / kGam * p1);
-      const double uA = oz0 * z1A, vA = oz0 * z2A, wAv = oz0 * z3A;
-      const double vn = uA * nn[0] + vA * nn[1] + wAv * nn[2];
-      const double kin = 0.5 * (uA * uA + vA * vA + wAv * wAv);
-      const double a2 = kGam * p2 / rA, aa = sqrt(a2);
-      const double H = a2 / kGm1 + kin, E = H - p2 / rA;
-      double abv1 = rA * (uA * dV1 + vA * dV2 + wAv * dV3);
-      double abv2 = abv1 + rA * (dV0 + H * dV4);
-      const double dr = abv1 + rA * (dV0 + E * dV4);
-      const double dru = uA * abv2 + p2 * dV1, drv = vA * abv2 + p2 * dV2, drw = wAv * abv2 + p2 * dV3;
-      const double drE = H * abv1 + rA * E * (dV0 + H * dV4) + p2 * kin * dV4;
-      const double fac = 0.5 * (1.0 + 1.0 / 6.0);
-      const double lam1 = fabs(vn + aa) * fac, lam2 = fabs(vn - aa) * fac, lam3 = 0.5 * fabs(vn);
-      abv1 = 0.5 * (lam1 + lam2); abv2 = 0.5 * (lam1 - lam2);
-      const double abv3 = abv1 - lam3;
-      const double abv4 = kGm1 * (kin * dr - uA * dru - vA * drv - wAv * drw + drE);
-      const double abv5 = nn[0] * dru + nn[1] * drv + nn[2] * drw - vn * dr;
-      const double abv6 = abv3 * abv4 / a2 + abv2 * abv5 / aa;
-      const double abv7 = abv2 * abv4 / aa + abv3 * abv5;
-      fl[0] = wA * (rA * vn - (lam3 * dr + abv6));
-      fl[1] = wA * (rA * vn * uA + p1 * nn[0] - (lam3 * dru + uA * abv6 + nn[0] * abv7));
-      fl[2] = wA * (rA * vn * vA + p1 * nn[1] - (lam3 * drv + vA * abv6 + nn[1] * abv7));
-      fl[3] = wA * (rA * vn * wAv + p1 * nn[2] - (lam3 * drw + wAv * abv6 + nn[2] * abv7));
-      fl[4] = wA * (rA * vn * H - (lam3 * drE + H * abv6 + vn * abv7));
-    }
-    // ---- viscous normal flux, penalty and symmetrising fluxes on the average
-    const double V4Inv = 2.0 / (V[0][4] + V[1][4]);
-    const double u = -V4Inv * 0.5 * (V[0][1] + V[1][1]), vv = -V4Inv * 0.5 * (V[0][2] + V[1][2]),
-                 w = -V4Inv * 0.5 * (V[0][3] + V[1][3]);
-    {
-      const double dudx = -V4Inv * (gx[1] + u * gx[4]), dvdx = -V4Inv * (gx[2] + vv * gx[4]), dwdx = -V4Inv * (gx[3] + w * gx[4]);
-      const double dudy = -V4Inv * (gy[1] + u * gy[4]), dvdy = -V4Inv * (gy[2] + vv * gy[4]), dwdy = -V4Inv * (gy[3] + w * gy[4]);
-      const double dudz = -V4Inv * (gz[1] + u * gz[4]), dvdz = -V4Inv * (gz[2] + vv * gz[4]), dwdz = -V4Inv * (gz[3] + w * gz[4]);
-      const double fe = V4Inv * V4Inv / kGm1;
-      const double dedx = fe * gx[4], dedy = fe * gy[4], dedz = fe * gz[4];
-      const double nx = wA * nn[0], ny = wA * nn[1], nz = wA * nn[2];
-      const double lamDiv = kLamOverMu * (dudx + dvdy + dwdz);
-      const double txx = mu * (2.0 * dudx + lamDiv), tyy = mu * (2.0 * dvdy + lamDiv), tzz = mu * (2.0 * dwdz + lamDiv);
-      const double txy = mu * (dudy + dvdx), txz = mu * (dudz + dwdx), tyz = mu * (dvdz + dwdy);
-      const double fh = mu * factHF;
-      const double qx = -fh * dedx, qy = -fh * dedy, qz = -fh * dedz;
-      fl[1] -= txx * nx + txy * ny + txz * nz;
-      fl[2] -= txy * nx + tyy * ny + tyz * nz;
-      fl[3] -= txz * nx + tyz * ny + tzz * nz;
-      fl[4] -= (u * txx + vv * txy + w * txz - qx) * nx + (u * txy + vv * tyy + w * tyz - qy) * ny
-             + (u * txz + vv * tyz + w * tzz - qz) * nz;
-    }
-    const double pOvRho = -V4Inv, eInt = pOvRho / kGm1;
-    const double dV1 = V[1][1] - V[0][1], dV2 = V[1][2] - V[0][2], dV3 = V[1][3] - V[0][3], dV4 = V[1][4] - V[0][4];
-    const double factFlux = pOvRho * wA * mu;
-    const double abvE = u * u + vv * vv + w * w + factHF * eInt;
-    const double un = u * nn[0] + vv * nn[1] + w * nn[2];
-    const double dVn = dV1 * nn[0] + dV2 * nn[1] + dV3 * nn[2];
-    const double ttau = tau * factFlux, abvMom = lamp1 * (dVn + un * dV4);
-    fl[1] += ttau * (abvMom * nn[0] + dV1 + u * dV4);
-    fl[2] += ttau * (abvMom * nn[1] + dV2 + vv * dV4);
-    fl[3] += ttau * (abvMom * nn[2] + dV3 + w * dV4);
-    fl[4] += ttau * (lamp1 * un * dVn + u * dV1 + vv * dV2 + w * dV3 + (abvE + lamp1 * un * un) * dV4);
-    const double sx = a.thetaHalf * factFlux * nn[0], sy = a.thetaHalf * factFlux * nn[1],
-                 sz = a.thetaHalf * factFlux * nn[2];
-    double FX[5], FY[5], FZ[5];
-    const double K11_14 = u * lamp2, K11_44 = abvE + u * u * lamp1;
-    const double K22_24 = vv * lamp2, K22_44 = abvE + vv * vv * lamp1;
-    const double K33_34 = w * lamp2, K33_44 = abvE + w * w * lamp1;
-    const double K12_14 = vv * lam, K12_42 = u * lam, K12_44 = u * vv * lamp1;
-    const double K13_14 = w * lam, K13_43 = u * lam, K13_44 = u * w * lamp1;
-    const double K23_24 = w * lam, K23_43 = vv * lam, K23_44 = vv * w * lamp1;
-    FX[0] = 0.0;
-    FX[1] = -(lamp2 * dV1 + K11_14 * dV4) * sx - (lam * dV2 + K12_14 * dV4) * sy - (lam * dV3 + K13_14 * dV4) * sz;
-    FX[2] = -(dV2 + vv * dV4) * sx - (dV1 + u * dV4) * sy;
-    FX[3] = -(dV3 + w * dV4) * sx - (dV1 + u * dV4) * sz;
-    FX[4] = -(K11_14 * dV1 + vv * dV2 + w * dV3 + K11_44 * dV4) * sx
-            - (vv * dV1 + K12_42 * dV2 + K12_44 * dV4) * sy - (w * dV1 + K13_43 * dV3 + K13_44 * dV4) * sz;
-    FY[0] = 0.0;
-    FY[1] = -(dV2 + vv * dV4) * sx - (dV1 + u * dV4) * sy;
-    FY[2] = -(lam * dV1 + K12_42 * dV4) * sx - (lamp2 * dV2 + K22_24 * dV4) * sy - (lam * dV3 + K23_24 * dV4) * sz;
-    FY[3] = -(dV3 + w * dV4) * sy - (dV2 + vv * dV4) * sz;
-    FY[4] = -(K12_14 * dV1 + u * dV2 + K12_44 * dV4) * sx
-            - (u * dV1 + K22_24 * dV2 + w * dV3 + K22_44 * dV4) * sy - (w * dV2 + K23_43 * dV3 + K23_44 * dV4) * sz;
-    FZ[0] = 0.0;
-    FZ[1] = -(dV3 + w * dV4) * sx - (dV1 + u * dV4) * sz;
-    FZ[2] = -(dV3 + w * dV4) * sy - (dV2 + vv * dV4) * sz;
-    FZ[3] = -(lam * dV1 + K13_43 * dV4) * sx - (lam * dV2 + K23_43 * dV4) * sy - (lamp2 * dV3 + K33_34 * dV4) * sz;
-    FZ[4] = -(K13_14 * dV1 + u * dV3 + K13_44 * dV4) * sx
-            - (K23_24 * dV2 + vv * dV3 + K23_44 * dV4) * sy - (u * dV1 + vv * dV2 + K33_34 * dV3 + K33_44 * dV4) * sz;
-    // parametric slot of each axis: r = 0, s = 1, t = 2 -> (normal, fast, slow)
-    constexpr int PN = DIR, PF = DIR == 0 ? 1 : 0, PS = DIR == 2 ? 1 : 2;
+    double fl[5], FX[5], FY[5], FZ[5];
+    point_flux(VL, VR, gx, gy, gz, nn, wA, tau, a.mu, a.thetaHalf, fl, FX, FY, FZ);
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
       sFlux[v * N2 + m] = fl[v];
-#pragma unroll
-      for (int side = 0; side < 2; ++side) {
-        double c[9];
-        if (a.bug) {
-#pragma unroll
-          for (int q = 0; q < 9; ++q) c[q] = sCm[side][v][q];
-        } else {
-#pragma unroll
-          for (int q = 0; q < 9; ++q) c[q] = side ? mL[q] : mR[q];
-        }
-        double par[3];
-        par[0] = FX[v] * c[0] + FY[v] * c[1] + FZ[v] * c[2];
-        par[1] = FX[v] * c[3] + FY[v] * c[4] + FZ[v] * c[5];
-        par[2] = FX[v] * c[6] + FY[v] * c[7] + FZ[v] * c[8];
-        sSym[side][0][v * N2 + m] = par[PN];
-        sSym[side][1][v * N2 + m] = par[PF];
-        sSym[side][2][v * N2 + m] = par[PS];
-      }
+      sG[v * N2 + m] = FX[v]; sG[(5 + v) * N2 + m] = FY[v]; sG[(10 + v) * N2 + m] = FZ[v];
     }
   }
   __syncthreads();
-  // ---- lift: fast-axis contraction P[side][0..2][v][s'][f]
+  // symmetrising fluxes to the parametric slots of each side
+  for (int idx = tid; idx < 5 * N2; idx += 64) {
+    const int v = idx / N2, m = idx - v * N2;
+    const double fx = sG[v * N2 + m], fy = sG[(5 + v) * N2 + m], fz = sG[(10 + v) * N2 + m];
 #pragma unroll
-  for (int side = 0; side < 2; ++side) {
+    for (int side = 0; side < 2; ++side) {
+      double c[9];
+      if (a.bug) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) c[q] = sm.Cm[side][v][q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) c[q] = g[(side ? 4 + q : 13 + q) * N2 + m];
+      }
+      double par[3];
+      par[0] = fx * c[0] + fy * c[1] + fz * c[2];
+      par[1] = fx * c[3] + fy * c[4] + fz * c[5];
+      par[2] = fx * c[6] + fy * c[7] + fz * c[8];
+      sSym[((side * 3 + 0) * 5 + v) * N2 + m] = par[PN];
+      sSym[((side * 3 + 1) * 5 + v) * N2 + m] = par[PF];
+      sSym[((side * 3 + 2) * 5 + v) * N2 + m] = par[PS];
+    }
+  }
+  __syncthreads();
+  // ---- lift: fast-axis contraction P1[side][0..2][v][s'][f]
+  for (int pc = tid; pc < 2 * 5 * M; pc += 64) {
+    const int side = pc / (5 * M), r = pc - side * 5 * M;
     const double sign = side ? 1.0 : -1.0;
-    for (int pc = tid; pc < 5 * M; pc += 64) {
-      const double *fx = &sFlux[pc * M], *yN = &sSym[side][0][pc * M], *yF = &sSym[side][1][pc * M],
-                   *yS = &sSym[side][2][pc * M];
-      double xf[M], xF[M], xS[M], xN[M];
+    const double *fx = &sFlux[r * M];
+    const double *yN = &sSym[(side * 3 + 0) * 5 * N2 + r * M], *yF = &sSym[(side * 3 + 1) * 5 * N2 + r * M],
+                 *yS = &sSym[(side * 3 + 2) * 5 * N2 + r * M];
+    double xf[M], xF[M], xS[M], xN[M];
 #pragma unroll
-      for (int q = 0; q < M; ++q) { xf[q] = sign * fx[q]; xF[q] = yF[q]; xS[q] = yS[q]; xN[q] = yN[q]; }
+    for (int q = 0; q < M; ++q) { xf[q] = sign * fx[q]; xF[q] = yF[q]; xS[q] = yS[q]; xN[q] = yN[q]; }
+    double *Ps = P1 + side * 3 * 5 * M * K;
 #pragma unroll
-      for (int n = 0; n < K; ++n) {
-        double p1 = 0.0, p2 = 0.0, p3 = 0.0;
+    for (int n = 0; n < K; ++n) {
+      double p1 = 0.0, p2 = 0.0, p3 = 0.0;
 #pragma unroll
-        for (int q = 0; q < M; ++q) {
-          p1 += A[q * K + n] * xf[q] + D[q * K + n] * xF[q];
-          p2 += A[q * K + n] * xS[q];
-          p3 += A[q * K + n] * xN[q];
-        }
-        sT[side][0][pc * K + n] = p1; sT[side][1][pc * K + n] = p2; sT[side][2][pc * K + n] = p3;
+      for (int q = 0; q < M; ++q) {
+        p1 += A[q * K + n] * xf[q] + D[q * K + n] * xF[q];
+        p2 += A[q * K + n] * xS[q];
+        p3 += A[q * K + n] * xN[q];
       }
+      Ps[r * K + n] = p1; Ps[5 * M * K + r * K + n] = p2; Ps[2 * 5 * M * K + r * K + n] = p3;
     }
   }
   __syncthreads();
-  // slow-axis contraction G[side][0|1][v][s][f]
+  // slow-axis contraction G2[side][0|1][v][s][f]
   for (int pc = tid; pc < 2 * 5 * K; pc += 64) {
     const int side = pc / (5 * K), r = pc - side * 5 * K;
     const int v = r / K, f = r - v * K;
+    const double *Ps = P1 + side * 3 * 5 * M * K;
     double x1[M], x2[M], x3[M];
 #pragma unroll
     for (int q = 0; q < M; ++q) {
       const int j = (v * M + q) * K + f;
-      x1[q] = sT[side][0][j]; x2[q] = sT[side][1][j]; x3[q] = sT[side][2][j];
+      x1[q] = Ps[j]; x2[q] = Ps[5 * M * K + j]; x3[q] = Ps[2 * 5 * M * K + j];
     }
 #pragma unroll
     for (int n = 0; n < K; ++n) {
       double g1 = 0.0, g2 = 0.0;
 #pragma unroll
       for (int q = 0; q < M; ++q) { g1 += A[q * K + n] * x1[q] + D[q * K + n] * x2[q]; g2 += A[q * K + n] * x3[q]; }
-      sB[side][0][(v * K + n) * K + f] = g1; sB[side][1][(v * K + n) * K + f] = g2;
+      G2[(side * 2 + 0) * 5 * K * K + (v * K + n) * K + f] = g1;
+      G2[(side * 2 + 1) * 5 * K * K + (v * K + n) * K + f] = g2;
     }
   }
   __syncthreads();
@@ -480,12 +540,12 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<K, M> a)
       const int i = d % K, j = (d / K) % K, k = d / (K * K);
       const int t = DIR == 0 ? i : DIR == 1 ? j : k;
       const int f = DIR == 0 ? j : i;
-      const int s = DIR == 2 ? j : k;
-      const int gi = (v * K + s) * K + f;
+      const int sl = DIR == 2 ? j : k;
+      const int gi = (v * K + sl) * K + f;
       double fv = 0.0, dfv = 0.0;
 #pragma unroll
       for (int q = 0; q < K; ++q) if (q == t) { fv = Fv[q]; dfv = dFv[q]; }
-      out[idx] = fv * sB[side][0][gi] + dfv * sB[side][1][gi];
+      out[idx] = fv * G2[(side * 2 + 0) * 5 * K * K + gi] + dfv * G2[(side * 2 + 1) * 5 * K * K + gi];
     }
   }
 }
@@ -514,6 +574,7 @@ struct VolArgs {
   double mu;
   StageCtl ctl;
   Ops<K, M> op;
+  double c0[K];                           // phi_q(0): basis at the element centre
 };
 
 // Characteristic-matching-layer source at one point (ElementClass.cpp:
@@ -530,7 +591,7 @@ __device__ __forceinline__ void cml_source(const double V[5], const double gx[5]
   const double u = -V4Inv * V[1], v = -V4Inv * V[2], w = -V4Inv * V[3];
   const double ek = 0.5 * (u * u + v * v + w * w);
   const double s = kGam - kGm1 * (V[0] - V[4] * ek);
-  const double rho = pow(-V[4] * exp(s), -1.0 / kGm1);
+  const double rho = pow_m2p5(-V[4] * exp(s));
   const double p = -rho * V4Inv, rE = p / kGm1 + rho * ek;
   const double H = rE / rho - V4Inv, a2 = -kGam * V4Inv, a = sqrt(a2);
   const double rhoa = rho * a, ovra = 1.0 / rhoa, ova2 = 1.0 / a2, rova = rho / a;
@@ -587,7 +648,7 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
   __shared__ double sBig[BIG];
   __shared__ double sPrim[6 * NI];                 // rho, u, v, w, p, w*J at the points
   __shared__ double sCm[5][9];
-  __shared__ double sRed[(NT / 64) * 3];
+  __shared__ double sRed[2 * (NT / 64) * 3];
   __shared__ double sDv[15];
   const int e = blockIdx.x;
   const int tid = threadIdx.x;
@@ -600,29 +661,6 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
   if (a.bug)
     for (int i = tid; i < 45; i += NT) sCm[i / 9][i % 9] = vg[(1 + i % 9) * NI + i / 9];
   __syncthreads();
-
-  // element-centre entropy variables -> preconditioner dV/dU / Jbar
-  if (tid < 64) {
-    double Vc[5] = {0, 0, 0, 0, 0};
-    for (int i = tid; i < ND; i += 64) {
-      const double b = a.tables[a.to.bc + i];
-#pragma unroll
-      for (int v = 0; v < 5; ++v) Vc[v] += b * sU[v * ND + i];
-    }
-#pragma unroll
-    for (int v = 0; v < 5; ++v) Vc[v] = wave_sum(Vc[v]);
-    if (tid == 0) {
-      double rho, u, v, w, p, eKin;
-      ent2prim(Vc, rho, u, v, w, p, eKin);
-      const double JacInv = 8.0 / a.econst[(size_t) e * kElemConst + 3];
-      const double pInv = 1.0 / p, a1 = kGm1 * rho * pInv * pInv, a2 = eKin * a1;
-      sDv[0] = JacInv * (kGam / kGm1 / rho + a2 * eKin); sDv[1] = -JacInv * u * a2; sDv[2] = -JacInv * v * a2;
-      sDv[3] = -JacInv * w * a2; sDv[4] = JacInv * (a2 - pInv); sDv[5] = JacInv * (a1 * u * u + pInv);
-      sDv[6] = JacInv * a1 * u * v; sDv[7] = JacInv * a1 * u * w; sDv[8] = -JacInv * a1 * u;
-      sDv[9] = JacInv * (a1 * v * v + pInv); sDv[10] = JacInv * a1 * v * w; sDv[11] = -JacInv * a1 * v;
-      sDv[12] = JacInv * (a1 * w * w + pInv); sDv[13] = -JacInv * a1 * w; sDv[14] = JacInv * a1;
-    }
-  }
 
   // ---------------- step 1: solution and parametric gradients at the points
   double *zA = sBig, *zD = zA + SZ, *yAA = zD + SZ, *yDA = yAA + SY, *yAD = yDA + SY;
@@ -638,6 +676,39 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
       for (int q = 0; q < K; ++q) { sa += A[n * K + q] * x[q]; sd += D[n * K + q] * x[q]; }
       zA[(v * M + n) * K * K + ji] = sa;
       zD[(v * M + n) * K * K + ji] = sd;
+    }
+  }
+  if (tid >= NT - 64) {
+    // element-centre entropy variables -> preconditioner dV/dU / Jbar; the
+    // last wave does this while the others run the first contraction (it has
+    // no pencil there when 5 K^2 <= NT - 64)
+    const int lane = tid & 63;
+    double Vc[5] = {0, 0, 0, 0, 0};
+    for (int i = lane; i < ND; i += 64) {
+      const int ii = i % K, jj = (i / K) % K, kk = i / (K * K);
+      double b = 0.0, bj = 0.0, bk = 0.0;
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        if (q == ii) b = a.c0[q];
+        if (q == jj) bj = a.c0[q];
+        if (q == kk) bk = a.c0[q];
+      }
+      b *= bj * bk;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) Vc[v] += b * sU[v * ND + i];
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) Vc[v] = wave_sum(Vc[v]);
+    if (lane == 0) {
+      double rho, u, v, w, p, eKin;
+      ent2prim(Vc, rho, u, v, w, p, eKin);
+      const double JacInv = 8.0 / a.econst[(size_t) e * kElemConst + 3];
+      const double pInv = 1.0 / p, a1 = kGm1 * rho * pInv * pInv, a2 = eKin * a1;
+      sDv[0] = JacInv * (kGam / kGm1 / rho + a2 * eKin); sDv[1] = -JacInv * u * a2; sDv[2] = -JacInv * v * a2;
+      sDv[3] = -JacInv * w * a2; sDv[4] = JacInv * (a2 - pInv); sDv[5] = JacInv * (a1 * u * u + pInv);
+      sDv[6] = JacInv * a1 * u * v; sDv[7] = JacInv * a1 * u * w; sDv[8] = -JacInv * a1 * u;
+      sDv[9] = JacInv * (a1 * v * v + pInv); sDv[10] = JacInv * a1 * v * w; sDv[11] = -JacInv * a1 * v;
+      sDv[12] = JacInv * (a1 * w * w + pInv); sDv[13] = -JacInv * a1 * w; sDv[14] = JacInv * a1;
     }
   }
   __syncthreads();
@@ -885,24 +956,26 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
   // ---------------- entropy-variable mass matrix: flexible PCG
   const double JacInv = 8.0 / a.econst[(size_t) e * kElemConst + 3];
   double x[DPT], z[DPT], pv[DPT];
+  int half = 0;
   double r0 = 0.0;
 #pragma unroll
   for (int q = 0; q < DPT; ++q) { x[q] = 0.0; r0 = fmax(r0, fabs(r[q])); }
-  { double red[1] = {r0}; block_reduce<NT, 1, 0>(red, sRed); r0 = red[0]; }
+  { double red[1] = {r0}; block_reduce1<NT, 1, 0>(red, sRed, half); r0 = red[0]; }
   int iters = 0;
   if (r0 * JacInv > 1e-10) {
+    double *cz = sBig, *cy = cz + SZ, *pI = cy + SY, *by = pI + 5 * NI;
+    double *sR = cy;    // exchange buffer of the preconditioner (cy is free outside the interpolation)
     // z = dV/dU(centre) r; the five variables of a DOF are exchanged through LDS
     auto precond = [&]() {
-      __syncthreads();
 #pragma unroll
-      for (int q = 0; q < DPT; ++q) { const int i = tid + q * NT; if (i < 5 * ND) sU[i] = r[q]; }
+      for (int q = 0; q < DPT; ++q) { const int i = tid + q * NT; if (i < 5 * ND) sR[i] = r[q]; }
       __syncthreads();
 #pragma unroll
       for (int q = 0; q < DPT; ++q) {
         const int i = tid + q * NT;
         if (i < 5 * ND) {
           const int l = i / ND, d = i - l * ND;
-          const double r0_ = sU[d], r1 = sU[ND + d], r2 = sU[2 * ND + d], r3 = sU[3 * ND + d], r4 = sU[4 * ND + d];
+          const double r0_ = sR[d], r1 = sR[ND + d], r2 = sR[2 * ND + d], r3 = sR[3 * ND + d], r4 = sR[4 * ND + d];
           const int c0 = l == 0 ? 0 : l == 1 ? 1 : l == 2 ? 2 : l == 3 ? 3 : 4;
           const int c1 = l == 0 ? 1 : l == 1 ? 5 : l == 2 ? 6 : l == 3 ? 7 : 8;
           const int c2 = l == 0 ? 2 : l == 1 ? 6 : l == 2 ? 9 : l == 3 ? 10 : 11;
@@ -918,11 +991,9 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
     double rz = 0.0;
 #pragma unroll
     for (int q = 0; q < DPT; ++q) { pv[q] = z[q]; rz += r[q] * z[q]; }
-    { double red[1] = {rz}; block_reduce<NT, 1, 1>(red, sRed); rz = red[0]; }
-    double *cz = sBig, *cy = cz + SZ, *pI = cy + SY, *by = pI + 5 * NI;
+    { double red[1] = {rz}; block_reduce1<NT, 1, 1>(red, sRed, half); rz = red[0]; }
     for (int it = 0;; ++it) {
       if (it == 100) { atomicOr(a.flags, kFlagCGNotConverged); break; }
-      __syncthreads();
 #pragma unroll
       for (int q = 0; q < DPT; ++q) { const int i = tid + q * NT; if (i < 5 * ND) sU[i] = pv[q]; }
       __syncthreads();
@@ -1010,6 +1081,8 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
         }
       }
       __syncthreads();
+      // Ap along k, with the partial p.Ap of the entries each thread computes
+      double pAp = 0.0;
       for (int pc = tid; pc < 5 * K * K; pc += NT) {
         const int v = pc / (K * K), ji = pc - v * K * K;
         double xx[M];
@@ -1020,19 +1093,15 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
           double s = 0.0;
 #pragma unroll
           for (int q = 0; q < M; ++q) s += A[q * K + n] * xx[q];
-          cz[(v * K + n) * K * K + ji] = s;                 // Ap
+          const int idx = (v * K + n) * K * K + ji;
+          cz[idx] = s;
+          pAp += sU[idx] * s;
         }
       }
-      __syncthreads();
+      { double red[1] = {pAp}; block_reduce1<NT, 1, 1>(red, sRed, half); pAp = red[0]; }
       double ap[DPT];
-      double pAp = 0.0;
 #pragma unroll
-      for (int q = 0; q < DPT; ++q) {
-        const int i = tid + q * NT;
-        ap[q] = (i < 5 * ND) ? cz[i] : 0.0;
-        pAp += pv[q] * ap[q];
-      }
-      { double red[1] = {pAp}; block_reduce<NT, 1, 1>(red, sRed); pAp = red[0]; }
+      for (int q = 0; q < DPT; ++q) { const int i = tid + q * NT; ap[q] = (i < 5 * ND) ? cz[i] : 0.0; }
       const double alpha = rz / pAp;
       double rmax = 0.0;
 #pragma unroll
@@ -1047,7 +1116,7 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
       for (int q = 0; q < DPT; ++q) { zAp += z[q] * ap[q]; rzn += r[q] * z[q]; }
       {
         double red[3] = {zAp, rzn, rmax};
-        block_reduce<NT, 3, 2>(red, sRed);
+        block_reduce1<NT, 3, 2>(red, sRed, half);
         zAp = red[0]; rzn = red[1]; rmax = red[2];
       }
       iters = it + 1;
@@ -1077,123 +1146,6 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
 }
 
 // ================================================= open-boundary kernels
-// Ismail-Roe + viscous + interior-penalty + symmetrising fluxes at one face
-// point: VL/VR entropy states (L -> R along nn), gx/gy/gz the averaged
-// Cartesian gradients, wA = weight * area factor. fl: normal flux, FX/FY/FZ:
-// Cartesian symmetrising fluxes (FluxesFace.cpp:813-975, :1290-1651,
-// :1152-1284).
-__device__ __forceinline__ void point_flux(const double VL[5], const double VR[5], const double gx[5],
-                                           const double gy[5], const double gz[5], const double nn[3],
-                                           double wA, double tau, double mu, double thetaHalf,
-                                           double fl[5], double FX[5], double FY[5], double FZ[5])
-{
-  const double factHF = kGam / kPrLam;
-  const double lam = kLamOverMu, lamp1 = lam + 1.0, lamp2 = lam + 2.0;
-  {
-    double rL, uL, vL, wL, pL, eL, rR, uR, vR, wR, pR, eR;
-    ent2prim(VL, rL, uL, vL, wL, pL, eL);
-    ent2prim(VR, rR, uR, vR, wR, pR, eR);
-    const double dV0 = VR[0] - VL[0], dV1 = VR[1] - VL[1], dV2 = VR[2] - VL[2],
-                 dV3 = VR[3] - VL[3], dV4 = VR[4] - VL[4];
-    const double z0L = sqrt(rL / pL), z0R = sqrt(rR / pR);
-    const double z4L = sqrt(rL * pL), z4R = sqrt(rR * pR);
-    const double z0A = 0.5 * (z0L + z0R), z4A = 0.5 * (z4L + z4R);
-    const double z1A = 0.5 * (uL * z0L + uR * z0R), z2A = 0.5 * (vL * z0L + vR * z0R),
-                 z3A = 0.5 * (wL * z0L + wR * z0R);
-    double zeta = z0L / z0R, f = (zeta - 1.0) / (zeta + 1.0), uu = f * f, F;
-    F = (uu < 0.01) ? 1.0 + uu / 3.0 + uu * uu / 5.0 + uu * uu * uu / 7.0 + uu * uu * uu * uu / 9.0
-                    : log(zeta) / (2.0 * f);
-    const double z0Log = z0A / F;
-    zeta = z4L / z4R; f = (zeta - 1.0) / (zeta + 1.0); uu = f * f;
-    F = (uu < 0.01) ? 1.0 + uu / 3.0 + uu * uu / 5.0 + uu * uu * uu / 7.0 + uu * uu * uu * uu / 9.0
-                    : log(zeta) / (2.0 * f);
-    const double z4Log = z4A / F;
-    const double oz0 = 1.0 / z0A;
-    const double rA = z0A * z4Log, p1 = oz0 * z4A;
-    const double p2 = 0.5 * ((kGam + 1.0) / kGam * z4Log / z0Log + kGm1 / kGam * p1);
-    const double uA = oz0 * z1A, vA = oz0 * z2A, wAv = oz0 * z3A;
-    const double vn = uA * nn[0] + vA * nn[1] + wAv * nn[2];
-    const double kin = 0.5 * (uA * uA + vA * vA + wAv * wAv);
-    const double a2 = kGam * p2 / rA, aa = sqrt(a2);
-    const double H = a2 / kGm1 + kin, E = H - p2 / rA;
-    double abv1 = rA * (uA * dV1 + vA * dV2 + wAv * dV3);
-    double abv2 = abv1 + rA * (dV0 + H * dV4);
-    const double dr = abv1 + rA * (dV0 + E * dV4);
-    const double dru = uA * abv2 + p2 * dV1, drv = vA * abv2 + p2 * dV2, drw = wAv * abv2 + p2 * dV3;
-    const double drE = H * abv1 + rA * E * (dV0 + H * dV4) + p2 * kin * dV4;
-    const double fac = 0.5 * (1.0 + 1.0 / 6.0);
-    const double lam1 = fabs(vn + aa) * fac, lam2 = fabs(vn - aa) * fac, lam3 = 0.5 * fabs(vn);
-    abv1 = 0.5 * (lam1 + lam2); abv2 = 0.5 * (lam1 - lam2);
-    const double abv3 = abv1 - lam3;
-    const double abv4 = kGm1 * (kin * dr - uA * dru - vA * drv - wAv * drw + drE);
-    const double abv5 = nn[0] * dru + nn[1] * drv + nn[2] * drw - vn * dr;
-    const double abv6 = abv3 * abv4 / a2 + abv2 * abv5 / aa;
-    const double abv7 = abv2 * abv4 / aa + abv3 * abv5;
-    fl[0] = wA * (rA * vn - (lam3 * dr + abv6));
-    fl[1] = wA * (rA * vn * uA + p1 * nn[0] - (lam3 * dru + uA * abv6 + nn[0] * abv7));
-    fl[2] = wA * (rA * vn * vA + p1 * nn[1] - (lam3 * drv + vA * abv6 + nn[1] * abv7));
-    fl[3] = wA * (rA * vn * wAv + p1 * nn[2] - (lam3 * drw + wAv * abv6 + nn[2] * abv7));
-    fl[4] = wA * (rA * vn * H - (lam3 * drE + H * abv6 + vn * abv7));
-  }
-  const double V4Inv = 2.0 / (VL[4] + VR[4]);
-  const double u = -V4Inv * 0.5 * (VL[1] + VR[1]), vv = -V4Inv * 0.5 * (VL[2] + VR[2]),
-               w = -V4Inv * 0.5 * (VL[3] + VR[3]);
-  {
-    const double dudx = -V4Inv * (gx[1] + u * gx[4]), dvdx = -V4Inv * (gx[2] + vv * gx[4]), dwdx = -V4Inv * (gx[3] + w * gx[4]);
-    const double dudy = -V4Inv * (gy[1] + u * gy[4]), dvdy = -V4Inv * (gy[2] + vv * gy[4]), dwdy = -V4Inv * (gy[3] + w * gy[4]);
-    const double dudz = -V4Inv * (gz[1] + u * gz[4]), dvdz = -V4Inv * (gz[2] + vv * gz[4]), dwdz = -V4Inv * (gz[3] + w * gz[4]);
-    const double fe = V4Inv * V4Inv / kGm1;
-    const double dedx = fe * gx[4], dedy = fe * gy[4], dedz = fe * gz[4];
-    const double nx = wA * nn[0], ny = wA * nn[1], nz = wA * nn[2];
-    const double lamDiv = kLamOverMu * (dudx + dvdy + dwdz);
-    const double txx = mu * (2.0 * dudx + lamDiv), tyy = mu * (2.0 * dvdy + lamDiv), tzz = mu * (2.0 * dwdz + lamDiv);
-    const double txy = mu * (dudy + dvdx), txz = mu * (dudz + dwdx), tyz = mu * (dvdz + dwdy);
-    const double fh = mu * factHF;
-    const double qx = -fh * dedx, qy = -fh * dedy, qz = -fh * dedz;
-    fl[1] -= txx * nx + txy * ny + txz * nz;
-    fl[2] -= txy * nx + tyy * ny + tyz * nz;
-    fl[3] -= txz * nx + tyz * ny + tzz * nz;
-    fl[4] -= (u * txx + vv * txy + w * txz - qx) * nx + (u * txy + vv * tyy + w * tyz - qy) * ny
-           + (u * txz + vv * tyz + w * tzz - qz) * nz;
-  }
-  const double pOvRho = -V4Inv, eInt = pOvRho / kGm1;
-  const double dV1 = VL[1] - VR[1], dV2 = VL[2] - VR[2], dV3 = VL[3] - VR[3], dV4 = VL[4] - VR[4];
-  const double factFlux = pOvRho * wA * mu;
-  const double abvE = u * u + vv * vv + w * w + factHF * eInt;
-  const double un = u * nn[0] + vv * nn[1] + w * nn[2];
-  const double dVn = dV1 * nn[0] + dV2 * nn[1] + dV3 * nn[2];
-  const double ttau = tau * factFlux, abvMom = lamp1 * (dVn + un * dV4);
-  fl[1] += ttau * (abvMom * nn[0] + dV1 + u * dV4);
-  fl[2] += ttau * (abvMom * nn[1] + dV2 + vv * dV4);
-  fl[3] += ttau * (abvMom * nn[2] + dV3 + w * dV4);
-  fl[4] += ttau * (lamp1 * un * dVn + u * dV1 + vv * dV2 + w * dV3 + (abvE + lamp1 * un * un) * dV4);
-  const double sx = thetaHalf * factFlux * nn[0], sy = thetaHalf * factFlux * nn[1], sz = thetaHalf * factFlux * nn[2];
-  const double K11_14 = u * lamp2, K11_44 = abvE + u * u * lamp1;
-  const double K22_24 = vv * lamp2, K22_44 = abvE + vv * vv * lamp1;
-  const double K33_34 = w * lamp2, K33_44 = abvE + w * w * lamp1;
-  const double K12_14 = vv * lam, K12_42 = u * lam, K12_44 = u * vv * lamp1;
-  const double K13_14 = w * lam, K13_43 = u * lam, K13_44 = u * w * lamp1;
-  const double K23_24 = w * lam, K23_43 = vv * lam, K23_44 = vv * w * lamp1;
-  FX[0] = 0.0;
-  FX[1] = -(lamp2 * dV1 + K11_14 * dV4) * sx - (lam * dV2 + K12_14 * dV4) * sy - (lam * dV3 + K13_14 * dV4) * sz;
-  FX[2] = -(dV2 + vv * dV4) * sx - (dV1 + u * dV4) * sy;
-  FX[3] = -(dV3 + w * dV4) * sx - (dV1 + u * dV4) * sz;
-  FX[4] = -(K11_14 * dV1 + vv * dV2 + w * dV3 + K11_44 * dV4) * sx
-          - (vv * dV1 + K12_42 * dV2 + K12_44 * dV4) * sy - (w * dV1 + K13_43 * dV3 + K13_44 * dV4) * sz;
-  FY[0] = 0.0;
-  FY[1] = -(dV2 + vv * dV4) * sx - (dV1 + u * dV4) * sy;
-  FY[2] = -(lam * dV1 + K12_42 * dV4) * sx - (lamp2 * dV2 + K22_24 * dV4) * sy - (lam * dV3 + K23_24 * dV4) * sz;
-  FY[3] = -(dV3 + w * dV4) * sy - (dV2 + vv * dV4) * sz;
-  FY[4] = -(K12_14 * dV1 + u * dV2 + K12_44 * dV4) * sx
-          - (u * dV1 + K22_24 * dV2 + w * dV3 + K22_44 * dV4) * sy - (w * dV2 + K23_43 * dV3 + K23_44 * dV4) * sz;
-  FZ[0] = 0.0;
-  FZ[1] = -(dV3 + w * dV4) * sx - (dV1 + u * dV4) * sz;
-  FZ[2] = -(dV3 + w * dV4) * sy - (dV2 + vv * dV4) * sz;
-  FZ[3] = -(lam * dV1 + K13_43 * dV4) * sx - (lam * dV2 + K23_43 * dV4) * sy - (lamp2 * dV3 + K33_34 * dV4) * sz;
-  FZ[4] = -(K13_14 * dV1 + u * dV3 + K13_44 * dV4) * sx
-          - (K23_24 * dV2 + vv * dV3 + K23_44 * dV4) * sy - (u * dV1 + vv * dV2 + K33_34 * dV3 + K33_44 * dV4) * sz;
-}
-
 template <int K, int M>
 struct BndArgs {
   const double *U, *tables, *econst, *bgeo;   // bgeo: [nBFace][13][nInt2D] own face metric
@@ -1346,7 +1298,7 @@ __global__ __launch_bounds__(64) void nscbc_kernel(BndArgs<K, M> a)
     const double ek = 0.5 * (u * u + v * v + w * w);
     const double sEnt = kGam - kGm1 * (V[0] - V[4] * ek);
     const double a2 = -kGam * V4Inv;
-    const double rho = pow(-V[4] * exp(sEnt), -1.0 / kGm1);
+    const double rho = pow_m2p5(-V[4] * exp(sEnt));
     const double p = a2 * rho / kGam;
     const double povrho = -V4Inv;
     const double E = povrho / kGm1 + ek, H = E + povrho;
@@ -1750,6 +1702,7 @@ hipError_t volume_impl(const DeviceCase &dc, const StageCtl &ctl, hipStream_t s)
   a.to = dc.to; a.nElem = dc.nElem; a.bug = dc.bug_compat; a.mu = dc.mu; a.ctl = ctl;
   a.spIdx = dc.nSponge ? dc.spIdx : nullptr; a.spCoef = dc.spCoef; a.dampFn = dc.dampFn;
   a.dampState = dc.dampState; a.srcBuf = dc.srcBuf; a.cml = dc.cml;
+  for (int q = 0; q < K; ++q) a.c0[q] = dc.opC0[q];
   hipLaunchKernelGGL((volume_kernel<K, M, 256>), dim3(dc.nElem), dim3(256), 0, s, a);
   return hipGetLastError();
 }

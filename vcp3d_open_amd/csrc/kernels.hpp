@@ -44,6 +44,7 @@ struct DeviceCase {
   double opA[128] = {}, opD[128] = {};   // host copies of the M x K operators
   double opF[4][16] = {};                // phi(-1), phi(+1), phi'(-1), phi'(+1)
   double opW[16] = {};                   // 1D Gauss weights
+  double opC0[16] = {};                  // phi_q(0), the basis at the element centre
   double *tables = nullptr;
   double *vgeo = nullptr, *fgeo = nullptr, *econst = nullptr;
   int *nbr = nullptr;

@@ -268,6 +268,7 @@ void build_tables(const CaseParams &p, Tables &t)
     }
   std::vector<double> c0(K);
   for (int q = 0; q < K; ++q) { double d; ortho_legendre(q, 0.0, c0[q], d); }
+  t.c0 = c0;
   t.basis_center.resize(K * K * K);
   for (int k = 0; k < K; ++k)
     for (int j = 0; j < K; ++j)

@@ -51,6 +51,7 @@ struct Tables {
   std::vector<double> VI;         // inverse of LD
   std::vector<double> f_min, f_max, df_min, df_max;  // K
   std::vector<double> w3, w2, basis_center;
+  std::vector<double> c0;         // K: phi_q(0)
   // open boundaries / sponge
   std::vector<double> Lq;         // M x K: nodal DOF Lagrange basis at the Gauss points
   std::vector<double> dLq;        // M x M: Gauss-point Lagrange derivative, [point][basis]

@@ -118,14 +118,41 @@ __device__ inline void contract2(const double *__restrict__ in1, const double *_
   }
 }
 
+// Wave64 reduction through DPP row shifts and row broadcasts (no LDS):
+// sum, or max of non-negative values; lanes without a source contribute 0.
+// The result (lane 63) is returned to every lane.
+template <int CTRL, int ROW_MASK>
+__device__ inline double dpp_f64(double v)
+{
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  const int rlo = __builtin_amdgcn_update_dpp(0, lo, CTRL, ROW_MASK, 0xf, false);
+  const int rhi = __builtin_amdgcn_update_dpp(0, hi, CTRL, ROW_MASK, 0xf, false);
+  return __hiloint2double(rhi, rlo);
+}
+template <bool MAX>
+__device__ inline double wave_reduce_dpp(double v)
+{
+  auto op = [](double x, double y) { return MAX ? fmax(x, y) : x + y; };
+  v = op(v, dpp_f64<0x111, 0xf>(v));   // row_shr:1
+  v = op(v, dpp_f64<0x112, 0xf>(v));   // row_shr:2
+  v = op(v, dpp_f64<0x114, 0xf>(v));   // row_shr:4
+  v = op(v, dpp_f64<0x118, 0xf>(v));   // row_shr:8   -> lane 15 of each row: row total
+  v = op(v, dpp_f64<0x142, 0xa>(v));   // row_bcast:15 into rows 1, 3
+  v = op(v, dpp_f64<0x143, 0xc>(v));   // row_bcast:31 into rows 2, 3 -> lane 63: total
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+  return __hiloint2double(hi, lo);
+}
+
 // Block reduction with one barrier: partial results go to alternating
 // halves of red (2 * NT/64 * NV doubles), so a half is rewritten only after
-// every thread has passed the barrier of the following reduction.
+// every thread has passed the barrier of the following reduction. The max
+// slots (q >= NS) take non-negative values.
 template <int NT, int NV, int NS>
 __device__ inline void block_reduce1(double (&v)[NV], double *red, int &half)
 {
 #pragma unroll
-  for (int q = 0; q < NV; ++q) v[q] = (q < NS) ? wave_sum(v[q]) : wave_max(v[q]);
+  for (int q = 0; q < NV; ++q) v[q] = (q < NS) ? wave_reduce_dpp<false>(v[q]) : wave_reduce_dpp<true>(v[q]);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   double *buf = red + half * (NT / 64) * NV;
   if (lane == 0)
@@ -839,6 +866,22 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
   }
   __syncthreads();
 
+  // face slots of this element, summed while the backward contractions run
+  // (DOF-major: thread d < ND owns DOF d of all five variables)
+  static_assert(ND <= NT, "one DOF per thread in the mass-matrix solve");
+  double r[5];
+  {
+    const double *fr = a.faceRes + (size_t) e * 6 * 5 * ND;
+#pragma unroll
+    for (int l = 0; l < 5; ++l) {
+      double s = 0.0;
+      if (tid < ND)
+#pragma unroll
+        for (int f = 0; f < 6; ++f) s += fr[f * 5 * ND + l * ND + tid];
+      r[l] = s;
+    }
+  }
+
   // ---------------- step 7: transposed contractions of the three fluxes
   for (int r = tid; r < 15 * M * M; r += NT) {        // rows along i, in place
     const int d = r / (5 * M * M);
@@ -939,63 +982,41 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
   }
 
   // ---------------- gather the six face contributions
-  double r[DPT];
-  const double *fr = a.faceRes + (size_t) e * 6 * 5 * ND;
+  if (tid < ND)
 #pragma unroll
-  for (int q = 0; q < DPT; ++q) {
-    const int i = tid + q * NT;
-    r[q] = 0.0;
-    if (i < 5 * ND) {
-      double s = sU[i];
-#pragma unroll
-      for (int f = 0; f < 6; ++f) s += fr[f * 5 * ND + i];
-      r[q] = s;
-    }
-  }
+    for (int l = 0; l < 5; ++l) r[l] += sU[l * ND + tid];
 
   // ---------------- entropy-variable mass matrix: flexible PCG
   const double JacInv = 8.0 / a.econst[(size_t) e * kElemConst + 3];
-  double x[DPT], z[DPT], pv[DPT];
+  const bool own = tid < ND;
+  double x[5], z[5], pv[5];
   int half = 0;
   double r0 = 0.0;
 #pragma unroll
-  for (int q = 0; q < DPT; ++q) { x[q] = 0.0; r0 = fmax(r0, fabs(r[q])); }
+  for (int l = 0; l < 5; ++l) { x[l] = 0.0; r0 = fmax(r0, fabs(r[l])); }
   { double red[1] = {r0}; block_reduce1<NT, 1, 0>(red, sRed, half); r0 = red[0]; }
   int iters = 0;
   if (r0 * JacInv > 1e-10) {
     double *cz = sBig, *cy = cz + SZ, *pI = cy + SY, *by = pI + 5 * NI;
-    double *sR = cy;    // exchange buffer of the preconditioner (cy is free outside the interpolation)
-    // z = dV/dU(centre) r; the five variables of a DOF are exchanged through LDS
-    auto precond = [&]() {
-#pragma unroll
-      for (int q = 0; q < DPT; ++q) { const int i = tid + q * NT; if (i < 5 * ND) sR[i] = r[q]; }
-      __syncthreads();
-#pragma unroll
-      for (int q = 0; q < DPT; ++q) {
-        const int i = tid + q * NT;
-        if (i < 5 * ND) {
-          const int l = i / ND, d = i - l * ND;
-          const double r0_ = sR[d], r1 = sR[ND + d], r2 = sR[2 * ND + d], r3 = sR[3 * ND + d], r4 = sR[4 * ND + d];
-          const int c0 = l == 0 ? 0 : l == 1 ? 1 : l == 2 ? 2 : l == 3 ? 3 : 4;
-          const int c1 = l == 0 ? 1 : l == 1 ? 5 : l == 2 ? 6 : l == 3 ? 7 : 8;
-          const int c2 = l == 0 ? 2 : l == 1 ? 6 : l == 2 ? 9 : l == 3 ? 10 : 11;
-          const int c3 = l == 0 ? 3 : l == 1 ? 7 : l == 2 ? 10 : l == 3 ? 12 : 13;
-          const int c4 = l == 0 ? 4 : l == 1 ? 8 : l == 2 ? 11 : l == 3 ? 13 : 14;
-          z[q] = sDv[c0] * r0_ + sDv[c1] * r1 + sDv[c2] * r2 + sDv[c3] * r3 + sDv[c4] * r4;
-        } else {
-          z[q] = 0.0;
-        }
-      }
-    };
-    precond();
+    // z = dV/dU(centre) r, local to the thread owning the DOF
+#define VCP_PRECOND()                                                                             \
+    do {                                                                                          \
+      z[0] = sDv[0] * r[0] + sDv[1] * r[1] + sDv[2] * r[2] + sDv[3] * r[3] + sDv[4] * r[4];       \
+      z[1] = sDv[1] * r[0] + sDv[5] * r[1] + sDv[6] * r[2] + sDv[7] * r[3] + sDv[8] * r[4];       \
+      z[2] = sDv[2] * r[0] + sDv[6] * r[1] + sDv[9] * r[2] + sDv[10] * r[3] + sDv[11] * r[4];     \
+      z[3] = sDv[3] * r[0] + sDv[7] * r[1] + sDv[10] * r[2] + sDv[12] * r[3] + sDv[13] * r[4];    \
+      z[4] = sDv[4] * r[0] + sDv[8] * r[1] + sDv[11] * r[2] + sDv[13] * r[3] + sDv[14] * r[4];    \
+    } while (0)
+    VCP_PRECOND();
     double rz = 0.0;
 #pragma unroll
-    for (int q = 0; q < DPT; ++q) { pv[q] = z[q]; rz += r[q] * z[q]; }
+    for (int l = 0; l < 5; ++l) { pv[l] = z[l]; rz += r[l] * z[l]; }
     { double red[1] = {rz}; block_reduce1<NT, 1, 1>(red, sRed, half); rz = red[0]; }
     for (int it = 0;; ++it) {
       if (it == 100) { atomicOr(a.flags, kFlagCGNotConverged); break; }
+      if (own)
 #pragma unroll
-      for (int q = 0; q < DPT; ++q) { const int i = tid + q * NT; if (i < 5 * ND) sU[i] = pv[q]; }
+        for (int l = 0; l < 5; ++l) sU[l * ND + tid] = pv[l];
       __syncthreads();
       for (int pc = tid; pc < 5 * K * K; pc += NT) {
         const int v = pc / (K * K), ji = pc - v * K * K;
@@ -1099,21 +1120,21 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
         }
       }
       { double red[1] = {pAp}; block_reduce1<NT, 1, 1>(red, sRed, half); pAp = red[0]; }
-      double ap[DPT];
+      double ap[5];
 #pragma unroll
-      for (int q = 0; q < DPT; ++q) { const int i = tid + q * NT; ap[q] = (i < 5 * ND) ? cz[i] : 0.0; }
+      for (int l = 0; l < 5; ++l) ap[l] = own ? cz[l * ND + tid] : 0.0;
       const double alpha = rz / pAp;
       double rmax = 0.0;
 #pragma unroll
-      for (int q = 0; q < DPT; ++q) {
-        x[q] += alpha * pv[q];
-        r[q] -= alpha * ap[q];
-        rmax = fmax(rmax, fabs(r[q]));
+      for (int l = 0; l < 5; ++l) {
+        x[l] += alpha * pv[l];
+        r[l] -= alpha * ap[l];
+        rmax = fmax(rmax, fabs(r[l]));
       }
-      precond();
+      VCP_PRECOND();
       double zAp = 0.0, rzn = 0.0;
 #pragma unroll
-      for (int q = 0; q < DPT; ++q) { zAp += z[q] * ap[q]; rzn += r[q] * z[q]; }
+      for (int l = 0; l < 5; ++l) { zAp += z[l] * ap[l]; rzn += r[l] * z[l]; }
       {
         double red[3] = {zAp, rzn, rmax};
         block_reduce1<NT, 3, 2>(red, sRed, half);
@@ -1123,9 +1144,10 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
       if (rmax < 1e-10 * r0) break;
       const double beta = -alpha * zAp / rz;
 #pragma unroll
-      for (int q = 0; q < DPT; ++q) pv[q] = beta * pv[q] + z[q];
+      for (int l = 0; l < 5; ++l) pv[l] = beta * pv[l] + z[l];
       rz = rzn;
     }
+#undef VCP_PRECOND
   }
   if (tid == 0) a.cgIter[e] = iters;
   // ---------------- outputs: R and/or the in-place RK stage update
@@ -1133,15 +1155,14 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
   double *Uw = a.U + (size_t) e * 5 * ND;
   const double *U0e = a.U0 + (size_t) e * 5 * ND;
   bool bad = false;
+  if (own)
 #pragma unroll
-  for (int q = 0; q < DPT; ++q) {
-    const int i = tid + q * NT;
-    if (i < 5 * ND) {
-      if (!isfinite(x[q])) bad = true;
-      if (a.ctl.store_res) Re[i] = x[q];
-      if (a.ctl.update) Uw[i] = (1.0 - a.ctl.a) * Uw[i] + a.ctl.a * U0e[i] - a.ctl.bdt * x[q];
+    for (int l = 0; l < 5; ++l) {
+      const int i = l * ND + tid;
+      if (!isfinite(x[l])) bad = true;
+      if (a.ctl.store_res) Re[i] = x[l];
+      if (a.ctl.update) Uw[i] = (1.0 - a.ctl.a) * Uw[i] + a.ctl.a * U0e[i] - a.ctl.bdt * x[l];
     }
-  }
   if (bad) atomicOr(a.flags, kFlagNonFinite);
 }
 

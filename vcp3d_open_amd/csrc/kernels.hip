@@ -349,13 +349,18 @@ struct FaceArgs {
 // itself (its min face), side 1 the neighbour L (its max face); the normal
 // points from L to the element. LDS regions are reused across the phases:
 //   R1: normal contraction B -> traces/gradients at the points -> lift stage 1
-//   R2: fast contraction T -> fluxes at the points -> lift stage 2
+//   R2: fast contraction T -> flux and Cartesian symmetrising fluxes at the
+//       points -> lift stage 2
+// The symmetrising fluxes are mapped to each side's parametric slots inside
+// the first lift stage.
 template <int K, int M>
 struct FaceSmem {
+  static constexpr int kR1a = 5 * 5 * M * M, kR1b = 2 * 3 * 5 * M * K;
+  static constexpr int kR2a = 4 * 5 * M * M, kR2b = 2 * 3 * 5 * K * M;
   double A[M * K], D[M * K], W2[M * M];
   double Cm[2][5][9];
-  double R1[(2 * 5 * M * M + 3 * 5 * M * M) > (2 * 3 * 5 * M * K) ? (5 * 5 * M * M) : (2 * 3 * 5 * M * K)];
-  double R2[(4 * 5 * M * M + 3 * 5 * M * M) > (2 * 3 * 5 * K * M) ? (7 * 5 * M * M) : (2 * 3 * 5 * K * M)];
+  double R1[kR1a > kR1b ? kR1a : kR1b];
+  double R2[kR2a > kR2b ? kR2a : kR2b];
 };
 
 template <int K, int M, int DIR>
@@ -381,7 +386,7 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<K, M> a)
   double *P1 = sm.R1;                         // lift stage 1: [side][3][5][M][K]
   // R2 views
   double *T = sm.R2;                          // [side][A|D|N][5][K][M]
-  double *sFlux = sm.R2, *sSym = sm.R2 + 5 * N2;  // flux [5][N2], sym [side][N|F|S][5][N2]
+  double *sFlux = sm.R2, *sFC = sm.R2 + 5 * N2;  // flux [5][N2], Cartesian sym [x|y|z][5][N2]
   double *G2 = sm.R2;                         // lift stage 2: [side][0|1][5][K][K]
   const int tid = threadIdx.x;
   const double *A = a.op.A, *D = a.op.D;
@@ -484,44 +489,35 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<K, M> a)
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
       sFlux[v * N2 + m] = fl[v];
-      sG[v * N2 + m] = FX[v]; sG[(5 + v) * N2 + m] = FY[v]; sG[(10 + v) * N2 + m] = FZ[v];
+      sFC[v * N2 + m] = FX[v]; sFC[(5 + v) * N2 + m] = FY[v]; sFC[(10 + v) * N2 + m] = FZ[v];
     }
   }
   __syncthreads();
-  // symmetrising fluxes to the parametric slots of each side
-  for (int idx = tid; idx < 5 * N2; idx += 64) {
-    const int v = idx / N2, m = idx - v * N2;
-    const double fx = sG[v * N2 + m], fy = sG[(5 + v) * N2 + m], fz = sG[(10 + v) * N2 + m];
+  // ---- lift: fast-axis contraction P1[side][0..2][v][s'][f]; the Cartesian
+  // symmetrising fluxes are mapped to the side's parametric slots on the way
+  for (int pc = tid; pc < 2 * 5 * M; pc += 64) {
+    const int side = pc / (5 * M), r = pc - side * 5 * M;
+    const int v = r / M, sp = r - v * M;
+    const double sign = side ? 1.0 : -1.0;
+    double xf[M], xF[M], xS[M], xN[M];
 #pragma unroll
-    for (int side = 0; side < 2; ++side) {
+    for (int q = 0; q < M; ++q) {
+      const int m = sp * M + q;
+      const double fx = sFC[v * N2 + m], fy = sFC[(5 + v) * N2 + m], fz = sFC[(10 + v) * N2 + m];
       double c[9];
       if (a.bug) {
 #pragma unroll
-        for (int q = 0; q < 9; ++q) c[q] = sm.Cm[side][v][q];
+        for (int t = 0; t < 9; ++t) c[t] = sm.Cm[side][v][t];
       } else {
 #pragma unroll
-        for (int q = 0; q < 9; ++q) c[q] = g[(side ? 4 + q : 13 + q) * N2 + m];
+        for (int t = 0; t < 9; ++t) c[t] = g[(side ? 4 + t : 13 + t) * N2 + m];
       }
       double par[3];
       par[0] = fx * c[0] + fy * c[1] + fz * c[2];
       par[1] = fx * c[3] + fy * c[4] + fz * c[5];
       par[2] = fx * c[6] + fy * c[7] + fz * c[8];
-      sSym[((side * 3 + 0) * 5 + v) * N2 + m] = par[PN];
-      sSym[((side * 3 + 1) * 5 + v) * N2 + m] = par[PF];
-      sSym[((side * 3 + 2) * 5 + v) * N2 + m] = par[PS];
+      xf[q] = sign * sFlux[v * N2 + m]; xN[q] = par[PN]; xF[q] = par[PF]; xS[q] = par[PS];
     }
-  }
-  __syncthreads();
-  // ---- lift: fast-axis contraction P1[side][0..2][v][s'][f]
-  for (int pc = tid; pc < 2 * 5 * M; pc += 64) {
-    const int side = pc / (5 * M), r = pc - side * 5 * M;
-    const double sign = side ? 1.0 : -1.0;
-    const double *fx = &sFlux[r * M];
-    const double *yN = &sSym[(side * 3 + 0) * 5 * N2 + r * M], *yF = &sSym[(side * 3 + 1) * 5 * N2 + r * M],
-                 *yS = &sSym[(side * 3 + 2) * 5 * N2 + r * M];
-    double xf[M], xF[M], xS[M], xN[M];
-#pragma unroll
-    for (int q = 0; q < M; ++q) { xf[q] = sign * fx[q]; xF[q] = yF[q]; xS[q] = yS[q]; xN[q] = yN[q]; }
     double *Ps = P1 + side * 3 * 5 * M * K;
 #pragma unroll
     for (int n = 0; n < K; ++n) {

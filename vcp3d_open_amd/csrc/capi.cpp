@@ -74,11 +74,6 @@ int check_flags(vcp_ctx *c)
   return 0;
 }
 
-int set_scalar(vcp_ctx *c, int slot, double v)
-{
-  HIP_TRY(hipMemcpyAsync(c->dc.scal + slot, &v, sizeof(double), hipMemcpyHostToDevice, c->stream));
-  return 0;
-}
 
 // open boundaries + faces + volume/mass matrix, with the given stage control
 int residual_pass(vcp_ctx *c, const vcp::StageCtl &ctl)
@@ -237,6 +232,7 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
   if (he == hipSuccess) he = c->alloc(&dc.flags, 4);
   if (he == hipSuccess) he = c->alloc(&dc.cgIter, (size_t) nElem);
   if (he == hipSuccess) he = c->alloc(&dc.dtElem, (size_t) nElem);
+  if (he == hipSuccess) he = c->alloc(&dc.mach2Elem, (size_t) nElem);
   if (he == hipSuccess && dc.nBFace) {
     he = c->alloc(&dc.bface, ob.faces.size());
     if (he == hipSuccess) he = c->alloc(&dc.bgeo, bgeo.size());
@@ -407,8 +403,7 @@ int vcp_get_residual(vcp_ctx *c, double *modal)
 int vcp_residual(vcp_ctx *c, int mon, double md[8])
 {
   if (!c) return fail("null context");
-  int rc = set_scalar(c, 1, 0.0);
-  if (rc) return rc;
+  int rc = 0;
   vcp::StageCtl ctl{1, 0, 0.0, 0.0, mon ? 1 : 0};
   if ((rc = residual_pass(c, ctl))) return rc;
   double mach2 = 0.0;
@@ -422,8 +417,6 @@ int vcp_residual(vcp_ctx *c, int mon, double md[8])
 int vcp_time_steps(vcp_ctx *c, double *dt_elem)
 {
   if (!c) return fail("null context");
-  int rc = set_scalar(c, 0, 1e10);
-  if (rc) return rc;
   HIP_TRY(vcp::launch_time_step(c->dc, c->stream));
   if (dt_elem)
     HIP_TRY(hipMemcpyAsync(dt_elem, c->dc.dtElem, c->dc.nElem * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -455,7 +448,6 @@ int vcp_sync_step(vcp_ctx *c, double md[8])
     vcp::StageCtl s1{0, 1, alphaRK[1], betaRK[1] * dt, 0};
     vcp::StageCtl s2{0, 1, alphaRK[2], betaRK[2] * dt, 0};
     vcp::StageCtl s3{1, 0, 0.0, 0.0, reached ? 1 : 0};
-    if (reached && (rc = set_scalar(c, 1, 0.0))) return rc;
     if ((rc = residual_pass(c, s1))) return rc;
     if ((rc = residual_pass(c, s2))) return rc;
     if ((rc = residual_pass(c, s3))) return rc;

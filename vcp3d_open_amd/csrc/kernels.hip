@@ -201,14 +201,6 @@ __device__ inline void ent2prim(const double V[5], double &rho, double &u, doubl
   p = -rho * V4Inv;
 }
 
-__device__ inline void atomic_max_pos(double *addr, double v)
-{
-  atomicMax(reinterpret_cast<unsigned long long *>(addr), (unsigned long long) __double_as_longlong(v));
-}
-__device__ inline void atomic_min_pos(double *addr, double v)
-{
-  atomicMin(reinterpret_cast<unsigned long long *>(addr), (unsigned long long) __double_as_longlong(v));
-}
 
 // Ismail-Roe + viscous + interior-penalty + symmetrising fluxes at one face
 // point: VL/VR entropy states (L -> R along nn), gx/gy/gz the averaged
@@ -589,6 +581,7 @@ struct VolArgs {
   const double *faceRes, *tables, *vgeo, *econst;
   double *scal;
   int *flags, *cgIter;
+  double *mach2Elem;                      // [nElem] maximum Mach^2 (monitoring)
   const int *spIdx;                       // sponge slot per element (null: no sponge)
   const double *spCoef, *dampFn, *dampState;
   double *srcBuf;
@@ -856,10 +849,6 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
       for (int l = 0; l < 5; ++l) sb[l * NI + m] = src[l];
     }
   }
-  if (a.ctl.monitor) {
-    mach2 = wave_max(mach2);
-    if ((tid & 63) == 0) atomic_max_pos(&a.scal[1], mach2);
-  }
   __syncthreads();
 
   // face slots of this element, summed while the backward contractions run
@@ -990,7 +979,13 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
   double r0 = 0.0;
 #pragma unroll
   for (int l = 0; l < 5; ++l) { x[l] = 0.0; r0 = fmax(r0, fabs(r[l])); }
-  { double red[1] = {r0}; block_reduce1<NT, 1, 0>(red, sRed, half); r0 = red[0]; }
+  {
+    // with the element's maximum Mach^2 for the monitoring data
+    double red[2] = {r0, mach2};
+    block_reduce1<NT, 2, 0>(red, sRed, half);
+    r0 = red[0];
+    if (a.ctl.monitor && tid == 0) a.mach2Elem[e] = red[1];
+  }
   int iters = 0;
   if (r0 * JacInv > 1e-10) {
     double *cz = sBig, *cy = cz + SZ, *pI = cy + SY, *by = pI + 5 * NI;
@@ -1663,7 +1658,39 @@ __global__ __launch_bounds__(64) void time_step_kernel(DtArgs a)
     if (a2min < 0.0) atomicOr(a.flags, kFlagNegativeA2);
     const double dt = a.cfl * a.u_ref / (f1[a.P] * radInv + f2[a.P] * radVis);
     a.dtElem[e] = dt;
-    atomic_min_pos(&a.scal[0], dt);
+  }
+}
+
+// Minimum of the per-element time steps into scal[0] (one workgroup; a
+// fixed reduction order, no atomics on one address from every element).
+__global__ __launch_bounds__(1024) void dt_min_kernel(const double *dtElem, int n, double *scal)
+{
+  __shared__ double red[16];
+  double m = 1e10;
+  for (int i = threadIdx.x; i < n; i += 1024) m = fmin(m, dtElem[i]);
+  m = -wave_max(-m);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = red[0];
+    for (int w = 1; w < 16; ++w) r = fmin(r, red[w]);
+    scal[0] = r;
+  }
+}
+
+// Maximum of n non-negative values into *out (one workgroup).
+__global__ __launch_bounds__(1024) void reduce_max_kernel(const double *v, int n, double *out)
+{
+  __shared__ double red[16];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) m = fmax(m, v[i]);
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = red[0];
+    for (int w = 1; w < 16; ++w) r = fmax(r, red[w]);
+    *out = r;
   }
 }
 
@@ -1717,10 +1744,13 @@ hipError_t volume_impl(const DeviceCase &dc, const StageCtl &ctl, hipStream_t s)
   a.U = dc.U; a.U0 = dc.U0; a.R = dc.R; a.faceRes = dc.faceRes; a.tables = dc.tables;
   a.vgeo = dc.vgeo; a.econst = dc.econst; a.scal = dc.scal; a.flags = dc.flags; a.cgIter = dc.cgIter;
   a.to = dc.to; a.nElem = dc.nElem; a.bug = dc.bug_compat; a.mu = dc.mu; a.ctl = ctl;
+  a.mach2Elem = dc.mach2Elem;
   a.spIdx = dc.nSponge ? dc.spIdx : nullptr; a.spCoef = dc.spCoef; a.dampFn = dc.dampFn;
   a.dampState = dc.dampState; a.srcBuf = dc.srcBuf; a.cml = dc.cml;
   for (int q = 0; q < K; ++q) a.c0[q] = dc.opC0[q];
   hipLaunchKernelGGL((volume_kernel<K, M, 256>), dim3(dc.nElem), dim3(256), 0, s, a);
+  if (ctl.monitor)
+    hipLaunchKernelGGL(reduce_max_kernel, dim3(1), dim3(1024), 0, s, dc.mach2Elem, dc.nElem, dc.scal + 1);
   return hipGetLastError();
 }
 
@@ -1732,6 +1762,7 @@ hipError_t dt_impl(const DeviceCase &dc, hipStream_t s)
   a.flags = dc.flags; a.to = dc.to; a.nElem = dc.nElem; a.P = dc.P; a.cfl = dc.cfl;
   a.u_ref = dc.u_ref; a.p_ref = dc.p_ref; a.rho_ref = dc.rho_ref; a.muDim = dc.mu * dc.p_ref / dc.u_ref;
   hipLaunchKernelGGL((time_step_kernel<K>), dim3(dc.nElem), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(dt_min_kernel, dim3(1), dim3(1024), 0, s, dc.dtElem, dc.nElem, dc.scal);
   return hipGetLastError();
 }
 

@@ -50,10 +50,11 @@ struct DeviceCase {
   int *nbr = nullptr;
   double *U = nullptr, *U0 = nullptr, *R = nullptr;
   double *faceRes = nullptr;      // [nElem][6][5][nDOF]
-  double *scal = nullptr;         // device scalars: [0] dt min (as bits), [1] Mach^2 max (bits)
+  double *scal = nullptr;         // device scalars: [0] minimum time step, [1] maximum Mach^2
   int *flags = nullptr;
   int *cgIter = nullptr;          // [nElem]
   double *dtElem = nullptr;       // [nElem]
+  double *mach2Elem = nullptr;    // [nElem] maximum Mach^2 of the last monitored residual
   // open boundaries
   int nBFace = 0;
   int *bface = nullptr;           // [nBFace][2]: element, block face

@@ -649,20 +649,39 @@ __device__ __forceinline__ void cml_source(const double V[5], const double gx[5]
 // sum-factorisation intermediates and the primitive state at the points;
 // the 1D operators come from the kernel arguments (scalar registers) in the
 // pencil stages and from LDS rows in the per-point stages.
+constexpr int cmax(int x, int y) { return x > y ? x : y; }
+
+// LDS layout of the volume kernel and the waves per SIMD it is compiled for
+// (NT = 256: three workgroups per CU when the LDS fits three times into 160 KB,
+// which caps the kernel at 168 VGPRs).
 template <int K, int M, int NT>
-__global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
+struct VolShape {
+  static constexpr int ND = K * K * K, NI = M * M * M, NR = M * M;
+  // Points go through the pointwise stages in two passes split at a row
+  // (jk) boundary: rows [0, R0) in the first, the remaining rows in the second.
+  static constexpr int R0 = (NT / M) < NR ? (NT / M) : NR;
+  static constexpr int P0 = R0 * M, P1 = NI - P0;
+  static constexpr int SZ = 5 * M * K * K;          // one z-stage buffer
+  static constexpr int SY = 15 * NR * K;            // y-stage buffers, [jk][3][5][K]
+  static constexpr int SF0 = 15 * P0, SF = 15 * NI; // fluxes of the first pass / all
+  static constexpr int SI = 15 * NR * K;            // backward i-stage output
+  static constexpr int BIG = cmax(cmax(2 * SZ + SY, SF), SI + 10 * M * K * K);
+  static constexpr int LDS = 8 * (2 * M * K + 5 * ND + BIG + 45 + 6 * (NT / 64) + 15);
+  static constexpr int WAVES = 3 * LDS <= 160 * 1024 ? 3 : 2;
+};
+
+template <int K, int M, int NT>
+__global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel(VolArgs<K, M> a)
 {
-  constexpr int ND = K * K * K, NI = M * M * M;
-  constexpr int DPT = (5 * ND + NT - 1) / NT;      // modal entries per thread
-  constexpr int SZ = 5 * M * K * K;                // z-stage buffer
-  constexpr int SY = 5 * M * M * K;                // y-stage buffer
-  constexpr int SF = 15 * NI;                      // weighted parametric fluxes
-  constexpr int SB = SF + 10 * M * K * K;          // fluxes + backward y-stage
-  constexpr int BIG = (2 * SZ + 3 * SY) > SB ? (2 * SZ + 3 * SY) : SB;
+  using S = VolShape<K, M, NT>;
+  constexpr int ND = S::ND, NI = S::NI, NR = S::NR, R0 = S::R0, P0 = S::P0, P1 = S::P1;
+  constexpr int SZ = S::SZ, SF0 = S::SF0, SI = S::SI, BIG = S::BIG;
+  static_assert(P1 <= NT, "two point passes");
+  // the first pass's fluxes must leave the y-stage rows of the second pass intact
+  static_assert(P1 == 0 || SF0 <= 2 * SZ + 15 * R0 * K, "flux / y-stage overlap");
   __shared__ double sA[M * K], sD[M * K];
   __shared__ double sU[5 * ND];
   __shared__ double sBig[BIG];
-  __shared__ double sPrim[6 * NI];                 // rho, u, v, w, p, w*J at the points
   __shared__ double sCm[5][9];
   __shared__ double sRed[2 * (NT / 64) * 3];
   __shared__ double sDv[15];
@@ -679,7 +698,9 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
   __syncthreads();
 
   // ---------------- step 1: solution and parametric gradients at the points
-  double *zA = sBig, *zD = zA + SZ, *yAA = zD + SZ, *yDA = yAA + SY, *yAD = yDA + SY;
+  // z-stage buffers, then the y-stage buffers row-major: yB[jk][3][5][K] with
+  // the (A, A), (D, A), (A, D) contractions of variable v at row jk = k * M + j
+  double *zA = sBig, *zD = zA + SZ, *yB = zD + SZ;
   for (int pc = tid; pc < 5 * K * K; pc += NT) {      // pencil (v, j, i) along k
     const int v = pc / (K * K), ji = pc - v * K * K;
     double x[K];
@@ -730,78 +751,85 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
   __syncthreads();
   for (int pc = tid; pc < 5 * M * K; pc += NT) {      // pencil (v, k', i) along j
     const int vk = pc / K, i = pc - vk * K;
+    const int v = vk / M, kp = vk - v * M;
     double xa[K], xd[K];
 #pragma unroll
     for (int q = 0; q < K; ++q) { xa[q] = zA[(vk * K + q) * K + i]; xd[q] = zD[(vk * K + q) * K + i]; }
+    double *yo = yB + (kp * M * 15 + v) * K + i;
 #pragma unroll
     for (int n = 0; n < M; ++n) {
       double s0 = 0.0, s1 = 0.0, s2 = 0.0;
 #pragma unroll
       for (int q = 0; q < K; ++q) { s0 += A[n * K + q] * xa[q]; s1 += D[n * K + q] * xa[q]; s2 += A[n * K + q] * xd[q]; }
-      yAA[(vk * M + n) * K + i] = s0;
-      yDA[(vk * M + n) * K + i] = s1;
-      yAD[(vk * M + n) * K + i] = s2;
+      yo[(n * 15 + 0) * K] = s0;
+      yo[(n * 15 + 5) * K] = s1;
+      yo[(n * 15 + 10) * K] = s2;
     }
   }
   __syncthreads();
-  // per-point x stage; results held until the flux buffer may overwrite the
-  // y-stage buffers
-  constexpr int PPT = (NI + NT - 1) / NT;
-  double Vp[PPT][5], Gr[PPT][5], Gs[PPT][5], Gt[PPT][5];
+
+  // ---------------- steps 2, 3, 5, 6: x stage, pointwise fluxes, primitive
+  // state, monitoring; in two passes over the points (rows [0, R0), then the
+  // rest), each thread one point per pass. The first pass's fluxes F[b][m]
+  // (b = 5 d + l) fill [0, SF0) and leave the y-stage rows of the second
+  // pass in place; the second pass's fluxes go to SF0 + [b][m - P0].
+  const double factHF = a.mu * kGam / kPrLam;
+  double mach2 = 0.0;
+  // (w J rho, u, v, w, w J p) at the thread's points, for dU/dV in the solve
+  double prim[2][5];
 #pragma unroll
-  for (int q = 0; q < PPT; ++q) {
-    const int m = tid + q * NT;
-    if (m < NI) {
+  for (int q = 0; q < (P1 > 0 ? 2 : 1); ++q) {
+    const int m = q == 0 ? tid : P0 + tid;
+    const bool act = q == 0 ? tid < P0 : tid < P1;
+    double Vp[5], Gr[5], Gs[5], Gt[5];
+    if (act) {
       const int i = m % M, jk = m / M;
       double ai[K], di[K];
 #pragma unroll
       for (int ii = 0; ii < K; ++ii) { ai[ii] = sA[i * K + ii]; di[ii] = sD[i * K + ii]; }
+      const double *row = yB + jk * 15 * K;
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
-        const double *ra = &yAA[(v * M * M + jk) * K], *rd = &yDA[(v * M * M + jk) * K],
-                     *rz = &yAD[(v * M * M + jk) * K];
+        const double *ra = row + v * K, *rd = row + (5 + v) * K, *rz = row + (10 + v) * K;
         double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
 #pragma unroll
         for (int ii = 0; ii < K; ++ii) {
           const double x = ra[ii];
           s0 += ai[ii] * x; s1 += di[ii] * x; s2 += ai[ii] * rd[ii]; s3 += ai[ii] * rz[ii];
         }
-        Vp[q][v] = s0; Gr[q][v] = s1; Gs[q][v] = s2; Gt[q][v] = s3;
+        Vp[v] = s0; Gr[v] = s1; Gs[v] = s2; Gt[v] = s3;
       }
     }
-  }
-  __syncthreads();
-
-  // ---------------- steps 3, 5, 6: pointwise fluxes, primitive state, monitoring
-  double *F0 = sBig, *F1 = F0 + 5 * NI, *F2 = F1 + 5 * NI;
-  const double factHF = a.mu * kGam / kPrLam;
-  double mach2 = 0.0;
-#pragma unroll
-  for (int q = 0; q < PPT; ++q) {
-    const int m = tid + q * NT;
-    if (m >= NI) continue;
+    __syncthreads();
+    if (!act) continue;
+    double *Fb = q == 0 ? sBig + m : sBig + SF0 + (m - P0);
+    const int fstride = q == 0 ? P0 : P1;
     const double J = vg[m];
     const double wJ = a.tables[a.to.w3 + m] * J;
+    double cm[9];                                      // dr/dx .. dt/dz at the point
+#pragma unroll
+    for (int t = 0; t < 9; ++t) cm[t] = vg[(1 + t) * NI + m];
     double gx[5], gy[5], gz[5];
+    if (a.bug) {
 #pragma unroll
-    for (int v = 0; v < 5; ++v) {
-      double c[9];
-      if (a.bug) {
-#pragma unroll
-        for (int t = 0; t < 9; ++t) c[t] = sCm[v][t];
-      } else {
-#pragma unroll
-        for (int t = 0; t < 9; ++t) c[t] = vg[(1 + t) * NI + m];
+      for (int v = 0; v < 5; ++v) {
+        const double *c = sCm[v];
+        gx[v] = Gr[v] * c[0] + Gs[v] * c[3] + Gt[v] * c[6];
+        gy[v] = Gr[v] * c[1] + Gs[v] * c[4] + Gt[v] * c[7];
+        gz[v] = Gr[v] * c[2] + Gs[v] * c[5] + Gt[v] * c[8];
       }
-      gx[v] = Gr[q][v] * c[0] + Gs[q][v] * c[3] + Gt[q][v] * c[6];
-      gy[v] = Gr[q][v] * c[1] + Gs[q][v] * c[4] + Gt[q][v] * c[7];
-      gz[v] = Gr[q][v] * c[2] + Gs[q][v] * c[5] + Gt[q][v] * c[8];
+    } else {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        gx[v] = Gr[v] * cm[0] + Gs[v] * cm[3] + Gt[v] * cm[6];
+        gy[v] = Gr[v] * cm[1] + Gs[v] * cm[4] + Gt[v] * cm[7];
+        gz[v] = Gr[v] * cm[2] + Gs[v] * cm[5] + Gt[v] * cm[8];
+      }
     }
     double rho, u, v, w, p, eKin;
-    ent2prim(Vp[q], rho, u, v, w, p, eKin);
-    sPrim[0 * NI + m] = rho; sPrim[1 * NI + m] = u; sPrim[2 * NI + m] = v;
-    sPrim[3 * NI + m] = w; sPrim[4 * NI + m] = p; sPrim[5 * NI + m] = wJ;
-    const double V4Inv = 1.0 / Vp[q][4];
+    ent2prim(Vp, rho, u, v, w, p, eKin);
+    prim[q][0] = wJ * rho; prim[q][1] = u; prim[q][2] = v; prim[q][3] = w; prim[q][4] = wJ * p;
+    const double V4Inv = 1.0 / Vp[4];
     const double rE = p / kGm1 + rho * eKin;
     const double dudx = -V4Inv * (gx[1] + u * gx[4]), dudy = -V4Inv * (gy[1] + u * gy[4]), dudz = -V4Inv * (gz[1] + u * gz[4]);
     const double dvdx = -V4Inv * (gx[2] + v * gx[4]), dvdy = -V4Inv * (gy[2] + v * gy[4]), dvdz = -V4Inv * (gz[2] + v * gz[4]);
@@ -821,11 +849,9 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
     const double weight = -wJ;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const double mx = weight * vg[(1 + 3 * d) * NI + m], my = weight * vg[(2 + 3 * d) * NI + m],
-                   mz = weight * vg[(3 + 3 * d) * NI + m];
-      double *Fd = d == 0 ? F0 : d == 1 ? F1 : F2;
+      const double mx = weight * cm[3 * d], my = weight * cm[3 * d + 1], mz = weight * cm[3 * d + 2];
 #pragma unroll
-      for (int l = 0; l < 5; ++l) Fd[l * NI + m] = fx[l] * mx + fy[l] * my + fz[l] * mz;
+      for (int l = 0; l < 5; ++l) Fb[(d * 5 + l) * fstride] = fx[l] * mx + fy[l] * my + fz[l] * mz;
     }
     if (sp >= 0) {
       // sponge damping sigma J w (U - U_target) and CML sources
@@ -842,7 +868,7 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
         const double *cf = a.spCoef + (size_t) sp * 8;
         const double wgt[3] = {weight * cf[0] * fn[m], weight * cf[1] * fn[NI + m], weight * cf[2] * fn[2 * NI + m]};
         const double nsg[3] = {cf[3], cf[4], cf[5]};
-        cml_source(Vp[q], gx, gy, gz, wgt, nsg, src);
+        cml_source(Vp, gx, gy, gz, wgt, nsg, src);
       }
       double *sb = a.srcBuf + (size_t) sp * 5 * NI;
 #pragma unroll
@@ -868,32 +894,51 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
   }
 
   // ---------------- step 7: transposed contractions of the three fluxes
-  for (int r = tid; r < 15 * M * M; r += NT) {        // rows along i, in place
-    const int d = r / (5 * M * M);
-    double x[M];
+  // rows (b, jk) along i, compacted to I[b][jk][K] once every row is read
+  {
+    constexpr int NRW = 15 * NR, RPT = (NRW + NT - 1) / NT;
+    double o[RPT][K];
 #pragma unroll
-    for (int q = 0; q < M; ++q) x[q] = sBig[r * M + q];
+    for (int t = 0; t < RPT; ++t) {
+      const int r = tid + t * NT;
+      if (r < NRW) {
+        const int b = r / NR, jk = r - b * NR;
+        const double *x = jk < R0 ? sBig + b * P0 + jk * M : sBig + SF0 + b * P1 + (jk - R0) * M;
+        double xx[M];
 #pragma unroll
-    for (int n = 0; n < K; ++n) {
-      double s = 0.0;
-      if (d == 0) {
+        for (int q = 0; q < M; ++q) xx[q] = x[q];
 #pragma unroll
-        for (int q = 0; q < M; ++q) s += D[q * K + n] * x[q];
-      } else {
+        for (int n = 0; n < K; ++n) {
+          double s = 0.0;
+          if (b < 5) {
 #pragma unroll
-        for (int q = 0; q < M; ++q) s += A[q * K + n] * x[q];
+            for (int q = 0; q < M; ++q) s += D[q * K + n] * xx[q];
+          } else {
+#pragma unroll
+            for (int q = 0; q < M; ++q) s += A[q * K + n] * xx[q];
+          }
+          o[t][n] = s;
+        }
       }
-      sBig[r * M + n] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < RPT; ++t) {
+      const int r = tid + t * NT;
+      if (r < NRW)
+#pragma unroll
+        for (int n = 0; n < K; ++n) sBig[r * K + n] = o[t][n];
     }
   }
   __syncthreads();
-  double *Y01 = sBig + SF, *Y2 = Y01 + 5 * M * K * K;   // after the flux block
+  const double *I0 = sBig, *I1 = I0 + 5 * NR * K, *I2 = I1 + 5 * NR * K;
+  double *Y01 = sBig + SI, *Y2 = Y01 + 5 * M * K * K;   // after the i-stage block
   for (int pc = tid; pc < 5 * M * K; pc += NT) {      // pencil (v, k, i') along j
     const int vk = pc / K, i = pc - vk * K;
     double x0[M], x1[M], x2[M];
 #pragma unroll
     for (int q = 0; q < M; ++q) {
-      x0[q] = F0[(vk * M + q) * M + i]; x1[q] = F1[(vk * M + q) * M + i]; x2[q] = F2[(vk * M + q) * M + i];
+      x0[q] = I0[(vk * M + q) * K + i]; x1[q] = I1[(vk * M + q) * K + i]; x2[q] = I2[(vk * M + q) * K + i];
     }
 #pragma unroll
     for (int n = 0; n < K; ++n) {
@@ -988,7 +1033,7 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
   }
   int iters = 0;
   if (r0 * JacInv > 1e-10) {
-    double *cz = sBig, *cy = cz + SZ, *pI = cy + SY, *by = pI + 5 * NI;
+    double *cz = sBig, *cy = cz + SZ, *pI = cy + 5 * M * M * K, *by = pI + 5 * NI;
     // z = dV/dU(centre) r, local to the thread owning the DOF
 #define VCP_PRECOND()                                                                             \
     do {                                                                                          \
@@ -1037,24 +1082,27 @@ __global__ __launch_bounds__(NT) void volume_kernel(VolArgs<K, M> a)
         }
       }
       __syncthreads();
-      for (int m = tid; m < NI; m += NT) {
+#pragma unroll
+      for (int q = 0; q < (P1 > 0 ? 2 : 1); ++q) {
+        const int m = q == 0 ? tid : P0 + tid;
+        if (!(q == 0 ? tid < P0 : tid < P1)) continue;
         const int i = m % M, jk = m / M;
         double ai[K];
 #pragma unroll
         for (int ii = 0; ii < K; ++ii) ai[ii] = sA[i * K + ii];
         double pp[5];
-        const double wJ = sPrim[5 * NI + m];
 #pragma unroll
         for (int l = 0; l < 5; ++l) {
           const double *ra = &cy[(l * M * M + jk) * K];
           double s = 0.0;
 #pragma unroll
           for (int ii = 0; ii < K; ++ii) s += ai[ii] * ra[ii];
-          pp[l] = wJ * s;
+          pp[l] = s;
         }
-        // dU/dV at the point from the primitive state (ElementClass.cpp:4228-4253)
-        const double rho = sPrim[m], u = sPrim[NI + m], v = sPrim[2 * NI + m], w = sPrim[3 * NI + m],
-                     p = sPrim[4 * NI + m];
+        // w J dU/dV at the point (ElementClass.cpp:4228-4253): every entry is
+        // homogeneous of degree one in (rho, p), so the scaled pair held in
+        // prim gives the weighted matrix directly
+        const double rho = prim[q][0], u = prim[q][1], v = prim[q][2], w = prim[q][3], p = prim[q][4];
         const double eKin = 0.5 * (u * u + v * v + w * w);
         const double rE = p / kGm1 + rho * eKin, rH = rE + p;
         const double ru = rho * u, rv = rho * v, rw = rho * w;

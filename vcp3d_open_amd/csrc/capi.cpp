@@ -227,7 +227,10 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
   if (he == hipSuccess) he = c->alloc(&dc.U, ns);
   if (he == hipSuccess) he = c->alloc(&dc.U0, ns);
   if (he == hipSuccess) he = c->alloc(&dc.R, ns);
-  if (he == hipSuccess) he = c->alloc(&dc.faceRes, 6 * ns);
+  // face slots: per element and block face, the value and normal-derivative
+  // coefficient planes [2][5][K][K] of the lifted face residual
+  const size_t nslot = (size_t) nElem * 6 * 10 * dc.K * dc.K;
+  if (he == hipSuccess) he = c->alloc(&dc.faceRes, nslot);
   if (he == hipSuccess) he = c->alloc(&dc.scal, 8);
   if (he == hipSuccess) he = c->alloc(&dc.flags, 4);
   if (he == hipSuccess) he = c->alloc(&dc.cgIter, (size_t) nElem);
@@ -260,7 +263,7 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
   if (he == hipSuccess) he = hipMemcpy(dc.nbr, g.nbr.data(), g.nbr.size() * sizeof(int), hipMemcpyHostToDevice);
   if (he == hipSuccess) he = hipMemset(dc.U, 0, ns * sizeof(double));
   if (he == hipSuccess) he = hipMemset(dc.R, 0, ns * sizeof(double));
-  if (he == hipSuccess) he = hipMemset(dc.faceRes, 0, 6 * ns * sizeof(double));
+  if (he == hipSuccess) he = hipMemset(dc.faceRes, 0, nslot * sizeof(double));
   if (he == hipSuccess) he = hipMemset(dc.flags, 0, 4 * sizeof(int));
   if (he != hipSuccess) {
     const std::string msg = std::string("device set-up failed: ") + hipGetErrorString(he);

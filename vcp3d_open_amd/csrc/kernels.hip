@@ -342,7 +342,7 @@ struct FaceArgs {
 // points from L to the element. LDS regions are reused across the phases:
 //   R1: normal contraction B -> traces/gradients at the points -> lift stage 1
 //   R2: fast contraction T -> flux and Cartesian symmetrising fluxes at the
-//       points -> lift stage 2
+//       points
 // The symmetrising fluxes are mapped to each side's parametric slots inside
 // the first lift stage.
 template <int K, int M>
@@ -364,7 +364,6 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<K, M> a)
   __shared__ FaceSmem<K, M> sm;
   constexpr int ND = K * K * K, N2 = M * M;
   static_assert(2 * 2 * 5 * K * K <= sizeof(sm.R1) / 8, "R1 holds the normal contraction");
-  static_assert(2 * 2 * 5 * K * K <= sizeof(sm.R2) / 8, "R2 holds the second lift stage");
   // strides of the (normal, fast, slow) axes in the [k][j][i] DOF layout
   constexpr int SN = DIR == 0 ? 1 : DIR == 1 ? K : K * K;
   constexpr int SFA = DIR == 0 ? K : 1;
@@ -379,7 +378,6 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<K, M> a)
   // R2 views
   double *T = sm.R2;                          // [side][A|D|N][5][K][M]
   double *sFlux = sm.R2, *sFC = sm.R2 + 5 * N2;  // flux [5][N2], Cartesian sym [x|y|z][5][N2]
-  double *G2 = sm.R2;                         // lift stage 2: [side][0|1][5][K][K]
   const int tid = threadIdx.x;
   const double *A = a.op.A, *D = a.op.D;
   for (int i = tid; i < M * K; i += 64) { sA[i] = A[i]; sD[i] = D[i]; }
@@ -524,11 +522,14 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<K, M> a)
     }
   }
   __syncthreads();
-  // slow-axis contraction G2[side][0|1][v][s][f]
+  // slow-axis contraction straight into the face slots of both elements: the
+  // value and normal-derivative coefficient planes [0|1][v][s][f]; the
+  // volume kernel applies the face vectors along the normal
   for (int pc = tid; pc < 2 * 5 * K; pc += 64) {
     const int side = pc / (5 * K), r = pc - side * 5 * K;
     const int v = r / K, f = r - v * K;
     const double *Ps = P1 + side * 3 * 5 * M * K;
+    double *out = a.faceRes + ((size_t) (side ? L : e) * 6 + 2 * DIR + side) * 10 * K * K;
     double x1[M], x2[M], x3[M];
 #pragma unroll
     for (int q = 0; q < M; ++q) {
@@ -540,27 +541,8 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<K, M> a)
       double g1 = 0.0, g2 = 0.0;
 #pragma unroll
       for (int q = 0; q < M; ++q) { g1 += A[q * K + n] * x1[q] + D[q * K + n] * x2[q]; g2 += A[q * K + n] * x3[q]; }
-      G2[(side * 2 + 0) * 5 * K * K + (v * K + n) * K + f] = g1;
-      G2[(side * 2 + 1) * 5 * K * K + (v * K + n) * K + f] = g2;
-    }
-  }
-  __syncthreads();
-  // normal direction: outer product with the face vectors, into the slots
-#pragma unroll
-  for (int side = 0; side < 2; ++side) {
-    double *out = a.faceRes + ((size_t) (side ? L : e) * 6 + 2 * DIR + side) * 5 * ND;
-    const double *Fv = a.op.F[side], *dFv = a.op.F[2 + side];
-    for (int idx = tid; idx < 5 * ND; idx += 64) {
-      const int v = idx / ND, d = idx - v * ND;
-      const int i = d % K, j = (d / K) % K, k = d / (K * K);
-      const int t = DIR == 0 ? i : DIR == 1 ? j : k;
-      const int f = DIR == 0 ? j : i;
-      const int sl = DIR == 2 ? j : k;
-      const int gi = (v * K + sl) * K + f;
-      double fv = 0.0, dfv = 0.0;
-#pragma unroll
-      for (int q = 0; q < K; ++q) if (q == t) { fv = Fv[q]; dfv = dFv[q]; }
-      out[idx] = fv * G2[(side * 2 + 0) * 5 * K * K + gi] + dfv * G2[(side * 2 + 1) * 5 * K * K + gi];
+      out[(v * K + n) * K + f] = g1;
+      out[5 * K * K + (v * K + n) * K + f] = g2;
     }
   }
 }
@@ -591,6 +573,7 @@ struct VolArgs {
   StageCtl ctl;
   Ops<K, M> op;
   double c0[K];                           // phi_q(0): basis at the element centre
+  double fvec[4][K];                      // phi(-1), phi(+1), phi'(-1), phi'(+1)
 };
 
 // Characteristic-matching-layer source at one point (ElementClass.cpp:
@@ -666,7 +649,7 @@ struct VolShape {
   static constexpr int SF0 = 15 * P0, SF = 15 * NI; // fluxes of the first pass / all
   static constexpr int SI = 15 * NR * K;            // backward i-stage output
   static constexpr int BIG = cmax(cmax(2 * SZ + SY, SF), SI + 10 * M * K * K);
-  static constexpr int LDS = 8 * (2 * M * K + 5 * ND + BIG + 45 + 6 * (NT / 64) + 15);
+  static constexpr int LDS = 8 * (2 * M * K + 5 * ND + BIG + 45 + 6 * (NT / 64) + 15 + 4 * K);
   static constexpr int WAVES = 3 * LDS <= 160 * 1024 ? 3 : 2;
 };
 
@@ -685,6 +668,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
   __shared__ double sCm[5][9];
   __shared__ double sRed[2 * (NT / 64) * 3];
   __shared__ double sDv[15];
+  __shared__ double sFv[4 * K];
   const int e = blockIdx.x;
   const int tid = threadIdx.x;
   const double *A = a.op.A, *D = a.op.D;
@@ -692,6 +676,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
   const double *vg = a.vgeo + (size_t) e * kVolGeo * NI;
   const int sp = a.spIdx ? a.spIdx[e] : -1;
   for (int i = tid; i < M * K; i += NT) { sA[i] = A[i]; sD[i] = D[i]; }
+  for (int i = tid; i < 4 * K; i += NT) sFv[i] = a.fvec[i / K][i % K];
   for (int i = tid; i < 5 * ND; i += NT) sU[i] = Ue[i];
   if (a.bug)
     for (int i = tid; i < 45; i += NT) sCm[i / 9][i % 9] = vg[(1 + i % 9) * NI + i / 9];
@@ -877,19 +862,25 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
   }
   __syncthreads();
 
-  // face slots of this element, summed while the backward contractions run
-  // (DOF-major: thread d < ND owns DOF d of all five variables)
+  // the six face contributions at this thread's DOF (DOF-major: thread
+  // d < ND owns DOF d of all five variables): each face slot holds the value
+  // and normal-derivative coefficient planes over its two tangential axes,
+  // times the face vectors phi(r_f), phi'(r_f) at the DOF's normal index
   static_assert(ND <= NT, "one DOF per thread in the mass-matrix solve");
-  double r[5];
-  {
-    const double *fr = a.faceRes + (size_t) e * 6 * 5 * ND;
+  double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (tid < ND) {
+    const int di = tid % K, dj = (tid / K) % K, dk = tid / (K * K);
+    const double *fr = a.faceRes + (size_t) e * 6 * 10 * K * K;
 #pragma unroll
-    for (int l = 0; l < 5; ++l) {
-      double s = 0.0;
-      if (tid < ND)
+    for (int f = 0; f < 6; ++f) {
+      const int dir = f >> 1, side = f & 1;
+      const int tn = dir == 0 ? di : dir == 1 ? dj : dk;
+      const int fa = dir == 0 ? dj : di;
+      const int sl = dir == 2 ? dj : dk;
+      const double cv = sFv[side * K + tn], cd = sFv[(2 + side) * K + tn];
+      const double *b = fr + f * 10 * K * K + sl * K + fa;
 #pragma unroll
-        for (int f = 0; f < 6; ++f) s += fr[f * 5 * ND + l * ND + tid];
-      r[l] = s;
+      for (int l = 0; l < 5; ++l) r[l] += cv * b[l * K * K] + cd * b[(5 + l) * K * K];
     }
   }
 
@@ -1624,6 +1615,9 @@ __global__ __launch_bounds__(64) void nscbc_kernel(BndArgs<K, M> a)
     }
   }
   __syncthreads();
+  // slow axis, straight into the face slot: value and normal-derivative
+  // coefficient planes [0|1][v][slow][fast]
+  double *out = a.faceRes + ((size_t) e * 6 + f) * 10 * K * K;
   for (int pc = tid; pc < 5 * K; pc += 64) {
     const int v = pc / K, fi = pc - v * K;
     double x1[M], x2[M], x3[M];
@@ -1636,19 +1630,8 @@ __global__ __launch_bounds__(64) void nscbc_kernel(BndArgs<K, M> a)
       double g1 = 0.0, g2 = 0.0;
 #pragma unroll
       for (int q = 0; q < M; ++q) { g1 += sm.A[q * K + n] * x1[q] + sm.D[q * K + n] * x2[q]; g2 += sm.A[q * K + n] * x3[q]; }
-      sm.B[0][(v * K + n) * K + fi] = g1; sm.B[1][(v * K + n) * K + fi] = g2;
+      out[(v * K + n) * K + fi] = g1; out[5 * K * K + (v * K + n) * K + fi] = g2;
     }
-  }
-  __syncthreads();
-  double *out = a.faceRes + ((size_t) e * 6 + f) * 5 * ND;
-  for (int idx = tid; idx < 5 * ND; idx += 64) {
-    const int v = idx / ND, d = idx - v * ND;
-    const int i = d % K, j = (d / K) % K, k = d / (K * K);
-    const int tn = dir == 0 ? i : dir == 1 ? j : k;
-    const int fa = dir == 0 ? j : i;
-    const int sl = dir == 2 ? j : k;
-    const int gi = (v * K + sl) * K + fa;
-    out[idx] = sm.F[0][tn] * sm.B[0][gi] + sm.F[1][tn] * sm.B[1][gi];
   }
 }
 
@@ -1795,7 +1778,10 @@ hipError_t volume_impl(const DeviceCase &dc, const StageCtl &ctl, hipStream_t s)
   a.mach2Elem = dc.mach2Elem;
   a.spIdx = dc.nSponge ? dc.spIdx : nullptr; a.spCoef = dc.spCoef; a.dampFn = dc.dampFn;
   a.dampState = dc.dampState; a.srcBuf = dc.srcBuf; a.cml = dc.cml;
-  for (int q = 0; q < K; ++q) a.c0[q] = dc.opC0[q];
+  for (int q = 0; q < K; ++q) {
+    a.c0[q] = dc.opC0[q];
+    for (int t = 0; t < 4; ++t) a.fvec[t][q] = dc.opF[t][q];
+  }
   hipLaunchKernelGGL((volume_kernel<K, M, 256>), dim3(dc.nElem), dim3(256), 0, s, a);
   if (ctl.monitor)
     hipLaunchKernelGGL(reduce_max_kernel, dim3(1), dim3(1024), 0, s, dc.mach2Elem, dc.nElem, dc.scal + 1);

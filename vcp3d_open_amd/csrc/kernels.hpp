@@ -49,7 +49,7 @@ struct DeviceCase {
   double *vgeo = nullptr, *fgeo = nullptr, *econst = nullptr;
   int *nbr = nullptr;
   double *U = nullptr, *U0 = nullptr, *R = nullptr;
-  double *faceRes = nullptr;      // [nElem][6][5][nDOF]
+  double *faceRes = nullptr;      // [nElem][6][2][5][K][K] face-slot coefficient planes
   double *scal = nullptr;         // device scalars: [0] minimum time step, [1] maximum Mach^2
   int *flags = nullptr;
   int *cgIter = nullptr;          // [nElem]

@@ -40,6 +40,7 @@ constexpr double kGam = 1.4;
 constexpr double kGm1 = 0.4;
 constexpr double kPrLam = 0.72;
 constexpr double kLamOverMu = -2.0 / 3.0;
+constexpr double kInvGm1 = 1.0 / kGm1;
 
 // ------------------------------------------------------------ utilities
 __device__ inline double wave_sum(double v)
@@ -220,27 +221,29 @@ __device__ __forceinline__ void point_flux(const double VL[5], const double VR[5
     ent2prim(VR, rR, uR, vR, wR, pR, eR);
     const double dV0 = VR[0] - VL[0], dV1 = VR[1] - VL[1], dV2 = VR[2] - VL[2],
                  dV3 = VR[3] - VL[3], dV4 = VR[4] - VL[4];
-    const double z0L = sqrt(rL / pL), z0R = sqrt(rR / pR);
+    // sqrt(rho / p) = sqrt(-V4)
+    const double z0L = sqrt(-VL[4]), z0R = sqrt(-VR[4]);
     const double z4L = sqrt(rL * pL), z4R = sqrt(rR * pR);
     const double z0A = 0.5 * (z0L + z0R), z4A = 0.5 * (z4L + z4R);
     const double z1A = 0.5 * (uL * z0L + uR * z0R), z2A = 0.5 * (vL * z0L + vR * z0R),
                  z3A = 0.5 * (wL * z0L + wR * z0R);
-    double zeta = z0L / z0R, f = (zeta - 1.0) / (zeta + 1.0), uu = f * f, F;
-    F = (uu < 0.01) ? 1.0 + uu / 3.0 + uu * uu / 5.0 + uu * uu * uu / 7.0 + uu * uu * uu * uu / 9.0
-                    : log(zeta) / (2.0 * f);
+    // logarithmic means: F = log(zeta) / (2 f), f = (zeta - 1) / (zeta + 1)
+    // = (zL - zR) / (zL + zR), by its series for small f
+    constexpr double c3 = 1.0 / 3.0, c5 = 1.0 / 5.0, c7 = 1.0 / 7.0, c9 = 1.0 / 9.0;
+    double f = (z0L - z0R) / (z0L + z0R), uu = f * f, F;
+    F = (uu < 0.01) ? 1.0 + uu * (c3 + uu * (c5 + uu * (c7 + uu * c9))) : log(z0L / z0R) / (2.0 * f);
     const double z0Log = z0A / F;
-    zeta = z4L / z4R; f = (zeta - 1.0) / (zeta + 1.0); uu = f * f;
-    F = (uu < 0.01) ? 1.0 + uu / 3.0 + uu * uu / 5.0 + uu * uu * uu / 7.0 + uu * uu * uu * uu / 9.0
-                    : log(zeta) / (2.0 * f);
+    f = (z4L - z4R) / (z4L + z4R); uu = f * f;
+    F = (uu < 0.01) ? 1.0 + uu * (c3 + uu * (c5 + uu * (c7 + uu * c9))) : log(z4L / z4R) / (2.0 * f);
     const double z4Log = z4A / F;
     const double oz0 = 1.0 / z0A;
-    const double rA = z0A * z4Log, p1 = oz0 * z4A;
+    const double rA = z0A * z4Log, p1 = oz0 * z4A, orA = 1.0 / rA;
     const double p2 = 0.5 * ((kGam + 1.0) / kGam * z4Log / z0Log + kGm1 / kGam * p1);
     const double uA = oz0 * z1A, vA = oz0 * z2A, wAv = oz0 * z3A;
     const double vn = uA * nn[0] + vA * nn[1] + wAv * nn[2];
     const double kin = 0.5 * (uA * uA + vA * vA + wAv * wAv);
-    const double a2 = kGam * p2 / rA, aa = sqrt(a2);
-    const double H = a2 / kGm1 + kin, E = H - p2 / rA;
+    const double a2 = kGam * p2 * orA, aa = sqrt(a2), oa = 1.0 / aa;
+    const double H = a2 * kInvGm1 + kin, E = H - p2 * orA;
     double abv1 = rA * (uA * dV1 + vA * dV2 + wAv * dV3);
     double abv2 = abv1 + rA * (dV0 + H * dV4);
     const double dr = abv1 + rA * (dV0 + E * dV4);
@@ -252,8 +255,8 @@ __device__ __forceinline__ void point_flux(const double VL[5], const double VR[5
     const double abv3 = abv1 - lam3;
     const double abv4 = kGm1 * (kin * dr - uA * dru - vA * drv - wAv * drw + drE);
     const double abv5 = nn[0] * dru + nn[1] * drv + nn[2] * drw - vn * dr;
-    const double abv6 = abv3 * abv4 / a2 + abv2 * abv5 / aa;
-    const double abv7 = abv2 * abv4 / aa + abv3 * abv5;
+    const double abv6 = abv3 * abv4 * (oa * oa) + abv2 * abv5 * oa;
+    const double abv7 = abv2 * abv4 * oa + abv3 * abv5;
     fl[0] = wA * (rA * vn - (lam3 * dr + abv6));
     fl[1] = wA * (rA * vn * uA + p1 * nn[0] - (lam3 * dru + uA * abv6 + nn[0] * abv7));
     fl[2] = wA * (rA * vn * vA + p1 * nn[1] - (lam3 * drv + vA * abv6 + nn[1] * abv7));
@@ -267,7 +270,7 @@ __device__ __forceinline__ void point_flux(const double VL[5], const double VR[5
     const double dudx = -V4Inv * (gx[1] + u * gx[4]), dvdx = -V4Inv * (gx[2] + vv * gx[4]), dwdx = -V4Inv * (gx[3] + w * gx[4]);
     const double dudy = -V4Inv * (gy[1] + u * gy[4]), dvdy = -V4Inv * (gy[2] + vv * gy[4]), dwdy = -V4Inv * (gy[3] + w * gy[4]);
     const double dudz = -V4Inv * (gz[1] + u * gz[4]), dvdz = -V4Inv * (gz[2] + vv * gz[4]), dwdz = -V4Inv * (gz[3] + w * gz[4]);
-    const double fe = V4Inv * V4Inv / kGm1;
+    const double fe = V4Inv * V4Inv * kInvGm1;
     const double dedx = fe * gx[4], dedy = fe * gy[4], dedz = fe * gz[4];
     const double nx = wA * nn[0], ny = wA * nn[1], nz = wA * nn[2];
     const double lamDiv = kLamOverMu * (dudx + dvdy + dwdz);
@@ -281,7 +284,7 @@ __device__ __forceinline__ void point_flux(const double VL[5], const double VR[5
     fl[4] -= (u * txx + vv * txy + w * txz - qx) * nx + (u * txy + vv * tyy + w * tyz - qy) * ny
            + (u * txz + vv * tyz + w * tzz - qz) * nz;
   }
-  const double pOvRho = -V4Inv, eInt = pOvRho / kGm1;
+  const double pOvRho = -V4Inv, eInt = pOvRho * kInvGm1;
   const double dV1 = VL[1] - VR[1], dV2 = VL[2] - VR[2], dV3 = VL[3] - VR[3], dV4 = VL[4] - VR[4];
   const double factFlux = pOvRho * wA * mu;
   const double abvE = u * u + vv * vv + w * w + factHF * eInt;
@@ -350,6 +353,7 @@ struct FaceSmem {
   static constexpr int kR1a = 5 * 5 * M * M, kR1b = 2 * 3 * 5 * M * K;
   static constexpr int kR2a = 4 * 5 * M * M, kR2b = 2 * 3 * 5 * K * M;
   double A[M * K], D[M * K], W2[M * M];
+  double F[4][K];                  // phi(-1), phi(+1), phi'(-1), phi'(+1)
   double Cm[2][5][9];
   double R1[kR1a > kR1b ? kR1a : kR1b];
   double R2[kR2a > kR2b ? kR2a : kR2b];
@@ -379,8 +383,11 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<K, M> a)
   double *T = sm.R2;                          // [side][A|D|N][5][K][M]
   double *sFlux = sm.R2, *sFC = sm.R2 + 5 * N2;  // flux [5][N2], Cartesian sym [x|y|z][5][N2]
   const int tid = threadIdx.x;
-  const double *A = a.op.A, *D = a.op.D;
-  for (int i = tid; i < M * K; i += 64) { sA[i] = A[i]; sD[i] = D[i]; }
+  // operators in LDS: read as wave-uniform LDS operands in the contractions
+  // (as kernel arguments they would not fit the scalar registers)
+  for (int i = tid; i < M * K; i += 64) { sA[i] = a.op.A[i]; sD[i] = a.op.D[i]; }
+  for (int i = tid; i < 4 * K; i += 64) sm.F[i / K][i % K] = a.op.F[i / K][i % K];
+  const double *A = sA, *D = sD;
   for (int i = tid; i < N2; i += 64) sW2[i] = a.op.w[i % M] * a.op.w[i / M];
   const double *g = a.fgeo + ((size_t) e * 3 + DIR) * kFaceGeo * N2;
   if (a.bug)
@@ -389,11 +396,12 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<K, M> a)
       const int pt = l < N2 ? l : 0;
       sm.Cm[side][l][q] = g[(side ? 4 + q : 13 + q) * N2 + pt];
     }
+  __syncthreads();
   // normal contraction straight from HBM: B[side][0|1][v][s][f]
 #pragma unroll
   for (int side = 0; side < 2; ++side) {
     const double *Us = a.U + (size_t) (side ? L : e) * 5 * ND;
-    const double *Fv = a.op.F[side], *dFv = a.op.F[2 + side];
+    const double *Fv = sm.F[side], *dFv = sm.F[2 + side];
     for (int pc = tid; pc < 5 * K * K; pc += 64) {
       const int v = pc / (K * K), sl = (pc / K) % K, f = pc % K;
       const double *u = Us + v * ND + sl * SSL + f * SFA;
@@ -760,8 +768,9 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
   // pass in place; the second pass's fluxes go to SF0 + [b][m - P0].
   const double factHF = a.mu * kGam / kPrLam;
   double mach2 = 0.0;
-  // (w J rho, u, v, w, w J p) at the thread's points, for dU/dV in the solve
-  double prim[2][5];
+  // (w J rho, u, v, w, w J p, w J dU/dV_44) at the thread's points, for the
+  // weighted dU/dV in the solve
+  double prim[2][6];
 #pragma unroll
   for (int q = 0; q < (P1 > 0 ? 2 : 1); ++q) {
     const int m = q == 0 ? tid : P0 + tid;
@@ -813,13 +822,14 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
     }
     double rho, u, v, w, p, eKin;
     ent2prim(Vp, rho, u, v, w, p, eKin);
-    prim[q][0] = wJ * rho; prim[q][1] = u; prim[q][2] = v; prim[q][3] = w; prim[q][4] = wJ * p;
     const double V4Inv = 1.0 / Vp[4];
-    const double rE = p / kGm1 + rho * eKin;
+    const double rE = p * kInvGm1 + rho * eKin;
+    prim[q][0] = wJ * rho; prim[q][1] = u; prim[q][2] = v; prim[q][3] = w; prim[q][4] = wJ * p;
+    prim[q][5] = wJ * (rE * (rE + p) / rho + p * eKin);
     const double dudx = -V4Inv * (gx[1] + u * gx[4]), dudy = -V4Inv * (gy[1] + u * gy[4]), dudz = -V4Inv * (gz[1] + u * gz[4]);
     const double dvdx = -V4Inv * (gx[2] + v * gx[4]), dvdy = -V4Inv * (gy[2] + v * gy[4]), dvdz = -V4Inv * (gz[2] + v * gz[4]);
     const double dwdx = -V4Inv * (gx[3] + w * gx[4]), dwdy = -V4Inv * (gy[3] + w * gy[4]), dwdz = -V4Inv * (gz[3] + w * gz[4]);
-    const double fe = V4Inv * V4Inv / kGm1;
+    const double fe = V4Inv * V4Inv * kInvGm1;
     const double dedx = fe * gx[4], dedy = fe * gy[4], dedz = fe * gz[4];
     if (a.ctl.monitor) mach2 = fmax(mach2, rho * (u * u + v * v + w * w) / (kGam * p));
     const double ru = rho * u, rv = rho * v, rw = rho * w;
@@ -1094,10 +1104,10 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
         // homogeneous of degree one in (rho, p), so the scaled pair held in
         // prim gives the weighted matrix directly
         const double rho = prim[q][0], u = prim[q][1], v = prim[q][2], w = prim[q][3], p = prim[q][4];
+        const double d44 = prim[q][5];
         const double eKin = 0.5 * (u * u + v * v + w * w);
-        const double rE = p / kGm1 + rho * eKin, rH = rE + p;
+        const double rE = p * kInvGm1 + rho * eKin, rH = rE + p;
         const double ru = rho * u, rv = rho * v, rw = rho * w;
-        const double d44 = rE * rH / rho + p * eKin;
         pI[0 * NI + m] = rho * pp[0] + ru * pp[1] + rv * pp[2] + rw * pp[3] + rE * pp[4];
         pI[1 * NI + m] = ru * pp[0] + (ru * u + p) * pp[1] + ru * v * pp[2] + ru * w * pp[3] + rH * u * pp[4];
         pI[2 * NI + m] = rv * pp[0] + ru * v * pp[1] + (rv * v + p) * pp[2] + rv * w * pp[3] + rH * v * pp[4];

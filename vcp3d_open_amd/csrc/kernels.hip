@@ -148,23 +148,26 @@ __device__ inline double wave_reduce_dpp(double v)
 // Block reduction with one barrier: partial results go to alternating
 // halves of red (2 * NT/64 * NV doubles), so a half is rewritten only after
 // every thread has passed the barrier of the following reduction. The max
-// slots (q >= NS) take non-negative values.
-template <int NT, int NV, int NS>
+// slots (q >= NS) take non-negative values. Only the first NW waves hold
+// contributions; the others skip the wave reduction.
+template <int NT, int NV, int NS, int NW = NT / 64>
 __device__ inline void block_reduce1(double (&v)[NV], double *red, int &half)
 {
-#pragma unroll
-  for (int q = 0; q < NV; ++q) v[q] = (q < NS) ? wave_reduce_dpp<false>(v[q]) : wave_reduce_dpp<true>(v[q]);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   double *buf = red + half * (NT / 64) * NV;
-  if (lane == 0)
+  if (wid < NW) {
 #pragma unroll
-    for (int q = 0; q < NV; ++q) buf[wid * NV + q] = v[q];
+    for (int q = 0; q < NV; ++q) v[q] = (q < NS) ? wave_reduce_dpp<false>(v[q]) : wave_reduce_dpp<true>(v[q]);
+    if (lane == 0)
+#pragma unroll
+      for (int q = 0; q < NV; ++q) buf[wid * NV + q] = v[q];
+  }
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
     double s = buf[q];
 #pragma unroll
-    for (int w = 1; w < NT / 64; ++w) s = (q < NS) ? s + buf[w * NV + q] : fmax(s, buf[w * NV + q]);
+    for (int w = 1; w < NW; ++w) s = (q < NS) ? s + buf[w * NV + q] : fmax(s, buf[w * NV + q]);
     v[q] = s;
   }
   half ^= 1;
@@ -359,7 +362,7 @@ struct FaceSmem {
   double R2[kR2a > kR2b ? kR2a : kR2b];
 };
 
-template <int K, int M, int DIR>
+template <int K, int M, int DIR, bool BUG>
 __global__ __launch_bounds__(64) void face_kernel(FaceArgs<K, M> a)
 {
   const int e = blockIdx.x;
@@ -390,7 +393,7 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<K, M> a)
   const double *A = sA, *D = sD;
   for (int i = tid; i < N2; i += 64) sW2[i] = a.op.w[i % M] * a.op.w[i / M];
   const double *g = a.fgeo + ((size_t) e * 3 + DIR) * kFaceGeo * N2;
-  if (a.bug)
+  if constexpr (BUG)
     for (int i = tid; i < 2 * 5 * 9; i += 64) {
       const int side = i / 45, l = (i / 9) % 5, q = i % 9;
       const int pt = l < N2 ? l : 0;
@@ -454,7 +457,7 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<K, M> a)
       const double gs = DIR == 0 ? gF : DIR == 1 ? gN : gS;
       const double gt = DIR == 2 ? gN : gS;
       double c[9];
-      if (a.bug) {
+      if constexpr (BUG) {
 #pragma unroll
         for (int q = 0; q < 9; ++q) c[q] = sm.Cm[side][v][q];
       } else {
@@ -503,7 +506,7 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<K, M> a)
       const int m = sp * M + q;
       const double fx = sFC[v * N2 + m], fy = sFC[(5 + v) * N2 + m], fz = sFC[(10 + v) * N2 + m];
       double c[9];
-      if (a.bug) {
+      if constexpr (BUG) {
 #pragma unroll
         for (int t = 0; t < 9; ++t) c[t] = sm.Cm[side][v][t];
       } else {
@@ -1048,7 +1051,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
     double rz = 0.0;
 #pragma unroll
     for (int l = 0; l < 5; ++l) { pv[l] = z[l]; rz += r[l] * z[l]; }
-    { double red[1] = {rz}; block_reduce1<NT, 1, 1>(red, sRed, half); rz = red[0]; }
+    { double red[1] = {rz}; block_reduce1<NT, 1, 1, (ND + 63) / 64>(red, sRed, half); rz = red[0]; }
     for (int it = 0;; ++it) {
       if (it == 100) { atomicOr(a.flags, kFlagCGNotConverged); break; }
       if (own)
@@ -1159,7 +1162,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
           pAp += sU[idx] * s;
         }
       }
-      { double red[1] = {pAp}; block_reduce1<NT, 1, 1>(red, sRed, half); pAp = red[0]; }
+      { double red[1] = {pAp}; block_reduce1<NT, 1, 1, (5 * K * K + 63) / 64>(red, sRed, half); pAp = red[0]; }
       double ap[5];
 #pragma unroll
       for (int l = 0; l < 5; ++l) ap[l] = own ? cz[l * ND + tid] : 0.0;
@@ -1177,7 +1180,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
       for (int l = 0; l < 5; ++l) { zAp += z[l] * ap[l]; rzn += r[l] * z[l]; }
       {
         double red[3] = {zAp, rzn, rmax};
-        block_reduce1<NT, 3, 2>(red, sRed, half);
+        block_reduce1<NT, 3, 2, (ND + 63) / 64>(red, sRed, half);
         zAp = red[0]; rzn = red[1]; rmax = red[2];
       }
       iters = it + 1;
@@ -1757,9 +1760,15 @@ hipError_t faces_impl(const DeviceCase &dc, hipStream_t s)
     a.op.F[2][q] = dc.opF[2][q]; a.op.F[3][q] = dc.opF[3][q];
   }
   for (int q = 0; q < M; ++q) a.op.w[q] = dc.opW[q];
-  hipLaunchKernelGGL((face_kernel<K, M, 0>), dim3(dc.nElem), dim3(64), 0, s, a);
-  hipLaunchKernelGGL((face_kernel<K, M, 1>), dim3(dc.nElem), dim3(64), 0, s, a);
-  hipLaunchKernelGGL((face_kernel<K, M, 2>), dim3(dc.nElem), dim3(64), 0, s, a);
+  if (dc.bug_compat) {
+    hipLaunchKernelGGL((face_kernel<K, M, 0, true>), dim3(dc.nElem), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((face_kernel<K, M, 1, true>), dim3(dc.nElem), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((face_kernel<K, M, 2, true>), dim3(dc.nElem), dim3(64), 0, s, a);
+  } else {
+    hipLaunchKernelGGL((face_kernel<K, M, 0, false>), dim3(dc.nElem), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((face_kernel<K, M, 1, false>), dim3(dc.nElem), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((face_kernel<K, M, 2, false>), dim3(dc.nElem), dim3(64), 0, s, a);
+  }
   return hipGetLastError();
 }
 

@@ -43,6 +43,8 @@ constexpr double kLamOverMu = -2.0 / 3.0;
 constexpr double kInvGm1 = 1.0 / kGm1;
 
 // ------------------------------------------------------------ utilities
+constexpr int cmax(int x, int y) { return x > y ? x : y; }
+
 __device__ inline double wave_sum(double v)
 {
 #pragma unroll
@@ -345,32 +347,35 @@ struct FaceArgs {
 // ============================================================ face kernel
 // One wave64 per element min face in direction DIR. Side 0 is the element
 // itself (its min face), side 1 the neighbour L (its max face); the normal
-// points from L to the element. LDS regions are reused across the phases:
-//   R1: normal contraction B -> traces/gradients at the points -> lift stage 1
-//   R2: fast contraction T -> flux and Cartesian symmetrising fluxes at the
-//       points
-// The symmetrising fluxes are mapped to each side's parametric slots inside
-// the first lift stage.
+// points from L to the element. Every phase maps its work items onto the 64
+// lanes in a single pass, so a phase may overwrite its own input in place
+// (all lanes load before any lane stores). LDS regions:
+//   X: normal contraction B -> fast contraction T (in place) -> flux and
+//      Cartesian symmetrising fluxes at the points (in place)
+//   Y: the first lift stage of one side, in two halves
 template <int K, int M>
 struct FaceSmem {
-  static constexpr int kR1a = 5 * 5 * M * M, kR1b = 2 * 3 * 5 * M * K;
-  static constexpr int kR2a = 4 * 5 * M * M, kR2b = 2 * 3 * 5 * K * M;
+  static constexpr int kX = cmax(cmax(2 * 2 * 5 * K * K, 2 * 3 * 5 * K * M), 20 * M * M);
+  static constexpr int kY = 2 * 5 * M * K;
+  static constexpr int kBytes = 8 * (2 * M * K + M * M + 4 * K + 90 + kX + kY);
+  // waves per SIMD to compile for: three when 12 faces fit a CU's 160 KB
+  static constexpr int kWaves = 12 * kBytes <= 160 * 1024 ? 3 : 2;
   double A[M * K], D[M * K], W2[M * M];
   double F[4][K];                  // phi(-1), phi(+1), phi'(-1), phi'(+1)
   double Cm[2][5][9];
-  double R1[kR1a > kR1b ? kR1a : kR1b];
-  double R2[kR2a > kR2b ? kR2a : kR2b];
+  double X[kX];
+  double Y[kY];
 };
 
 template <int K, int M, int DIR, bool BUG>
-__global__ __launch_bounds__(64) void face_kernel(FaceArgs<K, M> a)
+__global__ __launch_bounds__(64, (FaceSmem<K, M>::kWaves)) void face_kernel(FaceArgs<K, M> a)
 {
   const int e = blockIdx.x;
   const int L = a.nbr[e * 6 + 2 * DIR];
   if (L < 0) return;  // physical boundary face
   __shared__ FaceSmem<K, M> sm;
   constexpr int ND = K * K * K, N2 = M * M;
-  static_assert(2 * 2 * 5 * K * K <= sizeof(sm.R1) / 8, "R1 holds the normal contraction");
+  static_assert(2 * 5 * K <= 64 && N2 <= 64 && 5 * M <= 64, "single-pass phases");
   // strides of the (normal, fast, slow) axes in the [k][j][i] DOF layout
   constexpr int SN = DIR == 0 ? 1 : DIR == 1 ? K : K * K;
   constexpr int SFA = DIR == 0 ? K : 1;
@@ -378,13 +383,9 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<K, M> a)
   // parametric slot of each axis: r = 0, s = 1, t = 2 -> (normal, fast, slow)
   constexpr int PN = DIR, PF = DIR == 0 ? 1 : 0, PS = DIR == 2 ? 1 : 2;
   double *sA = sm.A, *sD = sm.D, *sW2 = sm.W2;
-  // R1 views
-  double *B = sm.R1;                          // [side][value|deriv][5][K][K]
-  double *sV = sm.R1, *sG = sm.R1 + 2 * 5 * N2;  // [side][5][N2], [x|y|z][5][N2]
-  double *P1 = sm.R1;                         // lift stage 1: [side][3][5][M][K]
-  // R2 views
-  double *T = sm.R2;                          // [side][A|D|N][5][K][M]
-  double *sFlux = sm.R2, *sFC = sm.R2 + 5 * N2;  // flux [5][N2], Cartesian sym [x|y|z][5][N2]
+  double *B = sm.X;                           // [side][value|deriv][5][K][K]
+  double *T = sm.X;                           // [side][A|D|N][5][K][M]
+  double *sFlux = sm.X, *sFC = sm.X + 5 * N2;  // flux [5][N2], Cartesian sym [x|y|z][5][N2]
   const int tid = threadIdx.x;
   // operators in LDS: read as wave-uniform LDS operands in the contractions
   // (as kernel arguments they would not fit the scalar registers)
@@ -416,9 +417,9 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<K, M> a)
     }
   }
   __syncthreads();
-  // fast tangential contraction: T[side][A|D|N][v][s][f'], f' < M
-  for (int pc = tid; pc < 2 * 5 * K; pc += 64) {
-    const int side = pc / (5 * K), vs = pc - side * 5 * K;
+  // fast tangential contraction, in place over B: T[side][A|D|N][v][s][f'], f' < M
+  if (tid < 2 * 5 * K) {
+    const int side = tid / (5 * K), vs = tid - side * 5 * K;
     double x0[K], x1[K];
 #pragma unroll
     for (int f = 0; f < K; ++f) {
@@ -435,58 +436,57 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<K, M> a)
     }
   }
   __syncthreads();
-  // traces and averaged Cartesian gradients of the entropy variables at the
-  // points, one (point, variable) pair per thread iteration
-  for (int idx = tid; idx < 5 * N2; idx += 64) {
-    const int v = idx / N2, m = idx - v * N2;
-    const int sp = m / M, fp = m - sp * M;
-    double gxa = 0.0, gya = 0.0, gza = 0.0;
-#pragma unroll
-    for (int side = 0; side < 2; ++side) {
-      const double *tA = T + (side * 3 + 0) * 5 * K * M + v * K * M + fp;
-      const double *tD = T + (side * 3 + 1) * 5 * K * M + v * K * M + fp;
-      const double *tN = T + (side * 3 + 2) * 5 * K * M + v * K * M + fp;
-      double val = 0.0, gN = 0.0, gF = 0.0, gS = 0.0;
-#pragma unroll
-      for (int sl = 0; sl < K; ++sl) {
-        const double as = sA[sp * K + sl], xa = tA[sl * M];
-        val += as * xa; gS += sD[sp * K + sl] * xa; gN += as * tN[sl * M]; gF += as * tD[sl * M];
-      }
-      sV[(side * 5 + v) * N2 + m] = val;
-      const double gr = DIR == 0 ? gN : gF;
-      const double gs = DIR == 0 ? gF : DIR == 1 ? gN : gS;
-      const double gt = DIR == 2 ? gN : gS;
-      double c[9];
-      if constexpr (BUG) {
-#pragma unroll
-        for (int q = 0; q < 9; ++q) c[q] = sm.Cm[side][v][q];
-      } else {
-#pragma unroll
-        for (int q = 0; q < 9; ++q) c[q] = g[(side ? 4 + q : 13 + q) * N2 + m];
-      }
-      gxa += 0.5 * (gr * c[0] + gs * c[3] + gt * c[6]);
-      gya += 0.5 * (gr * c[1] + gs * c[4] + gt * c[7]);
-      gza += 0.5 * (gr * c[2] + gs * c[5] + gt * c[8]);
-    }
-    sG[(0 * 5 + v) * N2 + m] = gxa;
-    sG[(1 * 5 + v) * N2 + m] = gya;
-    sG[(2 * 5 + v) * N2 + m] = gza;
-  }
-  __syncthreads();
-  // fluxes at the points (Ismail-Roe, viscous, penalty, symmetrising)
+  // per point: traces, averaged Cartesian gradients of the entropy variables
+  // and the fluxes (Ismail-Roe, viscous, penalty, symmetrising); the outputs
+  // overwrite T in place
   const double lenScale = fmin(a.econst[(size_t) e * kElemConst + DIR], a.econst[(size_t) L * kElemConst + DIR]);
   const double tau = a.tauFact / lenScale;
-  for (int m = tid; m < N2; m += 64) {
-    double VL[5], VR[5], gx[5], gy[5], gz[5];
+  if (tid < N2) {
+    const int m = tid;
+    const int sp = m / M, fp = m - sp * M;
+    double ai[K], di[K];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) {
-      VL[v] = sV[(5 + v) * N2 + m]; VR[v] = sV[v * N2 + m];
-      gx[v] = sG[v * N2 + m]; gy[v] = sG[(5 + v) * N2 + m]; gz[v] = sG[(10 + v) * N2 + m];
+    for (int sl = 0; sl < K; ++sl) { ai[sl] = sA[sp * K + sl]; di[sl] = sD[sp * K + sl]; }
+    double Vs[2][5], gx[5], gy[5], gz[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) { gx[v] = 0.0; gy[v] = 0.0; gz[v] = 0.0; }
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      double cm[9];
+      if constexpr (!BUG) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) cm[q] = g[(side ? 4 + q : 13 + q) * N2 + m];
+      }
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        const double *tA = T + (side * 3 + 0) * 5 * K * M + v * K * M + fp;
+        const double *tD = T + (side * 3 + 1) * 5 * K * M + v * K * M + fp;
+        const double *tN = T + (side * 3 + 2) * 5 * K * M + v * K * M + fp;
+        double val = 0.0, gN = 0.0, gF = 0.0, gS = 0.0;
+#pragma unroll
+        for (int sl = 0; sl < K; ++sl) {
+          const double xa = tA[sl * M];
+          val += ai[sl] * xa; gS += di[sl] * xa; gN += ai[sl] * tN[sl * M]; gF += ai[sl] * tD[sl * M];
+        }
+        Vs[side][v] = val;
+        const double gr = DIR == 0 ? gN : gF;
+        const double gs = DIR == 0 ? gF : DIR == 1 ? gN : gS;
+        const double gt = DIR == 2 ? gN : gS;
+        const double *c = cm;
+        if constexpr (BUG) c = sm.Cm[side][v];
+        gx[v] += 0.5 * (gr * c[0] + gs * c[3] + gt * c[6]);
+        gy[v] += 0.5 * (gr * c[1] + gs * c[4] + gt * c[7]);
+        gz[v] += 0.5 * (gr * c[2] + gs * c[5] + gt * c[8]);
+      }
     }
+    // materialise the traces and gradients here: otherwise the compiler sinks
+    // their computation into the flux and keeps the LDS operands live
+#pragma unroll
+    for (int v = 0; v < 5; ++v) asm volatile("" : "+v"(Vs[0][v]), "+v"(Vs[1][v]), "+v"(gx[v]), "+v"(gy[v]), "+v"(gz[v]));
     const double nn[3] = {g[0 * N2 + m], g[1 * N2 + m], g[2 * N2 + m]};
     const double wA = sW2[m] * g[3 * N2 + m];
     double fl[5], FX[5], FY[5], FZ[5];
-    point_flux(VL, VR, gx, gy, gz, nn, wA, tau, a.mu, a.thetaHalf, fl, FX, FY, FZ);
+    point_flux(Vs[1], Vs[0], gx, gy, gz, nn, wA, tau, a.mu, a.thetaHalf, fl, FX, FY, FZ);
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
       sFlux[v * N2 + m] = fl[v];
@@ -494,66 +494,82 @@ __global__ __launch_bounds__(64) void face_kernel(FaceArgs<K, M> a)
     }
   }
   __syncthreads();
-  // ---- lift: fast-axis contraction P1[side][0..2][v][s'][f]; the Cartesian
-  // symmetrising fluxes are mapped to the side's parametric slots on the way
-  for (int pc = tid; pc < 2 * 5 * M; pc += 64) {
-    const int side = pc / (5 * M), r = pc - side * 5 * M;
-    const int v = r / M, sp = r - v * M;
+  // ---- lift, one side at a time: fast-axis contraction into Y (first the
+  // flux + fast-slot and the slow-slot parts, then the normal-slot part),
+  // each followed by the slow-axis contraction straight into the face slot
+  // of that side: the value and normal-derivative coefficient planes
+  // [0|1][v][slow][fast]; the volume kernel applies the face vectors
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {
     const double sign = side ? 1.0 : -1.0;
-    double xf[M], xF[M], xS[M], xN[M];
-#pragma unroll
-    for (int q = 0; q < M; ++q) {
-      const int m = sp * M + q;
-      const double fx = sFC[v * N2 + m], fy = sFC[(5 + v) * N2 + m], fz = sFC[(10 + v) * N2 + m];
-      double c[9];
-      if constexpr (BUG) {
-#pragma unroll
-        for (int t = 0; t < 9; ++t) c[t] = sm.Cm[side][v][t];
-      } else {
-#pragma unroll
-        for (int t = 0; t < 9; ++t) c[t] = g[(side ? 4 + t : 13 + t) * N2 + m];
-      }
-      double par[3];
-      par[0] = fx * c[0] + fy * c[1] + fz * c[2];
-      par[1] = fx * c[3] + fy * c[4] + fz * c[5];
-      par[2] = fx * c[6] + fy * c[7] + fz * c[8];
-      xf[q] = sign * sFlux[v * N2 + m]; xN[q] = par[PN]; xF[q] = par[PF]; xS[q] = par[PS];
-    }
-    double *Ps = P1 + side * 3 * 5 * M * K;
-#pragma unroll
-    for (int n = 0; n < K; ++n) {
-      double p1 = 0.0, p2 = 0.0, p3 = 0.0;
-#pragma unroll
-      for (int q = 0; q < M; ++q) {
-        p1 += A[q * K + n] * xf[q] + D[q * K + n] * xF[q];
-        p2 += A[q * K + n] * xS[q];
-        p3 += A[q * K + n] * xN[q];
-      }
-      Ps[r * K + n] = p1; Ps[5 * M * K + r * K + n] = p2; Ps[2 * 5 * M * K + r * K + n] = p3;
-    }
-  }
-  __syncthreads();
-  // slow-axis contraction straight into the face slots of both elements: the
-  // value and normal-derivative coefficient planes [0|1][v][s][f]; the
-  // volume kernel applies the face vectors along the normal
-  for (int pc = tid; pc < 2 * 5 * K; pc += 64) {
-    const int side = pc / (5 * K), r = pc - side * 5 * K;
-    const int v = r / K, f = r - v * K;
-    const double *Ps = P1 + side * 3 * 5 * M * K;
     double *out = a.faceRes + ((size_t) (side ? L : e) * 6 + 2 * DIR + side) * 10 * K * K;
-    double x1[M], x2[M], x3[M];
+    double gv[K];   // value plane of this lane's (v, f) column, over both parts
 #pragma unroll
-    for (int q = 0; q < M; ++q) {
-      const int j = (v * M + q) * K + f;
-      x1[q] = Ps[j]; x2[q] = Ps[5 * M * K + j]; x3[q] = Ps[2 * 5 * M * K + j];
-    }
+    for (int part = 0; part < 2; ++part) {
+      // Cartesian symmetrising flux mapped to this side's parametric slot(s)
+      if (tid < 5 * M) {
+        const int v = tid / M, sp = tid - v * M;
+        double x0[M], x1[M];
 #pragma unroll
-    for (int n = 0; n < K; ++n) {
-      double g1 = 0.0, g2 = 0.0;
+        for (int q = 0; q < M; ++q) {
+          const int m = sp * M + q;
+          const double fx = sFC[v * N2 + m], fy = sFC[(5 + v) * N2 + m], fz = sFC[(10 + v) * N2 + m];
+          double c[9];
+          if constexpr (BUG) {
 #pragma unroll
-      for (int q = 0; q < M; ++q) { g1 += A[q * K + n] * x1[q] + D[q * K + n] * x2[q]; g2 += A[q * K + n] * x3[q]; }
-      out[(v * K + n) * K + f] = g1;
-      out[5 * K * K + (v * K + n) * K + f] = g2;
+            for (int t = 0; t < 9; ++t) c[t] = sm.Cm[side][v][t];
+          } else {
+#pragma unroll
+            for (int t = 0; t < 9; ++t) c[t] = g[(side ? 4 + t : 13 + t) * N2 + m];
+          }
+          if (part == 0) {
+            x0[q] = sign * sFlux[v * N2 + m];                                   // lifted with A
+            x1[q] = fx * c[3 * PF] + fy * c[3 * PF + 1] + fz * c[3 * PF + 2];   // lifted with D
+          } else {
+            x0[q] = fx * c[3 * PS] + fy * c[3 * PS + 1] + fz * c[3 * PS + 2];   // slow slot
+            x1[q] = fx * c[3 * PN] + fy * c[3 * PN + 1] + fz * c[3 * PN + 2];   // normal slot
+          }
+        }
+        double *Ys = sm.Y;
+#pragma unroll
+        for (int n = 0; n < K; ++n) {
+          double p0 = 0.0, p1 = 0.0;
+#pragma unroll
+          for (int q = 0; q < M; ++q) {
+            if (part == 0) p0 += A[q * K + n] * x0[q] + D[q * K + n] * x1[q];
+            else { p0 += A[q * K + n] * x0[q]; p1 += A[q * K + n] * x1[q]; }
+          }
+          Ys[tid * K + n] = p0;
+          if (part == 1) Ys[5 * M * K + tid * K + n] = p1;
+        }
+      }
+      __syncthreads();
+      // slow axis: part 0 gives the value plane with A (flux + fast slot)
+      // and, after part 1, D (slow slot); the normal slot gives the
+      // derivative plane with A
+      if (tid < 5 * K) {
+        const int v = tid / K, f = tid - v * K;
+        double x0[M], x1[M];
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+          const int j = (v * M + q) * K + f;
+          x0[q] = sm.Y[j];
+          if (part == 1) x1[q] = sm.Y[5 * M * K + j];
+        }
+#pragma unroll
+        for (int n = 0; n < K; ++n) {
+          double g0 = 0.0, g1 = 0.0;
+#pragma unroll
+          for (int q = 0; q < M; ++q) {
+            if (part == 0) g0 += A[q * K + n] * x0[q];
+            else { g0 += D[q * K + n] * x0[q]; g1 += A[q * K + n] * x1[q]; }
+          }
+          const int o = (v * K + n) * K + f;
+          if (part == 0) gv[n] = g0;
+          else { out[o] = gv[n] + g0; out[5 * K * K + o] = g1; }
+        }
+      }
+      __syncthreads();
     }
   }
 }
@@ -643,8 +659,6 @@ __device__ __forceinline__ void cml_source(const double V[5], const double gx[5]
 // sum-factorisation intermediates and the primitive state at the points;
 // the 1D operators come from the kernel arguments (scalar registers) in the
 // pencil stages and from LDS rows in the per-point stages.
-constexpr int cmax(int x, int y) { return x > y ? x : y; }
-
 // LDS layout of the volume kernel and the waves per SIMD it is compiled for
 // (NT = 256: three workgroups per CU when the LDS fits three times into 160 KB,
 // which caps the kernel at 168 VGPRs).

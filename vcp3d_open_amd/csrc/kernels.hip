@@ -132,14 +132,24 @@ __device__ inline double dpp_f64(double v)
   const int rhi = __builtin_amdgcn_update_dpp(0, hi, CTRL, ROW_MASK, 0xf, false);
   return __hiloint2double(rhi, rlo);
 }
+// Row shift with every row enabled: lanes shifted in from outside the row
+// read 0 (bound_ctrl), so no zeroed destination is needed.
+template <int CTRL>
+__device__ inline double dpp_shr_f64(double v)
+{
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  const int rlo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xf, 0xf, true);
+  const int rhi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xf, 0xf, true);
+  return __hiloint2double(rhi, rlo);
+}
 template <bool MAX>
 __device__ inline double wave_reduce_dpp(double v)
 {
   auto op = [](double x, double y) { return MAX ? fmax(x, y) : x + y; };
-  v = op(v, dpp_f64<0x111, 0xf>(v));   // row_shr:1
-  v = op(v, dpp_f64<0x112, 0xf>(v));   // row_shr:2
-  v = op(v, dpp_f64<0x114, 0xf>(v));   // row_shr:4
-  v = op(v, dpp_f64<0x118, 0xf>(v));   // row_shr:8   -> lane 15 of each row: row total
+  v = op(v, dpp_shr_f64<0x111>(v));    // row_shr:1
+  v = op(v, dpp_shr_f64<0x112>(v));    // row_shr:2
+  v = op(v, dpp_shr_f64<0x114>(v));    // row_shr:4
+  v = op(v, dpp_shr_f64<0x118>(v));    // row_shr:8   -> lane 15 of each row: row total
   v = op(v, dpp_f64<0x142, 0xa>(v));   // row_bcast:15 into rows 1, 3
   v = op(v, dpp_f64<0x143, 0xc>(v));   // row_bcast:31 into rows 2, 3 -> lane 63: total
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);

@@ -37,7 +37,7 @@ def volume_kernel_flops(K: int, M: int, cg_iters: float) -> float:
     integration point for the pointwise flux work)."""
     fwd = 5 * (2 * K**3 * M + 3 * K**2 * M**2 + 4 * K * M**3)
     bwd = 5 * (3 * K * M**3 + 3 * K**2 * M**2 + 2 * K**3 * M)
-    gather = 30 * K**3
+    gather = 60 * K**3      # six faces x (value, derivative plane) x 5 variables
     cg = 5 * 2 * (K**3 * M + K**2 * M**2 + K * M**3) + 30 * M**3 + 25 * K**3 + 30 * K**3
     return 2.0 * (fwd + bwd + gather + cg_iters * cg) + 350.0 * M**3
 
@@ -45,8 +45,8 @@ def volume_kernel_flops(K: int, M: int, cg_iters: float) -> float:
 def volume_kernel_bytes(K: int, M: int) -> float:
     """Algorithmic HBM bytes per element of a residual launch: U, geometry
     (J + 9 metric terms per point), 16 element constants and the six face
-    slots read; R written."""
-    return 8.0 * (5 * K**3 + 30 * K**3 + 10 * M**3 + 16 + 5 * K**3)
+    slots (value and normal-derivative planes, 10 K^2 each) read; R written."""
+    return 8.0 * (5 * K**3 + 60 * K**2 + 10 * M**3 + 16 + 5 * K**3)
 
 
 def measured_traffic(kernel: str):
@@ -62,15 +62,16 @@ def measured_traffic(kernel: str):
 
 
 def face_kernel_flops(K: int, M: int) -> float:
-    """Algorithmic FLOPs per element-face (both traces, flux, both lifts);
-    700 flops per face point for the Ismail-Roe + viscous + penalty work."""
+    """Algorithmic FLOPs per element-face (both traces, flux, both lifts to
+    the tangential coefficient planes); 700 flops per face point for the
+    Ismail-Roe + viscous + penalty work."""
     trace = 2 * 5 * (2 * K**3 + 3 * K**2 * M + 4 * K * M**2)
-    lift = 2 * 5 * (4 * K * M**2 + 3 * K**2 * M + 2 * K**3)
+    lift = 2 * 5 * (4 * K * M**2 + 3 * K**2 * M)
     return 2.0 * (trace + lift) + 700.0 * M**2
 
 
 def face_kernel_bytes(K: int, M: int) -> float:
-    return 8.0 * (2 * 5 * K**3 + 22 * M**2 + 2 * 5 * K**3 + 2)
+    return 8.0 * (2 * 5 * K**3 + 22 * M**2 + 2 * 10 * K**2 + 2)
 
 
 def calib_copy(nbytes: int) -> int:

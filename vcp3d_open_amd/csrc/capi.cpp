@@ -39,6 +39,9 @@ struct vcp_ctx {
   vcp::DeviceCase dc;
   vcp::Sponge sponge;
   hipStream_t stream = nullptr;
+  // pinned host words the per-step scalars come back through:
+  // [0] time step, [1] maximum Mach^2, [2] error flags (as int)
+  double *host = nullptr;
   std::vector<void *> allocs;
   bool res_valid = false;
   bool damping_set = false;
@@ -55,6 +58,7 @@ struct vcp_ctx {
   ~vcp_ctx()
   {
     for (void *p : allocs) (void) hipFree(p);
+    if (host) (void) hipHostFree(host);
     if (stream) (void) hipStreamDestroy(stream);
   }
 };
@@ -63,15 +67,20 @@ namespace {
 
 size_t state_size(const vcp_ctx *c) { return (size_t) c->dc.nElem * 5 * c->dc.nDOF; }
 
+int flag_error(int flags)
+{
+  if (flags & vcp::kFlagNegativeA2) return fail("Negative square speed of sound", -4);
+  if (flags & vcp::kFlagCGNotConverged) return fail("Convergence not reached for CG algorithm", -5);
+  if (flags & vcp::kFlagNonFinite) return fail("non-finite residual", -6);
+  return 0;
+}
+
 int check_flags(vcp_ctx *c)
 {
   int flags = 0;
   HIP_TRY(hipMemcpyAsync(&flags, c->dc.flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  if (flags & vcp::kFlagNegativeA2) return fail("Negative square speed of sound", -4);
-  if (flags & vcp::kFlagCGNotConverged) return fail("Convergence not reached for CG algorithm", -5);
-  if (flags & vcp::kFlagNonFinite) return fail("non-finite residual", -6);
-  return 0;
+  return flag_error(flags);
 }
 
 
@@ -218,6 +227,7 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
     }
   }
   hipError_t he = hipStreamCreate(&c->stream);
+  if (he == hipSuccess) he = hipHostMalloc((void **) &c->host, 4 * sizeof(double), hipHostMallocDefault);
   const size_t ns = (size_t) nElem * 5 * dc.nDOF;
   if (he == hipSuccess) he = c->alloc(&dc.tables, tb.size());
   if (he == hipSuccess) he = c->alloc(&dc.vgeo, vgeo.size());
@@ -441,10 +451,15 @@ int vcp_sync_step(vcp_ctx *c, double md[8])
   int nSteps = 0;
   while (!reached) {
     ++c->time_step; ++nSteps;
-    if ((rc = vcp_time_steps(c, nullptr))) return rc;
-    double dt = 0.0;
-    HIP_TRY(hipMemcpyAsync(&dt, c->dc.scal, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    // time step and error flags back in one round trip
+    HIP_TRY(vcp::launch_time_step(c->dc, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->host, c->dc.scal, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->host + 2, c->dc.flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    int flags = 0;
+    std::memcpy(&flags, c->host + 2, sizeof(int));
+    if ((rc = flag_error(flags))) return rc;
+    double dt = c->host[0];
     if (elapsed + dt >= 0.99 * dtSync) { dt = dtSync - elapsed; reached = true; }
     elapsed += dt;
     HIP_TRY(vcp::launch_stage0(c->dc, dt, c->stream));

@@ -58,69 +58,6 @@ __device__ inline double wave_max(double v)
   return v;
 }
 
-// Block reduction of NV values (sum for the first NS, max for the rest).
-template <int NT, int NV, int NS>
-__device__ inline void block_reduce(double (&v)[NV], double *red)
-{
-#pragma unroll
-  for (int q = 0; q < NV; ++q) v[q] = (q < NS) ? wave_sum(v[q]) : wave_max(v[q]);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0)
-#pragma unroll
-    for (int q = 0; q < NV; ++q) red[wid * NV + q] = v[q];
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < NV; ++q) {
-    double s = red[q];
-#pragma unroll
-    for (int w = 1; w < NT / 64; ++w) s = (q < NS) ? s + red[w * NV + q] : fmax(s, red[w * NV + q]);
-    v[q] = s;
-  }
-}
-
-// out[o][n][i] = sum_q op[n*NIN+q] * in[o][q][i]   (i < INNER)
-// in row stride IR, out row stride OR (in units of doubles).
-template <int NIN, int NOUT, int OUTER, int INNER, int IR, int OR, int NT>
-__device__ inline void contract(const double *__restrict__ in, double *__restrict__ out,
-                                const double *__restrict__ op)
-{
-  constexpr int TOT = OUTER * NOUT * INNER;
-  for (int idx = threadIdx.x; idx < TOT; idx += NT) {
-    const int o = idx / (NOUT * INNER);
-    const int r = idx - o * (NOUT * INNER);
-    const int n = r / INNER;
-    const int i = r - n * INNER;
-    const double *src = in + o * NIN * IR + i;
-    const double *w = op + n * NIN;
-    double s = 0.0;
-#pragma unroll
-    for (int q = 0; q < NIN; ++q) s += w[q] * src[q * IR];
-    out[(o * NOUT + n) * OR + i] = s;
-  }
-}
-
-// Same with two inputs and two operators summed.
-template <int NIN, int NOUT, int OUTER, int INNER, int IR, int OR, int NT>
-__device__ inline void contract2(const double *__restrict__ in1, const double *__restrict__ op1,
-                                 const double *__restrict__ in2, const double *__restrict__ op2,
-                                 double *__restrict__ out)
-{
-  constexpr int TOT = OUTER * NOUT * INNER;
-  for (int idx = threadIdx.x; idx < TOT; idx += NT) {
-    const int o = idx / (NOUT * INNER);
-    const int r = idx - o * (NOUT * INNER);
-    const int n = r / INNER;
-    const int i = r - n * INNER;
-    const double *s1 = in1 + o * NIN * IR + i, *s2 = in2 + o * NIN * IR + i;
-    const double *w1 = op1 + n * NIN, *w2 = op2 + n * NIN;
-    double s = 0.0;
-#pragma unroll
-    for (int q = 0; q < NIN; ++q) s += w1[q] * s1[q * IR] + w2[q] * s2[q * IR];
-    out[(o * NOUT + n) * OR + i] = s;
-  }
-}
-
 // Wave64 reduction through DPP row shifts and row broadcasts (no LDS):
 // sum, or max of non-negative values; lanes without a source contribute 0.
 // The result (lane 63) is returned to every lane.
@@ -183,24 +120,6 @@ __device__ inline void block_reduce1(double (&v)[NV], double *red, int &half)
     v[q] = s;
   }
   half ^= 1;
-}
-
-// In-place contraction of NROWS contiguous rows of length NIN -> NOUT.
-template <int NIN, int NOUT, int NROWS, int NT>
-__device__ inline void rows_inplace(double *buf, const double *__restrict__ op)
-{
-  for (int r = threadIdx.x; r < NROWS; r += NT) {
-    double x[NIN];
-#pragma unroll
-    for (int i = 0; i < NIN; ++i) x[i] = buf[r * NIN + i];
-#pragma unroll
-    for (int q = 0; q < NOUT; ++q) {
-      double s = 0.0;
-#pragma unroll
-      for (int i = 0; i < NIN; ++i) s += op[q * NIN + i] * x[i];
-      buf[r * NIN + q] = s;
-    }
-  }
 }
 
 // t^(-1/(gamma-1)) = t^(-5/2) for gamma = 1.4, as 1 / (t^2 sqrt(t)).
@@ -546,7 +465,7 @@ __global__ __launch_bounds__(64, (FaceSmem<K, M>::kWaves)) void face_kernel(Face
           double p0 = 0.0, p1 = 0.0;
 #pragma unroll
           for (int q = 0; q < M; ++q) {
-            if (part == 0) p0 += A[q * K + n] * x0[q] + D[q * K + n] * x1[q];
+            if (part == 0) { p0 += A[q * K + n] * x0[q]; p0 += D[q * K + n] * x1[q]; }
             else { p0 += A[q * K + n] * x0[q]; p1 += A[q * K + n] * x1[q]; }
           }
           Ys[tid * K + n] = p0;
@@ -808,7 +727,9 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
       double ai[K], di[K];
 #pragma unroll
       for (int ii = 0; ii < K; ++ii) { ai[ii] = sA[i * K + ii]; di[ii] = sD[i * K + ii]; }
-      const double *row = yB + jk * 15 * K;
+      int roff = jk * 15 * K;
+      asm volatile("" : "+v"(roff));   // one base register, immediate LDS offsets
+      const double *row = yB + roff;
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
         const double *ra = row + v * K, *rd = row + (5 + v) * K, *rz = row + (10 + v) * K;
@@ -1187,21 +1108,23 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
         }
       }
       { double red[1] = {pAp}; block_reduce1<NT, 1, 1, (5 * K * K + 63) / 64>(red, sRed, half); pAp = red[0]; }
-      double ap[5];
+      // vector updates only in the waves that own DOFs
+      double zAp = 0.0, rzn = 0.0, rmax = 0.0, alpha = 0.0;
+      if (own) {
+        double ap[5];
 #pragma unroll
-      for (int l = 0; l < 5; ++l) ap[l] = own ? cz[l * ND + tid] : 0.0;
-      const double alpha = rz / pAp;
-      double rmax = 0.0;
+        for (int l = 0; l < 5; ++l) ap[l] = cz[l * ND + tid];
+        alpha = rz / pAp;
 #pragma unroll
-      for (int l = 0; l < 5; ++l) {
-        x[l] += alpha * pv[l];
-        r[l] -= alpha * ap[l];
-        rmax = fmax(rmax, fabs(r[l]));
+        for (int l = 0; l < 5; ++l) {
+          x[l] += alpha * pv[l];
+          r[l] -= alpha * ap[l];
+          rmax = fmax(rmax, fabs(r[l]));
+        }
+        VCP_PRECOND();
+#pragma unroll
+        for (int l = 0; l < 5; ++l) { zAp += z[l] * ap[l]; rzn += r[l] * z[l]; }
       }
-      VCP_PRECOND();
-      double zAp = 0.0, rzn = 0.0;
-#pragma unroll
-      for (int l = 0; l < 5; ++l) { zAp += z[l] * ap[l]; rzn += r[l] * z[l]; }
       {
         double red[3] = {zAp, rzn, rmax};
         block_reduce1<NT, 3, 2, (ND + 63) / 64>(red, sRed, half);
@@ -1209,9 +1132,11 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
       }
       iters = it + 1;
       if (rmax < 1e-10 * r0) break;
-      const double beta = -alpha * zAp / rz;
+      if (own) {
+        const double beta = -alpha * zAp / rz;
 #pragma unroll
-      for (int l = 0; l < 5; ++l) pv[l] = beta * pv[l] + z[l];
+        for (int l = 0; l < 5; ++l) pv[l] = beta * pv[l] + z[l];
+      }
       rz = rzn;
     }
 #undef VCP_PRECOND
@@ -1644,7 +1569,7 @@ __global__ __launch_bounds__(64) void nscbc_kernel(BndArgs<K, M> a)
       double p1 = 0.0, p2 = 0.0, p3 = 0.0;
 #pragma unroll
       for (int q = 0; q < M; ++q) {
-        p1 += sm.A[q * K + n] * xf[q] + sm.D[q * K + n] * xF[q];
+        p1 += sm.A[q * K + n] * xf[q]; p1 += sm.D[q * K + n] * xF[q];
         p2 += sm.A[q * K + n] * xS[q];
         p3 += sm.A[q * K + n] * xN[q];
       }
@@ -1682,22 +1607,59 @@ struct DtArgs {
   double cfl, u_ref, p_ref, rho_ref, muDim;
 };
 
+// In-place contraction of the 5 K^2 lines of sU along the axis of stride ST
+// with the K x K matrix L, by one wave: every lane reads all its lines
+// before any lane writes.
+template <int K, int ST>
+__device__ __forceinline__ void lines_inplace(double *sU, const double *L, int lane)
+{
+  constexpr int NL = 5 * K * K, PP = (NL + 63) / 64;
+  double o[PP][K];
+#pragma unroll
+  for (int t = 0; t < PP; ++t) {
+    const int p = lane + 64 * t;
+    if (p < NL) {
+      const int b = ST == 1 ? p * K : ST == K ? (p / K) * K * K + p % K : (p / (K * K)) * K * K * K + p % (K * K);
+      double x[K];
+#pragma unroll
+      for (int q = 0; q < K; ++q) x[q] = sU[b + q * ST];
+#pragma unroll
+      for (int n = 0; n < K; ++n) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < K; ++q) s += L[n * K + q] * x[q];
+        o[t][n] = s;
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < PP; ++t) {
+    const int p = lane + 64 * t;
+    if (p < NL) {
+      const int b = ST == 1 ? p * K : ST == K ? (p / K) * K * K + p % K : (p / (K * K)) * K * K * K + p % (K * K);
+#pragma unroll
+      for (int n = 0; n < K; ++n) sU[b + n * ST] = o[t][n];
+    }
+  }
+}
+
 template <int K>
 __global__ __launch_bounds__(64) void time_step_kernel(DtArgs a)
 {
   constexpr int ND = K * K * K;
-  __shared__ double sL[K * K], sU[5 * ND], sT1[5 * ND], sT2[5 * ND];
-  __shared__ double sRed[3];
+  __shared__ double sL[K * K], sU[5 * ND];
   const int e = blockIdx.x, tid = threadIdx.x;
   for (int i = tid; i < K * K; i += 64) sL[i] = a.tables[a.to.LD + i];
   for (int i = tid; i < 5 * ND; i += 64) sU[i] = a.U[(size_t) e * 5 * ND + i];
   __syncthreads();
-  contract<K, K, 5 * K * K, 1, 1, 1, 64>(sU, sT1, sL);
+  // modal -> nodal values along i, j and k, in place
+  lines_inplace<K, 1>(sU, sL, tid);
   __syncthreads();
-  contract<K, K, 5 * K, K, K, K, 64>(sT1, sT2, sL);
+  lines_inplace<K, K>(sU, sL, tid);
   __syncthreads();
-  contract<K, K, 5, K * K, K * K, K * K, 64>(sT2, sT1, sL);
+  lines_inplace<K, K * K>(sU, sL, tid);
   __syncthreads();
+  const double *sT1 = sU;
   const double *ec = a.econst + (size_t) e * kElemConst;
   const double di = 1.0 / ec[0], dj = 1.0 / ec[1], dk = 1.0 / ec[2];
   const double dl2 = di * di + dj * dj + dk * dk;

@@ -44,6 +44,7 @@ struct vcp_ctx {
   double *host = nullptr;
   std::vector<void *> allocs;
   bool res_valid = false;
+  bool dt_ready = false;          // host[0] holds the time step of the current state
   bool damping_set = false;
   int time_step = 0;
 
@@ -381,6 +382,7 @@ int vcp_set_solution(vcp_ctx *c, const double *modal)
   HIP_TRY(hipMemcpyAsync(c->dc.U, modal, state_size(c) * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->res_valid = false;
+  c->dt_ready = false;
   return 0;
 }
 
@@ -430,6 +432,7 @@ int vcp_residual(vcp_ctx *c, int mon, double md[8])
 int vcp_time_steps(vcp_ctx *c, double *dt_elem)
 {
   if (!c) return fail("null context");
+  c->dt_ready = false;
   HIP_TRY(vcp::launch_time_step(c->dc, c->stream));
   if (dt_elem)
     HIP_TRY(hipMemcpyAsync(dt_elem, c->dc.dtElem, c->dc.nElem * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -451,15 +454,22 @@ int vcp_sync_step(vcp_ctx *c, double md[8])
   int nSteps = 0;
   while (!reached) {
     ++c->time_step; ++nSteps;
-    // time step and error flags back in one round trip
-    HIP_TRY(vcp::launch_time_step(c->dc, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->host, c->dc.scal, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->host + 2, c->dc.flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    int flags = 0;
-    std::memcpy(&flags, c->host + 2, sizeof(int));
-    if ((rc = flag_error(flags))) return rc;
-    double dt = c->host[0];
+    double dt = 0.0;
+    if (c->dt_ready) {
+      // computed, with its error flags checked, at the end of the previous call
+      dt = c->host[0];
+      c->dt_ready = false;
+    } else {
+      // time step and error flags back in one round trip
+      HIP_TRY(vcp::launch_time_step(c->dc, c->stream));
+      HIP_TRY(hipMemcpyAsync(c->host, c->dc.scal, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(hipMemcpyAsync(c->host + 2, c->dc.flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      int flags = 0;
+      std::memcpy(&flags, c->host + 2, sizeof(int));
+      if ((rc = flag_error(flags))) return rc;
+      dt = c->host[0];
+    }
     if (elapsed + dt >= 0.99 * dtSync) { dt = dtSync - elapsed; reached = true; }
     elapsed += dt;
     HIP_TRY(vcp::launch_stage0(c->dc, dt, c->stream));
@@ -470,13 +480,20 @@ int vcp_sync_step(vcp_ctx *c, double md[8])
     if ((rc = residual_pass(c, s2))) return rc;
     if ((rc = residual_pass(c, s3))) return rc;
     if (reached) {
-      double mach2 = 0.0;
-      HIP_TRY(hipMemcpyAsync(&mach2, c->dc.scal + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-      if ((rc = check_flags(c))) return rc;
-      fill_monitoring(md, true, mach2);
+      // the time step of the new state for the next call, read back with the
+      // monitoring maximum and the error flags in one round trip
+      HIP_TRY(vcp::launch_time_step(c->dc, c->stream));
+      HIP_TRY(hipMemcpyAsync(c->host, c->dc.scal, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(hipMemcpyAsync(c->host + 2, c->dc.flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      int flags = 0;
+      std::memcpy(&flags, c->host + 2, sizeof(int));
+      // a failure of the next step's time step is reported by the next call
+      if ((rc = flag_error(flags & ~vcp::kFlagNegativeA2))) return rc;
+      c->dt_ready = !(flags & vcp::kFlagNegativeA2);
+      fill_monitoring(md, true, c->host[1]);
     }
   }
-  if ((rc = check_flags(c))) return rc;
   c->res_valid = true;
   return nSteps;
 }

@@ -73,3 +73,31 @@ def test_rk_steps_match_oracle(lib, oracle_lib, name, kw):
     assert rel(g.solution(), o.modal()) < 1e-12
     assert rel(g.WriteRestartSolution(), o.nodal()) < 1e-12
     assert rel(g.residual_array(), o.residual_array()) < 1e-9
+
+
+def test_time_step_reuse_across_calls(lib, oracle_lib):
+    """Several RK steps per synchronization step: the second call starts from
+    the time step computed at the end of the first; a new state set between
+    calls must not reuse it."""
+    from vcp3d_open_amd.solver import SolverClass
+    kw = CASES[1][1]
+    probe = oracle_lib.Oracle(TGVCase(**kw))
+    nodal = TGVCase(**kw).initial_conservative()
+    probe.set_restart(nodal)
+    dt_min = probe.time_steps().min() / probe.reference()["u_ref"]
+    case = TGVCase(**kw, dt_synchr=2.5 * dt_min)
+    print("[case] curv_n4_p4 multi-step synchronization", flush=True)
+    o = oracle_lib.Oracle(case)
+    g = SolverClass(case)
+    o.set_restart(nodal)
+    g.ReadRestartSolution(nodal)
+    for _ in range(2):
+        o.sync_step()
+        g.ExecuteTimeSynchronizationStep()
+    assert rel(g.solution(), o.modal()) < 1e-12
+    o.set_restart(nodal)
+    g.ReadRestartSolution(nodal)
+    o.sync_step()
+    g.ExecuteTimeSynchronizationStep()
+    assert rel(g.solution(), o.modal()) < 1e-12
+    assert rel(g.residual_array(), o.residual_array()) < 1e-9

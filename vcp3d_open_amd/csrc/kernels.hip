@@ -356,11 +356,12 @@ __global__ __launch_bounds__(64, (FaceSmem<K, M>::kWaves)) void face_kernel(Face
       x1[f] = B[(side * 2 + 1) * 5 * K * K + vs * K + f];
     }
     double *Ts = T + side * 3 * 5 * K * M;
-#pragma unroll
+    const double *KA = a.op.A, *KD = a.op.D;   // one operator row per iteration, scalar loads
+#pragma unroll 1
     for (int n = 0; n < M; ++n) {
       double tA = 0.0, tD = 0.0, tN = 0.0;
 #pragma unroll
-      for (int f = 0; f < K; ++f) { tA += A[n * K + f] * x0[f]; tD += D[n * K + f] * x0[f]; tN += A[n * K + f] * x1[f]; }
+      for (int f = 0; f < K; ++f) { tA += KA[n * K + f] * x0[f]; tD += KD[n * K + f] * x0[f]; tN += KA[n * K + f] * x1[f]; }
       Ts[vs * M + n] = tA; Ts[5 * K * M + vs * M + n] = tD; Ts[2 * 5 * K * M + vs * M + n] = tN;
     }
   }
@@ -460,13 +461,14 @@ __global__ __launch_bounds__(64, (FaceSmem<K, M>::kWaves)) void face_kernel(Face
           }
         }
         double *Ys = sm.Y;
-#pragma unroll
+        const double *KA = a.op.A, *KD = a.op.D;
+#pragma unroll 1
         for (int n = 0; n < K; ++n) {
           double p0 = 0.0, p1 = 0.0;
 #pragma unroll
           for (int q = 0; q < M; ++q) {
-            if (part == 0) { p0 += A[q * K + n] * x0[q]; p0 += D[q * K + n] * x1[q]; }
-            else { p0 += A[q * K + n] * x0[q]; p1 += A[q * K + n] * x1[q]; }
+            if (part == 0) { p0 += KA[q * K + n] * x0[q]; p0 += KD[q * K + n] * x1[q]; }
+            else { p0 += KA[q * K + n] * x0[q]; p1 += KA[q * K + n] * x1[q]; }
           }
           Ys[tid * K + n] = p0;
           if (part == 1) Ys[5 * M * K + tid * K + n] = p1;

@@ -487,17 +487,19 @@ __global__ __launch_bounds__(64, (FaceSmem<K, M>::kWaves)) void face_kernel(Face
           x0[q] = sm.Y[j];
           if (part == 1) x1[q] = sm.Y[5 * M * K + j];
         }
+        const double *KA = a.op.A, *KD = a.op.D;
 #pragma unroll
         for (int n = 0; n < K; ++n) {
           double g0 = 0.0, g1 = 0.0;
 #pragma unroll
           for (int q = 0; q < M; ++q) {
-            if (part == 0) g0 += A[q * K + n] * x0[q];
-            else { g0 += D[q * K + n] * x0[q]; g1 += A[q * K + n] * x1[q]; }
+            if (part == 0) g0 += KA[q * K + n] * x0[q];
+            else { g0 += KD[q * K + n] * x0[q]; g1 += KA[q * K + n] * x1[q]; }
           }
           const int o = (v * K + n) * K + f;
           if (part == 0) gv[n] = g0;
           else { out[o] = gv[n] + g0; out[5 * K * K + o] = g1; }
+          asm volatile("" ::: "memory");   // one operator column's scalar loads at a time
         }
       }
       __syncthreads();

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "kernels.hpp"
@@ -472,6 +473,7 @@ int vcp_sync_step(vcp_ctx *c, double md[8])
     }
     if (elapsed + dt >= 0.99 * dtSync) { dt = dtSync - elapsed; reached = true; }
     elapsed += dt;
+    std::swap(c->dc.U, c->dc.U0);   // the step's state becomes U0, stage 0 writes U
     HIP_TRY(vcp::launch_stage0(c->dc, dt, c->stream));
     vcp::StageCtl s1{0, 1, alphaRK[1], betaRK[1] * dt, 0};
     vcp::StageCtl s2{0, 1, alphaRK[2], betaRK[2] * dt, 0};

@@ -12,7 +12,7 @@
 //                        state, polynomial-correction reconstruction, flux
 //                        and lift into the element's face slot
 //   time_step_kernel     per-element stable time step and global minimum
-//   stage0_kernel        U0 <- U, U <- U0 - dt R
+//   stage0_kernel        U <- U0 - dt R (buffers swapped by the host)
 //
 // Reference behaviour followed (VCP3D_CurviLinear/src/):
 //   ElementClass::VolumeResidual          ElementClass.cpp:3809-4443
@@ -297,12 +297,10 @@ struct FaceSmem {
 };
 
 template <int K, int M, int DIR, bool BUG>
-__global__ __launch_bounds__(64, (FaceSmem<K, M>::kWaves)) void face_kernel(FaceArgs<K, M> a)
+__device__ __forceinline__ void face_body(FaceArgs<K, M> a, FaceSmem<K, M> &sm, int e)
 {
-  const int e = blockIdx.x;
   const int L = a.nbr[e * 6 + 2 * DIR];
   if (L < 0) return;  // physical boundary face
-  __shared__ FaceSmem<K, M> sm;
   constexpr int ND = K * K * K, N2 = M * M;
   static_assert(2 * 5 * K <= 64 && N2 <= 64 && 5 * M <= 64, "single-pass phases");
   // strides of the (normal, fast, slow) axes in the [k][j][i] DOF layout
@@ -505,6 +503,17 @@ __global__ __launch_bounds__(64, (FaceSmem<K, M>::kWaves)) void face_kernel(Face
       __syncthreads();
     }
   }
+}
+
+// The three directions in one launch: blockIdx.y is the direction.
+template <int K, int M, bool BUG>
+__global__ __launch_bounds__(64, (FaceSmem<K, M>::kWaves)) void face_kernel(FaceArgs<K, M> a)
+{
+  __shared__ FaceSmem<K, M> sm;
+  const int e = blockIdx.x;
+  if (blockIdx.y == 0) face_body<K, M, 0, BUG>(a, sm, e);
+  else if (blockIdx.y == 1) face_body<K, M, 1, BUG>(a, sm, e);
+  else face_body<K, M, 2, BUG>(a, sm, e);
 }
 
 // 1D operators of the standard element as kernel arguments: indexed with
@@ -1728,13 +1737,12 @@ __global__ __launch_bounds__(1024) void reduce_max_kernel(const double *v, int n
   }
 }
 
-__global__ void stage0_kernel(double *U, double *U0, const double *R, size_t n, double dt)
+// First RK stage: U <- U0 - dt R, where U0 already holds the state of the
+// step (the host swaps the two buffers instead of copying).
+__global__ void stage0_kernel(double *U, const double *U0, const double *R, size_t n, double dt)
 {
-  for (size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x; i < n; i += (size_t) gridDim.x * blockDim.x) {
-    const double u = U[i];
-    U0[i] = u;
-    U[i] = u - dt * R[i];
-  }
+  for (size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x; i < n; i += (size_t) gridDim.x * blockDim.x)
+    U[i] = U0[i] - dt * R[i];
 }
 
 template <int K, int M>
@@ -1750,15 +1758,10 @@ hipError_t faces_impl(const DeviceCase &dc, hipStream_t s)
     a.op.F[2][q] = dc.opF[2][q]; a.op.F[3][q] = dc.opF[3][q];
   }
   for (int q = 0; q < M; ++q) a.op.w[q] = dc.opW[q];
-  if (dc.bug_compat) {
-    hipLaunchKernelGGL((face_kernel<K, M, 0, true>), dim3(dc.nElem), dim3(64), 0, s, a);
-    hipLaunchKernelGGL((face_kernel<K, M, 1, true>), dim3(dc.nElem), dim3(64), 0, s, a);
-    hipLaunchKernelGGL((face_kernel<K, M, 2, true>), dim3(dc.nElem), dim3(64), 0, s, a);
-  } else {
-    hipLaunchKernelGGL((face_kernel<K, M, 0, false>), dim3(dc.nElem), dim3(64), 0, s, a);
-    hipLaunchKernelGGL((face_kernel<K, M, 1, false>), dim3(dc.nElem), dim3(64), 0, s, a);
-    hipLaunchKernelGGL((face_kernel<K, M, 2, false>), dim3(dc.nElem), dim3(64), 0, s, a);
-  }
+  if (dc.bug_compat)
+    hipLaunchKernelGGL((face_kernel<K, M, true>), dim3(dc.nElem, 3), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL((face_kernel<K, M, false>), dim3(dc.nElem, 3), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 

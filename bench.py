@@ -168,7 +168,7 @@ def main() -> int:
         flops = volume_kernel_flops(K, M, cg_mean) * case.n_elems
         bytes_ = volume_kernel_bytes(K, M) * case.n_elems
     else:
-        name, ms = "face_kernel<5, 7, DIR> x3", ms_face
+        name, ms = "face_kernel<5, 7, BUG> (3 directions)", ms_face
         flops = face_kernel_flops(K, M) * 3 * case.n_elems
         bytes_ = face_kernel_bytes(K, M) * 3 * case.n_elems
     achieved = flops / (ms * 1e-3) / 1e12

@@ -40,6 +40,9 @@ struct vcp_ctx {
   vcp::DeviceCase dc;
   vcp::Sponge sponge;
   hipStream_t stream = nullptr;
+  // side stream for the next call's time step, which overlaps the last stage
+  hipStream_t side = nullptr;
+  hipEvent_t evState = nullptr, evDt = nullptr;
   // pinned host words the per-step scalars come back through:
   // [0] time step, [1] maximum Mach^2, [2] error flags (as int)
   double *host = nullptr;
@@ -61,6 +64,9 @@ struct vcp_ctx {
   {
     for (void *p : allocs) (void) hipFree(p);
     if (host) (void) hipHostFree(host);
+    if (evState) (void) hipEventDestroy(evState);
+    if (evDt) (void) hipEventDestroy(evDt);
+    if (side) (void) hipStreamDestroy(side);
     if (stream) (void) hipStreamDestroy(stream);
   }
 };
@@ -230,6 +236,9 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
   }
   hipError_t he = hipStreamCreate(&c->stream);
   if (he == hipSuccess) he = hipHostMalloc((void **) &c->host, 4 * sizeof(double), hipHostMallocDefault);
+  if (he == hipSuccess) he = hipStreamCreate(&c->side);
+  if (he == hipSuccess) he = hipEventCreateWithFlags(&c->evState, hipEventDisableTiming);
+  if (he == hipSuccess) he = hipEventCreateWithFlags(&c->evDt, hipEventDisableTiming);
   const size_t ns = (size_t) nElem * 5 * dc.nDOF;
   if (he == hipSuccess) he = c->alloc(&dc.tables, tb.size());
   if (he == hipSuccess) he = c->alloc(&dc.vgeo, vgeo.size());
@@ -480,11 +489,20 @@ int vcp_sync_step(vcp_ctx *c, double md[8])
     vcp::StageCtl s3{1, 0, 0.0, 0.0, reached ? 1 : 0};
     if ((rc = residual_pass(c, s1))) return rc;
     if ((rc = residual_pass(c, s2))) return rc;
+    if (reached) {
+      // U is final after the second stage (the third only evaluates the
+      // residual for the next call): the time step of the new state runs on
+      // the side stream while the third stage runs
+      HIP_TRY(hipEventRecord(c->evState, c->stream));
+      HIP_TRY(hipStreamWaitEvent(c->side, c->evState, 0));
+      HIP_TRY(vcp::launch_time_step(c->dc, c->side));
+      HIP_TRY(hipEventRecord(c->evDt, c->side));
+    }
     if ((rc = residual_pass(c, s3))) return rc;
     if (reached) {
-      // the time step of the new state for the next call, read back with the
-      // monitoring maximum and the error flags in one round trip
-      HIP_TRY(vcp::launch_time_step(c->dc, c->stream));
+      // read back with the monitoring maximum and the error flags in one
+      // round trip
+      HIP_TRY(hipStreamWaitEvent(c->stream, c->evDt, 0));
       HIP_TRY(hipMemcpyAsync(c->host, c->dc.scal, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
       HIP_TRY(hipMemcpyAsync(c->host + 2, c->dc.flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
       HIP_TRY(hipStreamSynchronize(c->stream));

@@ -268,7 +268,7 @@ struct FaceArgs {
   const double *U, *fgeo, *econst;
   const int *nbr;
   double *faceRes;
-  int nElem, bug;
+  int nElem;                     // the metric-indexing mode is the kernel's BUG parameter
   double mu, tauFact, thetaHalf;
   FaceOps<K, M> op;
 };
@@ -1750,7 +1750,7 @@ hipError_t faces_impl(const DeviceCase &dc, hipStream_t s)
 {
   FaceArgs<K, M> a;
   a.U = dc.U; a.fgeo = dc.fgeo; a.econst = dc.econst; a.nbr = dc.nbr;
-  a.faceRes = dc.faceRes; a.nElem = dc.nElem; a.bug = dc.bug_compat;
+  a.faceRes = dc.faceRes; a.nElem = dc.nElem;
   a.mu = dc.mu; a.tauFact = (dc.P + 1.0) * (dc.P + 1.0) * dc.rel_penalty; a.thetaHalf = 0.5 * dc.theta_sym;
   for (int i = 0; i < M * K; ++i) { a.op.A[i] = dc.opA[i]; a.op.D[i] = dc.opD[i]; }
   for (int q = 0; q < K; ++q) {

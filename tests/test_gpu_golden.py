@@ -24,7 +24,8 @@ def gpu_case(case):
     return g
 
 
-@pytest.mark.parametrize("name,steps", [("tgv_cart_n3_p2", 0), ("tgv_curv_n4_p4", 1), ("tgv_curv_n4_p5", 4)])
+@pytest.mark.parametrize("name,steps", [("tgv_cart_n3_p2", 0), ("tgv_curv_n4_p4", 1), ("tgv_curv_n4_p5", 4),
+                                        ("tgv_curv_n4_p6", 0)])
 def test_small_cases(name, steps):
     print(f"[case] golden {name}", flush=True)
     case, ref = load(name)

@@ -15,7 +15,7 @@ from vcp3d_open_amd.cases import GAMMA, OpenBoxCase
 
 pytestmark = pytest.mark.gpu
 
-SMALL = ["ob_static_p3", "ob_total_p3", "ob_total_p2", "ob_static_p4", "ob_static_p5"]
+SMALL = ["ob_static_p3", "ob_total_p3", "ob_total_p2", "ob_static_p4", "ob_static_p5", "ob_static_p6"]
 
 
 def rel(a, b):

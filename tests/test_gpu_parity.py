@@ -21,6 +21,7 @@ CASES = [
     ("curv_n4_p5", dict(n=4, p=5, curvilinear=True)),
     ("cart_n2_p5", dict(n=2, p=5, rule=14)),
     ("cart_n3_p3_rule10", dict(n=3, p=3, rule=10)),
+    ("curv_n4_p6", dict(n=4, p=6, curvilinear=True)),
 ]
 
 
@@ -55,7 +56,7 @@ def test_residual_matches_oracle(lib, oracle_lib, name, kw, bug):
     assert np.abs(it_o - it_g).max() <= 1
 
 
-@pytest.mark.parametrize("name,kw", CASES[:4], ids=[c[0] for c in CASES[:4]])
+@pytest.mark.parametrize("name,kw", CASES[:4] + CASES[-1:], ids=[c[0] for c in CASES[:4] + CASES[-1:]])
 def test_rk_steps_match_oracle(lib, oracle_lib, name, kw):
     from vcp3d_open_amd.solver import SolverClass
     case = TGVCase(**kw)

@@ -47,7 +47,8 @@ def test_reference_conditions(orc):
     np.testing.assert_allclose([r["u_ref"], r["p_ref"], r["rho_ref"], r["mu"]], g["reference"], rtol=1e-14)
 
 
-@pytest.mark.parametrize("name,steps", [("tgv_cart_n3_p2", 0), ("tgv_curv_n4_p4", 1), ("tgv_curv_n4_p5", 4)])
+@pytest.mark.parametrize("name,steps", [("tgv_cart_n3_p2", 0), ("tgv_curv_n4_p4", 1), ("tgv_curv_n4_p5", 4),
+                                        ("tgv_curv_n4_p6", 0)])
 def test_small_cases(orc, name, steps):
     case, g = load(name)
     _, out = run_oracle(orc, case, steps)

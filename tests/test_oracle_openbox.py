@@ -36,7 +36,8 @@ def start(orc, case):
     return o
 
 
-@pytest.mark.parametrize("name", ["ob_static_p3", "ob_total_p3", "ob_total_p2", "ob_static_p4", "ob_static_p5"])
+@pytest.mark.parametrize("name", ["ob_static_p3", "ob_total_p3", "ob_total_p2", "ob_static_p4", "ob_static_p5",
+                                  "ob_static_p6"])
 def test_open_box_against_reference(orc, name):
     case, g = load(name)
     o = start(orc, case)
@@ -68,7 +69,7 @@ def test_open_box_full_size_subset(orc):
     np.testing.assert_allclose(np.sqrt((R ** 2).sum(axis=(0, 2))), g["res0_norms"], rtol=RES_TOL)
 
 
-@pytest.mark.parametrize("p,rule", [(2, 6), (3, 9), (4, 12), (5, 15)])
+@pytest.mark.parametrize("p,rule", [(2, 6), (3, 9), (4, 12), (5, 15), (6, 16)])
 def test_pc_matrix_is_scaled_projector(orc, p, rule):
     """PC = (1/c) Q Q^T, Q an orthonormal basis of the degree-P polynomials
     on the M x M Gauss points (any basis spans the same space; Legendre is
@@ -85,7 +86,7 @@ def test_pc_matrix_is_scaled_projector(orc, p, rule):
     np.testing.assert_allclose(o.table(6), (Q @ Q.T) / c, atol=1e-12)
 
 
-@pytest.mark.parametrize("rule", [6, 9, 12, 15])
+@pytest.mark.parametrize("rule", [6, 9, 12, 15, 16])
 def test_gauss_lagrange_derivative(orc, rule):
     case = OpenBoxCase(n=(3, 2, 2), p=2, rule=rule, sponge_elems=0)
     o = orc.Oracle(case)

@@ -276,15 +276,20 @@ struct FaceArgs {
 // ============================================================ face kernel
 // One wave64 per element min face in direction DIR. Side 0 is the element
 // itself (its min face), side 1 the neighbour L (its max face); the normal
-// points from L to the element. Every phase maps its work items onto the 64
-// lanes in a single pass, so a phase may overwrite its own input in place
-// (all lanes load before any lane stores). LDS regions:
-//   X: normal contraction B -> fast contraction T (in place) -> flux and
-//      Cartesian symmetrising fluxes at the points (in place)
+// points from L to the element. When the fast contraction (2 * 5 K columns)
+// and the points (M^2) each fit the 64 lanes in a single pass, those phases
+// overwrite their own input in place (all lanes load before any lane
+// stores); otherwise (K = 7, M = 9) they loop over their items and T gets a
+// region of its own. LDS regions:
+//   X: normal contraction B -> fast contraction T -> flux and Cartesian
+//      symmetrising fluxes at the points (over B, or over T when in place)
 //   Y: the first lift stage of one side, in two halves
 template <int K, int M>
 struct FaceSmem {
-  static constexpr int kX = cmax(cmax(2 * 2 * 5 * K * K, 2 * 3 * 5 * K * M), 20 * M * M);
+  static constexpr bool kInPlace = 2 * 5 * K <= 64 && M * M <= 64;
+  static constexpr int kB = 2 * 2 * 5 * K * K, kT = 2 * 3 * 5 * K * M, kP = 20 * M * M;
+  static constexpr int kOffT = kInPlace ? 0 : cmax(kB, kP);   // the fluxes start at 0
+  static constexpr int kX = kInPlace ? cmax(cmax(kB, kT), kP) : kOffT + kT;
   static constexpr int kY = 2 * 5 * M * K;
   static constexpr int kBytes = 8 * (2 * M * K + M * M + 4 * K + 90 + kX + kY);
   // waves per SIMD to compile for: three when 12 faces fit a CU's 160 KB
@@ -302,7 +307,8 @@ __device__ __forceinline__ void face_body(FaceArgs<K, M> a, FaceSmem<K, M> &sm, 
   const int L = a.nbr[e * 6 + 2 * DIR];
   if (L < 0) return;  // physical boundary face
   constexpr int ND = K * K * K, N2 = M * M;
-  static_assert(2 * 5 * K <= 64 && N2 <= 64 && 5 * M <= 64, "single-pass phases");
+  using S = FaceSmem<K, M>;
+  static_assert(5 * M <= 64 && 5 * K <= 64, "single-pass lift stages");
   // strides of the (normal, fast, slow) axes in the [k][j][i] DOF layout
   constexpr int SN = DIR == 0 ? 1 : DIR == 1 ? K : K * K;
   constexpr int SFA = DIR == 0 ? K : 1;
@@ -311,7 +317,7 @@ __device__ __forceinline__ void face_body(FaceArgs<K, M> a, FaceSmem<K, M> &sm, 
   constexpr int PN = DIR, PF = DIR == 0 ? 1 : 0, PS = DIR == 2 ? 1 : 2;
   double *sA = sm.A, *sD = sm.D, *sW2 = sm.W2;
   double *B = sm.X;                           // [side][value|deriv][5][K][K]
-  double *T = sm.X;                           // [side][A|D|N][5][K][M]
+  double *T = sm.X + S::kOffT;                // [side][A|D|N][5][K][M]
   double *sFlux = sm.X, *sFC = sm.X + 5 * N2;  // flux [5][N2], Cartesian sym [x|y|z][5][N2]
   const int tid = threadIdx.x;
   // operators in LDS: read as wave-uniform LDS operands in the contractions
@@ -319,6 +325,12 @@ __device__ __forceinline__ void face_body(FaceArgs<K, M> a, FaceSmem<K, M> &sm, 
   for (int i = tid; i < M * K; i += 64) { sA[i] = a.op.A[i]; sD[i] = a.op.D[i]; }
   for (int i = tid; i < 4 * K; i += 64) sm.F[i / K][i % K] = a.op.F[i / K][i % K];
   const double *A = sA, *D = sD;
+  // operators of the pencil contractions: the kernel arguments (wave-uniform
+  // scalar loads) while the argument block is under 1 KB; a larger one
+  // (K >= 6) the compiler copies to scratch when it is indexed at run time,
+  // so those read the LDS copies instead
+  constexpr bool kArgOps = sizeof(FaceOps<K, M>) < 1024;
+  const double *KA = kArgOps ? a.op.A : sA, *KD = kArgOps ? a.op.D : sD;
   for (int i = tid; i < N2; i += 64) sW2[i] = a.op.w[i % M] * a.op.w[i / M];
   const double *g = a.fgeo + ((size_t) e * 3 + DIR) * kFaceGeo * N2;
   if constexpr (BUG)
@@ -344,9 +356,12 @@ __device__ __forceinline__ void face_body(FaceArgs<K, M> a, FaceSmem<K, M> &sm, 
     }
   }
   __syncthreads();
-  // fast tangential contraction, in place over B: T[side][A|D|N][v][s][f'], f' < M
-  if (tid < 2 * 5 * K) {
-    const int side = tid / (5 * K), vs = tid - side * 5 * K;
+  // fast tangential contraction (in place over B when single-pass):
+  // T[side][A|D|N][v][s][f'], f' < M
+  for (int c0 = 0; c0 < 2 * 5 * K; c0 += 64) {
+    const int c = c0 + tid;
+    if (c >= 2 * 5 * K) break;
+    const int side = c / (5 * K), vs = c - side * 5 * K;
     double x0[K], x1[K];
 #pragma unroll
     for (int f = 0; f < K; ++f) {
@@ -354,7 +369,6 @@ __device__ __forceinline__ void face_body(FaceArgs<K, M> a, FaceSmem<K, M> &sm, 
       x1[f] = B[(side * 2 + 1) * 5 * K * K + vs * K + f];
     }
     double *Ts = T + side * 3 * 5 * K * M;
-    const double *KA = a.op.A, *KD = a.op.D;   // one operator row per iteration, scalar loads
 #pragma unroll 1
     for (int n = 0; n < M; ++n) {
       double tA = 0.0, tD = 0.0, tN = 0.0;
@@ -366,11 +380,12 @@ __device__ __forceinline__ void face_body(FaceArgs<K, M> a, FaceSmem<K, M> &sm, 
   __syncthreads();
   // per point: traces, averaged Cartesian gradients of the entropy variables
   // and the fluxes (Ismail-Roe, viscous, penalty, symmetrising); the outputs
-  // overwrite T in place
+  // overwrite T in place when single-pass, else the dead B
   const double lenScale = fmin(a.econst[(size_t) e * kElemConst + DIR], a.econst[(size_t) L * kElemConst + DIR]);
   const double tau = a.tauFact / lenScale;
-  if (tid < N2) {
-    const int m = tid;
+  for (int m0 = 0; m0 < N2; m0 += 64) {
+    const int m = m0 + tid;
+    if (m >= N2) break;
     const int sp = m / M, fp = m - sp * M;
     double ai[K], di[K];
 #pragma unroll
@@ -459,7 +474,6 @@ __device__ __forceinline__ void face_body(FaceArgs<K, M> a, FaceSmem<K, M> &sm, 
           }
         }
         double *Ys = sm.Y;
-        const double *KA = a.op.A, *KD = a.op.D;
 #pragma unroll 1
         for (int n = 0; n < K; ++n) {
           double p0 = 0.0, p1 = 0.0;
@@ -485,7 +499,6 @@ __device__ __forceinline__ void face_body(FaceArgs<K, M> a, FaceSmem<K, M> &sm, 
           x0[q] = sm.Y[j];
           if (part == 1) x1[q] = sm.Y[5 * M * K + j];
         }
-        const double *KA = a.op.A, *KD = a.op.D;
 #pragma unroll
         for (int n = 0; n < K; ++n) {
           double g0 = 0.0, g1 = 0.0;
@@ -601,9 +614,17 @@ __device__ __forceinline__ void cml_source(const double V[5], const double gx[5]
 // sum-factorisation intermediates and the primitive state at the points;
 // the 1D operators come from the kernel arguments (scalar registers) in the
 // pencil stages and from LDS rows in the per-point stages.
+// Workgroup size of the volume kernel: 256 threads, or 512 when the two point
+// passes or the DOFs do not fit 256 threads (K = 7, M = 9).
+constexpr int vol_threads(int K, int M)
+{
+  return K * K * K <= 256 && M * M * M <= 256 + (256 / M) * M ? 256 : 512;
+}
+
 // LDS layout of the volume kernel and the waves per SIMD it is compiled for
 // (NT = 256: three workgroups per CU when the LDS fits three times into 160 KB,
-// which caps the kernel at 168 VGPRs).
+// which caps the kernel at 168 VGPRs; NT = 512: one or more workgroups of
+// two waves per SIMD).
 template <int K, int M, int NT>
 struct VolShape {
   static constexpr int ND = K * K * K, NI = M * M * M, NR = M * M;
@@ -617,7 +638,11 @@ struct VolShape {
   static constexpr int SI = 15 * NR * K;            // backward i-stage output
   static constexpr int BIG = cmax(cmax(2 * SZ + SY, SF), SI + 10 * M * K * K);
   static constexpr int LDS = 8 * (2 * M * K + 5 * ND + BIG + 45 + 6 * (NT / 64) + 15 + 4 * K);
-  static constexpr int WAVES = 3 * LDS <= 160 * 1024 ? 3 : 2;
+  static constexpr int WGS = (160 * 1024) / LDS;   // workgroups per CU by LDS
+  static constexpr int WAVES = NT == 256 ? (WGS >= 3 ? 3 : 2) : (NT / 256) * (WGS > 1 ? WGS : 1);
+  static_assert(WGS >= 1, "volume kernel LDS exceeds 160 KB");
+  // mass-matrix solve buffers cz, cy, pI, by inside sBig
+  static_assert(SZ + 5 * M * M * K + 5 * NI + 5 * M * K * K <= BIG, "CG buffers");
 };
 
 template <int K, int M, int NT>
@@ -1794,7 +1819,8 @@ hipError_t volume_impl(const DeviceCase &dc, const StageCtl &ctl, hipStream_t s)
     a.c0[q] = dc.opC0[q];
     for (int t = 0; t < 4; ++t) a.fvec[t][q] = dc.opF[t][q];
   }
-  hipLaunchKernelGGL((volume_kernel<K, M, 256>), dim3(dc.nElem), dim3(256), 0, s, a);
+  constexpr int NT = vol_threads(K, M);
+  hipLaunchKernelGGL((volume_kernel<K, M, NT>), dim3(dc.nElem), dim3(NT), 0, s, a);
   if (ctl.monitor)
     hipLaunchKernelGGL(reduce_max_kernel, dim3(1), dim3(1024), 0, s, dc.mach2Elem, dc.nElem, dc.scal + 1);
   return hipGetLastError();
@@ -1814,7 +1840,7 @@ hipError_t dt_impl(const DeviceCase &dc, hipStream_t s)
 
 }  // namespace
 
-#define VCP_PAIRS(X) X(3, 4) X(4, 5) X(4, 6) X(5, 7) X(6, 8)
+#define VCP_PAIRS(X) X(3, 4) X(4, 5) X(4, 6) X(5, 7) X(6, 8) X(7, 9)
 
 bool supported_pair(int K, int M)
 {

@@ -12,8 +12,10 @@ value = 5 * nElem * (P+1)^3 * 3 stages * steps / seconds / 1e9
         (GDOF-updates/s per RK stage), inputs resident in HBM.
 
 Also reported: the roofline position of the dominant kernel (its mean launch
-time measured with HIP events on the library's stream) and a CPU baseline
-(the C oracle, OpenMP over the host cores, one RK step of the same case).
+time measured with HIP events on the library's stream) and a CPU baseline:
+one RK step of the same case by the reference itself, built from its sources
+(oracle/_ref), on the host cores, with the C oracle's rate beside it (the
+oracle alone where the reference build is absent or cannot run).
 """
 from __future__ import annotations
 
@@ -93,11 +95,15 @@ def calib_copy(nbytes: int) -> int:
     return nbytes
 
 
-def cpu_baseline(case, steps: int = 1) -> dict:
+def cpu_threads() -> int:
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def port_baseline(case, label: str, steps: int = 1) -> dict:
     """C oracle (OpenMP over the host cores) on the same case: one RK step."""
     from oracle.pyoracle import Oracle, build
     build()
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = cpu_threads()
     o = Oracle(case)
     o.set_restart(case.initial_conservative())
     if hasattr(case, "damping_primitive") and getattr(case, "sponge_elems", 0) > 0:
@@ -109,8 +115,53 @@ def cpu_baseline(case, steps: int = 1) -> dict:
     dt = time.perf_counter() - t0
     gdof = 5 * case.n_elems * case.n_dofs * 3 * steps / dt / 1e9
     return {"value": gdof, "unit": "GDOF/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} SSP-RK3 step(s) of the same 32^3 P=4 case, C oracle, "
+            "sample": f"{steps} SSP-RK3 step(s) of the same {label} case, C oracle, "
                       f"{threads} OpenMP threads, {dt:.2f} s"}
+
+
+def reference_baseline(case, label: str):
+    """The reference itself (VCP3D_CurviLinear built from its sources with its
+    own Makefile flags into oracle/_ref/) on the same case, driven through its
+    public API: the wall time of one ExecuteTimeSynchronizationStep (one RK
+    step; its setup and input reading are not timed). Returns (result, None),
+    or (None, reason) when the build is absent or does not run on this host."""
+    import tempfile
+    from oracle.pyoracle import REF_DRIVER, run_reference
+    if not os.path.exists(REF_DRIVER):
+        return None, "reference build not present"
+    threads = cpu_threads()
+    try:
+        with tempfile.TemporaryDirectory(prefix="vcp_ref_") as d:
+            prm = case.write_inputs(d)
+            t0 = time.perf_counter()
+            info = run_reference(prm, 1, (), threads=threads, timeout=600)
+            wall = time.perf_counter() - t0
+    except Exception as exc:  # a missing library or an unsupported instruction on this host
+        return None, f"reference build did not run: {exc}"
+    step = float(info["step_seconds"][0])
+    gdof = 5 * case.n_elems * case.n_dofs * 3 / step / 1e9
+    return {"value": gdof, "unit": "GDOF/s", "cores": threads, "kind": "reference",
+            "sample": f"1 SSP-RK3 step (ExecuteTimeSynchronizationStep) of the same {label} case, "
+                      f"reference VCP3D_CurviLinear built from its sources (-O2 -march=native, "
+                      f"OpenMP), {threads} threads, {step:.2f} s per step ({wall:.1f} s with "
+                      f"setup and I/O, untimed)"}, None
+
+
+def cpu_baseline(case, workload: str) -> dict:
+    """The reference build where it runs; the C oracle otherwise (and for the
+    open box, where the reference's first step at P = 4 writes past a nodal
+    buffer and aborts). The oracle's rate is reported beside it either way."""
+    label = "48x24x24 P=4 open-box" if workload == "openbox" else "32^3 P=4"
+    port = port_baseline(case, label)
+    if workload == "openbox":
+        return port
+    ref, why = reference_baseline(case, label)
+    if ref is None:
+        port["note"] = why
+        return port
+    ref["port_value"] = port["value"]
+    ref["port_sample"] = port["sample"]
+    return ref
 
 
 def main() -> int:
@@ -192,10 +243,7 @@ def main() -> int:
     if args.calib_copy_mib:
         out["calib_copy_bytes"] = calib_copy(args.calib_copy_mib << 20)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(case)
-        if args.workload == "openbox":
-            out["cpu_baseline"]["sample"] = out["cpu_baseline"]["sample"].replace(
-                "32^3 P=4", "48x24x24 P=4 open-box")
+        out["cpu_baseline"] = cpu_baseline(case, args.workload)
     print(json.dumps(out))
     return 0
 

@@ -136,6 +136,17 @@ int vcp_get_residual(vcp_ctx *ctx, double *modal);
    number of RK steps taken (>= 1) or a negative code. */
 int vcp_sync_step(vcp_ctx *ctx, double monitoring[8]);
 
+/* Execution option with no counterpart in the reference (the reference runs
+   each stage as OpenMP loops, SolverClass.cpp:2200-2414): with enable != 0,
+   vcp_sync_step replays every RK step (the stage-0 update, the three residual
+   stages and, on the last step of a call, the next time step on a side
+   stream) as an instantiated HIP graph, captured on first use. The stage
+   kernels read the step's time step from device memory, so four captures
+   (which state buffer is current x last step of the call or not) serve every
+   step. Results are identical to the launch-by-launch path (enable = 0, the
+   default). */
+int vcp_set_graph(vcp_ctx *ctx, int enable);
+
 /* ElementClass::ComputeTimeStep of every element (ElementClass.cpp:1522-1644). */
 int vcp_time_steps(vcp_ctx *ctx, double *dt_elem);
 /* Iterations of the last per-element mass-matrix solve

@@ -51,6 +51,11 @@ struct vcp_ctx {
   bool dt_ready = false;          // host[0] holds the time step of the current state
   bool damping_set = false;
   int time_step = 0;
+  // RK steps replayed as captured HIP graphs (vcp_set_graph), one per
+  // [state buffer that is current][last step of the call], captured on first use
+  bool use_graph = false;
+  double *U_first = nullptr;
+  hipGraphExec_t graph[2][2] = {};
 
   template <class T>
   hipError_t alloc(T **p, size_t n)
@@ -62,6 +67,9 @@ struct vcp_ctx {
   }
   ~vcp_ctx()
   {
+    for (auto &row : graph)
+      for (hipGraphExec_t g : row)
+        if (g) (void) hipGraphExecDestroy(g);
     for (void *p : allocs) (void) hipFree(p);
     if (host) (void) hipHostFree(host);
     if (evState) (void) hipEventDestroy(evState);
@@ -100,6 +108,63 @@ int residual_pass(vcp_ctx *c, const vcp::StageCtl &ctl)
   HIP_TRY(vcp::launch_boundary(c->dc, c->stream));
   HIP_TRY(vcp::launch_faces(c->dc, c->stream));
   HIP_TRY(vcp::launch_volume(c->dc, ctl, c->stream));
+  return 0;
+}
+
+// The stages of one SSP-RK3 step (Shu-Osher form, SolverClass.cpp:2192-2194,
+// :2385-2393), with U0 holding the step's state; its time step dt is passed to
+// the stage-0 kernel, which stores it to scal[kScalDt] for the other stages,
+// or (dt null, for a captured step) is already in scal[kScalDt]. Stage 0 uses the residual of the previous evaluation (first same as
+// last); U is final after the second stage, so on the last step of a call the
+// next call's time step runs on the side stream beside the third stage, which
+// only evaluates the residual, and is joined at the end.
+int rk_stages(vcp_ctx *c, bool last, const double *dt)
+{
+  static const double alphaRK[] = {1.0, 3.0 / 4.0, 1.0 / 3.0};
+  static const double betaRK[] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
+  int rc = 0;
+  HIP_TRY(vcp::launch_stage0(c->dc, dt, c->stream));
+  const vcp::StageCtl s1{0, 1, alphaRK[1], betaRK[1], 0};
+  const vcp::StageCtl s2{0, 1, alphaRK[2], betaRK[2], 0};
+  const vcp::StageCtl s3{1, 0, 0.0, 0.0, last ? 1 : 0};
+  if ((rc = residual_pass(c, s1))) return rc;
+  if ((rc = residual_pass(c, s2))) return rc;
+  if (last) {
+    HIP_TRY(hipEventRecord(c->evState, c->stream));
+    HIP_TRY(hipStreamWaitEvent(c->side, c->evState, 0));
+    HIP_TRY(vcp::launch_time_step(c->dc, c->side));
+    HIP_TRY(hipEventRecord(c->evDt, c->side));
+  }
+  if ((rc = residual_pass(c, s3))) return rc;
+  if (last) HIP_TRY(hipStreamWaitEvent(c->stream, c->evDt, 0));
+  return 0;
+}
+
+// rk_stages replayed from an instantiated graph. The launches of a step
+// depend only on which state buffer is current and on whether it is the last
+// step of the call (the time step is read from device memory, where a kernel
+// launched before the graph puts it), so four captures cover every step.
+int rk_graph(vcp_ctx *c, bool last, double dt)
+{
+  HIP_TRY(vcp::launch_set_dt(c->dc, dt, c->stream));
+  hipGraphExec_t &exec = c->graph[c->dc.U == c->U_first ? 0 : 1][last ? 1 : 0];
+  if (!exec) {
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = rk_stages(c, last, nullptr);
+    const hipError_t he = hipStreamEndCapture(c->stream, &g);
+    if (rc || he != hipSuccess) {
+      if (g) (void) hipGraphDestroy(g);
+      return rc ? rc : fail(std::string("hipStreamEndCapture: ") + hipGetErrorString(he), -3);
+    }
+    const hipError_t hi = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void) hipGraphDestroy(g);
+    if (hi != hipSuccess) {
+      exec = nullptr;
+      return fail(std::string("hipGraphInstantiate: ") + hipGetErrorString(hi), -3);
+    }
+  }
+  HIP_TRY(hipGraphLaunch(exec, c->stream));
   return 0;
 }
 
@@ -247,6 +312,7 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
   if (he == hipSuccess) he = c->alloc(&dc.nbr, g.nbr.size());
   if (he == hipSuccess) he = c->alloc(&dc.U, ns);
   if (he == hipSuccess) he = c->alloc(&dc.U0, ns);
+  c->U_first = dc.U;
   if (he == hipSuccess) he = c->alloc(&dc.R, ns);
   // face slots: per element and block face, the value and normal-derivative
   // coefficient planes [2][5][K][K] of the lifted face residual
@@ -451,8 +517,6 @@ int vcp_time_steps(vcp_ctx *c, double *dt_elem)
 
 int vcp_sync_step(vcp_ctx *c, double md[8])
 {
-  static const double alphaRK[] = {1.0, 3.0 / 4.0, 1.0 / 3.0};
-  static const double betaRK[] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
   if (!c) return fail("null context");
   double dummy[8];
   if (!md) md = dummy;
@@ -483,26 +547,10 @@ int vcp_sync_step(vcp_ctx *c, double md[8])
     if (elapsed + dt >= 0.99 * dtSync) { dt = dtSync - elapsed; reached = true; }
     elapsed += dt;
     std::swap(c->dc.U, c->dc.U0);   // the step's state becomes U0, stage 0 writes U
-    HIP_TRY(vcp::launch_stage0(c->dc, dt, c->stream));
-    vcp::StageCtl s1{0, 1, alphaRK[1], betaRK[1] * dt, 0};
-    vcp::StageCtl s2{0, 1, alphaRK[2], betaRK[2] * dt, 0};
-    vcp::StageCtl s3{1, 0, 0.0, 0.0, reached ? 1 : 0};
-    if ((rc = residual_pass(c, s1))) return rc;
-    if ((rc = residual_pass(c, s2))) return rc;
+    if ((rc = c->use_graph ? rk_graph(c, reached, dt) : rk_stages(c, reached, &dt))) return rc;
     if (reached) {
-      // U is final after the second stage (the third only evaluates the
-      // residual for the next call): the time step of the new state runs on
-      // the side stream while the third stage runs
-      HIP_TRY(hipEventRecord(c->evState, c->stream));
-      HIP_TRY(hipStreamWaitEvent(c->side, c->evState, 0));
-      HIP_TRY(vcp::launch_time_step(c->dc, c->side));
-      HIP_TRY(hipEventRecord(c->evDt, c->side));
-    }
-    if ((rc = residual_pass(c, s3))) return rc;
-    if (reached) {
-      // read back with the monitoring maximum and the error flags in one
-      // round trip
-      HIP_TRY(hipStreamWaitEvent(c->stream, c->evDt, 0));
+      // the next call's time step with the monitoring maximum and the error
+      // flags in one round trip
       HIP_TRY(hipMemcpyAsync(c->host, c->dc.scal, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
       HIP_TRY(hipMemcpyAsync(c->host + 2, c->dc.flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
       HIP_TRY(hipStreamSynchronize(c->stream));
@@ -516,6 +564,13 @@ int vcp_sync_step(vcp_ctx *c, double md[8])
   }
   c->res_valid = true;
   return nSteps;
+}
+
+int vcp_set_graph(vcp_ctx *c, int enable)
+{
+  if (!c) return fail("null context");
+  c->use_graph = enable != 0;
+  return 0;
 }
 
 int vcp_cg_iterations(vcp_ctx *c, int *iters)

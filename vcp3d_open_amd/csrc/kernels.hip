@@ -1184,6 +1184,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
   double *Re = a.R + (size_t) e * 5 * ND;
   double *Uw = a.U + (size_t) e * 5 * ND;
   const double *U0e = a.U0 + (size_t) e * 5 * ND;
+  const double bdt = a.ctl.update ? a.ctl.b * a.scal[kScalDt] : 0.0;
   bool bad = false;
   if (own)
 #pragma unroll
@@ -1191,7 +1192,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
       const int i = l * ND + tid;
       if (!isfinite(x[l])) bad = true;
       if (a.ctl.store_res) Re[i] = x[l];
-      if (a.ctl.update) Uw[i] = (1.0 - a.ctl.a) * Uw[i] + a.ctl.a * U0e[i] - a.ctl.bdt * x[l];
+      if (a.ctl.update) Uw[i] = (1.0 - a.ctl.a) * Uw[i] + a.ctl.a * U0e[i] - bdt * x[l];
     }
   if (bad) atomicOr(a.flags, kFlagNonFinite);
 }
@@ -1763,11 +1764,23 @@ __global__ __launch_bounds__(1024) void reduce_max_kernel(const double *v, int n
 }
 
 // First RK stage: U <- U0 - dt R, where U0 already holds the state of the
-// step (the host swaps the two buffers instead of copying).
-__global__ void stage0_kernel(double *U, const double *U0, const double *R, size_t n, double dt)
+// step (the host swaps the two buffers instead of copying). dt is the
+// argument, which the first lane also stores to dtOut for the later stages,
+// or (dtIn non-null) is read from device memory where the host put it.
+__global__ void stage0_kernel(double *U, const double *U0, const double *R, size_t n, double dtArg,
+                              const double *dtIn, double *dtOut)
 {
+  const double dt = dtIn ? *dtIn : dtArg;
+  if (dtOut && blockIdx.x == 0 && threadIdx.x == 0) *dtOut = dt;
   for (size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x; i < n; i += (size_t) gridDim.x * blockDim.x)
     U[i] = U0[i] - dt * R[i];
+}
+
+// The time step of the RK step about to run, into device memory (one lane,
+// a vector store).
+__global__ void set_dt_kernel(double *p, double v)
+{
+  if (threadIdx.x == 0) *p = v;
 }
 
 template <int K, int M>
@@ -1886,11 +1899,18 @@ hipError_t launch_time_step(const DeviceCase &dc, hipStream_t s)
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_stage0(const DeviceCase &dc, double dt, hipStream_t s)
+hipError_t launch_set_dt(const DeviceCase &dc, double dt, hipStream_t s)
+{
+  hipLaunchKernelGGL(set_dt_kernel, dim3(1), dim3(64), 0, s, dc.scal + kScalDt, dt);
+  return hipGetLastError();
+}
+
+hipError_t launch_stage0(const DeviceCase &dc, const double *dt, hipStream_t s)
 {
   const size_t n = (size_t) dc.nElem * 5 * dc.nDOF;
   const int blocks = (int) ((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536);
-  hipLaunchKernelGGL(stage0_kernel, dim3(blocks), dim3(256), 0, s, dc.U, dc.U0, dc.R, n, dt);
+  hipLaunchKernelGGL(stage0_kernel, dim3(blocks), dim3(256), 0, s, dc.U, dc.U0, dc.R, n,
+                     dt ? *dt : 0.0, dt ? nullptr : dc.scal + kScalDt, dt ? dc.scal + kScalDt : nullptr);
   return hipGetLastError();
 }
 

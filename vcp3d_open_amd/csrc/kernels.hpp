@@ -50,7 +50,10 @@ struct DeviceCase {
   int *nbr = nullptr;
   double *U = nullptr, *U0 = nullptr, *R = nullptr;
   double *faceRes = nullptr;      // [nElem][6][2][5][K][K] face-slot coefficient planes
-  double *scal = nullptr;         // device scalars: [0] minimum time step, [1] maximum Mach^2
+  // device scalars: [0] minimum time step, [1] maximum Mach^2, [kScalDt] the
+  // time step of the RK step being taken (written by the host before its
+  // stages, so that the launches of a step do not depend on its value)
+  double *scal = nullptr;
   int *flags = nullptr;
   int *cgIter = nullptr;          // [nElem]
   double *dtElem = nullptr;       // [nElem]
@@ -70,11 +73,13 @@ struct DeviceCase {
   double *srcBuf = nullptr;       // [nSponge][5][nInt] source terms of the current residual
 };
 
+constexpr int kScalDt = 2;
+
 // Stage control for the fused volume/mass-matrix kernel.
 struct StageCtl {
   int store_res;      // write R = M^-1 res
-  int update;         // U = (1-a) U + a U0 - bdt R in place
-  double a, bdt;
+  int update;         // U = (1-a) U + a U0 - b dt R in place, dt = scal[kScalDt]
+  double a, b;
   int monitor;        // Mach^2 max
 };
 
@@ -84,6 +89,10 @@ hipError_t launch_faces(const DeviceCase &dc, hipStream_t s);
 hipError_t launch_boundary(const DeviceCase &dc, hipStream_t s);
 hipError_t launch_volume(const DeviceCase &dc, const StageCtl &ctl, hipStream_t s);
 hipError_t launch_time_step(const DeviceCase &dc, hipStream_t s);
-hipError_t launch_stage0(const DeviceCase &dc, double dt, hipStream_t s);
+// scal[kScalDt] = dt, ordered on the stream
+hipError_t launch_set_dt(const DeviceCase &dc, double dt, hipStream_t s);
+// U = U0 - dt R: with dt given, the kernel also stores it to scal[kScalDt];
+// with dt null it reads scal[kScalDt]
+hipError_t launch_stage0(const DeviceCase &dc, const double *dt, hipStream_t s);
 
 }  // namespace vcp

@@ -47,7 +47,7 @@ class CaseDesc(C.Structure):
 EXPORTS = ["vcp_create", "vcp_destroy", "vcp_sizes", "vcp_reference", "vcp_set_restart",
            "vcp_set_damping_state", "vcp_nscbc_average",
            "vcp_set_solution", "vcp_get_solution", "vcp_get_nodal", "vcp_residual",
-           "vcp_get_residual", "vcp_sync_step", "vcp_time_steps", "vcp_cg_iterations",
+           "vcp_get_residual", "vcp_sync_step", "vcp_set_graph", "vcp_time_steps", "vcp_cg_iterations",
            "vcp_profile_residual", "vcp_host_setup", "vcp_last_error"]
 
 _lib = None
@@ -77,6 +77,7 @@ def load_library(path: str = LIB_PATH):
     L.vcp_residual.argtypes = [vp, C.c_int, dp]
     L.vcp_get_residual.argtypes = [vp, dp]
     L.vcp_sync_step.argtypes = [vp, dp]
+    L.vcp_set_graph.argtypes = [vp, C.c_int]
     L.vcp_time_steps.argtypes = [vp, dp]
     L.vcp_cg_iterations.argtypes = [vp, ip]
     L.vcp_profile_residual.argtypes = [vp, C.c_int, dp]
@@ -134,7 +135,8 @@ def host_setup(case, bug_compat: bool = True) -> dict:
 class SolverClass:
     """One case on the GPU. Mirrors the reference SolverClass hot-path API."""
 
-    def __init__(self, case, bug_compat: bool = True, elem_grid: np.ndarray | None = None):
+    def __init__(self, case, bug_compat: bool = True, elem_grid: np.ndarray | None = None,
+                 use_graph: bool = False):
         self._L = load_library()
         self.case = case
         desc = desc_from_case(case, bug_compat)
@@ -147,6 +149,8 @@ class SolverClass:
         self._L.vcp_sizes(h, C.byref(ne), C.byref(k), C.byref(m))
         self.n_elem, self.K, self.M = ne.value, k.value, m.value
         self.n_dofs = self.K ** 3
+        if use_graph:
+            self.set_graph(True)
 
     def _check(self, rc: int) -> int:
         if rc < 0:
@@ -200,6 +204,11 @@ class SolverClass:
         out = np.zeros(4)
         self._check(self._L.vcp_reference(self.h, out))
         return dict(u_ref=out[0], p_ref=out[1], rho_ref=out[2], mu=out[3])
+
+    def set_graph(self, enable: bool) -> None:
+        """Replay each RK step of ExecuteTimeSynchronizationStep as a captured
+        HIP graph (identical results; fewer launches)."""
+        self._check(self._L.vcp_set_graph(self.h, 1 if enable else 0))
 
     def set_solution(self, modal: np.ndarray) -> None:
         self._check(self._L.vcp_set_solution(self.h, np.ascontiguousarray(modal, dtype=np.float64)))

@@ -1,0 +1,148 @@
+"""Case decks (vcp3d_open_amd/deck.py): parsing the reference's parameter
+files, building the same case description and initial state as the case
+builders, and the oracle run from the reference's own test decks
+(tests/golden/ref_n3_8x4x4/, copied from VCP3D_CurviLinear/test/n3_8x4x4)
+against the reference's outputs on them."""
+import os
+
+import numpy as np
+import pytest
+
+from tests.golden.make_golden import HERE as GOLDEN, load
+from vcp3d_open_amd import io as vio
+from vcp3d_open_amd.cases import OpenBoxCase, TGVCase
+from vcp3d_open_amd.deck import DeckCase, DeckError, ParamFile, write_restart
+
+STATIC = os.path.join(GOLDEN, "ref_n3_8x4x4", "param_static.cfg")
+TOTAL = os.path.join(GOLDEN, "ref_n3_8x4x4", "param_total.cfg")
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def test_reference_deck_is_parsed_as_the_reference_reads_it():
+    c = DeckCase(STATIC)
+    assert c.n == (8, 4, 4) and (c.p, c.p_grid, c.rule, c.dof_location) == (3, 3, 9, 2)
+    assert c.periodic == [0, 1, 1] and c.bc == [2, 1, 0, 0, 0, 0]
+    assert c.bc_data[0][:4] == [1.110, 35.57, 0.0, 0.0] and c.bc_data[1][0] == 101325.0
+    # values after '#' are comments; later keyword lines override nothing here
+    assert c.prm.text("fem working variables") == "Entropy"
+    assert c.prm.text("riemann solver") == "IsmailRoe"
+    assert c.nscbc == dict(outlet_len=1.0, outlet_sigma=0.25, outlet_beta_l=1.0, outlet_beta_t=-1.0,
+                           outlet_one_minus_beta=1, outlet_average_local=0, inlet_len=1.0,
+                           inlet_sigma=1.0, inlet_incoming=1)
+    assert c.pulse == dict(centre=[1.0, 0.5, 0.5], strength=0.1, width=0.1, one_d=False, direction=1.0)
+    assert (c.dt_synchr, c.cfl, c.n_sync_steps) == (2.5e-5, 2.0, 200)
+    t = DeckCase(TOTAL)
+    assert t.bc == [3, 1, 0, 0, 0, 0] and t.bc_data[0] == [102029.0, 320.207, 1.0, 0.0, 0.0]
+
+
+def test_free_stream_from_the_boundary_data():
+    """Static inflow: (rho, u) from the inlet, p from the outlet. Total
+    inflow: the Mach number from p_t / p, then the isentropic relations."""
+    q = DeckCase(STATIC).free_stream_primitive()
+    np.testing.assert_array_equal(q, [1.110, 35.57, 0.0, 0.0, 101325.0])
+    q = DeckCase(TOTAL).free_stream_primitive()
+    g = 1.4
+    m2 = ((102029.0 / 101325.0) ** (0.4 / g) - 1.0) * 2.0 / 0.4
+    t = 1.0 + 0.2 * m2
+    rho = 102029.0 / (287.058 * 320.207) / t ** (1.0 / 0.4)
+    np.testing.assert_allclose(q, [rho, np.sqrt(m2 * g * 101325.0 / rho), 0.0, 0.0, 101325.0], rtol=1e-14)
+    # isentropic: p / rho^gamma equals the total state's
+    assert abs(q[4] / q[0] ** g / (102029.0 / (102029.0 / (287.058 * 320.207)) ** g) - 1.0) < 1e-13
+
+
+def test_pressure_pulse_only_changes_the_pressure():
+    c = DeckCase(STATIC)
+    q = c.initial_primitive()
+    xyz = c.nodal_coords()
+    r2 = ((xyz - np.array([1.0, 0.5, 0.5])) ** 2).sum(axis=-1)
+    np.testing.assert_allclose(q[..., 4], 101325.0 * (1.0 + 0.1 * np.exp(-r2 * 0.5 / 0.01)), rtol=1e-15)
+    np.testing.assert_array_equal(q[..., :4], np.broadcast_to([1.110, 35.57, 0.0, 0.0], q[..., :4].shape))
+
+
+@pytest.mark.parametrize("make", [lambda: TGVCase(n=3, p=3, curvilinear=True),
+                                  lambda: OpenBoxCase(n=(4, 2, 2), p=3, sponge_elems=1),
+                                  lambda: OpenBoxCase(n=(4, 2, 2), p=2, rule=6, inflow="total",
+                                                      sponge_elems=0)],
+                         ids=["tgv", "openbox_static_sponge", "openbox_total"])
+def test_written_deck_reads_back_to_the_same_case(tmp_path, make):
+    case = make()
+    deck = DeckCase(case.write_inputs(str(tmp_path)))
+    want, got = case.description(), deck.description()
+    for key, val in want.items():
+        assert np.array_equal(np.asarray(got[key], dtype=float), np.asarray(val, dtype=float)), key
+    np.testing.assert_array_equal(deck.element_grid(), case.element_grid())
+    np.testing.assert_array_equal(deck.initial_conservative(), case.initial_conservative())
+    if getattr(case, "sponge_elems", 0) > 0:
+        np.testing.assert_array_equal(deck.damping_primitive(), case.damping_primitive())
+
+
+def test_restart_round_trip(tmp_path):
+    """write_restart writes what DeckCase reads back with "Restart: yes"
+    (SU2 restart payload, time step stored; no pulse at a nonzero step)."""
+    c = DeckCase(STATIC)
+    state = c.initial_conservative() * (1.0 + 1e-3 * np.random.default_rng(1).standard_normal((c.n_elems, c.n_dofs, 5)))
+
+    class Held:
+        def WriteRestartSolution(self):
+            return state
+
+    d = tmp_path / "run"
+    (d / "grid").mkdir(parents=True)
+    (d / "grid" / "Box_nPoly3_8x4x4.xyz").write_bytes(open(c.grid_file, "rb").read())
+    text = open(STATIC).read().replace("Restart: no", "Restart: yes")
+    (d / "param.cfg").write_text(text)
+    (d / "init").mkdir()
+    write_restart(Held(), c, str(d / "init" / "Restart.dat"), 7)
+    names, data, step = vio.read_su2_restart(str(d / "init" / "Restart.dat"), c.n_dofs)
+    assert step == 7 and names[:8] == ["x", "y", "z", "Density", "Momentum_x", "Momentum_y", "Momentum_z", "Energy"]
+    r = DeckCase(str(d / "param.cfg"))
+    np.testing.assert_array_equal(r.initial_conservative(), state)
+
+
+@pytest.mark.parametrize("edit,msg", [
+    (("FEM working variables: Entropy", "FEM working variables: Conservative"), "fem working variables"),
+    (("BCOutflowNSCBC  ", "BCOutflowSubsonic"), "not supported"),
+    (("BCOutflowNSCBC  ", "BCOutflowCharacteristic"), "unknown boundary condition"),
+    (("Subgrid scale model: None", "Subgrid scale model: WALE"), "Subgrid"),
+    ((" Fluctuations to add to initial solution: Pressure pulse",
+      " Fluctuations to add to initial solution: Random fluctuations"), "random fluctuations"),
+])
+def test_unsupported_settings_are_rejected(tmp_path, edit, msg):
+    text = open(STATIC).read()
+    assert edit[0] in text
+    (tmp_path / "grid").mkdir()
+    (tmp_path / "grid" / "Box_nPoly3_8x4x4.xyz").write_bytes(
+        open(os.path.join(GOLDEN, "ref_n3_8x4x4", "grid", "Box_nPoly3_8x4x4.xyz"), "rb").read())
+    (tmp_path / "p.cfg").write_text(text.replace(edit[0], edit[1], 1))
+    with pytest.raises(DeckError, match=f"(?i){msg}"):
+        DeckCase(str(tmp_path / "p.cfg"))
+
+
+def test_keyword_lines_follow_the_reference_rules(tmp_path):
+    p = tmp_path / "k.cfg"
+    p.write_text("\tCFL Number :  1.5   # comment: 9\n"
+                 "cfl number: 2.5\n"
+                 "empty value:\n"
+                 "Subface information\n" + "".join(f"{f} 1\n Periodic1to1Trans 1 2 1 2 1 2\n" for f in range(1, 7)))
+    prm = ParamFile(str(p))
+    assert prm.number("cfl number") == 2.5 and not prm.has("empty value")
+
+
+@pytest.mark.parametrize("name", ["deck_n3_8x4x4_static", "deck_n3_8x4x4_total"])
+def test_oracle_on_reference_decks(oracle_lib, name):
+    """The oracle set up from the reference's own test deck (free stream from
+    the boundary data plus the pressure pulse) against the reference run on
+    the same deck: R(U_0), and the state and residual after 3 steps."""
+    case, g = load(name)
+    o = oracle_lib.Oracle(case)
+    o.set_restart(case.initial_conservative())
+    np.testing.assert_allclose(list(o.reference().values()), g["reference"], rtol=1e-14)
+    o.residual()
+    assert rel(o.residual_array(), g["res0"]) < 1e-9
+    for _ in range(3):
+        o.sync_step()
+    assert rel(o.nodal(), g["sol_3"]) < 1e-12
+    assert rel(o.residual_array(), g["res_3"]) < 1e-9

@@ -1,0 +1,66 @@
+"""The GPU path set up from case decks (vcp3d_open_amd/deck.py), through the
+C ABI: the reference's own test decks (tests/golden/ref_n3_8x4x4/, from
+VCP3D_CurviLinear/test/n3_8x4x4) against the reference's outputs on them,
+and a checkpoint written by write_restart resumed from a deck.
+
+Tolerances as for the other parity tests: residual (R = M^-1 res) within
+relative L2 1e-9, fields within 1e-12.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from tests.golden.make_golden import HERE as GOLDEN, load
+from vcp3d_open_amd.deck import open_solver, write_restart
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from vcp3d_open_amd.solver import load_library
+    return load_library()
+
+
+@pytest.mark.parametrize("name,deck", [("deck_n3_8x4x4_static", "param_static.cfg"),
+                                       ("deck_n3_8x4x4_total", "param_total.cfg")])
+def test_reference_deck_matches_reference(lib, name, deck):
+    print(f"[case] {name}", flush=True)
+    _, gold = load(name)
+    s, case = open_solver(os.path.join(GOLDEN, "ref_n3_8x4x4", deck))
+    np.testing.assert_allclose(list(s.reference().values()), gold["reference"], rtol=1e-14)
+    s.Residual(True)
+    assert rel(s.residual_array(), gold["res0"]) < 1e-9
+    for _ in range(3):
+        md = s.ExecuteTimeSynchronizationStep()
+    # the reference driver prints the monitoring data of its last step
+    assert abs(md[6] - gold["monitoring"][6]) <= 1e-9 * gold["monitoring"][6]
+    assert rel(s.WriteRestartSolution(), gold["sol_3"]) < 1e-12
+    assert rel(s.residual_array(), gold["res_3"]) < 1e-9
+
+
+def test_checkpoint_and_resume(lib, tmp_path):
+    """Two steps, a restart file, a new solver from a deck with "Restart: yes",
+    one more step: the same state as three uninterrupted steps."""
+    src = os.path.join(GOLDEN, "ref_n3_8x4x4")
+    print("[case] deck checkpoint/resume", flush=True)
+    a, case = open_solver(os.path.join(src, "param_static.cfg"))
+    b, _ = open_solver(os.path.join(src, "param_static.cfg"))
+    for _ in range(3):
+        a.ExecuteTimeSynchronizationStep()
+    for _ in range(2):
+        b.ExecuteTimeSynchronizationStep()
+    (tmp_path / "grid").mkdir()
+    (tmp_path / "grid" / "Box_nPoly3_8x4x4.xyz").write_bytes(open(case.grid_file, "rb").read())
+    (tmp_path / "init").mkdir()
+    write_restart(b, case, str(tmp_path / "init" / "Restart.dat"), 2)
+    text = open(os.path.join(src, "param_static.cfg")).read().replace("Restart: no", "Restart: yes")
+    (tmp_path / "param.cfg").write_text(text)
+    c, _ = open_solver(str(tmp_path / "param.cfg"))
+    c.ExecuteTimeSynchronizationStep()
+    assert rel(c.WriteRestartSolution(), a.WriteRestartSolution()) < 1e-12

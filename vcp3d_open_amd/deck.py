@@ -312,7 +312,15 @@ class DeckCase:
                               strength=prm.number("pressure pulse strength", 0.5),
                               width=prm.number("pressure pulse width", 0.5),
                               one_d=prm.yes("one-dimensional pulse", False),
-                              direction=prm.number("pulse (x-)direction", 1.0))
+                              direction=self._pulse_direction(prm))
+
+    @staticmethod
+    def _pulse_direction(prm: ParamFile) -> float:
+        """"Pulse (x-)direction": upstream -1, downstream +1 (default +1)."""
+        v = prm.text("pulse (x-)direction", "downstream").lower()
+        if v not in ("upstream", "downstream"):
+            raise DeckError('unknown "Pulse (x-)direction" (upstream or downstream)')
+        return -1.0 if v == "upstream" else 1.0
 
     # ------------------------------------------------------------ case API
     @property

@@ -59,10 +59,13 @@ typedef struct vcp_case_desc {
      (InputParamClass.cpp:1441-1576, :1340-1398). */
   int    bc[6];              /* VCP_BC_* */
   double bc_data[6][5];      /* dimensional prescribed data of the face:
-                                outflow {Pressure}; static inflow {Density,
-                                VelocityX, VelocityY, VelocityZ}; total inflow
+                                NSCBC or subsonic outflow {Pressure}; static
+                                NSCBC inflow {Density, VelocityX, VelocityY,
+                                VelocityZ}; total NSCBC or subsonic inflow
                                 {TotalPressure, TotalTemperature, VelocityDirX,
-                                VelocityDirY, VelocityDirZ}                      */
+                                VelocityDirY, VelocityDirZ}; heat-flux wall
+                                {HeatFlux}; isothermal wall {Temperature};
+                                inviscid wall {}                                 */
   /* "NSCBC tuning parameters" (InputParamClass.cpp:406-424) */
   double outlet_len;         /* "Outlet characteristic length"                  */
   double outlet_sigma;       /* "Outlet normal relaxation (sigma)"              */
@@ -86,8 +89,13 @@ typedef struct vcp_case_desc {
   double damping_exponent[6];
 } vcp_case_desc;
 
+/* Block-face boundary conditions ("Subface information",
+   InputParamClass.cpp:1530-1548): BCOutflowNSCBC, BCInflowStaticNSCBC,
+   BCInflowTotalNSCBC, BCInflowSubsonic (total conditions), BCOutflowSubsonic,
+   BCWallHeatFlux, BCWallIsothermal, BCWallInviscid. */
 enum { VCP_BC_PERIODIC = 0, VCP_BC_OUTFLOW_NSCBC = 1, VCP_BC_INFLOW_STATIC_NSCBC = 2,
-       VCP_BC_INFLOW_TOTAL_NSCBC = 3 };
+       VCP_BC_INFLOW_TOTAL_NSCBC = 3, VCP_BC_INFLOW_SUBSONIC = 4, VCP_BC_OUTFLOW_SUBSONIC = 5,
+       VCP_BC_WALL_HEATFLUX = 6, VCP_BC_WALL_ISOTHERMAL = 7, VCP_BC_WALL_INVISCID = 8 };
 
 typedef struct vcp_ctx vcp_ctx;
 

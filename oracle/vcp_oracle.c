@@ -779,7 +779,11 @@ static void face_lift(const orc_case *c, int f, const double *flux, const double
 
 /* Ismail-Roe entropy-variable flux (FluxesFace.cpp:813-975) at one point;
    n is the unit normal from L to R, area = weight * |n| factor. */
-static void ismail_roe(const double *VL, const double *VR, const double n[3], double area, double *flux)
+/* InviscidFluxesFace, Ismail-Roe branch for entropy variables
+   (FluxesFace.cpp:813-975); factU (factUNorm, 0 on walls) multiplies the
+   convective and dissipation terms, not the pressure term. */
+static void ismail_roe(const double *VL, const double *VR, const double n[3], double area, double factU,
+                       double *flux)
 {
   const double beta = 1.0 / 6.0, alphaMax = 0.0;
   const double gam = GAM, gm1 = GAM - 1.0, ovgm1 = 1.0 / gm1;
@@ -821,11 +825,11 @@ static void ismail_roe(const double *VL, const double *VR, const double n[3], do
   const double abv5 = n[0] * dru + n[1] * drv + n[2] * drw - vn * dr;
   const double abv6 = abv3 * abv4 * ova2 + abv2 * abv5 * ova;
   const double abv7 = abv2 * abv4 * ova + abv3 * abv5;
-  flux[0] = rA * vn - (lam3 * dr + abv6);
-  flux[1] = rA * vn * uA + p1 * n[0] - (lam3 * dru + uA * abv6 + n[0] * abv7);
-  flux[2] = rA * vn * vA + p1 * n[1] - (lam3 * drv + vA * abv6 + n[1] * abv7);
-  flux[3] = rA * vn * wA + p1 * n[2] - (lam3 * drw + wA * abv6 + n[2] * abv7);
-  flux[4] = rA * vn * H - (lam3 * drE + H * abv6 + vn * abv7);
+  flux[0] = factU * rA * vn - factU * (lam3 * dr + abv6);
+  flux[1] = factU * rA * vn * uA + p1 * n[0] - factU * (lam3 * dru + uA * abv6 + n[0] * abv7);
+  flux[2] = factU * rA * vn * vA + p1 * n[1] - factU * (lam3 * drv + vA * abv6 + n[1] * abv7);
+  flux[3] = factU * rA * vn * wA + p1 * n[2] - factU * (lam3 * drw + wA * abv6 + n[2] * abv7);
+  flux[4] = factU * rA * vn * H - factU * (lam3 * drE + H * abv6 + vn * abv7);
   for (int l = 0; l < 5; ++l) flux[l] *= area;
 }
 
@@ -835,10 +839,15 @@ static void ismail_roe(const double *VL, const double *VR, const double n[3], do
    parametric symmetrising fluxes for L and R. fn multiplies the unit normal
    of mL (factNorm: +1 interior and max faces, -1 boundary min faces,
    FluxesFace.cpp:105-106, :168-174); a boundary face passes its own metric
-   as mL and mR (FluxesFace.cpp:112-147, :191-210). */
+   as mL and mR (FluxesFace.cpp:112-147, :191-210). On a wall, factU = 0
+   (wallPerm) and, with a prescribed heat flux qWall (hf = 1), the laminar
+   heat-flux factor is 0 in the viscous, penalty and symmetrising fluxes and
+   w A qWall is added to the energy flux (FluxesFace.cpp:1312-1318,
+   :1586-1593). */
 static void internal_face_flux(const orc_case *c, double fn, double lenScale,
                                const double *SL, double *GrL, double *GsL, double *GtL, const double *mL,
                                const double *SR, double *GrR, double *GsR, double *GtR, const double *mR,
+                               double factU, int hf, double qWall,
                                double *flux, double *sxL, double *syL, double *szL,
                                double *sxR, double *syR, double *szR)
 {
@@ -859,7 +868,7 @@ static void internal_face_flux(const orc_case *c, double fn, double lenScale,
       gx[i] = 0.5 * (xL + xR); gy[i] = 0.5 * (yL + yR); gz[i] = 0.5 * (zL + zR);
     }
   const double gm1 = GAM - 1.0, ovgm1 = 1.0 / gm1;
-  const double mu = c->mu, factHF = GAM / PRANDTL_LAM;
+  const double mu = c->mu, factHF = hf ? 0.0 : GAM / PRANDTL_LAM;
   const double tau = (c->P + 1.0) * (c->P + 1.0) * c->prm.rel_penalty / lenScale;
   const double lam = LAMBDA_OVER_MU, lamp1 = lam + 1.0, lamp2 = lam + 2.0;
   for (int m = 0; m < n2; ++m) {
@@ -867,7 +876,7 @@ static void internal_face_flux(const orc_case *c, double fn, double lenScale,
     for (int l = 0; l < 5; ++l) { VL[l] = SL[l * n2 + m]; VR[l] = SR[l * n2 + m]; }
     const double nn[3] = {fn * mL[0 * n2 + m], fn * mL[1 * n2 + m], fn * mL[2 * n2 + m]};
     const double wA = c->w2[m] * mL[3 * n2 + m];
-    ismail_roe(VL, VR, nn, wA, fl);
+    ismail_roe(VL, VR, nn, wA, factU, fl);
     /* viscous flux on the averaged state */
     double Va[5];
     for (int l = 0; l < 5; ++l) Va[l] = 0.5 * (VL[l] + VR[l]);
@@ -891,6 +900,7 @@ static void internal_face_flux(const orc_case *c, double fn, double lenScale,
     fl[3] -= txz * nx + tyz * ny + tzz * nz;
     fl[4] -= (u * txx + v * txy + w * txz - qx) * nx + (u * txy + v * tyy + w * tyz - qy) * ny
            + (u * txz + v * tyz + w * tzz - qz) * nz;
+    if (hf) fl[4] += wA * qWall;
     /* penalty and symmetrising fluxes (FluxesFace.cpp:1152-1284) */
     const double pOvRho = -V4Inv, e = pOvRho * ovgm1;
     const double dV1 = VL[1] - VR[1], dV2 = VL[2] - VR[2], dV3 = VL[3] - VR[3], dV4 = VL[4] - VR[4];
@@ -1184,24 +1194,113 @@ static void nscbc_state(const orc_case *c, int f, int e, const double *SL, doubl
       }
 }
 
-/* Physical (NSCBC) face f of element e: FluxesBoundaryFace
-   (FluxesFace.cpp:16-214) and the lift of the boundary branch of
-   ElementClass::{I,J,K}FaceResiduals (ElementClass.cpp:2751-2850). */
+/* Boundary state of a face with a pointwise boundary condition
+   (ComputeBoundaryState of BCInflowSubsonic, BCOutflowSubsonic,
+   BCHeatFluxWall, BCIsothermalWall, BCInviscidWall for entropy variables,
+   BoundaryState.cpp:40-118, :156-251, :252-343, :344-372, :399-604,
+   :636-792); m: the face metric (unit normal rows 0-2), fn: factNorm,
+   d: the non-dimensional prescribed data of the face. */
+static void pointwise_state(const orc_case *c, int bc, const double *m, double fn, const double *d,
+                            const double *SL, double *SR)
+{
+  const int n2 = c->nInt2D;
+  const double gm1 = GAM - 1.0, ovgm1 = 1.0 / gm1, govgm1 = GAM * ovgm1, ovg = 1.0 / GAM;
+  for (int l = 0; l < n2; ++l) {
+    double V[5], R[5];
+    for (int q = 0; q < 5; ++q) V[q] = SL[q * n2 + l];
+    const double n0 = m[l], n1 = m[n2 + l], n2_ = m[2 * n2 + l];
+    if (bc == ORC_BC_WALL_INVISCID) {
+      const double Vn = V[1] * n0 + V[2] * n1 + V[3] * n2_;
+      R[0] = V[0] - 0.5 * Vn * Vn / V[4];
+      R[1] = V[1] - Vn * n0; R[2] = V[2] - Vn * n1; R[3] = V[3] - Vn * n2_;
+      R[4] = V[4];
+    } else if (bc == ORC_BC_WALL_HEATFLUX || bc == ORC_BC_WALL_ISOTHERMAL) {
+      R[0] = V[0] - 0.5 * (V[1] * V[1] + V[2] * V[2] + V[3] * V[3]) / V[4];
+      R[1] = R[2] = R[3] = 0.0;
+      R[4] = bc == ORC_BC_WALL_HEATFLUX ? V[4] : -c->pRef / (RGAS * c->rhoRef) / d[0];
+    } else if (bc == ORC_BC_INFLOW_SUBSONIC) {
+      const double V4Inv = 1.0 / V[4];
+      const double u = -V4Inv * V[1], v = -V4Inv * V[2], w = -V4Inv * V[3];
+      const double a2 = -GAM * V4Inv, a = sqrt(a2);
+      const double nx = fn * n0, ny = fn * n1, nz = fn * n2_;
+      const double un = u * nx + v * ny + w * nz;
+      double riemann = 2.0 * a * ovgm1 + un;
+      const double H = a2 * ovgm1 + 0.5 * (u * u + v * v + w * w);
+      riemann *= sqrt(d[1] / H);
+      const double alpha = nx * d[2] + ny * d[3] + nz * d[4];
+      const double aa = 1.0 + 0.5 * gm1 * alpha * alpha;
+      const double bb = -gm1 * alpha * riemann;
+      const double cc = 0.5 * gm1 * riemann * riemann - 2.0 * d[1];
+      double dd = bb * bb - 4.0 * aa * cc; dd = sqrt(dd > 0.0 ? dd : 0.0);
+      double qR = (-bb + dd) / (2.0 * aa); qR = qR > 0.0 ? qR : 0.0;
+      double qR2 = qR * qR;
+      double aR2 = gm1 * (d[1] - 0.5 * qR2);
+      double MR2 = qR2 / aR2; MR2 = MR2 < 1.0 ? MR2 : 1.0;
+      const double tmp = 1.0 / (1.0 + 0.5 * gm1 * MR2);
+      aR2 = gm1 * d[1] * tmp;
+      qR2 = MR2 * aR2;
+      qR = sqrt(qR2);
+      const double pR = d[0] * pow(tmp, govgm1);
+      const double rhoR = GAM * pR / aR2;
+      const double pRInv = 1.0 / pR;
+      const double sR = log(pR / pow(rhoR, GAM));
+      R[0] = (GAM - sR) * ovgm1 - 0.5 * pRInv * rhoR * qR2;
+      R[1] = rhoR * pRInv * qR * d[2]; R[2] = rhoR * pRInv * qR * d[3]; R[3] = rhoR * pRInv * qR * d[4];
+      R[4] = -rhoR * pRInv;
+    } else {  /* subsonic outflow */
+      const double V4Inv = 1.0 / V[4];
+      const double u = -V4Inv * V[1], v = -V4Inv * V[2], w = -V4Inv * V[3];
+      const double eKin = 0.5 * (u * u + v * v + w * w);
+      const double s = GAM - gm1 * (V[0] - V[4] * eKin);
+      const double a = sqrt(-GAM * V4Inv);
+      const double nx = fn * n0, ny = fn * n1, nz = fn * n2_;
+      const double un = u * nx + v * ny + w * nz;
+      const double riemann = 2.0 * a * ovgm1 + un;
+      const double pR = d[0];
+      const double rhoR = pow(pR / exp(s), ovg);
+      const double aR = sqrt(GAM * pR / rhoR);
+      const double unR = riemann - 2.0 * aR * ovgm1;
+      const double uR = u + (unR - un) * nx, vR = v + (unR - un) * ny, wR = w + (unR - un) * nz;
+      const double pRInv = 1.0 / pR;
+      R[0] = (GAM - s) * ovgm1 - 0.5 * pRInv * rhoR * (uR * uR + vR * vR + wR * wR);
+      R[1] = rhoR * pRInv * uR; R[2] = rhoR * pRInv * vR; R[3] = rhoR * pRInv * wR;
+      R[4] = -rhoR * pRInv;
+    }
+    for (int q = 0; q < 5; ++q) SR[q * n2 + l] = R[q];
+  }
+}
+
+static int is_wall(int bc)
+{
+  return bc == ORC_BC_WALL_HEATFLUX || bc == ORC_BC_WALL_ISOTHERMAL || bc == ORC_BC_WALL_INVISCID;
+}
+
+/* Physical face f of element e: FluxesBoundaryFace (FluxesFace.cpp:16-214)
+   and the lift of the boundary branch of ElementClass::{I,J,K}FaceResiduals
+   (ElementClass.cpp:2751-2850). An NSCBC face averages the interior and
+   reconstructed gradients; a pointwise boundary condition uses the interior
+   gradient alone (passed as both sides, FluxesFace.cpp:60-80, :117-147). */
 static int boundary_face(orc_case *c, int f, int e, double *buf)
 {
-  const int n2 = c->nInt2D, nD = c->nDOFs;
-  if (!is_nscbc(c->prm.bc[f])) return fail("unsupported physical boundary");
+  const int n2 = c->nInt2D, nD = c->nDOFs, bc = c->prm.bc[f];
   double *SL = buf, *G0 = SL + 5 * n2, *G1 = G0 + 5 * n2, *G2 = G1 + 5 * n2;
   double *SR = G2 + 5 * n2, *H0 = SR + 5 * n2, *H1 = H0 + 5 * n2, *H2 = H1 + 5 * n2;
   double *flux = H2 + 5 * n2, *sxL = flux + 5 * n2, *syL = sxL + 5 * n2, *szL = syL + 5 * n2;
   double *sxR = szL + 5 * n2, *syR = sxR + 5 * n2, *szR = syR + 5 * n2;
   face_trace(c, f, c->sol + (size_t) e * NVAR * nD, SL, G0, G1, G2);
-  double *GL[3] = {G0, G1, G2}, *GR[3] = {H0, H1, H2};
-  nscbc_state(c, f, e, SL, GL, SR, GR);
   const double *m = c->sm[f] + (size_t) e * 16 * n2;
   const double fn = (f % 2) ? 1.0 : -1.0;
-  internal_face_flux(c, fn, c->len[(size_t) e * 4 + f / 2], SL, G0, G1, G2, m, SR, H0, H1, H2, m,
-                     flux, sxL, syL, szL, sxR, syR, szR);
+  const double lenScale = c->len[(size_t) e * 4 + f / 2];
+  if (is_nscbc(bc)) {
+    double *GL[3] = {G0, G1, G2}, *GR[3] = {H0, H1, H2};
+    nscbc_state(c, f, e, SL, GL, SR, GR);
+    internal_face_flux(c, fn, lenScale, SL, G0, G1, G2, m, SR, H0, H1, H2, m, 1.0, 0, 0.0,
+                       flux, sxL, syL, szL, sxR, syR, szR);
+  } else {
+    pointwise_state(c, bc, m, fn, c->bcData[f], SL, SR);
+    internal_face_flux(c, fn, lenScale, SL, G0, G1, G2, m, SR, G0, G1, G2, m, is_wall(bc) ? 0.0 : 1.0,
+                       bc == ORC_BC_WALL_HEATFLUX, c->bcData[f][0], flux, sxL, syL, szL, sxR, syR, szR);
+  }
   face_lift(c, f, flux, sxL, syL, szL, c->res + (size_t) e * NVAR * nD);
   return 0;
 }
@@ -1229,7 +1328,8 @@ static int face_residual(orc_case *c, int dir, int e)
   const double lenScale = fmin(c->len[(size_t) e * 4 + dir], c->len[(size_t) L * 4 + dir]);
   const double *mL = c->sm[2 * dir + 1] + (size_t) L * 16 * n2;
   const double *mR = c->sm[2 * dir] + (size_t) e * 16 * n2;
-  internal_face_flux(c, 1.0, lenScale, SL, GrL, GsL, GtL, mL, SR, GrR, GsR, GtR, mR, flux, sxL, syL, szL, sxR, syR, szR);
+  internal_face_flux(c, 1.0, lenScale, SL, GrL, GsL, GtL, mL, SR, GrR, GsR, GtR, mR, 1.0, 0, 0.0,
+                     flux, sxL, syL, szL, sxR, syR, szR);
   double *rmin = c->resMin[dir] + (size_t) e * NVAR * nD;
   for (int i = 0; i < NVAR * nD; ++i) rmin[i] = 0.0;
   face_lift(c, 2 * dir + 1, flux, sxL, syL, szL, rmin);
@@ -1318,13 +1418,15 @@ static int is_nscbc(int bc)
   return bc == ORC_BC_OUTFLOW_NSCBC || bc == ORC_BC_INFLOW_STATIC_NSCBC || bc == ORC_BC_INFLOW_TOTAL_NSCBC;
 }
 
+static int is_physical(int bc) { return bc >= ORC_BC_OUTFLOW_NSCBC && bc <= ORC_BC_WALL_INVISCID; }
+
 static int check_params(const orc_case *c)
 {
   const orc_params *p = &c->prm;
   for (int f = 0; f < 6; ++f) {
     const int per = p->periodic[f / 2];
     if (per && p->bc[f] != ORC_BC_PERIODIC) return fail("a periodic direction cannot carry a physical boundary");
-    if (!per && !is_nscbc(p->bc[f])) return fail("unsupported boundary condition on a non-periodic block face");
+    if (!per && !is_physical(p->bc[f])) return fail("unsupported boundary condition on a non-periodic block face");
   }
   if (p->outlet_average_local)
     return fail("\"Outlet use local (element) averaged Mach number\" is not supported");
@@ -1341,25 +1443,34 @@ static int check_params(const orc_case *c)
   return 0;
 }
 
-/* InputParamClass::ComputeReferenceConditions (InputParamClass.cpp:605-758)
-   and the non-dimensionalisation of the prescribed data
-   (SubfaceClasses.cpp:1304-1313, :1385-1400, :1453-1475). */
+/* InputParamClass::ComputeReferenceConditions (InputParamClass.cpp:605-758):
+   pressure, total pressure, density, temperature and total temperature are
+   taken from the prescribed data of the physical faces, a later face
+   overriding an earlier one; and the non-dimensionalisation of the
+   prescribed data (ConvertPrescribedData, SubfaceClasses.cpp:739-751,
+   :863-915, :1169-1180, :1304-1313, :1385-1400, :1453-1475; an isothermal
+   wall keeps its dimensional temperature). */
 static void reference_conditions(orc_case *c)
 {
   const orc_params *q = &c->prm;
   const double muDim = 1.716e-5;
-  double p = -1.0, pTot = -1.0, rho = -1.0, TTot = -1.0;
+  double p = -1.0, pTot = -1.0, rho = -1.0, T = -1.0, TTot = -1.0;
   for (int f = 0; f < 6; ++f) {
     if (q->periodic[f / 2]) continue;
-    if (q->bc[f] == ORC_BC_OUTFLOW_NSCBC) p = q->bc_data[f][0];
-    if (q->bc[f] == ORC_BC_INFLOW_STATIC_NSCBC) rho = q->bc_data[f][0];
-    if (q->bc[f] == ORC_BC_INFLOW_TOTAL_NSCBC) { pTot = q->bc_data[f][0]; TTot = q->bc_data[f][1]; }
+    const int bc = q->bc[f];
+    if (bc == ORC_BC_OUTFLOW_NSCBC || bc == ORC_BC_OUTFLOW_SUBSONIC) p = q->bc_data[f][0];
+    if (bc == ORC_BC_INFLOW_STATIC_NSCBC) rho = q->bc_data[f][0];
+    if (bc == ORC_BC_INFLOW_TOTAL_NSCBC || bc == ORC_BC_INFLOW_SUBSONIC) { pTot = q->bc_data[f][0]; TTot = q->bc_data[f][1]; }
+    if (bc == ORC_BC_WALL_ISOTHERMAL) T = q->bc_data[f][0];
   }
   c->pRef = -1.0;
   if (p > 0.0) c->pRef = p; else if (pTot > 0.0) c->pRef = pTot;
   c->rhoRef = -1.0;
   if (rho > 0.0) c->rhoRef = rho;
-  if (c->rhoRef < 0.0 && c->pRef > 0.0 && TTot > 0.0) c->rhoRef = c->pRef / (RGAS * TTot);
+  if (c->rhoRef < 0.0 && c->pRef > 0.0) {
+    const double TRef = T > 0.0 ? T : TTot;
+    if (TRef > 0.0) c->rhoRef = c->pRef / (RGAS * TRef);
+  }
   if (c->pRef < 0.0 || c->rhoRef < 0.0) {
     const double vInf = q->mach * sqrt(GAM * RGAS * q->temp_free_stream);
     c->rhoRef = q->reynolds * muDim / (q->reynolds_length * vInf);
@@ -1368,18 +1479,25 @@ static void reference_conditions(orc_case *c)
   c->uRef = sqrt(c->pRef / c->rhoRef);
   c->mu = muDim * c->uRef / c->pRef;
   const double TRef = c->pRef / c->rhoRef;
+  const double cp = GAM * RGAS / (GAM - 1.0);
   for (int f = 0; f < 6; ++f) {
     const double *d = q->bc_data[f];
     double *o = c->bcData[f];
     for (int l = 0; l < 5; ++l) o[l] = 0.0;
     if (q->periodic[f / 2]) continue;
-    if (q->bc[f] == ORC_BC_OUTFLOW_NSCBC) o[0] = d[0] / c->pRef;
-    else if (q->bc[f] == ORC_BC_INFLOW_STATIC_NSCBC) {
+    const int bc = q->bc[f];
+    if (bc == ORC_BC_OUTFLOW_NSCBC || bc == ORC_BC_OUTFLOW_SUBSONIC) o[0] = d[0] / c->pRef;
+    else if (bc == ORC_BC_INFLOW_STATIC_NSCBC) {
       o[0] = d[0] / c->rhoRef; o[1] = d[1] / c->uRef; o[2] = d[2] / c->uRef; o[3] = d[3] / c->uRef;
-    } else if (q->bc[f] == ORC_BC_INFLOW_TOTAL_NSCBC) {
+    } else if (bc == ORC_BC_INFLOW_TOTAL_NSCBC) {
       const double inv = 1.0 / sqrt(d[2] * d[2] + d[3] * d[3] + d[4] * d[4]);
       o[0] = d[0] / c->pRef; o[1] = d[1] / TRef; o[2] = d[2] * inv; o[3] = d[3] * inv; o[4] = d[4] * inv;
-    }
+    } else if (bc == ORC_BC_INFLOW_SUBSONIC) {
+      const double inv = 1.0 / sqrt(d[2] * d[2] + d[3] * d[3] + d[4] * d[4]);
+      o[0] = d[0] / c->pRef; o[1] = d[1] * (cp / (c->uRef * c->uRef));
+      o[2] = d[2] * inv; o[3] = d[3] * inv; o[4] = d[4] * inv;
+    } else if (bc == ORC_BC_WALL_HEATFLUX) o[0] = d[0] / (c->pRef * c->uRef);
+    else if (bc == ORC_BC_WALL_ISOTHERMAL) o[0] = d[0];
   }
 }
 
@@ -1394,7 +1512,7 @@ static int setup_nscbc(orc_case *c)
   c->nscbc = 0;
   for (int f = 0; f < 6; ++f) {
     c->faceArea[f] = 0.0; c->machAvg[f] = 0.0;
-    if (c->prm.periodic[f / 2]) continue;
+    if (c->prm.periodic[f / 2] || !is_nscbc(c->prm.bc[f])) continue;
     c->nscbc = 1;
     int first = -1;
     for (int e = 0; e < c->nElem; ++e) {
@@ -1794,7 +1912,7 @@ static void nscbc_average(orc_case *c)
   const int n2 = c->nInt2D, nD = c->nDOFs;
   for (int f = 0; f < 6; ++f) {
     double sum = 0.0;
-    if (!c->prm.periodic[f / 2]) {
+    if (!c->prm.periodic[f / 2] && is_nscbc(c->prm.bc[f])) {
 #pragma omp parallel reduction(+ : sum)
       {
         double *S = dalloc(5 * (size_t) n2), *G = dalloc(15 * (size_t) n2);

@@ -7,9 +7,10 @@
  * never links, loads or calls it.
  *
  * Scope: entropy working variables, Ismail-Roe flux, no SGS model, no wall
- * model, structured single block whose faces are periodic (translation) or
- * PC-NSCBC in/outflow, with optional sponge layer and characteristic
- * matching layer.
+ * model, structured single block whose faces are periodic (translation),
+ * PC-NSCBC in/outflow, subsonic (total-condition) inflow, subsonic outflow,
+ * or heat-flux / isothermal / inviscid walls, with optional sponge layer and
+ * characteristic matching layer.
  * Each function cites the reference file:line it restates; paths are
  * relative to VCP3D_CurviLinear/.
  */
@@ -39,8 +40,10 @@ typedef struct {
      iMin, iMax, jMin, jMax, kMin, kMax (one subface per block face). */
   int    bc[6];            /* ORC_BC_* */
   double bc_data[6][5];    /* dimensional prescribed data of the face:
-                              outflow {p}; static inflow {rho, u, v, w};
-                              total inflow {p_t, T_t, dir_x, dir_y, dir_z} */
+                              NSCBC or subsonic outflow {p}; static NSCBC
+                              inflow {rho, u, v, w}; total NSCBC or subsonic
+                              inflow {p_t, T_t, dir_x, dir_y, dir_z};
+                              heat-flux wall {q}; isothermal wall {T} */
   /* "NSCBC tuning parameters" (InputParamClass.cpp:406-424) */
   double outlet_len, outlet_sigma, outlet_beta_l, outlet_beta_t;
   int    outlet_one_minus_beta, outlet_average_local;
@@ -54,7 +57,10 @@ typedef struct {
 } orc_params;
 
 enum { ORC_BC_PERIODIC = 0, ORC_BC_OUTFLOW_NSCBC = 1,
-       ORC_BC_INFLOW_STATIC_NSCBC = 2, ORC_BC_INFLOW_TOTAL_NSCBC = 3 };
+       ORC_BC_INFLOW_STATIC_NSCBC = 2, ORC_BC_INFLOW_TOTAL_NSCBC = 3,
+       ORC_BC_INFLOW_SUBSONIC = 4,      /* total conditions */
+       ORC_BC_OUTFLOW_SUBSONIC = 5, ORC_BC_WALL_HEATFLUX = 6,
+       ORC_BC_WALL_ISOTHERMAL = 7, ORC_BC_WALL_INVISCID = 8 };
 
 typedef struct orc_case orc_case;
 

@@ -1,7 +1,9 @@
 """Case decks (vcp3d_open_amd/deck.py): parsing the reference's parameter
 files, building the same case description and initial state as the case
 builders, and the oracle run from the reference's own test decks
-(tests/golden/ref_n3_8x4x4/, copied from VCP3D_CurviLinear/test/n3_8x4x4)
+(tests/golden/ref_n3_8x4x4/ and ref_duct/, copied from
+VCP3D_CurviLinear/test/n3_8x4x4 and n5_15x8x8/duct) and from wall variants of
+the first (param_walls_{a,b}.cfg, written by make_golden.derived_decks)
 against the reference's outputs on them."""
 import os
 
@@ -104,7 +106,7 @@ def test_restart_round_trip(tmp_path):
 
 @pytest.mark.parametrize("edit,msg", [
     (("FEM working variables: Entropy", "FEM working variables: Conservative"), "fem working variables"),
-    (("BCOutflowNSCBC  ", "BCOutflowSubsonic"), "not supported"),
+    (("BCOutflowNSCBC  ", "BCOutflowSupersonic"), "not supported"),
     (("BCOutflowNSCBC  ", "BCOutflowCharacteristic"), "unknown boundary condition"),
     (("Subgrid scale model: None", "Subgrid scale model: WALE"), "Subgrid"),
     ((" Fluctuations to add to initial solution: Pressure pulse",
@@ -131,7 +133,18 @@ def test_keyword_lines_follow_the_reference_rules(tmp_path):
     assert prm.number("cfl number") == 2.5 and not prm.has("empty value")
 
 
-@pytest.mark.parametrize("name", ["deck_n3_8x4x4_static", "deck_n3_8x4x4_total"])
+def test_wall_decks_are_parsed():
+    a = DeckCase(os.path.join(GOLDEN, "ref_n3_8x4x4", "param_walls_a.cfg"))
+    assert a.bc == [4, 5, 7, 7, 0, 0] and a.periodic == [0, 0, 1]
+    assert a.bc_data[2][0] == 300.0 and a.bc_data[3][0] == 330.0 and a.bc_data[1][0] == 101325.0
+    b = DeckCase(os.path.join(GOLDEN, "ref_n3_8x4x4", "param_walls_b.cfg"))
+    assert b.bc == [2, 1, 8, 6, 0, 0] and b.bc_data[3][0] == 2500.0
+    d = DeckCase(os.path.join(GOLDEN, "ref_duct", "extrapolation", "start.cfg"))
+    assert d.n == (15, 8, 8) and d.p == 5 and d.bc == [4, 1, 6, 6, 6, 6] and d.periodic == [0, 0, 0]
+    assert d.pulse["direction"] == -1.0 and d.pulse["strength"] == 0.0
+
+
+@pytest.mark.parametrize("name", ["deck_n3_8x4x4_static", "deck_n3_8x4x4_total", "deck_walls_a", "deck_walls_b"])
 def test_oracle_on_reference_decks(oracle_lib, name):
     """The oracle set up from the reference's own test deck (free stream from
     the boundary data plus the pressure pulse) against the reference run on
@@ -146,3 +159,21 @@ def test_oracle_on_reference_decks(oracle_lib, name):
         o.sync_step()
     assert rel(o.nodal(), g["sol_3"]) < 1e-12
     assert rel(o.residual_array(), g["res_3"]) < 1e-9
+
+
+def test_oracle_on_reference_duct_deck(oracle_lib):
+    """The reference's duct start deck (15 x 8 x 8, P = 5, subsonic
+    total-condition inflow, NSCBC outflow, four adiabatic walls): the state
+    after 3 steps on every 13th element and the global norms of the five
+    conservative variables."""
+    case, g = load("deck_duct_start")
+    o = oracle_lib.Oracle(case)
+    o.set_restart(case.initial_conservative())
+    np.testing.assert_allclose(list(o.reference().values()), g["reference"], rtol=1e-14)
+    o.residual()
+    for _ in range(3):
+        o.sync_step()
+    sol = o.nodal()
+    assert rel(sol[g["elements"]], g["sol_3"][..., :5]) < 1e-12
+    # per-variable norms as one vector (the transverse momenta are ~1e-6 of it)
+    assert rel(np.sqrt((sol ** 2).sum(axis=(0, 1))), g["sol_3_norms"][:5]) < 1e-12

@@ -1,7 +1,8 @@
 """The GPU path set up from case decks (vcp3d_open_amd/deck.py), through the
-C ABI: the reference's own test decks (tests/golden/ref_n3_8x4x4/, from
-VCP3D_CurviLinear/test/n3_8x4x4) against the reference's outputs on them,
-and a checkpoint written by write_restart resumed from a deck.
+C ABI: the reference's own test decks (tests/golden/ref_n3_8x4x4/ and
+ref_duct/, from VCP3D_CurviLinear/test/n3_8x4x4 and n5_15x8x8/duct) and the
+wall variants of the first against the reference's outputs on them, and a
+checkpoint written by write_restart resumed from a deck.
 
 Tolerances as for the other parity tests: residual (R = M^-1 res) within
 relative L2 1e-9, fields within 1e-12.
@@ -28,7 +29,9 @@ def lib():
 
 
 @pytest.mark.parametrize("name,deck", [("deck_n3_8x4x4_static", "param_static.cfg"),
-                                       ("deck_n3_8x4x4_total", "param_total.cfg")])
+                                       ("deck_n3_8x4x4_total", "param_total.cfg"),
+                                       ("deck_walls_a", "param_walls_a.cfg"),
+                                       ("deck_walls_b", "param_walls_b.cfg")])
 def test_reference_deck_matches_reference(lib, name, deck):
     print(f"[case] {name}", flush=True)
     _, gold = load(name)
@@ -42,6 +45,22 @@ def test_reference_deck_matches_reference(lib, name, deck):
     assert abs(md[6] - gold["monitoring"][6]) <= 1e-9 * gold["monitoring"][6]
     assert rel(s.WriteRestartSolution(), gold["sol_3"]) < 1e-12
     assert rel(s.residual_array(), gold["res_3"]) < 1e-9
+
+
+def test_reference_duct_deck(lib):
+    """The reference's duct start deck: 15 x 8 x 8, P = 5, subsonic
+    total-condition inflow, NSCBC outflow, four adiabatic walls; the state
+    after 3 steps on every 13th element and the global conservative norms."""
+    print("[case] deck_duct_start", flush=True)
+    _, gold = load("deck_duct_start")
+    s, case = open_solver(os.path.join(GOLDEN, "ref_duct", "extrapolation", "start.cfg"))
+    s.Residual(True)
+    for _ in range(3):
+        s.ExecuteTimeSynchronizationStep()
+    sol = s.WriteRestartSolution()
+    assert rel(sol[gold["elements"]], gold["sol_3"][..., :5]) < 1e-12
+    # per-variable norms as one vector (the transverse momenta are ~1e-6 of it)
+    assert rel(np.sqrt((sol ** 2).sum(axis=(0, 1))), gold["sol_3_norms"][:5]) < 1e-12
 
 
 def test_checkpoint_and_resume(lib, tmp_path):

@@ -35,7 +35,8 @@
 namespace vcp {
 namespace {
 
-enum : int { kBCOutflowNSCBC = 1, kBCInflowStaticNSCBC = 2, kBCInflowTotalNSCBC = 3 };
+enum : int { kBCOutflowNSCBC = 1, kBCInflowStaticNSCBC = 2, kBCInflowTotalNSCBC = 3, kBCInflowSubsonic = 4,
+              kBCOutflowSubsonic = 5, kBCWallHeatFlux = 6, kBCWallIsothermal = 7, kBCWallInviscid = 8 };
 constexpr double kGam = 1.4;
 constexpr double kGm1 = 0.4;
 constexpr double kPrLam = 0.72;
@@ -141,13 +142,18 @@ __device__ inline void ent2prim(const double V[5], double &rho, double &u, doubl
 // point: VL/VR entropy states (L -> R along nn), gx/gy/gz the averaged
 // Cartesian gradients, wA = weight * area factor. fl: normal flux, FX/FY/FZ:
 // Cartesian symmetrising fluxes (FluxesFace.cpp:813-975, :1290-1651,
-// :1152-1284).
+// :1152-1284). BND: the boundary form, where factU (wallPerm, 0 on walls)
+// multiplies the convective and dissipation parts of the inviscid flux and a
+// prescribed wall heat flux (hf) zeroes the laminar heat-flux factor and adds
+// wA * qWall to the energy flux (FluxesFace.cpp:1312-1318, :1586-1593).
+template <bool BND = false>
 __device__ __forceinline__ void point_flux(const double VL[5], const double VR[5], const double gx[5],
                                            const double gy[5], const double gz[5], const double nn[3],
                                            double wA, double tau, double mu, double thetaHalf,
-                                           double fl[5], double FX[5], double FY[5], double FZ[5])
+                                           double fl[5], double FX[5], double FY[5], double FZ[5],
+                                           double factU = 1.0, bool hf = false, double qWall = 0.0)
 {
-  const double factHF = kGam / kPrLam;
+  const double factHF = (BND && hf) ? 0.0 : kGam / kPrLam;
   const double lam = kLamOverMu, lamp1 = lam + 1.0, lamp2 = lam + 2.0;
   {
     double rL, uL, vL, wL, pL, eL, rR, uR, vR, wR, pR, eR;
@@ -191,11 +197,19 @@ __device__ __forceinline__ void point_flux(const double VL[5], const double VR[5
     const double abv5 = nn[0] * dru + nn[1] * drv + nn[2] * drw - vn * dr;
     const double abv6 = abv3 * abv4 * (oa * oa) + abv2 * abv5 * oa;
     const double abv7 = abv2 * abv4 * oa + abv3 * abv5;
-    fl[0] = wA * (rA * vn - (lam3 * dr + abv6));
-    fl[1] = wA * (rA * vn * uA + p1 * nn[0] - (lam3 * dru + uA * abv6 + nn[0] * abv7));
-    fl[2] = wA * (rA * vn * vA + p1 * nn[1] - (lam3 * drv + vA * abv6 + nn[1] * abv7));
-    fl[3] = wA * (rA * vn * wAv + p1 * nn[2] - (lam3 * drw + wAv * abv6 + nn[2] * abv7));
-    fl[4] = wA * (rA * vn * H - (lam3 * drE + H * abv6 + vn * abv7));
+    if constexpr (BND) {
+      fl[0] = wA * (factU * rA * vn - factU * (lam3 * dr + abv6));
+      fl[1] = wA * (factU * rA * vn * uA + p1 * nn[0] - factU * (lam3 * dru + uA * abv6 + nn[0] * abv7));
+      fl[2] = wA * (factU * rA * vn * vA + p1 * nn[1] - factU * (lam3 * drv + vA * abv6 + nn[1] * abv7));
+      fl[3] = wA * (factU * rA * vn * wAv + p1 * nn[2] - factU * (lam3 * drw + wAv * abv6 + nn[2] * abv7));
+      fl[4] = wA * (factU * rA * vn * H - factU * (lam3 * drE + H * abv6 + vn * abv7));
+    } else {
+      fl[0] = wA * (rA * vn - (lam3 * dr + abv6));
+      fl[1] = wA * (rA * vn * uA + p1 * nn[0] - (lam3 * dru + uA * abv6 + nn[0] * abv7));
+      fl[2] = wA * (rA * vn * vA + p1 * nn[1] - (lam3 * drv + vA * abv6 + nn[1] * abv7));
+      fl[3] = wA * (rA * vn * wAv + p1 * nn[2] - (lam3 * drw + wAv * abv6 + nn[2] * abv7));
+      fl[4] = wA * (rA * vn * H - (lam3 * drE + H * abv6 + vn * abv7));
+    }
   }
   const double V4Inv = 2.0 / (VL[4] + VR[4]);
   const double u = -V4Inv * 0.5 * (VL[1] + VR[1]), vv = -V4Inv * 0.5 * (VL[2] + VR[2]),
@@ -217,6 +231,7 @@ __device__ __forceinline__ void point_flux(const double VL[5], const double VR[5
     fl[3] -= txz * nx + tyz * ny + tzz * nz;
     fl[4] -= (u * txx + vv * txy + w * txz - qx) * nx + (u * txy + vv * tyy + w * tyz - qy) * ny
            + (u * txz + vv * tyz + w * tzz - qz) * nz;
+    if (BND && hf) fl[4] += wA * qWall;
   }
   const double pOvRho = -V4Inv, eInt = pOvRho * kInvGm1;
   const double dV1 = VL[1] - VR[1], dV2 = VL[2] - VR[2], dV3 = VL[3] - VR[3], dV4 = VL[4] - VR[4];
@@ -1296,6 +1311,10 @@ __global__ __launch_bounds__(64) void mach_kernel(BndArgs<K, M> a)
   const int b = blockIdx.x;
   const int e = a.bface[2 * b], f = a.bface[2 * b + 1];
   const int dir = f / 2, side = f % 2;
+  if (a.nb.bc[f] > kBCInflowTotalNSCBC) {   // not a characteristic face: no part in the average
+    if (threadIdx.x == 0) a.machPart[b] = 0.0;
+    return;
+  }
   bnd_load_ops(sm, a, side);
   bnd_trace(sm, a.U + (size_t) e * 5 * ND, dir, side, false);
   const double *g = a.bgeo + (size_t) b * 13 * N2;
@@ -1314,10 +1333,12 @@ __global__ __launch_bounds__(64) void mach_kernel(BndArgs<K, M> a)
   if (threadIdx.x == 0) a.machPart[b] = sum;
 }
 
-// One wave per PC-NSCBC boundary face: characteristic boundary state
-// (BoundaryState.cpp:996-2241), its polynomial-correction reconstruction,
-// the boundary flux (FluxesBoundaryFace, FluxesFace.cpp:16-214) and its lift
-// into the element's face slot.
+// One wave per physical boundary face: on a PC-NSCBC face the characteristic
+// boundary state (BoundaryState.cpp:996-2241) and its polynomial-correction
+// reconstruction, on the other faces (subsonic in/outflow, walls) a pointwise
+// boundary state; then the boundary flux (FluxesBoundaryFace,
+// FluxesFace.cpp:16-214) and its lift into the element's face slot. The
+// branch is uniform over the workgroup (one face each).
 template <int K, int M>
 __global__ __launch_bounds__(64) void nscbc_kernel(BndArgs<K, M> a)
 {
@@ -1333,227 +1354,301 @@ __global__ __launch_bounds__(64) void nscbc_kernel(BndArgs<K, M> a)
   const double *g = a.bgeo + (size_t) b * 13 * N2;
   // parametric slot (0 r, 1 s, 2 t) of the normal, fast and slow face axes
   const int pN = dir, pF = dir == 0 ? 1 : 0, pS = dir == 2 ? 1 : 2;
-  // boundary-averaged Mach number (fixed summation order)
-  double mach = 0.0;
-  for (int i = a.nb.begin[f] + tid; i < a.nb.begin[f + 1]; i += 64) mach += a.machPart[i];
-  mach = wave_sum(mach) * a.nb.invArea[f];
-  const double dell = fn * a.nb.dell0;
-  const int iPhi = fn > 0.0 ? 0 : 4, iPsi = iPhi == 0 ? 4 : 0;
+  const bool nscbc = bc <= kBCInflowTotalNSCBC;
+  if (nscbc) {
+    // boundary-averaged Mach number (fixed summation order)
+    double mach = 0.0;
+    for (int i = a.nb.begin[f] + tid; i < a.nb.begin[f + 1]; i += 64) mach += a.machPart[i];
+    mach = wave_sum(mach) * a.nb.invArea[f];
+    const double dell = fn * a.nb.dell0;
+    const int iPhi = fn > 0.0 ? 0 : 4, iPsi = iPhi == 0 ? 4 : 0;
 
-  for (int m = tid; m < N2; m += 64) {
-    // entropy -> primitive state and derivatives (BoundaryState.cpp:1002-1108)
-    double V[5], Q[5], dQ[3][5];
-#pragma unroll
-    for (int v = 0; v < 5; ++v) V[v] = sm.S[0][v * N2 + m];
-    const double V4Inv = 1.0 / V[4];
-    const double u = -V4Inv * V[1], v = -V4Inv * V[2], w = -V4Inv * V[3];
-    const double ek = 0.5 * (u * u + v * v + w * w);
-    const double sEnt = kGam - kGm1 * (V[0] - V[4] * ek);
-    const double a2 = -kGam * V4Inv;
-    const double rho = pow_m2p5(-V[4] * exp(sEnt));
-    const double p = a2 * rho / kGam;
-    const double povrho = -V4Inv;
-    const double E = povrho / kGm1 + ek, H = E + povrho;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const double *dV = sm.S[1 + c];
-      const double d0 = dV[m], d1 = dV[N2 + m], d2 = dV[2 * N2 + m], d3 = dV[3 * N2 + m], d4 = dV[4 * N2 + m];
-      dQ[c][0] = rho * d0 + rho * u * d1 + rho * v * d2 + rho * w * d3 + rho * E * d4;
-      dQ[c][1] = povrho * d1 + povrho * u * d4;
-      dQ[c][2] = povrho * d2 + povrho * v * d4;
-      dQ[c][3] = povrho * d3 + povrho * w * d4;
-      dQ[c][4] = p * d0 + p * u * d1 + p * v * d2 + p * w * d3 + p * H * d4;
-    }
-    Q[0] = rho; Q[1] = u; Q[2] = v; Q[3] = w; Q[4] = p;
-    // normal wave amplitudes with the incoming ones replaced
-    const double rr[3] = {g[(4 + 3 * pN) * N2 + m], g[(5 + 3 * pN) * N2 + m], g[(6 + 3 * pN) * N2 + m]};
-    const double magrr = sqrt(rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2]);
-    const double ovmagrr = 1.0 / magrr;
-    const double rn[3] = {rr[0] * ovmagrr, rr[1] * ovmagrr, rr[2] * ovmagrr};
-    const double drhodr = dQ[0][0], dudr = dQ[0][1], dvdr = dQ[0][2], dwdr = dQ[0][3], dpdr = dQ[0][4];
-    const double ovrho = 1.0 / rho, c2 = kGam * p * ovrho, c = sqrt(c2);
-    const double ova2 = 1.0 / c2, rhoa = rho * c, ovrhoa = 1.0 / rhoa, rhova = rho / c;
-    const double dsdr = drhodr - ova2 * dpdr;
-    const double dw1dr = rn[1] * dwdr - rn[2] * dvdr;
-    const double dw2dr = rn[2] * dudr - rn[0] * dwdr;
-    const double dw3dr = rn[0] * dvdr - rn[1] * dudr;
-    const double durdr = rn[0] * dudr + rn[1] * dvdr + rn[2] * dwdr;
-    double LL[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    double lmb1, lmb3;
-    if (bc == kBCOutflowNSCBC) {
-      // BoundaryState.cpp:1318-1519
-      const double sign = iPhi == 0 ? -1.0 : 1.0;
-      const double kk = 0.5 * a.nb.outlet_sigma * (1.0 - mach * mach) / a.nb.outlet_len;
-      const double bl = a.nb.outlet_beta_l < 0.0 ? mach : a.nb.outlet_beta_l;
-      const double bt = a.nb.outlet_beta_t < 0.0 ? mach : a.nb.outlet_beta_t;
-      const double cbl = a.nb.outlet_one_minus_beta ? fmin(1.0, 1.0 - bl) : fmin(1.0, bl);
-      const double cbt = a.nb.outlet_one_minus_beta ? fmin(1.0, 1.0 - bt) : fmin(1.0, bt);
-      const double ss[3] = {g[(4 + 3 * pF) * N2 + m], g[(5 + 3 * pF) * N2 + m], g[(6 + 3 * pF) * N2 + m]};
-      const double tt[3] = {g[(4 + 3 * pS) * N2 + m], g[(5 + 3 * pS) * N2 + m], g[(6 + 3 * pS) * N2 + m]};
-      const double duds = dQ[1][1], dudt = dQ[2][1], dvds = dQ[1][2], dvdt = dQ[2][2];
-      const double dwds = dQ[1][3], dwdt = dQ[2][3], dpds = dQ[1][4], dpdt = dQ[2][4];
-      const double ur = u * rr[0] + v * rr[1] + w * rr[2];
-      const double us = u * ss[0] + v * ss[1] + w * ss[2];
-      const double ut = u * tt[0] + v * tt[1] + w * tt[2];
-      const double arr = c * magrr;
-      const double ars = c * (rn[0] * ss[0] + rn[1] * ss[1] + rn[2] * ss[2]);
-      const double art = c * (rn[0] * tt[0] + rn[1] * tt[1] + rn[2] * tt[2]);
-      lmb1 = ur - arr; lmb3 = ur + arr;
-      const double durds = rn[0] * duds + rn[1] * dvds + rn[2] * dwds;
-      const double durdt = rn[0] * dudt + rn[1] * dvdt + rn[2] * dwdt;
-      LL[0] = 0.5 * lmb1 * (ovrhoa * dpdr - durdr);
-      LL[1] = rn[0] * dsdr - dw1dr;
-      LL[2] = rn[1] * dsdr - dw2dr;
-      LL[3] = rn[2] * dsdr - dw3dr;
-      LL[4] = 0.5 * lmb3 * (ovrhoa * dpdr + durdr);
-      const double LT = 0.5 * ovrhoa * (us + sign * ars) * dpds + sign * 0.5 * us * durds
-                      + 0.5 * ovrhoa * (ut + sign * art) * dpdt + sign * 0.5 * ut * durdt;
-      const double TT = -0.5 * c * (ss[0] * duds + ss[1] * dvds + ss[2] * dwds
-                                  + tt[0] * dudt + tt[1] * dvdt + tt[2] * dwdt);
-      LL[iPhi] = kk * ovrho * magrr * (p - a.nb.data[f][0]) - cbl * LT + cbt * TT;
-    } else if (bc == kBCInflowStaticNSCBC) {
-      // BoundaryState.cpp:1596-1783
-      const double sign = fn > 0.0 ? -1.0 : 1.0;
-      const double ka = sign * 0.5 * a.nb.inlet_sigma * (1.0 - mach * mach) / a.nb.inlet_len;
-      const double ku = a.nb.inlet_sigma / a.nb.inlet_len;
-      const double ur = u * rr[0] + v * rr[1] + w * rr[2];
-      const double arr = c * magrr;
-      lmb1 = ur - arr; lmb3 = ur + arr;
-      if (a.nb.inlet_incoming) {
-        LL[0] = 0.5 * lmb1 * (ovrhoa * dpdr - durdr);
-        LL[4] = 0.5 * lmb3 * (ovrhoa * dpdr + durdr);
+    for (int m = tid; m < N2; m += 64) {
+      // entropy -> primitive state and derivatives (BoundaryState.cpp:1002-1108)
+      double V[5], Q[5], dQ[3][5];
+  #pragma unroll
+      for (int v = 0; v < 5; ++v) V[v] = sm.S[0][v * N2 + m];
+      const double V4Inv = 1.0 / V[4];
+      const double u = -V4Inv * V[1], v = -V4Inv * V[2], w = -V4Inv * V[3];
+      const double ek = 0.5 * (u * u + v * v + w * w);
+      const double sEnt = kGam - kGm1 * (V[0] - V[4] * ek);
+      const double a2 = -kGam * V4Inv;
+      const double rho = pow_m2p5(-V[4] * exp(sEnt));
+      const double p = a2 * rho / kGam;
+      const double povrho = -V4Inv;
+      const double E = povrho / kGm1 + ek, H = E + povrho;
+  #pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double *dV = sm.S[1 + c];
+        const double d0 = dV[m], d1 = dV[N2 + m], d2 = dV[2 * N2 + m], d3 = dV[3 * N2 + m], d4 = dV[4 * N2 + m];
+        dQ[c][0] = rho * d0 + rho * u * d1 + rho * v * d2 + rho * w * d3 + rho * E * d4;
+        dQ[c][1] = povrho * d1 + povrho * u * d4;
+        dQ[c][2] = povrho * d2 + povrho * v * d4;
+        dQ[c][3] = povrho * d3 + povrho * w * d4;
+        dQ[c][4] = p * d0 + p * u * d1 + p * v * d2 + p * w * d3 + p * H * d4;
       }
-      const double drho = rho - a.nb.data[f][0], du = u - a.nb.data[f][1], dv = v - a.nb.data[f][2],
-                   dw = w - a.nb.data[f][3];
-      const double kaol = ku * c * (1.0 / ur);
-      LL[iPhi] = ka * c * (rr[0] * du + rr[1] * dv + rr[2] * dw);
-      LL[1] = kaol * (rr[2] * dv - rr[1] * dw + rr[0] * drho);
-      LL[2] = kaol * (rr[0] * dw - rr[2] * du + rr[1] * drho);
-      LL[3] = kaol * (rr[1] * du - rr[0] * dv + rr[2] * drho);
-    } else {
-      // total-condition inflow, BoundaryState.cpp:1866-2184
-      const double ovrg = 1.0 / 287.058, ovcp = kGm1 / (kGam * 287.058);
-      const double kk = -a.nb.inlet_sigma / a.nb.inlet_len;
-      const double ova = 1.0 / c, T = p * ovrho * ovrg;
-      const double ur = u * rn[0] + v * rn[1] + w * rn[2];
-      const double w1 = rn[1] * w - rn[2] * v, w2 = rn[2] * u - rn[0] * w, w3 = rn[0] * v - rn[1] * u;
-      lmb1 = ur - c; lmb3 = ur + c;
-      const double ovlmb2 = 1.0 / ur;
-      if (a.nb.inlet_incoming) {
+      Q[0] = rho; Q[1] = u; Q[2] = v; Q[3] = w; Q[4] = p;
+      // normal wave amplitudes with the incoming ones replaced
+      const double rr[3] = {g[(4 + 3 * pN) * N2 + m], g[(5 + 3 * pN) * N2 + m], g[(6 + 3 * pN) * N2 + m]};
+      const double magrr = sqrt(rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2]);
+      const double ovmagrr = 1.0 / magrr;
+      const double rn[3] = {rr[0] * ovmagrr, rr[1] * ovmagrr, rr[2] * ovmagrr};
+      const double drhodr = dQ[0][0], dudr = dQ[0][1], dvdr = dQ[0][2], dwdr = dQ[0][3], dpdr = dQ[0][4];
+      const double ovrho = 1.0 / rho, c2 = kGam * p * ovrho, c = sqrt(c2);
+      const double ova2 = 1.0 / c2, rhoa = rho * c, ovrhoa = 1.0 / rhoa, rhova = rho / c;
+      const double dsdr = drhodr - ova2 * dpdr;
+      const double dw1dr = rn[1] * dwdr - rn[2] * dvdr;
+      const double dw2dr = rn[2] * dudr - rn[0] * dwdr;
+      const double dw3dr = rn[0] * dvdr - rn[1] * dudr;
+      const double durdr = rn[0] * dudr + rn[1] * dvdr + rn[2] * dwdr;
+      double LL[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      double lmb1, lmb3;
+      if (bc == kBCOutflowNSCBC) {
+        // BoundaryState.cpp:1318-1519
+        const double sign = iPhi == 0 ? -1.0 : 1.0;
+        const double kk = 0.5 * a.nb.outlet_sigma * (1.0 - mach * mach) / a.nb.outlet_len;
+        const double bl = a.nb.outlet_beta_l < 0.0 ? mach : a.nb.outlet_beta_l;
+        const double bt = a.nb.outlet_beta_t < 0.0 ? mach : a.nb.outlet_beta_t;
+        const double cbl = a.nb.outlet_one_minus_beta ? fmin(1.0, 1.0 - bl) : fmin(1.0, bl);
+        const double cbt = a.nb.outlet_one_minus_beta ? fmin(1.0, 1.0 - bt) : fmin(1.0, bt);
+        const double ss[3] = {g[(4 + 3 * pF) * N2 + m], g[(5 + 3 * pF) * N2 + m], g[(6 + 3 * pF) * N2 + m]};
+        const double tt[3] = {g[(4 + 3 * pS) * N2 + m], g[(5 + 3 * pS) * N2 + m], g[(6 + 3 * pS) * N2 + m]};
+        const double duds = dQ[1][1], dudt = dQ[2][1], dvds = dQ[1][2], dvdt = dQ[2][2];
+        const double dwds = dQ[1][3], dwdt = dQ[2][3], dpds = dQ[1][4], dpdt = dQ[2][4];
+        const double ur = u * rr[0] + v * rr[1] + w * rr[2];
+        const double us = u * ss[0] + v * ss[1] + w * ss[2];
+        const double ut = u * tt[0] + v * tt[1] + w * tt[2];
+        const double arr = c * magrr;
+        const double ars = c * (rn[0] * ss[0] + rn[1] * ss[1] + rn[2] * ss[2]);
+        const double art = c * (rn[0] * tt[0] + rn[1] * tt[1] + rn[2] * tt[2]);
+        lmb1 = ur - arr; lmb3 = ur + arr;
+        const double durds = rn[0] * duds + rn[1] * dvds + rn[2] * dwds;
+        const double durdt = rn[0] * dudt + rn[1] * dvdt + rn[2] * dwdt;
         LL[0] = 0.5 * lmb1 * (ovrhoa * dpdr - durdr);
+        LL[1] = rn[0] * dsdr - dw1dr;
+        LL[2] = rn[1] * dsdr - dw2dr;
+        LL[3] = rn[2] * dsdr - dw3dr;
         LL[4] = 0.5 * lmb3 * (ovrhoa * dpdr + durdr);
+        const double LT = 0.5 * ovrhoa * (us + sign * ars) * dpds + sign * 0.5 * us * durds
+                        + 0.5 * ovrhoa * (ut + sign * art) * dpdt + sign * 0.5 * ut * durdt;
+        const double TT = -0.5 * c * (ss[0] * duds + ss[1] * dvds + ss[2] * dwds
+                                    + tt[0] * dudt + tt[1] * dvdt + tt[2] * dwdt);
+        LL[iPhi] = kk * ovrho * magrr * (p - a.nb.data[f][0]) - cbl * LT + cbt * TT;
+      } else if (bc == kBCInflowStaticNSCBC) {
+        // BoundaryState.cpp:1596-1783
+        const double sign = fn > 0.0 ? -1.0 : 1.0;
+        const double ka = sign * 0.5 * a.nb.inlet_sigma * (1.0 - mach * mach) / a.nb.inlet_len;
+        const double ku = a.nb.inlet_sigma / a.nb.inlet_len;
+        const double ur = u * rr[0] + v * rr[1] + w * rr[2];
+        const double arr = c * magrr;
+        lmb1 = ur - arr; lmb3 = ur + arr;
+        if (a.nb.inlet_incoming) {
+          LL[0] = 0.5 * lmb1 * (ovrhoa * dpdr - durdr);
+          LL[4] = 0.5 * lmb3 * (ovrhoa * dpdr + durdr);
+        }
+        const double drho = rho - a.nb.data[f][0], du = u - a.nb.data[f][1], dv = v - a.nb.data[f][2],
+                     dw = w - a.nb.data[f][3];
+        const double kaol = ku * c * (1.0 / ur);
+        LL[iPhi] = ka * c * (rr[0] * du + rr[1] * dv + rr[2] * dw);
+        LL[1] = kaol * (rr[2] * dv - rr[1] * dw + rr[0] * drho);
+        LL[2] = kaol * (rr[0] * dw - rr[2] * du + rr[1] * drho);
+        LL[3] = kaol * (rr[1] * du - rr[0] * dv + rr[2] * drho);
+      } else {
+        // total-condition inflow, BoundaryState.cpp:1866-2184
+        const double ovrg = 1.0 / 287.058, ovcp = kGm1 / (kGam * 287.058);
+        const double kk = -a.nb.inlet_sigma / a.nb.inlet_len;
+        const double ova = 1.0 / c, T = p * ovrho * ovrg;
+        const double ur = u * rn[0] + v * rn[1] + w * rn[2];
+        const double w1 = rn[1] * w - rn[2] * v, w2 = rn[2] * u - rn[0] * w, w3 = rn[0] * v - rn[1] * u;
+        lmb1 = ur - c; lmb3 = ur + c;
+        const double ovlmb2 = 1.0 / ur;
+        if (a.nb.inlet_incoming) {
+          LL[0] = 0.5 * lmb1 * (ovrhoa * dpdr - durdr);
+          LL[4] = 0.5 * lmb3 * (ovrhoa * dpdr + durdr);
+        }
+        const double magU2 = 2.0 * ek, magU = sqrt(magU2);
+        const double ff = 1.0 + 0.5 * kGm1 * (magU2 / c2);
+        const double Pt = p * pow(ff, kGam / kGm1), Tt = T * ff;
+        const double Zt = v / magU, Zp = w / magU;
+        const double ka = kk * c;
+        const double X1 = ka * (Pt - a.nb.data[f][0]), X2 = ka * (Tt - a.nb.data[f][1]);
+        const double X3 = ka * (Zt - a.nb.data[f][3]), X4 = ka * (Zp - a.nb.data[f][4]);
+        const double ovrPtovTt = ovrg * (Pt / Tt), Ttovrho = Tt * ovrho;
+        const double govaPt = kGam * ova * Pt, ekovrho = ek * ovrho, gm1ekova = kGm1 * ek * ova;
+        const double gm1ovaTt = kGm1 * ova * Tt, ovmagU2 = 1.0 / magU2;
+        const double gp = gm1ekova + ur, gmn = gm1ekova - ur;
+        const double f1 = w1 - rn[0] * ekovrho, f2 = w2 - rn[1] * ekovrho, f3 = w3 - rn[2] * ekovrho;
+        const double A1 = gp * ovrPtovTt - govaPt, A2 = f1 * ovrPtovTt, A3 = f2 * ovrPtovTt, A4 = f3 * ovrPtovTt;
+        const double A5 = gmn * ovrPtovTt - govaPt;
+        const double B1 = gp * ovcp - gm1ovaTt, B2 = f1 * ovcp + rn[0] * Ttovrho, B3 = f2 * ovcp + rn[1] * Ttovrho;
+        const double B4 = f3 * ovcp + rn[2] * Ttovrho, B5 = gmn * ovcp - gm1ovaTt;
+        const double C1 = (rn[1] * magU - ur * Zt) * ovmagU2, C2 = -(rn[2] * magU + w1 * Zt) * ovmagU2;
+        const double C3 = (-w2 * Zt) * ovmagU2, C4 = (rn[0] * magU - w3 * Zt) * ovmagU2, C5 = -C1;
+        const double D1 = (rn[2] * magU - ur * Zp) * ovmagU2, D2 = (rn[1] * magU - w1 * Zp) * ovmagU2;
+        const double D3 = -(rn[0] * magU + w2 * Zp) * ovmagU2, D4 = (-w3 * Zp) * ovmagU2, D5 = -D1;
+        const double Y1 = LL[iPsi] * A1, Y2 = LL[iPsi] * B1, Y3 = LL[iPsi] * C1, Y4 = LL[iPsi] * D1;
+        // elimination of the 4 x 4 system for the incoming amplitudes
+        const double ovA2 = 1.0 / A2;
+        const double E0 = ovA2 * X1, E3 = -ovA2 * A3, E4 = -ovA2 * A4, E5 = -ovA2 * A5, P2 = -ovA2 * Y1;
+        const double ovdF = 1.0 / (D3 + D2 * E3);
+        const double F0 = ovdF * (X4 - D2 * E0), F4 = -ovdF * (D4 + D2 * E4), F5 = -ovdF * (D5 + D2 * E5);
+        const double P3 = -ovdF * (Y4 + D2 * P2);
+        const double ovdG = 1.0 / (C4 + C2 * E4 + C3 * F4 + C2 * E3 * F4);
+        const double G0 = ovdG * (X3 - C2 * E0 - C3 * F0 - C2 * E3 * F0);
+        const double G5 = -ovdG * (C5 + C2 * E5 + C3 * F5 + C2 * E3 * F5);
+        const double P4 = -ovdG * (Y3 + C2 * P2 + C3 * P3 + C2 * E3 * P3);
+        const double ovdH = 1.0 / (B5 + B2 * E5 + B3 * F5 + B4 * G5 + B2 * E3 * F5 + B2 * E4 * G5 + B3 * F4 * G5
+                                   + B2 * E3 * F4 * G5);
+        const double H0 = ovdH * (X2 - B2 * E0 - B3 * F0 - B4 * G0 - B2 * E3 * F0 - B2 * E4 * G0 - B3 * F4 * G0
+                                  - B2 * E3 * F4 * G0);
+        const double P5 = -ovdH * (Y2 + B2 * P2 + B3 * P3 + B4 * P4 + B2 * E3 * P3 + B2 * E4 * P4 + B3 * F4 * P4
+                                   + B2 * E3 * F4 * P4);
+        LL[iPhi] = H0 + P5;
+        LL[3] = G0 + G5 * LL[iPhi] + P4;
+        LL[2] = F0 + F4 * LL[3] + F5 * LL[iPhi] + P3;
+        LL[1] = E0 + E3 * LL[2] + E4 * LL[3] + E5 * LL[iPhi] + P2;
+        LL[1] *= ovlmb2; LL[2] *= ovlmb2; LL[3] *= ovlmb2;
       }
-      const double magU2 = 2.0 * ek, magU = sqrt(magU2);
-      const double ff = 1.0 + 0.5 * kGm1 * (magU2 / c2);
-      const double Pt = p * pow(ff, kGam / kGm1), Tt = T * ff;
-      const double Zt = v / magU, Zp = w / magU;
-      const double ka = kk * c;
-      const double X1 = ka * (Pt - a.nb.data[f][0]), X2 = ka * (Tt - a.nb.data[f][1]);
-      const double X3 = ka * (Zt - a.nb.data[f][3]), X4 = ka * (Zp - a.nb.data[f][4]);
-      const double ovrPtovTt = ovrg * (Pt / Tt), Ttovrho = Tt * ovrho;
-      const double govaPt = kGam * ova * Pt, ekovrho = ek * ovrho, gm1ekova = kGm1 * ek * ova;
-      const double gm1ovaTt = kGm1 * ova * Tt, ovmagU2 = 1.0 / magU2;
-      const double gp = gm1ekova + ur, gmn = gm1ekova - ur;
-      const double f1 = w1 - rn[0] * ekovrho, f2 = w2 - rn[1] * ekovrho, f3 = w3 - rn[2] * ekovrho;
-      const double A1 = gp * ovrPtovTt - govaPt, A2 = f1 * ovrPtovTt, A3 = f2 * ovrPtovTt, A4 = f3 * ovrPtovTt;
-      const double A5 = gmn * ovrPtovTt - govaPt;
-      const double B1 = gp * ovcp - gm1ovaTt, B2 = f1 * ovcp + rn[0] * Ttovrho, B3 = f2 * ovcp + rn[1] * Ttovrho;
-      const double B4 = f3 * ovcp + rn[2] * Ttovrho, B5 = gmn * ovcp - gm1ovaTt;
-      const double C1 = (rn[1] * magU - ur * Zt) * ovmagU2, C2 = -(rn[2] * magU + w1 * Zt) * ovmagU2;
-      const double C3 = (-w2 * Zt) * ovmagU2, C4 = (rn[0] * magU - w3 * Zt) * ovmagU2, C5 = -C1;
-      const double D1 = (rn[2] * magU - ur * Zp) * ovmagU2, D2 = (rn[1] * magU - w1 * Zp) * ovmagU2;
-      const double D3 = -(rn[0] * magU + w2 * Zp) * ovmagU2, D4 = (-w3 * Zp) * ovmagU2, D5 = -D1;
-      const double Y1 = LL[iPsi] * A1, Y2 = LL[iPsi] * B1, Y3 = LL[iPsi] * C1, Y4 = LL[iPsi] * D1;
-      // elimination of the 4 x 4 system for the incoming amplitudes
-      const double ovA2 = 1.0 / A2;
-      const double E0 = ovA2 * X1, E3 = -ovA2 * A3, E4 = -ovA2 * A4, E5 = -ovA2 * A5, P2 = -ovA2 * Y1;
-      const double ovdF = 1.0 / (D3 + D2 * E3);
-      const double F0 = ovdF * (X4 - D2 * E0), F4 = -ovdF * (D4 + D2 * E4), F5 = -ovdF * (D5 + D2 * E5);
-      const double P3 = -ovdF * (Y4 + D2 * P2);
-      const double ovdG = 1.0 / (C4 + C2 * E4 + C3 * F4 + C2 * E3 * F4);
-      const double G0 = ovdG * (X3 - C2 * E0 - C3 * F0 - C2 * E3 * F0);
-      const double G5 = -ovdG * (C5 + C2 * E5 + C3 * F5 + C2 * E3 * F5);
-      const double P4 = -ovdG * (Y3 + C2 * P2 + C3 * P3 + C2 * E3 * P3);
-      const double ovdH = 1.0 / (B5 + B2 * E5 + B3 * F5 + B4 * G5 + B2 * E3 * F5 + B2 * E4 * G5 + B3 * F4 * G5
-                                 + B2 * E3 * F4 * G5);
-      const double H0 = ovdH * (X2 - B2 * E0 - B3 * F0 - B4 * G0 - B2 * E3 * F0 - B2 * E4 * G0 - B3 * F4 * G0
-                                - B2 * E3 * F4 * G0);
-      const double P5 = -ovdH * (Y2 + B2 * P2 + B3 * P3 + B4 * P4 + B2 * E3 * P3 + B2 * E4 * P4 + B3 * F4 * P4
-                                 + B2 * E3 * F4 * P4);
-      LL[iPhi] = H0 + P5;
-      LL[3] = G0 + G5 * LL[iPhi] + P4;
-      LL[2] = F0 + F4 * LL[3] + F5 * LL[iPhi] + P3;
-      LL[1] = E0 + E3 * LL[2] + E4 * LL[3] + E5 * LL[iPhi] + P2;
-      LL[1] *= ovlmb2; LL[2] *= ovlmb2; LL[3] *= ovlmb2;
+      // target normal derivative of the primitive state and the B matrix
+      const double ovlmb1 = 1.0 / lmb1, ovlmb3 = 1.0 / lmb3;
+      double t[5];
+      t[0] = ovlmb1 * rhova * LL[0] + rn[0] * LL[1] + rn[1] * LL[2] + rn[2] * LL[3] + ovlmb3 * rhova * LL[4];
+      t[1] = -ovlmb1 * rn[0] * LL[0] - rn[2] * LL[2] + rn[1] * LL[3] + ovlmb3 * rn[0] * LL[4];
+      t[2] = -ovlmb1 * rn[1] * LL[0] + rn[2] * LL[1] - rn[0] * LL[3] + ovlmb3 * rn[1] * LL[4];
+      t[3] = -ovlmb1 * rn[2] * LL[0] - rn[1] * LL[1] + rn[0] * LL[2] + ovlmb3 * rn[2] * LL[4];
+      t[4] = ovlmb1 * rhoa * LL[0] + ovlmb3 * rhoa * LL[4];
+  #pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        sm.Q[1][q * N2 + m] = t[q];
+        sm.X[0][q * N2 + m] = t[q] - (dQ[0][q] - Q[q] * dell);
+      }
     }
-    // target normal derivative of the primitive state and the B matrix
-    const double ovlmb1 = 1.0 / lmb1, ovlmb3 = 1.0 / lmb3;
-    double t[5];
-    t[0] = ovlmb1 * rhova * LL[0] + rn[0] * LL[1] + rn[1] * LL[2] + rn[2] * LL[3] + ovlmb3 * rhova * LL[4];
-    t[1] = -ovlmb1 * rn[0] * LL[0] - rn[2] * LL[2] + rn[1] * LL[3] + ovlmb3 * rn[0] * LL[4];
-    t[2] = -ovlmb1 * rn[1] * LL[0] + rn[2] * LL[1] - rn[0] * LL[3] + ovlmb3 * rn[1] * LL[4];
-    t[3] = -ovlmb1 * rn[2] * LL[0] - rn[1] * LL[1] + rn[0] * LL[2] + ovlmb3 * rn[2] * LL[4];
-    t[4] = ovlmb1 * rhoa * LL[0] + ovlmb3 * rhoa * LL[4];
-#pragma unroll
-    for (int q = 0; q < 5; ++q) {
-      sm.Q[1][q * N2 + m] = t[q];
-      sm.X[0][q * N2 + m] = t[q] - (dQ[0][q] - Q[q] * dell);
+    __syncthreads();
+    // polynomial correction Q_b = (P1 x P1) B / dell (sign of the face):
+    // fast axis, then slow axis
+    for (int idx = tid; idx < 5 * N2; idx += 64) {
+      const int row = idx / M, j = idx - row * M;
+      double s = 0.0;
+  #pragma unroll
+      for (int q = 0; q < M; ++q) s += sm.P1[j * M + q] * sm.X[0][row * M + q];
+      sm.X[1][idx] = s;
     }
-  }
-  __syncthreads();
-  // polynomial correction Q_b = (P1 x P1) B / dell (sign of the face):
-  // fast axis, then slow axis
-  for (int idx = tid; idx < 5 * N2; idx += 64) {
-    const int row = idx / M, j = idx - row * M;
-    double s = 0.0;
+    __syncthreads();
+    const double scale = 1.0 / dell;
+    for (int idx = tid; idx < 5 * N2; idx += 64) {
+      const int v = idx / N2, m = idx - v * N2, i = m / M, j = m - i * M;
+      double s = 0.0;
+  #pragma unroll
+      for (int q = 0; q < M; ++q) s += sm.P1[i * M + q] * sm.X[1][(v * M + q) * M + j];
+      sm.Q[0][idx] = s * scale;
+    }
+    __syncthreads();
+    // primitive -> entropy variables for the state and its normal derivative
+    for (int m = tid; m < N2; m += 64) {
+      const double rho = sm.Q[0][m], u = sm.Q[0][N2 + m], v = sm.Q[0][2 * N2 + m], w = sm.Q[0][3 * N2 + m],
+                   p = sm.Q[0][4 * N2 + m];
+      const double dr = sm.Q[1][m], du = sm.Q[1][N2 + m], dv = sm.Q[1][2 * N2 + m], dw = sm.Q[1][3 * N2 + m],
+                   dp = sm.Q[1][4 * N2 + m];
+      const double sEnt = log(p / pow(rho, kGam));
+      const double ovrho = 1.0 / rho, ovp = 1.0 / p, rovp = rho * ovp;
+      const double ek = 0.5 * (u * u + v * v + w * w);
+      const double W1 = rovp * u, W2 = rovp * v, W3 = rovp * w;
+      sm.Q[0][m] = -rovp * ek + (kGam - sEnt) / kGm1;
+      sm.Q[0][N2 + m] = W1; sm.Q[0][2 * N2 + m] = W2; sm.Q[0][3 * N2 + m] = W3; sm.Q[0][4 * N2 + m] = -rovp;
+      const double q0 = kGam / kGm1 * ovrho - ovp * ek, q4 = ovp * (rovp * ek - 1.0 / kGm1);
+      sm.Q[1][m] = q0 * dr - W1 * du - W2 * dv - W3 * dw + q4 * dp;
+      sm.Q[1][N2 + m] = u * ovp * dr + rovp * du - ovp * W1 * dp;
+      sm.Q[1][2 * N2 + m] = v * ovp * dr + rovp * dv - ovp * W2 * dp;
+      sm.Q[1][3 * N2 + m] = w * ovp * dr + rovp * dw - ovp * W3 * dp;
+      sm.Q[1][4 * N2 + m] = -ovp * dr + ovp * rovp * dp;
+    }
+    __syncthreads();
+    // transverse derivatives of the reconstructed state on the Gauss points;
+    // the reference stores the slow-axis derivative in the fast-axis slot
+    // (BoundaryState.cpp:1261-1287)
+    double *dSlow = a.bug ? sm.Q[2] : sm.Q[3];
+    double *dFast = a.bug ? sm.Q[3] : sm.Q[2];
+    for (int idx = tid; idx < 5 * N2; idx += 64) {
+      const int v = idx / N2, m = idx - v * N2, i = m / M, j = m - i * M;
+      const double *q0 = sm.Q[0] + v * N2;
+      double ds = 0.0, dt = 0.0;
+  #pragma unroll
+      for (int k = 0; k < M; ++k) { ds += sm.dL[i * M + k] * q0[k * M + j]; dt += sm.dL[j * M + k] * q0[i * M + k]; }
+      dSlow[idx] = ds; dFast[idx] = dt;
+    }
+    __syncthreads();
+  } else {
+    // pointwise boundary state (BoundaryState.cpp:40-118, :156-372,
+    // :399-604, :636-792, entropy variables); d: the face's non-dimensional
+    // prescribed data
+    const double *d = a.nb.data[f];
+    for (int m = tid; m < N2; m += 64) {
+      double V[5], R[5];
 #pragma unroll
-    for (int q = 0; q < M; ++q) s += sm.P1[j * M + q] * sm.X[0][row * M + q];
-    sm.X[1][idx] = s;
-  }
-  __syncthreads();
-  const double scale = 1.0 / dell;
-  for (int idx = tid; idx < 5 * N2; idx += 64) {
-    const int v = idx / N2, m = idx - v * N2, i = m / M, j = m - i * M;
-    double s = 0.0;
+      for (int v = 0; v < 5; ++v) V[v] = sm.S[0][v * N2 + m];
+      const double n0 = g[m], n1 = g[N2 + m], n2 = g[2 * N2 + m];
+      if (bc == kBCWallInviscid) {
+        const double Vn = V[1] * n0 + V[2] * n1 + V[3] * n2;
+        R[0] = V[0] - 0.5 * Vn * Vn / V[4];
+        R[1] = V[1] - Vn * n0; R[2] = V[2] - Vn * n1; R[3] = V[3] - Vn * n2;
+        R[4] = V[4];
+      } else if (bc == kBCWallHeatFlux || bc == kBCWallIsothermal) {
+        R[0] = V[0] - 0.5 * (V[1] * V[1] + V[2] * V[2] + V[3] * V[3]) / V[4];
+        R[1] = 0.0; R[2] = 0.0; R[3] = 0.0;
+        R[4] = bc == kBCWallHeatFlux ? V[4] : d[1] / d[0];   // d[1] = -pRef / (R rhoRef)
+      } else if (bc == kBCInflowSubsonic) {
+        const double V4Inv = 1.0 / V[4];
+        const double u = -V4Inv * V[1], v = -V4Inv * V[2], w = -V4Inv * V[3];
+        const double a2 = -kGam * V4Inv, c = sqrt(a2);
+        const double nx = fn * n0, ny = fn * n1, nz = fn * n2;
+        const double un = u * nx + v * ny + w * nz;
+        double riemann = 2.0 * c * kInvGm1 + un;
+        const double H = a2 * kInvGm1 + 0.5 * (u * u + v * v + w * w);
+        riemann *= sqrt(d[1] / H);
+        const double alpha = nx * d[2] + ny * d[3] + nz * d[4];
+        const double aa = 1.0 + 0.5 * kGm1 * alpha * alpha;
+        const double bb = -kGm1 * alpha * riemann;
+        const double cc = 0.5 * kGm1 * riemann * riemann - 2.0 * d[1];
+        double dd = bb * bb - 4.0 * aa * cc; dd = sqrt(fmax(0.0, dd));
+        double qR = (-bb + dd) / (2.0 * aa); qR = fmax(0.0, qR);
+        double qR2 = qR * qR;
+        double aR2 = kGm1 * (d[1] - 0.5 * qR2);
+        double MR2 = qR2 / aR2; MR2 = fmin(1.0, MR2);
+        const double tmp = 1.0 / (1.0 + 0.5 * kGm1 * MR2);
+        aR2 = kGm1 * d[1] * tmp;
+        qR2 = MR2 * aR2;
+        qR = sqrt(qR2);
+        const double pR = d[0] * pow(tmp, kGam * kInvGm1);
+        const double rhoR = kGam * pR / aR2;
+        const double pRInv = 1.0 / pR;
+        const double sR = log(pR / pow(rhoR, kGam));
+        R[0] = (kGam - sR) * kInvGm1 - 0.5 * pRInv * rhoR * qR2;
+        R[1] = rhoR * pRInv * qR * d[2]; R[2] = rhoR * pRInv * qR * d[3]; R[3] = rhoR * pRInv * qR * d[4];
+        R[4] = -rhoR * pRInv;
+      } else {   // subsonic outflow
+        const double V4Inv = 1.0 / V[4];
+        const double u = -V4Inv * V[1], v = -V4Inv * V[2], w = -V4Inv * V[3];
+        const double eKin = 0.5 * (u * u + v * v + w * w);
+        const double sEnt = kGam - kGm1 * (V[0] - V[4] * eKin);
+        const double c = sqrt(-kGam * V4Inv);
+        const double nx = fn * n0, ny = fn * n1, nz = fn * n2;
+        const double un = u * nx + v * ny + w * nz;
+        const double riemann = 2.0 * c * kInvGm1 + un;
+        const double pR = d[0];
+        const double rhoR = pow(pR / exp(sEnt), 1.0 / kGam);
+        const double aR = sqrt(kGam * pR / rhoR);
+        const double unR = riemann - 2.0 * aR * kInvGm1;
+        const double uR = u + (unR - un) * nx, vR = v + (unR - un) * ny, wR = w + (unR - un) * nz;
+        const double pRInv = 1.0 / pR;
+        R[0] = (kGam - sEnt) * kInvGm1 - 0.5 * pRInv * rhoR * (uR * uR + vR * vR + wR * wR);
+        R[1] = rhoR * pRInv * uR; R[2] = rhoR * pRInv * vR; R[3] = rhoR * pRInv * wR;
+        R[4] = -rhoR * pRInv;
+      }
 #pragma unroll
-    for (int q = 0; q < M; ++q) s += sm.P1[i * M + q] * sm.X[1][(v * M + q) * M + j];
-    sm.Q[0][idx] = s * scale;
+      for (int v = 0; v < 5; ++v) sm.Q[0][v * N2 + m] = R[v];
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  // primitive -> entropy variables for the state and its normal derivative
-  for (int m = tid; m < N2; m += 64) {
-    const double rho = sm.Q[0][m], u = sm.Q[0][N2 + m], v = sm.Q[0][2 * N2 + m], w = sm.Q[0][3 * N2 + m],
-                 p = sm.Q[0][4 * N2 + m];
-    const double dr = sm.Q[1][m], du = sm.Q[1][N2 + m], dv = sm.Q[1][2 * N2 + m], dw = sm.Q[1][3 * N2 + m],
-                 dp = sm.Q[1][4 * N2 + m];
-    const double sEnt = log(p / pow(rho, kGam));
-    const double ovrho = 1.0 / rho, ovp = 1.0 / p, rovp = rho * ovp;
-    const double ek = 0.5 * (u * u + v * v + w * w);
-    const double W1 = rovp * u, W2 = rovp * v, W3 = rovp * w;
-    sm.Q[0][m] = -rovp * ek + (kGam - sEnt) / kGm1;
-    sm.Q[0][N2 + m] = W1; sm.Q[0][2 * N2 + m] = W2; sm.Q[0][3 * N2 + m] = W3; sm.Q[0][4 * N2 + m] = -rovp;
-    const double q0 = kGam / kGm1 * ovrho - ovp * ek, q4 = ovp * (rovp * ek - 1.0 / kGm1);
-    sm.Q[1][m] = q0 * dr - W1 * du - W2 * dv - W3 * dw + q4 * dp;
-    sm.Q[1][N2 + m] = u * ovp * dr + rovp * du - ovp * W1 * dp;
-    sm.Q[1][2 * N2 + m] = v * ovp * dr + rovp * dv - ovp * W2 * dp;
-    sm.Q[1][3 * N2 + m] = w * ovp * dr + rovp * dw - ovp * W3 * dp;
-    sm.Q[1][4 * N2 + m] = -ovp * dr + ovp * rovp * dp;
-  }
-  __syncthreads();
-  // transverse derivatives of the reconstructed state on the Gauss points;
-  // the reference stores the slow-axis derivative in the fast-axis slot
-  // (BoundaryState.cpp:1261-1287)
-  double *dSlow = a.bug ? sm.Q[2] : sm.Q[3];
-  double *dFast = a.bug ? sm.Q[3] : sm.Q[2];
-  for (int idx = tid; idx < 5 * N2; idx += 64) {
-    const int v = idx / N2, m = idx - v * N2, i = m / M, j = m - i * M;
-    const double *q0 = sm.Q[0] + v * N2;
-    double ds = 0.0, dt = 0.0;
-#pragma unroll
-    for (int k = 0; k < M; ++k) { ds += sm.dL[i * M + k] * q0[k * M + j]; dt += sm.dL[j * M + k] * q0[i * M + k]; }
-    dSlow[idx] = ds; dFast[idx] = dt;
-  }
-  __syncthreads();
   // fluxes with the face normal oriented out of the element
   const double lenScale = a.econst[(size_t) e * kElemConst + dir];
   const double tau = a.tauFact / lenScale;
@@ -1569,16 +1664,27 @@ __global__ __launch_bounds__(64) void nscbc_kernel(BndArgs<K, M> a)
       for (int q = 0; q < 9; ++q) c[q] = g[(4 + q) * N2 + pt];
       double pl[3], pr[3];
       pl[pN] = sm.S[1][v * N2 + m]; pl[pF] = sm.S[2][v * N2 + m]; pl[pS] = sm.S[3][v * N2 + m];
-      pr[pN] = sm.Q[1][v * N2 + m]; pr[pF] = sm.Q[2][v * N2 + m]; pr[pS] = sm.Q[3][v * N2 + m];
-      const double xL = pl[0] * c[0] + pl[1] * c[3] + pl[2] * c[6], xR = pr[0] * c[0] + pr[1] * c[3] + pr[2] * c[6];
-      const double yL = pl[0] * c[1] + pl[1] * c[4] + pl[2] * c[7], yR = pr[0] * c[1] + pr[1] * c[4] + pr[2] * c[7];
-      const double zL = pl[0] * c[2] + pl[1] * c[5] + pl[2] * c[8], zR = pr[0] * c[2] + pr[1] * c[5] + pr[2] * c[8];
-      gx[v] = 0.5 * (xL + xR); gy[v] = 0.5 * (yL + yR); gz[v] = 0.5 * (zL + zR);
+      const double xL = pl[0] * c[0] + pl[1] * c[3] + pl[2] * c[6];
+      const double yL = pl[0] * c[1] + pl[1] * c[4] + pl[2] * c[7];
+      const double zL = pl[0] * c[2] + pl[1] * c[5] + pl[2] * c[8];
+      if (nscbc) {
+        // the interior and the reconstructed gradients averaged
+        pr[pN] = sm.Q[1][v * N2 + m]; pr[pF] = sm.Q[2][v * N2 + m]; pr[pS] = sm.Q[3][v * N2 + m];
+        const double xR = pr[0] * c[0] + pr[1] * c[3] + pr[2] * c[6];
+        const double yR = pr[0] * c[1] + pr[1] * c[4] + pr[2] * c[7];
+        const double zR = pr[0] * c[2] + pr[1] * c[5] + pr[2] * c[8];
+        gx[v] = 0.5 * (xL + xR); gy[v] = 0.5 * (yL + yR); gz[v] = 0.5 * (zL + zR);
+      } else {
+        // the interior gradient (FluxesFace.cpp:60-80)
+        gx[v] = xL; gy[v] = yL; gz[v] = zL;
+      }
     }
     const double nn[3] = {fn * g[m], fn * g[N2 + m], fn * g[2 * N2 + m]};
     const double wA = sm.W2[m] * g[3 * N2 + m];
     double fl[5], FX[5], FY[5], FZ[5];
-    point_flux(VL, VR, gx, gy, gz, nn, wA, tau, a.mu, a.thetaHalf, fl, FX, FY, FZ);
+    const bool wall = bc >= kBCWallHeatFlux;
+    point_flux<true>(VL, VR, gx, gy, gz, nn, wA, tau, a.mu, a.thetaHalf, fl, FX, FY, FZ, wall ? 0.0 : 1.0,
+                     bc == kBCWallHeatFlux, a.nb.data[f][0]);
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
       const int pt = a.bug ? (v < N2 ? v : 0) : m;

@@ -314,7 +314,7 @@ std::string check_case(const CaseParams &p)
   for (int f = 0; f < 6; ++f) {
     const bool per = p.periodic[f / 2] != 0;
     if (per && p.bc[f] != kBCPeriodic) return "a periodic direction cannot carry a physical boundary";
-    if (!per && (p.bc[f] < kBCOutflowNSCBC || p.bc[f] > kBCInflowTotalNSCBC))
+    if (!per && (p.bc[f] < kBCOutflowNSCBC || p.bc[f] > kBCWallInviscid))
       return "unsupported boundary condition on a non-periodic block face";
   }
   if (p.outlet_average_local)
@@ -450,18 +450,29 @@ std::string build_geometry(const CaseParams &p, Tables &t, const double *grid, G
 // Pressure from an outlet, density from a static inlet (or the total
 // temperature of a total-condition inlet) fix the reference state; a block
 // without such data uses the free stream (Mach, Reynolds, temperature).
+// InputParamClass::ComputeReferenceConditions (InputParamClass.cpp:605-758):
+// pressure, total pressure, density, temperature and total temperature from
+// the prescribed data of the physical faces (a later face overriding an
+// earlier one), else from the free-stream Mach number, Reynolds number and
+// temperature.
 RefConditions reference_conditions(const CaseParams &p)
 {
   RefConditions r{};
-  double pOut = -1.0, pTot = -1.0, rhoIn = -1.0, TTot = -1.0;
+  double pOut = -1.0, pTot = -1.0, rhoIn = -1.0, T = -1.0, TTot = -1.0;
   for (int f = 0; f < 6; ++f) {
     if (p.periodic[f / 2]) continue;
-    if (p.bc[f] == kBCOutflowNSCBC) pOut = p.bc_data[f][0];
-    if (p.bc[f] == kBCInflowStaticNSCBC) rhoIn = p.bc_data[f][0];
-    if (p.bc[f] == kBCInflowTotalNSCBC) { pTot = p.bc_data[f][0]; TTot = p.bc_data[f][1]; }
+    const int bc = p.bc[f];
+    if (bc == kBCOutflowNSCBC || bc == kBCOutflowSubsonic) pOut = p.bc_data[f][0];
+    if (bc == kBCInflowStaticNSCBC) rhoIn = p.bc_data[f][0];
+    if (bc == kBCInflowTotalNSCBC || bc == kBCInflowSubsonic) { pTot = p.bc_data[f][0]; TTot = p.bc_data[f][1]; }
+    if (bc == kBCWallIsothermal) T = p.bc_data[f][0];
   }
   r.p_ref = pOut > 0.0 ? pOut : pTot > 0.0 ? pTot : -1.0;
-  r.rho_ref = rhoIn > 0.0 ? rhoIn : (r.p_ref > 0.0 && TTot > 0.0) ? r.p_ref / (kRGas * TTot) : -1.0;
+  r.rho_ref = rhoIn > 0.0 ? rhoIn : -1.0;
+  if (r.rho_ref <= 0.0 && r.p_ref > 0.0) {
+    const double TRef = T > 0.0 ? T : TTot;
+    if (TRef > 0.0) r.rho_ref = r.p_ref / (kRGas * TRef);
+  }
   if (r.p_ref <= 0.0 || r.rho_ref <= 0.0) {
     const double vInf = p.mach * std::sqrt(kGamma * kRGas * p.temp_free_stream);
     r.rho_ref = p.reynolds * kMuDim / (p.reynolds_length * vInf);
@@ -484,16 +495,29 @@ std::string setup_open_boundary(const CaseParams &p, const Tables &t, const Geom
     ob.any = true;
     const double *d = p.bc_data[f];
     double *o = ob.data[f];
-    if (p.bc[f] == kBCOutflowNSCBC) {
+    const int bc = p.bc[f];
+    // ConvertPrescribedData (SubfaceClasses.cpp:739-751, :863-915,
+    // :1169-1180, :1304-1313, :1385-1400, :1453-1475); an isothermal wall
+    // keeps its dimensional temperature
+    if (bc == kBCOutflowNSCBC || bc == kBCOutflowSubsonic) {
       o[0] = d[0] / rc.p_ref;
-    } else if (p.bc[f] == kBCInflowStaticNSCBC) {
+    } else if (bc == kBCInflowStaticNSCBC) {
       o[0] = d[0] / rc.rho_ref;
       for (int c = 1; c < 4; ++c) o[c] = d[c] / rc.u_ref;
-    } else {
+    } else if (bc == kBCInflowTotalNSCBC || bc == kBCInflowSubsonic) {
       const double len = std::sqrt(d[2] * d[2] + d[3] * d[3] + d[4] * d[4]);
       if (!(len > 0.0)) return "total-condition inflow needs a velocity direction";
-      o[0] = d[0] / rc.p_ref; o[1] = d[1] / TRef;
+      o[0] = d[0] / rc.p_ref;
+      o[1] = bc == kBCInflowSubsonic ? d[1] * (kGamma * kRGas / (kGamma - 1.0) / (rc.u_ref * rc.u_ref)) : d[1] / TRef;
       for (int c = 2; c < 5; ++c) o[c] = d[c] / len;
+    } else if (bc == kBCWallHeatFlux) {
+      o[0] = d[0] / (rc.p_ref * rc.u_ref);
+    } else if (bc == kBCWallIsothermal) {
+      if (!(d[0] > 0.0)) return "isothermal wall needs a positive temperature";
+      o[0] = d[0];
+      o[1] = -rc.p_ref / (kRGas * rc.rho_ref);   // the wall state's V4 is o[1] / o[0]
+    } else if (bc != kBCWallInviscid) {
+      return "unsupported boundary condition";
     }
     int first = -1;
     for (int e = 0; e < nElem; ++e) {
@@ -514,6 +538,7 @@ std::string setup_open_boundary(const CaseParams &p, const Tables &t, const Geom
       const double ct = (sm[0] * gr[0] + sm[n2] * gr[n2] + sm[2 * n2] * gr[2 * n2]) / len;
       if (std::fabs(ct - 1.0) < 1e-10) normal = i;
     }
+    if (!bc_is_nscbc(bc)) continue;   // the orientation serves the characteristic faces only
     if (normal < 0) return "Could not identify relevant boundary indices!";
     if (normal != f / 2) return "NSCBC face whose normal is not along its own parametric direction";
     ob.ig[f][0] = normal;

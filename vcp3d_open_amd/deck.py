@@ -30,8 +30,10 @@ ReadRestartSolution replaces it when "Restart: yes", and AddFluctuations adds
 the pressure pulse while the time step is 0 (ElementClass.cpp:515-665).
 
 Only what the GPU path supports is accepted (entropy variables, Ismail-Roe, no
-SGS or wall model, no body force, periodic / PC-NSCBC block faces with one
-subface each); anything else raises DeckError naming the setting.
+SGS or wall model, no body force; block faces periodic, PC-NSCBC in/outflow,
+subsonic total-condition inflow, subsonic outflow, heat-flux, isothermal or
+inviscid walls, one subface each); anything else raises DeckError naming the
+setting.
 """
 from __future__ import annotations
 
@@ -44,7 +46,13 @@ from . import io as vio
 from .cases import GAMMA, MU_DIM, R_GAS
 
 BC_CODES = {"periodic1to1trans": 0, "bcoutflownscbc": 1, "bcinflowstaticnscbc": 2,
-            "bcinflowtotalnscbc": 3}
+            "bcinflowtotalnscbc": 3, "bcinflowsubsonic": 4, "bcoutflowsubsonic": 5,
+            "bcwallheatflux": 6, "bcwallisothermal": 7, "bcwallinviscid": 8}
+# prescribed data per boundary condition, in the order of vcp_case_desc.bc_data
+_BC_DATA = {1: ["Pressure"], 2: ["Density", "VelocityX", "VelocityY", "VelocityZ"],
+            3: ["TotalPressure", "TotalTemperature", "VelocityDirX", "VelocityDirY", "VelocityDirZ"],
+            4: ["TotalPressure", "TotalTemperature", "VelocityDirX", "VelocityDirY", "VelocityDirZ"],
+            5: ["Pressure"], 6: ["HeatFlux"], 7: ["Temperature"], 8: []}
 _KNOWN_BC = ("internal1to1", "periodic1to1trans", "bcfarfield", "bcwallisothermal",
              "bcwallisothermalwm", "bcwallheatflux", "bcwallheatfluxwm", "bcwallinviscid",
              "bcsymmetryplane", "bcinflowsubsonic", "bcinflowsupersonic", "bcoutflowsubsonic",
@@ -267,10 +275,10 @@ class DeckCase:
             self.periodic[f // 2] = 0
             self.bc[f] = BC_CODES[bc]
             data = prm.prescribed.get((f, 0), {})
-            names = {1: ["Pressure"], 2: ["Density", "VelocityX", "VelocityY", "VelocityZ"],
-                     3: ["TotalPressure", "TotalTemperature", "VelocityDirX", "VelocityDirY",
-                         "VelocityDirZ"]}[self.bc[f]]
-            for q, nm in enumerate(names):
+            if self.bc[f] == 4 and "TotalPressure" not in data:
+                raise DeckError(f"face {f + 1}: BCInflowSubsonic is supported with total conditions "
+                                "(TotalPressure, TotalTemperature, VelocityDir*)")
+            for q, nm in enumerate(_BC_DATA[self.bc[f]]):
                 if nm not in data:
                     raise DeckError(f'face {f + 1}: prescribed "{nm}" missing')
                 if len(set(data[nm])) != 1:
@@ -436,7 +444,8 @@ class DeckCase:
             xyz = self.nodal_coords()
             c = self.pulse["centre"]
             d0, w0 = self.pulse["strength"], self.pulse["width"]
-            cc = 0.5 / (w0 * w0)
+            with np.errstate(divide="ignore"):
+                cc = np.float64(0.5) / np.float64(w0 * w0)   # inf for a zero width, as in the reference
             dx, dy, dz = (xyz[..., i] - c[i] for i in range(3))
             if self.pulse["one_d"]:
                 pinf, rinf = q[..., 4].copy(), q[..., 0]

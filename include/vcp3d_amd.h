@@ -87,6 +87,12 @@ typedef struct vcp_case_desc {
   int    sponge_elems[6];    /* layer thickness in elements per block face      */
   double damping_constant[6];
   double damping_exponent[6];
+  /* "Monitor forces" and "Reference area force coefficients"
+     (InputParamClass.cpp:79-80): force coefficients of the walls in
+     monitoring[0..5], scaled by 2 / (gamma Mach^2 area) (SolverClass.cpp:
+     3775-3783); reference_area <= 0 means the reference's default 1 */
+  int    monitor_forces;
+  double reference_area;
 } vcp_case_desc;
 
 /* Block-face boundary conditions ("Subface information",

@@ -76,6 +76,8 @@ struct orc_case {
   double *sol, *solOld, *res, *resMin[3], *dUdV;
   int *cgIt;
   int resValid, timeStep;
+  int monNow;             /* the residual being evaluated computes monitoring data */
+  double *forceElem;      /* [nElem][6] wall force sums of the element's faces */
   /* per-thread scratch */
   double **works;
   int nthreads;
@@ -847,7 +849,7 @@ static void ismail_roe(const double *VL, const double *VR, const double n[3], do
 static void internal_face_flux(const orc_case *c, double fn, double lenScale,
                                const double *SL, double *GrL, double *GsL, double *GtL, const double *mL,
                                const double *SR, double *GrR, double *GsR, double *GtR, const double *mR,
-                               double factU, int hf, double qWall,
+                               double factU, int hf, double qWall, double *force,
                                double *flux, double *sxL, double *syL, double *szL,
                                double *sxR, double *syR, double *szR)
 {
@@ -901,6 +903,21 @@ static void internal_face_flux(const orc_case *c, double fn, double lenScale,
     fl[4] -= (u * txx + v * txy + w * txz - qx) * nx + (u * txy + v * tyy + w * tyz - qy) * ny
            + (u * txz + v * tyz + w * tzz - qz) * nz;
     if (hf) fl[4] += wA * qWall;
+    if (force) {
+      /* wall forces (FluxesFace.cpp:999-1013, :1590-1612): the averaged
+         pressure of the two states and the viscous stress, with the normal
+         pointing into the boundary */
+      double rL_, uL_, vL_, wL_, pL_, eL_, rR_, uR_, vR_, wR_, pR_, eR_;
+      ent2prim(VL, &rL_, &uL_, &vL_, &wL_, &pL_, &eL_);
+      ent2prim(VR, &rR_, &uR_, &vR_, &wR_, &pR_, &eR_);
+      const double abv = 0.5 * fn * c->w2[m] * mL[3 * n2 + m] * (pL_ + pR_);
+      force[0] += abv * mL[0 * n2 + m]; force[1] += abv * mL[1 * n2 + m]; force[2] += abv * mL[2 * n2 + m];
+      const double area = fn * c->w2[m] * mL[3 * n2 + m];
+      const double ax = area * mL[0 * n2 + m], ay = area * mL[1 * n2 + m], az = area * mL[2 * n2 + m];
+      force[3] -= txx * ax + txy * ay + txz * az;
+      force[4] -= txy * ax + tyy * ay + tyz * az;
+      force[5] -= txz * ax + tyz * ay + tzz * az;
+    }
     /* penalty and symmetrising fluxes (FluxesFace.cpp:1152-1284) */
     const double pOvRho = -V4Inv, e = pOvRho * ovgm1;
     const double dV1 = VL[1] - VR[1], dV2 = VL[2] - VR[2], dV3 = VL[3] - VR[3], dV4 = VL[4] - VR[4];
@@ -1294,12 +1311,14 @@ static int boundary_face(orc_case *c, int f, int e, double *buf)
   if (is_nscbc(bc)) {
     double *GL[3] = {G0, G1, G2}, *GR[3] = {H0, H1, H2};
     nscbc_state(c, f, e, SL, GL, SR, GR);
-    internal_face_flux(c, fn, lenScale, SL, G0, G1, G2, m, SR, H0, H1, H2, m, 1.0, 0, 0.0,
+    internal_face_flux(c, fn, lenScale, SL, G0, G1, G2, m, SR, H0, H1, H2, m, 1.0, 0, 0.0, NULL,
                        flux, sxL, syL, szL, sxR, syR, szR);
   } else {
     pointwise_state(c, bc, m, fn, c->bcData[f], SL, SR);
+    /* forces of the walls when monitoring (SolverClass.cpp:3386, FluxesFace.cpp:98-101) */
+    double *force = (c->monNow && c->prm.monitor_forces && is_wall(bc)) ? c->forceElem + (size_t) e * 6 : NULL;
     internal_face_flux(c, fn, lenScale, SL, G0, G1, G2, m, SR, G0, G1, G2, m, is_wall(bc) ? 0.0 : 1.0,
-                       bc == ORC_BC_WALL_HEATFLUX, c->bcData[f][0], flux, sxL, syL, szL, sxR, syR, szR);
+                       bc == ORC_BC_WALL_HEATFLUX, c->bcData[f][0], force, flux, sxL, syL, szL, sxR, syR, szR);
   }
   face_lift(c, f, flux, sxL, syL, szL, c->res + (size_t) e * NVAR * nD);
   return 0;
@@ -1328,7 +1347,7 @@ static int face_residual(orc_case *c, int dir, int e)
   const double lenScale = fmin(c->len[(size_t) e * 4 + dir], c->len[(size_t) L * 4 + dir]);
   const double *mL = c->sm[2 * dir + 1] + (size_t) L * 16 * n2;
   const double *mR = c->sm[2 * dir] + (size_t) e * 16 * n2;
-  internal_face_flux(c, 1.0, lenScale, SL, GrL, GsL, GtL, mL, SR, GrR, GsR, GtR, mR, 1.0, 0, 0.0,
+  internal_face_flux(c, 1.0, lenScale, SL, GrL, GsL, GtL, mL, SR, GrR, GsR, GtR, mR, 1.0, 0, 0.0, NULL,
                      flux, sxL, syL, szL, sxR, syR, szR);
   double *rmin = c->resMin[dir] + (size_t) e * NVAR * nD;
   for (int i = 0; i < NVAR * nD; ++i) rmin[i] = 0.0;
@@ -1626,6 +1645,7 @@ orc_case *orc_create(const orc_params *prm, const double *elem_grid)
   const size_t ns = (size_t) c->nElem * NVAR * c->nDOFs;
   c->sol = dalloc(ns); c->solOld = dalloc(ns); c->res = dalloc(ns);
   for (int d = 0; d < 3; ++d) c->resMin[d] = dalloc(ns);
+  c->forceElem = dalloc((size_t) c->nElem * 6);
   c->dUdV = dalloc((size_t) c->nElem * 15 * c->nInt);
   c->cgIt = (int *) calloc(c->nElem, sizeof(int));
   return c;
@@ -1638,7 +1658,7 @@ void orc_destroy(orc_case *c)
                     c->FMin, c->FMax, c->dFMin, c->dFMax, c->w3, c->w2, c->basisCenter, c->vm,
                     c->sm[0], c->sm[1], c->sm[2], c->sm[3], c->sm[4], c->sm[5], c->len, c->aveNorm,
                     c->sol, c->solOld, c->res, c->resMin[0], c->resMin[1], c->resMin[2], c->dUdV,
-                    c->Lq, c->dLq, c->PC, c->dampFn, c->dampState, c->matchScale};
+                    c->Lq, c->dLq, c->PC, c->dampFn, c->dampState, c->matchScale, c->forceElem};
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
   if (c->works) for (int t = 0; t < c->nthreads; ++t) free(c->works[t]);
   free(c->works);
@@ -1728,6 +1748,8 @@ int orc_residual(orc_case *c, int mon, double md[8])
   /* the reference calls AverageDataNSCBC before every Residual
      (VCP3D_CurviLinear.cpp:89, SolverClass.cpp:2405) */
   if (c->nscbc) nscbc_average(c);
+  c->monNow = mon;
+  if (mon) for (size_t i = 0; i < (size_t) c->nElem * 6; ++i) c->forceElem[i] = 0.0;
 #pragma omp parallel for schedule(static) reduction(max : mach2)
   for (int e = 0; e < c->nElem; ++e) {
     double m2 = 0.0;
@@ -1754,7 +1776,16 @@ int orc_residual(orc_case *c, int mon, double md[8])
   for (int e = 0; e < c->nElem; ++e)
     if (cg_entropy(c, e)) err |= 1;
   if (err) return -1;
+  /* force coefficients: element sums in element order, times
+     2 / (gamma Mach^2 area) (SolverClass.cpp:3754-3783) */
   for (int i = 0; i < 6; ++i) md[i] = 0.0;
+  if (mon) {
+    for (int e = 0; e < c->nElem; ++e)
+      for (int i = 0; i < 6; ++i) md[i] += c->forceElem[(size_t) e * 6 + i];
+    const double area = c->prm.reference_area > 0.0 ? c->prm.reference_area : 1.0;
+    const double fc = 2.0 / (GAM * c->prm.mach * c->prm.mach * area);
+    for (int i = 0; i < 6; ++i) md[i] *= fc;
+  }
   md[6] = mach2; md[7] = mon ? 0.0 : -1.0;
   c->resValid = 1;
   return 0;

@@ -54,6 +54,9 @@ typedef struct {
   int    sponge_layer, cml;
   int    sponge_elems[6];
   double damping_constant[6], damping_exponent[6];
+  /* "Monitor forces", "Reference area force coefficients" (<= 0: 1) */
+  int    monitor_forces;
+  double reference_area;
 } orc_params;
 
 enum { ORC_BC_PERIODIC = 0, ORC_BC_OUTFLOW_NSCBC = 1,

@@ -156,9 +156,19 @@ def test_oracle_on_reference_decks(oracle_lib, name):
     o.residual()
     assert rel(o.residual_array(), g["res0"]) < 1e-9
     for _ in range(3):
-        o.sync_step()
+        md = o.sync_step()
     assert rel(o.nodal(), g["sol_3"]) < 1e-12
     assert rel(o.residual_array(), g["res_3"]) < 1e-9
+    check_monitoring(md, g["monitoring"])
+
+
+def check_monitoring(md, gold):
+    """Monitoring data of the last step: wall force coefficients (pressure
+    x, y, z, viscous x, y, z) and the maximum Mach^2. The pressure components
+    of closed walls cancel from terms of order 1e2 to ~1e-12, hence the
+    absolute part of the force tolerance."""
+    np.testing.assert_allclose(md[:6], gold[:6], rtol=1e-8, atol=1e-10)
+    assert abs(md[6] - gold[6]) <= 1e-9 * gold[6]
 
 
 def test_oracle_on_reference_duct_deck(oracle_lib):
@@ -172,7 +182,8 @@ def test_oracle_on_reference_duct_deck(oracle_lib):
     np.testing.assert_allclose(list(o.reference().values()), g["reference"], rtol=1e-14)
     o.residual()
     for _ in range(3):
-        o.sync_step()
+        md = o.sync_step()
+    check_monitoring(md, g["monitoring"])
     sol = o.nodal()
     assert rel(sol[g["elements"]], g["sol_3"][..., :5]) < 1e-12
     # per-variable norms as one vector (the transverse momenta are ~1e-6 of it)

@@ -16,6 +16,10 @@
 
 namespace {
 
+// device scalars read back with the monitoring data, and the slot of the
+// error flags in the pinned host buffer
+constexpr int kScalRead = vcp::kScalForce + 6, kHostFlags = vcp::kScalCount;
+
 thread_local std::string g_error;
 
 int fail(const std::string &msg, int code = -1)
@@ -43,8 +47,9 @@ struct vcp_ctx {
   // side stream for the next call's time step, which overlaps the last stage
   hipStream_t side = nullptr;
   hipEvent_t evState = nullptr, evDt = nullptr;
-  // pinned host words the per-step scalars come back through:
-  // [0] time step, [1] maximum Mach^2, [2] error flags (as int)
+  // pinned host words the per-step scalars come back through: [0, kScalRead)
+  // the device scalars (time step, maximum Mach^2, ..., wall forces),
+  // [kHostFlags] the error flags (as int)
   double *host = nullptr;
   std::vector<void *> allocs;
   bool res_valid = false;
@@ -105,7 +110,7 @@ int residual_pass(vcp_ctx *c, const vcp::StageCtl &ctl)
 {
   if (c->prm.sponge_layer && !c->damping_set)
     return fail("sponge layer damping state not set (vcp_set_damping_state)");
-  HIP_TRY(vcp::launch_boundary(c->dc, c->stream));
+  HIP_TRY(vcp::launch_boundary(c->dc, c->stream, ctl.monitor != 0));
   HIP_TRY(vcp::launch_faces(c->dc, c->stream));
   HIP_TRY(vcp::launch_volume(c->dc, ctl, c->stream));
   return 0;
@@ -184,6 +189,8 @@ void params_from_desc(const vcp_case_desc *d, vcp::CaseParams &p)
     p.damping_constant[f] = d->damping_constant[f];
     p.damping_exponent[f] = d->damping_exponent[f];
   }
+  p.monitor_forces = d->monitor_forces;
+  p.reference_area = d->reference_area > 0.0 ? d->reference_area : 1.0;
   p.outlet_len = d->outlet_len; p.outlet_sigma = d->outlet_sigma;
   p.outlet_beta_l = d->outlet_beta_l; p.outlet_beta_t = d->outlet_beta_t;
   p.outlet_one_minus_beta = d->outlet_one_minus_beta; p.outlet_average_local = d->outlet_average_local;
@@ -191,11 +198,17 @@ void params_from_desc(const vcp_case_desc *d, vcp::CaseParams &p)
   p.sponge_layer = d->sponge_layer; p.cml = d->cml;
 }
 
-void fill_monitoring(double md[8], bool mon, double mach2)
+// Monitoring data of SolverClass::Residual (SolverClass.cpp:3754-3783) from
+// the device scalars sc[0..kScalForce+6): force coefficients = wall force sums
+// times 2 / (gamma Mach^2 area), maximum Mach^2, maximum eddy viscosity (0: no
+// SGS model).
+void fill_monitoring(const vcp_ctx *c, double md[8], bool mon, const double *sc)
 {
-  for (int i = 0; i < 6; ++i) md[i] = 0.0;   // no walls: force coefficients vanish
-  md[6] = mon ? mach2 : 0.0;
-  md[7] = mon ? 0.0 : -1.0;                  // no SGS model: eddy viscosity 0
+  const double area = c->prm.reference_area > 0.0 ? c->prm.reference_area : 1.0;
+  const double fc = 2.0 / (1.4 * c->prm.mach * c->prm.mach * area);
+  for (int i = 0; i < 6; ++i) md[i] = mon && c->dc.monitor_forces ? sc[vcp::kScalForce + i] * fc : 0.0;
+  md[6] = mon ? sc[1] : 0.0;
+  md[7] = mon ? 0.0 : -1.0;
 }
 
 }  // namespace
@@ -273,6 +286,10 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
     nb.invArea[f] = ob.area[f] > 0.0 ? 1.0 / ob.area[f] : 0.0;
   }
   for (int f = 0; f < 7; ++f) nb.begin[f] = ob.begin[f];
+  // "Monitor forces" is switched off without walls (InputParamClass.cpp:1572-1573)
+  dc.monitor_forces = 0;
+  for (int f = 0; f < 6; ++f)
+    if (p.monitor_forces && !p.periodic[f / 2] && vcp::bc_is_wall(p.bc[f])) dc.monitor_forces = 1;
   dc.nSponge = c->sponge.n; dc.cml = p.cml;
 
   // device layouts of the geometry
@@ -300,7 +317,7 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
     }
   }
   hipError_t he = hipStreamCreate(&c->stream);
-  if (he == hipSuccess) he = hipHostMalloc((void **) &c->host, 4 * sizeof(double), hipHostMallocDefault);
+  if (he == hipSuccess) he = hipHostMalloc((void **) &c->host, (kHostFlags + 1) * sizeof(double), hipHostMallocDefault);
   if (he == hipSuccess) he = hipStreamCreate(&c->side);
   if (he == hipSuccess) he = hipEventCreateWithFlags(&c->evState, hipEventDisableTiming);
   if (he == hipSuccess) he = hipEventCreateWithFlags(&c->evDt, hipEventDisableTiming);
@@ -318,7 +335,7 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
   // coefficient planes [2][5][K][K] of the lifted face residual
   const size_t nslot = (size_t) nElem * 6 * 10 * dc.K * dc.K;
   if (he == hipSuccess) he = c->alloc(&dc.faceRes, nslot);
-  if (he == hipSuccess) he = c->alloc(&dc.scal, 8);
+  if (he == hipSuccess) he = c->alloc(&dc.scal, vcp::kScalCount);
   if (he == hipSuccess) he = c->alloc(&dc.flags, 4);
   if (he == hipSuccess) he = c->alloc(&dc.cgIter, (size_t) nElem);
   if (he == hipSuccess) he = c->alloc(&dc.dtElem, (size_t) nElem);
@@ -327,6 +344,7 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
     he = c->alloc(&dc.bface, ob.faces.size());
     if (he == hipSuccess) he = c->alloc(&dc.bgeo, bgeo.size());
     if (he == hipSuccess) he = c->alloc(&dc.machPart, (size_t) dc.nBFace);
+    if (he == hipSuccess) he = c->alloc(&dc.forcePart, (size_t) dc.nBFace * 6);
     if (he == hipSuccess) he = hipMemcpy(dc.bface, ob.faces.data(), ob.faces.size() * sizeof(int), hipMemcpyHostToDevice);
     if (he == hipSuccess) he = hipMemcpy(dc.bgeo, bgeo.data(), bgeo.size() * sizeof(double), hipMemcpyHostToDevice);
   }
@@ -352,6 +370,7 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
   if (he == hipSuccess) he = hipMemset(dc.R, 0, ns * sizeof(double));
   if (he == hipSuccess) he = hipMemset(dc.faceRes, 0, nslot * sizeof(double));
   if (he == hipSuccess) he = hipMemset(dc.flags, 0, 4 * sizeof(int));
+  if (he == hipSuccess) he = hipMemset(dc.scal, 0, vcp::kScalCount * sizeof(double));
   if (he != hipSuccess) {
     const std::string msg = std::string("device set-up failed: ") + hipGetErrorString(he);
     delete c;
@@ -497,10 +516,10 @@ int vcp_residual(vcp_ctx *c, int mon, double md[8])
   int rc = 0;
   vcp::StageCtl ctl{1, 0, 0.0, 0.0, mon ? 1 : 0};
   if ((rc = residual_pass(c, ctl))) return rc;
-  double mach2 = 0.0;
-  HIP_TRY(hipMemcpyAsync(&mach2, c->dc.scal + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  double sc[vcp::kScalCount] = {};
+  HIP_TRY(hipMemcpyAsync(sc, c->dc.scal, kScalRead * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   if ((rc = check_flags(c))) return rc;
-  if (md) fill_monitoring(md, mon != 0, mach2);
+  if (md) fill_monitoring(c, md, mon != 0, sc);
   c->res_valid = true;
   return 0;
 }
@@ -537,10 +556,10 @@ int vcp_sync_step(vcp_ctx *c, double md[8])
       // time step and error flags back in one round trip
       HIP_TRY(vcp::launch_time_step(c->dc, c->stream));
       HIP_TRY(hipMemcpyAsync(c->host, c->dc.scal, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-      HIP_TRY(hipMemcpyAsync(c->host + 2, c->dc.flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(hipMemcpyAsync(c->host + kHostFlags, c->dc.flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
       HIP_TRY(hipStreamSynchronize(c->stream));
       int flags = 0;
-      std::memcpy(&flags, c->host + 2, sizeof(int));
+      std::memcpy(&flags, c->host + kHostFlags, sizeof(int));
       if ((rc = flag_error(flags))) return rc;
       dt = c->host[0];
     }
@@ -551,15 +570,15 @@ int vcp_sync_step(vcp_ctx *c, double md[8])
     if (reached) {
       // the next call's time step with the monitoring maximum and the error
       // flags in one round trip
-      HIP_TRY(hipMemcpyAsync(c->host, c->dc.scal, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-      HIP_TRY(hipMemcpyAsync(c->host + 2, c->dc.flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(hipMemcpyAsync(c->host, c->dc.scal, kScalRead * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(hipMemcpyAsync(c->host + kHostFlags, c->dc.flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
       HIP_TRY(hipStreamSynchronize(c->stream));
       int flags = 0;
-      std::memcpy(&flags, c->host + 2, sizeof(int));
+      std::memcpy(&flags, c->host + kHostFlags, sizeof(int));
       // a failure of the next step's time step is reported by the next call
       if ((rc = flag_error(flags & ~vcp::kFlagNegativeA2))) return rc;
       c->dt_ready = !(flags & vcp::kFlagNegativeA2);
-      fill_monitoring(md, true, c->host[1]);
+      fill_monitoring(c, md, true, c->host);
     }
   }
   c->res_valid = true;

@@ -151,7 +151,8 @@ __device__ __forceinline__ void point_flux(const double VL[5], const double VR[5
                                            const double gy[5], const double gz[5], const double nn[3],
                                            double wA, double tau, double mu, double thetaHalf,
                                            double fl[5], double FX[5], double FY[5], double FZ[5],
-                                           double factU = 1.0, bool hf = false, double qWall = 0.0)
+                                           double factU = 1.0, bool hf = false, double qWall = 0.0,
+                                           double *aux = nullptr)
 {
   const double factHF = (BND && hf) ? 0.0 : kGam / kPrLam;
   const double lam = kLamOverMu, lamp1 = lam + 1.0, lamp2 = lam + 2.0;
@@ -159,6 +160,7 @@ __device__ __forceinline__ void point_flux(const double VL[5], const double VR[5
     double rL, uL, vL, wL, pL, eL, rR, uR, vR, wR, pR, eR;
     ent2prim(VL, rL, uL, vL, wL, pL, eL);
     ent2prim(VR, rR, uR, vR, wR, pR, eR);
+    if (BND && aux) aux[0] = pL + pR;   // for the wall forces
     const double dV0 = VR[0] - VL[0], dV1 = VR[1] - VL[1], dV2 = VR[2] - VL[2],
                  dV3 = VR[3] - VL[3], dV4 = VR[4] - VL[4];
     // sqrt(rho / p) = sqrt(-V4)
@@ -232,6 +234,7 @@ __device__ __forceinline__ void point_flux(const double VL[5], const double VR[5
     fl[4] -= (u * txx + vv * txy + w * txz - qx) * nx + (u * txy + vv * tyy + w * tyz - qy) * ny
            + (u * txz + vv * tyz + w * tzz - qz) * nz;
     if (BND && hf) fl[4] += wA * qWall;
+    if (BND && aux) { aux[1] = txx; aux[2] = tyy; aux[3] = tzz; aux[4] = txy; aux[5] = txz; aux[6] = tyz; }
   }
   const double pOvRho = -V4Inv, eInt = pOvRho * kInvGm1;
   const double dV1 = VL[1] - VR[1], dV2 = VL[2] - VR[2], dV3 = VL[3] - VR[3], dV4 = VL[4] - VR[4];
@@ -1217,9 +1220,9 @@ template <int K, int M>
 struct BndArgs {
   const double *U, *tables, *econst, *bgeo;   // bgeo: [nBFace][13][nInt2D] own face metric
   const int *bface;
-  double *machPart, *faceRes;
+  double *machPart, *faceRes, *forcePart;
   TableOffsets to;
-  int bug;
+  int bug, monitor;             // monitor: wall forces of this residual into forcePart
   double mu, tauFact, thetaHalf;
   NscbcParams nb;
 };
@@ -1652,6 +1655,8 @@ __global__ __launch_bounds__(64) void nscbc_kernel(BndArgs<K, M> a)
   // fluxes with the face normal oriented out of the element
   const double lenScale = a.econst[(size_t) e * kElemConst + dir];
   const double tau = a.tauFact / lenScale;
+  const bool forces = a.monitor && bc >= kBCWallHeatFlux;
+  double frc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int m = tid; m < N2; m += 64) {
     double VL[5], VR[5], gx[5], gy[5], gz[5];
 #pragma unroll
@@ -1683,8 +1688,20 @@ __global__ __launch_bounds__(64) void nscbc_kernel(BndArgs<K, M> a)
     const double wA = sm.W2[m] * g[3 * N2 + m];
     double fl[5], FX[5], FY[5], FZ[5];
     const bool wall = bc >= kBCWallHeatFlux;
+    double aux[7];
     point_flux<true>(VL, VR, gx, gy, gz, nn, wA, tau, a.mu, a.thetaHalf, fl, FX, FY, FZ, wall ? 0.0 : 1.0,
-                     bc == kBCWallHeatFlux, a.nb.data[f][0]);
+                     bc == kBCWallHeatFlux, a.nb.data[f][0], forces ? aux : nullptr);
+    if (forces) {
+      // wall forces (FluxesFace.cpp:999-1013, :1590-1612): the averaged
+      // pressure and the viscous stress on the normal into the boundary
+      const double abv = 0.5 * fn * sm.W2[m] * g[3 * N2 + m] * aux[0];
+      frc[0] += abv * g[m]; frc[1] += abv * g[N2 + m]; frc[2] += abv * g[2 * N2 + m];
+      const double area = fn * sm.W2[m] * g[3 * N2 + m];
+      const double ax = area * g[m], ay = area * g[N2 + m], az = area * g[2 * N2 + m];
+      frc[3] -= aux[1] * ax + aux[4] * ay + aux[5] * az;
+      frc[4] -= aux[4] * ax + aux[2] * ay + aux[6] * az;
+      frc[5] -= aux[5] * ax + aux[6] * ay + aux[3] * az;
+    }
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
       const int pt = a.bug ? (v < N2 ? v : 0) : m;
@@ -1700,6 +1717,15 @@ __global__ __launch_bounds__(64) void nscbc_kernel(BndArgs<K, M> a)
       sm.Sym[1][v * N2 + m] = par[pF];
       sm.Sym[2][v * N2 + m] = par[pS];
     }
+  }
+  if (a.monitor) {
+    // this face's force sums (zero off the walls), summed in a fixed order
+    // over the faces by force_sum_kernel
+#pragma unroll
+    for (int i = 0; i < 6; ++i) frc[i] = wave_sum(frc[i]);
+    if (tid == 0)
+#pragma unroll
+      for (int i = 0; i < 6; ++i) a.forcePart[(size_t) b * 6 + i] = frc[i];
   }
   __syncthreads();
   // lift: fast axis, slow axis, then the face vectors along the normal
@@ -1869,6 +1895,26 @@ __global__ __launch_bounds__(1024) void reduce_max_kernel(const double *v, int n
   }
 }
 
+// Sums of the six wall-force components over the n boundary faces into
+// out[0..6), in face order (one wave, each lane a fixed stride of faces, then
+// the lanes in order).
+__global__ __launch_bounds__(64) void force_sum_kernel(const double *part, int n, double *out)
+{
+  __shared__ double red[64][6];
+  double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < n; b += 64)
+#pragma unroll
+    for (int i = 0; i < 6; ++i) s[i] += part[(size_t) b * 6 + i];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) red[threadIdx.x][i] = s[i];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    double t = 0.0;
+    for (int l = 0; l < 64; ++l) t += red[l][threadIdx.x];
+    out[threadIdx.x] = t;
+  }
+}
+
 // First RK stage: U <- U0 - dt R, where U0 already holds the state of the
 // step (the host swaps the two buffers instead of copying). dt is the
 // argument, which the first lane also stores to dtOut for the later stages,
@@ -1910,16 +1956,19 @@ hipError_t faces_impl(const DeviceCase &dc, hipStream_t s)
 }
 
 template <int K, int M>
-hipError_t boundary_impl(const DeviceCase &dc, hipStream_t s)
+hipError_t boundary_impl(const DeviceCase &dc, hipStream_t s, bool monitor)
 {
   if (dc.nBFace == 0) return hipSuccess;
   BndArgs<K, M> a;
   a.U = dc.U; a.tables = dc.tables; a.econst = dc.econst; a.bgeo = dc.bgeo; a.bface = dc.bface;
-  a.machPart = dc.machPart; a.faceRes = dc.faceRes; a.to = dc.to; a.bug = dc.bug_compat;
+  a.machPart = dc.machPart; a.faceRes = dc.faceRes; a.forcePart = dc.forcePart; a.to = dc.to;
+  a.bug = dc.bug_compat; a.monitor = monitor && dc.monitor_forces;
   a.mu = dc.mu; a.tauFact = (dc.P + 1.0) * (dc.P + 1.0) * dc.rel_penalty; a.thetaHalf = 0.5 * dc.theta_sym;
   a.nb = dc.nscbc;
   hipLaunchKernelGGL((mach_kernel<K, M>), dim3(dc.nBFace), dim3(64), 0, s, a);
   hipLaunchKernelGGL((nscbc_kernel<K, M>), dim3(dc.nBFace), dim3(64), 0, s, a);
+  if (a.monitor)
+    hipLaunchKernelGGL(force_sum_kernel, dim3(1), dim3(64), 0, s, dc.forcePart, dc.nBFace, dc.scal + kScalForce);
   return hipGetLastError();
 }
 
@@ -1977,9 +2026,9 @@ hipError_t launch_faces(const DeviceCase &dc, hipStream_t s)
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_boundary(const DeviceCase &dc, hipStream_t s)
+hipError_t launch_boundary(const DeviceCase &dc, hipStream_t s, bool monitor)
 {
-#define X(k, m) if (dc.K == k && dc.M == m) return boundary_impl<k, m>(dc, s);
+#define X(k, m) if (dc.K == k && dc.M == m) return boundary_impl<k, m>(dc, s, monitor);
   VCP_PAIRS(X)
 #undef X
   return hipErrorInvalidValue;

@@ -52,7 +52,8 @@ struct DeviceCase {
   double *faceRes = nullptr;      // [nElem][6][2][5][K][K] face-slot coefficient planes
   // device scalars: [0] minimum time step, [1] maximum Mach^2, [kScalDt] the
   // time step of the RK step being taken (written by the host before its
-  // stages, so that the launches of a step do not depend on its value)
+  // stages, so that the launches of a step do not depend on its value),
+  // [kScalForce .. +6) the wall force sums of the last monitored residual
   double *scal = nullptr;
   int *flags = nullptr;
   int *cgIter = nullptr;          // [nElem]
@@ -63,6 +64,8 @@ struct DeviceCase {
   int *bface = nullptr;           // [nBFace][2]: element, block face
   double *bgeo = nullptr;         // [nBFace][13][nInt2D]: the face's normal, area factor, inverse metric
   double *machPart = nullptr;     // [nBFace] weighted face-normal Mach number
+  double *forcePart = nullptr;    // [nBFace][6] wall forces (pressure xyz, viscous xyz)
+  int monitor_forces = 0;
   NscbcParams nscbc{};
   // sponge layer / characteristic matching layer
   int nSponge = 0, cml = 0;
@@ -73,7 +76,7 @@ struct DeviceCase {
   double *srcBuf = nullptr;       // [nSponge][5][nInt] source terms of the current residual
 };
 
-constexpr int kScalDt = 2;
+constexpr int kScalDt = 2, kScalForce = 3, kScalCount = 16;
 
 // Stage control for the fused volume/mass-matrix kernel.
 struct StageCtl {
@@ -86,7 +89,8 @@ struct StageCtl {
 bool supported_pair(int K, int M);
 hipError_t launch_faces(const DeviceCase &dc, hipStream_t s);
 // boundary-averaged Mach number partial sums, then the PC-NSCBC faces
-hipError_t launch_boundary(const DeviceCase &dc, hipStream_t s);
+// with monitor, also the wall forces into scal[kScalForce ..]
+hipError_t launch_boundary(const DeviceCase &dc, hipStream_t s, bool monitor = false);
 hipError_t launch_volume(const DeviceCase &dc, const StageCtl &ctl, hipStream_t s);
 hipError_t launch_time_step(const DeviceCase &dc, hipStream_t s);
 // scal[kScalDt] = dt, ordered on the stream

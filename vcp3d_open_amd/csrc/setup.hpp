@@ -44,6 +44,8 @@ struct CaseParams {
   int sponge_layer = 0, cml = 0;
   int sponge_elems[6] = {0, 0, 0, 0, 0, 0};
   double damping_constant[6] = {}, damping_exponent[6] = {};
+  int monitor_forces = 0;         // "Monitor forces"
+  double reference_area = 1.0;    // "Reference area force coefficients"
 };
 
 struct Tables {

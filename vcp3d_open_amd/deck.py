@@ -228,6 +228,8 @@ class DeckCase:
         self.cfl = prm.number("cfl number")
         self.n_sync_steps = prm.integer("number of synchronization time steps", -1)
         self.theta_sym = prm.number("theta parameter symmetrizing terms", 1.0)
+        self.monitor_forces = prm.yes("monitor forces", True)
+        self.reference_area = prm.number("reference area force coefficients", 1.0)
         self.rel_penalty = prm.number("relative penalty parameter", 1.0)
         self.grid_file = os.path.join(self.dir, prm.text("grid file name"))
         self.restart = prm.yes("restart", False)
@@ -349,7 +351,8 @@ class DeckCase:
                  periodic=tuple(self.periodic), mach=self.mach, reynolds=self.reynolds,
                  reynolds_length=self.reynolds_length, temp_free_stream=self.temp,
                  theta_sym=self.theta_sym, rel_penalty=self.rel_penalty, cfl=self.cfl,
-                 dt_synchr=self.dt_synchr, bc=self.bc, bc_data=self.bc_data, **self.nscbc)
+                 dt_synchr=self.dt_synchr, bc=self.bc, bc_data=self.bc_data,
+                 monitor_forces=int(self.monitor_forces), reference_area=self.reference_area, **self.nscbc)
         if self.sponge_layer or self.cml:
             d.update(sponge_layer=int(self.sponge_layer), cml=int(self.cml),
                      sponge_elems=[r[0] for r in self.sponge_rows],

@@ -41,6 +41,7 @@ class CaseDesc(C.Structure):
         ("sponge_layer", C.c_int), ("cml", C.c_int),
         ("sponge_elems", C.c_int * 6),
         ("damping_constant", C.c_double * 6), ("damping_exponent", C.c_double * 6),
+        ("monitor_forces", C.c_int), ("reference_area", C.c_double),
     ]
 
 

@@ -65,7 +65,9 @@ typedef struct vcp_case_desc {
                                 {TotalPressure, TotalTemperature, VelocityDirX,
                                 VelocityDirY, VelocityDirZ}; heat-flux wall
                                 {HeatFlux}; isothermal wall {Temperature};
-                                inviscid wall {}                                 */
+                                supersonic inflow {Density, VelocityX,
+                                VelocityY, VelocityZ, Pressure}; inviscid wall
+                                and supersonic outflow {}                        */
   /* "NSCBC tuning parameters" (InputParamClass.cpp:406-424) */
   double outlet_len;         /* "Outlet characteristic length"                  */
   double outlet_sigma;       /* "Outlet normal relaxation (sigma)"              */
@@ -98,10 +100,12 @@ typedef struct vcp_case_desc {
 /* Block-face boundary conditions ("Subface information",
    InputParamClass.cpp:1530-1548): BCOutflowNSCBC, BCInflowStaticNSCBC,
    BCInflowTotalNSCBC, BCInflowSubsonic (total conditions), BCOutflowSubsonic,
-   BCWallHeatFlux, BCWallIsothermal, BCWallInviscid. */
+   BCWallHeatFlux, BCWallIsothermal, BCWallInviscid, BCInflowSupersonic,
+   BCOutflowSupersonic. */
 enum { VCP_BC_PERIODIC = 0, VCP_BC_OUTFLOW_NSCBC = 1, VCP_BC_INFLOW_STATIC_NSCBC = 2,
        VCP_BC_INFLOW_TOTAL_NSCBC = 3, VCP_BC_INFLOW_SUBSONIC = 4, VCP_BC_OUTFLOW_SUBSONIC = 5,
-       VCP_BC_WALL_HEATFLUX = 6, VCP_BC_WALL_ISOTHERMAL = 7, VCP_BC_WALL_INVISCID = 8 };
+       VCP_BC_WALL_HEATFLUX = 6, VCP_BC_WALL_ISOTHERMAL = 7, VCP_BC_WALL_INVISCID = 8,
+       VCP_BC_INFLOW_SUPERSONIC = 9, VCP_BC_OUTFLOW_SUPERSONIC = 10 };
 
 typedef struct vcp_ctx vcp_ctx;
 

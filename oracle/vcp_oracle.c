@@ -1264,6 +1264,10 @@ static void pointwise_state(const orc_case *c, int bc, const double *m, double f
       R[0] = (GAM - sR) * ovgm1 - 0.5 * pRInv * rhoR * qR2;
       R[1] = rhoR * pRInv * qR * d[2]; R[2] = rhoR * pRInv * qR * d[3]; R[3] = rhoR * pRInv * qR * d[4];
       R[4] = -rhoR * pRInv;
+    } else if (bc == ORC_BC_INFLOW_SUPERSONIC) {   /* BoundaryState.cpp:605-635 */
+      for (int q = 0; q < 5; ++q) R[q] = d[q];
+    } else if (bc == ORC_BC_OUTFLOW_SUPERSONIC) {  /* BoundaryState.cpp:793-818 */
+      for (int q = 0; q < 5; ++q) R[q] = V[q];
     } else {  /* subsonic outflow */
       const double V4Inv = 1.0 / V[4];
       const double u = -V4Inv * V[1], v = -V4Inv * V[2], w = -V4Inv * V[3];
@@ -1437,7 +1441,7 @@ static int is_nscbc(int bc)
   return bc == ORC_BC_OUTFLOW_NSCBC || bc == ORC_BC_INFLOW_STATIC_NSCBC || bc == ORC_BC_INFLOW_TOTAL_NSCBC;
 }
 
-static int is_physical(int bc) { return bc >= ORC_BC_OUTFLOW_NSCBC && bc <= ORC_BC_WALL_INVISCID; }
+static int is_physical(int bc) { return bc >= ORC_BC_OUTFLOW_NSCBC && bc <= ORC_BC_OUTFLOW_SUPERSONIC; }
 
 static int check_params(const orc_case *c)
 {
@@ -1481,6 +1485,7 @@ static void reference_conditions(orc_case *c)
     if (bc == ORC_BC_INFLOW_STATIC_NSCBC) rho = q->bc_data[f][0];
     if (bc == ORC_BC_INFLOW_TOTAL_NSCBC || bc == ORC_BC_INFLOW_SUBSONIC) { pTot = q->bc_data[f][0]; TTot = q->bc_data[f][1]; }
     if (bc == ORC_BC_WALL_ISOTHERMAL) T = q->bc_data[f][0];
+    if (bc == ORC_BC_INFLOW_SUPERSONIC) { rho = q->bc_data[f][0]; p = q->bc_data[f][4]; }
   }
   c->pRef = -1.0;
   if (p > 0.0) c->pRef = p; else if (pTot > 0.0) c->pRef = pTot;
@@ -1517,6 +1522,14 @@ static void reference_conditions(orc_case *c)
       o[2] = d[2] * inv; o[3] = d[3] * inv; o[4] = d[4] * inv;
     } else if (bc == ORC_BC_WALL_HEATFLUX) o[0] = d[0] / (c->pRef * c->uRef);
     else if (bc == ORC_BC_WALL_ISOTHERMAL) o[0] = d[0];
+    else if (bc == ORC_BC_INFLOW_SUPERSONIC) {
+      /* the prescribed state as entropy variables (SubfaceClasses.cpp:1026-1106) */
+      const double rho = d[0] / c->rhoRef, u = d[1] / c->uRef, v = d[2] / c->uRef, w = d[3] / c->uRef;
+      const double pr = d[4] / c->pRef;
+      const double s = log(pr / pow(rho, GAM)), pInv = 1.0 / pr;
+      o[0] = (GAM - s) / (GAM - 1.0) - 0.5 * pInv * rho * (u * u + v * v + w * w);
+      o[1] = rho * u * pInv; o[2] = rho * v * pInv; o[3] = rho * w * pInv; o[4] = -rho * pInv;
+    }
   }
 }
 

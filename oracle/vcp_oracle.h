@@ -43,7 +43,8 @@ typedef struct {
                               NSCBC or subsonic outflow {p}; static NSCBC
                               inflow {rho, u, v, w}; total NSCBC or subsonic
                               inflow {p_t, T_t, dir_x, dir_y, dir_z};
-                              heat-flux wall {q}; isothermal wall {T} */
+                              heat-flux wall {q}; isothermal wall {T};
+                              supersonic inflow {rho, u, v, w, p} */
   /* "NSCBC tuning parameters" (InputParamClass.cpp:406-424) */
   double outlet_len, outlet_sigma, outlet_beta_l, outlet_beta_t;
   int    outlet_one_minus_beta, outlet_average_local;
@@ -63,7 +64,8 @@ enum { ORC_BC_PERIODIC = 0, ORC_BC_OUTFLOW_NSCBC = 1,
        ORC_BC_INFLOW_STATIC_NSCBC = 2, ORC_BC_INFLOW_TOTAL_NSCBC = 3,
        ORC_BC_INFLOW_SUBSONIC = 4,      /* total conditions */
        ORC_BC_OUTFLOW_SUBSONIC = 5, ORC_BC_WALL_HEATFLUX = 6,
-       ORC_BC_WALL_ISOTHERMAL = 7, ORC_BC_WALL_INVISCID = 8 };
+       ORC_BC_WALL_ISOTHERMAL = 7, ORC_BC_WALL_INVISCID = 8,
+       ORC_BC_INFLOW_SUPERSONIC = 9, ORC_BC_OUTFLOW_SUPERSONIC = 10 };
 
 typedef struct orc_case orc_case;
 

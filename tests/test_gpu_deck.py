@@ -31,7 +31,8 @@ def lib():
 @pytest.mark.parametrize("name,deck", [("deck_n3_8x4x4_static", "param_static.cfg"),
                                        ("deck_n3_8x4x4_total", "param_total.cfg"),
                                        ("deck_walls_a", "param_walls_a.cfg"),
-                                       ("deck_walls_b", "param_walls_b.cfg")])
+                                       ("deck_walls_b", "param_walls_b.cfg"),
+                                       ("deck_walls_c", "param_walls_c.cfg")])
 def test_reference_deck_matches_reference(lib, name, deck):
     print(f"[case] {name}", flush=True)
     _, gold = load(name)

@@ -36,7 +36,8 @@ namespace vcp {
 namespace {
 
 enum : int { kBCOutflowNSCBC = 1, kBCInflowStaticNSCBC = 2, kBCInflowTotalNSCBC = 3, kBCInflowSubsonic = 4,
-              kBCOutflowSubsonic = 5, kBCWallHeatFlux = 6, kBCWallIsothermal = 7, kBCWallInviscid = 8 };
+              kBCOutflowSubsonic = 5, kBCWallHeatFlux = 6, kBCWallIsothermal = 7, kBCWallInviscid = 8,
+              kBCInflowSupersonic = 9, kBCOutflowSupersonic = 10 };
 constexpr double kGam = 1.4;
 constexpr double kGm1 = 0.4;
 constexpr double kPrLam = 0.72;
@@ -1628,6 +1629,12 @@ __global__ __launch_bounds__(64) void nscbc_kernel(BndArgs<K, M> a)
         R[0] = (kGam - sR) * kInvGm1 - 0.5 * pRInv * rhoR * qR2;
         R[1] = rhoR * pRInv * qR * d[2]; R[2] = rhoR * pRInv * qR * d[3]; R[3] = rhoR * pRInv * qR * d[4];
         R[4] = -rhoR * pRInv;
+      } else if (bc == kBCInflowSupersonic) {   // the prescribed state
+#pragma unroll
+        for (int v = 0; v < 5; ++v) R[v] = d[v];
+      } else if (bc == kBCOutflowSupersonic) {  // the interior state
+#pragma unroll
+        for (int v = 0; v < 5; ++v) R[v] = V[v];
       } else {   // subsonic outflow
         const double V4Inv = 1.0 / V[4];
         const double u = -V4Inv * V[1], v = -V4Inv * V[2], w = -V4Inv * V[3];
@@ -1655,7 +1662,8 @@ __global__ __launch_bounds__(64) void nscbc_kernel(BndArgs<K, M> a)
   // fluxes with the face normal oriented out of the element
   const double lenScale = a.econst[(size_t) e * kElemConst + dir];
   const double tau = a.tauFact / lenScale;
-  const bool forces = a.monitor && bc >= kBCWallHeatFlux;
+  const bool wall = bc >= kBCWallHeatFlux && bc <= kBCWallInviscid;
+  const bool forces = a.monitor && wall;
   double frc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int m = tid; m < N2; m += 64) {
     double VL[5], VR[5], gx[5], gy[5], gz[5];
@@ -1687,7 +1695,6 @@ __global__ __launch_bounds__(64) void nscbc_kernel(BndArgs<K, M> a)
     const double nn[3] = {fn * g[m], fn * g[N2 + m], fn * g[2 * N2 + m]};
     const double wA = sm.W2[m] * g[3 * N2 + m];
     double fl[5], FX[5], FY[5], FZ[5];
-    const bool wall = bc >= kBCWallHeatFlux;
     double aux[7];
     point_flux<true>(VL, VR, gx, gy, gz, nn, wA, tau, a.mu, a.thetaHalf, fl, FX, FY, FZ, wall ? 0.0 : 1.0,
                      bc == kBCWallHeatFlux, a.nb.data[f][0], forces ? aux : nullptr);

@@ -314,7 +314,7 @@ std::string check_case(const CaseParams &p)
   for (int f = 0; f < 6; ++f) {
     const bool per = p.periodic[f / 2] != 0;
     if (per && p.bc[f] != kBCPeriodic) return "a periodic direction cannot carry a physical boundary";
-    if (!per && (p.bc[f] < kBCOutflowNSCBC || p.bc[f] > kBCWallInviscid))
+    if (!per && (p.bc[f] < kBCOutflowNSCBC || p.bc[f] > kBCOutflowSupersonic))
       return "unsupported boundary condition on a non-periodic block face";
   }
   if (p.outlet_average_local)
@@ -466,6 +466,7 @@ RefConditions reference_conditions(const CaseParams &p)
     if (bc == kBCInflowStaticNSCBC) rhoIn = p.bc_data[f][0];
     if (bc == kBCInflowTotalNSCBC || bc == kBCInflowSubsonic) { pTot = p.bc_data[f][0]; TTot = p.bc_data[f][1]; }
     if (bc == kBCWallIsothermal) T = p.bc_data[f][0];
+    if (bc == kBCInflowSupersonic) { rhoIn = p.bc_data[f][0]; pOut = p.bc_data[f][4]; }
   }
   r.p_ref = pOut > 0.0 ? pOut : pTot > 0.0 ? pTot : -1.0;
   r.rho_ref = rhoIn > 0.0 ? rhoIn : -1.0;
@@ -516,7 +517,14 @@ std::string setup_open_boundary(const CaseParams &p, const Tables &t, const Geom
       if (!(d[0] > 0.0)) return "isothermal wall needs a positive temperature";
       o[0] = d[0];
       o[1] = -rc.p_ref / (kRGas * rc.rho_ref);   // the wall state's V4 is o[1] / o[0]
-    } else if (bc != kBCWallInviscid) {
+    } else if (bc == kBCInflowSupersonic) {
+      // the prescribed state as entropy variables (SubfaceClasses.cpp:1026-1106)
+      const double rho = d[0] / rc.rho_ref, u = d[1] / rc.u_ref, v = d[2] / rc.u_ref, w = d[3] / rc.u_ref;
+      const double pr = d[4] / rc.p_ref;
+      const double sEnt = std::log(pr / std::pow(rho, kGamma)), pInv = 1.0 / pr;
+      o[0] = (kGamma - sEnt) / (kGamma - 1.0) - 0.5 * pInv * rho * (u * u + v * v + w * w);
+      o[1] = rho * u * pInv; o[2] = rho * v * pInv; o[3] = rho * w * pInv; o[4] = -rho * pInv;
+    } else if (bc != kBCWallInviscid && bc != kBCOutflowSupersonic) {
       return "unsupported boundary condition";
     }
     int first = -1;

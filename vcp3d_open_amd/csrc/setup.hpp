@@ -18,7 +18,7 @@ constexpr double kMuDim = 1.716e-5;       // InputParamClass.cpp:607
 // Physical block-face kinds (one subface per block face).
 enum : int { kBCPeriodic = 0, kBCOutflowNSCBC = 1, kBCInflowStaticNSCBC = 2, kBCInflowTotalNSCBC = 3,
               kBCInflowSubsonic = 4, kBCOutflowSubsonic = 5, kBCWallHeatFlux = 6, kBCWallIsothermal = 7,
-              kBCWallInviscid = 8 };
+              kBCWallInviscid = 8, kBCInflowSupersonic = 9, kBCOutflowSupersonic = 10 };
 
 inline bool bc_is_nscbc(int bc) { return bc >= kBCOutflowNSCBC && bc <= kBCInflowTotalNSCBC; }
 inline bool bc_is_wall(int bc) { return bc >= kBCWallHeatFlux && bc <= kBCWallInviscid; }

@@ -31,9 +31,9 @@ the pressure pulse while the time step is 0 (ElementClass.cpp:515-665).
 
 Only what the GPU path supports is accepted (entropy variables, Ismail-Roe, no
 SGS or wall model, no body force; block faces periodic, PC-NSCBC in/outflow,
-subsonic total-condition inflow, subsonic outflow, heat-flux, isothermal or
-inviscid walls, one subface each); anything else raises DeckError naming the
-setting.
+subsonic total-condition inflow, subsonic outflow, supersonic inflow and
+outflow, heat-flux, isothermal or inviscid walls, one subface each); anything
+else raises DeckError naming the setting.
 """
 from __future__ import annotations
 
@@ -47,12 +47,14 @@ from .cases import GAMMA, MU_DIM, R_GAS
 
 BC_CODES = {"periodic1to1trans": 0, "bcoutflownscbc": 1, "bcinflowstaticnscbc": 2,
             "bcinflowtotalnscbc": 3, "bcinflowsubsonic": 4, "bcoutflowsubsonic": 5,
-            "bcwallheatflux": 6, "bcwallisothermal": 7, "bcwallinviscid": 8}
+            "bcwallheatflux": 6, "bcwallisothermal": 7, "bcwallinviscid": 8,
+            "bcinflowsupersonic": 9, "bcoutflowsupersonic": 10}
 # prescribed data per boundary condition, in the order of vcp_case_desc.bc_data
 _BC_DATA = {1: ["Pressure"], 2: ["Density", "VelocityX", "VelocityY", "VelocityZ"],
             3: ["TotalPressure", "TotalTemperature", "VelocityDirX", "VelocityDirY", "VelocityDirZ"],
             4: ["TotalPressure", "TotalTemperature", "VelocityDirX", "VelocityDirY", "VelocityDirZ"],
-            5: ["Pressure"], 6: ["HeatFlux"], 7: ["Temperature"], 8: []}
+            5: ["Pressure"], 6: ["HeatFlux"], 7: ["Temperature"], 8: [],
+            9: ["Density", "VelocityX", "VelocityY", "VelocityZ", "Pressure"], 10: []}
 _KNOWN_BC = ("internal1to1", "periodic1to1trans", "bcfarfield", "bcwallisothermal",
              "bcwallisothermalwm", "bcwallheatflux", "bcwallheatfluxwm", "bcwallinviscid",
              "bcsymmetryplane", "bcinflowsubsonic", "bcinflowsupersonic", "bcoutflowsubsonic",

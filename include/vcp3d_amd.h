@@ -95,6 +95,9 @@ typedef struct vcp_case_desc {
      3775-3783); reference_area <= 0 means the reference's default 1 */
   int    monitor_forces;
   double reference_area;
+  /* "Compute average solution" (InputParamClass.cpp:365): running averages
+     updated after every synchronization step (SolverClass.cpp:2415-2421) */
+  int    compute_average;
 } vcp_case_desc;
 
 /* Block-face boundary conditions ("Subface information",
@@ -164,6 +167,17 @@ int vcp_sync_step(vcp_ctx *ctx, double monitoring[8]);
    step. Results are identical to the launch-by-launch path (enable = 0, the
    default). */
 int vcp_set_graph(vcp_ctx *ctx, int enable);
+
+/* The running averages (ElementClass::mAvePrim, mAveVelProd, mAveEddyVis;
+   SolverClass::UpdateAverageData, SolverClass.cpp:3800-3850,
+   ElementClass.cpp:3741-3804) of a case created with compute_average:
+   avg [nElem][nDOF][12] dimensional (rho, u, v, w, p, uu, vv, ww, uv, uw, vw,
+   eddy viscosity = 0), the averaged columns of the reference's restart file;
+   *n_steps the number of synchronization steps averaged. Set: to continue
+   averaging from a restart ("Continue averaging from restart",
+   SolverClass.cpp:3227-3240). */
+int vcp_get_averages(vcp_ctx *ctx, double *avg, int *n_steps);
+int vcp_set_averages(vcp_ctx *ctx, const double *avg, int n_steps);
 
 /* ElementClass::ComputeTimeStep of every element (ElementClass.cpp:1522-1644). */
 int vcp_time_steps(vcp_ctx *ctx, double *dt_elem);

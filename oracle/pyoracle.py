@@ -40,6 +40,7 @@ class OrcParams(C.Structure):
         ("sponge_elems", C.c_int * 6),
         ("damping_constant", C.c_double * 6), ("damping_exponent", C.c_double * 6),
         ("monitor_forces", C.c_int), ("reference_area", C.c_double),
+        ("compute_average", C.c_int),
     ]
 
 
@@ -102,6 +103,7 @@ def lib():
         L.orc_get_dudv.argtypes = [C.c_void_p, C.c_int, dp]
         L.orc_set_damping_state.argtypes = [C.c_void_p, dp]
         L.orc_nscbc_average.argtypes = [C.c_void_p, dp]
+        L.orc_get_averages.argtypes = [C.c_void_p, dp, C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -200,6 +202,14 @@ class Oracle:
         x = np.zeros((5, self.n_dofs))
         it = self._check(lib().orc_cg_solve(self.h, e, np.ascontiguousarray(rhs, dtype=np.float64), x))
         return x, it
+
+    def averages(self):
+        """Running averages [nElem][nDOFs][12] and the number of averaged steps."""
+        out = np.zeros((self.n_elem, self.n_dofs, 12))
+        n = C.c_int()
+        if lib().orc_get_averages(self.h, out, C.byref(n)) != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        return out, n.value
 
     def dudv(self, e):
         out = np.zeros((15, self.M ** 3))

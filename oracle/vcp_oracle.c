@@ -78,6 +78,8 @@ struct orc_case {
   int resValid, timeStep;
   int monNow;             /* the residual being evaluated computes monitoring data */
   double *forceElem;      /* [nElem][6] wall force sums of the element's faces */
+  double *avg;            /* [nElem][nDOFs][12] running averages (compute_average) */
+  int nAvg;
   /* per-thread scratch */
   double **works;
   int nthreads;
@@ -1659,6 +1661,7 @@ orc_case *orc_create(const orc_params *prm, const double *elem_grid)
   c->sol = dalloc(ns); c->solOld = dalloc(ns); c->res = dalloc(ns);
   for (int d = 0; d < 3; ++d) c->resMin[d] = dalloc(ns);
   c->forceElem = dalloc((size_t) c->nElem * 6);
+  c->avg = dalloc(c->prm.compute_average ? (size_t) c->nElem * c->nDOFs * 12 : 1);
   c->dUdV = dalloc((size_t) c->nElem * 15 * c->nInt);
   c->cgIt = (int *) calloc(c->nElem, sizeof(int));
   return c;
@@ -1671,7 +1674,7 @@ void orc_destroy(orc_case *c)
                     c->FMin, c->FMax, c->dFMin, c->dFMax, c->w3, c->w2, c->basisCenter, c->vm,
                     c->sm[0], c->sm[1], c->sm[2], c->sm[3], c->sm[4], c->sm[5], c->len, c->aveNorm,
                     c->sol, c->solOld, c->res, c->resMin[0], c->resMin[1], c->resMin[2], c->dUdV,
-                    c->Lq, c->dLq, c->PC, c->dampFn, c->dampState, c->matchScale, c->forceElem};
+                    c->Lq, c->dLq, c->PC, c->dampFn, c->dampState, c->matchScale, c->forceElem, c->avg};
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
   if (c->works) for (int t = 0; t < c->nthreads; ++t) free(c->works[t]);
   free(c->works);
@@ -1875,7 +1878,38 @@ int orc_sync_step(orc_case *c, double md[8])
     }
   }
   free(dte);
+  if (c->prm.compute_average) {
+    /* SolverClass.cpp:2415-2421 and ElementClass::UpdateAverageData
+       (ElementClass.cpp:3741-3782): nodal primitive variables
+       (ComputePrimitiveVariablesNodalDOFs, :1500-1517, dimensional) */
+    ++c->nAvg;
+    const double factNew = 1.0 / c->nAvg, factOld = 1.0 - factNew;
+    const int K = c->K, nD = c->nDOFs;
+    double *node = dalloc(5 * (size_t) nD);
+    for (int e = 0; e < c->nElem; ++e) {
+      for (int l = 0; l < 5; ++l)
+        tp3(K, K, K, K, K, K, c->LD, c->LD, c->LD, c->sol + ((size_t) e * 5 + l) * nD, node + l * nD, 0, W(c));
+      for (int d = 0; d < nD; ++d) {
+        double V[5], rho, u, v, w, p, eKin;
+        for (int l = 0; l < 5; ++l) V[l] = node[l * nD + d];
+        ent2prim(V, &rho, &u, &v, &w, &p, &eKin);
+        rho *= c->rhoRef; u *= c->uRef; v *= c->uRef; w *= c->uRef; p *= c->pRef;
+        const double q[11] = {rho, u, v, w, p, u * u, v * v, w * w, u * v, u * w, v * w};
+        double *a = c->avg + ((size_t) e * nD + d) * 12;
+        for (int i = 0; i < 11; ++i) a[i] = factOld * a[i] + factNew * q[i];
+      }
+    }
+    free(node);
+  }
   return nSteps;
+}
+
+int orc_get_averages(const orc_case *c, double *avg, int *n_steps)
+{
+  if (!c->prm.compute_average) return fail("the case was created without compute_average");
+  memcpy(avg, c->avg, sizeof(double) * (size_t) c->nElem * c->nDOFs * 12);
+  *n_steps = c->nAvg;
+  return 0;
 }
 
 void orc_cg_iterations(const orc_case *c, int *iters)

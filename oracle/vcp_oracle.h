@@ -58,6 +58,7 @@ typedef struct {
   /* "Monitor forces", "Reference area force coefficients" (<= 0: 1) */
   int    monitor_forces;
   double reference_area;
+  int    compute_average;  /* "Compute average solution" */
 } orc_params;
 
 enum { ORC_BC_PERIODIC = 0, ORC_BC_OUTFLOW_NSCBC = 1,
@@ -89,6 +90,9 @@ void orc_get_modal(const orc_case *c, double *modal);
 void orc_get_residual(const orc_case *c, double *res);
 /* nodal dimensional conservative [nElem][nDOFs][5] (restart writer). */
 void orc_get_nodal(const orc_case *c, double *nodal);
+/* running averages [nElem][nDOFs][12] (rho, u, v, w, p, uu, vv, ww, uv, uw,
+   vw, eddy viscosity), dimensional, and the number of averaged steps */
+int  orc_get_averages(const orc_case *c, double *avg, int *n_steps);
 
 /* SolverClass::Residual; returns 0 or <0 on error. */
 int  orc_residual(orc_case *c, int compute_monitoring, double monitoring[8]);

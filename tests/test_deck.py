@@ -189,5 +189,10 @@ def test_oracle_on_reference_duct_deck(oracle_lib):
     check_monitoring(md, g["monitoring"])
     sol = o.nodal()
     assert rel(sol[g["elements"]], g["sol_3"][..., :5]) < 1e-12
+    # running averages over the 3 synchronization steps (the deck computes
+    # them): the reference's averaged restart columns
+    avg, n = o.averages()
+    assert n == 3
+    assert rel(avg[g["elements"]], g["sol_3"][..., 5:17]) < 1e-12
     # per-variable norms as one vector (the transverse momenta are ~1e-6 of it)
     assert rel(np.sqrt((sol ** 2).sum(axis=(0, 1))), g["sol_3_norms"][:5]) < 1e-12

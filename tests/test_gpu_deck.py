@@ -65,6 +65,9 @@ def test_reference_duct_deck(lib):
     assert abs(md[6] - gold["monitoring"][6]) <= 1e-9 * gold["monitoring"][6]
     sol = s.WriteRestartSolution()
     assert rel(sol[gold["elements"]], gold["sol_3"][..., :5]) < 1e-12
+    avg, n = s.averages()
+    assert n == 3
+    assert rel(avg[gold["elements"]], gold["sol_3"][..., 5:17]) < 1e-12
     # per-variable norms as one vector (the transverse momenta are ~1e-6 of it)
     assert rel(np.sqrt((sol ** 2).sum(axis=(0, 1))), gold["sol_3_norms"][:5]) < 1e-12
 
@@ -89,3 +92,35 @@ def test_checkpoint_and_resume(lib, tmp_path):
     c, _ = open_solver(str(tmp_path / "param.cfg"))
     c.ExecuteTimeSynchronizationStep()
     assert rel(c.WriteRestartSolution(), a.WriteRestartSolution()) < 1e-12
+
+
+def test_checkpoint_and_resume_with_averages(lib, tmp_path):
+    """The duct deck (running averages on): two steps, a restart with the
+    averaged columns, a new solver with "Restart: yes" and "Continue averaging
+    from restart: yes", one more step: state and averages as after three
+    uninterrupted steps."""
+    src = os.path.join(GOLDEN, "ref_duct")
+    print("[case] duct checkpoint/resume with averages", flush=True)
+    deck = os.path.join(src, "extrapolation", "start.cfg")
+    a, case = open_solver(deck)
+    b, _ = open_solver(deck)
+    for _ in range(3):
+        a.ExecuteTimeSynchronizationStep()
+    for _ in range(2):
+        b.ExecuteTimeSynchronizationStep()
+    (tmp_path / "grid").mkdir()
+    (tmp_path / "grid" / "Box_nPoly5_15x8x8.xyz").write_bytes(open(case.grid_file, "rb").read())
+    run = tmp_path / "extrapolation"
+    (run / "init").mkdir(parents=True)
+    write_restart(b, case, str(run / "init" / "Restart.dat"), 2)
+    text = open(deck).read().replace("Restart: no", "Restart: yes")
+    text = text.replace("Continue averaging from restart: no", "Continue averaging from restart: yes")
+    (run / "start.cfg").write_text(text)
+    c, c_case = open_solver(str(run / "start.cfg"))
+    assert c_case.continue_averaging
+    c.ExecuteTimeSynchronizationStep()
+    assert rel(c.WriteRestartSolution(), a.WriteRestartSolution()) < 1e-12
+    avg_a, n_a = a.averages()
+    avg_c, n_c = c.averages()
+    assert n_a == n_c == 3
+    assert rel(avg_c, avg_a) < 1e-12

@@ -56,6 +56,7 @@ struct vcp_ctx {
   bool dt_ready = false;          // host[0] holds the time step of the current state
   bool damping_set = false;
   int time_step = 0;
+  int n_avg = 0;                  // synchronization steps in the running averages
   // RK steps replayed as captured HIP graphs (vcp_set_graph), one per
   // [state buffer that is current][last step of the call], captured on first use
   bool use_graph = false;
@@ -190,6 +191,7 @@ void params_from_desc(const vcp_case_desc *d, vcp::CaseParams &p)
     p.damping_exponent[f] = d->damping_exponent[f];
   }
   p.monitor_forces = d->monitor_forces;
+  p.compute_average = d->compute_average;
   p.reference_area = d->reference_area > 0.0 ? d->reference_area : 1.0;
   p.outlet_len = d->outlet_len; p.outlet_sigma = d->outlet_sigma;
   p.outlet_beta_l = d->outlet_beta_l; p.outlet_beta_t = d->outlet_beta_t;
@@ -340,6 +342,10 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
   if (he == hipSuccess) he = c->alloc(&dc.cgIter, (size_t) nElem);
   if (he == hipSuccess) he = c->alloc(&dc.dtElem, (size_t) nElem);
   if (he == hipSuccess) he = c->alloc(&dc.mach2Elem, (size_t) nElem);
+  if (he == hipSuccess && p.compute_average) {
+    he = c->alloc(&dc.avg, (size_t) nElem * 11 * dc.nDOF);
+    if (he == hipSuccess) he = hipMemset(dc.avg, 0, (size_t) nElem * 11 * dc.nDOF * sizeof(double));
+  }
   if (he == hipSuccess && dc.nBFace) {
     he = c->alloc(&dc.bface, ob.faces.size());
     if (he == hipSuccess) he = c->alloc(&dc.bgeo, bgeo.size());
@@ -567,6 +573,11 @@ int vcp_sync_step(vcp_ctx *c, double md[8])
     elapsed += dt;
     std::swap(c->dc.U, c->dc.U0);   // the step's state becomes U0, stage 0 writes U
     if ((rc = c->use_graph ? rk_graph(c, reached, dt) : rk_stages(c, reached, &dt))) return rc;
+    if (reached && c->dc.avg) {
+      // running averages of the new state (SolverClass.cpp:2415-2421)
+      ++c->n_avg;
+      HIP_TRY(vcp::launch_average(c->dc, 1.0 / c->n_avg, c->stream));
+    }
     if (reached) {
       // the next call's time step with the monitoring maximum and the error
       // flags in one round trip
@@ -583,6 +594,40 @@ int vcp_sync_step(vcp_ctx *c, double md[8])
   }
   c->res_valid = true;
   return nSteps;
+}
+
+int vcp_get_averages(vcp_ctx *c, double *avg, int *n_steps)
+{
+  if (!c || !avg || !n_steps) return fail("null argument");
+  if (!c->dc.avg) return fail("the case was created without compute_average");
+  const int nD = c->dc.nDOF;
+  std::vector<double> buf((size_t) c->dc.nElem * 11 * nD);
+  HIP_TRY(hipMemcpyAsync(buf.data(), c->dc.avg, buf.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int e = 0; e < c->dc.nElem; ++e)
+    for (int d = 0; d < nD; ++d) {
+      double *o = avg + ((size_t) e * nD + d) * 12;
+      for (int i = 0; i < 11; ++i) o[i] = buf[((size_t) e * 11 + i) * nD + d];
+      o[11] = 0.0;   // no SGS model: the averaged eddy viscosity is 0
+    }
+  *n_steps = c->n_avg;
+  return 0;
+}
+
+int vcp_set_averages(vcp_ctx *c, const double *avg, int n_steps)
+{
+  if (!c || !avg) return fail("null argument");
+  if (!c->dc.avg) return fail("the case was created without compute_average");
+  if (n_steps < 0) return fail("negative number of averaged steps");
+  const int nD = c->dc.nDOF;
+  std::vector<double> buf((size_t) c->dc.nElem * 11 * nD);
+  for (int e = 0; e < c->dc.nElem; ++e)
+    for (int d = 0; d < nD; ++d)
+      for (int i = 0; i < 11; ++i) buf[((size_t) e * 11 + i) * nD + d] = avg[((size_t) e * nD + d) * 12 + i];
+  HIP_TRY(hipMemcpyAsync(c->dc.avg, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->n_avg = n_steps;
+  return 0;
 }
 
 int vcp_set_graph(vcp_ctx *c, int enable)

@@ -1869,6 +1869,46 @@ __global__ __launch_bounds__(64) void time_step_kernel(DtArgs a)
   }
 }
 
+// Running averages of the dimensional primitive variables and the velocity
+// products at the nodal DOFs (ElementClass::UpdateAverageData,
+// ElementClass.cpp:3741-3782): avg = factOld avg + factNew q, with q =
+// (rho, u, v, w, p, uu, vv, ww, uv, uw, vw), layout [nElem][11][nDOF].
+struct AvgArgs {
+  const double *U, *tables;
+  double *avg;
+  TableOffsets to;
+  double u_ref, p_ref, rho_ref, factNew, factOld;
+};
+
+template <int K>
+__global__ __launch_bounds__(64) void average_kernel(AvgArgs a)
+{
+  constexpr int ND = K * K * K;
+  __shared__ double sL[K * K], sU[5 * ND];
+  const int e = blockIdx.x, tid = threadIdx.x;
+  for (int i = tid; i < K * K; i += 64) sL[i] = a.tables[a.to.LD + i];
+  for (int i = tid; i < 5 * ND; i += 64) sU[i] = a.U[(size_t) e * 5 * ND + i];
+  __syncthreads();
+  lines_inplace<K, 1>(sU, sL, tid);
+  __syncthreads();
+  lines_inplace<K, K>(sU, sL, tid);
+  __syncthreads();
+  lines_inplace<K, K * K>(sU, sL, tid);
+  __syncthreads();
+  double *av = a.avg + (size_t) e * 11 * ND;
+  for (int d = tid; d < ND; d += 64) {
+    double V[5];
+#pragma unroll
+    for (int l = 0; l < 5; ++l) V[l] = sU[l * ND + d];
+    double rho, u, v, w, p, ek;
+    ent2prim(V, rho, u, v, w, p, ek);
+    rho *= a.rho_ref; u *= a.u_ref; v *= a.u_ref; w *= a.u_ref; p *= a.p_ref;
+    const double q[11] = {rho, u, v, w, p, u * u, v * v, w * w, u * v, u * w, v * w};
+#pragma unroll
+    for (int i = 0; i < 11; ++i) av[i * ND + d] = a.factOld * av[i * ND + d] + a.factNew * q[i];
+  }
+}
+
 // Minimum of the per-element time steps into scal[0] (one workgroup; a
 // fixed reduction order, no atomics on one address from every element).
 __global__ __launch_bounds__(1024) void dt_min_kernel(const double *dtElem, int n, double *scal)
@@ -2057,6 +2097,29 @@ hipError_t launch_time_step(const DeviceCase &dc, hipStream_t s)
     case 5: return dt_impl<5>(dc, s);
     case 6: return dt_impl<6>(dc, s);
     case 7: return dt_impl<7>(dc, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <int K>
+hipError_t avg_impl(const DeviceCase &dc, double factNew, hipStream_t s)
+{
+  AvgArgs a;
+  a.U = dc.U; a.tables = dc.tables; a.avg = dc.avg; a.to = dc.to;
+  a.u_ref = dc.u_ref; a.p_ref = dc.p_ref; a.rho_ref = dc.rho_ref;
+  a.factNew = factNew; a.factOld = 1.0 - factNew;
+  hipLaunchKernelGGL((average_kernel<K>), dim3(dc.nElem), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_average(const DeviceCase &dc, double factNew, hipStream_t s)
+{
+  switch (dc.K) {
+    case 3: return avg_impl<3>(dc, factNew, s);
+    case 4: return avg_impl<4>(dc, factNew, s);
+    case 5: return avg_impl<5>(dc, factNew, s);
+    case 6: return avg_impl<6>(dc, factNew, s);
+    case 7: return avg_impl<7>(dc, factNew, s);
   }
   return hipErrorInvalidValue;
 }

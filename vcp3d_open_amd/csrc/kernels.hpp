@@ -65,6 +65,7 @@ struct DeviceCase {
   double *bgeo = nullptr;         // [nBFace][13][nInt2D]: the face's normal, area factor, inverse metric
   double *machPart = nullptr;     // [nBFace] weighted face-normal Mach number
   double *forcePart = nullptr;    // [nBFace][6] wall forces (pressure xyz, viscous xyz)
+  double *avg = nullptr;          // [nElem][11][nDOF] running averages (null: not computed)
   int monitor_forces = 0;
   NscbcParams nscbc{};
   // sponge layer / characteristic matching layer
@@ -93,6 +94,8 @@ hipError_t launch_faces(const DeviceCase &dc, hipStream_t s);
 hipError_t launch_boundary(const DeviceCase &dc, hipStream_t s, bool monitor = false);
 hipError_t launch_volume(const DeviceCase &dc, const StageCtl &ctl, hipStream_t s);
 hipError_t launch_time_step(const DeviceCase &dc, hipStream_t s);
+// running averages avg = (1 - factNew) avg + factNew q of the current state
+hipError_t launch_average(const DeviceCase &dc, double factNew, hipStream_t s);
 // scal[kScalDt] = dt, ordered on the stream
 hipError_t launch_set_dt(const DeviceCase &dc, double dt, hipStream_t s);
 // U = U0 - dt R: with dt given, the kernel also stores it to scal[kScalDt];

@@ -45,6 +45,7 @@ struct CaseParams {
   int sponge_elems[6] = {0, 0, 0, 0, 0, 0};
   double damping_constant[6] = {}, damping_exponent[6] = {};
   int monitor_forces = 0;         // "Monitor forces"
+  int compute_average = 0;        // "Compute average solution"
   double reference_area = 1.0;    // "Reference area force coefficients"
 };
 

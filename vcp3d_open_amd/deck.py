@@ -231,6 +231,9 @@ class DeckCase:
         self.n_sync_steps = prm.integer("number of synchronization time steps", -1)
         self.theta_sym = prm.number("theta parameter symmetrizing terms", 1.0)
         self.monitor_forces = prm.yes("monitor forces", True)
+        self.compute_average = prm.yes("compute average solution", False)
+        # InputParamClass.cpp:599: no continuation without averaging
+        self.continue_averaging = self.compute_average and prm.yes("continue averaging from restart", False)
         self.reference_area = prm.number("reference area force coefficients", 1.0)
         self.rel_penalty = prm.number("relative penalty parameter", 1.0)
         self.grid_file = os.path.join(self.dir, prm.text("grid file name"))
@@ -354,7 +357,8 @@ class DeckCase:
                  reynolds_length=self.reynolds_length, temp_free_stream=self.temp,
                  theta_sym=self.theta_sym, rel_penalty=self.rel_penalty, cfl=self.cfl,
                  dt_synchr=self.dt_synchr, bc=self.bc, bc_data=self.bc_data,
-                 monitor_forces=int(self.monitor_forces), reference_area=self.reference_area, **self.nscbc)
+                 monitor_forces=int(self.monitor_forces), reference_area=self.reference_area,
+                 compute_average=int(self.compute_average), **self.nscbc)
         if self.sponge_layer or self.cml:
             d.update(sponge_layer=int(self.sponge_layer), cml=int(self.cml),
                      sponge_elems=[r[0] for r in self.sponge_rows],
@@ -495,11 +499,31 @@ def open_solver(param_path: str, bug_compat: bool = True, **kw):
     s.ReadRestartSolution(case.initial_conservative())
     if case.sponge_layer or case.cml:
         s.SetDampingState(case.damping_primitive())
+    if case.continue_averaging and case.restart:
+        # the averaged columns and the number of averaged steps of the restart
+        # (SolverClass.cpp:3227-3240, :4401-4414)
+        _, data, n = vio.read_su2_restart(case.restart_file, case.n_dofs)
+        if data.shape[2] < 20:
+            raise DeckError("restart file without averaged columns for \"Continue averaging from restart\"")
+        s.set_averages(data[:, :, 8:20], n)
     return s, case
+
+
+AVERAGE_NAMES = ["Density_Average", "VelocityX_Average", "VelocityY_Average", "VelocityZ_Average",
+                 "Pressure_Average", "VelocityX_VelocityX_Average", "VelocityY_VelocityY_Average",
+                 "VelocityZ_VelocityZ_Average", "VelocityX_VelocityY_Average", "VelocityX_VelocityZ_Average",
+                 "VelocityY_VelocityZ_Average", "EddyVis_Average"]
 
 
 def write_restart(solver, case, path: str, time_step: int) -> None:
     """SolverClass::WriteRestartSolution (SolverClass.cpp:4375-4804): the
     nodal dimensional conservative state with the DOF coordinates as an SU2
-    restart file the reference can read back."""
-    vio.write_su2_restart(path, case.nodal_coords(), solver.WriteRestartSolution(), time_step=time_step)
+    restart file the reference can read back; with running averages their
+    twelve columns follow and the stored step count is the number of
+    averaged steps, as the reference writes it (SolverClass.cpp:4401-4414)."""
+    extra = names = None
+    if getattr(case, "compute_average", False):
+        extra, time_step = solver.averages()
+        names = AVERAGE_NAMES
+    vio.write_su2_restart(path, case.nodal_coords(), solver.WriteRestartSolution(), time_step=time_step,
+                          extra=extra, extra_names=names)

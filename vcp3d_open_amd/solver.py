@@ -42,13 +42,15 @@ class CaseDesc(C.Structure):
         ("sponge_elems", C.c_int * 6),
         ("damping_constant", C.c_double * 6), ("damping_exponent", C.c_double * 6),
         ("monitor_forces", C.c_int), ("reference_area", C.c_double),
+        ("compute_average", C.c_int),
     ]
 
 
 EXPORTS = ["vcp_create", "vcp_destroy", "vcp_sizes", "vcp_reference", "vcp_set_restart",
            "vcp_set_damping_state", "vcp_nscbc_average",
            "vcp_set_solution", "vcp_get_solution", "vcp_get_nodal", "vcp_residual",
-           "vcp_get_residual", "vcp_sync_step", "vcp_set_graph", "vcp_time_steps", "vcp_cg_iterations",
+           "vcp_get_residual", "vcp_sync_step", "vcp_set_graph", "vcp_get_averages", "vcp_set_averages",
+           "vcp_time_steps", "vcp_cg_iterations",
            "vcp_profile_residual", "vcp_host_setup", "vcp_last_error"]
 
 _lib = None
@@ -79,6 +81,8 @@ def load_library(path: str = LIB_PATH):
     L.vcp_get_residual.argtypes = [vp, dp]
     L.vcp_sync_step.argtypes = [vp, dp]
     L.vcp_set_graph.argtypes = [vp, C.c_int]
+    L.vcp_get_averages.argtypes = [vp, dp, C.POINTER(C.c_int)]
+    L.vcp_set_averages.argtypes = [vp, dp, C.c_int]
     L.vcp_time_steps.argtypes = [vp, dp]
     L.vcp_cg_iterations.argtypes = [vp, ip]
     L.vcp_profile_residual.argtypes = [vp, C.c_int, dp]
@@ -205,6 +209,17 @@ class SolverClass:
         out = np.zeros(4)
         self._check(self._L.vcp_reference(self.h, out))
         return dict(u_ref=out[0], p_ref=out[1], rho_ref=out[2], mu=out[3])
+
+    def averages(self):
+        """Running averages [nElem][nDOFs][12] (the averaged columns of the
+        reference's restart file) and the number of averaged steps."""
+        out = np.zeros((self.n_elem, self.n_dofs, 12))
+        n = C.c_int()
+        self._check(self._L.vcp_get_averages(self.h, out, C.byref(n)))
+        return out, n.value
+
+    def set_averages(self, avg: np.ndarray, n_steps: int) -> None:
+        self._check(self._L.vcp_set_averages(self.h, np.ascontiguousarray(avg, dtype=np.float64), int(n_steps)))
 
     def set_graph(self, enable: bool) -> None:
         """Replay each RK step of ExecuteTimeSynchronizationStep as a captured

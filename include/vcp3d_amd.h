@@ -98,17 +98,21 @@ typedef struct vcp_case_desc {
   /* "Compute average solution" (InputParamClass.cpp:365): running averages
      updated after every synchronization step (SolverClass.cpp:2415-2421) */
   int    compute_average;
+  /* "Free stream velocity direction" (InputParamClass.cpp:214-216): the
+     farfield state when the prescribed data do not fix the free stream
+     (InputParamClass.cpp:991-1191) */
+  double free_stream_dir[3];
 } vcp_case_desc;
 
 /* Block-face boundary conditions ("Subface information",
    InputParamClass.cpp:1530-1548): BCOutflowNSCBC, BCInflowStaticNSCBC,
    BCInflowTotalNSCBC, BCInflowSubsonic (total conditions), BCOutflowSubsonic,
    BCWallHeatFlux, BCWallIsothermal, BCWallInviscid, BCInflowSupersonic,
-   BCOutflowSupersonic. */
+   BCOutflowSupersonic, BCFarfield (the free stream). */
 enum { VCP_BC_PERIODIC = 0, VCP_BC_OUTFLOW_NSCBC = 1, VCP_BC_INFLOW_STATIC_NSCBC = 2,
        VCP_BC_INFLOW_TOTAL_NSCBC = 3, VCP_BC_INFLOW_SUBSONIC = 4, VCP_BC_OUTFLOW_SUBSONIC = 5,
        VCP_BC_WALL_HEATFLUX = 6, VCP_BC_WALL_ISOTHERMAL = 7, VCP_BC_WALL_INVISCID = 8,
-       VCP_BC_INFLOW_SUPERSONIC = 9, VCP_BC_OUTFLOW_SUPERSONIC = 10 };
+       VCP_BC_INFLOW_SUPERSONIC = 9, VCP_BC_OUTFLOW_SUPERSONIC = 10, VCP_BC_FARFIELD = 11 };
 
 typedef struct vcp_ctx vcp_ctx;
 

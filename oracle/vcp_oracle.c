@@ -79,6 +79,7 @@ struct orc_case {
   int monNow;             /* the residual being evaluated computes monitoring data */
   double *forceElem;      /* [nElem][6] wall force sums of the element's faces */
   double *avg;            /* [nElem][nDOFs][12] running averages (compute_average) */
+  int refFallback;        /* pRef, rhoRef from the free-stream data of the parameter file */
   int nAvg;
   /* per-thread scratch */
   double **works;
@@ -1266,7 +1267,7 @@ static void pointwise_state(const orc_case *c, int bc, const double *m, double f
       R[0] = (GAM - sR) * ovgm1 - 0.5 * pRInv * rhoR * qR2;
       R[1] = rhoR * pRInv * qR * d[2]; R[2] = rhoR * pRInv * qR * d[3]; R[3] = rhoR * pRInv * qR * d[4];
       R[4] = -rhoR * pRInv;
-    } else if (bc == ORC_BC_INFLOW_SUPERSONIC) {   /* BoundaryState.cpp:605-635 */
+    } else if (bc == ORC_BC_INFLOW_SUPERSONIC || bc == ORC_BC_FARFIELD) {   /* BoundaryState.cpp:605-635, :124-150 */
       for (int q = 0; q < 5; ++q) R[q] = d[q];
     } else if (bc == ORC_BC_OUTFLOW_SUPERSONIC) {  /* BoundaryState.cpp:793-818 */
       for (int q = 0; q < 5; ++q) R[q] = V[q];
@@ -1443,7 +1444,7 @@ static int is_nscbc(int bc)
   return bc == ORC_BC_OUTFLOW_NSCBC || bc == ORC_BC_INFLOW_STATIC_NSCBC || bc == ORC_BC_INFLOW_TOTAL_NSCBC;
 }
 
-static int is_physical(int bc) { return bc >= ORC_BC_OUTFLOW_NSCBC && bc <= ORC_BC_OUTFLOW_SUPERSONIC; }
+static int is_physical(int bc) { return bc >= ORC_BC_OUTFLOW_NSCBC && bc <= ORC_BC_FARFIELD; }
 
 static int check_params(const orc_case *c)
 {
@@ -1466,6 +1467,72 @@ static int check_params(const orc_case *c)
     if (!any) return fail("No sponge layer elements have been detected in a sponge layer simulation.");
   }
   return 0;
+}
+
+/* InputParamClass::DetermineNonDimPrimVarFreeStream (InputParamClass.cpp:
+   991-1191): non-dimensional (rho, u, v, w, p) of the free stream, from the
+   prescribed data when they fix it, else from the free-stream data of the
+   parameter file (the direction normalised only when the reference
+   conditions came from those data, InputParamClass.cpp:697-707). */
+static void free_stream_nondim(const orc_case *c, double q[5])
+{
+  const orc_params *P = &c->prm;
+  const double big = 1e300, gm1 = GAM - 1.0;
+  double pr = -1.0, pTot = -1.0, rho = -1.0, T = -1.0, TTot = -1.0;
+  double vel[3] = {big, big, big}, dir[3] = {big, big, big};
+  for (int f = 0; f < 6; ++f) {
+    if (P->periodic[f / 2]) continue;
+    const int bc = P->bc[f];
+    const double *d = P->bc_data[f];
+    if (bc == ORC_BC_OUTFLOW_NSCBC || bc == ORC_BC_OUTFLOW_SUBSONIC) pr = d[0];
+    if (bc == ORC_BC_INFLOW_STATIC_NSCBC) { rho = d[0]; vel[0] = d[1]; vel[1] = d[2]; vel[2] = d[3]; }
+    if (bc == ORC_BC_INFLOW_TOTAL_NSCBC || bc == ORC_BC_INFLOW_SUBSONIC) {
+      pTot = d[0]; TTot = d[1]; dir[0] = d[2]; dir[1] = d[3]; dir[2] = d[4];
+    }
+    if (bc == ORC_BC_INFLOW_SUPERSONIC) { rho = d[0]; vel[0] = d[1]; vel[1] = d[2]; vel[2] = d[3]; pr = d[4]; }
+    if (bc == ORC_BC_WALL_ISOTHERMAL) T = d[0];
+  }
+  int found = 0;
+  if (vel[0] < big && vel[1] < big && vel[2] < big) {
+    const int n = (pr > 0.0) + (rho > 0.0) + (T > 0.0);
+    if (n >= 2) {
+      if (pr <= 0.0) pr = rho * RGAS * T;
+      if (rho <= 0.0) rho = pr / (RGAS * T);
+      found = 1;
+    } else if (n == 1 && T <= 0.0) {
+      const double vm2 = vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2];
+      if (pr <= 0.0) pr = 0.95 * rho * vm2 / GAM; else rho = GAM * pr / (0.95 * vm2);
+      found = 1;
+    }
+    if (found) { q[0] = rho; q[1] = vel[0]; q[2] = vel[1]; q[3] = vel[2]; q[4] = pr; }
+  } else if (dir[0] < big && dir[1] < big && dir[2] < big) {
+    const double inv = 1.0 / sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    double M2 = 0.95 * 0.95;
+    if (pTot > 0.0 && pr > 0.0) M2 = (pow(pTot / pr, gm1 / GAM) - 1.0) * 2.0 / gm1;
+    else if (TTot > 0.0 && T > 0.0) M2 = (TTot / T - 1.0) * 2.0 / gm1;
+    if ((pTot > 0.0) + (TTot > 0.0) >= 2) {
+      const double rhoTot = pTot / (RGAS * TTot);
+      const double tmp = 1.0 + 0.5 * gm1 * M2;
+      pr = pTot / pow(tmp, GAM / gm1);
+      rho = rhoTot / pow(tmp, 1.0 / gm1);
+      const double vm = sqrt(M2 * GAM * pr / rho);
+      q[0] = rho; q[1] = dir[0] * inv * vm; q[2] = dir[1] * inv * vm; q[3] = dir[2] * inv * vm; q[4] = pr;
+      found = 1;
+    }
+  }
+  if (!found) {
+    double dv[3] = {P->free_stream_dir[0], P->free_stream_dir[1], P->free_stream_dir[2]};
+    if (c->refFallback) {
+      const double inv = 1.0 / sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+      for (int i = 0; i < 3; ++i) dv[i] *= inv;
+    }
+    const double vInf = P->mach * sqrt(GAM * RGAS * P->temp_free_stream);
+    const double muDim = c->mu * c->pRef / c->uRef;
+    const double rhoInf = P->reynolds * muDim / (P->reynolds_length * vInf);
+    q[0] = rhoInf; q[1] = vInf * dv[0]; q[2] = vInf * dv[1]; q[3] = vInf * dv[2];
+    q[4] = rhoInf * RGAS * P->temp_free_stream;
+  }
+  q[0] /= c->rhoRef; q[1] /= c->uRef; q[2] /= c->uRef; q[3] /= c->uRef; q[4] /= c->pRef;
 }
 
 /* InputParamClass::ComputeReferenceConditions (InputParamClass.cpp:605-758):
@@ -1497,7 +1564,9 @@ static void reference_conditions(orc_case *c)
     const double TRef = T > 0.0 ? T : TTot;
     if (TRef > 0.0) c->rhoRef = c->pRef / (RGAS * TRef);
   }
+  c->refFallback = 0;
   if (c->pRef < 0.0 || c->rhoRef < 0.0) {
+    c->refFallback = 1;
     const double vInf = q->mach * sqrt(GAM * RGAS * q->temp_free_stream);
     c->rhoRef = q->reynolds * muDim / (q->reynolds_length * vInf);
     c->pRef = c->rhoRef * RGAS * q->temp_free_stream;
@@ -1524,7 +1593,14 @@ static void reference_conditions(orc_case *c)
       o[2] = d[2] * inv; o[3] = d[3] * inv; o[4] = d[4] * inv;
     } else if (bc == ORC_BC_WALL_HEATFLUX) o[0] = d[0] / (c->pRef * c->uRef);
     else if (bc == ORC_BC_WALL_ISOTHERMAL) o[0] = d[0];
-    else if (bc == ORC_BC_INFLOW_SUPERSONIC) {
+    else if (bc == ORC_BC_FARFIELD) {
+      /* the free stream as entropy variables (SubfaceClasses.cpp:542-622) */
+      double fs[5];
+      free_stream_nondim(c, fs);
+      const double s = log(fs[4] / pow(fs[0], GAM)), pInv = 1.0 / fs[4];
+      o[0] = (GAM - s) / (GAM - 1.0) - 0.5 * pInv * fs[0] * (fs[1] * fs[1] + fs[2] * fs[2] + fs[3] * fs[3]);
+      o[1] = fs[0] * fs[1] * pInv; o[2] = fs[0] * fs[2] * pInv; o[3] = fs[0] * fs[3] * pInv; o[4] = -fs[0] * pInv;
+    } else if (bc == ORC_BC_INFLOW_SUPERSONIC) {
       /* the prescribed state as entropy variables (SubfaceClasses.cpp:1026-1106) */
       const double rho = d[0] / c->rhoRef, u = d[1] / c->uRef, v = d[2] / c->uRef, w = d[3] / c->uRef;
       const double pr = d[4] / c->pRef;

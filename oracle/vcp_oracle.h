@@ -59,6 +59,7 @@ typedef struct {
   int    monitor_forces;
   double reference_area;
   int    compute_average;  /* "Compute average solution" */
+  double free_stream_dir[3]; /* "Free stream velocity direction" */
 } orc_params;
 
 enum { ORC_BC_PERIODIC = 0, ORC_BC_OUTFLOW_NSCBC = 1,
@@ -66,7 +67,7 @@ enum { ORC_BC_PERIODIC = 0, ORC_BC_OUTFLOW_NSCBC = 1,
        ORC_BC_INFLOW_SUBSONIC = 4,      /* total conditions */
        ORC_BC_OUTFLOW_SUBSONIC = 5, ORC_BC_WALL_HEATFLUX = 6,
        ORC_BC_WALL_ISOTHERMAL = 7, ORC_BC_WALL_INVISCID = 8,
-       ORC_BC_INFLOW_SUPERSONIC = 9, ORC_BC_OUTFLOW_SUPERSONIC = 10 };
+       ORC_BC_INFLOW_SUPERSONIC = 9, ORC_BC_OUTFLOW_SUPERSONIC = 10, ORC_BC_FARFIELD = 11 };
 
 typedef struct orc_case orc_case;
 

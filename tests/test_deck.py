@@ -106,7 +106,7 @@ def test_restart_round_trip(tmp_path):
 
 @pytest.mark.parametrize("edit,msg", [
     (("FEM working variables: Entropy", "FEM working variables: Conservative"), "fem working variables"),
-    (("BCOutflowNSCBC  ", "BCFarfield"), "not supported"),
+    (("BCOutflowNSCBC  ", "BCSymmetryPlane"), "not supported"),
     (("BCOutflowNSCBC  ", "BCOutflowCharacteristic"), "unknown boundary condition"),
     (("Subgrid scale model: None", "Subgrid scale model: WALE"), "Subgrid"),
     ((" Fluctuations to add to initial solution: Pressure pulse",
@@ -141,13 +141,15 @@ def test_wall_decks_are_parsed():
     assert b.bc == [2, 1, 8, 6, 0, 0] and b.bc_data[3][0] == 2500.0
     c = DeckCase(os.path.join(GOLDEN, "ref_n3_8x4x4", "param_walls_c.cfg"))
     assert c.bc == [9, 10, 8, 8, 0, 0] and c.bc_data[0] == [1.110, 600.0, 20.0, 0.0, 101325.0]
+    e = DeckCase(os.path.join(GOLDEN, "ref_n3_8x4x4", "param_walls_e.cfg"))
+    assert e.bc == [11, 11, 11, 11, 0, 0] and e.description()["free_stream_dir"] == [1.0, 0.0, 0.0]
     d = DeckCase(os.path.join(GOLDEN, "ref_duct", "extrapolation", "start.cfg"))
     assert d.n == (15, 8, 8) and d.p == 5 and d.bc == [4, 1, 6, 6, 6, 6] and d.periodic == [0, 0, 0]
     assert d.pulse["direction"] == -1.0 and d.pulse["strength"] == 0.0
 
 
 @pytest.mark.parametrize("name", ["deck_n3_8x4x4_static", "deck_n3_8x4x4_total", "deck_walls_a", "deck_walls_b",
-                                  "deck_walls_c"])
+                                  "deck_walls_c", "deck_walls_d", "deck_walls_e"])
 def test_oracle_on_reference_decks(oracle_lib, name):
     """The oracle set up from the reference's own test deck (free stream from
     the boundary data plus the pressure pulse) against the reference run on

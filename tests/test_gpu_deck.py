@@ -32,7 +32,9 @@ def lib():
                                        ("deck_n3_8x4x4_total", "param_total.cfg"),
                                        ("deck_walls_a", "param_walls_a.cfg"),
                                        ("deck_walls_b", "param_walls_b.cfg"),
-                                       ("deck_walls_c", "param_walls_c.cfg")])
+                                       ("deck_walls_c", "param_walls_c.cfg"),
+                                       ("deck_walls_d", "param_walls_d.cfg"),
+                                       ("deck_walls_e", "param_walls_e.cfg")])
 def test_reference_deck_matches_reference(lib, name, deck):
     print(f"[case] {name}", flush=True)
     _, gold = load(name)

@@ -192,6 +192,7 @@ void params_from_desc(const vcp_case_desc *d, vcp::CaseParams &p)
   }
   p.monitor_forces = d->monitor_forces;
   p.compute_average = d->compute_average;
+  for (int i = 0; i < 3; ++i) p.free_stream_dir[i] = d->free_stream_dir[i];
   p.reference_area = d->reference_area > 0.0 ? d->reference_area : 1.0;
   p.outlet_len = d->outlet_len; p.outlet_sigma = d->outlet_sigma;
   p.outlet_beta_l = d->outlet_beta_l; p.outlet_beta_t = d->outlet_beta_t;

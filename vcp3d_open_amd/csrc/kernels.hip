@@ -37,7 +37,7 @@ namespace {
 
 enum : int { kBCOutflowNSCBC = 1, kBCInflowStaticNSCBC = 2, kBCInflowTotalNSCBC = 3, kBCInflowSubsonic = 4,
               kBCOutflowSubsonic = 5, kBCWallHeatFlux = 6, kBCWallIsothermal = 7, kBCWallInviscid = 8,
-              kBCInflowSupersonic = 9, kBCOutflowSupersonic = 10 };
+              kBCInflowSupersonic = 9, kBCOutflowSupersonic = 10, kBCFarfield = 11 };
 constexpr double kGam = 1.4;
 constexpr double kGm1 = 0.4;
 constexpr double kPrLam = 0.72;
@@ -1629,7 +1629,7 @@ __global__ __launch_bounds__(64) void nscbc_kernel(BndArgs<K, M> a)
         R[0] = (kGam - sR) * kInvGm1 - 0.5 * pRInv * rhoR * qR2;
         R[1] = rhoR * pRInv * qR * d[2]; R[2] = rhoR * pRInv * qR * d[3]; R[3] = rhoR * pRInv * qR * d[4];
         R[4] = -rhoR * pRInv;
-      } else if (bc == kBCInflowSupersonic) {   // the prescribed state
+      } else if (bc == kBCInflowSupersonic || bc == kBCFarfield) {   // the prescribed / free-stream state
 #pragma unroll
         for (int v = 0; v < 5; ++v) R[v] = d[v];
       } else if (bc == kBCOutflowSupersonic) {  // the interior state

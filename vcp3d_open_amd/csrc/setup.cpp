@@ -314,7 +314,7 @@ std::string check_case(const CaseParams &p)
   for (int f = 0; f < 6; ++f) {
     const bool per = p.periodic[f / 2] != 0;
     if (per && p.bc[f] != kBCPeriodic) return "a periodic direction cannot carry a physical boundary";
-    if (!per && (p.bc[f] < kBCOutflowNSCBC || p.bc[f] > kBCOutflowSupersonic))
+    if (!per && (p.bc[f] < kBCOutflowNSCBC || p.bc[f] > kBCFarfield))
       return "unsupported boundary condition on a non-periodic block face";
   }
   if (p.outlet_average_local)
@@ -474,7 +474,9 @@ RefConditions reference_conditions(const CaseParams &p)
     const double TRef = T > 0.0 ? T : TTot;
     if (TRef > 0.0) r.rho_ref = r.p_ref / (kRGas * TRef);
   }
+  r.fallback = 0;
   if (r.p_ref <= 0.0 || r.rho_ref <= 0.0) {
+    r.fallback = 1;
     const double vInf = p.mach * std::sqrt(kGamma * kRGas * p.temp_free_stream);
     r.rho_ref = p.reynolds * kMuDim / (p.reynolds_length * vInf);
     r.p_ref = r.rho_ref * kRGas * p.temp_free_stream;
@@ -482,6 +484,74 @@ RefConditions reference_conditions(const CaseParams &p)
   r.u_ref = std::sqrt(r.p_ref / r.rho_ref);
   r.mu = kMuDim * r.u_ref / r.p_ref;
   return r;
+}
+
+// InputParamClass::DetermineNonDimPrimVarFreeStream (InputParamClass.cpp:
+// 991-1191): the free stream from the prescribed data when they fix it
+// (velocity and two of p, rho, T; or a velocity direction and two total
+// conditions with the Mach number from a static/total pair), else from the
+// Mach number, temperature, direction (unit when the reference conditions
+// came from the same data, InputParamClass.cpp:697-707) and Reynolds number;
+// non-dimensional (rho, u, v, w, p).
+static void free_stream_nondim(const CaseParams &p, const RefConditions &rc, double q[5])
+{
+  const double big = 1e300;
+  double pr = -1.0, pTot = -1.0, rho = -1.0, T = -1.0, TTot = -1.0;
+  double vel[3] = {big, big, big}, dir[3] = {big, big, big};
+  for (int f = 0; f < 6; ++f) {
+    if (p.periodic[f / 2]) continue;
+    const int bc = p.bc[f];
+    const double *d = p.bc_data[f];
+    if (bc == kBCOutflowNSCBC || bc == kBCOutflowSubsonic) pr = d[0];
+    if (bc == kBCInflowStaticNSCBC) { rho = d[0]; vel[0] = d[1]; vel[1] = d[2]; vel[2] = d[3]; }
+    if (bc == kBCInflowTotalNSCBC || bc == kBCInflowSubsonic) {
+      pTot = d[0]; TTot = d[1]; dir[0] = d[2]; dir[1] = d[3]; dir[2] = d[4];
+    }
+    if (bc == kBCInflowSupersonic) { rho = d[0]; vel[0] = d[1]; vel[1] = d[2]; vel[2] = d[3]; pr = d[4]; }
+    if (bc == kBCWallIsothermal) T = d[0];
+  }
+  const double gm1 = kGamma - 1.0;
+  bool found = false;
+  if (vel[0] < big && vel[1] < big && vel[2] < big) {
+    const int n = (pr > 0.0) + (rho > 0.0) + (T > 0.0);
+    if (n >= 2) {
+      if (pr <= 0.0) pr = rho * kRGas * T;
+      if (rho <= 0.0) rho = pr / (kRGas * T);
+      found = true;
+    } else if (n == 1 && T <= 0.0) {
+      const double vm2 = vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2];
+      if (pr <= 0.0) pr = 0.95 * rho * vm2 / kGamma; else rho = kGamma * pr / (0.95 * vm2);
+      found = true;
+    }
+    if (found) { q[0] = rho; q[1] = vel[0]; q[2] = vel[1]; q[3] = vel[2]; q[4] = pr; }
+  } else if (dir[0] < big && dir[1] < big && dir[2] < big) {
+    const double inv = 1.0 / std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    double M2 = 0.95 * 0.95;
+    if (pTot > 0.0 && pr > 0.0) M2 = (std::pow(pTot / pr, gm1 / kGamma) - 1.0) * 2.0 / gm1;
+    else if (TTot > 0.0 && T > 0.0) M2 = (TTot / T - 1.0) * 2.0 / gm1;
+    if ((pTot > 0.0) + (TTot > 0.0) >= 2) {
+      const double rhoTot = pTot / (kRGas * TTot);
+      const double tmp = 1.0 + 0.5 * gm1 * M2;
+      pr = pTot / std::pow(tmp, kGamma / gm1);
+      rho = rhoTot / std::pow(tmp, 1.0 / gm1);
+      const double vm = std::sqrt(M2 * kGamma * pr / rho);
+      q[0] = rho; q[1] = dir[0] * inv * vm; q[2] = dir[1] * inv * vm; q[3] = dir[2] * inv * vm; q[4] = pr;
+      found = true;
+    }
+  }
+  if (!found) {
+    double dv[3] = {p.free_stream_dir[0], p.free_stream_dir[1], p.free_stream_dir[2]};
+    if (rc.fallback) {
+      const double inv = 1.0 / std::sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+      for (double &x : dv) x *= inv;
+    }
+    const double vInf = p.mach * std::sqrt(kGamma * kRGas * p.temp_free_stream);
+    const double muDim = rc.mu * rc.p_ref / rc.u_ref;
+    const double rhoInf = p.reynolds * muDim / (p.reynolds_length * vInf);
+    q[0] = rhoInf; q[1] = vInf * dv[0]; q[2] = vInf * dv[1]; q[3] = vInf * dv[2];
+    q[4] = rhoInf * kRGas * p.temp_free_stream;
+  }
+  q[0] /= rc.rho_ref; q[1] /= rc.u_ref; q[2] /= rc.u_ref; q[3] /= rc.u_ref; q[4] /= rc.p_ref;
 }
 
 std::string setup_open_boundary(const CaseParams &p, const Tables &t, const Geometry &g,
@@ -524,6 +594,13 @@ std::string setup_open_boundary(const CaseParams &p, const Tables &t, const Geom
       const double sEnt = std::log(pr / std::pow(rho, kGamma)), pInv = 1.0 / pr;
       o[0] = (kGamma - sEnt) / (kGamma - 1.0) - 0.5 * pInv * rho * (u * u + v * v + w * w);
       o[1] = rho * u * pInv; o[2] = rho * v * pInv; o[3] = rho * w * pInv; o[4] = -rho * pInv;
+    } else if (bc == kBCFarfield) {
+      // the free stream as entropy variables (SubfaceClasses.cpp:542-622)
+      double q[5];
+      free_stream_nondim(p, rc, q);
+      const double sEnt = std::log(q[4] / std::pow(q[0], kGamma)), pInv = 1.0 / q[4];
+      o[0] = (kGamma - sEnt) / (kGamma - 1.0) - 0.5 * pInv * q[0] * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+      o[1] = q[0] * q[1] * pInv; o[2] = q[0] * q[2] * pInv; o[3] = q[0] * q[3] * pInv; o[4] = -q[0] * pInv;
     } else if (bc != kBCWallInviscid && bc != kBCOutflowSupersonic) {
       return "unsupported boundary condition";
     }

@@ -18,7 +18,7 @@ constexpr double kMuDim = 1.716e-5;       // InputParamClass.cpp:607
 // Physical block-face kinds (one subface per block face).
 enum : int { kBCPeriodic = 0, kBCOutflowNSCBC = 1, kBCInflowStaticNSCBC = 2, kBCInflowTotalNSCBC = 3,
               kBCInflowSubsonic = 4, kBCOutflowSubsonic = 5, kBCWallHeatFlux = 6, kBCWallIsothermal = 7,
-              kBCWallInviscid = 8, kBCInflowSupersonic = 9, kBCOutflowSupersonic = 10 };
+              kBCWallInviscid = 8, kBCInflowSupersonic = 9, kBCOutflowSupersonic = 10, kBCFarfield = 11 };
 
 inline bool bc_is_nscbc(int bc) { return bc >= kBCOutflowNSCBC && bc <= kBCInflowTotalNSCBC; }
 inline bool bc_is_wall(int bc) { return bc >= kBCWallHeatFlux && bc <= kBCWallInviscid; }
@@ -30,6 +30,7 @@ struct CaseParams {
   int dof_location = 2;        // 1 equidistant, 2 LGL
   int periodic[3] = {1, 1, 1};
   double mach = 0, reynolds = 0, reynolds_length = 1, temp_free_stream = 0;
+  double free_stream_dir[3] = {1.0, 0.0, 0.0};   // "Free stream velocity direction"
   double theta_sym = 1, rel_penalty = 1, cfl = 1, dt_synchr = 0;
   int bug_compat = 1;
   // physical faces iMin..kMax (used where periodic[f/2] == 0)
@@ -76,7 +77,8 @@ struct Geometry {
   std::vector<int> nbr;           // [nElem][6], -1 physical boundary
 };
 
-struct RefConditions { double p_ref, rho_ref, u_ref, mu; };
+// fallback: pRef and rhoRef came from the free-stream data of the parameter file
+struct RefConditions { double p_ref, rho_ref, u_ref, mu; int fallback; };
 
 // PC-NSCBC faces of the block.
 struct OpenBoundary {

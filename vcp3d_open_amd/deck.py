@@ -32,8 +32,8 @@ the pressure pulse while the time step is 0 (ElementClass.cpp:515-665).
 Only what the GPU path supports is accepted (entropy variables, Ismail-Roe, no
 SGS or wall model, no body force; block faces periodic, PC-NSCBC in/outflow,
 subsonic total-condition inflow, subsonic outflow, supersonic inflow and
-outflow, heat-flux, isothermal or inviscid walls, one subface each); anything
-else raises DeckError naming the setting.
+outflow, heat-flux, isothermal or inviscid walls, farfield, one subface each);
+anything else raises DeckError naming the setting.
 """
 from __future__ import annotations
 
@@ -48,13 +48,13 @@ from .cases import GAMMA, MU_DIM, R_GAS
 BC_CODES = {"periodic1to1trans": 0, "bcoutflownscbc": 1, "bcinflowstaticnscbc": 2,
             "bcinflowtotalnscbc": 3, "bcinflowsubsonic": 4, "bcoutflowsubsonic": 5,
             "bcwallheatflux": 6, "bcwallisothermal": 7, "bcwallinviscid": 8,
-            "bcinflowsupersonic": 9, "bcoutflowsupersonic": 10}
+            "bcinflowsupersonic": 9, "bcoutflowsupersonic": 10, "bcfarfield": 11}
 # prescribed data per boundary condition, in the order of vcp_case_desc.bc_data
 _BC_DATA = {1: ["Pressure"], 2: ["Density", "VelocityX", "VelocityY", "VelocityZ"],
             3: ["TotalPressure", "TotalTemperature", "VelocityDirX", "VelocityDirY", "VelocityDirZ"],
             4: ["TotalPressure", "TotalTemperature", "VelocityDirX", "VelocityDirY", "VelocityDirZ"],
             5: ["Pressure"], 6: ["HeatFlux"], 7: ["Temperature"], 8: [],
-            9: ["Density", "VelocityX", "VelocityY", "VelocityZ", "Pressure"], 10: []}
+            9: ["Density", "VelocityX", "VelocityY", "VelocityZ", "Pressure"], 10: [], 11: []}
 _KNOWN_BC = ("internal1to1", "periodic1to1trans", "bcfarfield", "bcwallisothermal",
              "bcwallisothermalwm", "bcwallheatflux", "bcwallheatfluxwm", "bcwallinviscid",
              "bcsymmetryplane", "bcinflowsubsonic", "bcinflowsupersonic", "bcoutflowsubsonic",
@@ -282,6 +282,8 @@ class DeckCase:
             self.periodic[f // 2] = 0
             self.bc[f] = BC_CODES[bc]
             data = prm.prescribed.get((f, 0), {})
+            if not _BC_DATA[self.bc[f]] and data:
+                raise DeckError(f"face {f + 1}: boundary data is prescribed, but this is not expected")
             if self.bc[f] == 4 and "TotalPressure" not in data:
                 raise DeckError(f"face {f + 1}: BCInflowSubsonic is supported with total conditions "
                                 "(TotalPressure, TotalTemperature, VelocityDir*)")
@@ -358,7 +360,8 @@ class DeckCase:
                  theta_sym=self.theta_sym, rel_penalty=self.rel_penalty, cfl=self.cfl,
                  dt_synchr=self.dt_synchr, bc=self.bc, bc_data=self.bc_data,
                  monitor_forces=int(self.monitor_forces), reference_area=self.reference_area,
-                 compute_average=int(self.compute_average), **self.nscbc)
+                 compute_average=int(self.compute_average),
+                 free_stream_dir=self.vel_dir if self.vel_dir is not None else (1.0, 0.0, 0.0), **self.nscbc)
         if self.sponge_layer or self.cml:
             d.update(sponge_layer=int(self.sponge_layer), cml=int(self.cml),
                      sponge_elems=[r[0] for r in self.sponge_rows],

@@ -42,7 +42,7 @@ class CaseDesc(C.Structure):
         ("sponge_elems", C.c_int * 6),
         ("damping_constant", C.c_double * 6), ("damping_exponent", C.c_double * 6),
         ("monitor_forces", C.c_int), ("reference_area", C.c_double),
-        ("compute_average", C.c_int),
+        ("compute_average", C.c_int), ("free_stream_dir", C.c_double * 3),
     ]
 
 

@@ -96,3 +96,29 @@ def test_open_box_host_setup_and_rejections():
         h = C.c_void_p()
         assert L.vcp_create(C.byref(d), grid, C.byref(h)) < 0
         assert msg in L.vcp_last_error().decode()
+
+
+def test_boundary_settings_are_validated_on_the_host():
+    """Boundary codes outside the supported set, or a physical code on a
+    periodic direction, are rejected by the host set-up with a message."""
+    from vcp3d_open_amd.deck import DeckCase
+    from vcp3d_open_amd.solver import VCPError
+    from tests.golden.make_golden import HERE as GOLDEN
+
+    class Edited:
+        def __init__(self, base, **kw):
+            self.base, self.kw = base, kw
+            self.rule, self.n_elems = base.rule, base.n_elems
+
+        def description(self):
+            return dict(self.base.description(), **self.kw)
+
+        def element_grid(self):
+            return self.base.element_grid()
+
+    deck = DeckCase(os.path.join(GOLDEN, "ref_n3_8x4x4", "param_walls_a.cfg"))
+    host_setup(deck)   # valid: subsonic inflow/outflow, isothermal walls
+    with pytest.raises(VCPError, match="unsupported boundary condition"):
+        host_setup(Edited(deck, bc=[12, 5, 7, 7, 0, 0]))
+    with pytest.raises(VCPError, match="periodic direction"):
+        host_setup(Edited(deck, bc=[4, 5, 7, 7, 6, 0]))

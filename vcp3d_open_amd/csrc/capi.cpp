@@ -396,6 +396,8 @@ int vcp_host_setup(const vcp_case_desc *d, const double *elem_grid, double *vol,
   if (p.n_elem[0] <= 0 || p.n_elem[1] <= 0 || p.n_elem[2] <= 0) return fail("invalid element counts");
   if (p.n_poly_sol <= 0 || p.n_poly_sol > 9 || p.n_poly_grid <= 0 || p.n_poly_grid > 9)
     return fail("polynomial degrees must be in 1..9");
+  const std::string chk = vcp::check_case(p);   // the same case checks as vcp_create
+  if (!chk.empty()) return fail(chk);
   vcp::Tables t;
   vcp::build_tables(p, t);
   vcp::Geometry g;

@@ -223,10 +223,17 @@ def main() -> int:
         flops = face_kernel_flops(K, M) * 3 * case.n_elems
         bytes_ = face_kernel_bytes(K, M) * 3 * case.n_elems
     achieved = flops / (ms * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+    # the binding roof: min(FP64 peak, HBM bandwidth x algorithmic intensity)
+    intensity = flops / bytes_
+    attainable = min(PEAK_FP64_TFLOPS, PEAK_HBM_GBS * intensity / 1e3)
+    hbm_gbs = bytes_ / (ms * 1e-3) / 1e9
+    roofline = {"bound": "mfma" if attainable >= PEAK_FP64_TFLOPS else "hbm", "achieved": achieved,
+                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(name) if args.workload == "tgv" else None, "kernel": name,
                 "ms_per_launch": ms, "algorithmic_flops": flops, "algorithmic_bytes": bytes_,
-                "hbm_gbs_algorithmic": bytes_ / (ms * 1e-3) / 1e9,
+                "intensity_flop_per_byte": intensity, "attainable_tflops": attainable,
+                "frac_of_attainable": achieved / attainable,
+                "hbm_gbs_algorithmic": hbm_gbs, "hbm_frac": hbm_gbs / PEAK_HBM_GBS,
                 "ms_face_kernel": ms_face, "ms_volume_kernel": ms_vol, "cg_iters_mean": cg_mean}
 
     out = {

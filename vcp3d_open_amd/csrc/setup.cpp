@@ -16,7 +16,9 @@
 #include "setup.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <thread>
 #include <stdexcept>
 
 namespace vcp {
@@ -337,12 +339,24 @@ std::string build_geometry(const CaseParams &p, Tables &t, const double *grid, G
   const int ni = p.n_elem[0], nj = p.n_elem[1], nk = p.n_elem[2];
   const int nElem = ni * nj * nk, M = t.M, Kg = t.Kg, nG = Kg * Kg * Kg;
   const int nInt = M * M * M, n2 = M * M;
+  const int nThreads = std::max(1, std::min(16, std::min((int) std::thread::hardware_concurrency(), nElem / 64)));
+  // f(e0, e1) on contiguous element ranges, one per host thread
+  auto in_parallel = [nThreads, nElem](auto &&f) {
+    if (nThreads == 1) { f(0, nElem); return; }
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nThreads; ++t)
+      pool.emplace_back([&f, t, nThreads, nElem] { f((int) ((long) nElem * t / nThreads), (int) ((long) nElem * (t + 1) / nThreads)); });
+    for (auto &th : pool) th.join();
+  };
   // grid DOF spacing: all LGL or all equidistant
+  std::vector<int> spacing(nElem);
+  in_parallel([&](int e0, int e1) {
+    for (int e = e0; e < e1; ++e) spacing[e] = classify_spacing(p.n_poly_grid, grid + (size_t) e * 3 * nG, nG);
+  });
   int nL = 0, nE = 0;
   for (int e = 0; e < nElem; ++e) {
-    const int c = classify_spacing(p.n_poly_grid, grid + (size_t) e * 3 * nG, nG);
-    if (c < 0) return "Not able to determine the point distribution";
-    (c ? nL : nE)++;
+    if (spacing[e] < 0) return "Not able to determine the point distribution";
+    (spacing[e] ? nL : nE)++;
   }
   if (nL && nE) return "Elements with both equidistant and LGL spacing encountered";
   t.r_grid = edge_points(Kg, nL ? 2 : 1);
@@ -357,8 +371,14 @@ std::string build_geometry(const CaseParams &p, Tables &t, const double *grid, G
   for (int f = 0; f < 6; ++f) g.face[f].assign((size_t) nElem * 16 * n2, 0.0);
   g.lens.assign((size_t) nElem * 4, 0.0);
   g.ave_norm.assign((size_t) nElem * 9, 0.0);
+  // elements are independent: contiguous ranges on host threads, each with
+  // its own scratch (the arithmetic per element is unchanged)
+  std::atomic<int> badJacobian{0};
+  auto elements = [&](int e0, int e1) {
   std::vector<double> X(3 * (size_t) nInt), DR(3 * (size_t) nInt), DS(3 * (size_t) nInt), DTt(3 * (size_t) nInt);
-  for (int e = 0; e < nElem; ++e) {
+  std::vector<double> C(12 * (size_t) n2);
+  for (int e = e0; e < e1; ++e) {
+    if (badJacobian.load(std::memory_order_relaxed)) return;
     const double *ge = grid + (size_t) e * 3 * nG;
     double *vm = &g.vol[(size_t) e * 13 * nInt];
     for (int c = 0; c < 3; ++c) {
@@ -373,7 +393,7 @@ std::string build_geometry(const CaseParams &p, Tables &t, const double *grid, G
       const double dt[3] = {DTt[l], DTt[nInt + l], DTt[2 * nInt + l]};
       double J, m[9];
       inverse_metric(dr, ds, dt, J, m);
-      if (!(J > 0.0)) return "Negative or zero Jacobians encountered";
+      if (!(J > 0.0)) { badJacobian.store(1); return; }
       vm[l] = J;
       for (int q = 0; q < 9; ++q) vm[(1 + q) * nInt + l] = m[q];
       for (int c = 0; c < 3; ++c) vm[(10 + c) * nInt + l] = X[c * nInt + l];
@@ -389,7 +409,6 @@ std::string build_geometry(const CaseParams &p, Tables &t, const double *grid, G
       if (dir == 0) { ox = 1; for (int a = 0; a < 4; ++a) Ox[a] = (a == 1) ? dfv : fv; }
       if (dir == 1) { oy = 1; for (int a = 0; a < 4; ++a) Oy[a] = (a == 2) ? dfv : fv; }
       if (dir == 2) { oz = 1; for (int a = 0; a < 4; ++a) Oz[a] = (a == 3) ? dfv : fv; }
-      std::vector<double> C(12 * (size_t) n2);
       for (int a = 0; a < 4; ++a)
         for (int c = 0; c < 3; ++c)
           apply3(Kg, Kg, Kg, ox, oy, oz, Ox[a], Oy[a], Oz[a], ge + c * nG, &C[(a * 3 + c) * n2]);
@@ -426,6 +445,9 @@ std::string build_geometry(const CaseParams &p, Tables &t, const double *grid, G
       for (int c = 0; c < 3; ++c) g.ave_norm[(size_t) e * 9 + 3 * dir + c] = an[c] / area;
     }
   }
+  };
+  in_parallel(elements);
+  if (badJacobian.load()) return "Negative or zero Jacobians encountered";
   // neighbours; a periodic block face aliases the opposite owned element
   g.nbr.assign((size_t) nElem * 6, -1);
   const int n[3] = {ni, nj, nk};
@@ -447,9 +469,6 @@ std::string build_geometry(const CaseParams &p, Tables &t, const double *grid, G
   return "";
 }
 
-// Pressure from an outlet, density from a static inlet (or the total
-// temperature of a total-condition inlet) fix the reference state; a block
-// without such data uses the free stream (Mach, Reynolds, temperature).
 // InputParamClass::ComputeReferenceConditions (InputParamClass.cpp:605-758):
 // pressure, total pressure, density, temperature and total temperature from
 // the prescribed data of the physical faces (a later face overriding an

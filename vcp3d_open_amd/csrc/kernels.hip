@@ -99,14 +99,15 @@ __device__ inline double wave_reduce_dpp(double v)
 // Block reduction with one barrier: partial results go to alternating
 // halves of red (2 * NT/64 * NV doubles), so a half is rewritten only after
 // every thread has passed the barrier of the following reduction. The max
-// slots (q >= NS) take non-negative values. Only the first NW waves hold
+// slots (q >= NS) take non-negative values. Only waves [W0, W0 + NW) hold
 // contributions; the others skip the wave reduction.
-template <int NT, int NV, int NS, int NW = NT / 64>
+template <int NT, int NV, int NS, int NW = NT / 64, int W0 = 0>
 __device__ inline void block_reduce1(double (&v)[NV], double *red, int &half)
 {
+  static_assert(W0 + NW <= NT / 64, "contributing waves");
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   double *buf = red + half * (NT / 64) * NV;
-  if (wid < NW) {
+  if (wid >= W0 && wid < W0 + NW) {
 #pragma unroll
     for (int q = 0; q < NV; ++q) v[q] = (q < NS) ? wave_reduce_dpp<false>(v[q]) : wave_reduce_dpp<true>(v[q]);
     if (lane == 0)
@@ -116,9 +117,9 @@ __device__ inline void block_reduce1(double (&v)[NV], double *red, int &half)
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
-    double s = buf[q];
+    double s = buf[W0 * NV + q];
 #pragma unroll
-    for (int w = 1; w < NW; ++w) s = (q < NS) ? s + buf[w * NV + q] : fmax(s, buf[w * NV + q]);
+    for (int w = W0 + 1; w < W0 + NW; ++w) s = (q < NS) ? s + buf[w * NV + q] : fmax(s, buf[w * NV + q]);
     v[q] = s;
   }
   half ^= 1;
@@ -1054,13 +1055,21 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
 #pragma unroll
     for (int l = 0; l < 5; ++l) { pv[l] = z[l]; rz += r[l] * z[l]; }
     { double red[1] = {rz}; block_reduce1<NT, 1, 1, (ND + 63) / 64>(red, sRed, half); rz = red[0]; }
+    // Stages with fewer items than threads start at different waves, so that
+    // the waves of the workgroup (one per SIMD) carry similar shares of the
+    // iteration: the DOF owners and the second point pass sit in the first
+    // waves, the two k stages in the last ones. rot(F) numbers the threads
+    // from wave F on.
+    constexpr int NWV = NT / 64, NWK = (5 * K * K + 63) / 64;
+    constexpr int FK = NWV - NWK < 2 ? NWV - NWK : 2;
+    auto rot = [tid](int F) { return (tid + NT - 64 * F) & (NT - 1); };
     for (int it = 0;; ++it) {
       if (it == 100) { atomicOr(a.flags, kFlagCGNotConverged); break; }
       if (own)
 #pragma unroll
         for (int l = 0; l < 5; ++l) sU[l * ND + tid] = pv[l];
       __syncthreads();
-      for (int pc = tid; pc < 5 * K * K; pc += NT) {
+      for (int pc = rot(FK); pc < 5 * K * K; pc += NT) {
         const int v = pc / (K * K), ji = pc - v * K * K;
         double xx[K];
 #pragma unroll
@@ -1074,7 +1083,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
         }
       }
       __syncthreads();
-      for (int pc = tid; pc < 5 * M * K; pc += NT) {
+      for (int pc = rot(1); pc < 5 * M * K; pc += NT) {
         const int vk = pc / K, i = pc - vk * K;
         double xx[K];
 #pragma unroll
@@ -1088,23 +1097,30 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
         }
       }
       __syncthreads();
+      // along i, one pencil (variable, row jk) per item with the operator from
+      // the kernel arguments: K LDS reads per M outputs, where a per-point
+      // contraction would read 5 K values and an operator row per point
+      for (int pc = tid; pc < 5 * M * M; pc += NT) {
+        const int l = pc / (M * M), jk = pc - l * M * M;
+        double xx[K];
+#pragma unroll
+        for (int q = 0; q < K; ++q) xx[q] = cy[pc * K + q];
+#pragma unroll
+        for (int n = 0; n < M; ++n) {
+          double s = 0.0;
+#pragma unroll
+          for (int q = 0; q < K; ++q) s += A[n * K + q] * xx[q];
+          pI[l * NI + jk * M + n] = s;
+        }
+      }
+      __syncthreads();
 #pragma unroll
       for (int q = 0; q < (P1 > 0 ? 2 : 1); ++q) {
         const int m = q == 0 ? tid : P0 + tid;
         if (!(q == 0 ? tid < P0 : tid < P1)) continue;
-        const int i = m % M, jk = m / M;
-        double ai[K];
-#pragma unroll
-        for (int ii = 0; ii < K; ++ii) ai[ii] = sA[i * K + ii];
         double pp[5];
 #pragma unroll
-        for (int l = 0; l < 5; ++l) {
-          const double *ra = &cy[(l * M * M + jk) * K];
-          double s = 0.0;
-#pragma unroll
-          for (int ii = 0; ii < K; ++ii) s += ai[ii] * ra[ii];
-          pp[l] = s;
-        }
+        for (int l = 0; l < 5; ++l) pp[l] = pI[l * NI + m];
         // w J dU/dV at the point (ElementClass.cpp:4228-4253): every entry is
         // homogeneous of degree one in (rho, p), so the scaled pair held in
         // prim gives the weighted matrix directly
@@ -1133,7 +1149,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
         }
       }
       __syncthreads();
-      for (int pc = tid; pc < 5 * M * K; pc += NT) {
+      for (int pc = rot(2); pc < 5 * M * K; pc += NT) {
         const int vk = pc / K, i = pc - vk * K;
         double xx[M];
 #pragma unroll
@@ -1149,7 +1165,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
       __syncthreads();
       // Ap along k, with the partial p.Ap of the entries each thread computes
       double pAp = 0.0;
-      for (int pc = tid; pc < 5 * K * K; pc += NT) {
+      for (int pc = rot(FK); pc < 5 * K * K; pc += NT) {
         const int v = pc / (K * K), ji = pc - v * K * K;
         double xx[M];
 #pragma unroll
@@ -1164,7 +1180,11 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
           pAp += sU[idx] * s;
         }
       }
-      { double red[1] = {pAp}; block_reduce1<NT, 1, 1, (5 * K * K + 63) / 64>(red, sRed, half); pAp = red[0]; }
+      {
+        double red[1] = {pAp};
+        block_reduce1<NT, 1, 1, (NWK < NWV ? NWK : NWV), (NWK < NWV ? FK : 0)>(red, sRed, half);
+        pAp = red[0];
+      }
       // vector updates only in the waves that own DOFs
       double zAp = 0.0, rzn = 0.0, rmax = 0.0, alpha = 0.0;
       if (own) {

@@ -115,6 +115,7 @@ enum { VCP_BC_PERIODIC = 0, VCP_BC_OUTFLOW_NSCBC = 1, VCP_BC_INFLOW_STATIC_NSCBC
        VCP_BC_INFLOW_SUPERSONIC = 9, VCP_BC_OUTFLOW_SUPERSONIC = 10, VCP_BC_FARFIELD = 11 };
 
 typedef struct vcp_ctx vcp_ctx;
+typedef struct vcp_batch vcp_batch;
 
 /* Setup: SolverClass::SolverClass + ReadGrid + ComputeMetricTerms
    (SolverClass.cpp:12-61, :2787-3034, :829-1016). */
@@ -193,6 +194,61 @@ int vcp_cg_iterations(vcp_ctx *ctx, int *iters);
    the context's stream around each kernel; writes the mean milliseconds per
    launch of [boundary + face kernels, volume+mass-matrix kernel] into ms[2]. */
 int vcp_profile_residual(vcp_ctx *ctx, int reps, double ms[2]);
+
+/* Execution option with no counterpart in the reference: the workgroup size
+   of the volume kernel (threads per element; 0 restores the default). The
+   sizes available for (nDOFs1D, nInt1D) come from vcp_volume_tiles, the
+   default first; vcp_volume_tiles returns their number. Results of different
+   sizes agree to rounding (the partial sums of the mass-matrix solve are
+   grouped differently); a given size gives identical results every run. */
+int vcp_set_tile(vcp_ctx *ctx, int nt);
+int vcp_volume_tiles(int n_dofs_1d, int n_int_1d, int out[4]);
+
+/* ---------------------------------------------------------------- batches
+   Many independent cases stepped together, with no counterpart in the
+   reference (one SolverClass per case, SolverClass.hpp:40-124): the cases
+   are grouped by (nDOFs1D, nInt1D, bug_compat), and each kernel of an RK
+   stage is one launch per group over a work list of (case, element) items.
+   Each case keeps its own time step and the synchronization-step clipping of
+   ExecuteTimeSynchronizationStep (SolverClass.cpp:2356-2365) on the device:
+   a case advanced n RK steps by the batch holds the state that n RK steps of
+   vcp_sync_step give it alone with the same volume workgroup size. A case
+   whose error flags are set (CG not converged, negative a^2, non-finite
+   residual: where the reference calls Terminate) or whose step budget is
+   spent is deactivated on the device; the others continue. */
+int vcp_batch_create(vcp_batch **out);
+/* Ends every membership (the contexts stay valid) and frees the batch. */
+int vcp_batch_destroy(vcp_batch *b);
+/* ctx joins with a budget of n_rk_steps RK steps (<= 0: unlimited). Its
+   state must be set; its first residual is evaluated by the next
+   vcp_batch_step. While it belongs to the batch, only the batch steps it:
+   vcp_residual / vcp_sync_step / vcp_set_* on it fail, and the vcp_get_*
+   calls wait for the batch's queued work. Cases that compute running
+   averages cannot join. */
+int vcp_batch_add(vcp_batch *b, vcp_ctx *ctx, int n_rk_steps);
+/* ctx leaves the batch (after the batch's queued work), keeping its state;
+   vcp_sync_step then continues from it with a new synchronization step. */
+int vcp_batch_remove(vcp_batch *b, vcp_ctx *ctx);
+/* Queues n_steps RK steps of every active case on the batch's stream and
+   returns without waiting. */
+int vcp_batch_step(vcp_batch *b, int n_steps);
+int vcp_batch_wait(vcp_batch *b);
+/* Waits, then writes the members (up to cap) and their status words
+   status[i][5] = {active, RK steps done, RK steps left, error flags,
+   synchronization steps done}; returns the number of members. */
+int vcp_batch_status(vcp_batch *b, vcp_ctx **ctxs, int *status, int cap);
+/* Replay each RK step of the batch as one captured HIP graph (recaptured
+   after a membership change). */
+int vcp_batch_set_graph(vcp_batch *b, int enable);
+/* Volume workgroup size of the group (nDOFs1D, nInt1D) (see vcp_set_tile). */
+int vcp_batch_set_tile(vcp_batch *b, int n_dofs_1d, int n_int_1d, int nt);
+/* Groups: info[g][6] = {nDOFs1D, nInt1D, bug_compat, cases, work items,
+   volume workgroup size}; returns the number of groups. */
+int vcp_batch_groups(vcp_batch *b, int *info, int cap);
+/* Measurement: mean milliseconds of `reps` residual evaluations per group,
+   ms[2 g] boundary + face kernels, ms[2 g + 1] volume kernel (cap: entries
+   of ms / 2); returns the number of groups. */
+int vcp_batch_profile(vcp_batch *b, int reps, double *ms, int cap);
 
 /* Host-only set-up (no device calls), for checking the tables and metrics
    without a GPU: SolverClass::ComputeMetricTerms (SolverClass.cpp:829-1016).

@@ -11,7 +11,7 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libvcp3d_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["kernels.hip", "setup.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "setup.cpp", "capi.cpp", "batch.cpp"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=gfx950",
           # no packed FP32 instructions anywhere in device code
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -1,91 +1,38 @@
 // C ABI of the GPU path (include/vcp3d_amd.h): context set-up, device
 // buffers and the host sequencing of SolverClass::Residual and
 // SolverClass::ExecuteTimeSynchronizationStep.
-#include "vcp3d_amd.h"
+#include "ctx.hpp"
 
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <utility>
 #include <vector>
 
-#include "kernels.hpp"
-#include "setup.hpp"
+namespace vcp {
+
+namespace {
+thread_local std::string g_error;
+}
+
+int fail(const std::string &msg, int code)
+{
+  g_error = msg;
+  return code;
+}
+
+}  // namespace vcp
+
+using vcp::fail;
 
 namespace {
 
 // device scalars read back with the monitoring data, and the slot of the
 // error flags in the pinned host buffer
 constexpr int kScalRead = vcp::kScalForce + 6, kHostFlags = vcp::kScalCount;
-
-thread_local std::string g_error;
-
-int fail(const std::string &msg, int code = -1)
-{
-  g_error = msg;
-  return code;
-}
-
-#define HIP_TRY(expr)                                                              \
-  do {                                                                             \
-    hipError_t e_ = (expr);                                                        \
-    if (e_ != hipSuccess)                                                          \
-      return fail(std::string(#expr) + ": " + hipGetErrorString(e_), -3);         \
-  } while (0)
-
-}  // namespace
-
-struct vcp_ctx {
-  vcp::CaseParams prm;
-  vcp::Tables tab;
-  vcp::RefConditions rc{};
-  vcp::DeviceCase dc;
-  vcp::Sponge sponge;
-  hipStream_t stream = nullptr;
-  // side stream for the next call's time step, which overlaps the last stage
-  hipStream_t side = nullptr;
-  hipEvent_t evState = nullptr, evDt = nullptr;
-  // pinned host words the per-step scalars come back through: [0, kScalRead)
-  // the device scalars (time step, maximum Mach^2, ..., wall forces),
-  // [kHostFlags] the error flags (as int)
-  double *host = nullptr;
-  std::vector<void *> allocs;
-  bool res_valid = false;
-  bool dt_ready = false;          // host[0] holds the time step of the current state
-  bool damping_set = false;
-  int time_step = 0;
-  int n_avg = 0;                  // synchronization steps in the running averages
-  // RK steps replayed as captured HIP graphs (vcp_set_graph), one per
-  // [state buffer that is current][last step of the call], captured on first use
-  bool use_graph = false;
-  double *U_first = nullptr;
-  hipGraphExec_t graph[2][2] = {};
-
-  template <class T>
-  hipError_t alloc(T **p, size_t n)
-  {
-    void *v = nullptr;
-    hipError_t e = hipMalloc(&v, n * sizeof(T) > 0 ? n * sizeof(T) : 16);
-    if (e == hipSuccess) { allocs.push_back(v); *p = static_cast<T *>(v); }
-    return e;
-  }
-  ~vcp_ctx()
-  {
-    for (auto &row : graph)
-      for (hipGraphExec_t g : row)
-        if (g) (void) hipGraphExecDestroy(g);
-    for (void *p : allocs) (void) hipFree(p);
-    if (host) (void) hipHostFree(host);
-    if (evState) (void) hipEventDestroy(evState);
-    if (evDt) (void) hipEventDestroy(evDt);
-    if (side) (void) hipStreamDestroy(side);
-    if (stream) (void) hipStreamDestroy(stream);
-  }
-};
-
-namespace {
 
 size_t state_size(const vcp_ctx *c) { return (size_t) c->dc.nElem * 5 * c->dc.nDOF; }
 
@@ -106,15 +53,15 @@ int check_flags(vcp_ctx *c)
 }
 
 
-// open boundaries + faces + volume/mass matrix, with the given stage control
 int residual_pass(vcp_ctx *c, const vcp::StageCtl &ctl)
 {
-  if (c->prm.sponge_layer && !c->damping_set)
-    return fail("sponge layer damping state not set (vcp_set_damping_state)");
-  HIP_TRY(vcp::launch_boundary(c->dc, c->stream, ctl.monitor != 0));
-  HIP_TRY(vcp::launch_faces(c->dc, c->stream));
-  HIP_TRY(vcp::launch_volume(c->dc, ctl, c->stream));
-  return 0;
+  return vcp::residual_pass(c, ctl, c->stream);
+}
+
+// a context that a batch steps is driven by the batch only
+int not_in_batch(const vcp_ctx *c)
+{
+  return c->batch ? fail("the context belongs to a batch (vcp_batch_remove it first)") : 0;
 }
 
 // The stages of one SSP-RK3 step (Shu-Osher form, SolverClass.cpp:2192-2194,
@@ -216,12 +163,27 @@ void fill_monitoring(const vcp_ctx *c, double md[8], bool mon, const double *sc)
 
 }  // namespace
 
+namespace vcp {
+
+int residual_pass(vcp_ctx *c, const StageCtl &ctl, hipStream_t s)
+{
+  if (c->prm.sponge_layer && !c->damping_set)
+    return fail("sponge layer damping state not set (vcp_set_damping_state)");
+  HIP_TRY(launch_boundary(c->dc, s, ctl.monitor != 0));
+  HIP_TRY(launch_faces(c->dc, s));
+  HIP_TRY(launch_volume(c->dc, ctl, s));
+  return 0;
+}
+
+}  // namespace vcp
+
 extern "C" {
 
-const char *vcp_last_error(void) { return g_error.c_str(); }
+const char *vcp_last_error(void) { return vcp::g_error.c_str(); }
 
 int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
 {
+  VCP_GUARD_BEGIN
   if (!d || !elem_grid || !out) return fail("null argument");
   *out = nullptr;
   vcp::CaseParams p;
@@ -238,17 +200,18 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
   const int K = p.n_poly_sol + 1, M = vcp::n_int1d(p.n_poly_int_rule);
   if (!vcp::supported_pair(K, M)) return fail("unsupported (nDOFs1D, nInt1D) pair");
 
-  vcp_ctx *c = new vcp_ctx();
+  std::unique_ptr<vcp_ctx> holder(new vcp_ctx());
+  vcp_ctx *c = holder.get();
   c->prm = p;
   vcp::build_tables(p, c->tab);
   vcp::Geometry g;
   const std::string err = vcp::build_geometry(p, c->tab, elem_grid, g);
-  if (!err.empty()) { delete c; return fail(err); }
+  if (!err.empty()) return fail(err);
   c->rc = vcp::reference_conditions(p);
   vcp::OpenBoundary ob;
   std::string serr = vcp::setup_open_boundary(p, c->tab, g, c->rc, ob);
   if (serr.empty()) serr = vcp::setup_sponge(p, c->tab, g, elem_grid, c->sponge);
-  if (!serr.empty()) { delete c; return fail(serr); }
+  if (!serr.empty()) return fail(serr);
 
   vcp::DeviceCase &dc = c->dc;
   const int nElem = p.n_elem[0] * p.n_elem[1] * p.n_elem[2];
@@ -379,17 +342,17 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
   if (he == hipSuccess) he = hipMemset(dc.flags, 0, 4 * sizeof(int));
   if (he == hipSuccess) he = hipMemset(dc.scal, 0, vcp::kScalCount * sizeof(double));
   if (he != hipSuccess) {
-    const std::string msg = std::string("device set-up failed: ") + hipGetErrorString(he);
-    delete c;
-    return fail(msg, -3);
+    return fail(std::string("device set-up failed: ") + hipGetErrorString(he), -3);
   }
-  *out = c;
+  *out = holder.release();
   return 0;
+  VCP_GUARD_END
 }
 
 int vcp_host_setup(const vcp_case_desc *d, const double *elem_grid, double *vol, double *faces,
                    double *lens, int *nbr)
 {
+  VCP_GUARD_BEGIN
   if (!d || !elem_grid) return fail("null argument");
   vcp::CaseParams p;
   params_from_desc(d, p);
@@ -410,44 +373,60 @@ int vcp_host_setup(const vcp_case_desc *d, const double *elem_grid, double *vol,
   if (lens) std::memcpy(lens, g.lens.data(), g.lens.size() * sizeof(double));
   if (nbr) std::memcpy(nbr, g.nbr.data(), g.nbr.size() * sizeof(int));
   return 0;
+  VCP_GUARD_END
 }
 
 int vcp_destroy(vcp_ctx *c)
 {
-  if (c) { (void) hipStreamSynchronize(c->stream); delete c; }
-  return 0;
+  VCP_GUARD_BEGIN
+  if (!c) return 0;
+  int rc = 0;
+  if (c->batch) rc = vcp_batch_remove(c->batch, c);
+  (void) hipStreamSynchronize(c->stream);
+  delete c;
+  return rc;
+  VCP_GUARD_END
 }
 
 int vcp_sizes(const vcp_ctx *c, int *n_elem, int *k, int *m)
 {
+  VCP_GUARD_BEGIN
   if (!c) return fail("null context");
   if (n_elem) *n_elem = c->dc.nElem;
   if (k) *k = c->dc.K;
   if (m) *m = c->dc.M;
   return 0;
+  VCP_GUARD_END
 }
 
 int vcp_reference(const vcp_ctx *c, double out[4])
 {
+  VCP_GUARD_BEGIN
   if (!c || !out) return fail("null argument");
   out[0] = c->rc.u_ref; out[1] = c->rc.p_ref; out[2] = c->rc.rho_ref; out[3] = c->rc.mu;
   return 0;
+  VCP_GUARD_END
 }
 
 int vcp_set_restart(vcp_ctx *c, const double *nodal)
 {
+  VCP_GUARD_BEGIN
   if (!c || !nodal) return fail("null argument");
+  if (int rc_ = not_in_batch(c)) return rc_;
   const int nD = c->dc.nDOF;
   std::vector<double> modal(state_size(c));
 #pragma omp parallel for schedule(static)
   for (int e = 0; e < c->dc.nElem; ++e)
     vcp::nodal_cons_to_modal_entropy(c->tab, c->rc, nodal + (size_t) e * nD * 5, &modal[(size_t) e * 5 * nD]);
   return vcp_set_solution(c, modal.data());
+  VCP_GUARD_END
 }
 
 int vcp_set_damping_state(vcp_ctx *c, const double *nodal_prim)
 {
+  VCP_GUARD_BEGIN
   if (!c || !nodal_prim) return fail("null argument");
+  if (int rc_ = not_in_batch(c)) return rc_;
   if (!c->prm.sponge_layer) return fail("no sponge layer in this case");
   const vcp::Sponge &sp = c->sponge;
   const int nD = c->dc.nDOF, nInt = c->dc.nInt;
@@ -461,11 +440,14 @@ int vcp_set_damping_state(vcp_ctx *c, const double *nodal_prim)
   c->damping_set = true;
   c->res_valid = false;
   return 0;
+  VCP_GUARD_END
 }
 
 int vcp_nscbc_average(vcp_ctx *c, double mach_avg[6])
 {
+  VCP_GUARD_BEGIN
   if (!c || !mach_avg) return fail("null argument");
+  if (int rc_ = not_in_batch(c)) return rc_;
   for (int f = 0; f < 6; ++f) mach_avg[f] = 0.0;
   if (!c->dc.nBFace) return 0;
   HIP_TRY(vcp::launch_boundary(c->dc, c->stream));
@@ -478,29 +460,38 @@ int vcp_nscbc_average(vcp_ctx *c, double mach_avg[6])
     mach_avg[f] = sum * c->dc.nscbc.invArea[f];
   }
   return 0;
+  VCP_GUARD_END
 }
 
 int vcp_set_solution(vcp_ctx *c, const double *modal)
 {
+  VCP_GUARD_BEGIN
   if (!c || !modal) return fail("null argument");
+  if (int rc_ = not_in_batch(c)) return rc_;
   HIP_TRY(hipMemcpyAsync(c->dc.U, modal, state_size(c) * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->res_valid = false;
   c->dt_ready = false;
   return 0;
+  VCP_GUARD_END
 }
 
 int vcp_get_solution(vcp_ctx *c, double *modal)
 {
+  VCP_GUARD_BEGIN
   if (!c || !modal) return fail("null argument");
+  if (int rc_ = vcp::batch_quiesce(c)) return rc_;
   HIP_TRY(hipMemcpyAsync(modal, c->dc.U, state_size(c) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return 0;
+  VCP_GUARD_END
 }
 
 int vcp_get_nodal(vcp_ctx *c, double *nodal)
 {
+  VCP_GUARD_BEGIN
   if (!c || !nodal) return fail("null argument");
+  if (int rc_ = vcp::batch_quiesce(c)) return rc_;
   std::vector<double> modal(state_size(c));
   const int rc = vcp_get_solution(c, modal.data());
   if (rc) return rc;
@@ -509,19 +500,25 @@ int vcp_get_nodal(vcp_ctx *c, double *nodal)
   for (int e = 0; e < c->dc.nElem; ++e)
     vcp::modal_entropy_to_nodal_cons(c->tab, c->rc, &modal[(size_t) e * 5 * nD], nodal + (size_t) e * nD * 5);
   return 0;
+  VCP_GUARD_END
 }
 
 int vcp_get_residual(vcp_ctx *c, double *modal)
 {
+  VCP_GUARD_BEGIN
   if (!c || !modal) return fail("null argument");
+  if (int rc_ = vcp::batch_quiesce(c)) return rc_;
   HIP_TRY(hipMemcpyAsync(modal, c->dc.R, state_size(c) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return 0;
+  VCP_GUARD_END
 }
 
 int vcp_residual(vcp_ctx *c, int mon, double md[8])
 {
+  VCP_GUARD_BEGIN
   if (!c) return fail("null context");
+  if (int rc_ = not_in_batch(c)) return rc_;
   int rc = 0;
   vcp::StageCtl ctl{1, 0, 0.0, 0.0, mon ? 1 : 0};
   if ((rc = residual_pass(c, ctl))) return rc;
@@ -531,21 +528,27 @@ int vcp_residual(vcp_ctx *c, int mon, double md[8])
   if (md) fill_monitoring(c, md, mon != 0, sc);
   c->res_valid = true;
   return 0;
+  VCP_GUARD_END
 }
 
 int vcp_time_steps(vcp_ctx *c, double *dt_elem)
 {
+  VCP_GUARD_BEGIN
   if (!c) return fail("null context");
+  if (int rc_ = not_in_batch(c)) return rc_;
   c->dt_ready = false;
   HIP_TRY(vcp::launch_time_step(c->dc, c->stream));
   if (dt_elem)
     HIP_TRY(hipMemcpyAsync(dt_elem, c->dc.dtElem, c->dc.nElem * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   return check_flags(c);
+  VCP_GUARD_END
 }
 
 int vcp_sync_step(vcp_ctx *c, double md[8])
 {
+  VCP_GUARD_BEGIN
   if (!c) return fail("null context");
+  if (int rc_ = not_in_batch(c)) return rc_;
   double dummy[8];
   if (!md) md = dummy;
   int rc = 0;
@@ -597,11 +600,14 @@ int vcp_sync_step(vcp_ctx *c, double md[8])
   }
   c->res_valid = true;
   return nSteps;
+  VCP_GUARD_END
 }
 
 int vcp_get_averages(vcp_ctx *c, double *avg, int *n_steps)
 {
+  VCP_GUARD_BEGIN
   if (!c || !avg || !n_steps) return fail("null argument");
+  if (int rc_ = vcp::batch_quiesce(c)) return rc_;
   if (!c->dc.avg) return fail("the case was created without compute_average");
   const int nD = c->dc.nDOF;
   std::vector<double> buf((size_t) c->dc.nElem * 11 * nD);
@@ -615,11 +621,14 @@ int vcp_get_averages(vcp_ctx *c, double *avg, int *n_steps)
     }
   *n_steps = c->n_avg;
   return 0;
+  VCP_GUARD_END
 }
 
 int vcp_set_averages(vcp_ctx *c, const double *avg, int n_steps)
 {
+  VCP_GUARD_BEGIN
   if (!c || !avg) return fail("null argument");
+  if (int rc_ = not_in_batch(c)) return rc_;
   if (!c->dc.avg) return fail("the case was created without compute_average");
   if (n_steps < 0) return fail("negative number of averaged steps");
   const int nD = c->dc.nDOF;
@@ -631,28 +640,64 @@ int vcp_set_averages(vcp_ctx *c, const double *avg, int n_steps)
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->n_avg = n_steps;
   return 0;
+  VCP_GUARD_END
 }
 
 int vcp_set_graph(vcp_ctx *c, int enable)
 {
+  VCP_GUARD_BEGIN
   if (!c) return fail("null context");
+  if (int rc_ = not_in_batch(c)) return rc_;
   c->use_graph = enable != 0;
   return 0;
+  VCP_GUARD_END
+}
+
+int vcp_set_tile(vcp_ctx *c, int nt)
+{
+  VCP_GUARD_BEGIN
+  if (!c) return fail("null context");
+  if (int rc_ = not_in_batch(c)) return rc_;
+  if (nt != 0) {
+    int t[4];
+    const int n = vcp::volume_tiles(c->dc.K, c->dc.M, t);
+    bool ok = false;
+    for (int i = 0; i < n; ++i) ok = ok || t[i] == nt;
+    if (!ok) return fail("volume workgroup size not available for this (nDOFs1D, nInt1D)");
+  }
+  c->dc.nt = nt;
+  c->drop_graphs();
+  return 0;
+  VCP_GUARD_END
+}
+
+int vcp_volume_tiles(int n_dofs_1d, int n_int_1d, int out[4])
+{
+  VCP_GUARD_BEGIN
+  if (!out) return fail("null argument");
+  return vcp::volume_tiles(n_dofs_1d, n_int_1d, out);
+  VCP_GUARD_END
 }
 
 int vcp_cg_iterations(vcp_ctx *c, int *iters)
 {
+  VCP_GUARD_BEGIN
   if (!c || !iters) return fail("null argument");
+  if (int rc_ = vcp::batch_quiesce(c)) return rc_;
   HIP_TRY(hipMemcpyAsync(iters, c->dc.cgIter, c->dc.nElem * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return 0;
+  VCP_GUARD_END
 }
 
 int vcp_profile_residual(vcp_ctx *c, int reps, double ms[2])
 {
+  VCP_GUARD_BEGIN
   if (!c || !ms || reps <= 0) return fail("invalid argument");
-  std::vector<hipEvent_t> ev(3 * reps);
-  for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+  if (int rc_ = not_in_batch(c)) return rc_;
+  vcp::EventSet es;
+  HIP_TRY(es.create(3 * (size_t) reps));
+  const std::vector<hipEvent_t> &ev = es.ev;
   vcp::StageCtl ctl{1, 0, 0.0, 0.0, 0};
   for (int r = 0; r < reps; ++r) {
     HIP_TRY(hipEventRecord(ev[3 * r], c->stream));
@@ -670,10 +715,10 @@ int vcp_profile_residual(vcp_ctx *c, int reps, double ms[2])
     HIP_TRY(hipEventElapsedTime(&b, ev[3 * r + 1], ev[3 * r + 2]));
     f += a; v += b;
   }
-  for (auto &e : ev) (void) hipEventDestroy(e);
   ms[0] = f / reps; ms[1] = v / reps;
   c->res_valid = true;
   return check_flags(c);
+  VCP_GUARD_END
 }
 
 }  // extern "C"

@@ -97,16 +97,19 @@ __device__ inline double wave_reduce_dpp(double v)
 }
 
 // Block reduction with one barrier: partial results go to alternating
-// halves of red (2 * NT/64 * NV doubles), so a half is rewritten only after
-// every thread has passed the barrier of the following reduction. The max
-// slots (q >= NS) take non-negative values. Only waves [W0, W0 + NW) hold
-// contributions; the others skip the wave reduction.
+// halves of red (2 * 3 * NT/64 doubles); every half starts at the same offset
+// whatever NV is, so a half is rewritten only after every thread has passed
+// the barrier of the following reduction, also when consecutive reductions
+// carry different numbers of values. The max slots (q >= NS) take
+// non-negative values. Only waves [W0, W0 + NW) hold contributions; the
+// others skip the wave reduction.
 template <int NT, int NV, int NS, int NW = NT / 64, int W0 = 0>
 __device__ inline void block_reduce1(double (&v)[NV], double *red, int &half)
 {
-  static_assert(W0 + NW <= NT / 64, "contributing waves");
+  static_assert(NV >= 1 && NV <= 3, "a half holds at most three values per wave");
+  static_assert(W0 >= 0 && W0 + NW <= NT / 64, "contributing waves");
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  double *buf = red + half * (NT / 64) * NV;
+  double *buf = red + half * 3 * (NT / 64);
   if (wid >= W0 && wid < W0 + NW) {
 #pragma unroll
     for (int q = 0; q < NV; ++q) v[q] = (q < NS) ? wave_reduce_dpp<false>(v[q]) : wave_reduce_dpp<true>(v[q]);
@@ -283,13 +286,16 @@ struct FaceOps {
   double w[M];                 // 1D Gauss weights
 };
 
+// Face kernel arguments: the case part (single-case launch) or the case
+// table and work items (batched launch), and the operators (the
+// metric-indexing mode is the kernel's BUG parameter).
 template <int K, int M>
 struct FaceArgs {
-  const double *U, *fgeo, *econst;
-  const int *nbr;
-  double *faceRes;
-  int nElem;                     // the metric-indexing mode is the kernel's BUG parameter
-  double mu, tauFact, thetaHalf;
+  CaseDev c;
+  const CaseDev *cases;
+  const WorkItem *work;
+  const FaceOps<K, M> *opg;      // batched launch: the operators in device memory
+  int w0;
   FaceOps<K, M> op;
 };
 
@@ -321,9 +327,15 @@ struct FaceSmem {
   double Y[kY];
 };
 
-template <int K, int M, int DIR, bool BUG>
-__device__ __forceinline__ void face_body(FaceArgs<K, M> a, FaceSmem<K, M> &sm, int e)
+// args: the kernel's argument block by value. The single-case kernel reads
+// the operators from it (run-time-indexed rows come from the kernel
+// arguments through scalar loads); the batched kernel reads them from the
+// device copy opg, since a by-value argument block that is indexed at run time
+// there is copied to scratch.
+template <int K, int M, int DIR, bool BUG, bool BATCH>
+__device__ __forceinline__ void face_body(FaceArgs<K, M> args, const CaseDev a, FaceSmem<K, M> &sm, int e)
 {
+#define VCP_FOPS (BATCH ? *args.opg : args.op)
   const int L = a.nbr[e * 6 + 2 * DIR];
   if (L < 0) return;  // physical boundary face
   constexpr int ND = K * K * K, N2 = M * M;
@@ -342,16 +354,16 @@ __device__ __forceinline__ void face_body(FaceArgs<K, M> a, FaceSmem<K, M> &sm, 
   const int tid = threadIdx.x;
   // operators in LDS: read as wave-uniform LDS operands in the contractions
   // (as kernel arguments they would not fit the scalar registers)
-  for (int i = tid; i < M * K; i += 64) { sA[i] = a.op.A[i]; sD[i] = a.op.D[i]; }
-  for (int i = tid; i < 4 * K; i += 64) sm.F[i / K][i % K] = a.op.F[i / K][i % K];
+  for (int i = tid; i < M * K; i += 64) { sA[i] = VCP_FOPS.A[i]; sD[i] = VCP_FOPS.D[i]; }
+  for (int i = tid; i < 4 * K; i += 64) sm.F[i / K][i % K] = VCP_FOPS.F[i / K][i % K];
   const double *A = sA, *D = sD;
   // operators of the pencil contractions: the kernel arguments (wave-uniform
   // scalar loads) while the argument block is under 1 KB; a larger one
   // (K >= 6) the compiler copies to scratch when it is indexed at run time,
   // so those read the LDS copies instead
   constexpr bool kArgOps = sizeof(FaceOps<K, M>) < 1024;
-  const double *KA = kArgOps ? a.op.A : sA, *KD = kArgOps ? a.op.D : sD;
-  for (int i = tid; i < N2; i += 64) sW2[i] = a.op.w[i % M] * a.op.w[i / M];
+  const double *KA = kArgOps ? VCP_FOPS.A : sA, *KD = kArgOps ? VCP_FOPS.D : sD;
+  for (int i = tid; i < N2; i += 64) sW2[i] = VCP_FOPS.w[i % M] * VCP_FOPS.w[i / M];
   const double *g = a.fgeo + ((size_t) e * 3 + DIR) * kFaceGeo * N2;
   if constexpr (BUG)
     for (int i = tid; i < 2 * 5 * 9; i += 64) {
@@ -536,17 +548,31 @@ __device__ __forceinline__ void face_body(FaceArgs<K, M> a, FaceSmem<K, M> &sm, 
       __syncthreads();
     }
   }
+#undef VCP_FOPS
 }
 
-// The three directions in one launch: blockIdx.y is the direction.
-template <int K, int M, bool BUG>
+// The three directions in one launch: blockIdx.y is the direction;
+// blockIdx.x the element, or (BATCH) a work item (case, element) of the
+// group. The argument block goes to face_body by value as a whole, so that
+// its run-time-indexed operator rows are read from the kernel arguments.
+template <int K, int M, bool BUG, bool BATCH>
 __global__ __launch_bounds__(64, (FaceSmem<K, M>::kWaves)) void face_kernel(FaceArgs<K, M> a)
 {
   __shared__ FaceSmem<K, M> sm;
-  const int e = blockIdx.x;
-  if (blockIdx.y == 0) face_body<K, M, 0, BUG>(a, sm, e);
-  else if (blockIdx.y == 1) face_body<K, M, 1, BUG>(a, sm, e);
-  else face_body<K, M, 2, BUG>(a, sm, e);
+  CaseDev c;
+  int e;
+  if constexpr (BATCH) {
+    const WorkItem w = a.work[a.w0 + blockIdx.x];
+    c = a.cases[w.c];
+    e = w.e;
+    if (c.flags[kWordActive] == 0) return;
+  } else {
+    c = a.c;
+    e = blockIdx.x;
+  }
+  if (blockIdx.y == 0) face_body<K, M, 0, BUG, BATCH>(a, c, sm, e);
+  else if (blockIdx.y == 1) face_body<K, M, 1, BUG, BATCH>(a, c, sm, e);
+  else face_body<K, M, 2, BUG, BATCH>(a, c, sm, e);
 }
 
 // 1D operators of the standard element as kernel arguments: indexed with
@@ -557,22 +583,18 @@ struct Ops {
   double D[M * K];   // phi'_q(r_p)
 };
 
+// Volume kernel arguments: the case part (single-case launch) or the case
+// table and work items (batched launch), the stage control and what depends
+// on (K, M) only.
 template <int K, int M>
 struct VolArgs {
-  double *U;
-  const double *U0;
-  double *R;
-  const double *faceRes, *tables, *vgeo, *econst;
-  double *scal;
-  int *flags, *cgIter;
-  double *mach2Elem;                      // [nElem] maximum Mach^2 (monitoring)
-  const int *spIdx;                       // sponge slot per element (null: no sponge)
-  const double *spCoef, *dampFn, *dampState;
-  double *srcBuf;
-  TableOffsets to;
-  int nElem, bug, cml;
-  double mu;
+  CaseDev c;
+  const CaseDev *cases;
+  const WorkItem *work;
+  int w0;
   StageCtl ctl;
+  TableOffsets to;
+  int bug;
   Ops<K, M> op;
   double c0[K];                           // phi_q(0): basis at the element centre
   double fvec[4][K];                      // phi(-1), phi(+1), phi'(-1), phi'(+1)
@@ -644,7 +666,9 @@ constexpr int vol_threads(int K, int M)
 // LDS layout of the volume kernel and the waves per SIMD it is compiled for
 // (NT = 256: three workgroups per CU when the LDS fits three times into 160 KB,
 // which caps the kernel at 168 VGPRs; NT = 512: one or more workgroups of
-// two waves per SIMD).
+// two waves per SIMD, or two workgroups when the LDS allows, which caps the
+// kernel at 128 VGPRs; NT = 64, 128: as many workgroups as the LDS allows, at
+// most three waves per SIMD).
 template <int K, int M, int NT>
 struct VolShape {
   static constexpr int ND = K * K * K, NI = M * M * M, NR = M * M;
@@ -659,19 +683,25 @@ struct VolShape {
   static constexpr int BIG = cmax(cmax(2 * SZ + SY, SF), SI + 10 * M * K * K);
   static constexpr int LDS = 8 * (2 * M * K + 5 * ND + BIG + 45 + 6 * (NT / 64) + 15 + 4 * K);
   static constexpr int WGS = (160 * 1024) / LDS;   // workgroups per CU by LDS
-  static constexpr int WAVES = NT == 256 ? (WGS >= 3 ? 3 : 2) : (NT / 256) * (WGS > 1 ? WGS : 1);
+  static constexpr int WAVES = NT == 256 ? (WGS >= 3 ? 3 : 2)
+                              : NT > 256 ? (NT / 256) * (WGS >= 2 ? 2 : 1)
+                                         : cmax(1, WGS * (NT / 64) / 4 < 3 ? WGS * (NT / 64) / 4 : 3);
   static_assert(WGS >= 1, "volume kernel LDS exceeds 160 KB");
   // mass-matrix solve buffers cz, cy, pI, by inside sBig
   static_assert(SZ + 5 * M * M * K + 5 * NI + 5 * M * K * K <= BIG, "CG buffers");
 };
 
-template <int K, int M, int NT>
-__global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel(VolArgs<K, M> a)
+// blockIdx.x is the element, or (BATCH) a work item (case, element) of the
+// group; a is the case part.
+template <int K, int M, int NT, bool BATCH>
+__global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel(VolArgs<K, M> args)
 {
   using S = VolShape<K, M, NT>;
   constexpr int ND = S::ND, NI = S::NI, NR = S::NR, R0 = S::R0, P0 = S::P0, P1 = S::P1;
   constexpr int SZ = S::SZ, SF0 = S::SF0, SI = S::SI, BIG = S::BIG;
   static_assert(P1 <= NT, "two point passes");
+  // rot() below wraps thread numbers with & (NT - 1)
+  static_assert((NT & (NT - 1)) == 0, "NT must be a power of two");
   // the first pass's fluxes must leave the y-stage rows of the second pass intact
   static_assert(P1 == 0 || SF0 <= 2 * SZ + 15 * R0 * K, "flux / y-stage overlap");
   __shared__ double sA[M * K], sD[M * K];
@@ -681,16 +711,29 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
   __shared__ double sRed[2 * (NT / 64) * 3];
   __shared__ double sDv[15];
   __shared__ double sFv[4 * K];
-  const int e = blockIdx.x;
+  // the case part is read where it is used: from the kernel arguments, or
+  // (BATCH) from the group's case table
+  const CaseDev *ap;
+  int e;
+  if constexpr (BATCH) {
+    const WorkItem w = args.work[args.w0 + blockIdx.x];
+    ap = args.cases + w.c;
+    e = w.e;
+    if (ap->flags[kWordActive] == 0) return;
+  } else {
+    ap = &args.c;
+    e = blockIdx.x;
+  }
+  const CaseDev &a = *ap;
   const int tid = threadIdx.x;
-  const double *A = a.op.A, *D = a.op.D;
+  const double *A = args.op.A, *D = args.op.D;
   const double *Ue = a.U + (size_t) e * 5 * ND;
   const double *vg = a.vgeo + (size_t) e * kVolGeo * NI;
   const int sp = a.spIdx ? a.spIdx[e] : -1;
   for (int i = tid; i < M * K; i += NT) { sA[i] = A[i]; sD[i] = D[i]; }
-  for (int i = tid; i < 4 * K; i += NT) sFv[i] = a.fvec[i / K][i % K];
+  for (int i = tid; i < 4 * K; i += NT) sFv[i] = args.fvec[i / K][i % K];
   for (int i = tid; i < 5 * ND; i += NT) sU[i] = Ue[i];
-  if (a.bug)
+  if (args.bug)
     for (int i = tid; i < 45; i += NT) sCm[i / 9][i % 9] = vg[(1 + i % 9) * NI + i / 9];
   __syncthreads();
 
@@ -723,9 +766,9 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
       double b = 0.0, bj = 0.0, bk = 0.0;
 #pragma unroll
       for (int q = 0; q < K; ++q) {
-        if (q == ii) b = a.c0[q];
-        if (q == jj) bj = a.c0[q];
-        if (q == kk) bk = a.c0[q];
+        if (q == ii) b = args.c0[q];
+        if (q == jj) bj = args.c0[q];
+        if (q == kk) bk = args.c0[q];
       }
       b *= bj * bk;
 #pragma unroll
@@ -805,12 +848,12 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
     double *Fb = q == 0 ? sBig + m : sBig + SF0 + (m - P0);
     const int fstride = q == 0 ? P0 : P1;
     const double J = vg[m];
-    const double wJ = a.tables[a.to.w3 + m] * J;
+    const double wJ = a.tables[args.to.w3 + m] * J;
     double cm[9];                                      // dr/dx .. dt/dz at the point
 #pragma unroll
     for (int t = 0; t < 9; ++t) cm[t] = vg[(1 + t) * NI + m];
     double gx[5], gy[5], gz[5];
-    if (a.bug) {
+    if (args.bug) {
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
         const double *c = sCm[v];
@@ -837,7 +880,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
     const double dwdx = -V4Inv * (gx[3] + w * gx[4]), dwdy = -V4Inv * (gy[3] + w * gy[4]), dwdz = -V4Inv * (gz[3] + w * gz[4]);
     const double fe = V4Inv * V4Inv * kInvGm1;
     const double dedx = fe * gx[4], dedy = fe * gy[4], dedz = fe * gz[4];
-    if (a.ctl.monitor) mach2 = fmax(mach2, rho * (u * u + v * v + w * w) / (kGam * p));
+    if (args.ctl.monitor) mach2 = fmax(mach2, rho * (u * u + v * v + w * w) / (kGam * p));
     const double ru = rho * u, rv = rho * v, rw = rho * w;
     const double mu = a.mu;
     const double divV = kLamOverMu * (dudx + dvdy + dwdz);
@@ -1036,7 +1079,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
     double red[2] = {r0, mach2};
     block_reduce1<NT, 2, 0>(red, sRed, half);
     r0 = red[0];
-    if (a.ctl.monitor && tid == 0) a.mach2Elem[e] = red[1];
+    if (args.ctl.monitor && tid == 0) a.mach2Elem[e] = red[1];
   }
   int iters = 0;
   if (r0 * JacInv > 1e-10) {
@@ -1223,15 +1266,15 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
   double *Re = a.R + (size_t) e * 5 * ND;
   double *Uw = a.U + (size_t) e * 5 * ND;
   const double *U0e = a.U0 + (size_t) e * 5 * ND;
-  const double bdt = a.ctl.update ? a.ctl.b * a.scal[kScalDt] : 0.0;
+  const double bdt = args.ctl.update ? args.ctl.b * a.scal[kScalDt] : 0.0;
   bool bad = false;
   if (own)
 #pragma unroll
     for (int l = 0; l < 5; ++l) {
       const int i = l * ND + tid;
       if (!isfinite(x[l])) bad = true;
-      if (a.ctl.store_res) Re[i] = x[l];
-      if (a.ctl.update) Uw[i] = (1.0 - a.ctl.a) * Uw[i] + a.ctl.a * U0e[i] - bdt * x[l];
+      if (args.ctl.store_res) Re[i] = x[l];
+      if (args.ctl.update) Uw[i] = (1.0 - args.ctl.a) * Uw[i] + args.ctl.a * U0e[i] - bdt * x[l];
     }
   if (bad) atomicOr(a.flags, kFlagNonFinite);
 }
@@ -1797,12 +1840,16 @@ __global__ __launch_bounds__(64) void nscbc_kernel(BndArgs<K, M> a)
 
 // ======================================================= time-step kernel
 struct DtArgs {
-  const double *U, *tables, *econst;
-  double *dtElem, *scal;
-  int *flags;
+  CaseDev c;
   TableOffsets to;
-  int nElem, P;
-  double cfl, u_ref, p_ref, rho_ref, muDim;
+  int P;
+};
+
+struct DtBatchArgs {
+  const CaseDev *cases;
+  const WorkItem *work;
+  TableOffsets to;
+  int P;
 };
 
 // In-place contraction of the 5 K^2 lines of sU along the axis of stride ST
@@ -1842,12 +1889,12 @@ __device__ __forceinline__ void lines_inplace(double *sU, const double *L, int l
 }
 
 template <int K>
-__global__ __launch_bounds__(64) void time_step_kernel(DtArgs a)
+__device__ __forceinline__ void time_step_body(const CaseDev &a, const TableOffsets &to, const int P, const int e)
 {
   constexpr int ND = K * K * K;
   __shared__ double sL[K * K], sU[5 * ND];
-  const int e = blockIdx.x, tid = threadIdx.x;
-  for (int i = tid; i < K * K; i += 64) sL[i] = a.tables[a.to.LD + i];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < K * K; i += 64) sL[i] = a.tables[to.LD + i];
   for (int i = tid; i < 5 * ND; i += 64) sU[i] = a.U[(size_t) e * 5 * ND + i];
   __syncthreads();
   // modal -> nodal values along i, j and k, in place
@@ -1884,9 +1931,24 @@ __global__ __launch_bounds__(64) void time_step_kernel(DtArgs a)
     const double f1[] = {2.0, 6.0, 12.0, 20.0, 28.0, 38.0, 50.0, 64.0, 80.0, 98.0};
     const double f2[] = {6.0, 36.0, 150.0, 420.0, 980.0, 1975.0, 3575.0, 7000.0, 14000.0, 28000.0};
     if (a2min < 0.0) atomicOr(a.flags, kFlagNegativeA2);
-    const double dt = a.cfl * a.u_ref / (f1[a.P] * radInv + f2[a.P] * radVis);
+    const double dt = a.cfl * a.u_ref / (f1[P] * radInv + f2[P] * radVis);
     a.dtElem[e] = dt;
   }
+}
+
+template <int K>
+__global__ __launch_bounds__(64) void time_step_kernel(DtArgs a)
+{
+  time_step_body<K>(a.c, a.to, a.P, blockIdx.x);
+}
+
+template <int K>
+__global__ __launch_bounds__(64) void time_step_batch_kernel(DtBatchArgs b)
+{
+  const WorkItem w = b.work[blockIdx.x];
+  const CaseDev c = b.cases[w.c];
+  if (c.flags[kWordActive] == 0) return;
+  time_step_body<K>(c, b.to, b.P, w.e);
 }
 
 // Running averages of the dimensional primitive variables and the velocity
@@ -1946,6 +2008,74 @@ __global__ __launch_bounds__(1024) void dt_min_kernel(const double *dtElem, int 
   }
 }
 
+// Per case of a batch group (one workgroup each): the minimum of the
+// per-element time steps as dt_min_kernel forms it, then the step control of
+// SolverClass::ExecuteTimeSynchronizationStep (SolverClass.cpp:2356-2365):
+// the step that reaches 0.99 of the synchronization time step is clipped to
+// end it exactly, and the next step starts a new one. A case with error flags
+// or no RK steps left is deactivated, so the batch's launches skip it.
+__global__ __launch_bounds__(1024) void batch_sync_kernel(const CaseDev *cases)
+{
+  __shared__ double red[16];
+  const CaseDev c = cases[blockIdx.x];
+  if (c.flags[kWordActive] == 0) return;
+  double m = 1e10;
+  for (int i = threadIdx.x; i < c.nElem; i += 1024) m = fmin(m, c.dtElem[i]);
+  m = -wave_max(-m);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = red[0];
+    for (int w = 1; w < 16; ++w) r = fmin(r, red[w]);
+    if (c.flags[kWordFlags] != 0 || c.flags[kWordStepsLeft] <= 0) {
+      c.flags[kWordActive] = 0;
+      return;
+    }
+    c.scal[0] = r;
+    double dt = r;
+    const double el = c.scal[kScalElapsed], dtSync = c.scal[kScalDtSync];
+    if (el + dt >= 0.99 * dtSync) {
+      dt = dtSync - el;
+      c.scal[kScalElapsed] = 0.0;
+      c.scal[kScalSyncDone] += 1.0;
+    } else {
+      c.scal[kScalElapsed] = el + dt;
+    }
+    c.scal[kScalDt] = dt;
+    c.flags[kWordStepsLeft] -= 1;
+    c.flags[kWordStepsDone] += 1;
+  }
+}
+
+// First RK stage of a batch: U0 <- U and U <- U - dt R per element, with the
+// case's time step (the same values stage0_kernel forms after its buffer swap).
+__global__ __launch_bounds__(256) void stage0_batch_kernel(const CaseDev *cases, const WorkItem *work, int nd5)
+{
+  const WorkItem w = work[blockIdx.x];
+  const CaseDev c = cases[w.c];
+  if (c.flags[kWordActive] == 0) return;
+  const double dt = c.scal[kScalDt];
+  const size_t base = (size_t) w.e * nd5;
+  for (int i = threadIdx.x; i < nd5; i += 256) {
+    const double u = c.U[base + i];
+    c.U0[base + i] = u;
+    c.U[base + i] = u - dt * c.R[base + i];
+  }
+}
+
+// Status words of the cases of a batch group, gathered for one readback.
+__global__ __launch_bounds__(64) void batch_status_kernel(const CaseDev *cases, int n, int *out)
+{
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  if (i >= n) return;
+  const CaseDev c = cases[i];
+  out[i * kStatusWords + 0] = c.flags[kWordActive];
+  out[i * kStatusWords + 1] = c.flags[kWordStepsDone];
+  out[i * kStatusWords + 2] = c.flags[kWordStepsLeft];
+  out[i * kStatusWords + 3] = c.flags[kWordFlags];
+  out[i * kStatusWords + 4] = (int) c.scal[kScalSyncDone];
+}
+
 // Maximum of n non-negative values into *out (one workgroup).
 __global__ __launch_bounds__(1024) void reduce_max_kernel(const double *v, int n, double *out)
 {
@@ -2003,22 +2133,48 @@ __global__ void set_dt_kernel(double *p, double v)
 }
 
 template <int K, int M>
-hipError_t faces_impl(const DeviceCase &dc, hipStream_t s)
+void fill_face_ops(const DeviceCase &dc, FaceOps<K, M> &op)
 {
-  FaceArgs<K, M> a;
-  a.U = dc.U; a.fgeo = dc.fgeo; a.econst = dc.econst; a.nbr = dc.nbr;
-  a.faceRes = dc.faceRes; a.nElem = dc.nElem;
-  a.mu = dc.mu; a.tauFact = (dc.P + 1.0) * (dc.P + 1.0) * dc.rel_penalty; a.thetaHalf = 0.5 * dc.theta_sym;
+  for (int i = 0; i < M * K; ++i) { op.A[i] = dc.opA[i]; op.D[i] = dc.opD[i]; }
+  for (int q = 0; q < K; ++q)
+    for (int t = 0; t < 4; ++t) op.F[t][q] = dc.opF[t][q];
+  for (int q = 0; q < M; ++q) op.w[q] = dc.opW[q];
+}
+
+template <int K, int M>
+void fill_vol_shape(const DeviceCase &dc, const StageCtl &ctl, VolArgs<K, M> &a)
+{
+  a.ctl = ctl; a.to = dc.to; a.bug = dc.bug_compat;
   for (int i = 0; i < M * K; ++i) { a.op.A[i] = dc.opA[i]; a.op.D[i] = dc.opD[i]; }
   for (int q = 0; q < K; ++q) {
-    a.op.F[0][q] = dc.opF[0][q]; a.op.F[1][q] = dc.opF[1][q];
-    a.op.F[2][q] = dc.opF[2][q]; a.op.F[3][q] = dc.opF[3][q];
+    a.c0[q] = dc.opC0[q];
+    for (int t = 0; t < 4; ++t) a.fvec[t][q] = dc.opF[t][q];
   }
-  for (int q = 0; q < M; ++q) a.op.w[q] = dc.opW[q];
+}
+
+template <int K, int M>
+hipError_t faces_impl(const DeviceCase &dc, hipStream_t s)
+{
+  FaceArgs<K, M> a{};
+  a.c = case_dev(dc);
+  fill_face_ops(dc, a.op);
   if (dc.bug_compat)
-    hipLaunchKernelGGL((face_kernel<K, M, true>), dim3(dc.nElem, 3), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((face_kernel<K, M, true, false>), dim3(dc.nElem, 3), dim3(64), 0, s, a);
   else
-    hipLaunchKernelGGL((face_kernel<K, M, false>), dim3(dc.nElem, 3), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((face_kernel<K, M, false, false>), dim3(dc.nElem, 3), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+template <int K, int M>
+hipError_t batch_faces_impl(const BatchGroup &g, int w0, int nw, hipStream_t s)
+{
+  FaceArgs<K, M> a{};
+  a.cases = g.cases; a.work = g.work; a.w0 = w0;
+  a.opg = static_cast<const FaceOps<K, M> *>(g.ops);
+  if (g.proto->bug_compat)
+    hipLaunchKernelGGL((face_kernel<K, M, true, true>), dim3(nw, 3), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL((face_kernel<K, M, false, true>), dim3(nw, 3), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 
@@ -2039,25 +2195,30 @@ hipError_t boundary_impl(const DeviceCase &dc, hipStream_t s, bool monitor)
   return hipGetLastError();
 }
 
-template <int K, int M>
-hipError_t volume_impl(const DeviceCase &dc, const StageCtl &ctl, hipStream_t s)
+// Volume workgroup sizes instantiated per (K, M), the default first: the
+// sizes for which the points fit two passes and the DOFs one thread each.
+#define VCP_TILES(X)                                                            \
+  X(3, 4, 256) X(3, 4, 128) X(3, 4, 64) X(4, 5, 256) X(4, 5, 128)              \
+  X(4, 6, 256) X(4, 6, 128) X(5, 7, 256) X(5, 7, 512) X(6, 8, 256) X(6, 8, 512) \
+  X(7, 9, 512)
+
+template <int K, int M, int NT>
+hipError_t volume_nt(const DeviceCase &dc, const StageCtl &ctl, hipStream_t s)
 {
-  VolArgs<K, M> a;
-  for (int i = 0; i < M * K; ++i) { a.op.A[i] = dc.opA[i]; a.op.D[i] = dc.opD[i]; }
-  a.U = dc.U; a.U0 = dc.U0; a.R = dc.R; a.faceRes = dc.faceRes; a.tables = dc.tables;
-  a.vgeo = dc.vgeo; a.econst = dc.econst; a.scal = dc.scal; a.flags = dc.flags; a.cgIter = dc.cgIter;
-  a.to = dc.to; a.nElem = dc.nElem; a.bug = dc.bug_compat; a.mu = dc.mu; a.ctl = ctl;
-  a.mach2Elem = dc.mach2Elem;
-  a.spIdx = dc.nSponge ? dc.spIdx : nullptr; a.spCoef = dc.spCoef; a.dampFn = dc.dampFn;
-  a.dampState = dc.dampState; a.srcBuf = dc.srcBuf; a.cml = dc.cml;
-  for (int q = 0; q < K; ++q) {
-    a.c0[q] = dc.opC0[q];
-    for (int t = 0; t < 4; ++t) a.fvec[t][q] = dc.opF[t][q];
-  }
-  constexpr int NT = vol_threads(K, M);
-  hipLaunchKernelGGL((volume_kernel<K, M, NT>), dim3(dc.nElem), dim3(NT), 0, s, a);
-  if (ctl.monitor)
-    hipLaunchKernelGGL(reduce_max_kernel, dim3(1), dim3(1024), 0, s, dc.mach2Elem, dc.nElem, dc.scal + 1);
+  VolArgs<K, M> a{};
+  a.c = case_dev(dc);
+  fill_vol_shape(dc, ctl, a);
+  hipLaunchKernelGGL((volume_kernel<K, M, NT, false>), dim3(dc.nElem), dim3(NT), 0, s, a);
+  return hipGetLastError();
+}
+
+template <int K, int M, int NT>
+hipError_t batch_volume_nt(const BatchGroup &g, const StageCtl &ctl, int w0, int nw, hipStream_t s)
+{
+  VolArgs<K, M> a{};
+  a.cases = g.cases; a.work = g.work; a.w0 = w0;
+  fill_vol_shape(*g.proto, ctl, a);
+  hipLaunchKernelGGL((volume_kernel<K, M, NT, true>), dim3(nw), dim3(NT), 0, s, a);
   return hipGetLastError();
 }
 
@@ -2065,15 +2226,39 @@ template <int K>
 hipError_t dt_impl(const DeviceCase &dc, hipStream_t s)
 {
   DtArgs a;
-  a.U = dc.U; a.tables = dc.tables; a.econst = dc.econst; a.dtElem = dc.dtElem; a.scal = dc.scal;
-  a.flags = dc.flags; a.to = dc.to; a.nElem = dc.nElem; a.P = dc.P; a.cfl = dc.cfl;
-  a.u_ref = dc.u_ref; a.p_ref = dc.p_ref; a.rho_ref = dc.rho_ref; a.muDim = dc.mu * dc.p_ref / dc.u_ref;
+  a.c = case_dev(dc); a.to = dc.to; a.P = dc.P;
   hipLaunchKernelGGL((time_step_kernel<K>), dim3(dc.nElem), dim3(64), 0, s, a);
   hipLaunchKernelGGL(dt_min_kernel, dim3(1), dim3(1024), 0, s, dc.dtElem, dc.nElem, dc.scal);
   return hipGetLastError();
 }
 
+template <int K>
+hipError_t batch_dt_impl(const BatchGroup &g, hipStream_t s)
+{
+  DtBatchArgs b;
+  b.cases = g.cases; b.work = g.work; b.to = g.proto->to; b.P = g.proto->P;
+  hipLaunchKernelGGL((time_step_batch_kernel<K>), dim3(g.nWork), dim3(64), 0, s, b);
+  hipLaunchKernelGGL(batch_sync_kernel, dim3(g.nCase), dim3(1024), 0, s, g.cases);
+  return hipGetLastError();
+}
+
 }  // namespace
+
+CaseDev case_dev(const DeviceCase &dc)
+{
+  CaseDev c{};
+  c.U = dc.U; c.U0 = dc.U0; c.R = dc.R; c.faceRes = dc.faceRes;
+  c.tables = dc.tables; c.vgeo = dc.vgeo; c.fgeo = dc.fgeo; c.econst = dc.econst; c.nbr = dc.nbr;
+  c.scal = dc.scal; c.flags = dc.flags; c.cgIter = dc.cgIter; c.dtElem = dc.dtElem;
+  c.mach2Elem = dc.mach2Elem;
+  c.spIdx = dc.nSponge ? dc.spIdx : nullptr; c.spCoef = dc.spCoef; c.dampFn = dc.dampFn;
+  c.dampState = dc.dampState; c.srcBuf = dc.srcBuf;
+  c.mu = dc.mu; c.tauFact = (dc.P + 1.0) * (dc.P + 1.0) * dc.rel_penalty; c.thetaHalf = 0.5 * dc.theta_sym;
+  c.cfl = dc.cfl; c.u_ref = dc.u_ref; c.p_ref = dc.p_ref; c.rho_ref = dc.rho_ref;
+  c.muDim = dc.mu * dc.p_ref / dc.u_ref;
+  c.nElem = dc.nElem; c.cml = dc.cml;
+  return c;
+}
 
 #define VCP_PAIRS(X) X(3, 4) X(4, 5) X(4, 6) X(5, 7) X(6, 8) X(7, 9)
 
@@ -2083,6 +2268,36 @@ bool supported_pair(int K, int M)
   VCP_PAIRS(X)
 #undef X
   return false;
+}
+
+size_t batch_ops_bytes(int K, int M)
+{
+#define X(k, m) if (K == k && M == m) return sizeof(FaceOps<k, m>);
+  VCP_PAIRS(X)
+#undef X
+  return 0;
+}
+
+void batch_ops_fill(const DeviceCase &dc, void *host)
+{
+#define X(k, m) if (dc.K == k && dc.M == m) fill_face_ops(dc, *static_cast<FaceOps<k, m> *>(host));
+  VCP_PAIRS(X)
+#undef X
+}
+
+int volume_tiles(int K, int M, int out[4])
+{
+  int n = 0;
+#define X(k, m, nt) if (K == k && M == m && n < 4) out[n++] = nt;
+  VCP_TILES(X)
+#undef X
+  return n;
+}
+
+int default_volume_tile(int K, int M)
+{
+  int t[4];
+  return volume_tiles(K, M, t) > 0 ? t[0] : 0;
 }
 
 hipError_t launch_faces(const DeviceCase &dc, hipStream_t s)
@@ -2103,10 +2318,16 @@ hipError_t launch_boundary(const DeviceCase &dc, hipStream_t s, bool monitor)
 
 hipError_t launch_volume(const DeviceCase &dc, const StageCtl &ctl, hipStream_t s)
 {
-#define X(k, m) if (dc.K == k && dc.M == m) return volume_impl<k, m>(dc, ctl, s);
-  VCP_PAIRS(X)
+  const int nt = dc.nt ? dc.nt : default_volume_tile(dc.K, dc.M);
+  const hipError_t e = [&]() -> hipError_t {
+#define X(k, m, t) if (dc.K == k && dc.M == m && nt == t) return volume_nt<k, m, t>(dc, ctl, s);
+    VCP_TILES(X)
 #undef X
-  return hipErrorInvalidValue;
+    return hipErrorInvalidValue;
+  }();
+  if (e == hipSuccess && ctl.monitor)
+    hipLaunchKernelGGL(reduce_max_kernel, dim3(1), dim3(1024), 0, s, dc.mach2Elem, dc.nElem, dc.scal + 1);
+  return e == hipSuccess ? hipGetLastError() : e;
 }
 
 hipError_t launch_time_step(const DeviceCase &dc, hipStream_t s)
@@ -2156,6 +2377,55 @@ hipError_t launch_stage0(const DeviceCase &dc, const double *dt, hipStream_t s)
   const int blocks = (int) ((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536);
   hipLaunchKernelGGL(stage0_kernel, dim3(blocks), dim3(256), 0, s, dc.U, dc.U0, dc.R, n,
                      dt ? *dt : 0.0, dt ? nullptr : dc.scal + kScalDt, dt ? dc.scal + kScalDt : nullptr);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_faces(const BatchGroup &g, int w0, int nw, hipStream_t s)
+{
+  if (nw <= 0) return hipSuccess;
+  const DeviceCase &p = *g.proto;
+#define X(k, m) if (p.K == k && p.M == m) return batch_faces_impl<k, m>(g, w0, nw, s);
+  VCP_PAIRS(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_batch_volume(const BatchGroup &g, const StageCtl &ctl, int w0, int nw, hipStream_t s)
+{
+  if (nw <= 0) return hipSuccess;
+  const DeviceCase &p = *g.proto;
+  const int nt = g.nt ? g.nt : default_volume_tile(p.K, p.M);
+#define X(k, m, t) if (p.K == k && p.M == m && nt == t) return batch_volume_nt<k, m, t>(g, ctl, w0, nw, s);
+  VCP_TILES(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_batch_time_step(const BatchGroup &g, hipStream_t s)
+{
+  if (g.nWork <= 0) return hipSuccess;
+  switch (g.proto->K) {
+    case 3: return batch_dt_impl<3>(g, s);
+    case 4: return batch_dt_impl<4>(g, s);
+    case 5: return batch_dt_impl<5>(g, s);
+    case 6: return batch_dt_impl<6>(g, s);
+    case 7: return batch_dt_impl<7>(g, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_batch_status(const BatchGroup &g, int *out, hipStream_t s)
+{
+  if (g.nCase <= 0) return hipSuccess;
+  hipLaunchKernelGGL(batch_status_kernel, dim3((g.nCase + 63) / 64), dim3(64), 0, s, g.cases, g.nCase, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_stage0(const BatchGroup &g, hipStream_t s)
+{
+  if (g.nWork <= 0) return hipSuccess;
+  hipLaunchKernelGGL(stage0_batch_kernel, dim3(g.nWork), dim3(256), 0, s, g.cases, g.work,
+                     5 * g.proto->nDOF);
   return hipGetLastError();
 }
 

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
+
 namespace vcp {
 
 // Offsets (in doubles) of the 1D tables in the device table block.
@@ -38,6 +40,7 @@ enum : int { kFlagCGNotConverged = 1, kFlagNegativeA2 = 2, kFlagNonFinite = 4 };
 struct DeviceCase {
   int nElem = 0, K = 0, M = 0, nDOF = 0, nInt = 0, nInt2D = 0, P = 0;
   int bug_compat = 1;
+  int nt = 0;                     // volume workgroup size (0: the default for (K, M))
   double mu = 0, theta_sym = 1, rel_penalty = 1;
   double cfl = 1, u_ref = 1, p_ref = 1, rho_ref = 1;
   TableOffsets to{};
@@ -77,7 +80,33 @@ struct DeviceCase {
   double *srcBuf = nullptr;       // [nSponge][5][nInt] source terms of the current residual
 };
 
-constexpr int kScalDt = 2, kScalForce = 3, kScalCount = 16;
+// Device scalars of a case (DeviceCase::scal): [kScalDt], [kScalForce ..
+// +6) as above; the per-case control of a batch: [kScalElapsed] time elapsed
+// in the current synchronization step, [kScalSyncDone] synchronization steps
+// completed, [kScalDtSync] the synchronization time step (non-dimensional).
+constexpr int kScalDt = 2, kScalForce = 3, kScalElapsed = 9, kScalSyncDone = 10, kScalDtSync = 12,
+              kScalCount = 16;
+// Device words of a case (DeviceCase::flags): [kWordFlags] error flags,
+// [kWordActive] 1 while a batch steps the case, [kWordStepsLeft] RK steps the
+// batch still takes for it, [kWordStepsDone] RK steps it took.
+constexpr int kWordFlags = 0, kWordActive = 1, kWordStepsLeft = 2, kWordStepsDone = 3, kWordCount = 4;
+
+// Per-case device pointers and scalars: the case part of the face, volume,
+// time-step and average kernels' arguments, and the entries of a batch's
+// device case table.
+struct CaseDev {
+  double *U, *U0, *R, *faceRes;
+  const double *tables, *vgeo, *fgeo, *econst;
+  const int *nbr;
+  double *scal;
+  int *flags, *cgIter;
+  double *dtElem, *mach2Elem;
+  const int *spIdx;               // sponge slot per element (null: no sponge)
+  const double *spCoef, *dampFn, *dampState;
+  double *srcBuf;
+  double mu, tauFact, thetaHalf, cfl, u_ref, p_ref, rho_ref, muDim;
+  int nElem, cml;
+};
 
 // Stage control for the fused volume/mass-matrix kernel.
 struct StageCtl {
@@ -87,7 +116,34 @@ struct StageCtl {
   int monitor;        // Mach^2 max
 };
 
+CaseDev case_dev(const DeviceCase &dc);
+
+// Work item of a batched launch: (index into the group's case table, element).
+struct WorkItem {
+  int c, e;
+};
+
+// One shape group of a batch: cases with the same (K, M) and metric-indexing
+// mode, run by one launch per kernel over a work list of (case, element)
+// items. proto is a member case: the 1D operators and table offsets, which
+// depend on (K, M) only.
+struct BatchGroup {
+  const DeviceCase *proto = nullptr;
+  const CaseDev *cases = nullptr;   // device [nCase]
+  const WorkItem *work = nullptr;   // device [nWork], ordered by case then element
+  const void *ops = nullptr;        // device copy of the face operators (batch_ops_fill)
+  int nCase = 0, nWork = 0;
+  int nt = 0;                       // volume workgroup size, 0: the default
+};
+
 bool supported_pair(int K, int M);
+// Bytes of a group's face-operator block and its contents (host memory) for
+// the (K, M) of dc; the batch keeps a device copy (BatchGroup::ops).
+size_t batch_ops_bytes(int K, int M);
+void batch_ops_fill(const DeviceCase &dc, void *host);
+// Volume workgroup sizes available for (K, M); the first is the default.
+int volume_tiles(int K, int M, int out[4]);
+int default_volume_tile(int K, int M);
 hipError_t launch_faces(const DeviceCase &dc, hipStream_t s);
 // boundary-averaged Mach number partial sums, then the PC-NSCBC faces
 // with monitor, also the wall forces into scal[kScalForce ..]
@@ -101,5 +157,20 @@ hipError_t launch_set_dt(const DeviceCase &dc, double dt, hipStream_t s);
 // U = U0 - dt R: with dt given, the kernel also stores it to scal[kScalDt];
 // with dt null it reads scal[kScalDt]
 hipError_t launch_stage0(const DeviceCase &dc, const double *dt, hipStream_t s);
+
+// Batched launches over the work items [w0, w0 + nw) of a group (cases whose
+// kWordActive word is 0 are skipped).
+hipError_t launch_batch_faces(const BatchGroup &g, int w0, int nw, hipStream_t s);
+hipError_t launch_batch_volume(const BatchGroup &g, const StageCtl &ctl, int w0, int nw, hipStream_t s);
+// per-element time steps, then per case: the minimum, the synchronization-step
+// clipping into scal[kScalDt], and the step counters; a case with error flags
+// or no steps left is deactivated instead
+hipError_t launch_batch_time_step(const BatchGroup &g, hipStream_t s);
+// U0 <- U, U <- U - dt R with each case's scal[kScalDt]
+hipError_t launch_batch_stage0(const BatchGroup &g, hipStream_t s);
+// per case of the group: out[c][kStatusWords] = active, RK steps done, RK
+// steps left, error flags, synchronization steps done
+constexpr int kStatusWords = 5;
+hipError_t launch_batch_status(const BatchGroup &g, int *out, hipStream_t s);
 
 }  // namespace vcp

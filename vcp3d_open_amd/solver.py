@@ -51,7 +51,10 @@ EXPORTS = ["vcp_create", "vcp_destroy", "vcp_sizes", "vcp_reference", "vcp_set_r
            "vcp_set_solution", "vcp_get_solution", "vcp_get_nodal", "vcp_residual",
            "vcp_get_residual", "vcp_sync_step", "vcp_set_graph", "vcp_get_averages", "vcp_set_averages",
            "vcp_time_steps", "vcp_cg_iterations",
-           "vcp_profile_residual", "vcp_host_setup", "vcp_last_error"]
+           "vcp_profile_residual", "vcp_host_setup", "vcp_set_tile", "vcp_volume_tiles",
+           "vcp_batch_create", "vcp_batch_destroy", "vcp_batch_add", "vcp_batch_remove",
+           "vcp_batch_step", "vcp_batch_wait", "vcp_batch_status", "vcp_batch_set_graph",
+           "vcp_batch_set_tile", "vcp_batch_groups", "vcp_batch_profile", "vcp_last_error"]
 
 _lib = None
 
@@ -87,6 +90,19 @@ def load_library(path: str = LIB_PATH):
     L.vcp_cg_iterations.argtypes = [vp, ip]
     L.vcp_profile_residual.argtypes = [vp, C.c_int, dp]
     L.vcp_host_setup.argtypes = [C.POINTER(CaseDesc), dp, dp, dp, dp, ip]
+    L.vcp_set_tile.argtypes = [vp, C.c_int]
+    L.vcp_volume_tiles.argtypes = [C.c_int, C.c_int, ip]
+    L.vcp_batch_create.argtypes = [C.POINTER(vp)]
+    L.vcp_batch_destroy.argtypes = [vp]
+    L.vcp_batch_add.argtypes = [vp, vp, C.c_int]
+    L.vcp_batch_remove.argtypes = [vp, vp]
+    L.vcp_batch_step.argtypes = [vp, C.c_int]
+    L.vcp_batch_wait.argtypes = [vp]
+    L.vcp_batch_status.argtypes = [vp, C.POINTER(vp), ip, C.c_int]
+    L.vcp_batch_set_graph.argtypes = [vp, C.c_int]
+    L.vcp_batch_set_tile.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.vcp_batch_groups.argtypes = [vp, ip, C.c_int]
+    L.vcp_batch_profile.argtypes = [vp, C.c_int, dp, C.c_int]
     L.vcp_last_error.restype = C.c_char_p
     for name in EXPORTS:
         if name != "vcp_last_error":
@@ -183,8 +199,9 @@ class SolverClass:
         return md
 
     def ExecuteTimeSynchronizationStep(self) -> np.ndarray:
+        """Monitoring data; the number of RK steps taken is in last_rk_steps."""
         md = np.zeros(8)
-        self._check(self._L.vcp_sync_step(self.h, md))
+        self.last_rk_steps = self._check(self._L.vcp_sync_step(self.h, md))
         return md
 
     def WriteRestartSolution(self) -> np.ndarray:
@@ -253,3 +270,98 @@ class SolverClass:
         ms = np.zeros(2)
         self._check(self._L.vcp_profile_residual(self.h, reps, ms))
         return float(ms[0]), float(ms[1])
+
+    def set_tile(self, nt: int) -> None:
+        """Volume workgroup size (threads per element; 0: the default)."""
+        self._check(self._L.vcp_set_tile(self.h, int(nt)))
+
+
+def volume_tiles(K: int, M: int) -> list:
+    """Volume workgroup sizes available for (K, M), the default first."""
+    L = load_library()
+    out = np.zeros(4, dtype=np.int32)
+    n = L.vcp_volume_tiles(K, M, out)
+    if n < 0:
+        raise VCPError(L.vcp_last_error().decode())
+    return [int(x) for x in out[:n]]
+
+
+# status words of vcp_batch_status
+ACTIVE, STEPS_DONE, STEPS_LEFT, FLAGS, SYNC_DONE = range(5)
+
+
+class Batch:
+    """Many independent cases stepped together (vcp_batch_*): one launch per
+    kernel and (K, M) group per RK stage, per-case time steps and
+    synchronization-step clipping on the device. Members are SolverClass
+    objects; a member's state is read with its own accessors after wait()."""
+
+    def __init__(self, use_graph: bool = False, tiles: dict | None = None):
+        self._L = load_library()
+        h = C.c_void_p()
+        self._check(self._L.vcp_batch_create(C.byref(h)))
+        self.h = h
+        self.members = []
+        self.tiles = dict(tiles or {})
+        if use_graph:
+            self._check(self._L.vcp_batch_set_graph(self.h, 1))
+
+    def _check(self, rc: int) -> int:
+        if rc < 0:
+            raise VCPError(self._L.vcp_last_error().decode())
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.vcp_batch_destroy(self.h)
+            self.h = None
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, solver: "SolverClass", n_rk_steps: int = 0) -> None:
+        self._check(self._L.vcp_batch_add(self.h, solver.h, int(n_rk_steps)))
+        self.members.append(solver)
+        nt = self.tiles.get((solver.K, solver.M))
+        if nt:
+            self._check(self._L.vcp_batch_set_tile(self.h, solver.K, solver.M, int(nt)))
+
+    def remove(self, solver: "SolverClass") -> None:
+        self._check(self._L.vcp_batch_remove(self.h, solver.h))
+        self.members = [m for m in self.members if m is not solver]
+
+    def set_tile(self, K: int, M: int, nt: int) -> None:
+        self.tiles[(K, M)] = nt
+        self._check(self._L.vcp_batch_set_tile(self.h, K, M, int(nt)))
+
+    def step(self, n_steps: int = 1) -> None:
+        """Queues n_steps RK steps of every active member (no wait)."""
+        self._check(self._L.vcp_batch_step(self.h, int(n_steps)))
+
+    def wait(self) -> None:
+        self._check(self._L.vcp_batch_wait(self.h))
+
+    def status(self) -> dict:
+        """{solver: status words} (waits for the queued work)."""
+        n = max(len(self.members), 1)
+        ctxs = (C.c_void_p * n)()
+        st = np.zeros((n, 5), dtype=np.int32)
+        k = self._check(self._L.vcp_batch_status(self.h, ctxs, st, n))
+        by_handle = {m.h.value: m for m in self.members}
+        return {by_handle[ctxs[i]]: st[i].copy() for i in range(k)}
+
+    def groups(self) -> list:
+        info = np.zeros((64, 6), dtype=np.int32)
+        n = self._check(self._L.vcp_batch_groups(self.h, info, 64))
+        return [dict(K=int(r[0]), M=int(r[1]), bug=int(r[2]), cases=int(r[3]), work=int(r[4]),
+                     nt=int(r[5])) for r in info[:n]]
+
+    def profile(self, reps: int = 5) -> list:
+        """Per group: (ms of boundary + face kernels, ms of the volume kernel)."""
+        ms = np.zeros(128)
+        n = self._check(self._L.vcp_batch_profile(self.h, int(reps), ms, 64))
+        return [(float(ms[2 * g]), float(ms[2 * g + 1])) for g in range(n)]

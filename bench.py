@@ -164,6 +164,114 @@ def cpu_baseline(case, workload: str) -> dict:
     return ref
 
 
+def bench_batch(args) -> int:
+    """Config-3 workload: args.cases Taylor-Green cases (P uniform in 2..6, n^3
+    elements with n uniform in [4, 24], Mach, Reynolds and phase drawn from
+    one seed) stepped together by one batch: every kernel of an RK stage is
+    one launch per (K, M) shape over all cases of that shape. value = the
+    aggregate GDOF-updates/s per RK stage over all cases."""
+    from vcp3d_open_amd import autotune
+    from vcp3d_open_amd.cases import batch_workload
+    from vcp3d_open_amd.solver import Batch, SolverClass
+
+    t_setup = time.perf_counter()
+    cases = batch_workload(seed=args.seed, count=args.cases)
+    batch = Batch(use_graph=args.graph)
+    members = []
+    for c in cases:
+        g = SolverClass(c)
+        g.ReadRestartSolution(c.initial_conservative())
+        batch.add(g, 0)
+        members.append(g)
+    t_setup = time.perf_counter() - t_setup
+    print(f"[bench] {len(cases)} cases set up in {t_setup:.1f} s", file=sys.stderr, flush=True)
+    if args.autotune:
+        tuned = autotune.tune(batch, reps=args.profile_reps)
+        autotune.save(tuned)
+        tiles = {k: v["nt"] for k, v in tuned.items()}
+    else:
+        tiles = autotune.load_tiles()
+        for (K, M), nt in tiles.items():
+            if any((g["K"], g["M"]) == (K, M) for g in batch.groups()):
+                batch.set_tile(K, M, nt)
+    batch.step(args.warmup)
+    batch.wait()
+    print(f"[bench] warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    batch.step(args.steps)
+    batch.wait()
+    secs = time.perf_counter() - t0
+    print(f"[bench] timed {args.steps} batched steps in {secs:.3f} s", file=sys.stderr, flush=True)
+    st = batch.status()
+    bad = [i for i, m in enumerate(members) if st[m][3] != 0]
+    if bad:
+        raise RuntimeError(f"cases {bad} raised error flags")
+    ndof = sum(5 * c.n_elems * c.n_dofs for c in cases)
+    value = ndof * 3 * args.steps / secs / 1e9
+
+    # dominant kernel: the volume launch of the shape group that takes longest
+    groups = batch.groups()
+    prof = batch.profile(args.profile_reps)
+    per_group = []
+    for g, (ms_face, ms_vol) in zip(groups, prof):
+        K, M = g["K"], g["M"]
+        mem = [m for m in members if (m.K, m.M) == (K, M)]
+        cg = float(sum(m.cg_iterations().sum() for m in mem) / max(1, sum(m.n_elem for m in mem)))
+        per_group.append({"P": K - 1, "K": K, "M": M, "cases": g["cases"], "elements": g["work"],
+                          "volume_threads": g["nt"], "ms_face_kernel": ms_face, "ms_volume_kernel": ms_vol,
+                          "cg_iters_mean": cg,
+                          "volume_tflops": volume_kernel_flops(K, M, cg) * g["work"] / (ms_vol * 1e-3) / 1e12})
+    top = max(per_group, key=lambda x: x["ms_volume_kernel"])
+    K, M = top["K"], top["M"]
+    flops = volume_kernel_flops(K, M, top["cg_iters_mean"]) * top["elements"]
+    bytes_ = volume_kernel_bytes(K, M) * top["elements"]
+    ms = top["ms_volume_kernel"]
+    achieved = flops / (ms * 1e-3) / 1e12
+    intensity = flops / bytes_
+    attainable = min(PEAK_FP64_TFLOPS, PEAK_HBM_GBS * intensity / 1e3)
+    roofline = {"bound": "valu-fp64" if attainable >= PEAK_FP64_TFLOPS else "hbm", "achieved": achieved,
+                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                "traffic": None, "kernel": f"volume_kernel<{K}, {M}, {top['volume_threads']}, batch>",
+                "ms_per_launch": ms, "algorithmic_flops": flops, "algorithmic_bytes": bytes_,
+                "intensity_flop_per_byte": intensity, "attainable_tflops": attainable,
+                "frac_of_attainable": achieved / attainable, "groups": per_group}
+    out = {
+        "metric": METRIC, "value": value, "unit": "GDOF/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": secs / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{len(cases)} batched Taylor-Green cases (seed {args.seed}): P uniform in "
+                               "2..6, n^3 hex with n uniform in [4, 24], Mach U[0.05, 0.4], Re log-U[400, 3200], "
+                               "random phases; FP64; one RK step of all cases per step",
+                   "cases": len(cases), "n_elem": sum(c.n_elems for c in cases), "dofs": ndof,
+                   "graph": bool(args.graph), "volume_threads": {f"{k},{m}": nt for (k, m), nt in tiles.items()},
+                   "setup_s": t_setup},
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        # the oracle on a bounded sample: one RK step of the smallest case of each degree
+        sample = []
+        for p in sorted({c.p for c in cases}):
+            sample.append(min((c for c in cases if c.p == p), key=lambda c: c.n_elems))
+        tot_dof, tot_s, threads = 0.0, 0.0, cpu_threads()
+        from oracle.pyoracle import Oracle, build
+        build()
+        for c in sample:
+            o = Oracle(c)
+            o.set_restart(c.initial_conservative())
+            o.residual()
+            t1 = time.perf_counter()
+            o.sync_step()
+            tot_s += time.perf_counter() - t1
+            tot_dof += 5 * c.n_elems * c.n_dofs * 3
+        out["cpu_baseline"] = {"value": tot_dof / tot_s / 1e9, "unit": "GDOF/s", "cores": threads, "kind": "port",
+                               "sample": "1 SSP-RK3 step of the smallest case of each degree of the batch ("
+                                         + ", ".join(f"P={c.p} n={c.n}" for c in sample)
+                                         + f"), C oracle, {threads} OpenMP threads, {tot_s:.2f} s"}
+    print(json.dumps(out))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,13 +282,21 @@ def main() -> int:
     ap.add_argument("--profile-reps", type=int, default=5)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (HBM counter calibration)")
-    ap.add_argument("--workload", choices=["tgv", "openbox"], default="tgv",
+    ap.add_argument("--workload", choices=["tgv", "openbox", "batch"], default="tgv",
                     help="tgv: curvilinear Taylor-Green 32^3 P=4 (default); openbox: 48x24x24 P=4 "
-                         "PC-NSCBC in/outflow, 8-element sponge + CML, M=0.5")
+                         "PC-NSCBC in/outflow, 8-element sponge + CML, M=0.5; batch: 64 seeded "
+                         "Taylor-Green cases of uneven P and size stepped together")
+    ap.add_argument("--seed", type=int, default=2024, help="batch: seed of the case draw")
+    ap.add_argument("--cases", type=int, default=64, help="batch: number of cases")
+    ap.add_argument("--autotune", action="store_true",
+                    help="batch: time every volume workgroup size per shape and record the table")
+    ap.add_argument("--graph", action="store_true", help="batch: replay each RK step as a HIP graph")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         return 2
+    if args.workload == "batch":
+        return bench_batch(args)
 
     from vcp3d_open_amd.cases import OpenBoxCase, TGVCase
     from vcp3d_open_amd.solver import SolverClass

@@ -234,7 +234,8 @@ int vcp_batch_remove(vcp_batch *b, vcp_ctx *ctx);
 int vcp_batch_step(vcp_batch *b, int n_steps);
 int vcp_batch_wait(vcp_batch *b);
 /* Waits, then writes the members (up to cap) and their status words
-   status[i][5] = {active, RK steps done, RK steps left, error flags,
+   status[i][5] = {active (the next RK step of the batch advances the case:
+   no error flags and steps left), RK steps done, RK steps left, error flags,
    synchronization steps done}; returns the number of members. */
 int vcp_batch_status(vcp_batch *b, vcp_ctx **ctxs, int *status, int cap);
 /* Replay each RK step of the batch as one captured HIP graph (recaptured

@@ -11,7 +11,9 @@ from vcp3d_open_amd.cases import TGVCase
 from vcp3d_open_amd.solver import host_setup
 
 
-@pytest.mark.parametrize("kw", [dict(n=3, p=2), dict(n=4, p=4, curvilinear=True), dict(n=4, p=5, curvilinear=True)])
+# n = 8 (512 elements) runs the element loops on several host threads
+@pytest.mark.parametrize("kw", [dict(n=3, p=2), dict(n=4, p=4, curvilinear=True), dict(n=4, p=5, curvilinear=True),
+                                dict(n=8, p=3, curvilinear=True)])
 def test_host_setup_matches_oracle(oracle_lib, kw):
     case = TGVCase(**kw)
     h = host_setup(case)
@@ -25,6 +27,15 @@ def test_host_setup_matches_oracle(oracle_lib, kw):
         assert np.all(np.abs(h["faces"][f] - fm) <= 1e-12 * scale), f
     np.testing.assert_allclose(h["lens"], o.setup(7), rtol=1e-12)
     assert np.array_equal(h["nbr"], o.setup(8).astype(np.int32))     # bit-exact maps
+
+
+def test_batched_test_cases_pass_the_grid_checks():
+    """The seeded cases of the batched GPU tests are accepted by the grid
+    reader's checks (perturbed coarse grids fail its LGL spacing test)."""
+    from tests.test_gpu_batch import seeded_cases
+    for seed, count in ((7, 7), (11, 6)):
+        for kw in seeded_cases(seed=seed, count=count):
+            host_setup(TGVCase(**kw))
 
 
 def test_unsupported_grid_is_rejected():

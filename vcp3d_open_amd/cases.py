@@ -502,3 +502,28 @@ class OpenBoxCase:
         path = os.path.join(directory, "param.cfg")
         vio.write_param_file(path, self.params(), self.subfaces(), self.prescribed(), self.sponge_block())
         return path
+
+
+# ------------------------------------------------------------------------
+# Batched workload: many independent Taylor-Green cases
+# ------------------------------------------------------------------------
+BATCH_RULE = {2: 6, 3: 9, 4: 12, 5: 15, 6: 16}
+
+
+def batch_workload(seed: int = 2024, count: int = 64, n_range=(4, 24), p_range=(2, 6),
+                   curvilinear: bool = False) -> list:
+    """Taylor-Green cases with per-case values drawn from one seeded generator:
+    P uniform in p_range, rule 3P (P <= 4), 15 (P = 5), 16 (P = 6); n^3
+    elements with n uniform in n_range; Mach uniform in [0.05, 0.4]; Reynolds
+    log-uniform in [400, 3200]; phase shifts uniform in [0, 2 pi)^3."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        p = int(rng.integers(p_range[0], p_range[1] + 1))
+        n = int(rng.integers(n_range[0], n_range[1] + 1))
+        mach = float(rng.uniform(0.05, 0.4))
+        re = float(math.exp(rng.uniform(math.log(400.0), math.log(3200.0))))
+        phase = tuple(float(x) for x in rng.uniform(0.0, 2.0 * math.pi, 3))
+        out.append(TGVCase(n=n, p=p, rule=BATCH_RULE[p], mach=mach, reynolds=re, phase=phase,
+                           curvilinear=curvilinear, name=f"tgv{i}"))
+    return out

@@ -30,7 +30,6 @@ struct vcp_batch {
     size_t capCases = 0;
     vcp::WorkItem *dWork = nullptr;
     size_t capWork = 0;
-    void *dOps = nullptr;
     int *dStatus = nullptr;
     int nWork = 0;
     int initFrom = -1;                 // cases [initFrom, end) need their first residual
@@ -39,7 +38,7 @@ struct vcp_batch {
     {
       vcp::BatchGroup g;
       g.proto = cases.empty() ? nullptr : &cases[0]->dc;
-      g.cases = dCases; g.work = dWork; g.ops = dOps;
+      g.cases = dCases; g.work = dWork;
       g.nCase = (int) cases.size(); g.nWork = nWork; g.nt = nt;
       return g;
     }
@@ -47,9 +46,8 @@ struct vcp_batch {
     {
       if (dCases) (void) hipFree(dCases);
       if (dWork) (void) hipFree(dWork);
-      if (dOps) (void) hipFree(dOps);
       if (dStatus) (void) hipFree(dStatus);
-      dCases = nullptr; dWork = nullptr; dOps = nullptr; dStatus = nullptr;
+      dCases = nullptr; dWork = nullptr; dStatus = nullptr;
       capCases = capWork = 0;
     }
   };
@@ -125,13 +123,6 @@ int upload(vcp_batch *b)
     HIP_TRY(grow(&g.dWork, g.capWork, work.size()));
     HIP_TRY(hipMemcpy(g.dCases, tab.data(), tab.size() * sizeof(vcp::CaseDev), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(g.dWork, work.data(), work.size() * sizeof(vcp::WorkItem), hipMemcpyHostToDevice));
-    if (!g.dOps) {
-      const size_t nb = vcp::batch_ops_bytes(g.K, g.M);
-      std::vector<unsigned char> ops(nb);
-      vcp::batch_ops_fill(g.cases[0]->dc, ops.data());
-      HIP_TRY(hipMalloc(&g.dOps, nb));
-      HIP_TRY(hipMemcpy(g.dOps, ops.data(), nb, hipMemcpyHostToDevice));
-    }
   }
   b->dirty = false;
   return 0;
@@ -234,14 +225,26 @@ int vcp_batch_destroy(vcp_batch *b)
 {
   VCP_GUARD_BEGIN
   if (!b) return 0;
-  (void) hipStreamSynchronize(b->stream);
+  // the members are stepped alone from now on; the batch is freed whatever
+  // the outcome, and the first failure is reported
+  hipError_t he = hipStreamSynchronize(b->stream);
   for (vcp_batch::Group &g : b->groups)
     for (vcp_ctx *c : g.cases) {
+      int words[vcp::kWordCount] = {};
+      hipError_t e = hipMemcpy(words, c->dc.flags, sizeof(words), hipMemcpyDeviceToHost);
+      if (e == hipSuccess) {
+        c->time_step += words[vcp::kWordStepsDone];
+        words[vcp::kWordActive] = 1;
+        words[vcp::kWordStepsLeft] = 0;
+        e = hipMemcpy(c->dc.flags, words, sizeof(words), hipMemcpyHostToDevice);
+      }
+      if (he == hipSuccess) he = e;
       c->batch = nullptr;
       c->res_valid = true;
       c->dt_ready = false;
     }
   delete b;
+  if (he != hipSuccess) return fail(std::string("vcp_batch_destroy: ") + hipGetErrorString(he), -3);
   return 0;
   VCP_GUARD_END
 }
@@ -263,7 +266,7 @@ int vcp_batch_add(vcp_batch *b, vcp_ctx *c, int n_rk_steps)
   words[vcp::kWordActive] = 1;
   words[vcp::kWordStepsLeft] = n_rk_steps > 0 ? n_rk_steps : INT_MAX;
   words[vcp::kWordStepsDone] = 0;
-  const double sc[4] = {0.0, 0.0, 0.0, c->prm.dt_synchr * c->rc.u_ref};   // elapsed, sync done, -, dtSync
+  const double sc[4] = {0.0, 0.0, 0.0, c->dt_sync};   // elapsed, sync done, -, dtSync
   static_assert(vcp::kScalSyncDone == vcp::kScalElapsed + 1 && vcp::kScalDtSync == vcp::kScalElapsed + 3,
                 "control scalars layout");
   HIP_TRY(hipMemcpy(c->dc.flags, words, sizeof(words), hipMemcpyHostToDevice));
@@ -295,7 +298,7 @@ int vcp_batch_remove(vcp_batch *b, vcp_ctx *c)
   int words[vcp::kWordCount] = {};
   HIP_TRY(hipMemcpy(words, c->dc.flags, sizeof(words), hipMemcpyDeviceToHost));
   c->time_step += words[vcp::kWordStepsDone];
-  words[vcp::kWordActive] = 0;
+  words[vcp::kWordActive] = 1;     // stepped alone from now on
   words[vcp::kWordStepsLeft] = 0;
   HIP_TRY(hipMemcpy(c->dc.flags, words, sizeof(words), hipMemcpyHostToDevice));
   g->cases.erase(g->cases.begin() + (long) i);

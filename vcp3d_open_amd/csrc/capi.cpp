@@ -208,6 +208,7 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
   const std::string err = vcp::build_geometry(p, c->tab, elem_grid, g);
   if (!err.empty()) return fail(err);
   c->rc = vcp::reference_conditions(p);
+  c->dt_sync = p.dt_synchr * c->rc.u_ref;
   vcp::OpenBoundary ob;
   std::string serr = vcp::setup_open_boundary(p, c->tab, g, c->rc, ob);
   if (serr.empty()) serr = vcp::setup_sponge(p, c->tab, g, elem_grid, c->sponge);
@@ -341,6 +342,11 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
   if (he == hipSuccess) he = hipMemset(dc.faceRes, 0, nslot * sizeof(double));
   if (he == hipSuccess) he = hipMemset(dc.flags, 0, 4 * sizeof(int));
   if (he == hipSuccess) he = hipMemset(dc.scal, 0, vcp::kScalCount * sizeof(double));
+  // a case is active outside a batch (the batch kernels skip inactive cases)
+  if (he == hipSuccess) {
+    const int active = 1;
+    he = hipMemcpy(dc.flags + vcp::kWordActive, &active, sizeof(int), hipMemcpyHostToDevice);
+  }
   if (he != hipSuccess) {
     return fail(std::string("device set-up failed: ") + hipGetErrorString(he), -3);
   }
@@ -553,7 +559,7 @@ int vcp_sync_step(vcp_ctx *c, double md[8])
   if (!md) md = dummy;
   int rc = 0;
   if (!c->res_valid && (rc = vcp_residual(c, 1, md))) return rc;
-  const double dtSync = c->prm.dt_synchr * c->rc.u_ref;
+  const double dtSync = c->dt_sync;
   double elapsed = 0.0;
   bool reached = false;
   int nSteps = 0;

@@ -55,6 +55,11 @@ struct vcp_ctx {
   bool res_valid = false;
   bool dt_ready = false;          // host[0] holds the time step of the current state
   bool damping_set = false;
+  // non-dimensional synchronization time step, rounded once at creation: the
+  // host loop of vcp_sync_step and a batch's device control both clip with
+  // this value (formed inside the clipping expression, the product could be
+  // fused into the subtraction and round differently)
+  double dt_sync = 0.0;
   int time_step = 0;
   int n_avg = 0;                  // synchronization steps in the running averages
   // RK steps replayed as captured HIP graphs (vcp_set_graph), one per

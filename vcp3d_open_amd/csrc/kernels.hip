@@ -294,7 +294,6 @@ struct FaceArgs {
   CaseDev c;
   const CaseDev *cases;
   const WorkItem *work;
-  const FaceOps<K, M> *opg;      // batched launch: the operators in device memory
   int w0;
   FaceOps<K, M> op;
 };
@@ -327,15 +326,14 @@ struct FaceSmem {
   double Y[kY];
 };
 
-// args: the kernel's argument block by value. The single-case kernel reads
-// the operators from it (run-time-indexed rows come from the kernel
-// arguments through scalar loads); the batched kernel reads them from the
-// device copy opg, since a by-value argument block that is indexed at run time
-// there is copied to scratch.
+// args: the kernel's argument block, read in place from the kernel-argument
+// segment by both forms, so that run-time-indexed operator rows come through
+// scalar loads and the single-case and batched kernels do the same
+// arithmetic.
 template <int K, int M, int DIR, bool BUG, bool BATCH>
-__device__ __forceinline__ void face_body(FaceArgs<K, M> args, const CaseDev a, FaceSmem<K, M> &sm, int e)
+__device__ __forceinline__ void face_body(const FaceArgs<K, M> &args, const CaseDev a, FaceSmem<K, M> &sm, int e)
 {
-#define VCP_FOPS (BATCH ? *args.opg : args.op)
+#define VCP_FOPS (args.op)
   const int L = a.nbr[e * 6 + 2 * DIR];
   if (L < 0) return;  // physical boundary face
   constexpr int ND = K * K * K, N2 = M * M;
@@ -553,12 +551,14 @@ __device__ __forceinline__ void face_body(FaceArgs<K, M> args, const CaseDev a, 
 
 // The three directions in one launch: blockIdx.y is the direction;
 // blockIdx.x the element, or (BATCH) a work item (case, element) of the
-// group. The argument block goes to face_body by value as a whole, so that
-// its run-time-indexed operator rows are read from the kernel arguments.
+// group. The argument block is read through the kernel-argument segment
+// pointer: a by-value copy indexed at run time would go to scratch.
 template <int K, int M, bool BUG, bool BATCH>
-__global__ __launch_bounds__(64, (FaceSmem<K, M>::kWaves)) void face_kernel(FaceArgs<K, M> a)
+__global__ __launch_bounds__(64, (FaceSmem<K, M>::kWaves)) void face_kernel(FaceArgs<K, M> a_)
 {
   __shared__ FaceSmem<K, M> sm;
+  (void) a_;
+  const FaceArgs<K, M> &a = *(const FaceArgs<K, M> *) __builtin_amdgcn_kernarg_segment_ptr();
   CaseDev c;
   int e;
   if constexpr (BATCH) {
@@ -1062,9 +1062,15 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
   }
 
   // ---------------- gather the six face contributions
+  // (a non-finite right-hand side is flagged: the solve's max-norms would
+  // pass over it)
+  bool bad = false;
   if (tid < ND)
 #pragma unroll
-    for (int l = 0; l < 5; ++l) r[l] += sU[l * ND + tid];
+    for (int l = 0; l < 5; ++l) {
+      r[l] += sU[l * ND + tid];
+      bad = bad || !isfinite(r[l]);
+    }
 
   // ---------------- entropy-variable mass matrix: flexible PCG
   const double JacInv = 8.0 / a.econst[(size_t) e * kElemConst + 3];
@@ -1267,7 +1273,6 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
   double *Uw = a.U + (size_t) e * 5 * ND;
   const double *U0e = a.U0 + (size_t) e * 5 * ND;
   const double bdt = args.ctl.update ? args.ctl.b * a.scal[kScalDt] : 0.0;
-  bool bad = false;
   if (own)
 #pragma unroll
     for (int l = 0; l < 5; ++l) {
@@ -1909,7 +1914,8 @@ __device__ __forceinline__ void time_step_body(const CaseDev &a, const TableOffs
   const double di = 1.0 / ec[0], dj = 1.0 / ec[1], dk = 1.0 / ec[2];
   const double dl2 = di * di + dj * dj + dk * dk;
   const double rvn = fmax(fmax(1.0, 2.0 + kLamOverMu), kGam / kPrLam) * dl2;
-  double radInv = 0.0, radVis = 0.0, a2min = 1e10;
+  double radInv = 0.0, radVis = 0.0;
+  int badA2 = 0;                     // a^2 negative or not a number at a DOF
   for (int d = tid; d < ND; d += 64) {
     double V[5];
 #pragma unroll
@@ -1924,13 +1930,14 @@ __device__ __forceinline__ void time_step_body(const CaseDev &a, const TableOffs
     const double c = sqrt(fabs(a2));
     radInv = fmax(radInv, (fabs(vI) + c) * di + (fabs(vJ) + c) * dj + (fabs(vK) + c) * dk);
     radVis = fmax(radVis, ri * a.muDim * rvn);
-    a2min = fmin(a2min, a2);
+    badA2 |= !(a2 >= 0.0);
   }
-  radInv = wave_max(radInv); radVis = wave_max(radVis); a2min = -wave_max(-a2min);
+  radInv = wave_max(radInv); radVis = wave_max(radVis);
+  const bool anyBad = __any(badA2);
   if (tid == 0) {
     const double f1[] = {2.0, 6.0, 12.0, 20.0, 28.0, 38.0, 50.0, 64.0, 80.0, 98.0};
     const double f2[] = {6.0, 36.0, 150.0, 420.0, 980.0, 1975.0, 3575.0, 7000.0, 14000.0, 28000.0};
-    if (a2min < 0.0) atomicOr(a.flags, kFlagNegativeA2);
+    if (anyBad) atomicOr(a.flags, kFlagNegativeA2);
     const double dt = a.cfl * a.u_ref / (f1[P] * radInv + f2[P] * radVis);
     a.dtElem[e] = dt;
   }
@@ -2063,13 +2070,16 @@ __global__ __launch_bounds__(256) void stage0_batch_kernel(const CaseDev *cases,
   }
 }
 
-// Status words of the cases of a batch group, gathered for one readback.
+// Status words of the cases of a batch group, gathered for one readback. A
+// case counts as active while the next RK step of the batch would advance it
+// (its own word is cleared only by that step's batch_sync_kernel).
 __global__ __launch_bounds__(64) void batch_status_kernel(const CaseDev *cases, int n, int *out)
 {
   const int i = blockIdx.x * 64 + threadIdx.x;
   if (i >= n) return;
   const CaseDev c = cases[i];
-  out[i * kStatusWords + 0] = c.flags[kWordActive];
+  out[i * kStatusWords + 0] =
+      c.flags[kWordActive] != 0 && c.flags[kWordFlags] == 0 && c.flags[kWordStepsLeft] > 0 ? 1 : 0;
   out[i * kStatusWords + 1] = c.flags[kWordStepsDone];
   out[i * kStatusWords + 2] = c.flags[kWordStepsLeft];
   out[i * kStatusWords + 3] = c.flags[kWordFlags];
@@ -2170,7 +2180,7 @@ hipError_t batch_faces_impl(const BatchGroup &g, int w0, int nw, hipStream_t s)
 {
   FaceArgs<K, M> a{};
   a.cases = g.cases; a.work = g.work; a.w0 = w0;
-  a.opg = static_cast<const FaceOps<K, M> *>(g.ops);
+  fill_face_ops(*g.proto, a.op);
   if (g.proto->bug_compat)
     hipLaunchKernelGGL((face_kernel<K, M, true, true>), dim3(nw, 3), dim3(64), 0, s, a);
   else
@@ -2197,8 +2207,11 @@ hipError_t boundary_impl(const DeviceCase &dc, hipStream_t s, bool monitor)
 
 // Volume workgroup sizes instantiated per (K, M), the default first: the
 // sizes for which the points fit two passes and the DOFs one thread each.
+// The defaults are the autotuner's choices on gfx950 (64 threads at (3, 4)
+// and 128 at (4, 5) run 2.4x and 1.6x faster than 256; the recorded table
+// is vcp3d_open_amd/tiles_gfx950.json).
 #define VCP_TILES(X)                                                            \
-  X(3, 4, 256) X(3, 4, 128) X(3, 4, 64) X(4, 5, 256) X(4, 5, 128)              \
+  X(3, 4, 64) X(3, 4, 128) X(3, 4, 256) X(4, 5, 128) X(4, 5, 256)              \
   X(4, 6, 256) X(4, 6, 128) X(5, 7, 256) X(5, 7, 512) X(6, 8, 256) X(6, 8, 512) \
   X(7, 9, 512)
 
@@ -2268,21 +2281,6 @@ bool supported_pair(int K, int M)
   VCP_PAIRS(X)
 #undef X
   return false;
-}
-
-size_t batch_ops_bytes(int K, int M)
-{
-#define X(k, m) if (K == k && M == m) return sizeof(FaceOps<k, m>);
-  VCP_PAIRS(X)
-#undef X
-  return 0;
-}
-
-void batch_ops_fill(const DeviceCase &dc, void *host)
-{
-#define X(k, m) if (dc.K == k && dc.M == m) fill_face_ops(dc, *static_cast<FaceOps<k, m> *>(host));
-  VCP_PAIRS(X)
-#undef X
 }
 
 int volume_tiles(int K, int M, int out[4])

@@ -87,8 +87,10 @@ struct DeviceCase {
 constexpr int kScalDt = 2, kScalForce = 3, kScalElapsed = 9, kScalSyncDone = 10, kScalDtSync = 12,
               kScalCount = 16;
 // Device words of a case (DeviceCase::flags): [kWordFlags] error flags,
-// [kWordActive] 1 while a batch steps the case, [kWordStepsLeft] RK steps the
-// batch still takes for it, [kWordStepsDone] RK steps it took.
+// [kWordActive] 1 while the case is stepped (always outside a batch; a batch
+// clears it for a case whose budget is spent or whose error flags are set, and
+// its kernels then skip the case), [kWordStepsLeft] RK steps the batch still
+// takes for it, [kWordStepsDone] RK steps it took.
 constexpr int kWordFlags = 0, kWordActive = 1, kWordStepsLeft = 2, kWordStepsDone = 3, kWordCount = 4;
 
 // Per-case device pointers and scalars: the case part of the face, volume,
@@ -131,16 +133,11 @@ struct BatchGroup {
   const DeviceCase *proto = nullptr;
   const CaseDev *cases = nullptr;   // device [nCase]
   const WorkItem *work = nullptr;   // device [nWork], ordered by case then element
-  const void *ops = nullptr;        // device copy of the face operators (batch_ops_fill)
   int nCase = 0, nWork = 0;
   int nt = 0;                       // volume workgroup size, 0: the default
 };
 
 bool supported_pair(int K, int M);
-// Bytes of a group's face-operator block and its contents (host memory) for
-// the (K, M) of dc; the batch keeps a device copy (BatchGroup::ops).
-size_t batch_ops_bytes(int K, int M);
-void batch_ops_fill(const DeviceCase &dc, void *host);
 // Volume workgroup sizes available for (K, M); the first is the default.
 int volume_tiles(int K, int M, int out[4]);
 int default_volume_tile(int K, int M);

@@ -152,6 +152,18 @@ int vcp_get_nodal(vcp_ctx *ctx, double *nodal);
    residual multiplied by the inverse (entropy) mass matrix. monitoring[8]
    as the reference: 6 force coefficients, max Mach^2, max eddy viscosity. */
 int vcp_residual(vcp_ctx *ctx, int compute_monitoring, double monitoring[8]);
+/* No counterpart in the reference: vcp_residual for callers that cannot
+   allocate device memory (code under HIP graph capture, serving loops,
+   harnesses that forbid hipMalloc). It makes no hipMalloc/hipFree or other
+   allocating call: the face-slot buffer that the face kernels fill and the
+   volume kernel gathers (60 (P+1)^2 doubles per element) lives in a fixed
+   64 MiB device array of the library, one per device, held until the
+   process ends, cleared at the start of each call. Cases whose buffer
+   exceeds it (e.g. more than 5592 elements at P = 4) are rejected with
+   code -7. Every call on a device shares that array, so calls that overlap
+   on one device (several host threads or streams) race. Results are
+   identical to vcp_residual's. */
+int vcp_residual_static(vcp_ctx *ctx, int compute_monitoring, double monitoring[8]);
 /* ElementClass::mRes of every element after vcp_residual / vcp_sync_step. */
 int vcp_get_residual(vcp_ctx *ctx, double *modal);
 

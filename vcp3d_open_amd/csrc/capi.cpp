@@ -537,6 +537,39 @@ int vcp_residual(vcp_ctx *c, int mon, double md[8])
   VCP_GUARD_END
 }
 
+int vcp_residual_static(vcp_ctx *c, int mon, double md[8])
+{
+  VCP_GUARD_BEGIN
+  if (!c) return fail("null context");
+  if (int rc_ = not_in_batch(c)) return rc_;
+  if (c->prm.sponge_layer && !c->damping_set)
+    return fail("sponge layer damping state not set (vcp_set_damping_state)");
+  // the face-slot buffer of the residual in the library's fixed device array
+  const size_t need = (size_t) c->dc.nElem * 6 * 10 * c->dc.K * c->dc.K;
+  void *scr = nullptr;
+  size_t cap = 0;
+  HIP_TRY(vcp::static_scratch(&scr, &cap));
+  if (need * sizeof(double) > cap)
+    return fail("vcp_residual_static: the face-slot buffer of this case (" + std::to_string(need * sizeof(double)) +
+                    " bytes) exceeds the fixed scratch array (" + std::to_string(cap) + " bytes)", -7);
+  vcp::DeviceCase d = c->dc;
+  d.faceRes = static_cast<double *>(scr);
+  // the array keeps the last call's contents: start from zero, as a new context does
+  HIP_TRY(vcp::launch_fill(d.faceRes, need, 0.0, c->stream));
+  const vcp::StageCtl ctl{1, 0, 0.0, 0.0, mon ? 1 : 0};
+  HIP_TRY(vcp::launch_boundary(d, c->stream, mon != 0));
+  HIP_TRY(vcp::launch_faces(d, c->stream));
+  HIP_TRY(vcp::launch_volume(d, ctl, c->stream));
+  double sc[vcp::kScalCount] = {};
+  HIP_TRY(hipMemcpyAsync(sc, c->dc.scal, kScalRead * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  int rc = 0;
+  if ((rc = check_flags(c))) return rc;
+  if (md) fill_monitoring(c, md, mon != 0, sc);
+  c->res_valid = true;
+  return 0;
+  VCP_GUARD_END
+}
+
 int vcp_time_steps(vcp_ctx *c, double *dt_elem)
 {
   VCP_GUARD_BEGIN

@@ -31,6 +31,13 @@
 #include "kernels.hpp"
 
 #include <cstdint>
+#include <mutex>
+
+// Fixed device array that vcp_residual_static uses for the face-slot buffer
+// instead of the context's own (for callers that cannot allocate device
+// memory). Host code never names it directly: its per-device address comes
+// from hipGetSymbolAddress (static_scratch).
+static __device__ __align__(256) double scratch[vcp::kStaticScratchBytes / sizeof(double)];
 
 namespace vcp {
 namespace {
@@ -2135,6 +2142,13 @@ __global__ void stage0_kernel(double *U, const double *U0, const double *R, size
     U[i] = U0[i] - dt * R[i];
 }
 
+// p[0, n) = v (a kernel, so that it can be captured in a graph).
+__global__ void fill_kernel(double *p, size_t n, double v)
+{
+  for (size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x; i < n; i += (size_t) gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
 // The time step of the RK step about to run, into device memory (one lane,
 // a vector store).
 __global__ void set_dt_kernel(double *p, double v)
@@ -2361,6 +2375,35 @@ hipError_t launch_average(const DeviceCase &dc, double factNew, hipStream_t s)
     case 7: return avg_impl<7>(dc, factNew, s);
   }
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_fill(double *p, size_t n, double v, hipStream_t s)
+{
+  if (n == 0) return hipSuccess;
+  const int blocks = (int) ((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096);
+  hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t static_scratch(void **ptr, size_t *bytes)
+{
+  // one address per device, looked up once with that device current
+  static std::mutex mu;
+  static void *addr[kMaxDevices] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void *a = nullptr;
+    e = hipGetSymbolAddress(&a, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[dev] = a;
+  }
+  *ptr = addr[dev];
+  *bytes = sizeof(scratch);
+  return hipSuccess;
 }
 
 hipError_t launch_set_dt(const DeviceCase &dc, double dt, hipStream_t s)

@@ -151,6 +151,14 @@ hipError_t launch_time_step(const DeviceCase &dc, hipStream_t s);
 hipError_t launch_average(const DeviceCase &dc, double factNew, hipStream_t s);
 // scal[kScalDt] = dt, ordered on the stream
 hipError_t launch_set_dt(const DeviceCase &dc, double dt, hipStream_t s);
+// p[0, n) = v on stream s
+hipError_t launch_fill(double *p, size_t n, double v, hipStream_t s);
+// The library's fixed device scratch array (64 MiB per device, held until the
+// process ends): its address on the current device, looked up once per device
+// with hipGetSymbolAddress, and its size. No allocation.
+constexpr size_t kStaticScratchBytes = size_t(64) << 20;
+constexpr int kMaxDevices = 64;
+hipError_t static_scratch(void **ptr, size_t *bytes);
 // U = U0 - dt R: with dt given, the kernel also stores it to scal[kScalDt];
 // with dt null it reads scal[kScalDt]
 hipError_t launch_stage0(const DeviceCase &dc, const double *dt, hipStream_t s);

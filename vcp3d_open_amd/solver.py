@@ -54,7 +54,8 @@ EXPORTS = ["vcp_create", "vcp_destroy", "vcp_sizes", "vcp_reference", "vcp_set_r
            "vcp_profile_residual", "vcp_host_setup", "vcp_set_tile", "vcp_volume_tiles",
            "vcp_batch_create", "vcp_batch_destroy", "vcp_batch_add", "vcp_batch_remove",
            "vcp_batch_step", "vcp_batch_wait", "vcp_batch_status", "vcp_batch_set_graph",
-           "vcp_batch_set_tile", "vcp_batch_groups", "vcp_batch_profile", "vcp_last_error"]
+           "vcp_batch_set_tile", "vcp_batch_groups", "vcp_batch_profile", "vcp_residual_static",
+           "vcp_last_error"]
 
 _lib = None
 
@@ -103,6 +104,7 @@ def load_library(path: str = LIB_PATH):
     L.vcp_batch_set_tile.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.vcp_batch_groups.argtypes = [vp, ip, C.c_int]
     L.vcp_batch_profile.argtypes = [vp, C.c_int, dp, C.c_int]
+    L.vcp_residual_static.argtypes = [vp, C.c_int, dp]
     L.vcp_last_error.restype = C.c_char_p
     for name in EXPORTS:
         if name != "vcp_last_error":

@@ -179,7 +179,7 @@ def bench_batch(args) -> int:
     batch = Batch(use_graph=args.graph)
     members = []
     for c in cases:
-        g = SolverClass(c)
+        g = SolverClass(c, precision=args.precision)
         g.ReadRestartSolution(c.initial_conservative())
         batch.add(g, 0)
         members.append(g)
@@ -238,11 +238,12 @@ def bench_batch(args) -> int:
     out = {
         "metric": METRIC, "value": value, "unit": "GDOF/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": secs / args.steps * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-contractions/f64",
         "data": "synthetic",
         "config": {"workload": f"{len(cases)} batched Taylor-Green cases (seed {args.seed}): P uniform in "
                                "2..6, n^3 hex with n uniform in [4, 24], Mach U[0.05, 0.4], Re log-U[400, 3200], "
-                               "random phases; FP64; one RK step of all cases per step",
+                               f"random phases; {args.precision}; one RK step of all cases per step",
                    "cases": len(cases), "n_elem": sum(c.n_elems for c in cases), "dofs": ndof,
                    "graph": bool(args.graph), "volume_threads": {f"{k},{m}": nt for (k, m), nt in tiles.items()},
                    "setup_s": t_setup},
@@ -291,6 +292,14 @@ def main() -> int:
     ap.add_argument("--autotune", action="store_true",
                     help="batch: time every volume workgroup size per shape and record the table")
     ap.add_argument("--graph", action="store_true", help="batch: replay each RK step as a HIP graph")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="arithmetic of the measured line: fp64 (default) or mixed (FP32 contractions and "
+                         "mass-matrix solve, FP64 state, pointwise physics, metric, boundaries and RK update)")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="tgv/openbox: skip the mixed-precision measurement beside the FP64 line")
+    ap.add_argument("--accuracy-steps", type=int, default=0,
+                    help="tgv/openbox: also advance FP64 and mixed this many RK steps from the same state "
+                         "and report their field, kinetic-energy and enstrophy differences")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -308,22 +317,29 @@ def main() -> int:
     else:
         case = TGVCase(n=args.n, p=4, rule=12, curvilinear=True)
         workload = f"curvilinear Taylor-Green vortex {args.n}^3 hex, P=4, rule 12, FP64"
-    solver = SolverClass(case)
-    solver.ReadRestartSolution(case.initial_conservative())
-    if args.workload == "openbox":
-        solver.SetDampingState(case.damping_primitive())
-    solver.Residual(True)
-    for _ in range(args.warmup):
-        solver.ExecuteTimeSynchronizationStep()
-    print(f"[bench] warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        md = solver.ExecuteTimeSynchronizationStep()
-    t1 = time.perf_counter()
-    secs = t1 - t0
+    def make(precision):
+        s_ = SolverClass(case, precision=precision)
+        s_.ReadRestartSolution(case.initial_conservative())
+        if args.workload == "openbox":
+            s_.SetDampingState(case.damping_primitive())
+        s_.Residual(True)
+        return s_
+
+    def timed(s_, label):
+        for _ in range(args.warmup):
+            s_.ExecuteTimeSynchronizationStep()
+        print(f"[bench] {label}: warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            md_ = s_.ExecuteTimeSynchronizationStep()
+        secs_ = time.perf_counter() - t0
+        print(f"[bench] {label}: timed {args.steps} steps in {secs_:.3f} s", file=sys.stderr, flush=True)
+        return secs_, md_
+
+    solver = make(args.precision)
+    secs, md = timed(solver, args.precision)
     ndof = 5 * case.n_elems * case.n_dofs
     value = ndof * 3 * args.steps / secs / 1e9
-    print(f"[bench] timed {args.steps} steps in {secs:.3f} s", file=sys.stderr, flush=True)
 
     # dominant kernel: per-launch times from HIP events on the library stream
     cg = solver.cg_iterations()
@@ -357,7 +373,8 @@ def main() -> int:
     out = {
         "metric": METRIC, "value": value, "unit": "GDOF/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": secs / args.steps * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-contractions/f64",
         "data": "synthetic",
         "config": {"workload": workload,
                    "n_elem": case.n_elems, "poly_degree": 4, "dofs": ndof, "mach": case.mach,
@@ -365,6 +382,39 @@ def main() -> int:
         "roofline": roofline,
         "mach2_max": float(md[6]),
     }
+    if args.precision == "fp64" and not args.no_mixed:
+        # the mixed-precision form on the same case, same steps, beside the FP64 line,
+        # with its difference from the FP64 state after the same steps
+        import numpy as np
+        from vcp3d_open_amd.diagnostics import flow_integrals
+        from vcp3d_open_amd.solver import host_setup
+        mixed = make("mixed")
+        msecs, _ = timed(mixed, "mixed")
+        mvalue = ndof * 3 * args.steps / msecs / 1e9
+        mf_face, mf_vol = mixed.profile_residual(args.profile_reps)
+        out["mixed"] = {"value": mvalue, "unit": "GDOF/s", "ms_per_step": msecs / args.steps * 1e3,
+                        "speedup_vs_fp64": mvalue / value, "dtype": "f32-contractions/f64",
+                        "ms_face_kernel": mf_face, "ms_volume_kernel": mf_vol,
+                        "cg_iters_mean": float(mixed.cg_iterations().mean())}
+        n_acc = args.accuracy_steps
+        ref_s, mix_s = solver, mixed
+        if n_acc:
+            ref_s, mix_s = make("fp64"), make("mixed")
+            for s_ in (ref_s, mix_s):
+                done = 0
+                while done < n_acc:
+                    s_.ExecuteTimeSynchronizationStep()
+                    done += s_.last_rk_steps
+        na, nb = mix_s.WriteRestartSolution(), ref_s.WriteRestartSolution()
+        vol = host_setup(case)["vol"]
+        fm, fo = flow_integrals(case, mix_s.solution(), vol), flow_integrals(case, ref_s.solution(), vol)
+        out["mixed"]["error_vs_fp64"] = {
+            "rk_steps": n_acc or args.warmup + args.steps,
+            "field_rel_l2": float(np.linalg.norm(na - nb) / np.linalg.norm(nb)),
+            "field_rel_l2_per_variable": [float(np.linalg.norm(na[..., v] - nb[..., v]) / np.linalg.norm(nb[..., v]))
+                                          for v in range(5)],
+            "kinetic_energy_rel": abs(fm["kinetic_energy"] / fo["kinetic_energy"] - 1),
+            "enstrophy_rel": abs(fm["enstrophy"] / fo["enstrophy"] - 1)}
     if args.calib_copy_mib:
         out["calib_copy_bytes"] = calib_copy(args.calib_copy_mib << 20)
     if not args.no_cpu_baseline:

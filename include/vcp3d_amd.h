@@ -214,12 +214,23 @@ int vcp_profile_residual(vcp_ctx *ctx, int reps, double ms[2]);
    sizes agree to rounding (the partial sums of the mass-matrix solve are
    grouped differently); a given size gives identical results every run. */
 int vcp_set_tile(vcp_ctx *ctx, int nt);
+
+/* Execution option with no counterpart in the reference (whose only precision
+   switch is the all-FP32 su2double typedef, VCP3D_CurviLinear.hpp:63-118):
+   VCP_PRECISION_MIXED runs the sum-factorisation contractions of the face and
+   volume kernels and the mass-matrix solve in FP32 (the solve stops at
+   ||r||_inf < 1e-5 ||r0||_inf), while the state and its element means, the
+   pointwise flux physics, the face-slot gather, the metric terms, the NSCBC
+   boundary kernels, the time step and the RK update stay in FP64.
+   VCP_PRECISION_FP64 (the default) is the reference's arithmetic. */
+enum { VCP_PRECISION_FP64 = 0, VCP_PRECISION_MIXED = 1 };
+int vcp_set_precision(vcp_ctx *ctx, int mode);
 int vcp_volume_tiles(int n_dofs_1d, int n_int_1d, int out[4]);
 
 /* ---------------------------------------------------------------- batches
    Many independent cases stepped together, with no counterpart in the
    reference (one SolverClass per case, SolverClass.hpp:40-124): the cases
-   are grouped by (nDOFs1D, nInt1D, bug_compat), and each kernel of an RK
+   are grouped by (nDOFs1D, nInt1D, bug_compat, precision), and each kernel of an RK
    stage is one launch per group over a work list of (case, element) items.
    Each case keeps its own time step and the synchronization-step clipping of
    ExecuteTimeSynchronizationStep (SolverClass.cpp:2356-2365) on the device:
@@ -255,8 +266,8 @@ int vcp_batch_status(vcp_batch *b, vcp_ctx **ctxs, int *status, int cap);
 int vcp_batch_set_graph(vcp_batch *b, int enable);
 /* Volume workgroup size of the group (nDOFs1D, nInt1D) (see vcp_set_tile). */
 int vcp_batch_set_tile(vcp_batch *b, int n_dofs_1d, int n_int_1d, int nt);
-/* Groups: info[g][6] = {nDOFs1D, nInt1D, bug_compat, cases, work items,
-   volume workgroup size}; returns the number of groups. */
+/* Groups: info[g][7] = {nDOFs1D, nInt1D, bug_compat, cases, work items,
+   volume workgroup size, precision}; returns the number of groups. */
 int vcp_batch_groups(vcp_batch *b, int *info, int cap);
 /* Measurement: mean milliseconds of `reps` residual evaluations per group,
    ms[2 g] boundary + face kernels, ms[2 g + 1] volume kernel (cap: entries

@@ -3,7 +3,7 @@
 // The reference runs one case per SolverClass (SolverClass.hpp:40-124) and
 // dispatches each tensor-product call on (nDOFs1D, nInt1D) at run time
 // (TensorProductSolAndGradVolume.cpp:27-56). Here a batch groups its cases by
-// (K, M, metric-indexing mode); each group is one work list of (case,
+// (K, M, metric-indexing mode, precision); each group is one work list of (case,
 // element) items over a device table of the cases' buffers, so every kernel
 // of an RK stage is one launch per group whatever the number of cases. The
 // per-case time step, the synchronization-step clipping of
@@ -23,7 +23,7 @@ using vcp::fail;
 
 struct vcp_batch {
   struct Group {
-    int K = 0, M = 0, bug = 0, nt = 0;
+    int K = 0, M = 0, bug = 0, mixed = 0, nt = 0;
     std::vector<vcp_ctx *> cases;      // case table order
     std::vector<int> first;            // first work item of each case, + total
     vcp::CaseDev *dCases = nullptr;
@@ -273,11 +273,11 @@ int vcp_batch_add(vcp_batch *b, vcp_ctx *c, int n_rk_steps)
   HIP_TRY(hipMemcpy(c->dc.scal + vcp::kScalElapsed, sc, sizeof(sc), hipMemcpyHostToDevice));
   vcp_batch::Group *grp = nullptr;
   for (vcp_batch::Group &g : b->groups)
-    if (g.K == c->dc.K && g.M == c->dc.M && g.bug == c->dc.bug_compat) grp = &g;
+    if (g.K == c->dc.K && g.M == c->dc.M && g.bug == c->dc.bug_compat && g.mixed == c->dc.mixed) grp = &g;
   if (!grp) {
     b->groups.emplace_back();
     grp = &b->groups.back();
-    grp->K = c->dc.K; grp->M = c->dc.M; grp->bug = c->dc.bug_compat;
+    grp->K = c->dc.K; grp->M = c->dc.M; grp->bug = c->dc.bug_compat; grp->mixed = c->dc.mixed;
   }
   if (grp->initFrom < 0) grp->initFrom = (int) grp->cases.size();
   grp->cases.push_back(c);
@@ -398,9 +398,10 @@ int vcp_batch_groups(vcp_batch *b, int *info, int cap)
   int n = 0;
   for (const vcp_batch::Group &g : b->groups) {
     if (n < cap) {
-      int *o = info + 6 * n;
+      int *o = info + 7 * n;
       o[0] = g.K; o[1] = g.M; o[2] = g.bug; o[3] = (int) g.cases.size(); o[4] = g.nWork;
       o[5] = g.nt ? g.nt : vcp::default_volume_tile(g.K, g.M);
+      o[6] = g.mixed;
     }
     ++n;
   }

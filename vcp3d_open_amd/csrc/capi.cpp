@@ -710,6 +710,20 @@ int vcp_set_tile(vcp_ctx *c, int nt)
   VCP_GUARD_END
 }
 
+int vcp_set_precision(vcp_ctx *c, int mode)
+{
+  VCP_GUARD_BEGIN
+  if (!c) return fail("null context");
+  if (int rc_ = not_in_batch(c)) return rc_;
+  if (mode != VCP_PRECISION_FP64 && mode != VCP_PRECISION_MIXED) return fail("unknown precision mode");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipStreamSynchronize(c->side));
+  c->dc.mixed = mode == VCP_PRECISION_MIXED ? 1 : 0;
+  c->drop_graphs();
+  return 0;
+  VCP_GUARD_END
+}
+
 int vcp_volume_tiles(int n_dofs_1d, int n_int_1d, int out[4])
 {
   VCP_GUARD_BEGIN

@@ -286,23 +286,24 @@ __device__ __forceinline__ void point_flux(const double VL[5], const double VR[5
           - (K23_24 * dV2 + vv * dV3 + K23_44 * dV4) * sy - (u * dV1 + vv * dV2 + K33_34 * dV3 + K33_44 * dV4) * sz;
 }
 
-template <int K, int M>
+template <int K, int M, typename T = double>
 struct FaceOps {
-  double A[M * K], D[M * K];   // phi_q(r_p), phi'_q(r_p), row-major (p, q)
-  double F[4][K];              // phi(-1), phi(+1), phi'(-1), phi'(+1)
+  T A[M * K], D[M * K];        // phi_q(r_p), phi'_q(r_p), row-major (p, q)
+  T F[4][K];                   // phi(-1), phi(+1), phi'(-1), phi'(+1)
   double w[M];                 // 1D Gauss weights
+  double c03;                  // phi_0^3, the value of the constant mode
 };
 
 // Face kernel arguments: the case part (single-case launch) or the case
 // table and work items (batched launch), and the operators (the
 // metric-indexing mode is the kernel's BUG parameter).
-template <int K, int M>
+template <int K, int M, typename T = double>
 struct FaceArgs {
   CaseDev c;
   const CaseDev *cases;
   const WorkItem *work;
   int w0;
-  FaceOps<K, M> op;
+  FaceOps<K, M, T> op;
 };
 
 // ============================================================ face kernel
@@ -316,35 +317,46 @@ struct FaceArgs {
 //   X: normal contraction B -> fast contraction T -> flux and Cartesian
 //      symmetrising fluxes at the points (over B, or over T when in place)
 //   Y: the first lift stage of one side, in two halves
-template <int K, int M>
+template <int K, int M, typename T = double>
 struct FaceSmem {
   static constexpr bool kInPlace = 2 * 5 * K <= 64 && M * M <= 64;
   static constexpr int kB = 2 * 2 * 5 * K * K, kT = 2 * 3 * 5 * K * M, kP = 20 * M * M;
   static constexpr int kOffT = kInPlace ? 0 : cmax(kB, kP);   // the fluxes start at 0
   static constexpr int kX = kInPlace ? cmax(cmax(kB, kT), kP) : kOffT + kT;
   static constexpr int kY = 2 * 5 * M * K;
-  static constexpr int kBytes = 8 * (2 * M * K + M * M + 4 * K + 90 + kX + kY);
+  static constexpr int kBytes = (int) sizeof(T) * (2 * M * K + 4 * K + kX + kY) + 8 * (M * M + 90 + 10);
   // waves per SIMD to compile for: three when 12 faces fit a CU's 160 KB
-  static constexpr int kWaves = 12 * kBytes <= 160 * 1024 ? 3 : 2;
-  double A[M * K], D[M * K], W2[M * M];
-  double F[4][K];                  // phi(-1), phi(+1), phi'(-1), phi'(+1)
+#ifndef VCP_F32_FACE_WAVES
+#define VCP_F32_FACE_WAVES 3
+#endif
+  static constexpr int kWaves = sizeof(T) == 4 && 4 * VCP_F32_FACE_WAVES * kBytes <= 160 * 1024 ? VCP_F32_FACE_WAVES
+                                : 12 * kBytes <= 160 * 1024 ? 3 : 2;
+  T A[M * K], D[M * K];
+  double W2[M * M];
+  T F[4][K];                       // phi(-1), phi(+1), phi'(-1), phi'(+1)
   double Cm[2][5][9];
-  double X[kX];
-  double Y[kY];
+  double Mean[2][5];               // float form: mode 0 of both elements
+  T X[kX];
+  T Y[kY];
 };
 
 // args: the kernel's argument block, read in place from the kernel-argument
 // segment by both forms, so that run-time-indexed operator rows come through
 // scalar loads and the single-case and batched kernels do the same
-// arithmetic.
-template <int K, int M, int DIR, bool BUG, bool BATCH>
-__device__ __forceinline__ void face_body(const FaceArgs<K, M> &args, const CaseDev a, FaceSmem<K, M> &sm, int e)
+// arithmetic. T: the type of the contractions (double, or float for the
+// mixed-precision form, where the traces carry the variation about each
+// element's mean, added back at the points in double, and the pointwise
+// fluxes stay in double).
+template <int K, int M, int DIR, bool BUG, bool BATCH, typename T>
+__device__ __forceinline__ void face_body(const FaceArgs<K, M, T> &args, const CaseDev a, FaceSmem<K, M, T> &sm,
+                                          int e)
 {
 #define VCP_FOPS (args.op)
+  constexpr bool kMixed = sizeof(T) == 4;
   const int L = a.nbr[e * 6 + 2 * DIR];
   if (L < 0) return;  // physical boundary face
   constexpr int ND = K * K * K, N2 = M * M;
-  using S = FaceSmem<K, M>;
+  using S = FaceSmem<K, M, T>;
   static_assert(5 * M <= 64 && 5 * K <= 64, "single-pass lift stages");
   // strides of the (normal, fast, slow) axes in the [k][j][i] DOF layout
   constexpr int SN = DIR == 0 ? 1 : DIR == 1 ? K : K * K;
@@ -352,22 +364,23 @@ __device__ __forceinline__ void face_body(const FaceArgs<K, M> &args, const Case
   constexpr int SSL = DIR == 2 ? K : K * K;
   // parametric slot of each axis: r = 0, s = 1, t = 2 -> (normal, fast, slow)
   constexpr int PN = DIR, PF = DIR == 0 ? 1 : 0, PS = DIR == 2 ? 1 : 2;
-  double *sA = sm.A, *sD = sm.D, *sW2 = sm.W2;
-  double *B = sm.X;                           // [side][value|deriv][5][K][K]
-  double *T = sm.X + S::kOffT;                // [side][A|D|N][5][K][M]
-  double *sFlux = sm.X, *sFC = sm.X + 5 * N2;  // flux [5][N2], Cartesian sym [x|y|z][5][N2]
+  T *sA = sm.A, *sD = sm.D;
+  double *sW2 = sm.W2;
+  T *B = sm.X;                                // [side][value|deriv][5][K][K]
+  T *Tb = sm.X + S::kOffT;                    // [side][A|D|N][5][K][M]
+  T *sFlux = sm.X, *sFC = sm.X + 5 * N2;      // flux [5][N2], Cartesian sym [x|y|z][5][N2]
   const int tid = threadIdx.x;
   // operators in LDS: read as wave-uniform LDS operands in the contractions
   // (as kernel arguments they would not fit the scalar registers)
   for (int i = tid; i < M * K; i += 64) { sA[i] = VCP_FOPS.A[i]; sD[i] = VCP_FOPS.D[i]; }
   for (int i = tid; i < 4 * K; i += 64) sm.F[i / K][i % K] = VCP_FOPS.F[i / K][i % K];
-  const double *A = sA, *D = sD;
+  const T *A = sA, *D = sD;
   // operators of the pencil contractions: the kernel arguments (wave-uniform
   // scalar loads) while the argument block is under 1 KB; a larger one
   // (K >= 6) the compiler copies to scratch when it is indexed at run time,
   // so those read the LDS copies instead
-  constexpr bool kArgOps = sizeof(FaceOps<K, M>) < 1024;
-  const double *KA = kArgOps ? VCP_FOPS.A : sA, *KD = kArgOps ? VCP_FOPS.D : sD;
+  constexpr bool kArgOps = sizeof(FaceOps<K, M, T>) < 1024;
+  const T *KA = kArgOps ? VCP_FOPS.A : sA, *KD = kArgOps ? VCP_FOPS.D : sD;
   for (int i = tid; i < N2; i += 64) sW2[i] = VCP_FOPS.w[i % M] * VCP_FOPS.w[i / M];
   const double *g = a.fgeo + ((size_t) e * 3 + DIR) * kFaceGeo * N2;
   if constexpr (BUG)
@@ -376,18 +389,24 @@ __device__ __forceinline__ void face_body(const FaceArgs<K, M> &args, const Case
       const int pt = l < N2 ? l : 0;
       sm.Cm[side][l][q] = g[(side ? 4 + q : 13 + q) * N2 + pt];
     }
+  if constexpr (kMixed)
+    if (tid < 10) sm.Mean[tid / 5][tid % 5] = a.U[(size_t) (tid < 5 ? e : L) * 5 * ND + (tid % 5) * ND];
   __syncthreads();
   // normal contraction straight from HBM: B[side][0|1][v][s][f]
 #pragma unroll
   for (int side = 0; side < 2; ++side) {
     const double *Us = a.U + (size_t) (side ? L : e) * 5 * ND;
-    const double *Fv = sm.F[side], *dFv = sm.F[2 + side];
+    const T *Fv = sm.F[side], *dFv = sm.F[2 + side];
     for (int pc = tid; pc < 5 * K * K; pc += 64) {
       const int v = pc / (K * K), sl = (pc / K) % K, f = pc % K;
       const double *u = Us + v * ND + sl * SSL + f * SFA;
-      double b0 = 0.0, b1 = 0.0;
+      T b0 = 0, b1 = 0;
 #pragma unroll
-      for (int t = 0; t < K; ++t) { const double x = u[t * SN]; b0 += Fv[t] * x; b1 += dFv[t] * x; }
+      for (int t = 0; t < K; ++t) {
+        // float form: the mean (mode 0) is added at the points
+        const T x = (kMixed && t == 0 && sl == 0 && f == 0) ? T(0) : T(u[t * SN]);
+        b0 += Fv[t] * x; b1 += dFv[t] * x;
+      }
       B[(side * 2 + 0) * 5 * K * K + pc] = b0;
       B[(side * 2 + 1) * 5 * K * K + pc] = b1;
     }
@@ -399,16 +418,16 @@ __device__ __forceinline__ void face_body(const FaceArgs<K, M> &args, const Case
     const int c = c0 + tid;
     if (c >= 2 * 5 * K) break;
     const int side = c / (5 * K), vs = c - side * 5 * K;
-    double x0[K], x1[K];
+    T x0[K], x1[K];
 #pragma unroll
     for (int f = 0; f < K; ++f) {
       x0[f] = B[(side * 2 + 0) * 5 * K * K + vs * K + f];
       x1[f] = B[(side * 2 + 1) * 5 * K * K + vs * K + f];
     }
-    double *Ts = T + side * 3 * 5 * K * M;
+    T *Ts = Tb + side * 3 * 5 * K * M;
 #pragma unroll 1
     for (int n = 0; n < M; ++n) {
-      double tA = 0.0, tD = 0.0, tN = 0.0;
+      T tA = 0, tD = 0, tN = 0;
 #pragma unroll
       for (int f = 0; f < K; ++f) { tA += KA[n * K + f] * x0[f]; tD += KD[n * K + f] * x0[f]; tN += KA[n * K + f] * x1[f]; }
       Ts[vs * M + n] = tA; Ts[5 * K * M + vs * M + n] = tD; Ts[2 * 5 * K * M + vs * M + n] = tN;
@@ -424,7 +443,7 @@ __device__ __forceinline__ void face_body(const FaceArgs<K, M> &args, const Case
     const int m = m0 + tid;
     if (m >= N2) break;
     const int sp = m / M, fp = m - sp * M;
-    double ai[K], di[K];
+    T ai[K], di[K];
 #pragma unroll
     for (int sl = 0; sl < K; ++sl) { ai[sl] = sA[sp * K + sl]; di[sl] = sD[sp * K + sl]; }
     double Vs[2][5], gx[5], gy[5], gz[5];
@@ -439,16 +458,16 @@ __device__ __forceinline__ void face_body(const FaceArgs<K, M> &args, const Case
       }
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
-        const double *tA = T + (side * 3 + 0) * 5 * K * M + v * K * M + fp;
-        const double *tD = T + (side * 3 + 1) * 5 * K * M + v * K * M + fp;
-        const double *tN = T + (side * 3 + 2) * 5 * K * M + v * K * M + fp;
-        double val = 0.0, gN = 0.0, gF = 0.0, gS = 0.0;
+        const T *tA = Tb + (side * 3 + 0) * 5 * K * M + v * K * M + fp;
+        const T *tD = Tb + (side * 3 + 1) * 5 * K * M + v * K * M + fp;
+        const T *tN = Tb + (side * 3 + 2) * 5 * K * M + v * K * M + fp;
+        T val = 0, gN = 0, gF = 0, gS = 0;
 #pragma unroll
         for (int sl = 0; sl < K; ++sl) {
-          const double xa = tA[sl * M];
+          const T xa = tA[sl * M];
           val += ai[sl] * xa; gS += di[sl] * xa; gN += ai[sl] * tN[sl * M]; gF += ai[sl] * tD[sl * M];
         }
-        Vs[side][v] = val;
+        Vs[side][v] = kMixed ? double(val) + VCP_FOPS.c03 * sm.Mean[side][v] : double(val);
         const double gr = DIR == 0 ? gN : gF;
         const double gs = DIR == 0 ? gF : DIR == 1 ? gN : gS;
         const double gt = DIR == 2 ? gN : gS;
@@ -469,8 +488,8 @@ __device__ __forceinline__ void face_body(const FaceArgs<K, M> &args, const Case
     point_flux(Vs[1], Vs[0], gx, gy, gz, nn, wA, tau, a.mu, a.thetaHalf, fl, FX, FY, FZ);
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-      sFlux[v * N2 + m] = fl[v];
-      sFC[v * N2 + m] = FX[v]; sFC[(5 + v) * N2 + m] = FY[v]; sFC[(10 + v) * N2 + m] = FZ[v];
+      sFlux[v * N2 + m] = T(fl[v]);
+      sFC[v * N2 + m] = T(FX[v]); sFC[(5 + v) * N2 + m] = T(FY[v]); sFC[(10 + v) * N2 + m] = T(FZ[v]);
     }
   }
   __syncthreads();
@@ -481,15 +500,15 @@ __device__ __forceinline__ void face_body(const FaceArgs<K, M> &args, const Case
   // [0|1][v][slow][fast]; the volume kernel applies the face vectors
 #pragma unroll
   for (int side = 0; side < 2; ++side) {
-    const double sign = side ? 1.0 : -1.0;
+    const T sign = side ? T(1) : T(-1);
     double *out = a.faceRes + ((size_t) (side ? L : e) * 6 + 2 * DIR + side) * 10 * K * K;
-    double gv[K];   // value plane of this lane's (v, f) column, over both parts
+    T gv[K];   // value plane of this lane's (v, f) column, over both parts
 #pragma unroll
     for (int part = 0; part < 2; ++part) {
       // Cartesian symmetrising flux mapped to this side's parametric slot(s)
       if (tid < 5 * M) {
         const int v = tid / M, sp = tid - v * M;
-        double x0[M], x1[M];
+        T x0[M], x1[M];
 #pragma unroll
         for (int q = 0; q < M; ++q) {
           const int m = sp * M + q;
@@ -503,17 +522,17 @@ __device__ __forceinline__ void face_body(const FaceArgs<K, M> &args, const Case
             for (int t = 0; t < 9; ++t) c[t] = g[(side ? 4 + t : 13 + t) * N2 + m];
           }
           if (part == 0) {
-            x0[q] = sign * sFlux[v * N2 + m];                                   // lifted with A
-            x1[q] = fx * c[3 * PF] + fy * c[3 * PF + 1] + fz * c[3 * PF + 2];   // lifted with D
+            x0[q] = sign * sFlux[v * N2 + m];                                      // lifted with A
+            x1[q] = T(fx * c[3 * PF] + fy * c[3 * PF + 1] + fz * c[3 * PF + 2]);   // lifted with D
           } else {
-            x0[q] = fx * c[3 * PS] + fy * c[3 * PS + 1] + fz * c[3 * PS + 2];   // slow slot
-            x1[q] = fx * c[3 * PN] + fy * c[3 * PN + 1] + fz * c[3 * PN + 2];   // normal slot
+            x0[q] = T(fx * c[3 * PS] + fy * c[3 * PS + 1] + fz * c[3 * PS + 2]);   // slow slot
+            x1[q] = T(fx * c[3 * PN] + fy * c[3 * PN + 1] + fz * c[3 * PN + 2]);   // normal slot
           }
         }
-        double *Ys = sm.Y;
+        T *Ys = sm.Y;
 #pragma unroll 1
         for (int n = 0; n < K; ++n) {
-          double p0 = 0.0, p1 = 0.0;
+          T p0 = 0, p1 = 0;
 #pragma unroll
           for (int q = 0; q < M; ++q) {
             if (part == 0) { p0 += KA[q * K + n] * x0[q]; p0 += KD[q * K + n] * x1[q]; }
@@ -529,7 +548,7 @@ __device__ __forceinline__ void face_body(const FaceArgs<K, M> &args, const Case
       // derivative plane with A
       if (tid < 5 * K) {
         const int v = tid / K, f = tid - v * K;
-        double x0[M], x1[M];
+        T x0[M], x1[M];
 #pragma unroll
         for (int q = 0; q < M; ++q) {
           const int j = (v * M + q) * K + f;
@@ -538,7 +557,7 @@ __device__ __forceinline__ void face_body(const FaceArgs<K, M> &args, const Case
         }
 #pragma unroll
         for (int n = 0; n < K; ++n) {
-          double g0 = 0.0, g1 = 0.0;
+          T g0 = 0, g1 = 0;
 #pragma unroll
           for (int q = 0; q < M; ++q) {
             if (part == 0) g0 += KA[q * K + n] * x0[q];
@@ -546,7 +565,7 @@ __device__ __forceinline__ void face_body(const FaceArgs<K, M> &args, const Case
           }
           const int o = (v * K + n) * K + f;
           if (part == 0) gv[n] = g0;
-          else { out[o] = gv[n] + g0; out[5 * K * K + o] = g1; }
+          else { out[o] = double(gv[n] + g0); out[5 * K * K + o] = double(g1); }
           asm volatile("" ::: "memory");   // one operator column's scalar loads at a time
         }
       }
@@ -560,12 +579,12 @@ __device__ __forceinline__ void face_body(const FaceArgs<K, M> &args, const Case
 // blockIdx.x the element, or (BATCH) a work item (case, element) of the
 // group. The argument block is read through the kernel-argument segment
 // pointer: a by-value copy indexed at run time would go to scratch.
-template <int K, int M, bool BUG, bool BATCH>
-__global__ __launch_bounds__(64, (FaceSmem<K, M>::kWaves)) void face_kernel(FaceArgs<K, M> a_)
+template <int K, int M, bool BUG, bool BATCH, typename T = double>
+__global__ __launch_bounds__(64, (FaceSmem<K, M, T>::kWaves)) void face_kernel(FaceArgs<K, M, T> a_)
 {
-  __shared__ FaceSmem<K, M> sm;
+  __shared__ FaceSmem<K, M, T> sm;
   (void) a_;
-  const FaceArgs<K, M> &a = *(const FaceArgs<K, M> *) __builtin_amdgcn_kernarg_segment_ptr();
+  const FaceArgs<K, M, T> &a = *(const FaceArgs<K, M, T> *) __builtin_amdgcn_kernarg_segment_ptr();
   CaseDev c;
   int e;
   if constexpr (BATCH) {
@@ -577,23 +596,23 @@ __global__ __launch_bounds__(64, (FaceSmem<K, M>::kWaves)) void face_kernel(Face
     c = a.c;
     e = blockIdx.x;
   }
-  if (blockIdx.y == 0) face_body<K, M, 0, BUG, BATCH>(a, c, sm, e);
-  else if (blockIdx.y == 1) face_body<K, M, 1, BUG, BATCH>(a, c, sm, e);
-  else face_body<K, M, 2, BUG, BATCH>(a, c, sm, e);
+  if (blockIdx.y == 0) face_body<K, M, 0, BUG, BATCH, T>(a, c, sm, e);
+  else if (blockIdx.y == 1) face_body<K, M, 1, BUG, BATCH, T>(a, c, sm, e);
+  else face_body<K, M, 2, BUG, BATCH, T>(a, c, sm, e);
 }
 
 // 1D operators of the standard element as kernel arguments: indexed with
 // compile-time constants after unrolling, they are read into scalar registers.
-template <int K, int M>
+template <int K, int M, typename T = double>
 struct Ops {
-  double A[M * K];   // phi_q(r_p), row-major (p, q)
-  double D[M * K];   // phi'_q(r_p)
+  T A[M * K];   // phi_q(r_p), row-major (p, q)
+  T D[M * K];   // phi'_q(r_p)
 };
 
 // Volume kernel arguments: the case part (single-case launch) or the case
 // table and work items (batched launch), the stage control and what depends
 // on (K, M) only.
-template <int K, int M>
+template <int K, int M, typename T = double>
 struct VolArgs {
   CaseDev c;
   const CaseDev *cases;
@@ -602,7 +621,7 @@ struct VolArgs {
   StageCtl ctl;
   TableOffsets to;
   int bug;
-  Ops<K, M> op;
+  Ops<K, M, T> op;
   double c0[K];                           // phi_q(0): basis at the element centre
   double fvec[4][K];                      // phi(-1), phi(+1), phi'(-1), phi'(+1)
 };
@@ -676,7 +695,7 @@ constexpr int vol_threads(int K, int M)
 // two waves per SIMD, or two workgroups when the LDS allows, which caps the
 // kernel at 128 VGPRs; NT = 64, 128: as many workgroups as the LDS allows, at
 // most three waves per SIMD).
-template <int K, int M, int NT>
+template <int K, int M, int NT, typename T = double>
 struct VolShape {
   static constexpr int ND = K * K * K, NI = M * M * M, NR = M * M;
   // Points go through the pointwise stages in two passes split at a row
@@ -688,9 +707,16 @@ struct VolShape {
   static constexpr int SF0 = 15 * P0, SF = 15 * NI; // fluxes of the first pass / all
   static constexpr int SI = 15 * NR * K;            // backward i-stage output
   static constexpr int BIG = cmax(cmax(2 * SZ + SY, SF), SI + 10 * M * K * K);
-  static constexpr int LDS = 8 * (2 * M * K + 5 * ND + BIG + 45 + 6 * (NT / 64) + 15 + 4 * K);
+  static constexpr int LDS = (int) sizeof(T) * (2 * M * K + 5 * ND + BIG + 15) + 8 * (45 + 6 * (NT / 64) + 4 * K + 5);
   static constexpr int WGS = (160 * 1024) / LDS;   // workgroups per CU by LDS
-  static constexpr int WAVES = NT == 256 ? (WGS >= 3 ? 3 : 2)
+  // FP32 contractions: the smaller LDS allows more waves per SIMD
+#ifndef VCP_MIXED_CG_TOL
+#define VCP_MIXED_CG_TOL 1e-5
+#endif
+#ifndef VCP_F32_WAVES
+#define VCP_F32_WAVES 3
+#endif
+  static constexpr int WAVES = NT == 256 ? (sizeof(T) == 4 && WGS >= VCP_F32_WAVES ? VCP_F32_WAVES : WGS >= 3 ? 3 : 2)
                               : NT > 256 ? (NT / 256) * (WGS >= 2 ? 2 : 1)
                                          : cmax(1, WGS * (NT / 64) / 4 < 3 ? WGS * (NT / 64) / 4 : 3);
   static_assert(WGS >= 1, "volume kernel LDS exceeds 160 KB");
@@ -699,11 +725,20 @@ struct VolShape {
 };
 
 // blockIdx.x is the element, or (BATCH) a work item (case, element) of the
-// group; a is the case part.
-template <int K, int M, int NT, bool BATCH>
-__global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel(VolArgs<K, M> args)
+// group; a is the case part. T is the type of the sum-factorisation
+// contractions, the LDS buffers and the mass-matrix solve: double, or float
+// for the mixed-precision form, where the state, its element means, the
+// pointwise physics, the face-slot gather, the metric and the RK update stay
+// in double. In the float form the mean (mode 0) of each variable is taken
+// out of the modal state before the contractions and added back at the
+// points in double (it contributes the same constant at every point and
+// nothing to the gradients), so the float contractions carry only the
+// variation about the mean.
+template <int K, int M, int NT, bool BATCH, typename T = double>
+__global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_kernel(VolArgs<K, M, T> args)
 {
-  using S = VolShape<K, M, NT>;
+  using S = VolShape<K, M, NT, T>;
+  constexpr bool kMixed = sizeof(T) == 4;
   constexpr int ND = S::ND, NI = S::NI, NR = S::NR, R0 = S::R0, P0 = S::P0, P1 = S::P1;
   constexpr int SZ = S::SZ, SF0 = S::SF0, SI = S::SI, BIG = S::BIG;
   static_assert(P1 <= NT, "two point passes");
@@ -711,13 +746,14 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
   static_assert((NT & (NT - 1)) == 0, "NT must be a power of two");
   // the first pass's fluxes must leave the y-stage rows of the second pass intact
   static_assert(P1 == 0 || SF0 <= 2 * SZ + 15 * R0 * K, "flux / y-stage overlap");
-  __shared__ double sA[M * K], sD[M * K];
-  __shared__ double sU[5 * ND];
-  __shared__ double sBig[BIG];
+  __shared__ T sA[M * K], sD[M * K];
+  __shared__ T sU[5 * ND];
+  __shared__ T sBig[BIG];
   __shared__ double sCm[5][9];
   __shared__ double sRed[2 * (NT / 64) * 3];
-  __shared__ double sDv[15];
+  __shared__ T sDv[15];
   __shared__ double sFv[4 * K];
+  __shared__ double sMean[5];     // float form: mode 0 of each variable
   // the case part is read where it is used: from the kernel arguments, or
   // (BATCH) from the group's case table
   const CaseDev *ap;
@@ -733,13 +769,21 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
   }
   const CaseDev &a = *ap;
   const int tid = threadIdx.x;
-  const double *A = args.op.A, *D = args.op.D;
+  const T *A = args.op.A, *D = args.op.D;
   const double *Ue = a.U + (size_t) e * 5 * ND;
   const double *vg = a.vgeo + (size_t) e * kVolGeo * NI;
   const int sp = a.spIdx ? a.spIdx[e] : -1;
   for (int i = tid; i < M * K; i += NT) { sA[i] = A[i]; sD[i] = D[i]; }
   for (int i = tid; i < 4 * K; i += NT) sFv[i] = args.fvec[i / K][i % K];
-  for (int i = tid; i < 5 * ND; i += NT) sU[i] = Ue[i];
+  for (int i = tid; i < 5 * ND; i += NT) {
+    if constexpr (kMixed) {
+      const double u = Ue[i];
+      if (i % ND == 0) sMean[i / ND] = u;
+      sU[i] = i % ND == 0 ? T(0) : T(u);
+    } else {
+      sU[i] = Ue[i];
+    }
+  }
   if (args.bug)
     for (int i = tid; i < 45; i += NT) sCm[i / 9][i % 9] = vg[(1 + i % 9) * NI + i / 9];
   __syncthreads();
@@ -747,15 +791,15 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
   // ---------------- step 1: solution and parametric gradients at the points
   // z-stage buffers, then the y-stage buffers row-major: yB[jk][3][5][K] with
   // the (A, A), (D, A), (A, D) contractions of variable v at row jk = k * M + j
-  double *zA = sBig, *zD = zA + SZ, *yB = zD + SZ;
+  T *zA = sBig, *zD = zA + SZ, *yB = zD + SZ;
   for (int pc = tid; pc < 5 * K * K; pc += NT) {      // pencil (v, j, i) along k
     const int v = pc / (K * K), ji = pc - v * K * K;
-    double x[K];
+    T x[K];
 #pragma unroll
     for (int q = 0; q < K; ++q) x[q] = sU[(v * K + q) * K * K + ji];
 #pragma unroll
     for (int n = 0; n < M; ++n) {
-      double sa = 0.0, sd = 0.0;
+      T sa = 0, sd = 0;
 #pragma unroll
       for (int q = 0; q < K; ++q) { sa += A[n * K + q] * x[q]; sd += D[n * K + q] * x[q]; }
       zA[(v * M + n) * K * K + ji] = sa;
@@ -783,29 +827,34 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
     }
 #pragma unroll
     for (int v = 0; v < 5; ++v) Vc[v] = wave_sum(Vc[v]);
+    if constexpr (kMixed) {
+      const double c03 = args.c0[0] * args.c0[0] * args.c0[0];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) Vc[v] += c03 * sMean[v];
+    }
     if (lane == 0) {
       double rho, u, v, w, p, eKin;
       ent2prim(Vc, rho, u, v, w, p, eKin);
       const double JacInv = 8.0 / a.econst[(size_t) e * kElemConst + 3];
       const double pInv = 1.0 / p, a1 = kGm1 * rho * pInv * pInv, a2 = eKin * a1;
-      sDv[0] = JacInv * (kGam / kGm1 / rho + a2 * eKin); sDv[1] = -JacInv * u * a2; sDv[2] = -JacInv * v * a2;
-      sDv[3] = -JacInv * w * a2; sDv[4] = JacInv * (a2 - pInv); sDv[5] = JacInv * (a1 * u * u + pInv);
-      sDv[6] = JacInv * a1 * u * v; sDv[7] = JacInv * a1 * u * w; sDv[8] = -JacInv * a1 * u;
-      sDv[9] = JacInv * (a1 * v * v + pInv); sDv[10] = JacInv * a1 * v * w; sDv[11] = -JacInv * a1 * v;
-      sDv[12] = JacInv * (a1 * w * w + pInv); sDv[13] = -JacInv * a1 * w; sDv[14] = JacInv * a1;
+      sDv[0] = T(JacInv * (kGam / kGm1 / rho + a2 * eKin)); sDv[1] = T(-JacInv * u * a2); sDv[2] = T(-JacInv * v * a2);
+      sDv[3] = T(-JacInv * w * a2); sDv[4] = T(JacInv * (a2 - pInv)); sDv[5] = T(JacInv * (a1 * u * u + pInv));
+      sDv[6] = T(JacInv * a1 * u * v); sDv[7] = T(JacInv * a1 * u * w); sDv[8] = T(-JacInv * a1 * u);
+      sDv[9] = T(JacInv * (a1 * v * v + pInv)); sDv[10] = T(JacInv * a1 * v * w); sDv[11] = T(-JacInv * a1 * v);
+      sDv[12] = T(JacInv * (a1 * w * w + pInv)); sDv[13] = T(-JacInv * a1 * w); sDv[14] = T(JacInv * a1);
     }
   }
   __syncthreads();
   for (int pc = tid; pc < 5 * M * K; pc += NT) {      // pencil (v, k', i) along j
     const int vk = pc / K, i = pc - vk * K;
     const int v = vk / M, kp = vk - v * M;
-    double xa[K], xd[K];
+    T xa[K], xd[K];
 #pragma unroll
     for (int q = 0; q < K; ++q) { xa[q] = zA[(vk * K + q) * K + i]; xd[q] = zD[(vk * K + q) * K + i]; }
-    double *yo = yB + (kp * M * 15 + v) * K + i;
+    T *yo = yB + (kp * M * 15 + v) * K + i;
 #pragma unroll
     for (int n = 0; n < M; ++n) {
-      double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+      T s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll
       for (int q = 0; q < K; ++q) { s0 += A[n * K + q] * xa[q]; s1 += D[n * K + q] * xa[q]; s2 += A[n * K + q] * xd[q]; }
       yo[(n * 15 + 0) * K] = s0;
@@ -824,7 +873,14 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
   double mach2 = 0.0;
   // (w J rho, u, v, w, w J p, w J dU/dV_44) at the thread's points, for the
   // weighted dU/dV in the solve
-  double prim[2][6];
+  T prim[2][6];
+  // float form: the constant contribution of the element means at every point
+  double meanPt[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if constexpr (kMixed) {
+    const double c03 = args.c0[0] * args.c0[0] * args.c0[0];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) meanPt[v] = c03 * sMean[v];
+  }
 #pragma unroll
   for (int q = 0; q < (P1 > 0 ? 2 : 1); ++q) {
     const int m = q == 0 ? tid : P0 + tid;
@@ -832,27 +888,28 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
     double Vp[5], Gr[5], Gs[5], Gt[5];
     if (act) {
       const int i = m % M, jk = m / M;
-      double ai[K], di[K];
+      T ai[K], di[K];
 #pragma unroll
       for (int ii = 0; ii < K; ++ii) { ai[ii] = sA[i * K + ii]; di[ii] = sD[i * K + ii]; }
       int roff = jk * 15 * K;
       asm volatile("" : "+v"(roff));   // one base register, immediate LDS offsets
-      const double *row = yB + roff;
+      const T *row = yB + roff;
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
-        const double *ra = row + v * K, *rd = row + (5 + v) * K, *rz = row + (10 + v) * K;
-        double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+        const T *ra = row + v * K, *rd = row + (5 + v) * K, *rz = row + (10 + v) * K;
+        T s0 = 0, s1 = 0, s2 = 0, s3 = 0;
 #pragma unroll
         for (int ii = 0; ii < K; ++ii) {
-          const double x = ra[ii];
+          const T x = ra[ii];
           s0 += ai[ii] * x; s1 += di[ii] * x; s2 += ai[ii] * rd[ii]; s3 += ai[ii] * rz[ii];
         }
-        Vp[v] = s0; Gr[v] = s1; Gs[v] = s2; Gt[v] = s3;
+        Vp[v] = kMixed ? double(s0) + meanPt[v] : double(s0);
+        Gr[v] = s1; Gs[v] = s2; Gt[v] = s3;
       }
     }
     __syncthreads();
     if (!act) continue;
-    double *Fb = q == 0 ? sBig + m : sBig + SF0 + (m - P0);
+    T *Fb = q == 0 ? sBig + m : sBig + SF0 + (m - P0);
     const int fstride = q == 0 ? P0 : P1;
     const double J = vg[m];
     const double wJ = a.tables[args.to.w3 + m] * J;
@@ -880,8 +937,8 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
     ent2prim(Vp, rho, u, v, w, p, eKin);
     const double V4Inv = 1.0 / Vp[4];
     const double rE = p * kInvGm1 + rho * eKin;
-    prim[q][0] = wJ * rho; prim[q][1] = u; prim[q][2] = v; prim[q][3] = w; prim[q][4] = wJ * p;
-    prim[q][5] = wJ * (rE * (rE + p) / rho + p * eKin);
+    prim[q][0] = T(wJ * rho); prim[q][1] = T(u); prim[q][2] = T(v); prim[q][3] = T(w); prim[q][4] = T(wJ * p);
+    prim[q][5] = T(wJ * (rE * (rE + p) / rho + p * eKin));
     const double dudx = -V4Inv * (gx[1] + u * gx[4]), dudy = -V4Inv * (gy[1] + u * gy[4]), dudz = -V4Inv * (gz[1] + u * gz[4]);
     const double dvdx = -V4Inv * (gx[2] + v * gx[4]), dvdy = -V4Inv * (gy[2] + v * gy[4]), dvdz = -V4Inv * (gz[2] + v * gz[4]);
     const double dwdx = -V4Inv * (gx[3] + w * gx[4]), dwdy = -V4Inv * (gy[3] + w * gy[4]), dwdz = -V4Inv * (gz[3] + w * gz[4]);
@@ -902,7 +959,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
     for (int d = 0; d < 3; ++d) {
       const double mx = weight * cm[3 * d], my = weight * cm[3 * d + 1], mz = weight * cm[3 * d + 2];
 #pragma unroll
-      for (int l = 0; l < 5; ++l) Fb[(d * 5 + l) * fstride] = fx[l] * mx + fy[l] * my + fz[l] * mz;
+      for (int l = 0; l < 5; ++l) Fb[(d * 5 + l) * fstride] = T(fx[l] * mx + fy[l] * my + fz[l] * mz);
     }
     if (sp >= 0) {
       // sponge damping sigma J w (U - U_target) and CML sources
@@ -933,7 +990,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
   // and normal-derivative coefficient planes over its two tangential axes,
   // times the face vectors phi(r_f), phi'(r_f) at the DOF's normal index
   static_assert(ND <= NT, "one DOF per thread in the mass-matrix solve");
-  double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double racc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if (tid < ND) {
     const int di = tid % K, dj = (tid / K) % K, dk = tid / (K * K);
     const double *fr = a.faceRes + (size_t) e * 6 * 10 * K * K;
@@ -946,7 +1003,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
       const double cv = sFv[side * K + tn], cd = sFv[(2 + side) * K + tn];
       const double *b = fr + f * 10 * K * K + sl * K + fa;
 #pragma unroll
-      for (int l = 0; l < 5; ++l) r[l] += cv * b[l * K * K] + cd * b[(5 + l) * K * K];
+      for (int l = 0; l < 5; ++l) racc[l] += cv * b[l * K * K] + cd * b[(5 + l) * K * K];
     }
   }
 
@@ -954,19 +1011,19 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
   // rows (b, jk) along i, compacted to I[b][jk][K] once every row is read
   {
     constexpr int NRW = 15 * NR, RPT = (NRW + NT - 1) / NT;
-    double o[RPT][K];
+    T o[RPT][K];
 #pragma unroll
     for (int t = 0; t < RPT; ++t) {
       const int r = tid + t * NT;
       if (r < NRW) {
         const int b = r / NR, jk = r - b * NR;
-        const double *x = jk < R0 ? sBig + b * P0 + jk * M : sBig + SF0 + b * P1 + (jk - R0) * M;
-        double xx[M];
+        const T *x = jk < R0 ? sBig + b * P0 + jk * M : sBig + SF0 + b * P1 + (jk - R0) * M;
+        T xx[M];
 #pragma unroll
         for (int q = 0; q < M; ++q) xx[q] = x[q];
 #pragma unroll
         for (int n = 0; n < K; ++n) {
-          double s = 0.0;
+          T s = 0;
           if (b < 5) {
 #pragma unroll
             for (int q = 0; q < M; ++q) s += D[q * K + n] * xx[q];
@@ -988,18 +1045,18 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
     }
   }
   __syncthreads();
-  const double *I0 = sBig, *I1 = I0 + 5 * NR * K, *I2 = I1 + 5 * NR * K;
-  double *Y01 = sBig + SI, *Y2 = Y01 + 5 * M * K * K;   // after the i-stage block
+  const T *I0 = sBig, *I1 = I0 + 5 * NR * K, *I2 = I1 + 5 * NR * K;
+  T *Y01 = sBig + SI, *Y2 = Y01 + 5 * M * K * K;   // after the i-stage block
   for (int pc = tid; pc < 5 * M * K; pc += NT) {      // pencil (v, k, i') along j
     const int vk = pc / K, i = pc - vk * K;
-    double x0[M], x1[M], x2[M];
+    T x0[M], x1[M], x2[M];
 #pragma unroll
     for (int q = 0; q < M; ++q) {
       x0[q] = I0[(vk * M + q) * K + i]; x1[q] = I1[(vk * M + q) * K + i]; x2[q] = I2[(vk * M + q) * K + i];
     }
 #pragma unroll
     for (int n = 0; n < K; ++n) {
-      double s01 = 0.0, s2 = 0.0;
+      T s01 = 0, s2 = 0;
 #pragma unroll
       for (int q = 0; q < M; ++q) { s01 += A[q * K + n] * x0[q] + D[q * K + n] * x1[q]; s2 += A[q * K + n] * x2[q]; }
       Y01[(vk * K + n) * K + i] = s01;
@@ -1009,12 +1066,12 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
   __syncthreads();
   for (int pc = tid; pc < 5 * K * K; pc += NT) {      // pencil (v, j', i') along k
     const int v = pc / (K * K), ji = pc - v * K * K;
-    double y0[M], y2[M];
+    T y0[M], y2[M];
 #pragma unroll
     for (int q = 0; q < M; ++q) { y0[q] = Y01[(v * M + q) * K * K + ji]; y2[q] = Y2[(v * M + q) * K * K + ji]; }
 #pragma unroll
     for (int n = 0; n < K; ++n) {
-      double s = 0.0;
+      T s = 0;
 #pragma unroll
       for (int q = 0; q < M; ++q) s += A[q * K + n] * y0[q] + D[q * K + n] * y2[q];
       sU[(v * K + n) * K * K + ji] = s;
@@ -1025,27 +1082,27 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
     // sponge element: source terms projected with the basis functions
     const double *sb = a.srcBuf + (size_t) sp * 5 * NI;
     for (int r = tid; r < 5 * M * M; r += NT) {       // rows along i
-      double xx[M];
+      T xx[M];
 #pragma unroll
-      for (int q = 0; q < M; ++q) xx[q] = sb[r * M + q];
+      for (int q = 0; q < M; ++q) xx[q] = T(sb[r * M + q]);
 #pragma unroll
       for (int n = 0; n < K; ++n) {
-        double s = 0.0;
+        T s = 0;
 #pragma unroll
         for (int q = 0; q < M; ++q) s += A[q * K + n] * xx[q];
         sBig[r * K + n] = s;
       }
     }
     __syncthreads();
-    double *Ys = sBig + 5 * M * M * K;
+    T *Ys = sBig + 5 * M * M * K;
     for (int pc = tid; pc < 5 * M * K; pc += NT) {    // pencils along j
       const int vk = pc / K, i = pc - vk * K;
-      double xx[M];
+      T xx[M];
 #pragma unroll
       for (int q = 0; q < M; ++q) xx[q] = sBig[(vk * M + q) * K + i];
 #pragma unroll
       for (int n = 0; n < K; ++n) {
-        double s = 0.0;
+        T s = 0;
 #pragma unroll
         for (int q = 0; q < M; ++q) s += A[q * K + n] * xx[q];
         Ys[(vk * K + n) * K + i] = s;
@@ -1054,12 +1111,12 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
     __syncthreads();
     for (int pc = tid; pc < 5 * K * K; pc += NT) {    // pencils along k, accumulated
       const int v = pc / (K * K), ji = pc - v * K * K;
-      double xx[M];
+      T xx[M];
 #pragma unroll
       for (int q = 0; q < M; ++q) xx[q] = Ys[(v * M + q) * K * K + ji];
 #pragma unroll
       for (int n = 0; n < K; ++n) {
-        double s = 0.0;
+        T s = 0;
 #pragma unroll
         for (int q = 0; q < M; ++q) s += A[q * K + n] * xx[q];
         sU[(v * K + n) * K * K + ji] += s;
@@ -1072,21 +1129,33 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
   // (a non-finite right-hand side is flagged: the solve's max-norms would
   // pass over it)
   bool bad = false;
-  if (tid < ND)
+  T r[5];
+  if constexpr (kMixed) {
 #pragma unroll
     for (int l = 0; l < 5; ++l) {
-      r[l] += sU[l * ND + tid];
-      bad = bad || !isfinite(r[l]);
+      if (tid < ND) racc[l] += double(sU[l * ND + tid]);
+      bad = bad || !isfinite(racc[l]);
+      r[l] = T(racc[l]);
     }
+  } else {
+    if (tid < ND)
+#pragma unroll
+      for (int l = 0; l < 5; ++l) {
+        racc[l] += sU[l * ND + tid];
+        bad = bad || !isfinite(racc[l]);
+      }
+#pragma unroll
+    for (int l = 0; l < 5; ++l) r[l] = racc[l];
+  }
 
   // ---------------- entropy-variable mass matrix: flexible PCG
   const double JacInv = 8.0 / a.econst[(size_t) e * kElemConst + 3];
   const bool own = tid < ND;
-  double x[5], z[5], pv[5];
+  T x[5], z[5], pv[5];
   int half = 0;
   double r0 = 0.0;
 #pragma unroll
-  for (int l = 0; l < 5; ++l) { x[l] = 0.0; r0 = fmax(r0, fabs(r[l])); }
+  for (int l = 0; l < 5; ++l) { x[l] = 0; r0 = fmax(r0, fabs(double(r[l]))); }
   {
     // with the element's maximum Mach^2 for the monitoring data
     double red[2] = {r0, mach2};
@@ -1096,7 +1165,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
   }
   int iters = 0;
   if (r0 * JacInv > 1e-10) {
-    double *cz = sBig, *cy = cz + SZ, *pI = cy + 5 * M * M * K, *by = pI + 5 * NI;
+    T *cz = sBig, *cy = cz + SZ, *pI = cy + 5 * M * M * K, *by = pI + 5 * NI;
     // z = dV/dU(centre) r, local to the thread owning the DOF
 #define VCP_PRECOND()                                                                             \
     do {                                                                                          \
@@ -1109,7 +1178,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
     VCP_PRECOND();
     double rz = 0.0;
 #pragma unroll
-    for (int l = 0; l < 5; ++l) { pv[l] = z[l]; rz += r[l] * z[l]; }
+    for (int l = 0; l < 5; ++l) { pv[l] = z[l]; rz += double(r[l]) * double(z[l]); }
     { double red[1] = {rz}; block_reduce1<NT, 1, 1, (ND + 63) / 64>(red, sRed, half); rz = red[0]; }
     // Stages with fewer items than threads start at different waves, so that
     // the waves of the workgroup (one per SIMD) carry similar shares of the
@@ -1127,12 +1196,12 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
       __syncthreads();
       for (int pc = rot(FK); pc < 5 * K * K; pc += NT) {
         const int v = pc / (K * K), ji = pc - v * K * K;
-        double xx[K];
+        T xx[K];
 #pragma unroll
         for (int q = 0; q < K; ++q) xx[q] = sU[(v * K + q) * K * K + ji];
 #pragma unroll
         for (int n = 0; n < M; ++n) {
-          double s = 0.0;
+          T s = 0;
 #pragma unroll
           for (int q = 0; q < K; ++q) s += A[n * K + q] * xx[q];
           cz[(v * M + n) * K * K + ji] = s;
@@ -1141,12 +1210,12 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
       __syncthreads();
       for (int pc = rot(1); pc < 5 * M * K; pc += NT) {
         const int vk = pc / K, i = pc - vk * K;
-        double xx[K];
+        T xx[K];
 #pragma unroll
         for (int q = 0; q < K; ++q) xx[q] = cz[(vk * K + q) * K + i];
 #pragma unroll
         for (int n = 0; n < M; ++n) {
-          double s = 0.0;
+          T s = 0;
 #pragma unroll
           for (int q = 0; q < K; ++q) s += A[n * K + q] * xx[q];
           cy[(vk * M + n) * K + i] = s;
@@ -1158,12 +1227,12 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
       // contraction would read 5 K values and an operator row per point
       for (int pc = tid; pc < 5 * M * M; pc += NT) {
         const int l = pc / (M * M), jk = pc - l * M * M;
-        double xx[K];
+        T xx[K];
 #pragma unroll
         for (int q = 0; q < K; ++q) xx[q] = cy[pc * K + q];
 #pragma unroll
         for (int n = 0; n < M; ++n) {
-          double s = 0.0;
+          T s = 0;
 #pragma unroll
           for (int q = 0; q < K; ++q) s += A[n * K + q] * xx[q];
           pI[l * NI + jk * M + n] = s;
@@ -1174,17 +1243,17 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
       for (int q = 0; q < (P1 > 0 ? 2 : 1); ++q) {
         const int m = q == 0 ? tid : P0 + tid;
         if (!(q == 0 ? tid < P0 : tid < P1)) continue;
-        double pp[5];
+        T pp[5];
 #pragma unroll
         for (int l = 0; l < 5; ++l) pp[l] = pI[l * NI + m];
         // w J dU/dV at the point (ElementClass.cpp:4228-4253): every entry is
         // homogeneous of degree one in (rho, p), so the scaled pair held in
         // prim gives the weighted matrix directly
-        const double rho = prim[q][0], u = prim[q][1], v = prim[q][2], w = prim[q][3], p = prim[q][4];
-        const double d44 = prim[q][5];
-        const double eKin = 0.5 * (u * u + v * v + w * w);
-        const double rE = p * kInvGm1 + rho * eKin, rH = rE + p;
-        const double ru = rho * u, rv = rho * v, rw = rho * w;
+        const T rho = prim[q][0], u = prim[q][1], v = prim[q][2], w = prim[q][3], p = prim[q][4];
+        const T d44 = prim[q][5];
+        const T eKin = T(0.5) * (u * u + v * v + w * w);
+        const T rE = p * T(kInvGm1) + rho * eKin, rH = rE + p;
+        const T ru = rho * u, rv = rho * v, rw = rho * w;
         pI[0 * NI + m] = rho * pp[0] + ru * pp[1] + rv * pp[2] + rw * pp[3] + rE * pp[4];
         pI[1 * NI + m] = ru * pp[0] + (ru * u + p) * pp[1] + ru * v * pp[2] + ru * w * pp[3] + rH * u * pp[4];
         pI[2 * NI + m] = rv * pp[0] + ru * v * pp[1] + (rv * v + p) * pp[2] + rv * w * pp[3] + rH * v * pp[4];
@@ -1193,12 +1262,12 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
       }
       __syncthreads();
       for (int rr = tid; rr < 5 * M * M; rr += NT) {
-        double xx[M];
+        T xx[M];
 #pragma unroll
         for (int q = 0; q < M; ++q) xx[q] = pI[rr * M + q];
 #pragma unroll
         for (int n = 0; n < K; ++n) {
-          double s = 0.0;
+          T s = 0;
 #pragma unroll
           for (int q = 0; q < M; ++q) s += A[q * K + n] * xx[q];
           pI[rr * M + n] = s;
@@ -1207,12 +1276,12 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
       __syncthreads();
       for (int pc = rot(2); pc < 5 * M * K; pc += NT) {
         const int vk = pc / K, i = pc - vk * K;
-        double xx[M];
+        T xx[M];
 #pragma unroll
         for (int q = 0; q < M; ++q) xx[q] = pI[(vk * M + q) * M + i];
 #pragma unroll
         for (int n = 0; n < K; ++n) {
-          double s = 0.0;
+          T s = 0;
 #pragma unroll
           for (int q = 0; q < M; ++q) s += A[q * K + n] * xx[q];
           by[(vk * K + n) * K + i] = s;
@@ -1223,17 +1292,17 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
       double pAp = 0.0;
       for (int pc = rot(FK); pc < 5 * K * K; pc += NT) {
         const int v = pc / (K * K), ji = pc - v * K * K;
-        double xx[M];
+        T xx[M];
 #pragma unroll
         for (int q = 0; q < M; ++q) xx[q] = by[(v * M + q) * K * K + ji];
 #pragma unroll
         for (int n = 0; n < K; ++n) {
-          double s = 0.0;
+          T s = 0;
 #pragma unroll
           for (int q = 0; q < M; ++q) s += A[q * K + n] * xx[q];
           const int idx = (v * K + n) * K * K + ji;
           cz[idx] = s;
-          pAp += sU[idx] * s;
+          pAp += double(sU[idx]) * double(s);
         }
       }
       {
@@ -1244,19 +1313,20 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
       // vector updates only in the waves that own DOFs
       double zAp = 0.0, rzn = 0.0, rmax = 0.0, alpha = 0.0;
       if (own) {
-        double ap[5];
+        T ap[5];
 #pragma unroll
         for (int l = 0; l < 5; ++l) ap[l] = cz[l * ND + tid];
         alpha = rz / pAp;
+        const T alphaT = T(alpha);
 #pragma unroll
         for (int l = 0; l < 5; ++l) {
-          x[l] += alpha * pv[l];
-          r[l] -= alpha * ap[l];
-          rmax = fmax(rmax, fabs(r[l]));
+          x[l] += alphaT * pv[l];
+          r[l] -= alphaT * ap[l];
+          rmax = fmax(rmax, fabs(double(r[l])));
         }
         VCP_PRECOND();
 #pragma unroll
-        for (int l = 0; l < 5; ++l) { zAp += z[l] * ap[l]; rzn += r[l] * z[l]; }
+        for (int l = 0; l < 5; ++l) { zAp += double(z[l]) * double(ap[l]); rzn += double(r[l]) * double(z[l]); }
       }
       {
         double red[3] = {zAp, rzn, rmax};
@@ -1264,9 +1334,11 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
         zAp = red[0]; rzn = red[1]; rmax = red[2];
       }
       iters = it + 1;
-      if (rmax < 1e-10 * r0) break;
+      // the reference's stop rule (CG_MassMatrix.cpp:465); the float form
+      // stops at 1e-5, a few units of its rounding level
+      if (rmax < (kMixed ? VCP_MIXED_CG_TOL : 1e-10) * r0) break;
       if (own) {
-        const double beta = -alpha * zAp / rz;
+        const T beta = T(-alpha * zAp / rz);
 #pragma unroll
         for (int l = 0; l < 5; ++l) pv[l] = beta * pv[l] + z[l];
       }
@@ -1284,9 +1356,10 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT>::WAVES)) void volume_kernel
 #pragma unroll
     for (int l = 0; l < 5; ++l) {
       const int i = l * ND + tid;
-      if (!isfinite(x[l])) bad = true;
-      if (args.ctl.store_res) Re[i] = x[l];
-      if (args.ctl.update) Uw[i] = (1.0 - args.ctl.a) * Uw[i] + args.ctl.a * U0e[i] - bdt * x[l];
+      const double xl = double(x[l]);
+      if (!isfinite(xl)) bad = true;
+      if (args.ctl.store_res) Re[i] = xl;
+      if (args.ctl.update) Uw[i] = (1.0 - args.ctl.a) * Uw[i] + args.ctl.a * U0e[i] - bdt * xl;
     }
   if (bad) atomicOr(a.flags, kFlagNonFinite);
 }
@@ -2156,50 +2229,64 @@ __global__ void set_dt_kernel(double *p, double v)
   if (threadIdx.x == 0) *p = v;
 }
 
-template <int K, int M>
-void fill_face_ops(const DeviceCase &dc, FaceOps<K, M> &op)
+template <int K, int M, typename T>
+void fill_face_ops(const DeviceCase &dc, FaceOps<K, M, T> &op)
 {
-  for (int i = 0; i < M * K; ++i) { op.A[i] = dc.opA[i]; op.D[i] = dc.opD[i]; }
+  for (int i = 0; i < M * K; ++i) { op.A[i] = T(dc.opA[i]); op.D[i] = T(dc.opD[i]); }
   for (int q = 0; q < K; ++q)
-    for (int t = 0; t < 4; ++t) op.F[t][q] = dc.opF[t][q];
+    for (int t = 0; t < 4; ++t) op.F[t][q] = T(dc.opF[t][q]);
   for (int q = 0; q < M; ++q) op.w[q] = dc.opW[q];
+  op.c03 = dc.opC0[0] * dc.opC0[0] * dc.opC0[0];
 }
 
-template <int K, int M>
-void fill_vol_shape(const DeviceCase &dc, const StageCtl &ctl, VolArgs<K, M> &a)
+template <int K, int M, typename T>
+void fill_vol_shape(const DeviceCase &dc, const StageCtl &ctl, VolArgs<K, M, T> &a)
 {
   a.ctl = ctl; a.to = dc.to; a.bug = dc.bug_compat;
-  for (int i = 0; i < M * K; ++i) { a.op.A[i] = dc.opA[i]; a.op.D[i] = dc.opD[i]; }
+  for (int i = 0; i < M * K; ++i) { a.op.A[i] = T(dc.opA[i]); a.op.D[i] = T(dc.opD[i]); }
   for (int q = 0; q < K; ++q) {
     a.c0[q] = dc.opC0[q];
     for (int t = 0; t < 4; ++t) a.fvec[t][q] = dc.opF[t][q];
   }
 }
 
-template <int K, int M>
-hipError_t faces_impl(const DeviceCase &dc, hipStream_t s)
+template <int K, int M, typename T>
+hipError_t faces_impl_t(const DeviceCase &dc, hipStream_t s)
 {
-  FaceArgs<K, M> a{};
+  FaceArgs<K, M, T> a{};
   a.c = case_dev(dc);
   fill_face_ops(dc, a.op);
   if (dc.bug_compat)
-    hipLaunchKernelGGL((face_kernel<K, M, true, false>), dim3(dc.nElem, 3), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((face_kernel<K, M, true, false, T>), dim3(dc.nElem, 3), dim3(64), 0, s, a);
   else
-    hipLaunchKernelGGL((face_kernel<K, M, false, false>), dim3(dc.nElem, 3), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((face_kernel<K, M, false, false, T>), dim3(dc.nElem, 3), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+template <int K, int M>
+hipError_t faces_impl(const DeviceCase &dc, hipStream_t s)
+{
+  return dc.mixed ? faces_impl_t<K, M, float>(dc, s) : faces_impl_t<K, M, double>(dc, s);
+}
+
+template <int K, int M, typename T>
+hipError_t batch_faces_impl_t(const BatchGroup &g, int w0, int nw, hipStream_t s)
+{
+  FaceArgs<K, M, T> a{};
+  a.cases = g.cases; a.work = g.work; a.w0 = w0;
+  fill_face_ops(*g.proto, a.op);
+  if (g.proto->bug_compat)
+    hipLaunchKernelGGL((face_kernel<K, M, true, true, T>), dim3(nw, 3), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL((face_kernel<K, M, false, true, T>), dim3(nw, 3), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 
 template <int K, int M>
 hipError_t batch_faces_impl(const BatchGroup &g, int w0, int nw, hipStream_t s)
 {
-  FaceArgs<K, M> a{};
-  a.cases = g.cases; a.work = g.work; a.w0 = w0;
-  fill_face_ops(*g.proto, a.op);
-  if (g.proto->bug_compat)
-    hipLaunchKernelGGL((face_kernel<K, M, true, true>), dim3(nw, 3), dim3(64), 0, s, a);
-  else
-    hipLaunchKernelGGL((face_kernel<K, M, false, true>), dim3(nw, 3), dim3(64), 0, s, a);
-  return hipGetLastError();
+  return g.proto->mixed ? batch_faces_impl_t<K, M, float>(g, w0, nw, s)
+                        : batch_faces_impl_t<K, M, double>(g, w0, nw, s);
 }
 
 template <int K, int M>
@@ -2229,24 +2316,37 @@ hipError_t boundary_impl(const DeviceCase &dc, hipStream_t s, bool monitor)
   X(4, 6, 256) X(4, 6, 128) X(5, 7, 256) X(5, 7, 512) X(6, 8, 256) X(6, 8, 512) \
   X(7, 9, 512)
 
+template <int K, int M, int NT, typename T>
+hipError_t volume_nt_t(const DeviceCase &dc, const StageCtl &ctl, hipStream_t s)
+{
+  VolArgs<K, M, T> a{};
+  a.c = case_dev(dc);
+  fill_vol_shape(dc, ctl, a);
+  hipLaunchKernelGGL((volume_kernel<K, M, NT, false, T>), dim3(dc.nElem), dim3(NT), 0, s, a);
+  return hipGetLastError();
+}
+
 template <int K, int M, int NT>
 hipError_t volume_nt(const DeviceCase &dc, const StageCtl &ctl, hipStream_t s)
 {
-  VolArgs<K, M> a{};
-  a.c = case_dev(dc);
-  fill_vol_shape(dc, ctl, a);
-  hipLaunchKernelGGL((volume_kernel<K, M, NT, false>), dim3(dc.nElem), dim3(NT), 0, s, a);
+  return dc.mixed ? volume_nt_t<K, M, NT, float>(dc, ctl, s) : volume_nt_t<K, M, NT, double>(dc, ctl, s);
+}
+
+template <int K, int M, int NT, typename T>
+hipError_t batch_volume_nt_t(const BatchGroup &g, const StageCtl &ctl, int w0, int nw, hipStream_t s)
+{
+  VolArgs<K, M, T> a{};
+  a.cases = g.cases; a.work = g.work; a.w0 = w0;
+  fill_vol_shape(*g.proto, ctl, a);
+  hipLaunchKernelGGL((volume_kernel<K, M, NT, true, T>), dim3(nw), dim3(NT), 0, s, a);
   return hipGetLastError();
 }
 
 template <int K, int M, int NT>
 hipError_t batch_volume_nt(const BatchGroup &g, const StageCtl &ctl, int w0, int nw, hipStream_t s)
 {
-  VolArgs<K, M> a{};
-  a.cases = g.cases; a.work = g.work; a.w0 = w0;
-  fill_vol_shape(*g.proto, ctl, a);
-  hipLaunchKernelGGL((volume_kernel<K, M, NT, true>), dim3(nw), dim3(NT), 0, s, a);
-  return hipGetLastError();
+  return g.proto->mixed ? batch_volume_nt_t<K, M, NT, float>(g, ctl, w0, nw, s)
+                        : batch_volume_nt_t<K, M, NT, double>(g, ctl, w0, nw, s);
 }
 
 template <int K>

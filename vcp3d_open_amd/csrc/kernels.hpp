@@ -41,6 +41,7 @@ struct DeviceCase {
   int nElem = 0, K = 0, M = 0, nDOF = 0, nInt = 0, nInt2D = 0, P = 0;
   int bug_compat = 1;
   int nt = 0;                     // volume workgroup size (0: the default for (K, M))
+  int mixed = 0;                  // 1: FP32 contractions and solve (vcp_set_precision)
   double mu = 0, theta_sym = 1, rel_penalty = 1;
   double cfl = 1, u_ref = 1, p_ref = 1, rho_ref = 1;
   TableOffsets to{};

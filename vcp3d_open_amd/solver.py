@@ -54,7 +54,7 @@ EXPORTS = ["vcp_create", "vcp_destroy", "vcp_sizes", "vcp_reference", "vcp_set_r
            "vcp_profile_residual", "vcp_host_setup", "vcp_set_tile", "vcp_volume_tiles",
            "vcp_batch_create", "vcp_batch_destroy", "vcp_batch_add", "vcp_batch_remove",
            "vcp_batch_step", "vcp_batch_wait", "vcp_batch_status", "vcp_batch_set_graph",
-           "vcp_batch_set_tile", "vcp_batch_groups", "vcp_batch_profile", "vcp_residual_static",
+           "vcp_batch_set_tile", "vcp_batch_groups", "vcp_batch_profile", "vcp_residual_static", "vcp_set_precision",
            "vcp_last_error"]
 
 _lib = None
@@ -105,6 +105,7 @@ def load_library(path: str = LIB_PATH):
     L.vcp_batch_groups.argtypes = [vp, ip, C.c_int]
     L.vcp_batch_profile.argtypes = [vp, C.c_int, dp, C.c_int]
     L.vcp_residual_static.argtypes = [vp, C.c_int, dp]
+    L.vcp_set_precision.argtypes = [vp, C.c_int]
     L.vcp_last_error.restype = C.c_char_p
     for name in EXPORTS:
         if name != "vcp_last_error":
@@ -159,7 +160,7 @@ class SolverClass:
     """One case on the GPU. Mirrors the reference SolverClass hot-path API."""
 
     def __init__(self, case, bug_compat: bool = True, elem_grid: np.ndarray | None = None,
-                 use_graph: bool = False):
+                 use_graph: bool = False, precision: str = "fp64"):
         self._L = load_library()
         self.case = case
         desc = desc_from_case(case, bug_compat)
@@ -174,6 +175,8 @@ class SolverClass:
         self.n_dofs = self.K ** 3
         if use_graph:
             self.set_graph(True)
+        if precision != "fp64":
+            self.set_precision(precision)
 
     def _check(self, rc: int) -> int:
         if rc < 0:
@@ -273,6 +276,16 @@ class SolverClass:
         self._check(self._L.vcp_profile_residual(self.h, reps, ms))
         return float(ms[0]), float(ms[1])
 
+    def set_precision(self, precision: str) -> None:
+        """"fp64" (the reference's arithmetic) or "mixed" (FP32 contractions
+        and mass-matrix solve; state, pointwise physics, metrics, boundary
+        kernels, time step and RK update in FP64)."""
+        modes = {"fp64": 0, "mixed": 1}
+        if precision not in modes:
+            raise ValueError(f"precision must be one of {sorted(modes)}")
+        self._check(self._L.vcp_set_precision(self.h, modes[precision]))
+        self.precision = precision
+
     def set_tile(self, nt: int) -> None:
         """Volume workgroup size (threads per element; 0: the default)."""
         self._check(self._L.vcp_set_tile(self.h, int(nt)))
@@ -357,10 +370,10 @@ class Batch:
         return {by_handle[ctxs[i]]: st[i].copy() for i in range(k)}
 
     def groups(self) -> list:
-        info = np.zeros((64, 6), dtype=np.int32)
+        info = np.zeros((64, 7), dtype=np.int32)
         n = self._check(self._L.vcp_batch_groups(self.h, info, 64))
         return [dict(K=int(r[0]), M=int(r[1]), bug=int(r[2]), cases=int(r[3]), work=int(r[4]),
-                     nt=int(r[5])) for r in info[:n]]
+                     nt=int(r[5]), mixed=int(r[6])) for r in info[:n]]
 
     def profile(self, reps: int = 5) -> list:
         """Per group: (ms of boundary + face kernels, ms of the volume kernel)."""

@@ -273,6 +273,65 @@ def bench_batch(args) -> int:
     return 0
 
 
+def bench_serve(args) -> int:
+    """Config-4 workload: args.cases requests arriving as a Poisson stream
+    (50 per second, seeded), 70 % Taylor-Green cases of the batched workload's
+    distribution and 30 % 48x24x24 P=4 open boxes (PC-NSCBC, sponge + CML,
+    Mach U[0.3, 0.7]), 200 RK steps each, served by continuous batching with
+    admission by device footprint (80 % of 288 GB). value = aggregate
+    GDOF-updates/s per RK stage of the completed requests over the makespan
+    (first arrival to last completion); p50/p99 request latency (arrival to
+    completion, set-up included) beside it."""
+    import numpy as np
+    from vcp3d_open_amd import autotune, serve
+    from vcp3d_open_amd.solver import Batch, SolverClass
+
+    reqs = serve.serve_workload(seed=args.seed, count=args.cases, rk_steps=args.rk_steps)
+    # per-request error of the mixed form against FP64: the three smallest requests
+    sample = sorted(reqs, key=lambda r: r.dofs)[:3] if args.precision == "mixed" else []
+    srv = serve.Server(precision=args.precision, steps_per_tick=args.tick, workers=4,
+                       keep_solutions={r.rid for r in sample}, tiles=autotune.load_tiles())
+    t0 = time.perf_counter()
+    res = srv.run(reqs)
+    wall = time.perf_counter() - t0
+    summ = serve.summary(res)
+    print(f"[bench] served {summ['done']}/{len(reqs)} requests in {wall:.1f} s", file=sys.stderr, flush=True)
+    errors = []
+    for r in sample:
+        s = SolverClass(r.case)
+        s.ReadRestartSolution(r.case.initial_conservative())
+        if getattr(r.case, "sponge_elems", 0):
+            s.SetDampingState(r.case.damping_primitive())
+        b = Batch()
+        b.add(s, r.rk_steps)
+        b.step(r.rk_steps)
+        b.wait()
+        ref = s.solution()
+        got = next(x for x in res if x.rid == r.rid).solution
+        errors.append({"rid": r.rid, "kind": r.kind, "P": r.case.p, "n_elem": list(r.case.n_elem),
+                       "modal_rel_l2_vs_fp64": float(np.linalg.norm(got - ref) / np.linalg.norm(ref))})
+        b.close()
+        s.close()
+    out = {
+        "metric": METRIC, "value": summ["gdof_per_s"], "unit": "GDOF/s", "n_gpus": 1,
+        "steps": args.rk_steps, "warmup": 0, "ms_per_step": None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-contractions/f64",
+        "data": "synthetic",
+        "config": {"workload": f"served Poisson stream (seed {args.seed}, 50 requests/s) of {args.cases} requests: "
+                               "70 % Taylor-Green (P 2..6, n in [4, 24]), 30 % open box 48x24x24 P=4 "
+                               f"(M in [0.3, 0.7]), {args.rk_steps} RK steps each; continuous batching, "
+                               "admission by device footprint <= 80 % of 288 GB",
+                   "requests": args.cases, "steps_per_tick": args.tick},
+        "serving": {**summ, "peak_device_bytes": srv.peak_bytes, "peak_concurrent": srv.peak_active,
+                    "ticks": srv.ticks, "wall_s": wall,
+                    "latency_s": {str(r.rid): round(r.latency, 4) for r in res if r.status == "done"}},
+        "error_vs_fp64_sample": errors,
+    }
+    print(json.dumps(out))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -283,12 +342,14 @@ def main() -> int:
     ap.add_argument("--profile-reps", type=int, default=5)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (HBM counter calibration)")
-    ap.add_argument("--workload", choices=["tgv", "openbox", "batch"], default="tgv",
+    ap.add_argument("--rk-steps", type=int, default=200, help="serve: RK steps per request")
+    ap.add_argument("--tick", type=int, default=8, help="serve: batched RK steps between status reads")
+    ap.add_argument("--workload", choices=["tgv", "openbox", "batch", "serve"], default="tgv",
                     help="tgv: curvilinear Taylor-Green 32^3 P=4 (default); openbox: 48x24x24 P=4 "
                          "PC-NSCBC in/outflow, 8-element sponge + CML, M=0.5; batch: 64 seeded "
                          "Taylor-Green cases of uneven P and size stepped together")
     ap.add_argument("--seed", type=int, default=2024, help="batch: seed of the case draw")
-    ap.add_argument("--cases", type=int, default=64, help="batch: number of cases")
+    ap.add_argument("--cases", type=int, default=64, help="batch: number of cases; serve: number of requests")
     ap.add_argument("--autotune", action="store_true",
                     help="batch: time every volume workgroup size per shape and record the table")
     ap.add_argument("--graph", action="store_true", help="batch: replay each RK step as a HIP graph")
@@ -306,6 +367,8 @@ def main() -> int:
         return 2
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     from vcp3d_open_amd.cases import OpenBoxCase, TGVCase
     from vcp3d_open_amd.solver import SolverClass

@@ -23,6 +23,8 @@
 #ifndef VCP3D_AMD_H
 #define VCP3D_AMD_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -121,6 +123,10 @@ typedef struct vcp_batch vcp_batch;
    (SolverClass.cpp:12-61, :2787-3034, :829-1016). */
 int vcp_create(const vcp_case_desc *desc, const double *elem_grid, vcp_ctx **out);
 int vcp_destroy(vcp_ctx *ctx);
+
+/* Device memory the context holds (bytes of its allocations), no
+   counterpart in the reference; the serving loop admits cases by it. */
+int vcp_device_bytes(const vcp_ctx *ctx, size_t *bytes);
 
 /* Sizes: number of elements, nDOFs1D (K), nInt1D (M). */
 int vcp_sizes(const vcp_ctx *ctx, int *n_elem, int *n_dofs_1d, int *n_int_1d);

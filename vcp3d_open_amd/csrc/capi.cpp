@@ -710,6 +710,15 @@ int vcp_set_tile(vcp_ctx *c, int nt)
   VCP_GUARD_END
 }
 
+int vcp_device_bytes(const vcp_ctx *c, size_t *bytes)
+{
+  VCP_GUARD_BEGIN
+  if (!c || !bytes) return fail("null argument");
+  *bytes = c->bytes;
+  return 0;
+  VCP_GUARD_END
+}
+
 int vcp_set_precision(vcp_ctx *c, int mode)
 {
   VCP_GUARD_BEGIN

@@ -52,6 +52,7 @@ struct vcp_ctx {
   // [kHostFlags] the error flags (as int)
   double *host = nullptr;
   std::vector<void *> allocs;
+  size_t bytes = 0;               // device bytes requested by alloc()
   bool res_valid = false;
   bool dt_ready = false;          // host[0] holds the time step of the current state
   bool damping_set = false;
@@ -74,8 +75,9 @@ struct vcp_ctx {
   hipError_t alloc(T **p, size_t n)
   {
     void *v = nullptr;
-    hipError_t e = hipMalloc(&v, n * sizeof(T) > 0 ? n * sizeof(T) : 16);
-    if (e == hipSuccess) { allocs.push_back(v); *p = static_cast<T *>(v); }
+    const size_t nb = n * sizeof(T) > 0 ? n * sizeof(T) : 16;
+    hipError_t e = hipMalloc(&v, nb);
+    if (e == hipSuccess) { allocs.push_back(v); bytes += nb; *p = static_cast<T *>(v); }
     return e;
   }
   void drop_graphs()

@@ -54,7 +54,7 @@ EXPORTS = ["vcp_create", "vcp_destroy", "vcp_sizes", "vcp_reference", "vcp_set_r
            "vcp_profile_residual", "vcp_host_setup", "vcp_set_tile", "vcp_volume_tiles",
            "vcp_batch_create", "vcp_batch_destroy", "vcp_batch_add", "vcp_batch_remove",
            "vcp_batch_step", "vcp_batch_wait", "vcp_batch_status", "vcp_batch_set_graph",
-           "vcp_batch_set_tile", "vcp_batch_groups", "vcp_batch_profile", "vcp_residual_static", "vcp_set_precision",
+           "vcp_batch_set_tile", "vcp_batch_groups", "vcp_batch_profile", "vcp_residual_static", "vcp_set_precision", "vcp_device_bytes",
            "vcp_last_error"]
 
 _lib = None
@@ -106,6 +106,7 @@ def load_library(path: str = LIB_PATH):
     L.vcp_batch_profile.argtypes = [vp, C.c_int, dp, C.c_int]
     L.vcp_residual_static.argtypes = [vp, C.c_int, dp]
     L.vcp_set_precision.argtypes = [vp, C.c_int]
+    L.vcp_device_bytes.argtypes = [vp, C.POINTER(C.c_size_t)]
     L.vcp_last_error.restype = C.c_char_p
     for name in EXPORTS:
         if name != "vcp_last_error":
@@ -275,6 +276,12 @@ class SolverClass:
         ms = np.zeros(2)
         self._check(self._L.vcp_profile_residual(self.h, reps, ms))
         return float(ms[0]), float(ms[1])
+
+    def device_bytes(self) -> int:
+        """Device memory held by the context (bytes of its allocations)."""
+        n = C.c_size_t()
+        self._check(self._L.vcp_device_bytes(self.h, C.byref(n)))
+        return int(n.value)
 
     def set_precision(self, precision: str) -> None:
         """"fp64" (the reference's arithmetic) or "mixed" (FP32 contractions

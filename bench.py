@@ -15,7 +15,11 @@ Also reported: the roofline position of the dominant kernel (its mean launch
 time measured with HIP events on the library's stream) and a CPU baseline:
 one RK step of the same case by the reference itself, built from its sources
 (oracle/_ref), on the host cores, with the C oracle's rate beside it (the
-oracle alone where the reference build is absent or cannot run).
+oracle alone where the reference build is absent or cannot run). Beside the
+FP64 line: the mixed-precision form on the same case with its difference
+from FP64 ("mixed"), and the batched (64 seeded Taylor-Green cases, FP64 and
+mixed) and served (40-request Poisson stream, mixed, p50/p99 latency) forms
+("forms"). --workload batch / serve / openbox measure those alone.
 """
 from __future__ import annotations
 
@@ -273,6 +277,57 @@ def bench_batch(args) -> int:
     return 0
 
 
+def forms_summary(args) -> dict:
+    """The batched (config 3) and served (config 4) forms beside the default
+    line: the 64-case batch in FP64 and in mixed precision (the same cases,
+    10 timed RK steps after 2 warm-up steps each), and the 40-request served
+    stream in mixed precision."""
+    from vcp3d_open_amd import autotune, serve
+    from vcp3d_open_amd.cases import batch_workload
+    from vcp3d_open_amd.solver import Batch, SolverClass
+
+    cases = batch_workload(seed=2024, count=64)
+    solvers = []
+    for c in cases:
+        g = SolverClass(c)
+        g.ReadRestartSolution(c.initial_conservative())
+        solvers.append(g)
+    ndof = sum(5 * c.n_elems * c.n_dofs for c in cases)
+    out = {"batch": {"cases": len(cases), "dofs": ndof, "rk_steps_timed": 10}}
+    for precision in ("fp64", "mixed"):
+        for g in solvers:
+            g.set_precision(precision)
+        b = Batch(tiles=autotune.load_tiles())
+        for g in solvers:
+            b.add(g, 0)
+        b.step(2)
+        b.wait()
+        t0 = time.perf_counter()
+        b.step(10)
+        b.wait()
+        secs = time.perf_counter() - t0
+        flags = [int(w[3]) for w in b.status().values()]
+        b.close()
+        out["batch"][precision] = {"value": ndof * 3 * 10 / secs / 1e9, "unit": "GDOF/s",
+                                   "ms_per_rk_step": secs / 10 * 1e3, "cases_with_error_flags": sum(f != 0 for f in flags)}
+        print(f"[bench] batch {precision}: {out['batch'][precision]['value']:.2f} GDOF/s", file=sys.stderr, flush=True)
+    out["batch"]["mixed_speedup"] = out["batch"]["mixed"]["value"] / out["batch"]["fp64"]["value"]
+    for g in solvers:
+        g.close()
+    reqs = serve.serve_workload(seed=2024, count=40)
+    srv = serve.Server(precision="mixed", tiles=autotune.load_tiles())
+    res = srv.run(reqs)
+    summ = serve.summary(res)
+    out["serve"] = {"precision": "mixed", "requests": len(reqs), "rate_per_s": 50, "rk_steps": 200,
+                    "value": summ["gdof_per_s"], "unit": "GDOF/s",
+                    "p50_latency_s": summ["p50_latency_s"], "p99_latency_s": summ["p99_latency_s"],
+                    "done": summ["done"], "failed": summ["failed"], "makespan_s": summ["makespan_s"],
+                    "peak_concurrent": srv.peak_active, "peak_device_bytes": srv.peak_bytes}
+    print(f"[bench] serve mixed: {summ['gdof_per_s']:.2f} GDOF/s, p50 {summ['p50_latency_s']:.2f} s, "
+          f"p99 {summ['p99_latency_s']:.2f} s", file=sys.stderr, flush=True)
+    return out
+
+
 def bench_serve(args) -> int:
     """Config-4 workload: args.cases requests arriving as a Poisson stream
     (50 per second, seeded), 70 % Taylor-Green cases of the batched workload's
@@ -358,6 +413,8 @@ def main() -> int:
                          "mass-matrix solve, FP64 state, pointwise physics, metric, boundaries and RK update)")
     ap.add_argument("--no-mixed", action="store_true",
                     help="tgv/openbox: skip the mixed-precision measurement beside the FP64 line")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="tgv: skip the batched and served forms beside the default line")
     ap.add_argument("--accuracy-steps", type=int, default=0,
                     help="tgv/openbox: also advance FP64 and mixed this many RK steps from the same state "
                          "and report their field, kinetic-energy and enstrophy differences")
@@ -478,6 +535,8 @@ def main() -> int:
                                           for v in range(5)],
             "kinetic_energy_rel": abs(fm["kinetic_energy"] / fo["kinetic_energy"] - 1),
             "enstrophy_rel": abs(fm["enstrophy"] / fo["enstrophy"] - 1)}
+    if args.workload == "tgv" and args.precision == "fp64" and not args.no_forms:
+        out["forms"] = forms_summary(args)
     if args.calib_copy_mib:
         out["calib_copy_bytes"] = calib_copy(args.calib_copy_mib << 20)
     if not args.no_cpu_baseline:

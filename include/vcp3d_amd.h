@@ -143,6 +143,14 @@ int vcp_set_restart(vcp_ctx *ctx, const double *nodal);
    and ElementClass::InitializeDampingState, SpongeLayer.cpp:940-997).
    Needed before the first residual of a case with a sponge layer. */
 int vcp_set_damping_state(vcp_ctx *ctx, const double *nodal_prim);
+/* The same target from the case's own running averages, on the device (a
+   case created with compute_average and a sponge layer, after at least one
+   averaged synchronization step): the averaged primitive variables that
+   SolverClass::ReadAveragedSolutionSpongeLayer reads from the averaged
+   restart file (SpongeLayer.cpp:665-838) are taken from the device and
+   converted as ElementClass::InitializeDampingState does (SpongeLayer.cpp:
+   940-997), with no file or host copy in between. */
+int vcp_damping_from_averages(vcp_ctx *ctx);
 /* SolverClass::AverageDataNSCBC (SolverClass.cpp:2502-2581): boundary-averaged
    face-normal Mach number of the six block faces for the current state
    (0 where the face has no NSCBC). vcp_residual and vcp_sync_step evaluate it

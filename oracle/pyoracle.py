@@ -97,6 +97,7 @@ def lib():
         L.orc_time_steps.argtypes = [C.c_void_p, dp]
         L.orc_cg_iterations.argtypes = [C.c_void_p, ip]
         L.orc_get_setup.argtypes = [C.c_void_p, C.c_int, dp]
+        L.orc_set_setup.argtypes = [C.c_void_p, C.c_int, dp]
         L.orc_get_table.argtypes = [C.c_void_p, C.c_int, dp]
         L.orc_tensor_apply.argtypes = [C.c_int] * 6 + [dp] * 5
         L.orc_cg_solve.argtypes = [C.c_void_p, C.c_int, dp, dp]
@@ -197,6 +198,14 @@ class Oracle:
         out = np.zeros(shape)
         lib().orc_get_setup(self.h, which, out)
         return out
+
+    def use_geometry(self, setup: dict):
+        """Evaluate on another set-up's geometry: the vol / faces / lens
+        arrays of vcp3d_open_amd.solver.host_setup (the GPU path's own)."""
+        lib().orc_set_setup(self.h, 0, np.ascontiguousarray(setup["vol"], dtype=np.float64))
+        for f in range(6):
+            lib().orc_set_setup(self.h, 1 + f, np.ascontiguousarray(setup["faces"][f], dtype=np.float64))
+        lib().orc_set_setup(self.h, 7, np.ascontiguousarray(setup["lens"], dtype=np.float64))
 
     def cg_solve(self, e, rhs):
         x = np.zeros((5, self.n_dofs))

@@ -2010,6 +2010,14 @@ void orc_get_setup(const orc_case *c, int which, double *out)
   }
 }
 
+void orc_set_setup(orc_case *c, int which, const double *in)
+{
+  const size_t E = c->nElem;
+  if (which == 0) memcpy(c->vm, in, sizeof(double) * E * 13 * c->nInt);
+  else if (which >= 1 && which <= 6) memcpy(c->sm[which - 1], in, sizeof(double) * E * 16 * c->nInt2D);
+  else if (which == 7) memcpy(c->len, in, sizeof(double) * E * 4);
+}
+
 void orc_get_table(const orc_case *c, int which, double *out)
 {
   const int K = c->K, M = c->M;

@@ -119,6 +119,10 @@ void orc_nscbc_average(orc_case *c, double mach_avg[6]);
    9 damping functions [nElem][3][nInt] (0 outside the sponge),
    10 damping state [nElem][5][nInt] (0 outside the sponge). */
 void orc_get_setup(const orc_case *c, int which, double *out);
+/* Test hook: replace the volume metric (0), a face metric (1..6) or the
+   length scales (7) with given arrays of the orc_get_setup layouts, so that
+   the residual can be evaluated on another set-up's geometry. */
+void orc_set_setup(orc_case *c, int which, const double *in);
 /* 1D tables: which 0 Gauss points, 1 weights, 2 legendre at int [K][M],
    3 derivative [K][M], 4 nodal DOF locations [K], 5 grid DOF locations,
    6 PC-NSCBC matrix [M^2][M^2], 7 Lagrange derivative on the Gauss points

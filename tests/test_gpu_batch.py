@@ -24,6 +24,8 @@ from vcp3d_open_amd.cases import OpenBoxCase, TGVCase
 
 pytestmark = pytest.mark.gpu
 
+STEPS_DONE_ = 1     # status word of vcp_batch_status: RK steps done
+
 
 def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
@@ -242,3 +244,28 @@ def test_tiles_match_oracle_and_alone(lib, oracle_lib):
             for _ in range(2):
                 o.sync_step()
             assert rel(g.solution(), o.modal()) < 1e-12, (kw, nt)
+
+
+def test_case_joining_mid_run_matches_alone(lib):
+    """A case added to a batch after the others have taken RK steps (as the
+    serving loop adds arriving requests) ends bit-identical to the same case
+    alone, and so do the cases that were already running."""
+    from vcp3d_open_amd.solver import Batch, SolverClass
+    first, late = TGVCase(n=3, p=3, mach=0.2), TGVCase(n=4, p=4, curvilinear=True)
+    print("[case] a case joins a running batch after 5 RK steps", flush=True)
+
+    def make(c):
+        g = SolverClass(c)
+        g.ReadRestartSolution(c.initial_conservative())
+        return g
+
+    a, b = make(first), make(late)
+    batch = Batch()
+    batch.add(a, 12)
+    batch.step(5)
+    batch.add(b, 9)
+    batch.step(9)
+    st = batch.status()
+    assert st[a][STEPS_DONE_] == 12 and st[b][STEPS_DONE_] == 9
+    assert np.array_equal(a.solution(), run_batch_of_one(first, 12).solution())
+    assert np.array_equal(b.solution(), run_batch_of_one(late, 9).solution())

@@ -109,3 +109,67 @@ def test_open_box_free_stream(lib, inflow):
     g.Residual(True)
     assert np.abs(g.residual_array()).max() < 1e-10
     np.testing.assert_allclose(g.AverageDataNSCBC(), [case.mach, case.mach, 0, 0, 0, 0], rtol=1e-12, atol=0)
+
+
+def reflected_amplitude(case, nodal, x_range=(1.3, 5.0 / 3.0)):
+    """Largest |p - p_inf| / (pulse amplitude p_inf) at the nodes between x0
+    and the start of the outlet sponge: after the pulse's downstream half has
+    left through the NSCBC outlet, what remains there is reflected."""
+    x = case.nodal_coords()[..., 0]
+    rho, ru, rv, rw, rE = (nodal[..., i] for i in range(5))
+    p = (GAMMA - 1.0) * (rE - 0.5 * (ru ** 2 + rv ** 2 + rw ** 2) / rho)
+    sel = (x >= x_range[0]) & (x < x_range[1])
+    return float(np.abs(p[sel] - case.p_inf).max() / (case.pulse_amp * case.p_inf))
+
+
+def test_pulse_reflection_against_reference(lib):
+    """A 1e-3 pressure pulse in a 12 x 2 x 2, P = 3, M = 0.5 open box
+    (PC-NSCBC static inflow/outflow, 2-element sponge + CML): after 500 and
+    1000 RK steps the GPU state equals the reference's within relative L2
+    1e-11, and the reflected-wave amplitude in front of the outlet sponge
+    at step 1000 (about 5 % of the pulse) equals the reference's to 1e-8."""
+    case, fx = load("ob_reflect_p3")
+    print("[case] ob_reflect_p3: 1000 RK steps against the reference", flush=True)
+    g = start_gpu(case)
+    steps = 0
+    for target in (500, 1000):
+        while steps < target:
+            g.ExecuteTimeSynchronizationStep()
+            steps += g.last_rk_steps
+        assert steps == target
+        got, ref = g.WriteRestartSolution(), fx[f"sol_{target}"]
+        err = rel(got, ref)
+        print(f"step {target}: state rel {err:.3e}", flush=True)
+        assert err < 1e-11
+    a_g, a_r = reflected_amplitude(case, got), reflected_amplitude(case, ref)
+    print(f"reflected amplitude {a_g:.6e} (reference {a_r:.6e})", flush=True)
+    assert 0.01 < a_r < 0.2
+    assert abs(a_g / a_r - 1) < 1e-8
+
+
+def test_sponge_target_from_running_averages(lib):
+    """The sponge target taken on the device from the case's own running
+    averages (vcp_damping_from_averages) equals the host conversion of the
+    same averaged primitives handed over as the averaged file's columns
+    (vcp_set_damping_state, the reference's ReadAveragedSolutionSpongeLayer
+    + InitializeDampingState path): the residuals agree to 1e-12, and differ
+    from the residual with the original free-stream target."""
+    case = OpenBoxCase(n=(6, 2, 2), p=3, sponge_elems=2, compute_average=True, pulse_amp=0.05, pulse_width=0.2)
+    print("[case] open box 6x2x2 P=3, sponge target from 3 averaged steps", flush=True)
+    a = start_gpu(case)
+    for _ in range(3):
+        a.ExecuteTimeSynchronizationStep()
+    avg, n = a.averages()
+    assert n == 3
+    a.Residual(True)
+    r_free = a.residual_array()
+    a.DampingStateFromAverages()
+    a.Residual(True)
+    r_dev = a.residual_array()
+    b = start_gpu(case)
+    b.set_solution(a.solution())
+    b.SetDampingState(np.ascontiguousarray(avg[..., :5]))
+    b.Residual(True)
+    r_host = b.residual_array()
+    assert rel(r_dev, r_host) < 1e-12, rel(r_dev, r_host)
+    assert rel(r_dev, r_free) > 1e-9

@@ -1,8 +1,10 @@
 """GPU parity: the gfx950 residual and RK step against the C oracle on the
 same seeded inputs, through the C ABI.
 
-Tolerances: FP64 fields after k RK steps within relative L2 1e-12; the
-residual (R = M^-1 res) within relative L2 1e-9, because the residual of
+Tolerances: FP64 fields after k RK steps within relative L2 1e-12 (over
+500 steps at 8^3 P=4: observed 2e-14); the residual (R = M^-1 res) within
+relative L2 3e-11 when the oracle runs on the GPU path's own metric arrays,
+and 1e-10 when each side computes its metric terms, because the residual of
 these low-Mach flows is conditioned at ~2e4 (a 1e-16 relative perturbation
 of the modal input moves it by ~2e-12, measured with the oracle), so
 summation-order differences alone reach ~1e-11.
@@ -50,7 +52,8 @@ def test_residual_matches_oracle(lib, oracle_lib, name, kw, bug):
     md_o = o.residual()
     md_g = g.Residual(True)
     ro, rg = o.residual_array(), g.residual_array()
-    assert rel(rg, ro) < 1e-9, rel(rg, ro)
+    print(f"[case] {name} bug={bug}: residual rel {rel(rg, ro):.3e} (own metric set-ups)", flush=True)
+    assert rel(rg, ro) < 1e-10, rel(rg, ro)
     assert abs(md_g[6] - md_o[6]) <= 1e-12 * abs(md_o[6])
     it_o, it_g = o.cg_iterations(), g.cg_iterations()
     assert np.abs(it_o - it_g).max() <= 1
@@ -102,3 +105,62 @@ def test_time_step_reuse_across_calls(lib, oracle_lib):
     g.ExecuteTimeSynchronizationStep()
     assert rel(g.solution(), o.modal()) < 1e-12
     assert rel(g.residual_array(), o.residual_array()) < 1e-9
+
+
+@pytest.mark.parametrize("name,kw", CASES, ids=[c[0] for c in CASES])
+def test_residual_on_shared_geometry(lib, oracle_lib, name, kw):
+    """The oracle evaluated on the GPU path's own metric arrays
+    (vcp_host_setup), so that only the kernels' arithmetic differs: the
+    residual agrees to 3e-11 relative L2 (observed 6e-13 .. 9.3e-12: the
+    residual's conditioning, ~2e4, turns summation-order differences of
+    1e-16 into 1e-12), and the CG iteration counts agree (histogram of the
+    differences printed; observed all equal)."""
+    from vcp3d_open_amd.solver import SolverClass, host_setup
+    case = TGVCase(**kw)
+    o = oracle_lib.Oracle(case)
+    o.use_geometry(host_setup(case))
+    g = SolverClass(case)
+    nodal = case.initial_conservative()
+    o.set_restart(nodal)
+    g.ReadRestartSolution(nodal)
+    o.residual()
+    g.Residual(True)
+    err = rel(g.residual_array(), o.residual_array())
+    d = np.abs(o.cg_iterations() - g.cg_iterations())
+    print(f"[case] {name}: residual rel {err:.3e}, CG iteration differences {np.bincount(d).tolist()}", flush=True)
+    assert err < 3e-11
+    assert d.max() <= 1 and (d == 0).mean() >= 0.99
+
+
+def test_long_p4_trajectory(lib, oracle_lib):
+    """8^3 P=4 curvilinear Taylor-Green vortex, 500 RK steps on the GPU and in
+    the oracle: states within relative L2 1e-12 at every 100th step, and the
+    kinetic energy and enstrophy series equal to 1e-12."""
+    from vcp3d_open_amd.diagnostics import flow_integrals
+    from vcp3d_open_amd.solver import SolverClass, host_setup
+    case = TGVCase(n=8, p=4, curvilinear=True)
+    print("[case] 8^3 P=4 curvilinear, 500 RK steps vs the oracle", flush=True)
+    g = SolverClass(case)
+    o = oracle_lib.Oracle(case)
+    nodal = case.initial_conservative()
+    g.ReadRestartSolution(nodal)
+    o.set_restart(nodal)
+    vol = host_setup(case)["vol"]
+    steps = 0
+    series = []
+    while steps < 500:
+        g.ExecuteTimeSynchronizationStep()
+        o.sync_step()
+        steps += g.last_rk_steps
+        if steps % 100 == 0:
+            ug, uo = g.solution(), o.modal()
+            fg, fo = flow_integrals(case, ug, vol), flow_integrals(case, uo, vol)
+            series.append((steps, rel(ug, uo), fg["kinetic_energy"], fg["enstrophy"]))
+            print(f"step {steps}: state rel {series[-1][1]:.3e}, KE {fg['kinetic_energy']:.12e} "
+                  f"(oracle {fo['kinetic_energy']:.12e}), enstrophy {fg['enstrophy']:.12e} "
+                  f"(oracle {fo['enstrophy']:.12e})", flush=True)
+            assert series[-1][1] < 1e-12
+            assert abs(fg["kinetic_energy"] / fo["kinetic_energy"] - 1) < 1e-12
+            assert abs(fg["enstrophy"] / fo["enstrophy"] - 1) < 1e-12
+    # the vortex decays: kinetic energy falls monotonically over the run
+    assert all(b[2] < a[2] for a, b in zip(series, series[1:]))

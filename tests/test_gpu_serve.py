@@ -27,7 +27,9 @@ class BadTGV(TGVCase):
 def requests():
     kws = [dict(n=3, p=2, mach=0.3), dict(n=4, p=3, curvilinear=True), dict(n=2, p=4), dict(n=4, p=5, curvilinear=True),
            dict(n=2, p=6, mach=0.2)]
-    out = [Request(i, TGVCase(**kw), 6 + 2 * i, 0.01 * i) for i, kw in enumerate(kws)]
+    # the first request runs long enough (400 RK steps) for the others to be
+    # built and join while it is on the GPU
+    out = [Request(i, TGVCase(**kw), 400 if i == 0 else 6 + 2 * i, 0.01 * i) for i, kw in enumerate(kws)]
     out.append(Request(5, OpenBoxCase(n=(6, 2, 2), p=3, sponge_elems=2), 8, 0.02, "openbox"))
     return out
 

@@ -289,6 +289,7 @@ class OpenBoxCase:
     theta_sym: float = 1.0
     rel_penalty: float = 1.0
     reynolds: float = 1.0e6          # written to the parameter file only
+    compute_average: bool = False    # running averages of the primitive variables
     name: str = "openbox"
 
     def __post_init__(self):
@@ -423,7 +424,7 @@ class OpenBoxCase:
                  outlet_beta_l=self.outlet_beta_l, outlet_beta_t=self.outlet_beta_t,
                  outlet_one_minus_beta=int(self.outlet_one_minus_beta), outlet_average_local=0,
                  inlet_len=self.inlet_len, inlet_sigma=self.inlet_sigma,
-                 inlet_incoming=int(self.inlet_incoming))
+                 inlet_incoming=int(self.inlet_incoming), compute_average=int(self.compute_average))
         if self.sponge_elems > 0:
             d.update(sponge_layer=1, cml=int(self.cml),
                      sponge_elems=[0, self.sponge_elems, 0, 0, 0, 0],

@@ -234,7 +234,7 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
   for (int q = 0; q < M; ++q) dc.opW[q] = t.w_int[q];
   for (int q = 0; q < K; ++q) dc.opC0[q] = t.c0[q];
   dc.to.w3 = put(t.w3); dc.to.w2 = put(t.w2); dc.to.bc = put(t.basis_center);
-  dc.to.P1 = put(t.P1); dc.to.dLq = put(t.dLq); dc.to.total = (int) tb.size();
+  dc.to.P1 = put(t.P1); dc.to.dLq = put(t.dLq); dc.to.Lq = put(t.Lq); dc.to.total = (int) tb.size();
   // open boundaries: face list, own face metric, parameters
   dc.nBFace = (int) ob.faces.size() / 2;
   std::vector<double> bgeo((size_t) dc.nBFace * 13 * M * M);
@@ -442,6 +442,22 @@ int vcp_set_damping_state(vcp_ctx *c, const double *nodal_prim)
     if (sp.idx[e] >= 0)
       vcp::damping_state(c->tab, c->rc, nodal_prim + (size_t) e * nD * 5, &ds[(size_t) sp.idx[e] * 5 * nInt]);
   HIP_TRY(hipMemcpyAsync(c->dc.dampState, ds.data(), ds.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->damping_set = true;
+  c->res_valid = false;
+  return 0;
+  VCP_GUARD_END
+}
+
+int vcp_damping_from_averages(vcp_ctx *c)
+{
+  VCP_GUARD_BEGIN
+  if (!c) return fail("null context");
+  if (int rc_ = not_in_batch(c)) return rc_;
+  if (!c->prm.sponge_layer) return fail("no sponge layer in this case");
+  if (!c->dc.avg) return fail("the case was created without compute_average");
+  if (c->n_avg < 1) return fail("no averaged synchronization step yet");
+  HIP_TRY(vcp::launch_damping_from_averages(c->dc, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->damping_set = true;
   c->res_valid = false;

@@ -2078,6 +2078,71 @@ __global__ __launch_bounds__(64) void average_kernel(AvgArgs a)
   }
 }
 
+// Sponge target from the running averages (ElementClass::InitializeDampingState,
+// SpongeLayer.cpp:943-997): the averaged nodal primitive variables
+// (dimensional, avg[e][0..4]) interpolated to the integration points with the
+// Lagrange basis of the nodal DOFs, made non-dimensional and converted to the
+// conservative variables, into dampState of the element's sponge slot. One
+// wave per element; elements outside the sponge return at once.
+struct DampAvgArgs {
+  const double *avg, *tables;
+  const int *spIdx;
+  double *dampState;
+  TableOffsets to;
+  double u_ref, p_ref, rho_ref;
+};
+
+template <int K, int M>
+__global__ __launch_bounds__(64) void damping_avg_kernel(DampAvgArgs a)
+{
+  constexpr int ND = K * K * K, NI = M * M * M;
+  __shared__ double sL[M * K], s0[5 * K * K * M], s1[5 * K * M * M], s2[5 * NI];
+  const int e = blockIdx.x, tid = threadIdx.x;
+  const int sp = a.spIdx[e];
+  if (sp < 0) return;
+  for (int i = tid; i < M * K; i += 64) sL[i] = a.tables[a.to.Lq + i];
+  __syncthreads();
+  const double *av = a.avg + (size_t) e * 11 * ND;
+  // along i: s0[l][k][j][p]
+  for (int o = tid; o < 5 * K * K * M; o += 64) {
+    const int p = o % M, r = o / M;          // r = (l k j)
+    const double *x = av + (r / (K * K)) * ND + (r % (K * K)) * K;
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < K; ++q) s += sL[p * K + q] * x[q];
+    s0[o] = s;
+  }
+  __syncthreads();
+  // along j: s1[l][k][pj][pi]
+  for (int o = tid; o < 5 * K * M * M; o += 64) {
+    const int pi = o % M, pj = (o / M) % M, lk = o / (M * M);
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < K; ++q) s += sL[pj * K + q] * s0[(lk * K + q) * M + pi];
+    s1[o] = s;
+  }
+  __syncthreads();
+  // along k: s2[l][pk][pj][pi]
+  for (int o = tid; o < 5 * NI; o += 64) {
+    const int pij = o % (M * M), pk = (o / (M * M)) % M, l = o / NI;
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < K; ++q) s += sL[pk * K + q] * s1[((l * K + q) * M * M) + pij];
+    s2[o] = s;
+  }
+  __syncthreads();
+  double *ds = a.dampState + (size_t) sp * 5 * NI;
+  for (int m = tid; m < NI; m += 64) {
+    const double rho = s2[m] / a.rho_ref, u = s2[NI + m] / a.u_ref, v = s2[2 * NI + m] / a.u_ref,
+                 w = s2[3 * NI + m] / a.u_ref, pr = s2[4 * NI + m] / a.p_ref;
+    ds[m] = rho;
+    ds[NI + m] = rho * u;
+    ds[2 * NI + m] = rho * v;
+    ds[3 * NI + m] = rho * w;
+    ds[4 * NI + m] = pr * kInvGm1 + 0.5 * rho * (u * u + v * v + w * w);
+  }
+}
+
 // Minimum of the per-element time steps into scal[0] (one workgroup; a
 // fixed reduction order, no atomics on one address from every element).
 __global__ __launch_bounds__(1024) void dt_min_kernel(const double *dtElem, int n, double *scal)
@@ -2474,6 +2539,22 @@ hipError_t launch_average(const DeviceCase &dc, double factNew, hipStream_t s)
     case 6: return avg_impl<6>(dc, factNew, s);
     case 7: return avg_impl<7>(dc, factNew, s);
   }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_damping_from_averages(const DeviceCase &dc, hipStream_t s)
+{
+  if (!dc.avg || !dc.nSponge) return hipErrorInvalidValue;
+  DampAvgArgs a;
+  a.avg = dc.avg; a.tables = dc.tables; a.spIdx = dc.spIdx; a.dampState = dc.dampState; a.to = dc.to;
+  a.u_ref = dc.u_ref; a.p_ref = dc.p_ref; a.rho_ref = dc.rho_ref;
+#define X(k, m)                                                                                    \
+  if (dc.K == k && dc.M == m) {                                                                    \
+    hipLaunchKernelGGL((damping_avg_kernel<k, m>), dim3(dc.nElem), dim3(64), 0, s, a);             \
+    return hipGetLastError();                                                                      \
+  }
+  VCP_PAIRS(X)
+#undef X
   return hipErrorInvalidValue;
 }
 

@@ -9,7 +9,7 @@ namespace vcp {
 
 // Offsets (in doubles) of the 1D tables in the device table block.
 struct TableOffsets {
-  int A, D, AT, DT, LD, fmin, fmax, dfmin, dfmax, w3, w2, bc, P1, dLq, total;
+  int A, D, AT, DT, LD, fmin, fmax, dfmin, dfmax, w3, w2, bc, P1, dLq, Lq, total;
 };
 
 // PC-NSCBC parameters of the block faces (InputParamClass "NSCBC tuning
@@ -150,6 +150,9 @@ hipError_t launch_volume(const DeviceCase &dc, const StageCtl &ctl, hipStream_t 
 hipError_t launch_time_step(const DeviceCase &dc, hipStream_t s);
 // running averages avg = (1 - factNew) avg + factNew q of the current state
 hipError_t launch_average(const DeviceCase &dc, double factNew, hipStream_t s);
+// sponge target (dampState) of every sponge element from the running
+// averages of the primitive variables (ElementClass::InitializeDampingState)
+hipError_t launch_damping_from_averages(const DeviceCase &dc, hipStream_t s);
 // scal[kScalDt] = dt, ordered on the stream
 hipError_t launch_set_dt(const DeviceCase &dc, double dt, hipStream_t s);
 // p[0, n) = v on stream s

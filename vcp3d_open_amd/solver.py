@@ -55,6 +55,7 @@ EXPORTS = ["vcp_create", "vcp_destroy", "vcp_sizes", "vcp_reference", "vcp_set_r
            "vcp_batch_create", "vcp_batch_destroy", "vcp_batch_add", "vcp_batch_remove",
            "vcp_batch_step", "vcp_batch_wait", "vcp_batch_status", "vcp_batch_set_graph",
            "vcp_batch_set_tile", "vcp_batch_groups", "vcp_batch_profile", "vcp_residual_static", "vcp_set_precision", "vcp_device_bytes",
+           "vcp_damping_from_averages",
            "vcp_last_error"]
 
 _lib = None
@@ -107,6 +108,7 @@ def load_library(path: str = LIB_PATH):
     L.vcp_residual_static.argtypes = [vp, C.c_int, dp]
     L.vcp_set_precision.argtypes = [vp, C.c_int]
     L.vcp_device_bytes.argtypes = [vp, C.POINTER(C.c_size_t)]
+    L.vcp_damping_from_averages.argtypes = [vp]
     L.vcp_last_error.restype = C.c_char_p
     for name in EXPORTS:
         if name != "vcp_last_error":
@@ -220,6 +222,11 @@ class SolverClass:
         """Averaged nodal primitive state of the sponge layer
         (SolverClass::ReadAveragedSolutionSpongeLayer)."""
         self._check(self._L.vcp_set_damping_state(self.h, np.ascontiguousarray(nodal_prim, dtype=np.float64)))
+
+    def DampingStateFromAverages(self) -> None:
+        """Sponge target from the case's own running averages, on the device
+        (in place of ReadAveragedSolutionSpongeLayer's averaged file)."""
+        self._check(self._L.vcp_damping_from_averages(self.h))
 
     def AverageDataNSCBC(self) -> np.ndarray:
         """Boundary-averaged face-normal Mach number of the six block faces."""

@@ -104,7 +104,15 @@ typedef struct vcp_case_desc {
      farfield state when the prescribed data do not fix the free stream
      (InputParamClass.cpp:991-1191) */
   double free_stream_dir[3];
+  /* "Subgrid scale model" (InputParamClass.cpp:241-250): VCP_SGS_NONE,
+     VCP_SGS_WALE or VCP_SGS_VREMAN (SGSModel.cpp:34-210); the eddy viscosity
+     enters the volume and face viscous fluxes, the penalty and symmetrising
+     terms, the time step, monitoring[7] and the averaged eddy viscosity of
+     vcp_get_averages */
+  int    sgs_model;
 } vcp_case_desc;
+
+enum { VCP_SGS_NONE = 0, VCP_SGS_WALE = 1, VCP_SGS_VREMAN = 2 };
 
 /* Block-face boundary conditions ("Subface information",
    InputParamClass.cpp:1530-1548): BCOutflowNSCBC, BCInflowStaticNSCBC,
@@ -203,7 +211,7 @@ int vcp_set_graph(vcp_ctx *ctx, int enable);
    SolverClass::UpdateAverageData, SolverClass.cpp:3800-3850,
    ElementClass.cpp:3741-3804) of a case created with compute_average:
    avg [nElem][nDOF][12] dimensional (rho, u, v, w, p, uu, vv, ww, uv, uw, vw,
-   eddy viscosity = 0), the averaged columns of the reference's restart file;
+   eddy viscosity; 0 without an SGS model), the averaged columns of the reference's restart file;
    *n_steps the number of synchronization steps averaged. Set: to continue
    averaging from a restart ("Continue averaging from restart",
    SolverClass.cpp:3227-3240). */

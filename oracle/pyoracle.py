@@ -41,6 +41,7 @@ class OrcParams(C.Structure):
         ("damping_constant", C.c_double * 6), ("damping_exponent", C.c_double * 6),
         ("monitor_forces", C.c_int), ("reference_area", C.c_double),
         ("compute_average", C.c_int), ("free_stream_dir", C.c_double * 3),
+        ("sgs_model", C.c_int),
     ]
 
 
@@ -105,6 +106,8 @@ def lib():
         L.orc_set_damping_state.argtypes = [C.c_void_p, dp]
         L.orc_nscbc_average.argtypes = [C.c_void_p, dp]
         L.orc_get_averages.argtypes = [C.c_void_p, dp, C.POINTER(C.c_int)]
+        L.orc_eddy_viscosity.restype = C.c_double
+        L.orc_eddy_viscosity.argtypes = [C.c_int, C.c_double, C.c_double, dp]
         _lib = L
     return _lib
 
@@ -232,6 +235,13 @@ class Oracle:
         out = np.zeros(shape)
         lib().orc_get_table(self.h, which, out)
         return out
+
+
+def eddy_viscosity(model: int, length: float, rho: float, grad) -> float:
+    """Eddy viscosity of SGS model `model` (1 WALE, 2 Vreman) for the velocity
+    gradient grad[i][j] = du_i/dx_j."""
+    g = np.ascontiguousarray(np.asarray(grad, dtype=np.float64).T.reshape(9))
+    return lib().orc_eddy_viscosity(model, length, rho, g)
 
 
 def tensor_apply(ox, oy, oz, tin):

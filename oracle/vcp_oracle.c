@@ -16,6 +16,10 @@
  *                  preservation, conservation, TGV symmetry (tests).
  *   CG          -- numpy.linalg.solve of the assembled mass matrix (tests).
  *   RK step     -- reference golden trajectories.
+ *   SGS models  -- closed forms (solid-body rotation, axisymmetric strain),
+ *                  zero in pure shear / isotropic expansion, frame invariance
+ *                  and homogeneity (tests); reference golden vectors of WALE
+ *                  and Vreman cases, including the monitored maximum.
  */
 #define _USE_MATH_DEFINES
 #include "vcp_oracle.h"
@@ -29,6 +33,8 @@
 #define NVAR 5
 static const double GAM = 1.4;
 static const double PRANDTL_LAM = 0.72;              /* VCP3D_CurviLinear.hpp:175 */
+static const double PRANDTL_TURB = 0.90;             /* VCP3D_CurviLinear.hpp:176 */
+static const double EPS_SMALL = 1.e-30;              /* VCP3D_CurviLinear.hpp:140 */
 static const double RGAS = 287.058;                  /* VCP3D_CurviLinear.hpp:177 */
 static const double LAMBDA_OVER_MU = -2.0 / 3.0;     /* VCP3D_CurviLinear.hpp:186 */
 
@@ -46,6 +52,7 @@ struct orc_case {
   double *A, *D;          /* M x K : phi_dof(r_int), phi'_dof(r_int)  */
   double *AT, *DT;        /* K x M                                  */
   double *LD;             /* K x K : phi_mode(r_nodal)  (modal->nodal) */
+  double *DLD;            /* K x K : phi'_mode(r_nodal) (modal->nodal derivative) */
   double *VI;             /* K x K : inverse of LD       (nodal->modal) */
   double *FMin, *FMax, *dFMin, *dFMax; /* 1 x K face vectors (also K x 1) */
   double *w3, *w2, *basisCenter;
@@ -70,6 +77,9 @@ struct orc_case {
   double *vm;             /* [nElem][13][nInt] */
   double *sm[6];          /* [nElem][16][nInt2D] iMin,iMax,jMin,jMax,kMin,kMax */
   double *len;            /* [nElem][4]: I, J, K, volume */
+  double *lenLES;         /* [nElem]: volume^(1/3) / P, the LES length scale */
+  double *vmN;            /* [nElem][10][nDOFs]: J, dr/dx .. dt/dz at the nodal
+                             DOFs (SGS model only) */
   double *aveNorm;        /* [nElem][3][3] */
   int *nbr;               /* [nElem][6] */
   /* state */
@@ -78,6 +88,7 @@ struct orc_case {
   int resValid, timeStep;
   int monNow;             /* the residual being evaluated computes monitoring data */
   double *forceElem;      /* [nElem][6] wall force sums of the element's faces */
+  double *eddyElem;       /* [nElem] eddy-viscosity maximum over the element's faces */
   double *avg;            /* [nElem][nDOFs][12] running averages (compute_average) */
   int refFallback;        /* pRef, rhoRef from the free-stream data of the parameter file */
   int nAvg;
@@ -273,9 +284,12 @@ static void standard_element(orc_case *c)
       c->A[p * K + q] = v; c->D[p * K + q] = d;
       c->AT[q * M + p] = v; c->DT[q * M + p] = d;
     }
-  c->LD = dalloc(K * K); c->VI = dalloc(K * K);
+  c->LD = dalloc(K * K); c->VI = dalloc(K * K); c->DLD = dalloc(K * K);
   for (int p = 0; p < K; ++p)
-    for (int q = 0; q < K; ++q) { double v, d; legendre(q, c->rDOF[p], &v, &d); c->LD[p * K + q] = v; c->VI[p * K + q] = v; }
+    for (int q = 0; q < K; ++q) {
+      double v, d; legendre(q, c->rDOF[p], &v, &d);
+      c->LD[p * K + q] = v; c->VI[p * K + q] = v; c->DLD[p * K + q] = d;
+    }
   invert(K, c->VI);
   c->FMin = dalloc(K); c->FMax = dalloc(K); c->dFMin = dalloc(K); c->dFMax = dalloc(K);
   for (int q = 0; q < K; ++q) {
@@ -419,7 +433,15 @@ static int metrics(orc_case *c, const double *grid)
   c->vm = dalloc((size_t) c->nElem * 13 * nInt);
   for (int f = 0; f < 6; ++f) c->sm[f] = dalloc((size_t) c->nElem * 16 * n2);
   c->len = dalloc((size_t) c->nElem * 4);
+  c->lenLES = dalloc((size_t) c->nElem);
   c->aveNorm = dalloc((size_t) c->nElem * 9);
+  /* metric at the nodal solution DOFs for the SGS model's velocity gradients
+     there (ElementClass::ComputeCoorNodalSolDOFs, ElementClass.cpp:820-918) */
+  const int K = c->K, nD = c->nDOFs, sgs = c->prm.sgs_model != ORC_SGS_NONE;
+  double *LgN = dalloc(K * Kg), *dLgN = dalloc(K * Kg), *dN[3][3];
+  for (int p = 0; p < K; ++p) lagrange(Kg, c->rGrid, c->rDOF[p], LgN + p * Kg, dLgN + p * Kg);
+  for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) dN[a][b] = dalloc(nD);
+  c->vmN = dalloc(sgs ? (size_t) c->nElem * 10 * nD : 1);
   double *xyz[4][3];
   for (int a = 0; a < 4; ++a) for (int b = 0; b < 3; ++b) xyz[a][b] = dalloc(nInt);
   double *w = W(c);
@@ -487,11 +509,40 @@ static int metrics(orc_case *c, const double *grid)
         for (int b = 0; b < 3; ++b) sm[(13 + b) * n2 + l] = C[0][b][l];
       }
     }
+    if (sgs) {
+      double *vn = c->vmN + (size_t) e * 10 * nD;
+      for (int b = 0; b < 3; ++b) {
+        tp3(Kg, Kg, Kg, K, K, K, dLgN, LgN, LgN, g + b * nG, dN[0][b], 0, w);
+        tp3(Kg, Kg, Kg, K, K, K, LgN, dLgN, LgN, g + b * nG, dN[1][b], 0, w);
+        tp3(Kg, Kg, Kg, K, K, K, LgN, LgN, dLgN, g + b * nG, dN[2][b], 0, w);
+      }
+      for (int l = 0; l < nD; ++l) {
+        const double dxdr = dN[0][0][l], dydr = dN[0][1][l], dzdr = dN[0][2][l];
+        const double dxds = dN[1][0][l], dyds = dN[1][1][l], dzds = dN[1][2][l];
+        const double dxdt = dN[2][0][l], dydt = dN[2][1][l], dzdt = dN[2][2][l];
+        const double J = dxdr * (dyds * dzdt - dzds * dydt) - dxds * (dydr * dzdt - dzdr * dydt)
+                       + dxdt * (dydr * dzds - dzdr * dyds);
+        if (J <= 0.0) return fail("Negative or zero Jacobians encountered");
+        const double Ji = 1.0 / J;
+        vn[0 * nD + l] = J;
+        vn[1 * nD + l] = Ji * (dyds * dzdt - dzds * dydt);
+        vn[2 * nD + l] = Ji * (dzds * dxdt - dxds * dzdt);
+        vn[3 * nD + l] = Ji * (dxds * dydt - dyds * dxdt);
+        vn[4 * nD + l] = Ji * (dzdr * dydt - dydr * dzdt);
+        vn[5 * nD + l] = Ji * (dxdr * dzdt - dzdr * dxdt);
+        vn[6 * nD + l] = Ji * (dydr * dxdt - dxdr * dydt);
+        vn[7 * nD + l] = Ji * (dydr * dzds - dzdr * dyds);
+        vn[8 * nD + l] = Ji * (dzdr * dxds - dxdr * dzds);
+        vn[9 * nD + l] = Ji * (dxdr * dyds - dydr * dxds);
+      }
+    }
     /* length scales and averaged normals */
     double vol = 0.0;
     for (int l = 0; l < nInt; ++l) vol += c->w3[l] * vm[l];
     double *ln = c->len + (size_t) e * 4;
     ln[3] = vol;
+    /* ElementClass::ComputeLengthScales (ElementClass.cpp:937-940) */
+    c->lenLES[e] = pow(vol, 1.0 / 3.0) / c->P;
     for (int dir = 0; dir < 3; ++dir) {
       const double *s0 = c->sm[2 * dir] + (size_t) e * 16 * n2, *s1 = c->sm[2 * dir + 1] + (size_t) e * 16 * n2;
       double an[3] = {0, 0, 0}, area = 0.0;
@@ -505,7 +556,8 @@ static int metrics(orc_case *c, const double *grid)
     }
   }
   for (int a = 0; a < 4; ++a) for (int b = 0; b < 3; ++b) free(xyz[a][b]);
-  free(Lg); free(dLg);
+  for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) free(dN[a][b]);
+  free(Lg); free(dLg); free(LgN); free(dLgN);
   return 0;
 }
 
@@ -554,6 +606,55 @@ static void ent2prim(const double *V, double *rho, double *u, double *v, double 
    conservative variables and weighted with the normalised damping
    functions. V: entropy variables, gx/gy/gz: their Cartesian gradients,
    wgt[3]: -J w sigma_d / sigma_max, nsg[3]: layer orientation signs. */
+/* Eddy viscosity mu_SGS = rho nu_t of the subgrid scale model at one point:
+   g = (du/dx, dv/dx, dw/dx, du/dy, dv/dy, dw/dy, du/dz, dv/dz, dw/dz), i.e.
+   G[i][j] = du_i/dx_j = g[3 j + i]; lenScale the element's LES length scale.
+   WALE (CWALEModel::EddyViscosity, SGSModel.cpp:34-113): S the symmetric
+   part of G, Sd the traceless symmetric part of G^2,
+   nu_t = (Cw len)^2 |Sd|^3 / max(|S|^5 + |Sd|^(5/2), eps), Cw = 0.325, with
+   |X|^2 = X_ij X_ij. Vreman (CVremanModel::EddyViscosity, SGSModel.cpp:
+   128-210): alpha = G with its trace removed from the diagonal, beta = len^2
+   alpha alpha^T, B_beta the sum of beta's principal 2 x 2 minors (clipped at
+   0), nu_t = c sqrt(B_beta / max(|alpha|^2, eps)), c = 0.07. */
+static double eddy_viscosity(int model, double lenScale, double rho, const double *g)
+{
+  double G[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) G[i][j] = g[3 * j + i];
+  if (model == ORC_SGS_WALE) {
+    const double len = 0.325 * lenScale;
+    double G2[3][3], tr = 0.0;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) G2[i][j] = G[i][0] * G[0][j] + G[i][1] * G[1][j] + G[i][2] * G[2][j];
+    for (int i = 0; i < 3; ++i) tr += G2[i][i];
+    double s2 = 0.0, sd2 = 0.0;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        const double S = 0.5 * (G[i][j] + G[j][i]);
+        const double Sd = 0.5 * (G2[i][j] + G2[j][i]) - (i == j ? (1.0 / 3.0) * tr : 0.0);
+        s2 += S * S; sd2 += Sd * Sd;
+      }
+    const double sd32 = sd2 * sqrt(sd2);
+    const double den = s2 * s2 * sqrt(s2) + sqrt(sd32 * sd2);
+    return rho * len * len * sd32 / fmax(den, EPS_SMALL);
+  }
+  if (model == ORC_SGS_VREMAN) {
+    const double tr = (1.0 / 3.0) * (G[0][0] + G[1][1] + G[2][2]);
+    double al[3][3], a2 = 0.0;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) { al[i][j] = G[i][j] - (i == j ? tr : 0.0); a2 += al[i][j] * al[i][j]; }
+    const double l2 = lenScale * lenScale;
+    double be[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) be[i][j] = l2 * (al[i][0] * al[j][0] + al[i][1] * al[j][1] + al[i][2] * al[j][2]);
+    double B = be[0][0] * be[1][1] - be[0][1] * be[0][1] + be[0][0] * be[2][2] - be[0][2] * be[0][2]
+             + be[1][1] * be[2][2] - be[1][2] * be[1][2];
+    if (B < 0.0) B = 0.0;
+    return rho * 0.07 * sqrt(B / fmax(a2, EPS_SMALL));
+  }
+  return 0.0;
+}
+
 static void cml_source(const double *V, const double *gx, const double *gy, const double *gz,
                        const double wgt[3], const double nsg[3], double *out)
 {
@@ -608,9 +709,9 @@ static void cml_source(const double *V, const double *gx, const double *gy, cons
   }
 }
 
-/* ElementClass::VolumeResidual, steps 1-3, 5-8 (ElementClass.cpp:3809-4489)
-   for entropy variables without SGS model or body force. */
-static void volume_residual(orc_case *c, int e, int mon, double *mach2max)
+/* ElementClass::VolumeResidual, steps 1-8 (ElementClass.cpp:3809-4489) for
+   entropy variables, with or without an SGS model, without body force. */
+static void volume_residual(orc_case *c, int e, int mon, double *mach2max, double *eddymax)
 {
   const int K = c->K, M = c->M, nInt = c->nInt, nD = c->nDOFs;
   const double *sol = c->sol + (size_t) e * NVAR * nD;
@@ -695,13 +796,23 @@ static void volume_residual(orc_case *c, int e, int mon, double *mach2max)
       Ssp[3 * nInt + m] = sigma * (rho * ww - dst[3 * nInt + m]);
       Ssp[4 * nInt + m] = sigma * (rE - dst[4 * nInt + m]);
     }
+    /* eddy viscosity (ElementClass.cpp:4268-4287) with its maximum for the
+       monitoring data (:4305-4306); total viscosity and heat-flux factor
+       (:4339-4342) */
+    double muT = c->mu, fhT = factHF;
+    if (c->prm.sgs_model != ORC_SGS_NONE) {
+      const double g9[9] = {dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz, dwdz};
+      const double ev = eddy_viscosity(c->prm.sgs_model, c->lenLES[e], rho, g9);
+      if (mon && ev > *eddymax) *eddymax = ev;
+      muT = c->mu + ev;
+      fhT = c->mu * (GAM / PRANDTL_LAM) + ev * (GAM / PRANDTL_TURB);
+    }
     /* fluxes (ElementClass.cpp:4314-4416) */
     const double weight = -vm[m] * c->w3[m];
-    const double muT = c->mu;
     const double divV = LAMBDA_OVER_MU * (dudx + dvdy + dwdz);
     const double txx = muT * (2.0 * dudx + divV), tyy = muT * (2.0 * dvdy + divV), tzz = muT * (2.0 * dwdz + divV);
     const double txy = muT * (dudy + dvdx), txz = muT * (dudz + dwdx), tyz = muT * (dvdz + dwdy);
-    const double qx = -factHF * dedx, qy = -factHF * dedy, qz = -factHF * dedz;
+    const double qx = -fhT * dedx, qy = -fhT * dedy, qz = -fhT * dedz;
     const double fx[5] = {ru, ru * u - txx + p, rv * u - txy, rw * u - txz, (rE + p) * u - (u * txx + v * txy + ww * txz - qx)};
     const double fy[5] = {rv, ru * v - txy, rv * v - tyy + p, rw * v - tyz, (rE + p) * v - (u * txy + v * tyy + ww * tyz - qy)};
     const double fz[5] = {rw, ru * ww - txz, rv * ww - tyz, rw * ww - tzz + p, (rE + p) * ww - (u * txz + v * tyz + ww * tzz - qz)};
@@ -849,10 +960,10 @@ static void ismail_roe(const double *VL, const double *VR, const double n[3], do
    heat-flux factor is 0 in the viscous, penalty and symmetrising fluxes and
    w A qWall is added to the energy flux (FluxesFace.cpp:1312-1318,
    :1586-1593). */
-static void internal_face_flux(const orc_case *c, double fn, double lenScale,
+static void internal_face_flux(const orc_case *c, double fn, double lenScale, double lenLES,
                                const double *SL, double *GrL, double *GsL, double *GtL, const double *mL,
                                const double *SR, double *GrR, double *GsR, double *GtR, const double *mR,
-                               double factU, int hf, double qWall, double *force,
+                               double factU, int hf, double qWall, double *force, double *eddymax,
                                double *flux, double *sxL, double *syL, double *szL,
                                double *sxR, double *syR, double *szR)
 {
@@ -873,7 +984,8 @@ static void internal_face_flux(const orc_case *c, double fn, double lenScale,
       gx[i] = 0.5 * (xL + xR); gy[i] = 0.5 * (yL + yR); gz[i] = 0.5 * (zL + zR);
     }
   const double gm1 = GAM - 1.0, ovgm1 = 1.0 / gm1;
-  const double mu = c->mu, factHF = hf ? 0.0 : GAM / PRANDTL_LAM;
+  const double mu = c->mu, factHF = hf ? 0.0 : GAM / PRANDTL_LAM, factHFT = hf ? 0.0 : GAM / PRANDTL_TURB;
+  const int sgs = c->prm.sgs_model;
   const double tau = (c->P + 1.0) * (c->P + 1.0) * c->prm.rel_penalty / lenScale;
   const double lam = LAMBDA_OVER_MU, lamp1 = lam + 1.0, lamp2 = lam + 2.0;
   for (int m = 0; m < n2; ++m) {
@@ -894,11 +1006,26 @@ static void internal_face_flux(const orc_case *c, double fn, double lenScale,
     const double dudz = -V4Inv * (dz[1] + u * dz[4]), dvdz = -V4Inv * (dz[2] + v * dz[4]), dwdz = -V4Inv * (dz[3] + w * dz[4]);
     const double fe = V4Inv * V4Inv * ovgm1;
     const double dedx = fe * dx[4], dedy = fe * dy[4], dedz = fe * dz[4];
+    /* eddy viscosity on the density of the averaged entropy variables
+       (FluxesFace.cpp:1409-1459, :1481-1494); total viscosity and heat-flux
+       factor (:1537-1554); factHF / factHFT are 0 on a prescribed heat flux
+       (:1312-1318) */
+    double muT = mu, fh = mu * factHF, fhRatio = factHF;
+    if (sgs != ORC_SGS_NONE) {
+      double rA, uA_, vA_, wA_, pA_, eA_;
+      ent2prim(Va, &rA, &uA_, &vA_, &wA_, &pA_, &eA_);
+      const double g9[9] = {dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz, dwdz};
+      const double ev = eddy_viscosity(sgs, lenLES, rA, g9);
+      muT = mu + ev;
+      fh = mu * factHF + ev * factHFT;
+      fhRatio = fh / muT;
+      /* face maximum for the monitoring data (FluxesFace.cpp:207-213, :336-342) */
+      if (eddymax && ev > *eddymax) *eddymax = ev;
+    }
     const double nx = wA * nn[0], ny = wA * nn[1], nz = wA * nn[2];
     const double lamDiv = LAMBDA_OVER_MU * (dudx + dvdy + dwdz);
-    const double txx = mu * (2.0 * dudx + lamDiv), tyy = mu * (2.0 * dvdy + lamDiv), tzz = mu * (2.0 * dwdz + lamDiv);
-    const double txy = mu * (dudy + dvdx), txz = mu * (dudz + dwdx), tyz = mu * (dvdz + dwdy);
-    const double fh = mu * factHF;
+    const double txx = muT * (2.0 * dudx + lamDiv), tyy = muT * (2.0 * dvdy + lamDiv), tzz = muT * (2.0 * dwdz + lamDiv);
+    const double txy = muT * (dudy + dvdx), txz = muT * (dudz + dwdx), tyz = muT * (dvdz + dwdy);
     const double qx = -fh * dedx, qy = -fh * dedy, qz = -fh * dedz;
     fl[1] -= txx * nx + txy * ny + txz * nz;
     fl[2] -= txy * nx + tyy * ny + tyz * nz;
@@ -924,9 +1051,9 @@ static void internal_face_flux(const orc_case *c, double fn, double lenScale,
     /* penalty and symmetrising fluxes (FluxesFace.cpp:1152-1284) */
     const double pOvRho = -V4Inv, e = pOvRho * ovgm1;
     const double dV1 = VL[1] - VR[1], dV2 = VL[2] - VR[2], dV3 = VL[3] - VR[3], dV4 = VL[4] - VR[4];
-    const double factFlux = pOvRho * c->w2[m] * mL[3 * n2 + m] * mu;
+    const double factFlux = pOvRho * c->w2[m] * mL[3 * n2 + m] * muT;
     double sx_ = factFlux * nn[0], sy_ = factFlux * nn[1], sz_ = factFlux * nn[2];
-    const double abvE = u * u + v * v + w * w + factHF * e;
+    const double abvE = u * u + v * v + w * w + fhRatio * e;
     const double K11_11 = lamp2, K11_14 = u * lamp2, K11_22 = 1.0, K11_24 = v, K11_33 = 1.0, K11_34 = w, K11_44 = abvE + u * u * lamp1;
     const double K22_11 = 1.0, K22_14 = u, K22_22 = lamp2, K22_24 = v * lamp2, K22_33 = 1.0, K22_34 = w, K22_44 = abvE + v * v * lamp1;
     const double K33_11 = 1.0, K33_14 = u, K33_22 = 1.0, K33_24 = v, K33_33 = lamp2, K33_34 = w * lamp2, K33_44 = abvE + w * w * lamp1;
@@ -1314,18 +1441,19 @@ static int boundary_face(orc_case *c, int f, int e, double *buf)
   face_trace(c, f, c->sol + (size_t) e * NVAR * nD, SL, G0, G1, G2);
   const double *m = c->sm[f] + (size_t) e * 16 * n2;
   const double fn = (f % 2) ? 1.0 : -1.0;
-  const double lenScale = c->len[(size_t) e * 4 + f / 2];
+  const double lenScale = c->len[(size_t) e * 4 + f / 2], lenLES = c->lenLES[e];
   if (is_nscbc(bc)) {
     double *GL[3] = {G0, G1, G2}, *GR[3] = {H0, H1, H2};
     nscbc_state(c, f, e, SL, GL, SR, GR);
-    internal_face_flux(c, fn, lenScale, SL, G0, G1, G2, m, SR, H0, H1, H2, m, 1.0, 0, 0.0, NULL,
-                       flux, sxL, syL, szL, sxR, syR, szR);
+    internal_face_flux(c, fn, lenScale, lenLES, SL, G0, G1, G2, m, SR, H0, H1, H2, m, 1.0, 0, 0.0, NULL,
+                       c->monNow ? c->eddyElem + e : NULL, flux, sxL, syL, szL, sxR, syR, szR);
   } else {
     pointwise_state(c, bc, m, fn, c->bcData[f], SL, SR);
     /* forces of the walls when monitoring (SolverClass.cpp:3386, FluxesFace.cpp:98-101) */
     double *force = (c->monNow && c->prm.monitor_forces && is_wall(bc)) ? c->forceElem + (size_t) e * 6 : NULL;
-    internal_face_flux(c, fn, lenScale, SL, G0, G1, G2, m, SR, G0, G1, G2, m, is_wall(bc) ? 0.0 : 1.0,
-                       bc == ORC_BC_WALL_HEATFLUX, c->bcData[f][0], force, flux, sxL, syL, szL, sxR, syR, szR);
+    internal_face_flux(c, fn, lenScale, lenLES, SL, G0, G1, G2, m, SR, G0, G1, G2, m, is_wall(bc) ? 0.0 : 1.0,
+                       bc == ORC_BC_WALL_HEATFLUX, c->bcData[f][0], force, c->monNow ? c->eddyElem + e : NULL,
+                       flux, sxL, syL, szL, sxR, syR, szR);
   }
   face_lift(c, f, flux, sxL, syL, szL, c->res + (size_t) e * NVAR * nD);
   return 0;
@@ -1352,10 +1480,11 @@ static int face_residual(orc_case *c, int dir, int e)
   face_trace(c, 2 * dir, c->sol + (size_t) e * NVAR * nD, SR, GrR, GsR, GtR);
   face_trace(c, 2 * dir + 1, c->sol + (size_t) L * NVAR * nD, SL, GrL, GsL, GtL);
   const double lenScale = fmin(c->len[(size_t) e * 4 + dir], c->len[(size_t) L * 4 + dir]);
+  const double lenLES = 0.5 * (c->lenLES[e] + c->lenLES[L]);
   const double *mL = c->sm[2 * dir + 1] + (size_t) L * 16 * n2;
   const double *mR = c->sm[2 * dir] + (size_t) e * 16 * n2;
-  internal_face_flux(c, 1.0, lenScale, SL, GrL, GsL, GtL, mL, SR, GrR, GsR, GtR, mR, 1.0, 0, 0.0, NULL,
-                     flux, sxL, syL, szL, sxR, syR, szR);
+  internal_face_flux(c, 1.0, lenScale, lenLES, SL, GrL, GsL, GtL, mL, SR, GrR, GsR, GtR, mR, 1.0, 0, 0.0, NULL,
+                     c->monNow ? c->eddyElem + e : NULL, flux, sxL, syL, szL, sxR, syR, szR);
   double *rmin = c->resMin[dir] + (size_t) e * NVAR * nD;
   for (int i = 0; i < NVAR * nD; ++i) rmin[i] = 0.0;
   face_lift(c, 2 * dir + 1, flux, sxL, syL, szL, rmin);
@@ -1737,6 +1866,7 @@ orc_case *orc_create(const orc_params *prm, const double *elem_grid)
   c->sol = dalloc(ns); c->solOld = dalloc(ns); c->res = dalloc(ns);
   for (int d = 0; d < 3; ++d) c->resMin[d] = dalloc(ns);
   c->forceElem = dalloc((size_t) c->nElem * 6);
+  c->eddyElem = dalloc((size_t) c->nElem);
   c->avg = dalloc(c->prm.compute_average ? (size_t) c->nElem * c->nDOFs * 12 : 1);
   c->dUdV = dalloc((size_t) c->nElem * 15 * c->nInt);
   c->cgIt = (int *) calloc(c->nElem, sizeof(int));
@@ -1750,7 +1880,8 @@ void orc_destroy(orc_case *c)
                     c->FMin, c->FMax, c->dFMin, c->dFMax, c->w3, c->w2, c->basisCenter, c->vm,
                     c->sm[0], c->sm[1], c->sm[2], c->sm[3], c->sm[4], c->sm[5], c->len, c->aveNorm,
                     c->sol, c->solOld, c->res, c->resMin[0], c->resMin[1], c->resMin[2], c->dUdV,
-                    c->Lq, c->dLq, c->PC, c->dampFn, c->dampState, c->matchScale, c->forceElem, c->avg};
+                    c->Lq, c->dLq, c->PC, c->dampFn, c->dampState, c->matchScale, c->forceElem, c->avg,
+                    c->DLD, c->lenLES, c->vmN, c->eddyElem};
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
   if (c->works) for (int t = 0; t < c->nthreads; ++t) free(c->works[t]);
   free(c->works);
@@ -1834,7 +1965,7 @@ void orc_get_nodal(const orc_case *c, double *nodal)
 /* SolverClass::Residual (SolverClass.cpp:3381-3795). */
 int orc_residual(orc_case *c, int mon, double md[8])
 {
-  double mach2 = 0.0;
+  double mach2 = 0.0, eddy = 0.0;
   int err = 0;
   if (c->prm.sponge_layer && !c->dampStateSet) return fail("sponge layer damping state not set");
   /* the reference calls AverageDataNSCBC before every Residual
@@ -1842,11 +1973,13 @@ int orc_residual(orc_case *c, int mon, double md[8])
   if (c->nscbc) nscbc_average(c);
   c->monNow = mon;
   if (mon) for (size_t i = 0; i < (size_t) c->nElem * 6; ++i) c->forceElem[i] = 0.0;
-#pragma omp parallel for schedule(static) reduction(max : mach2)
+  if (mon) for (int e = 0; e < c->nElem; ++e) c->eddyElem[e] = 0.0;
+#pragma omp parallel for schedule(static) reduction(max : mach2, eddy)
   for (int e = 0; e < c->nElem; ++e) {
-    double m2 = 0.0;
-    volume_residual(c, e, mon, &m2);
+    double m2 = 0.0, ev = 0.0;
+    volume_residual(c, e, mon, &m2, &ev);
     mach2 = fmax(mach2, m2);
+    eddy = fmax(eddy, ev);
   }
   for (int d = 0; d < 3; ++d) {
 #pragma omp parallel for schedule(static) reduction(| : err)
@@ -1878,9 +2011,49 @@ int orc_residual(orc_case *c, int mon, double md[8])
     const double fc = 2.0 / (GAM * c->prm.mach * c->prm.mach * area);
     for (int i = 0; i < 6; ++i) md[i] *= fc;
   }
-  md[6] = mach2; md[7] = mon ? 0.0 : -1.0;
+  if (mon) for (int e = 0; e < c->nElem; ++e) eddy = fmax(eddy, c->eddyElem[e]);
+  md[6] = mach2; md[7] = mon ? eddy : -1.0;
   c->resValid = 1;
   return 0;
+}
+
+/* Dimensional eddy viscosity ev[nDOFs] at the nodal DOFs of element e
+   (ElementClass.cpp:1562-1573, :3785-3797): the dimensional nodal primitive
+   variables (ComputePrimitiveVariablesNodalDOFs, :1500-1517), the
+   dimensional velocity gradients of the entropy variables with the nodal
+   metric (ComputeVelocityGradientsNodalDOFs, :1649-1790) and the model with
+   the element's LES length scale. node: nodal entropy variables [5][nDOFs];
+   gb: scratch of 12 nDOFs. */
+static void nodal_eddy(const orc_case *c, int e, const double *node, double *gb, double *ev)
+{
+  const int K = c->K, nD = c->nDOFs;
+  const double *sol = c->sol + (size_t) e * NVAR * nD, *vn = c->vmN + (size_t) e * 10 * nD;
+  double *dr = gb, *ds = gb + 4 * nD, *dt = gb + 8 * nD;
+  for (int l = 1; l < 5; ++l) {
+    tp3(K, K, K, K, K, K, c->DLD, c->LD, c->LD, sol + l * nD, dr + (l - 1) * nD, 0, W(c));
+    tp3(K, K, K, K, K, K, c->LD, c->DLD, c->LD, sol + l * nD, ds + (l - 1) * nD, 0, W(c));
+    tp3(K, K, K, K, K, K, c->LD, c->LD, c->DLD, sol + l * nD, dt + (l - 1) * nD, 0, W(c));
+  }
+  for (int d = 0; d < nD; ++d) {
+    double V[5], rho, u, v, w, p, eKin;
+    for (int l = 0; l < 5; ++l) V[l] = node[l * nD + d];
+    ent2prim(V, &rho, &u, &v, &w, &p, &eKin);
+    rho *= c->rhoRef; p *= c->pRef;
+    const double V4Inv = -p / (c->uRef * c->uRef) / rho;
+    const double vel[3] = {u, v, w};
+    double gp[3][3];   /* [velocity][r|s|t] */
+    for (int i = 0; i < 3; ++i) {
+      gp[i][0] = -V4Inv * (dr[i * nD + d] + vel[i] * dr[3 * nD + d]);
+      gp[i][1] = -V4Inv * (ds[i * nD + d] + vel[i] * ds[3 * nD + d]);
+      gp[i][2] = -V4Inv * (dt[i * nD + d] + vel[i] * dt[3 * nD + d]);
+    }
+    double g9[9];
+    for (int x = 0; x < 3; ++x)
+      for (int i = 0; i < 3; ++i)
+        g9[3 * x + i] = c->uRef * (gp[i][0] * vn[(1 + x) * nD + d] + gp[i][1] * vn[(4 + x) * nD + d]
+                                   + gp[i][2] * vn[(7 + x) * nD + d]);
+    ev[d] = eddy_viscosity(c->prm.sgs_model, c->lenLES[e], rho, g9);
+  }
 }
 
 /* ElementClass::ComputeTimeStep (ElementClass.cpp:1522-1644). */
@@ -1891,18 +2064,21 @@ int orc_time_steps(orc_case *c, double *dt)
   const int K = c->K, nD = c->nDOFs;
   const double muDim = c->mu * c->pRef / c->uRef;
   const double rvl = fmax(fmax(1.0, 2.0 + LAMBDA_OVER_MU), GAM / PRANDTL_LAM);
+  const double rvt = fmax(fmax(1.0, 2.0 + LAMBDA_OVER_MU), GAM / PRANDTL_TURB);
+  const int sgs = c->prm.sgs_model != ORC_SGS_NONE;
   int bad = 0;
 #pragma omp parallel reduction(| : bad)
   {
-    double *node = dalloc(5 * (size_t) nD);
+    double *node = dalloc(5 * (size_t) nD), *gb = dalloc(12 * (size_t) nD), *ev = dalloc(nD);
 #pragma omp for schedule(static)
     for (int e = 0; e < c->nElem; ++e) {
       const double *ln = c->len + (size_t) e * 4, *an = c->aveNorm + (size_t) e * 9;
       const double di = 1.0 / ln[0], dj = 1.0 / ln[1], dk = 1.0 / ln[2];
       const double dl2 = di * di + dj * dj + dk * dk;
-      const double radVisOverNu = rvl * dl2;
+      const double radVisOverNu = rvl * dl2, radVisOverNuT = rvt * dl2;
       for (int l = 0; l < 5; ++l)
         tp3(K, K, K, K, K, K, c->LD, c->LD, c->LD, c->sol + ((size_t) e * 5 + l) * nD, node + l * nD, 0, W(c));
+      if (sgs) nodal_eddy(c, e, node, gb, ev);
       double radInvMax = 0.0, radVisMax = 0.0, a2Min = 1e10;
       for (int d = 0; d < nD; ++d) {
         double V[5], rho, u, v, w, p, eKin;
@@ -1915,14 +2091,14 @@ int orc_time_steps(orc_case *c, double *dt)
         const double vK = u * an[6] + v * an[7] + w * an[8];
         const double a = sqrt(fabs(a2));
         const double ri_ = (fabs(vI) + a) * di + (fabs(vJ) + a) * dj + (fabs(vK) + a) * dk;
-        const double rv_ = ri * (muDim * radVisOverNu);
+        const double rv_ = sgs ? ri * (muDim * radVisOverNu + ev[d] * radVisOverNuT) : ri * (muDim * radVisOverNu);
         radInvMax = fmax(radInvMax, ri_); radVisMax = fmax(radVisMax, rv_); a2Min = fmin(a2Min, a2);
       }
       if (a2Min < 0.0) bad = 1;
       const double dtInv = f1[c->P] * radInvMax + f2[c->P] * radVisMax;
       dt[e] = c->prm.cfl * c->uRef / dtInv;
     }
-    free(node);
+    free(node); free(gb); free(ev);
   }
   if (bad) return fail("Negative square speed of sound");
   return 0;
@@ -1960,11 +2136,13 @@ int orc_sync_step(orc_case *c, double md[8])
        (ComputePrimitiveVariablesNodalDOFs, :1500-1517, dimensional) */
     ++c->nAvg;
     const double factNew = 1.0 / c->nAvg, factOld = 1.0 - factNew;
-    const int K = c->K, nD = c->nDOFs;
-    double *node = dalloc(5 * (size_t) nD);
+    const int K = c->K, nD = c->nDOFs, sgs = c->prm.sgs_model != ORC_SGS_NONE;
+    double *node = dalloc(5 * (size_t) nD), *gb = dalloc(12 * (size_t) nD), *ev = dalloc(nD);
     for (int e = 0; e < c->nElem; ++e) {
       for (int l = 0; l < 5; ++l)
         tp3(K, K, K, K, K, K, c->LD, c->LD, c->LD, c->sol + ((size_t) e * 5 + l) * nD, node + l * nD, 0, W(c));
+      /* averaged eddy viscosity (ElementClass.cpp:3785-3803) */
+      if (sgs) nodal_eddy(c, e, node, gb, ev);
       for (int d = 0; d < nD; ++d) {
         double V[5], rho, u, v, w, p, eKin;
         for (int l = 0; l < 5; ++l) V[l] = node[l * nD + d];
@@ -1973,9 +2151,10 @@ int orc_sync_step(orc_case *c, double md[8])
         const double q[11] = {rho, u, v, w, p, u * u, v * v, w * w, u * v, u * w, v * w};
         double *a = c->avg + ((size_t) e * nD + d) * 12;
         for (int i = 0; i < 11; ++i) a[i] = factOld * a[i] + factNew * q[i];
+        if (sgs) a[11] = factOld * a[11] + factNew * ev[d];
       }
     }
-    free(node);
+    free(node); free(gb); free(ev);
   }
   return nSteps;
 }
@@ -2015,7 +2194,15 @@ void orc_set_setup(orc_case *c, int which, const double *in)
   const size_t E = c->nElem;
   if (which == 0) memcpy(c->vm, in, sizeof(double) * E * 13 * c->nInt);
   else if (which >= 1 && which <= 6) memcpy(c->sm[which - 1], in, sizeof(double) * E * 16 * c->nInt2D);
-  else if (which == 7) memcpy(c->len, in, sizeof(double) * E * 4);
+  else if (which == 7) {
+    memcpy(c->len, in, sizeof(double) * E * 4);
+    for (size_t e = 0; e < E; ++e) c->lenLES[e] = pow(c->len[e * 4 + 3], 1.0 / 3.0) / c->P;
+  }
+}
+
+double orc_eddy_viscosity(int model, double len, double rho, const double g[9])
+{
+  return eddy_viscosity(model, len, rho, g);
 }
 
 void orc_get_table(const orc_case *c, int which, double *out)

@@ -6,8 +6,8 @@
  * cpu_baseline leg of bench.py as the checker. The product (vcp3d_open_amd)
  * never links, loads or calls it.
  *
- * Scope: entropy working variables, Ismail-Roe flux, no SGS model, no wall
- * model, structured single block whose faces are periodic (translation),
+ * Scope: entropy working variables, Ismail-Roe flux, no SGS model or the
+ * WALE / Vreman model, no wall model, structured single block whose faces are periodic (translation),
  * PC-NSCBC in/outflow, subsonic (total-condition) inflow, subsonic outflow,
  * or heat-flux / isothermal / inviscid walls, with optional sponge layer and
  * characteristic matching layer.
@@ -60,7 +60,10 @@ typedef struct {
   double reference_area;
   int    compute_average;  /* "Compute average solution" */
   double free_stream_dir[3]; /* "Free stream velocity direction" */
+  int    sgs_model;        /* "Subgrid scale model": ORC_SGS_* */
 } orc_params;
+
+enum { ORC_SGS_NONE = 0, ORC_SGS_WALE = 1, ORC_SGS_VREMAN = 2 };
 
 enum { ORC_BC_PERIODIC = 0, ORC_BC_OUTFLOW_NSCBC = 1,
        ORC_BC_INFLOW_STATIC_NSCBC = 2, ORC_BC_INFLOW_TOTAL_NSCBC = 3,
@@ -123,6 +126,12 @@ void orc_get_setup(const orc_case *c, int which, double *out);
    length scales (7) with given arrays of the orc_get_setup layouts, so that
    the residual can be evaluated on another set-up's geometry. */
 void orc_set_setup(orc_case *c, int which, const double *in);
+
+/* Eddy viscosity rho nu_t of subgrid scale model `model` (ORC_SGS_*) for the
+   LES length scale len, density rho and velocity gradients
+   g = (du/dx, dv/dx, dw/dx, du/dy, dv/dy, dw/dy, du/dz, dv/dz, dw/dz)
+   (SGSModel.cpp:34-210). */
+double orc_eddy_viscosity(int model, double len, double rho, const double g[9]);
 /* 1D tables: which 0 Gauss points, 1 weights, 2 legendre at int [K][M],
    3 derivative [K][M], 4 nodal DOF locations [K], 5 grid DOF locations,
    6 PC-NSCBC matrix [M^2][M^2], 7 Lagrange derivative on the Gauss points

@@ -108,7 +108,7 @@ def test_restart_round_trip(tmp_path):
     (("FEM working variables: Entropy", "FEM working variables: Conservative"), "fem working variables"),
     (("BCOutflowNSCBC  ", "BCSymmetryPlane"), "not supported"),
     (("BCOutflowNSCBC  ", "BCOutflowCharacteristic"), "unknown boundary condition"),
-    (("Subgrid scale model: None", "Subgrid scale model: WALE"), "Subgrid"),
+    (("Subgrid scale model: None", "Subgrid scale model: Smagorinsky"), "Subgrid"),
     ((" Fluctuations to add to initial solution: Pressure pulse",
       " Fluctuations to add to initial solution: Random fluctuations"), "random fluctuations"),
 ])

@@ -51,6 +51,11 @@ def reference_conditions(mach: float, reynolds: float, length: float, temp: floa
                 mu_nondim=MU_DIM * u_ref / p_ref)
 
 
+# "Subgrid scale model" keyword of the parameter file and vcp_case_desc code
+SGS_KEYWORD = {"none": "None", "wale": "WALE", "vreman": "Vreman"}
+SGS_CODE = {"none": 0, "wale": 1, "vreman": 2}
+
+
 def periodic_subfaces(n_elem, lengths):
     """Six all-periodic subface lines (translation = donor minus own face)."""
     ni, nj, nk = n_elem
@@ -93,6 +98,7 @@ class TGVCase:
     theta_sym: float = 1.0
     rel_penalty: float = 1.0
     name: str = "tgv"
+    sgs: str = "none"             # "Subgrid scale model": none, wale or vreman
 
     def __post_init__(self):
         if self.rule is None:
@@ -193,7 +199,7 @@ class TGVCase:
             "Write time history info every": 1,
             "FEM working variables": "Entropy",
             "Riemann solver": "IsmailRoe",
-            "Subgrid scale model": "None",
+            "Subgrid scale model": SGS_KEYWORD[self.sgs],
             "Wall modeling": "No wall model",
             "Theta parameter symmetrizing terms": self.theta_sym,
             "Relative penalty parameter": self.rel_penalty,
@@ -219,7 +225,8 @@ class TGVCase:
                     n_poly_int_rule=self.rule, dof_location=2, periodic=(1, 1, 1),
                     mach=self.mach, reynolds=self.reynolds, reynolds_length=1.0,
                     temp_free_stream=self.temp, theta_sym=self.theta_sym,
-                    rel_penalty=self.rel_penalty, cfl=self.cfl, dt_synchr=self.dt_synchr)
+                    rel_penalty=self.rel_penalty, cfl=self.cfl, dt_synchr=self.dt_synchr,
+                    sgs_model=SGS_CODE[self.sgs])
 
     def write_inputs(self, directory: str) -> str:
         """Write grid, restart and parameter file; returns the parameter path."""
@@ -291,6 +298,7 @@ class OpenBoxCase:
     reynolds: float = 1.0e6          # written to the parameter file only
     compute_average: bool = False    # running averages of the primitive variables
     name: str = "openbox"
+    sgs: str = "none"                # "Subgrid scale model": none, wale or vreman
 
     def __post_init__(self):
         if self.rule is None:
@@ -424,7 +432,8 @@ class OpenBoxCase:
                  outlet_beta_l=self.outlet_beta_l, outlet_beta_t=self.outlet_beta_t,
                  outlet_one_minus_beta=int(self.outlet_one_minus_beta), outlet_average_local=0,
                  inlet_len=self.inlet_len, inlet_sigma=self.inlet_sigma,
-                 inlet_incoming=int(self.inlet_incoming), compute_average=int(self.compute_average))
+                 inlet_incoming=int(self.inlet_incoming), compute_average=int(self.compute_average),
+                 sgs_model=SGS_CODE[self.sgs])
         if self.sponge_elems > 0:
             d.update(sponge_layer=1, cml=int(self.cml),
                      sponge_elems=[0, self.sponge_elems, 0, 0, 0, 0],

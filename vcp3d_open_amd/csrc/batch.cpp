@@ -23,7 +23,7 @@ using vcp::fail;
 
 struct vcp_batch {
   struct Group {
-    int K = 0, M = 0, bug = 0, mixed = 0, nt = 0;
+    int K = 0, M = 0, bug = 0, mixed = 0, sgs = 0, nt = 0;   // sgs: any SGS model (time-step kernel)
     std::vector<vcp_ctx *> cases;      // case table order
     std::vector<int> first;            // first work item of each case, + total
     vcp::CaseDev *dCases = nullptr;
@@ -273,11 +273,14 @@ int vcp_batch_add(vcp_batch *b, vcp_ctx *c, int n_rk_steps)
   HIP_TRY(hipMemcpy(c->dc.scal + vcp::kScalElapsed, sc, sizeof(sc), hipMemcpyHostToDevice));
   vcp_batch::Group *grp = nullptr;
   for (vcp_batch::Group &g : b->groups)
-    if (g.K == c->dc.K && g.M == c->dc.M && g.bug == c->dc.bug_compat && g.mixed == c->dc.mixed) grp = &g;
+    if (g.K == c->dc.K && g.M == c->dc.M && g.bug == c->dc.bug_compat && g.mixed == c->dc.mixed
+        && g.sgs == (c->dc.sgs != 0))
+      grp = &g;
   if (!grp) {
     b->groups.emplace_back();
     grp = &b->groups.back();
     grp->K = c->dc.K; grp->M = c->dc.M; grp->bug = c->dc.bug_compat; grp->mixed = c->dc.mixed;
+    grp->sgs = c->dc.sgs != 0;
   }
   if (grp->initFrom < 0) grp->initFrom = (int) grp->cases.size();
   grp->cases.push_back(c);

@@ -32,7 +32,7 @@ namespace {
 
 // device scalars read back with the monitoring data, and the slot of the
 // error flags in the pinned host buffer
-constexpr int kScalRead = vcp::kScalForce + 6, kHostFlags = vcp::kScalCount;
+constexpr int kScalRead = vcp::kScalEddy + 1, kHostFlags = vcp::kScalCount;
 
 size_t state_size(const vcp_ctx *c) { return (size_t) c->dc.nElem * 5 * c->dc.nDOF; }
 
@@ -146,6 +146,7 @@ void params_from_desc(const vcp_case_desc *d, vcp::CaseParams &p)
   p.outlet_one_minus_beta = d->outlet_one_minus_beta; p.outlet_average_local = d->outlet_average_local;
   p.inlet_len = d->inlet_len; p.inlet_sigma = d->inlet_sigma; p.inlet_incoming = d->inlet_incoming;
   p.sponge_layer = d->sponge_layer; p.cml = d->cml;
+  p.sgs_model = d->sgs_model;
 }
 
 // Monitoring data of SolverClass::Residual (SolverClass.cpp:3754-3783) from
@@ -158,7 +159,7 @@ void fill_monitoring(const vcp_ctx *c, double md[8], bool mon, const double *sc)
   const double fc = 2.0 / (1.4 * c->prm.mach * c->prm.mach * area);
   for (int i = 0; i < 6; ++i) md[i] = mon && c->dc.monitor_forces ? sc[vcp::kScalForce + i] * fc : 0.0;
   md[6] = mon ? sc[1] : 0.0;
-  md[7] = mon ? 0.0 : -1.0;
+  md[7] = mon ? (c->dc.sgs ? sc[vcp::kScalEddy] : 0.0) : -1.0;
 }
 
 }  // namespace
@@ -220,12 +221,14 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
   dc.nDOF = K * K * K; dc.nInt = M * M * M; dc.nInt2D = M * M;
   dc.bug_compat = p.bug_compat; dc.mu = c->rc.mu; dc.theta_sym = p.theta_sym; dc.rel_penalty = p.rel_penalty;
   dc.cfl = p.cfl; dc.u_ref = c->rc.u_ref; dc.p_ref = c->rc.p_ref; dc.rho_ref = c->rc.rho_ref;
+  dc.sgs = p.sgs_model; dc.nAvg = dc.sgs ? 12 : 11;
 
   // table block
   const vcp::Tables &t = c->tab;
   std::vector<double> tb;
   auto put = [&](const std::vector<double> &v) { const int off = (int) tb.size(); tb.insert(tb.end(), v.begin(), v.end()); return off; };
   dc.to.A = put(t.A); dc.to.D = put(t.D); dc.to.AT = put(t.AT); dc.to.DT = put(t.DT); dc.to.LD = put(t.LD);
+  dc.to.DLD = put(t.DLD);
   dc.to.fmin = put(t.f_min); dc.to.fmax = put(t.f_max); dc.to.dfmin = put(t.df_min); dc.to.dfmax = put(t.df_max);
   for (int i = 0; i < M * K; ++i) { dc.opA[i] = t.A[i]; dc.opD[i] = t.D[i]; }
   for (int q = 0; q < K; ++q) {
@@ -269,6 +272,7 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
                 sizeof(double) * vcp::kVolGeo * nInt);
     for (int q = 0; q < 4; ++q) econ[(size_t) e * vcp::kElemConst + q] = g.lens[(size_t) e * 4 + q];
     for (int q = 0; q < 9; ++q) econ[(size_t) e * vcp::kElemConst + 4 + q] = g.ave_norm[(size_t) e * 9 + q];
+    econ[(size_t) e * vcp::kElemConst + vcp::kElemLenLES] = g.len_les[e];
     for (int dir = 0; dir < 3; ++dir) {
       const int L = g.nbr[(size_t) e * 6 + 2 * dir];
       double *dst = &fgeo[((size_t) e * 3 + dir) * vcp::kFaceGeo * n2];
@@ -308,8 +312,18 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
   if (he == hipSuccess) he = c->alloc(&dc.dtElem, (size_t) nElem);
   if (he == hipSuccess) he = c->alloc(&dc.mach2Elem, (size_t) nElem);
   if (he == hipSuccess && p.compute_average) {
-    he = c->alloc(&dc.avg, (size_t) nElem * 11 * dc.nDOF);
-    if (he == hipSuccess) he = hipMemset(dc.avg, 0, (size_t) nElem * 11 * dc.nDOF * sizeof(double));
+    he = c->alloc(&dc.avg, (size_t) nElem * dc.nAvg * dc.nDOF);
+    if (he == hipSuccess) he = hipMemset(dc.avg, 0, (size_t) nElem * dc.nAvg * dc.nDOF * sizeof(double));
+  }
+  if (he == hipSuccess && dc.sgs) {
+    // nodal metric and the eddy-viscosity maxima (entries of faces that are
+    // never written stay 0)
+    he = c->alloc(&dc.vgeoN, g.nodal.size());
+    if (he == hipSuccess)
+      he = hipMemcpy(dc.vgeoN, g.nodal.data(), g.nodal.size() * sizeof(double), hipMemcpyHostToDevice);
+    const size_t ne = (size_t) 4 * nElem + (size_t) ob.faces.size() / 2;
+    if (he == hipSuccess) he = c->alloc(&dc.eddyPart, ne);
+    if (he == hipSuccess) he = hipMemset(dc.eddyPart, 0, ne * sizeof(double));
   }
   if (he == hipSuccess && dc.nBFace) {
     he = c->alloc(&dc.bface, ob.faces.size());
@@ -664,15 +678,15 @@ int vcp_get_averages(vcp_ctx *c, double *avg, int *n_steps)
   if (!c || !avg || !n_steps) return fail("null argument");
   if (int rc_ = vcp::batch_quiesce(c)) return rc_;
   if (!c->dc.avg) return fail("the case was created without compute_average");
-  const int nD = c->dc.nDOF;
-  std::vector<double> buf((size_t) c->dc.nElem * 11 * nD);
+  const int nD = c->dc.nDOF, na = c->dc.nAvg;
+  std::vector<double> buf((size_t) c->dc.nElem * na * nD);
   HIP_TRY(hipMemcpyAsync(buf.data(), c->dc.avg, buf.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   for (int e = 0; e < c->dc.nElem; ++e)
     for (int d = 0; d < nD; ++d) {
       double *o = avg + ((size_t) e * nD + d) * 12;
-      for (int i = 0; i < 11; ++i) o[i] = buf[((size_t) e * 11 + i) * nD + d];
-      o[11] = 0.0;   // no SGS model: the averaged eddy viscosity is 0
+      for (int i = 0; i < na; ++i) o[i] = buf[((size_t) e * na + i) * nD + d];
+      if (na == 11) o[11] = 0.0;   // no SGS model: the averaged eddy viscosity is 0
     }
   *n_steps = c->n_avg;
   return 0;
@@ -686,11 +700,11 @@ int vcp_set_averages(vcp_ctx *c, const double *avg, int n_steps)
   if (int rc_ = not_in_batch(c)) return rc_;
   if (!c->dc.avg) return fail("the case was created without compute_average");
   if (n_steps < 0) return fail("negative number of averaged steps");
-  const int nD = c->dc.nDOF;
-  std::vector<double> buf((size_t) c->dc.nElem * 11 * nD);
+  const int nD = c->dc.nDOF, na = c->dc.nAvg;
+  std::vector<double> buf((size_t) c->dc.nElem * na * nD);
   for (int e = 0; e < c->dc.nElem; ++e)
     for (int d = 0; d < nD; ++d)
-      for (int i = 0; i < 11; ++i) buf[((size_t) e * 11 + i) * nD + d] = avg[((size_t) e * nD + d) * 12 + i];
+      for (int i = 0; i < na; ++i) buf[((size_t) e * na + i) * nD + d] = avg[((size_t) e * nD + d) * 12 + i];
   HIP_TRY(hipMemcpyAsync(c->dc.avg, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->n_avg = n_steps;

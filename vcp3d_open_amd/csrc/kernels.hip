@@ -48,11 +48,13 @@ enum : int { kBCOutflowNSCBC = 1, kBCInflowStaticNSCBC = 2, kBCInflowTotalNSCBC 
 constexpr double kGam = 1.4;
 constexpr double kGm1 = 0.4;
 constexpr double kPrLam = 0.72;
+constexpr double kPrTurb = 0.90;       // VCP3D_CurviLinear.hpp:176
 constexpr double kLamOverMu = -2.0 / 3.0;
 constexpr double kInvGm1 = 1.0 / kGm1;
 
 // ------------------------------------------------------------ utilities
 constexpr int cmax(int x, int y) { return x > y ? x : y; }
+constexpr int cmin(int x, int y) { return x < y ? x : y; }
 
 __device__ inline double wave_sum(double v)
 {
@@ -150,6 +152,91 @@ __device__ inline void ent2prim(const double V[5], double &rho, double &u, doubl
 }
 
 
+// Eddy viscosity mu_SGS = rho nu_t of the subgrid scale model at one point,
+// from the velocity gradient G[i][j] = du_i/dx_j and the LES length scale len.
+// WALE (CWALEModel::EddyViscosity, SGSModel.cpp:34-113): with S the
+// symmetric part of G and Sd the traceless symmetric part of G^2,
+// nu_t = (0.325 len)^2 |Sd|^3 / max(|S|^5 + |Sd|^(5/2), 1e-30), |X|^2 = X_ij X_ij.
+// Vreman (CVremanModel::EddyViscosity, :128-210): alpha = G with its trace
+// taken off the diagonal, beta = len^2 alpha alpha^T, B the sum of the
+// principal 2 x 2 minors of beta (clipped at 0),
+// nu_t = 0.07 sqrt(B / max(|alpha|^2, 1e-30)).
+__device__ inline double eddy_viscosity(int model, double len, double rho, const double G[3][3])
+{
+  constexpr double third = 1.0 / 3.0, eps = 1e-30;
+  if (model == kSgsWale) {
+    double G2[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) G2[i][j] = G[i][0] * G[0][j] + G[i][1] * G[1][j] + G[i][2] * G[2][j];
+    const double tr = third * (G2[0][0] + G2[1][1] + G2[2][2]);
+    double s2 = 0.0, sd2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const double d = G2[i][i] - tr;
+      s2 += G[i][i] * G[i][i];
+      sd2 += d * d;
+#pragma unroll
+      for (int j = i + 1; j < 3; ++j) {
+        const double sij = 0.5 * (G[i][j] + G[j][i]), dij = 0.5 * (G2[i][j] + G2[j][i]);
+        s2 += 2.0 * sij * sij;
+        sd2 += 2.0 * dij * dij;
+      }
+    }
+    const double sd32 = sd2 * sqrt(sd2);
+    const double den = s2 * s2 * sqrt(s2) + sqrt(sd32 * sd2);
+    const double l = 0.325 * len;
+    return rho * (l * l) * sd32 / fmax(den, eps);
+  }
+  if (model == kSgsVreman) {
+    const double tr = third * (G[0][0] + G[1][1] + G[2][2]);
+    double al[3][3], a2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        al[i][j] = i == j ? G[i][j] - tr : G[i][j];
+        a2 += al[i][j] * al[i][j];
+      }
+    const double l2 = len * len;
+    double be[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = i; j < 3; ++j) be[i][j] = l2 * (al[i][0] * al[j][0] + al[i][1] * al[j][1] + al[i][2] * al[j][2]);
+    const double B = be[0][0] * be[1][1] - be[0][1] * be[0][1] + be[0][0] * be[2][2] - be[0][2] * be[0][2]
+                   + be[1][1] * be[2][2] - be[1][2] * be[1][2];
+    return rho * 0.07 * sqrt(fmax(B, 0.0) / fmax(a2, eps));
+  }
+  return 0.0;
+}
+
+// Eddy viscosity at a face point (FluxesFace.cpp:1409-1459, :1481-1494): the
+// model on the density of the averaged entropy variables and the velocity
+// gradients of the averaged state and the averaged Cartesian gradients
+// gx, gy, gz, with the face's LES length scale. Evaluated before the fluxes,
+// so that its temporaries are dead where those peak in registers.
+__device__ __forceinline__ double face_eddy(int sgs, double lenLES, const double VL[5], const double VR[5],
+                                            const double gx[5], const double gy[5], const double gz[5])
+{
+  const double V0a = 0.5 * (VL[0] + VR[0]), V4a = 0.5 * (VL[4] + VR[4]);
+  const double V4Inv = 2.0 / (VL[4] + VR[4]);
+  const double u = -V4Inv * 0.5 * (VL[1] + VR[1]), v = -V4Inv * 0.5 * (VL[2] + VR[2]),
+               w = -V4Inv * 0.5 * (VL[3] + VR[3]);
+  const double ek = 0.5 * (u * u + v * v + w * w);
+  const double rhoA = pow_m2p5(-V4a * exp(kGam - kGm1 * (V0a - V4a * ek)));
+  const double vel[3] = {u, v, w};
+  double G[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    G[i][0] = -V4Inv * (gx[1 + i] + vel[i] * gx[4]);
+    G[i][1] = -V4Inv * (gy[1 + i] + vel[i] * gy[4]);
+    G[i][2] = -V4Inv * (gz[1 + i] + vel[i] * gz[4]);
+  }
+  return eddy_viscosity(sgs, lenLES, rhoA, G);
+}
+
 // Ismail-Roe + viscous + interior-penalty + symmetrising fluxes at one face
 // point: VL/VR entropy states (L -> R along nn), gx/gy/gz the averaged
 // Cartesian gradients, wA = weight * area factor. fl: normal flux, FX/FY/FZ:
@@ -158,15 +245,22 @@ __device__ inline void ent2prim(const double V[5], double &rho, double &u, doubl
 // multiplies the convective and dissipation parts of the inviscid flux and a
 // prescribed wall heat flux (hf) zeroes the laminar heat-flux factor and adds
 // wA * qWall to the energy flux (FluxesFace.cpp:1312-1318, :1586-1593).
+// sgs: with a subgrid scale model, its eddy viscosity ev at the point
+// (face_eddy) adds to mu in the stresses, the penalty and the symmetrising
+// fluxes, with the turbulent Prandtl number in the heat flux
+// (FluxesFace.cpp:1537-1554, :1096-1103, :1198-1213).
 template <bool BND = false>
 __device__ __forceinline__ void point_flux(const double VL[5], const double VR[5], const double gx[5],
                                            const double gy[5], const double gz[5], const double nn[3],
                                            double wA, double tau, double mu, double thetaHalf,
                                            double fl[5], double FX[5], double FY[5], double FZ[5],
                                            double factU = 1.0, bool hf = false, double qWall = 0.0,
-                                           double *aux = nullptr)
+                                           double *aux = nullptr, bool sgs = false, double ev = 0.0)
 {
   const double factHF = (BND && hf) ? 0.0 : kGam / kPrLam;
+  const double factHFT = (BND && hf) ? 0.0 : kGam / kPrTurb;
+  // total viscosity, total heat-flux factor and their ratio
+  double muT = mu, fhT = mu * factHF, fhR = factHF;
   const double lam = kLamOverMu, lamp1 = lam + 1.0, lamp2 = lam + 2.0;
   {
     double rL, uL, vL, wL, pL, eL, rR, uR, vR, wR, pR, eR;
@@ -234,12 +328,16 @@ __device__ __forceinline__ void point_flux(const double VL[5], const double VR[5
     const double dudz = -V4Inv * (gz[1] + u * gz[4]), dvdz = -V4Inv * (gz[2] + vv * gz[4]), dwdz = -V4Inv * (gz[3] + w * gz[4]);
     const double fe = V4Inv * V4Inv * kInvGm1;
     const double dedx = fe * gx[4], dedy = fe * gy[4], dedz = fe * gz[4];
+    if (sgs) {
+      muT = mu + ev;
+      fhT = mu * factHF + ev * factHFT;
+      fhR = fhT / muT;
+    }
     const double nx = wA * nn[0], ny = wA * nn[1], nz = wA * nn[2];
     const double lamDiv = kLamOverMu * (dudx + dvdy + dwdz);
-    const double txx = mu * (2.0 * dudx + lamDiv), tyy = mu * (2.0 * dvdy + lamDiv), tzz = mu * (2.0 * dwdz + lamDiv);
-    const double txy = mu * (dudy + dvdx), txz = mu * (dudz + dwdx), tyz = mu * (dvdz + dwdy);
-    const double fh = mu * factHF;
-    const double qx = -fh * dedx, qy = -fh * dedy, qz = -fh * dedz;
+    const double txx = muT * (2.0 * dudx + lamDiv), tyy = muT * (2.0 * dvdy + lamDiv), tzz = muT * (2.0 * dwdz + lamDiv);
+    const double txy = muT * (dudy + dvdx), txz = muT * (dudz + dwdx), tyz = muT * (dvdz + dwdy);
+    const double qx = -fhT * dedx, qy = -fhT * dedy, qz = -fhT * dedz;
     fl[1] -= txx * nx + txy * ny + txz * nz;
     fl[2] -= txy * nx + tyy * ny + tyz * nz;
     fl[3] -= txz * nx + tyz * ny + tzz * nz;
@@ -250,8 +348,8 @@ __device__ __forceinline__ void point_flux(const double VL[5], const double VR[5
   }
   const double pOvRho = -V4Inv, eInt = pOvRho * kInvGm1;
   const double dV1 = VL[1] - VR[1], dV2 = VL[2] - VR[2], dV3 = VL[3] - VR[3], dV4 = VL[4] - VR[4];
-  const double factFlux = pOvRho * wA * mu;
-  const double abvE = u * u + vv * vv + w * w + factHF * eInt;
+  const double factFlux = pOvRho * wA * muT;
+  const double abvE = u * u + vv * vv + w * w + fhR * eInt;
   const double un = u * nn[0] + vv * nn[1] + w * nn[2];
   const double dVn = dV1 * nn[0] + dV2 * nn[1] + dV3 * nn[2];
   const double ttau = tau * factFlux, abvMom = lamp1 * (dVn + un * dV4);
@@ -346,8 +444,9 @@ struct FaceSmem {
 // arithmetic. T: the type of the contractions (double, or float for the
 // mixed-precision form, where the traces carry the variation about each
 // element's mean, added back at the points in double, and the pointwise
-// fluxes stay in double).
-template <int K, int M, int DIR, bool BUG, bool BATCH, typename T>
+// fluxes stay in double). SGS: the case has a subgrid scale model (a
+// separate instantiation: its eddy-viscosity code raises the register peak).
+template <int K, int M, int DIR, bool BUG, bool BATCH, typename T, bool SGS>
 __device__ __forceinline__ void face_body(const FaceArgs<K, M, T> &args, const CaseDev a, FaceSmem<K, M, T> &sm,
                                           int e)
 {
@@ -439,6 +538,11 @@ __device__ __forceinline__ void face_body(const FaceArgs<K, M, T> &args, const C
   // overwrite T in place when single-pass, else the dead B
   const double lenScale = fmin(a.econst[(size_t) e * kElemConst + DIR], a.econst[(size_t) L * kElemConst + DIR]);
   const double tau = a.tauFact / lenScale;
+  // LES length scale of the face: the mean of the two elements'
+  // (ElementClass.cpp:2655)
+  const double lenLES = SGS ? 0.5 * (a.econst[(size_t) e * kElemConst + kElemLenLES]
+                                     + a.econst[(size_t) L * kElemConst + kElemLenLES]) : 0.0;
+  double evMax = 0.0;
   for (int m0 = 0; m0 < N2; m0 += 64) {
     const int m = m0 + tid;
     if (m >= N2) break;
@@ -484,13 +588,24 @@ __device__ __forceinline__ void face_body(const FaceArgs<K, M, T> &args, const C
     for (int v = 0; v < 5; ++v) asm volatile("" : "+v"(Vs[0][v]), "+v"(Vs[1][v]), "+v"(gx[v]), "+v"(gy[v]), "+v"(gz[v]));
     const double nn[3] = {g[0 * N2 + m], g[1 * N2 + m], g[2 * N2 + m]};
     const double wA = sW2[m] * g[3 * N2 + m];
-    double fl[5], FX[5], FY[5], FZ[5];
-    point_flux(Vs[1], Vs[0], gx, gy, gz, nn, wA, tau, a.mu, a.thetaHalf, fl, FX, FY, FZ);
+    double fl[5], FX[5], FY[5], FZ[5], ev = 0.0;
+    if constexpr (SGS) {
+      ev = face_eddy(a.sgs, lenLES, Vs[1], Vs[0], gx, gy, gz);
+      evMax = fmax(evMax, ev);
+    }
+    point_flux(Vs[1], Vs[0], gx, gy, gz, nn, wA, tau, a.mu, a.thetaHalf, fl, FX, FY, FZ, 1.0, false, 0.0, nullptr,
+               SGS, ev);
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
       sFlux[v * N2 + m] = T(fl[v]);
       sFC[v * N2 + m] = T(FX[v]); sFC[(5 + v) * N2 + m] = T(FY[v]); sFC[(10 + v) * N2 + m] = T(FZ[v]);
     }
+  }
+  if (SGS && a.eddyPart) {
+    // the face's eddy-viscosity maximum for the monitoring data
+    // (FluxesFace.cpp:336-342)
+    evMax = wave_max(evMax);
+    if (tid == 0) a.eddyPart[a.nElem + 3 * e + DIR] = evMax;
   }
   __syncthreads();
   // ---- lift, one side at a time: fast-axis contraction into Y (first the
@@ -578,9 +693,12 @@ __device__ __forceinline__ void face_body(const FaceArgs<K, M, T> &args, const C
 // The three directions in one launch: blockIdx.y is the direction;
 // blockIdx.x the element, or (BATCH) a work item (case, element) of the
 // group. The argument block is read through the kernel-argument segment
-// pointer: a by-value copy indexed at run time would go to scratch.
-template <int K, int M, bool BUG, bool BATCH, typename T = double>
-__global__ __launch_bounds__(64, (FaceSmem<K, M, T>::kWaves)) void face_kernel(FaceArgs<K, M, T> a_)
+// pointer: a by-value copy indexed at run time would go to scratch. The SGS
+// form is compiled for two waves per SIMD, where its eddy-viscosity code fits
+// the registers without spilling.
+template <int K, int M, bool BUG, bool BATCH, typename T, bool SGS>
+__global__ __launch_bounds__(64, (SGS ? cmin(2, FaceSmem<K, M, T>::kWaves) : FaceSmem<K, M, T>::kWaves))
+void face_kernel(FaceArgs<K, M, T> a_)
 {
   __shared__ FaceSmem<K, M, T> sm;
   (void) a_;
@@ -596,9 +714,9 @@ __global__ __launch_bounds__(64, (FaceSmem<K, M, T>::kWaves)) void face_kernel(F
     c = a.c;
     e = blockIdx.x;
   }
-  if (blockIdx.y == 0) face_body<K, M, 0, BUG, BATCH, T>(a, c, sm, e);
-  else if (blockIdx.y == 1) face_body<K, M, 1, BUG, BATCH, T>(a, c, sm, e);
-  else face_body<K, M, 2, BUG, BATCH, T>(a, c, sm, e);
+  if (blockIdx.y == 0) face_body<K, M, 0, BUG, BATCH, T, SGS>(a, c, sm, e);
+  else if (blockIdx.y == 1) face_body<K, M, 1, BUG, BATCH, T, SGS>(a, c, sm, e);
+  else face_body<K, M, 2, BUG, BATCH, T, SGS>(a, c, sm, e);
 }
 
 // 1D operators of the standard element as kernel arguments: indexed with
@@ -870,7 +988,8 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
   // (b = 5 d + l) fill [0, SF0) and leave the y-stage rows of the second
   // pass in place; the second pass's fluxes go to SF0 + [b][m - P0].
   const double factHF = a.mu * kGam / kPrLam;
-  double mach2 = 0.0;
+  const double lenLES = a.sgs ? a.econst[(size_t) e * kElemConst + kElemLenLES] : 0.0;
+  double mach2 = 0.0, evMax = 0.0;
   // (w J rho, u, v, w, w J p, w J dU/dV_44) at the thread's points, for the
   // weighted dU/dV in the solve
   T prim[2][6];
@@ -946,11 +1065,20 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
     const double dedx = fe * gx[4], dedy = fe * gy[4], dedz = fe * gz[4];
     if (args.ctl.monitor) mach2 = fmax(mach2, rho * (u * u + v * v + w * w) / (kGam * p));
     const double ru = rho * u, rv = rho * v, rw = rho * w;
-    const double mu = a.mu;
+    // eddy viscosity of the SGS model (ElementClass.cpp:4268-4287, :4305-4306)
+    // in the total viscosity and heat-flux factor (:4339-4342)
+    double mu = a.mu, fhT = factHF;
+    if (a.sgs != kSgsNone) {
+      const double G[3][3] = {{dudx, dudy, dudz}, {dvdx, dvdy, dvdz}, {dwdx, dwdy, dwdz}};
+      const double ev = eddy_viscosity(a.sgs, lenLES, rho, G);
+      evMax = fmax(evMax, ev);
+      mu = a.mu + ev;
+      fhT = a.mu * (kGam / kPrLam) + ev * (kGam / kPrTurb);
+    }
     const double divV = kLamOverMu * (dudx + dvdy + dwdz);
     const double txx = mu * (2.0 * dudx + divV), tyy = mu * (2.0 * dvdy + divV), tzz = mu * (2.0 * dwdz + divV);
     const double txy = mu * (dudy + dvdx), txz = mu * (dudz + dwdx), tyz = mu * (dvdz + dwdy);
-    const double qx = -factHF * dedx, qy = -factHF * dedy, qz = -factHF * dedz;
+    const double qx = -fhT * dedx, qy = -fhT * dedy, qz = -fhT * dedz;
     const double fx[5] = {ru, ru * u - txx + p, rv * u - txy, rw * u - txz, (rE + p) * u - (u * txx + v * txy + w * txz - qx)};
     const double fy[5] = {rv, ru * v - txy, rv * v - tyy + p, rw * v - tyz, (rE + p) * v - (u * txy + v * tyy + w * tyz - qy)};
     const double fz[5] = {rw, ru * w - txz, rv * w - tyz, rw * w - tzz + p, (rE + p) * w - (u * txz + v * tyz + w * tzz - qz)};
@@ -1157,11 +1285,13 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
 #pragma unroll
   for (int l = 0; l < 5; ++l) { x[l] = 0; r0 = fmax(r0, fabs(double(r[l]))); }
   {
-    // with the element's maximum Mach^2 for the monitoring data
-    double red[2] = {r0, mach2};
-    block_reduce1<NT, 2, 0>(red, sRed, half);
+    // with the element's maximum Mach^2 and eddy viscosity for the
+    // monitoring data
+    double red[3] = {r0, mach2, evMax};
+    block_reduce1<NT, 3, 0>(red, sRed, half);
     r0 = red[0];
     if (args.ctl.monitor && tid == 0) a.mach2Elem[e] = red[1];
+    if (a.eddyPart && tid == 0) a.eddyPart[e] = red[2];
   }
   int iters = 0;
   if (r0 * JacInv > 1e-10) {
@@ -1373,6 +1503,8 @@ struct BndArgs {
   TableOffsets to;
   int bug, monitor;             // monitor: wall forces of this residual into forcePart
   double mu, tauFact, thetaHalf;
+  int sgs, nElem;
+  double *eddyPart;             // eddy-viscosity maxima (null: no SGS model)
   NscbcParams nb;
 };
 
@@ -1810,6 +1942,9 @@ __global__ __launch_bounds__(64) void nscbc_kernel(BndArgs<K, M> a)
   // fluxes with the face normal oriented out of the element
   const double lenScale = a.econst[(size_t) e * kElemConst + dir];
   const double tau = a.tauFact / lenScale;
+  // the element's own LES length scale (ElementClass.cpp:2788-2790)
+  const double lenLES = a.econst[(size_t) e * kElemConst + kElemLenLES];
+  double evMax = 0.0;
   const bool wall = bc >= kBCWallHeatFlux && bc <= kBCWallInviscid;
   const bool forces = a.monitor && wall;
   double frc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -1843,9 +1978,13 @@ __global__ __launch_bounds__(64) void nscbc_kernel(BndArgs<K, M> a)
     const double nn[3] = {fn * g[m], fn * g[N2 + m], fn * g[2 * N2 + m]};
     const double wA = sm.W2[m] * g[3 * N2 + m];
     double fl[5], FX[5], FY[5], FZ[5];
-    double aux[7];
+    double aux[7], ev = 0.0;
+    if (a.sgs) {
+      ev = face_eddy(a.sgs, lenLES, VL, VR, gx, gy, gz);
+      evMax = fmax(evMax, ev);
+    }
     point_flux<true>(VL, VR, gx, gy, gz, nn, wA, tau, a.mu, a.thetaHalf, fl, FX, FY, FZ, wall ? 0.0 : 1.0,
-                     bc == kBCWallHeatFlux, a.nb.data[f][0], forces ? aux : nullptr);
+                     bc == kBCWallHeatFlux, a.nb.data[f][0], forces ? aux : nullptr, a.sgs != kSgsNone, ev);
     if (forces) {
       // wall forces (FluxesFace.cpp:999-1013, :1590-1612): the averaged
       // pressure and the viscous stress on the normal into the boundary
@@ -1872,6 +2011,10 @@ __global__ __launch_bounds__(64) void nscbc_kernel(BndArgs<K, M> a)
       sm.Sym[1][v * N2 + m] = par[pF];
       sm.Sym[2][v * N2 + m] = par[pS];
     }
+  }
+  if (a.eddyPart) {
+    evMax = wave_max(evMax);
+    if (tid == 0) a.eddyPart[4 * a.nElem + b] = evMax;
   }
   if (a.monitor) {
     // this face's force sums (zero off the walls), summed in a fixed order
@@ -1973,27 +2116,89 @@ __device__ __forceinline__ void lines_inplace(double *sU, const double *L, int l
   }
 }
 
+// Nodal entropy variables of element e and (GRAD) their parametric
+// derivatives along r, s and t, in LDS: sV[q][v][ND], q = 0 the values, 1..3
+// the derivatives (the modal basis or its derivative along each axis); one
+// wave. sL, sDL: the K x K tables phi_q(r_p), phi'_q(r_p) at the nodal DOFs.
+template <int K, bool GRAD>
+__device__ __forceinline__ void nodal_state(const double *Ue, const double *sL, const double *sDL, double *sV,
+                                            int lane)
+{
+  constexpr int ND = K * K * K, NQ = GRAD ? 4 : 1;
+  for (int i = lane; i < 5 * ND; i += 64) {
+    const double u = Ue[i];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) sV[q * 5 * ND + i] = u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) lines_inplace<K, 1>(sV + q * 5 * ND, q == 1 ? sDL : sL, lane);
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) lines_inplace<K, K>(sV + q * 5 * ND, q == 2 ? sDL : sL, lane);
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) lines_inplace<K, K * K>(sV + q * 5 * ND, q == 3 ? sDL : sL, lane);
+  __syncthreads();
+}
+
+// Dimensional eddy viscosity at nodal DOF d from nodal_state<K, true>: the
+// dimensional density and velocity gradients of the entropy variables with
+// the nodal metric vn (ElementClass::ComputeVelocityGradientsNodalDOFs,
+// ElementClass.cpp:1649-1790) and the element's LES length scale
+// (ElementClass.cpp:1562-1573, :3785-3797).
 template <int K>
+__device__ __forceinline__ double nodal_eddy(const CaseDev &a, const double *sV, const double *vn, double lenLES,
+                                             int d)
+{
+  constexpr int ND = K * K * K;
+  double V[5], rho, u, v, w, p, ek;
+#pragma unroll
+  for (int l = 0; l < 5; ++l) V[l] = sV[l * ND + d];
+  ent2prim(V, rho, u, v, w, p, ek);
+  rho *= a.rho_ref; p *= a.p_ref;
+  const double V4Inv = -p / (a.u_ref * a.u_ref) / rho;
+  const double vel[3] = {u, v, w};
+  double gp[3][3];     // [velocity][r | s | t]
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const double *D = sV + (1 + r) * 5 * ND;
+      gp[i][r] = -V4Inv * (D[(1 + i) * ND + d] + vel[i] * D[4 * ND + d]);
+    }
+  double G[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int x = 0; x < 3; ++x)
+      G[i][x] = a.u_ref * (gp[i][0] * vn[(1 + x) * ND + d] + gp[i][1] * vn[(4 + x) * ND + d]
+                           + gp[i][2] * vn[(7 + x) * ND + d]);
+  return eddy_viscosity(a.sgs, lenLES, rho, G);
+}
+
+// Per-element time step (ElementClass::ComputeTimeStep, ElementClass.cpp:
+// 1522-1644); SGS: with the eddy viscosity at the nodal DOFs in the viscous
+// spectral radius, with the turbulent Prandtl number.
+template <int K, bool SGS>
 __device__ __forceinline__ void time_step_body(const CaseDev &a, const TableOffsets &to, const int P, const int e)
 {
   constexpr int ND = K * K * K;
-  __shared__ double sL[K * K], sU[5 * ND];
+  __shared__ double sL[K * K], sDL[SGS ? K * K : 1], sU[(SGS ? 20 : 5) * ND];
   const int tid = threadIdx.x;
-  for (int i = tid; i < K * K; i += 64) sL[i] = a.tables[to.LD + i];
-  for (int i = tid; i < 5 * ND; i += 64) sU[i] = a.U[(size_t) e * 5 * ND + i];
-  __syncthreads();
-  // modal -> nodal values along i, j and k, in place
-  lines_inplace<K, 1>(sU, sL, tid);
-  __syncthreads();
-  lines_inplace<K, K>(sU, sL, tid);
-  __syncthreads();
-  lines_inplace<K, K * K>(sU, sL, tid);
-  __syncthreads();
+  for (int i = tid; i < K * K; i += 64) {
+    sL[i] = a.tables[to.LD + i];
+    if constexpr (SGS) sDL[i] = a.tables[to.DLD + i];
+  }
+  // modal -> nodal values (and derivatives) along i, j and k, in place
+  nodal_state<K, SGS>(a.U + (size_t) e * 5 * ND, sL, sDL, sU, tid);
   const double *sT1 = sU;
   const double *ec = a.econst + (size_t) e * kElemConst;
   const double di = 1.0 / ec[0], dj = 1.0 / ec[1], dk = 1.0 / ec[2];
   const double dl2 = di * di + dj * dj + dk * dk;
   const double rvn = fmax(fmax(1.0, 2.0 + kLamOverMu), kGam / kPrLam) * dl2;
+  const double rvnT = fmax(fmax(1.0, 2.0 + kLamOverMu), kGam / kPrTurb) * dl2;
+  const double *vn = SGS ? a.vgeoN + (size_t) e * kNodalGeo * ND : nullptr;
   double radInv = 0.0, radVis = 0.0;
   int badA2 = 0;                     // a^2 negative or not a number at a DOF
   for (int d = tid; d < ND; d += 64) {
@@ -2009,7 +2214,12 @@ __device__ __forceinline__ void time_step_body(const CaseDev &a, const TableOffs
     const double vK = u * ec[10] + v * ec[11] + w * ec[12];
     const double c = sqrt(fabs(a2));
     radInv = fmax(radInv, (fabs(vI) + c) * di + (fabs(vJ) + c) * dj + (fabs(vK) + c) * dk);
-    radVis = fmax(radVis, ri * a.muDim * rvn);
+    if constexpr (SGS) {
+      const double ev = nodal_eddy<K>(a, sU, vn, ec[kElemLenLES], d);
+      radVis = fmax(radVis, ri * (a.muDim * rvn + ev * rvnT));
+    } else {
+      radVis = fmax(radVis, ri * a.muDim * rvn);
+    }
     badA2 |= !(a2 >= 0.0);
   }
   radInv = wave_max(radInv); radVis = wave_max(radVis);
@@ -2023,58 +2233,61 @@ __device__ __forceinline__ void time_step_body(const CaseDev &a, const TableOffs
   }
 }
 
-template <int K>
+template <int K, bool SGS>
 __global__ __launch_bounds__(64) void time_step_kernel(DtArgs a)
 {
-  time_step_body<K>(a.c, a.to, a.P, blockIdx.x);
+  time_step_body<K, SGS>(a.c, a.to, a.P, blockIdx.x);
 }
 
-template <int K>
+template <int K, bool SGS>
 __global__ __launch_bounds__(64) void time_step_batch_kernel(DtBatchArgs b)
 {
   const WorkItem w = b.work[blockIdx.x];
   const CaseDev c = b.cases[w.c];
   if (c.flags[kWordActive] == 0) return;
-  time_step_body<K>(c, b.to, b.P, w.e);
+  time_step_body<K, SGS>(c, b.to, b.P, w.e);
 }
 
 // Running averages of the dimensional primitive variables and the velocity
 // products at the nodal DOFs (ElementClass::UpdateAverageData,
 // ElementClass.cpp:3741-3782): avg = factOld avg + factNew q, with q =
-// (rho, u, v, w, p, uu, vv, ww, uv, uw, vw), layout [nElem][11][nDOF].
+// (rho, u, v, w, p, uu, vv, ww, uv, uw, vw) and, with an SGS model, the eddy
+// viscosity (:3785-3803); layout [nElem][11 | 12][nDOF].
 struct AvgArgs {
-  const double *U, *tables;
+  CaseDev c;
   double *avg;
   TableOffsets to;
-  double u_ref, p_ref, rho_ref, factNew, factOld;
+  double factNew, factOld;
 };
 
-template <int K>
+template <int K, bool SGS>
 __global__ __launch_bounds__(64) void average_kernel(AvgArgs a)
 {
-  constexpr int ND = K * K * K;
-  __shared__ double sL[K * K], sU[5 * ND];
+  constexpr int ND = K * K * K, NA = SGS ? 12 : 11;
+  __shared__ double sL[K * K], sDL[SGS ? K * K : 1], sU[(SGS ? 20 : 5) * ND];
+  const CaseDev &c = a.c;
   const int e = blockIdx.x, tid = threadIdx.x;
-  for (int i = tid; i < K * K; i += 64) sL[i] = a.tables[a.to.LD + i];
-  for (int i = tid; i < 5 * ND; i += 64) sU[i] = a.U[(size_t) e * 5 * ND + i];
-  __syncthreads();
-  lines_inplace<K, 1>(sU, sL, tid);
-  __syncthreads();
-  lines_inplace<K, K>(sU, sL, tid);
-  __syncthreads();
-  lines_inplace<K, K * K>(sU, sL, tid);
-  __syncthreads();
-  double *av = a.avg + (size_t) e * 11 * ND;
+  for (int i = tid; i < K * K; i += 64) {
+    sL[i] = c.tables[a.to.LD + i];
+    if constexpr (SGS) sDL[i] = c.tables[a.to.DLD + i];
+  }
+  nodal_state<K, SGS>(c.U + (size_t) e * 5 * ND, sL, sDL, sU, tid);
+  double *av = a.avg + (size_t) e * NA * ND;
+  const double *vn = SGS ? c.vgeoN + (size_t) e * kNodalGeo * ND : nullptr;
   for (int d = tid; d < ND; d += 64) {
     double V[5];
 #pragma unroll
     for (int l = 0; l < 5; ++l) V[l] = sU[l * ND + d];
     double rho, u, v, w, p, ek;
     ent2prim(V, rho, u, v, w, p, ek);
-    rho *= a.rho_ref; u *= a.u_ref; v *= a.u_ref; w *= a.u_ref; p *= a.p_ref;
+    rho *= c.rho_ref; u *= c.u_ref; v *= c.u_ref; w *= c.u_ref; p *= c.p_ref;
     const double q[11] = {rho, u, v, w, p, u * u, v * v, w * w, u * v, u * w, v * w};
 #pragma unroll
     for (int i = 0; i < 11; ++i) av[i * ND + d] = a.factOld * av[i * ND + d] + a.factNew * q[i];
+    if constexpr (SGS) {
+      const double ev = nodal_eddy<K>(c, sU, vn, c.econst[(size_t) e * kElemConst + kElemLenLES], d);
+      av[11 * ND + d] = a.factOld * av[11 * ND + d] + a.factNew * ev;
+    }
   }
 }
 
@@ -2090,6 +2303,7 @@ struct DampAvgArgs {
   double *dampState;
   TableOffsets to;
   double u_ref, p_ref, rho_ref;
+  int nAvg;                     // averaged quantities per DOF
 };
 
 template <int K, int M>
@@ -2102,7 +2316,7 @@ __global__ __launch_bounds__(64) void damping_avg_kernel(DampAvgArgs a)
   if (sp < 0) return;
   for (int i = tid; i < M * K; i += 64) sL[i] = a.tables[a.to.Lq + i];
   __syncthreads();
-  const double *av = a.avg + (size_t) e * 11 * ND;
+  const double *av = a.avg + (size_t) e * a.nAvg * ND;
   // along i: s0[l][k][j][p]
   for (int o = tid; o < 5 * K * K * M; o += 64) {
     const int p = o % M, r = o / M;          // r = (l k j)
@@ -2315,16 +2529,23 @@ void fill_vol_shape(const DeviceCase &dc, const StageCtl &ctl, VolArgs<K, M, T> 
   }
 }
 
+template <int K, int M, bool BATCH, typename T, bool SGS>
+void launch_face_kernel(bool bug, int n, const FaceArgs<K, M, T> &a, hipStream_t s)
+{
+  if (bug)
+    hipLaunchKernelGGL((face_kernel<K, M, true, BATCH, T, SGS>), dim3(n, 3), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL((face_kernel<K, M, false, BATCH, T, SGS>), dim3(n, 3), dim3(64), 0, s, a);
+}
+
 template <int K, int M, typename T>
 hipError_t faces_impl_t(const DeviceCase &dc, hipStream_t s)
 {
   FaceArgs<K, M, T> a{};
   a.c = case_dev(dc);
   fill_face_ops(dc, a.op);
-  if (dc.bug_compat)
-    hipLaunchKernelGGL((face_kernel<K, M, true, false, T>), dim3(dc.nElem, 3), dim3(64), 0, s, a);
-  else
-    hipLaunchKernelGGL((face_kernel<K, M, false, false, T>), dim3(dc.nElem, 3), dim3(64), 0, s, a);
+  if (dc.sgs) launch_face_kernel<K, M, false, T, true>(dc.bug_compat, dc.nElem, a, s);
+  else launch_face_kernel<K, M, false, T, false>(dc.bug_compat, dc.nElem, a, s);
   return hipGetLastError();
 }
 
@@ -2340,10 +2561,8 @@ hipError_t batch_faces_impl_t(const BatchGroup &g, int w0, int nw, hipStream_t s
   FaceArgs<K, M, T> a{};
   a.cases = g.cases; a.work = g.work; a.w0 = w0;
   fill_face_ops(*g.proto, a.op);
-  if (g.proto->bug_compat)
-    hipLaunchKernelGGL((face_kernel<K, M, true, true, T>), dim3(nw, 3), dim3(64), 0, s, a);
-  else
-    hipLaunchKernelGGL((face_kernel<K, M, false, true, T>), dim3(nw, 3), dim3(64), 0, s, a);
+  if (g.proto->sgs) launch_face_kernel<K, M, true, T, true>(g.proto->bug_compat, nw, a, s);
+  else launch_face_kernel<K, M, true, T, false>(g.proto->bug_compat, nw, a, s);
   return hipGetLastError();
 }
 
@@ -2364,6 +2583,7 @@ hipError_t boundary_impl(const DeviceCase &dc, hipStream_t s, bool monitor)
   a.bug = dc.bug_compat; a.monitor = monitor && dc.monitor_forces;
   a.mu = dc.mu; a.tauFact = (dc.P + 1.0) * (dc.P + 1.0) * dc.rel_penalty; a.thetaHalf = 0.5 * dc.theta_sym;
   a.nb = dc.nscbc;
+  a.sgs = dc.sgs; a.nElem = dc.nElem; a.eddyPart = dc.sgs ? dc.eddyPart : nullptr;
   hipLaunchKernelGGL((mach_kernel<K, M>), dim3(dc.nBFace), dim3(64), 0, s, a);
   hipLaunchKernelGGL((nscbc_kernel<K, M>), dim3(dc.nBFace), dim3(64), 0, s, a);
   if (a.monitor)
@@ -2419,7 +2639,10 @@ hipError_t dt_impl(const DeviceCase &dc, hipStream_t s)
 {
   DtArgs a;
   a.c = case_dev(dc); a.to = dc.to; a.P = dc.P;
-  hipLaunchKernelGGL((time_step_kernel<K>), dim3(dc.nElem), dim3(64), 0, s, a);
+  if (dc.sgs)
+    hipLaunchKernelGGL((time_step_kernel<K, true>), dim3(dc.nElem), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL((time_step_kernel<K, false>), dim3(dc.nElem), dim3(64), 0, s, a);
   hipLaunchKernelGGL(dt_min_kernel, dim3(1), dim3(1024), 0, s, dc.dtElem, dc.nElem, dc.scal);
   return hipGetLastError();
 }
@@ -2429,7 +2652,10 @@ hipError_t batch_dt_impl(const BatchGroup &g, hipStream_t s)
 {
   DtBatchArgs b;
   b.cases = g.cases; b.work = g.work; b.to = g.proto->to; b.P = g.proto->P;
-  hipLaunchKernelGGL((time_step_batch_kernel<K>), dim3(g.nWork), dim3(64), 0, s, b);
+  if (g.proto->sgs)
+    hipLaunchKernelGGL((time_step_batch_kernel<K, true>), dim3(g.nWork), dim3(64), 0, s, b);
+  else
+    hipLaunchKernelGGL((time_step_batch_kernel<K, false>), dim3(g.nWork), dim3(64), 0, s, b);
   hipLaunchKernelGGL(batch_sync_kernel, dim3(g.nCase), dim3(1024), 0, s, g.cases);
   return hipGetLastError();
 }
@@ -2449,6 +2675,7 @@ CaseDev case_dev(const DeviceCase &dc)
   c.cfl = dc.cfl; c.u_ref = dc.u_ref; c.p_ref = dc.p_ref; c.rho_ref = dc.rho_ref;
   c.muDim = dc.mu * dc.p_ref / dc.u_ref;
   c.nElem = dc.nElem; c.cml = dc.cml;
+  c.sgs = dc.sgs; c.vgeoN = dc.vgeoN; c.eddyPart = dc.sgs ? dc.eddyPart : nullptr;
   return c;
 }
 
@@ -2502,8 +2729,13 @@ hipError_t launch_volume(const DeviceCase &dc, const StageCtl &ctl, hipStream_t 
 #undef X
     return hipErrorInvalidValue;
   }();
-  if (e == hipSuccess && ctl.monitor)
+  if (e == hipSuccess && ctl.monitor) {
     hipLaunchKernelGGL(reduce_max_kernel, dim3(1), dim3(1024), 0, s, dc.mach2Elem, dc.nElem, dc.scal + 1);
+    // volume, internal-face and boundary-face maxima of the eddy viscosity
+    if (dc.sgs)
+      hipLaunchKernelGGL(reduce_max_kernel, dim3(1), dim3(1024), 0, s, dc.eddyPart, 4 * dc.nElem + dc.nBFace,
+                         dc.scal + kScalEddy);
+  }
   return e == hipSuccess ? hipGetLastError() : e;
 }
 
@@ -2523,10 +2755,12 @@ template <int K>
 hipError_t avg_impl(const DeviceCase &dc, double factNew, hipStream_t s)
 {
   AvgArgs a;
-  a.U = dc.U; a.tables = dc.tables; a.avg = dc.avg; a.to = dc.to;
-  a.u_ref = dc.u_ref; a.p_ref = dc.p_ref; a.rho_ref = dc.rho_ref;
+  a.c = case_dev(dc); a.avg = dc.avg; a.to = dc.to;
   a.factNew = factNew; a.factOld = 1.0 - factNew;
-  hipLaunchKernelGGL((average_kernel<K>), dim3(dc.nElem), dim3(64), 0, s, a);
+  if (dc.sgs)
+    hipLaunchKernelGGL((average_kernel<K, true>), dim3(dc.nElem), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL((average_kernel<K, false>), dim3(dc.nElem), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 
@@ -2547,7 +2781,7 @@ hipError_t launch_damping_from_averages(const DeviceCase &dc, hipStream_t s)
   if (!dc.avg || !dc.nSponge) return hipErrorInvalidValue;
   DampAvgArgs a;
   a.avg = dc.avg; a.tables = dc.tables; a.spIdx = dc.spIdx; a.dampState = dc.dampState; a.to = dc.to;
-  a.u_ref = dc.u_ref; a.p_ref = dc.p_ref; a.rho_ref = dc.rho_ref;
+  a.u_ref = dc.u_ref; a.p_ref = dc.p_ref; a.rho_ref = dc.rho_ref; a.nAvg = dc.nAvg;
 #define X(k, m)                                                                                    \
   if (dc.K == k && dc.M == m) {                                                                    \
     hipLaunchKernelGGL((damping_avg_kernel<k, m>), dim3(dc.nElem), dim3(64), 0, s, a);             \

@@ -9,7 +9,7 @@ namespace vcp {
 
 // Offsets (in doubles) of the 1D tables in the device table block.
 struct TableOffsets {
-  int A, D, AT, DT, LD, fmin, fmax, dfmin, dfmax, w3, w2, bc, P1, dLq, Lq, total;
+  int A, D, AT, DT, LD, DLD, fmin, fmax, dfmin, dfmax, w3, w2, bc, P1, dLq, Lq, total;
 };
 
 // PC-NSCBC parameters of the block faces (InputParamClass "NSCBC tuning
@@ -31,8 +31,14 @@ struct NscbcParams {
 constexpr int kFaceGeo = 22;
 // Per-element volume geometry, [nElem][kVolGeo][nInt]: J, dr/dx .. dt/dz.
 constexpr int kVolGeo = 10;
-// Per-element constants, [nElem][kElemConst]: len I,J,K, volume, aveNorm[9].
-constexpr int kElemConst = 16;
+// Per-element constants, [nElem][kElemConst]: len I,J,K, volume, aveNorm[9],
+// the LES length scale (slot kElemLenLES).
+constexpr int kElemConst = 16, kElemLenLES = 13;
+// Per-element metric at the nodal solution DOFs (SGS model only),
+// [nElem][kNodalGeo][nDOF]: J, dr/dx .. dt/dz.
+constexpr int kNodalGeo = 10;
+// Subgrid scale models (DeviceCase::sgs).
+enum : int { kSgsNone = 0, kSgsWale = 1, kSgsVreman = 2 };
 
 // Device status flags.
 enum : int { kFlagCGNotConverged = 1, kFlagNegativeA2 = 2, kFlagNonFinite = 4 };
@@ -42,6 +48,8 @@ struct DeviceCase {
   int bug_compat = 1;
   int nt = 0;                     // volume workgroup size (0: the default for (K, M))
   int mixed = 0;                  // 1: FP32 contractions and solve (vcp_set_precision)
+  int sgs = kSgsNone;             // subgrid scale model
+  int nAvg = 11;                  // averaged quantities per DOF (12 with an SGS model)
   double mu = 0, theta_sym = 1, rel_penalty = 1;
   double cfl = 1, u_ref = 1, p_ref = 1, rho_ref = 1;
   TableOffsets to{};
@@ -51,6 +59,11 @@ struct DeviceCase {
   double opC0[16] = {};                  // phi_q(0), the basis at the element centre
   double *tables = nullptr;
   double *vgeo = nullptr, *fgeo = nullptr, *econst = nullptr;
+  double *vgeoN = nullptr;        // [nElem][kNodalGeo][nDOF] (SGS model only)
+  // eddy-viscosity maxima of the last residual (SGS model only):
+  // [0, nElem) volume points per element, [nElem, 4 nElem) internal-face
+  // points per (element, direction), [4 nElem, 4 nElem + nBFace) boundary faces
+  double *eddyPart = nullptr;
   int *nbr = nullptr;
   double *U = nullptr, *U0 = nullptr, *R = nullptr;
   double *faceRes = nullptr;      // [nElem][6][2][5][K][K] face-slot coefficient planes
@@ -69,7 +82,7 @@ struct DeviceCase {
   double *bgeo = nullptr;         // [nBFace][13][nInt2D]: the face's normal, area factor, inverse metric
   double *machPart = nullptr;     // [nBFace] weighted face-normal Mach number
   double *forcePart = nullptr;    // [nBFace][6] wall forces (pressure xyz, viscous xyz)
-  double *avg = nullptr;          // [nElem][11][nDOF] running averages (null: not computed)
+  double *avg = nullptr;          // [nElem][nAvg][nDOF] running averages (null: not computed)
   int monitor_forces = 0;
   NscbcParams nscbc{};
   // sponge layer / characteristic matching layer
@@ -82,11 +95,12 @@ struct DeviceCase {
 };
 
 // Device scalars of a case (DeviceCase::scal): [kScalDt], [kScalForce ..
-// +6) as above; the per-case control of a batch: [kScalElapsed] time elapsed
+// +6) as above; [kScalEddy] the maximum eddy viscosity of the last monitored
+// residual (SGS model); the per-case control of a batch: [kScalElapsed] time elapsed
 // in the current synchronization step, [kScalSyncDone] synchronization steps
 // completed, [kScalDtSync] the synchronization time step (non-dimensional).
 constexpr int kScalDt = 2, kScalForce = 3, kScalElapsed = 9, kScalSyncDone = 10, kScalDtSync = 12,
-              kScalCount = 16;
+              kScalEddy = 13, kScalCount = 16;
 // Device words of a case (DeviceCase::flags): [kWordFlags] error flags,
 // [kWordActive] 1 while the case is stepped (always outside a batch; a batch
 // clears it for a case whose budget is spent or whose error flags are set, and
@@ -99,7 +113,7 @@ constexpr int kWordFlags = 0, kWordActive = 1, kWordStepsLeft = 2, kWordStepsDon
 // device case table.
 struct CaseDev {
   double *U, *U0, *R, *faceRes;
-  const double *tables, *vgeo, *fgeo, *econst;
+  const double *tables, *vgeo, *fgeo, *econst, *vgeoN;
   const int *nbr;
   double *scal;
   int *flags, *cgIter;
@@ -107,8 +121,9 @@ struct CaseDev {
   const int *spIdx;               // sponge slot per element (null: no sponge)
   const double *spCoef, *dampFn, *dampState;
   double *srcBuf;
+  double *eddyPart;               // eddy-viscosity maxima (null: no SGS model)
   double mu, tauFact, thetaHalf, cfl, u_ref, p_ref, rho_ref, muDim;
-  int nElem, cml;
+  int nElem, cml, sgs;
 };
 
 // Stage control for the fused volume/mass-matrix kernel.

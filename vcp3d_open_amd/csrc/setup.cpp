@@ -252,9 +252,13 @@ void build_tables(const CaseParams &p, Tables &t)
       t.A[pt * K + q] = v; t.D[pt * K + q] = d;
       t.AT[q * M + pt] = v; t.DT[q * M + pt] = d;
     }
-  t.LD.assign(K * K, 0);
+  t.LD.assign(K * K, 0); t.DLD.assign(K * K, 0);
   for (int pt = 0; pt < K; ++pt)
-    for (int q = 0; q < K; ++q) { double v, d; ortho_legendre(q, t.r_dof[pt], v, d); t.LD[pt * K + q] = v; }
+    for (int q = 0; q < K; ++q) {
+      double v, d;
+      ortho_legendre(q, t.r_dof[pt], v, d);
+      t.LD[pt * K + q] = v; t.DLD[pt * K + q] = d;
+    }
   t.VI = t.LD;
   invert_dense(K, t.VI);
   t.f_min.resize(K); t.f_max.resize(K); t.df_min.resize(K); t.df_max.resize(K);
@@ -321,6 +325,7 @@ std::string check_case(const CaseParams &p)
   }
   if (p.outlet_average_local)
     return "\"Outlet use local (element) averaged Mach number\" is not supported";
+  if (p.sgs_model < kSGSNone || p.sgs_model > kSGSVreman) return "Unknown \"Subgrid scale model\" specified";
   if (p.cml && !p.sponge_layer) return "a characteristic matching layer needs the sponge layer";
   if (p.sponge_layer) {
     if (p.n_poly_grid != p.n_poly_sol) return "sponge layer needs equal grid and solution degree";
@@ -370,13 +375,23 @@ std::string build_geometry(const CaseParams &p, Tables &t, const double *grid, G
   g.vol.assign((size_t) nElem * 13 * nInt, 0.0);
   for (int f = 0; f < 6; ++f) g.face[f].assign((size_t) nElem * 16 * n2, 0.0);
   g.lens.assign((size_t) nElem * 4, 0.0);
+  g.len_les.assign((size_t) nElem, 0.0);
   g.ave_norm.assign((size_t) nElem * 9, 0.0);
+  // metric terms at the nodal solution DOFs, for the velocity gradients of
+  // the SGS model there (ElementClass::ComputeCoorNodalSolDOFs,
+  // ElementClass.cpp:820-918)
+  const int K = t.K, nD = K * K * K;
+  const bool sgs = p.sgs_model != kSGSNone;
+  std::vector<double> LgN(K * Kg), dLgN(K * Kg);
+  for (int pt = 0; pt < K; ++pt) lagrange_basis(t.r_grid, t.r_dof[pt], &LgN[pt * Kg], &dLgN[pt * Kg]);
+  g.nodal.assign(sgs ? (size_t) nElem * 10 * nD : 0, 0.0);
   // elements are independent: contiguous ranges on host threads, each with
   // its own scratch (the arithmetic per element is unchanged)
   std::atomic<int> badJacobian{0};
   auto elements = [&](int e0, int e1) {
   std::vector<double> X(3 * (size_t) nInt), DR(3 * (size_t) nInt), DS(3 * (size_t) nInt), DTt(3 * (size_t) nInt);
   std::vector<double> C(12 * (size_t) n2);
+  std::vector<double> NR(sgs ? 3 * (size_t) nD : 0), NS(NR.size()), NT(NR.size());
   for (int e = e0; e < e1; ++e) {
     if (badJacobian.load(std::memory_order_relaxed)) return;
     const double *ge = grid + (size_t) e * 3 * nG;
@@ -430,9 +445,29 @@ std::string build_geometry(const CaseParams &p, Tables &t, const double *grid, G
         for (int c = 0; c < 3; ++c) sm[(13 + c) * n2 + l] = C[c * n2 + l];
       }
     }
+    if (sgs) {
+      for (int c = 0; c < 3; ++c) {
+        apply3(Kg, Kg, Kg, K, K, K, dLgN.data(), LgN.data(), LgN.data(), ge + c * nG, &NR[c * nD]);
+        apply3(Kg, Kg, Kg, K, K, K, LgN.data(), dLgN.data(), LgN.data(), ge + c * nG, &NS[c * nD]);
+        apply3(Kg, Kg, Kg, K, K, K, LgN.data(), LgN.data(), dLgN.data(), ge + c * nG, &NT[c * nD]);
+      }
+      double *vn = &g.nodal[(size_t) e * 10 * nD];
+      for (int l = 0; l < nD; ++l) {
+        const double dr[3] = {NR[l], NR[nD + l], NR[2 * nD + l]};
+        const double ds[3] = {NS[l], NS[nD + l], NS[2 * nD + l]};
+        const double dt[3] = {NT[l], NT[nD + l], NT[2 * nD + l]};
+        double J, m[9];
+        inverse_metric(dr, ds, dt, J, m);
+        if (!(J > 0.0)) { badJacobian.store(1); return; }
+        vn[l] = J;
+        for (int q = 0; q < 9; ++q) vn[(1 + q) * nD + l] = m[q];
+      }
+    }
     double vol = 0.0;
     for (int l = 0; l < nInt; ++l) vol += t.w3[l] * vm[l];
     g.lens[(size_t) e * 4 + 3] = vol;
+    // ElementClass::ComputeLengthScales (ElementClass.cpp:937-940)
+    g.len_les[e] = std::pow(vol, 1.0 / 3.0) / t.P;
     for (int dir = 0; dir < 3; ++dir) {
       const double *s0 = &g.face[2 * dir][(size_t) e * 16 * n2], *s1 = &g.face[2 * dir + 1][(size_t) e * 16 * n2];
       double an[3] = {0, 0, 0}, area = 0.0;

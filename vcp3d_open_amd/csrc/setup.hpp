@@ -11,6 +11,7 @@ namespace vcp {
 constexpr int kNVar = 5;
 constexpr double kGamma = 1.4;            // VCP3D_CurviLinear.hpp:174
 constexpr double kPrandtlLam = 0.72;      // VCP3D_CurviLinear.hpp:175
+constexpr double kPrandtlTurb = 0.90;     // VCP3D_CurviLinear.hpp:176
 constexpr double kRGas = 287.058;         // VCP3D_CurviLinear.hpp:177
 constexpr double kLambdaOverMu = -2.0 / 3.0;
 constexpr double kMuDim = 1.716e-5;       // InputParamClass.cpp:607
@@ -48,7 +49,11 @@ struct CaseParams {
   int monitor_forces = 0;         // "Monitor forces"
   int compute_average = 0;        // "Compute average solution"
   double reference_area = 1.0;    // "Reference area force coefficients"
+  int sgs_model = 0;              // "Subgrid scale model": 0 None, 1 WALE, 2 Vreman
 };
+
+// Subgrid scale models (InputParamClass.cpp:241-250).
+enum : int { kSGSNone = 0, kSGSWale = 1, kSGSVreman = 2 };
 
 struct Tables {
   int P = 0, K = 0, M = 0, Kg = 0;
@@ -57,6 +62,7 @@ struct Tables {
   std::vector<double> A, D;       // M x K: phi_q(r_p), phi'_q(r_p)
   std::vector<double> AT, DT;     // K x M
   std::vector<double> LD;         // K x K: phi_q(r_dof_p)
+  std::vector<double> DLD;        // K x K: phi'_q(r_dof_p)
   std::vector<double> VI;         // inverse of LD
   std::vector<double> f_min, f_max, df_min, df_max;  // K
   std::vector<double> w3, w2, basis_center;
@@ -73,6 +79,9 @@ struct Geometry {
   std::vector<double> vol;        // [nElem][13][nInt]   J, dr/dx..dt/dz, x,y,z
   std::vector<double> face[6];    // [nElem][16][nInt2D] iMin,iMax,jMin,jMax,kMin,kMax
   std::vector<double> lens;       // [nElem][4] len I, J, K, volume
+  std::vector<double> len_les;    // [nElem] LES length scale volume^(1/3) / P
+  std::vector<double> nodal;      // [nElem][10][nDOF] J, dr/dx..dt/dz at the nodal
+                                  // solution DOFs (SGS model only)
   std::vector<double> ave_norm;   // [nElem][9]
   std::vector<int> nbr;           // [nElem][6], -1 physical boundary
 };

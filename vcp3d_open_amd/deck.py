@@ -45,6 +45,8 @@ import numpy as np
 from . import io as vio
 from .cases import GAMMA, MU_DIM, R_GAS
 
+# "Subgrid scale model" (InputParamClass.cpp:241-250) -> vcp_case_desc.sgs_model
+SGS_CODES = {"none": 0, "wale": 1, "vreman": 2}
 BC_CODES = {"periodic1to1trans": 0, "bcoutflownscbc": 1, "bcinflowstaticnscbc": 2,
             "bcinflowtotalnscbc": 3, "bcinflowsubsonic": 4, "bcoutflowsubsonic": 5,
             "bcwallheatflux": 6, "bcwallisothermal": 7, "bcwallinviscid": 8,
@@ -232,6 +234,7 @@ class DeckCase:
         self.theta_sym = prm.number("theta parameter symmetrizing terms", 1.0)
         self.monitor_forces = prm.yes("monitor forces", True)
         self.compute_average = prm.yes("compute average solution", False)
+        self.sgs_model = SGS_CODES[prm.text("subgrid scale model", "none").lower()]
         # InputParamClass.cpp:599: no continuation without averaging
         self.continue_averaging = self.compute_average and prm.yes("continue averaging from restart", False)
         self.reference_area = prm.number("reference area force coefficients", 1.0)
@@ -257,8 +260,9 @@ class DeckCase:
         for key, val in need.items():
             if prm.text(key, "").lower() != val:
                 raise DeckError(f'"{key}": only {val} is supported on this path')
-        if prm.text("subgrid scale model", "none").lower() != "none":
-            raise DeckError('"Subgrid scale model": only None is supported on this path')
+        if prm.text("subgrid scale model", "none").lower() not in SGS_CODES:
+            # InputParamClass.cpp:241-250
+            raise DeckError('Unknown "Subgrid scale model" specified (None, WALE or Vreman)')
         if prm.text("wall modeling", "no wall model").lower() != "no wall model":
             raise DeckError('"Wall modeling": only "No wall model" is supported on this path')
         fb = prm.vector("body force per unit volume (x,y,z)", (0.0, 0.0, 0.0))
@@ -360,7 +364,7 @@ class DeckCase:
                  theta_sym=self.theta_sym, rel_penalty=self.rel_penalty, cfl=self.cfl,
                  dt_synchr=self.dt_synchr, bc=self.bc, bc_data=self.bc_data,
                  monitor_forces=int(self.monitor_forces), reference_area=self.reference_area,
-                 compute_average=int(self.compute_average),
+                 compute_average=int(self.compute_average), sgs_model=self.sgs_model,
                  free_stream_dir=self.vel_dir if self.vel_dir is not None else (1.0, 0.0, 0.0), **self.nscbc)
         if self.sponge_layer or self.cml:
             d.update(sponge_layer=int(self.sponge_layer), cml=int(self.cml),

@@ -825,7 +825,12 @@ struct VolShape {
   static constexpr int SF0 = 15 * P0, SF = 15 * NI; // fluxes of the first pass / all
   static constexpr int SI = 15 * NR * K;            // backward i-stage output
   static constexpr int BIG = cmax(cmax(2 * SZ + SY, SF), SI + 10 * M * K * K);
-  static constexpr int LDS = (int) sizeof(T) * (2 * M * K + 5 * ND + BIG + 15) + 8 * (45 + 6 * (NT / 64) + 4 * K + 5);
+  // variable stride of the modal state in LDS: congruent to K^2 modulo the
+  // bank count in elements, so that the (variable, j, i) pencil items of a
+  // wave run on across a variable boundary without bank conflicts
+  static constexpr int MODB = 256 / (int) sizeof(T);
+  static constexpr int SUV = ND + ((K * K - ND) % MODB + MODB) % MODB;
+  static constexpr int LDS = (int) sizeof(T) * (2 * M * K + 4 * SUV + ND + BIG + 15) + 8 * (45 + 6 * (NT / 64) + 4 * K + 5);
   static constexpr int WGS = (160 * 1024) / LDS;   // workgroups per CU by LDS
   // FP32 contractions: the smaller LDS allows more waves per SIMD
 #ifndef VCP_MIXED_CG_TOL
@@ -840,6 +845,8 @@ struct VolShape {
   static_assert(WGS >= 1, "volume kernel LDS exceeds 160 KB");
   // mass-matrix solve buffers cz, cy, pI, by inside sBig
   static_assert(SZ + 5 * M * M * K + 5 * NI + 5 * M * K * K <= BIG, "CG buffers");
+  // A p in the padded state layout runs from cz into the (then dead) cy
+  static_assert(4 * SUV + ND <= SZ + 5 * M * M * K, "A p buffer");
 };
 
 // blockIdx.x is the element, or (BATCH) a work item (case, element) of the
@@ -865,7 +872,8 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
   // the first pass's fluxes must leave the y-stage rows of the second pass intact
   static_assert(P1 == 0 || SF0 <= 2 * SZ + 15 * R0 * K, "flux / y-stage overlap");
   __shared__ T sA[M * K], sD[M * K];
-  __shared__ T sU[5 * ND];
+  constexpr int SUV = S::SUV;
+  __shared__ T sU[4 * SUV + ND];
   __shared__ T sBig[BIG];
   __shared__ double sCm[5][9];
   __shared__ double sRed[2 * (NT / 64) * 3];
@@ -897,9 +905,9 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
     if constexpr (kMixed) {
       const double u = Ue[i];
       if (i % ND == 0) sMean[i / ND] = u;
-      sU[i] = i % ND == 0 ? T(0) : T(u);
+      sU[(i / ND) * SUV + i % ND] = i % ND == 0 ? T(0) : T(u);
     } else {
-      sU[i] = Ue[i];
+      sU[(i / ND) * SUV + i % ND] = Ue[i];
     }
   }
   if (args.bug)
@@ -914,7 +922,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
     const int v = pc / (K * K), ji = pc - v * K * K;
     T x[K];
 #pragma unroll
-    for (int q = 0; q < K; ++q) x[q] = sU[(v * K + q) * K * K + ji];
+    for (int q = 0; q < K; ++q) x[q] = sU[v * SUV + q * K * K + ji];
 #pragma unroll
     for (int n = 0; n < M; ++n) {
       T sa = 0, sd = 0;
@@ -941,7 +949,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
       }
       b *= bj * bk;
 #pragma unroll
-      for (int v = 0; v < 5; ++v) Vc[v] += b * sU[v * ND + i];
+      for (int v = 0; v < 5; ++v) Vc[v] += b * sU[v * SUV + i];
     }
 #pragma unroll
     for (int v = 0; v < 5; ++v) Vc[v] = wave_sum(Vc[v]);
@@ -1202,7 +1210,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
       T s = 0;
 #pragma unroll
       for (int q = 0; q < M; ++q) s += A[q * K + n] * y0[q] + D[q * K + n] * y2[q];
-      sU[(v * K + n) * K * K + ji] = s;
+      sU[v * SUV + n * K * K + ji] = s;
     }
   }
   __syncthreads();
@@ -1247,7 +1255,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
         T s = 0;
 #pragma unroll
         for (int q = 0; q < M; ++q) s += A[q * K + n] * xx[q];
-        sU[(v * K + n) * K * K + ji] += s;
+        sU[v * SUV + n * K * K + ji] += s;
       }
     }
     __syncthreads();
@@ -1261,7 +1269,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
   if constexpr (kMixed) {
 #pragma unroll
     for (int l = 0; l < 5; ++l) {
-      if (tid < ND) racc[l] += double(sU[l * ND + tid]);
+      if (tid < ND) racc[l] += double(sU[l * SUV + tid]);
       bad = bad || !isfinite(racc[l]);
       r[l] = T(racc[l]);
     }
@@ -1269,7 +1277,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
     if (tid < ND)
 #pragma unroll
       for (int l = 0; l < 5; ++l) {
-        racc[l] += sU[l * ND + tid];
+        racc[l] += sU[l * SUV + tid];
         bad = bad || !isfinite(racc[l]);
       }
 #pragma unroll
@@ -1322,13 +1330,13 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
       if (it == 100) { atomicOr(a.flags, kFlagCGNotConverged); break; }
       if (own)
 #pragma unroll
-        for (int l = 0; l < 5; ++l) sU[l * ND + tid] = pv[l];
+        for (int l = 0; l < 5; ++l) sU[l * SUV + tid] = pv[l];
       __syncthreads();
       for (int pc = rot(FK); pc < 5 * K * K; pc += NT) {
         const int v = pc / (K * K), ji = pc - v * K * K;
         T xx[K];
 #pragma unroll
-        for (int q = 0; q < K; ++q) xx[q] = sU[(v * K + q) * K * K + ji];
+        for (int q = 0; q < K; ++q) xx[q] = sU[v * SUV + q * K * K + ji];
 #pragma unroll
         for (int n = 0; n < M; ++n) {
           T s = 0;
@@ -1339,7 +1347,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
       }
       __syncthreads();
       for (int pc = rot(1); pc < 5 * M * K; pc += NT) {
-        const int vk = pc / K, i = pc - vk * K;
+        const int i = pc / (5 * M), vk = pc - i * (5 * M);
         T xx[K];
 #pragma unroll
         for (int q = 0; q < K; ++q) xx[q] = cz[(vk * K + q) * K + i];
@@ -1405,7 +1413,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
       }
       __syncthreads();
       for (int pc = rot(2); pc < 5 * M * K; pc += NT) {
-        const int vk = pc / K, i = pc - vk * K;
+        const int i = pc / (5 * M), vk = pc - i * (5 * M);
         T xx[M];
 #pragma unroll
         for (int q = 0; q < M; ++q) xx[q] = pI[(vk * M + q) * M + i];
@@ -1430,7 +1438,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
           T s = 0;
 #pragma unroll
           for (int q = 0; q < M; ++q) s += A[q * K + n] * xx[q];
-          const int idx = (v * K + n) * K * K + ji;
+          const int idx = v * SUV + n * K * K + ji;
           cz[idx] = s;
           pAp += double(sU[idx]) * double(s);
         }
@@ -1445,7 +1453,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
       if (own) {
         T ap[5];
 #pragma unroll
-        for (int l = 0; l < 5; ++l) ap[l] = cz[l * ND + tid];
+        for (int l = 0; l < 5; ++l) ap[l] = cz[l * SUV + tid];
         alpha = rz / pAp;
         const T alphaT = T(alpha);
 #pragma unroll

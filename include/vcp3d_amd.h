@@ -220,6 +220,17 @@ int vcp_set_averages(vcp_ctx *ctx, const double *avg, int n_steps);
 
 /* ElementClass::ComputeTimeStep of every element (ElementClass.cpp:1522-1644). */
 int vcp_time_steps(vcp_ctx *ctx, double *dt_elem);
+
+/* No counterpart in the reference: the metric terms vcp_create computed on the
+   device (ElementClass::ComputeMetricTerms, ElementClass.cpp:1027-1407;
+   FinalMetric{I,J,K}Face, :1907-2088; ComputeLengthScales, :927-1022), for
+   checks: vgeo [nElem][10][nInt] (J, dr/dx .. dt/dz at the integration
+   points), fgeo [nElem][3][22][nInt2D] (per element and direction: unit
+   normal and area factor of the min face seen from the neighbour below, that
+   neighbour's inverse metric at its max face, the element's at its min face),
+   econst [nElem][16] (length scales i, j, k, volume, averaged normals [3][3],
+   LES length scale); any pointer may be NULL. */
+int vcp_get_geometry(vcp_ctx *ctx, double *vgeo, double *fgeo, double *econst);
 /* Iterations of the last per-element mass-matrix solve
    (CG_EntropyVarMassMatrix, CG_MassMatrix.cpp:250-504). */
 int vcp_cg_iterations(vcp_ctx *ctx, int *iters);

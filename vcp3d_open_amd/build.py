@@ -11,7 +11,10 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libvcp3d_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["kernels.hip", "setup.cpp", "capi.cpp", "batch.cpp"]
+SOURCES = ["kernels.hip", "geometry.hip", "setup.cpp", "capi.cpp", "batch.cpp"]
+# per-source flags: the device metric terms repeat the host set-up's
+# arithmetic exactly, so neither side contracts multiply-adds
+EXTRA = {"geometry.hip": ["-ffp-contract=off"], "setup.cpp": ["-ffp-contract=off"]}
 COMMON = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=gfx950",
           # no packed FP32 instructions anywhere in device code
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -25,7 +28,7 @@ def build(verbose: bool = False) -> str:
     procs = []
     for src in SOURCES:
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
-        cmd = [HIPCC, *COMMON, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC, *COMMON, *EXTRA.get(src, []), "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))

@@ -2,6 +2,7 @@
 // buffers and the host sequencing of SolverClass::Residual and
 // SolverClass::ExecuteTimeSynchronizationStep.
 #include "ctx.hpp"
+#include "geometry.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -162,6 +163,47 @@ void fill_monitoring(const vcp_ctx *c, double md[8], bool mon, const double *sc)
   md[7] = mon ? (c->dc.sgs ? sc[vcp::kScalEddy] : 0.0) : -1.0;
 }
 
+// Metric terms of every element on the device (launch_geometry) into the
+// context's vgeo, fgeo and econst: the grid and the tables are uploaded to
+// temporary buffers, freed before returning; the LES length scale
+// volume^(1/3) / P of an SGS case is set on the host from the volumes.
+int device_geometry(vcp_ctx *c, const vcp::CaseParams &p, const double *elem_grid)
+{
+  vcp::DeviceCase &dc = c->dc;
+  const int Kg = p.n_poly_grid + 1, nG = Kg * Kg * Kg;
+  const std::vector<double> tab = vcp::geometry_tables(c->tab);
+  double *dGrid = nullptr, *dTab = nullptr;
+  int *dFlag = nullptr, flag = 0;
+  const size_t nGrid = (size_t) dc.nElem * 3 * nG;
+  hipError_t he = hipMalloc((void **) &dGrid, nGrid * sizeof(double));
+  if (he == hipSuccess) he = hipMalloc((void **) &dTab, tab.size() * sizeof(double));
+  if (he == hipSuccess) he = hipMalloc((void **) &dFlag, sizeof(int));
+  if (he == hipSuccess) he = hipMemcpy(dGrid, elem_grid, nGrid * sizeof(double), hipMemcpyHostToDevice);
+  if (he == hipSuccess) he = hipMemcpy(dTab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (he == hipSuccess) he = hipMemset(dFlag, 0, sizeof(int));
+  if (he == hipSuccess)
+    he = hipMemset(dc.fgeo, 0, (size_t) dc.nElem * 3 * vcp::kFaceGeo * dc.nInt2D * sizeof(double));
+  if (he == hipSuccess) {
+    vcp::GeoArgs a{dGrid, dTab, dc.nbr, dc.vgeo, dc.fgeo, dc.econst, dFlag, dc.nElem, Kg};
+    he = vcp::launch_geometry(a, dc.M, c->stream);
+  }
+  if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+  if (he == hipSuccess) he = hipMemcpy(&flag, dFlag, sizeof(int), hipMemcpyDeviceToHost);
+  if (dGrid) (void) hipFree(dGrid);
+  if (dTab) (void) hipFree(dTab);
+  if (dFlag) (void) hipFree(dFlag);
+  if (he != hipSuccess) return fail(std::string("device metric terms: ") + hipGetErrorString(he), -3);
+  if (flag) return fail("Negative or zero Jacobians encountered");
+  if (dc.sgs) {
+    std::vector<double> ec((size_t) dc.nElem * vcp::kElemConst);
+    HIP_TRY(hipMemcpy(ec.data(), dc.econst, ec.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int e = 0; e < dc.nElem; ++e)
+      ec[(size_t) e * vcp::kElemConst + vcp::kElemLenLES] = std::pow(ec[(size_t) e * vcp::kElemConst + 3], 1.0 / 3.0) / p.n_poly_sol;
+    HIP_TRY(hipMemcpy(dc.econst, ec.data(), ec.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
 }  // namespace
 
 namespace vcp {
@@ -205,8 +247,11 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
   vcp_ctx *c = holder.get();
   c->prm = p;
   vcp::build_tables(p, c->tab);
+  // host: the spacing check, neighbours and the metric terms of the elements
+  // the boundary and sponge set-ups read; the device computes all elements'
+  // metric terms below
   vcp::Geometry g;
-  const std::string err = vcp::build_geometry(p, c->tab, elem_grid, g);
+  const std::string err = vcp::build_geometry(p, c->tab, elem_grid, g, false);
   if (!err.empty()) return fail(err);
   c->rc = vcp::reference_conditions(p);
   c->dt_sync = p.dt_synchr * c->rc.u_ref;
@@ -262,31 +307,9 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
     if (p.monitor_forces && !p.periodic[f / 2] && vcp::bc_is_wall(p.bc[f])) dc.monitor_forces = 1;
   dc.nSponge = c->sponge.n; dc.cml = p.cml;
 
-  // device layouts of the geometry
   const int nInt = dc.nInt, n2 = dc.nInt2D;
-  std::vector<double> vgeo((size_t) nElem * vcp::kVolGeo * nInt);
-  std::vector<double> fgeo((size_t) nElem * 3 * vcp::kFaceGeo * n2);
-  std::vector<double> econ((size_t) nElem * vcp::kElemConst, 0.0);
-  for (int e = 0; e < nElem; ++e) {
-    std::memcpy(&vgeo[(size_t) e * vcp::kVolGeo * nInt], &g.vol[(size_t) e * 13 * nInt],
-                sizeof(double) * vcp::kVolGeo * nInt);
-    for (int q = 0; q < 4; ++q) econ[(size_t) e * vcp::kElemConst + q] = g.lens[(size_t) e * 4 + q];
-    for (int q = 0; q < 9; ++q) econ[(size_t) e * vcp::kElemConst + 4 + q] = g.ave_norm[(size_t) e * 9 + q];
-    econ[(size_t) e * vcp::kElemConst + vcp::kElemLenLES] = g.len_les[e];
-    for (int dir = 0; dir < 3; ++dir) {
-      const int L = g.nbr[(size_t) e * 6 + 2 * dir];
-      double *dst = &fgeo[((size_t) e * 3 + dir) * vcp::kFaceGeo * n2];
-      const double *mR = &g.face[2 * dir][(size_t) e * 16 * n2];
-      const double *mL = L >= 0 ? &g.face[2 * dir + 1][(size_t) L * 16 * n2] : mR;
-      for (int l = 0; l < n2; ++l) {
-        for (int q = 0; q < 4; ++q) dst[q * n2 + l] = mL[q * n2 + l];
-        for (int q = 0; q < 9; ++q) {
-          dst[(4 + q) * n2 + l] = mL[(4 + q) * n2 + l];
-          dst[(13 + q) * n2 + l] = mR[(4 + q) * n2 + l];
-        }
-      }
-    }
-  }
+  const size_t nVgeo = (size_t) nElem * vcp::kVolGeo * nInt, nFgeo = (size_t) nElem * 3 * vcp::kFaceGeo * n2;
+  const size_t nEcon = (size_t) nElem * vcp::kElemConst;
   hipError_t he = hipStreamCreate(&c->stream);
   if (he == hipSuccess) he = hipHostMalloc((void **) &c->host, (kHostFlags + 1) * sizeof(double), hipHostMallocDefault);
   if (he == hipSuccess) he = hipStreamCreate(&c->side);
@@ -294,9 +317,9 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
   if (he == hipSuccess) he = hipEventCreateWithFlags(&c->evDt, hipEventDisableTiming);
   const size_t ns = (size_t) nElem * 5 * dc.nDOF;
   if (he == hipSuccess) he = c->alloc(&dc.tables, tb.size());
-  if (he == hipSuccess) he = c->alloc(&dc.vgeo, vgeo.size());
-  if (he == hipSuccess) he = c->alloc(&dc.fgeo, fgeo.size());
-  if (he == hipSuccess) he = c->alloc(&dc.econst, econ.size());
+  if (he == hipSuccess) he = c->alloc(&dc.vgeo, nVgeo);
+  if (he == hipSuccess) he = c->alloc(&dc.fgeo, nFgeo);
+  if (he == hipSuccess) he = c->alloc(&dc.econst, nEcon);
   if (he == hipSuccess) he = c->alloc(&dc.nbr, g.nbr.size());
   if (he == hipSuccess) he = c->alloc(&dc.U, ns);
   if (he == hipSuccess) he = c->alloc(&dc.U0, ns);
@@ -347,10 +370,9 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
     if (he == hipSuccess) he = hipMemset(dc.dampState, 0, ns5 * sizeof(double));
   }
   if (he == hipSuccess) he = hipMemcpy(dc.tables, tb.data(), tb.size() * sizeof(double), hipMemcpyHostToDevice);
-  if (he == hipSuccess) he = hipMemcpy(dc.vgeo, vgeo.data(), vgeo.size() * sizeof(double), hipMemcpyHostToDevice);
-  if (he == hipSuccess) he = hipMemcpy(dc.fgeo, fgeo.data(), fgeo.size() * sizeof(double), hipMemcpyHostToDevice);
-  if (he == hipSuccess) he = hipMemcpy(dc.econst, econ.data(), econ.size() * sizeof(double), hipMemcpyHostToDevice);
   if (he == hipSuccess) he = hipMemcpy(dc.nbr, g.nbr.data(), g.nbr.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (he != hipSuccess) return fail(std::string("device allocation/upload: ") + hipGetErrorString(he), -3);
+  if (int rc_ = device_geometry(c, p, elem_grid)) return rc_;
   if (he == hipSuccess) he = hipMemset(dc.U, 0, ns * sizeof(double));
   if (he == hipSuccess) he = hipMemset(dc.R, 0, ns * sizeof(double));
   if (he == hipSuccess) he = hipMemset(dc.faceRes, 0, nslot * sizeof(double));
@@ -596,6 +618,22 @@ int vcp_residual_static(vcp_ctx *c, int mon, double md[8])
   if ((rc = check_flags(c))) return rc;
   if (md) fill_monitoring(c, md, mon != 0, sc);
   c->res_valid = true;
+  return 0;
+  VCP_GUARD_END
+}
+
+int vcp_get_geometry(vcp_ctx *c, double *vgeo, double *fgeo, double *econst)
+{
+  VCP_GUARD_BEGIN
+  if (!c) return fail("null context");
+  const vcp::DeviceCase &dc = c->dc;
+  if (vgeo)
+    HIP_TRY(hipMemcpy(vgeo, dc.vgeo, (size_t) dc.nElem * vcp::kVolGeo * dc.nInt * sizeof(double), hipMemcpyDeviceToHost));
+  if (fgeo)
+    HIP_TRY(hipMemcpy(fgeo, dc.fgeo, (size_t) dc.nElem * 3 * vcp::kFaceGeo * dc.nInt2D * sizeof(double),
+                      hipMemcpyDeviceToHost));
+  if (econst)
+    HIP_TRY(hipMemcpy(econst, dc.econst, (size_t) dc.nElem * vcp::kElemConst * sizeof(double), hipMemcpyDeviceToHost));
   return 0;
   VCP_GUARD_END
 }

@@ -339,7 +339,35 @@ std::string check_case(const CaseParams &p)
   return "";
 }
 
-std::string build_geometry(const CaseParams &p, Tables &t, const double *grid, Geometry &g)
+// Lagrange tables of the grid DOFs: values and derivatives at the Gauss
+// points [M][Kg], the face vectors (0/1) and derivatives at r = -1, +1 [Kg].
+static void grid_lagrange(const Tables &t, std::vector<double> &Lg, std::vector<double> &dLg,
+                          std::vector<double> lf[2], std::vector<double> dlf[2])
+{
+  const int M = t.M, Kg = t.Kg;
+  Lg.assign(M * Kg, 0.0); dLg.assign(M * Kg, 0.0);
+  for (int pt = 0; pt < M; ++pt) lagrange_basis(t.r_grid, t.r_int[pt], &Lg[pt * Kg], &dLg[pt * Kg]);
+  for (int s = 0; s < 2; ++s) {
+    lf[s].assign(Kg, 0.0); dlf[s].assign(Kg, 0.0);
+    lf[s][s ? Kg - 1 : 0] = 1.0;
+    lagrange_basis(t.r_grid, s ? 1.0 : -1.0, nullptr, dlf[s].data());
+  }
+}
+
+std::vector<double> geometry_tables(const Tables &t)
+{
+  std::vector<double> Lg, dLg, lf[2], dlf[2];
+  grid_lagrange(t, Lg, dLg, lf, dlf);
+  std::vector<double> out(Lg);
+  out.insert(out.end(), dLg.begin(), dLg.end());
+  for (int s = 0; s < 2; ++s) out.insert(out.end(), lf[s].begin(), lf[s].end());
+  for (int s = 0; s < 2; ++s) out.insert(out.end(), dlf[s].begin(), dlf[s].end());
+  out.insert(out.end(), t.w2.begin(), t.w2.end());
+  out.insert(out.end(), t.w3.begin(), t.w3.end());
+  return out;
+}
+
+std::string build_geometry(const CaseParams &p, Tables &t, const double *grid, Geometry &g, bool all)
 {
   const int ni = p.n_elem[0], nj = p.n_elem[1], nk = p.n_elem[2];
   const int nElem = ni * nj * nk, M = t.M, Kg = t.Kg, nG = Kg * Kg * Kg;
@@ -365,15 +393,45 @@ std::string build_geometry(const CaseParams &p, Tables &t, const double *grid, G
   }
   if (nL && nE) return "Elements with both equidistant and LGL spacing encountered";
   t.r_grid = edge_points(Kg, nL ? 2 : 1);
-  std::vector<double> Lg(M * Kg), dLg(M * Kg), lf[2], dlf[2];
-  for (int pt = 0; pt < M; ++pt) lagrange_basis(t.r_grid, t.r_int[pt], &Lg[pt * Kg], &dLg[pt * Kg]);
-  for (int s = 0; s < 2; ++s) {
-    lf[s].assign(Kg, 0.0); dlf[s].assign(Kg, 0.0);
-    lf[s][s ? Kg - 1 : 0] = 1.0;
-    lagrange_basis(t.r_grid, s ? 1.0 : -1.0, nullptr, dlf[s].data());
-  }
-  g.vol.assign((size_t) nElem * 13 * nInt, 0.0);
-  for (int f = 0; f < 6; ++f) g.face[f].assign((size_t) nElem * 16 * n2, 0.0);
+  std::vector<double> Lg, dLg, lf[2], dlf[2];
+  grid_lagrange(t, Lg, dLg, lf, dlf);
+  // neighbours; a periodic block face aliases the opposite owned element
+  g.nbr.assign((size_t) nElem * 6, -1);
+  const int n[3] = {ni, nj, nk};
+  for (int k = 0; k < nk; ++k)
+    for (int j = 0; j < nj; ++j)
+      for (int i = 0; i < ni; ++i) {
+        const int e = i + ni * (j + nj * k);
+        for (int d = 0; d < 3; ++d)
+          for (int s = 0; s < 2; ++s) {
+            int idx[3] = {i, j, k};
+            idx[d] += s ? 1 : -1;
+            if (idx[d] < 0 || idx[d] >= n[d]) {
+              if (!p.periodic[d]) continue;
+              idx[d] = (idx[d] + n[d]) % n[d];
+            }
+            g.nbr[(size_t) e * 6 + 2 * d + s] = idx[0] + ni * (idx[1] + nj * idx[2]);
+          }
+      }
+  // elements whose host metric terms are needed: all, or (the device
+  // computes the rest) those with a physical face (open-boundary set-up) or
+  // in a sponge layer (damping functions at the point coordinates)
+  std::vector<char> need(nElem, all ? 1 : 0);
+  int nNeed = all ? nElem : 0;
+  if (!all)
+    for (int e = 0; e < nElem; ++e) {
+      bool b = false;
+      for (int f = 0; f < 6; ++f) b = b || g.nbr[(size_t) e * 6 + f] < 0;
+      if (p.sponge_layer) {
+        const int idx[3] = {e % ni, (e / ni) % nj, e / (ni * nj)};
+        for (int d = 0; d < 3; ++d)
+          b = b || idx[d] < p.sponge_elems[2 * d] || idx[d] >= n[d] - p.sponge_elems[2 * d + 1];
+      }
+      need[e] = b;
+      nNeed += b;
+    }
+  g.vol.assign(nNeed ? (size_t) nElem * 13 * nInt : 0, 0.0);
+  for (int f = 0; f < 6; ++f) g.face[f].assign(nNeed ? (size_t) nElem * 16 * n2 : 0, 0.0);
   g.lens.assign((size_t) nElem * 4, 0.0);
   g.len_les.assign((size_t) nElem, 0.0);
   g.ave_norm.assign((size_t) nElem * 9, 0.0);
@@ -395,6 +453,25 @@ std::string build_geometry(const CaseParams &p, Tables &t, const double *grid, G
   for (int e = e0; e < e1; ++e) {
     if (badJacobian.load(std::memory_order_relaxed)) return;
     const double *ge = grid + (size_t) e * 3 * nG;
+    if (sgs) {
+      for (int c = 0; c < 3; ++c) {
+        apply3(Kg, Kg, Kg, K, K, K, dLgN.data(), LgN.data(), LgN.data(), ge + c * nG, &NR[c * nD]);
+        apply3(Kg, Kg, Kg, K, K, K, LgN.data(), dLgN.data(), LgN.data(), ge + c * nG, &NS[c * nD]);
+        apply3(Kg, Kg, Kg, K, K, K, LgN.data(), LgN.data(), dLgN.data(), ge + c * nG, &NT[c * nD]);
+      }
+      double *vn = &g.nodal[(size_t) e * 10 * nD];
+      for (int l = 0; l < nD; ++l) {
+        const double dr[3] = {NR[l], NR[nD + l], NR[2 * nD + l]};
+        const double ds[3] = {NS[l], NS[nD + l], NS[2 * nD + l]};
+        const double dt[3] = {NT[l], NT[nD + l], NT[2 * nD + l]};
+        double J, m[9];
+        inverse_metric(dr, ds, dt, J, m);
+        if (!(J > 0.0)) { badJacobian.store(1); return; }
+        vn[l] = J;
+        for (int q = 0; q < 9; ++q) vn[(1 + q) * nD + l] = m[q];
+      }
+    }
+    if (!need[e]) continue;
     double *vm = &g.vol[(size_t) e * 13 * nInt];
     for (int c = 0; c < 3; ++c) {
       apply3(Kg, Kg, Kg, M, M, M, Lg.data(), Lg.data(), Lg.data(), ge + c * nG, &X[c * nInt]);
@@ -445,24 +522,6 @@ std::string build_geometry(const CaseParams &p, Tables &t, const double *grid, G
         for (int c = 0; c < 3; ++c) sm[(13 + c) * n2 + l] = C[c * n2 + l];
       }
     }
-    if (sgs) {
-      for (int c = 0; c < 3; ++c) {
-        apply3(Kg, Kg, Kg, K, K, K, dLgN.data(), LgN.data(), LgN.data(), ge + c * nG, &NR[c * nD]);
-        apply3(Kg, Kg, Kg, K, K, K, LgN.data(), dLgN.data(), LgN.data(), ge + c * nG, &NS[c * nD]);
-        apply3(Kg, Kg, Kg, K, K, K, LgN.data(), LgN.data(), dLgN.data(), ge + c * nG, &NT[c * nD]);
-      }
-      double *vn = &g.nodal[(size_t) e * 10 * nD];
-      for (int l = 0; l < nD; ++l) {
-        const double dr[3] = {NR[l], NR[nD + l], NR[2 * nD + l]};
-        const double ds[3] = {NS[l], NS[nD + l], NS[2 * nD + l]};
-        const double dt[3] = {NT[l], NT[nD + l], NT[2 * nD + l]};
-        double J, m[9];
-        inverse_metric(dr, ds, dt, J, m);
-        if (!(J > 0.0)) { badJacobian.store(1); return; }
-        vn[l] = J;
-        for (int q = 0; q < 9; ++q) vn[(1 + q) * nD + l] = m[q];
-      }
-    }
     double vol = 0.0;
     for (int l = 0; l < nInt; ++l) vol += t.w3[l] * vm[l];
     g.lens[(size_t) e * 4 + 3] = vol;
@@ -483,24 +542,6 @@ std::string build_geometry(const CaseParams &p, Tables &t, const double *grid, G
   };
   in_parallel(elements);
   if (badJacobian.load()) return "Negative or zero Jacobians encountered";
-  // neighbours; a periodic block face aliases the opposite owned element
-  g.nbr.assign((size_t) nElem * 6, -1);
-  const int n[3] = {ni, nj, nk};
-  for (int k = 0; k < nk; ++k)
-    for (int j = 0; j < nj; ++j)
-      for (int i = 0; i < ni; ++i) {
-        const int e = i + ni * (j + nj * k);
-        for (int d = 0; d < 3; ++d)
-          for (int s = 0; s < 2; ++s) {
-            int idx[3] = {i, j, k};
-            idx[d] += s ? 1 : -1;
-            if (idx[d] < 0 || idx[d] >= n[d]) {
-              if (!p.periodic[d]) continue;
-              idx[d] = (idx[d] + n[d]) % n[d];
-            }
-            g.nbr[(size_t) e * 6 + 2 * d + s] = idx[0] + ni * (idx[1] + nj * idx[2]);
-          }
-      }
   return "";
 }
 

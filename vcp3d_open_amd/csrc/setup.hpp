@@ -112,8 +112,16 @@ void build_tables(const CaseParams &p, Tables &t);
 // Consistency of the boundary, NSCBC and sponge parameters ("" if valid).
 std::string check_case(const CaseParams &p);
 // Computes metrics from per-element grid DOFs [nElem][3][Kg^3]; returns an
-// error message or "" on success.
-std::string build_geometry(const CaseParams &p, Tables &t, const double *elem_grid, Geometry &g);
+// error message or "" on success. all = false: the volume and face metric
+// terms only of the elements the open-boundary and sponge set-ups read (those
+// with a physical face or in a sponge layer; the arrays stay empty when there
+// are none), for a caller that computes the rest on the device
+// (launch_geometry); the spacing check, the grid nodes, the neighbours and
+// the SGS nodal metric as always.
+std::string build_geometry(const CaseParams &p, Tables &t, const double *elem_grid, Geometry &g, bool all = true);
+// Device table block of launch_geometry (after build_geometry): Lg, dLg
+// [M][Kg], lf, dlf [2][Kg], w2 [M^2], w3 [M^3].
+std::vector<double> geometry_tables(const Tables &t);
 RefConditions reference_conditions(const CaseParams &p);
 std::string setup_open_boundary(const CaseParams &p, const Tables &t, const Geometry &g,
                                 const RefConditions &rc, OpenBoundary &ob);

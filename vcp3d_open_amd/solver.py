@@ -51,7 +51,7 @@ EXPORTS = ["vcp_create", "vcp_destroy", "vcp_sizes", "vcp_reference", "vcp_set_r
            "vcp_set_damping_state", "vcp_nscbc_average",
            "vcp_set_solution", "vcp_get_solution", "vcp_get_nodal", "vcp_residual",
            "vcp_get_residual", "vcp_sync_step", "vcp_set_graph", "vcp_get_averages", "vcp_set_averages",
-           "vcp_time_steps", "vcp_cg_iterations",
+           "vcp_time_steps", "vcp_get_geometry", "vcp_cg_iterations",
            "vcp_profile_residual", "vcp_host_setup", "vcp_set_tile", "vcp_volume_tiles",
            "vcp_batch_create", "vcp_batch_destroy", "vcp_batch_add", "vcp_batch_remove",
            "vcp_batch_step", "vcp_batch_wait", "vcp_batch_status", "vcp_batch_set_graph",
@@ -90,6 +90,7 @@ def load_library(path: str = LIB_PATH):
     L.vcp_get_averages.argtypes = [vp, dp, C.POINTER(C.c_int)]
     L.vcp_set_averages.argtypes = [vp, dp, C.c_int]
     L.vcp_time_steps.argtypes = [vp, dp]
+    L.vcp_get_geometry.argtypes = [vp, dp, dp, dp]
     L.vcp_cg_iterations.argtypes = [vp, ip]
     L.vcp_profile_residual.argtypes = [vp, C.c_int, dp]
     L.vcp_host_setup.argtypes = [C.POINTER(CaseDesc), dp, dp, dp, dp, ip]
@@ -274,6 +275,16 @@ class SolverClass:
         out = np.zeros(self.n_elem)
         self._check(self._L.vcp_time_steps(self.h, out))
         return out
+
+    def device_geometry(self) -> dict:
+        """The metric terms computed on the device at creation: vgeo
+        [nElem][10][M^3], fgeo [nElem][3][22][M^2], econst [nElem][16]."""
+        M = self.M
+        vg = np.zeros((self.n_elem, 10, M ** 3))
+        fg = np.zeros((self.n_elem, 3, 22, M * M))
+        ec = np.zeros((self.n_elem, 16))
+        self._check(self._L.vcp_get_geometry(self.h, vg, fg, ec))
+        return dict(vgeo=vg, fgeo=fg, econst=ec)
 
     def cg_iterations(self) -> np.ndarray:
         out = np.zeros(self.n_elem, dtype=np.int32)

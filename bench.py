@@ -387,6 +387,41 @@ def bench_serve(args) -> int:
     return 0
 
 
+def bench_interpolation(args) -> int:
+    """The mass-matrix solve's forward interpolation (5 variables, K^3 -> M^3,
+    K = 5, M = 7) over 32768 synthetic elements, 20 times per launch, as FP64
+    VALU pencils (the volume kernel's form) and as 16x16x4 FP64 MFMA tiles
+    (libvcp3d_ab.so); both launch times, the rate in useful FLOP/s and the
+    relative difference of the two outputs."""
+    import ctypes as C
+    import os
+    lib = C.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)), "vcp3d_open_amd", "libvcp3d_ab.so"))
+    lib.vcp_ab_interpolation.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double)]
+    n_elem, inner, reps = 32768, 20, max(args.steps, 3)
+    K, M = 5, 7
+    flop = 2.0 * 5 * (K * K * K * M + K * K * M * M + K * M * M * M) * n_elem * inner   # useful
+    res = {}
+    for impl, name in ((0, "valu"), (1, "mfma")):
+        ms, tot = C.c_double(), C.c_double()
+        rc = lib.vcp_ab_interpolation(impl, n_elem, inner, reps, C.byref(ms), C.byref(tot))
+        if rc != 0:
+            print(json.dumps({"error": f"vcp_ab_interpolation({name}) failed: {rc}"}))
+            return 1
+        res[name] = (ms.value, tot.value)
+    rate = {k: flop / (v[0] * 1e-3) / 1e12 for k, v in res.items()}
+    out = {"metric": "mass_matrix_interpolation_tflops", "value": rate["valu"], "unit": "TFLOP/s",
+           "higher_is_better": True, "n_gpus": 1, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"forward K^3->M^3 interpolation of 5 variables, K=5, M=7, {n_elem} elements "
+                                  f"x {inner} per launch, {reps} launches"},
+           "valu_ms": res["valu"][0], "mfma_ms": res["mfma"][0],
+           "valu_tflops": rate["valu"], "mfma_tflops": rate["mfma"],
+           "peak_fp64_tflops": PEAK_FP64_TFLOPS,
+           "rel_diff": abs(res["mfma"][1] - res["valu"][1]) / abs(res["valu"][1])}
+    print(json.dumps(out))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -399,7 +434,7 @@ def main() -> int:
                     help="also run one device-to-device copy of this size (HBM counter calibration)")
     ap.add_argument("--rk-steps", type=int, default=200, help="serve: RK steps per request")
     ap.add_argument("--tick", type=int, default=8, help="serve: batched RK steps between status reads")
-    ap.add_argument("--workload", choices=["tgv", "openbox", "batch", "serve"], default="tgv",
+    ap.add_argument("--workload", choices=["tgv", "openbox", "batch", "serve", "interpolation"], default="tgv",
                     help="tgv: curvilinear Taylor-Green 32^3 P=4 (default); openbox: 48x24x24 P=4 "
                          "PC-NSCBC in/outflow, 8-element sponge + CML, M=0.5; batch: 64 seeded "
                          "Taylor-Green cases of uneven P and size stepped together")
@@ -426,6 +461,8 @@ def main() -> int:
         return bench_batch(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "interpolation":
+        return bench_interpolation(args)
 
     from vcp3d_open_amd.cases import OpenBoxCase, TGVCase
     from vcp3d_open_amd.solver import SolverClass

@@ -1,4 +1,6 @@
-"""Builds the in-tree gfx950 library vcp3d_open_amd/libvcp3d_amd.so with hipcc."""
+"""Builds the in-tree gfx950 libraries with hipcc: vcp3d_open_amd/libvcp3d_amd.so
+(the path) and vcp3d_open_amd/libvcp3d_ab.so (the VALU / MFMA interpolation
+measurement that bench.py --workload interpolation runs)."""
 from __future__ import annotations
 
 import os
@@ -9,9 +11,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libvcp3d_amd.so")
+AB_LIB = os.path.join(HERE, "libvcp3d_ab.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["kernels.hip", "geometry.hip", "setup.cpp", "capi.cpp", "batch.cpp"]
+SOURCES = ["kernels.hip", "geometry.hip", "setup.cpp", "capi.cpp", "batch.cpp", "contraction_ab.hip"]
+AB_SOURCES = ["contraction_ab.hip"]
 # per-source flags: the device metric terms repeat the host set-up's
 # arithmetic exactly, so neither side contracts multiply-adds
 EXTRA = {"geometry.hip": ["-ffp-contract=off"], "setup.cpp": ["-ffp-contract=off"]}
@@ -43,10 +47,12 @@ def build(verbose: bool = False) -> str:
             sys.stderr.write(out)
     if failed:
         raise RuntimeError("hipcc failed for " + ", ".join(failed))
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
-    subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB)
+    for lib, srcs in ((LIB, [s for s in SOURCES if s not in AB_SOURCES]), (AB_LIB, AB_SOURCES)):
+        tmp = lib + ".tmp"
+        objs_l = [os.path.join(objdir, os.path.splitext(s)[0] + ".o") for s in srcs]
+        cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs_l]
+        subprocess.run(cmd, check=True)
+        os.replace(tmp, lib)
     return LIB
 
 

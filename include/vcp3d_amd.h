@@ -217,6 +217,14 @@ int vcp_set_graph(vcp_ctx *ctx, int enable);
    SolverClass.cpp:3227-3240). */
 int vcp_get_averages(vcp_ctx *ctx, double *avg, int *n_steps);
 int vcp_set_averages(vcp_ctx *ctx, const double *avg, int n_steps);
+/* The time-averaged force coefficients (SolverClass::mCfxAve, mCfyAve,
+   mCfzAve, mCfxVisAve, mCfyVisAve, mCfzVisAve) of a case created with
+   compute_average: updated from monitoring[0..5] after every averaged
+   synchronization step (SolverClass.cpp:3858-3869), written as the first six
+   restart meta data doubles (SolverClass.cpp:4407-4409) and read back from
+   them when averaging continues from a restart (SolverClass.cpp:3236-3249). */
+int vcp_get_force_averages(vcp_ctx *ctx, double cf[6]);
+int vcp_set_force_averages(vcp_ctx *ctx, const double cf[6]);
 
 /* ElementClass::ComputeTimeStep of every element (ElementClass.cpp:1522-1644). */
 int vcp_time_steps(vcp_ctx *ctx, double *dt_elem);

@@ -106,21 +106,33 @@ def test_checkpoint_and_resume_with_averages(lib, tmp_path):
     deck = os.path.join(src, "extrapolation", "start.cfg")
     a, case = open_solver(deck)
     b, _ = open_solver(deck)
-    for _ in range(3):
-        a.ExecuteTimeSynchronizationStep()
+    md_a = [a.ExecuteTimeSynchronizationStep()[:6] for _ in range(3)]
     for _ in range(2):
         b.ExecuteTimeSynchronizationStep()
+    # the running mean with weights 1/n of each step's force coefficients is
+    # their arithmetic mean (the duct's four walls give nonzero coefficients)
+    cf_a = a.force_averages()
+    assert np.abs(cf_a).max() > 0.0
+    np.testing.assert_allclose(cf_a, np.mean(md_a, axis=0), rtol=1e-13, atol=1e-14 * np.abs(md_a).max())
     (tmp_path / "grid").mkdir()
     (tmp_path / "grid" / "Box_nPoly5_15x8x8.xyz").write_bytes(open(case.grid_file, "rb").read())
     run = tmp_path / "extrapolation"
     (run / "init").mkdir(parents=True)
     write_restart(b, case, str(run / "init" / "Restart.dat"), 2)
+    # the restart's meta data carry b's averaged force coefficients
+    from vcp3d_open_amd import io as vio
+    *_, meta = vio.read_su2_restart(str(run / "init" / "Restart.dat"), case.n_dofs, with_meta=True)
+    assert np.array_equal(meta[:6], b.force_averages()) and not meta[6:].any()
     text = open(deck).read().replace("Restart: no", "Restart: yes")
     text = text.replace("Continue averaging from restart: no", "Continue averaging from restart: yes")
     (run / "start.cfg").write_text(text)
     c, c_case = open_solver(str(run / "start.cfg"))
     assert c_case.continue_averaging
+    assert np.array_equal(c.force_averages(), meta[:6])
     c.ExecuteTimeSynchronizationStep()
+    # continued, not restarted from zero: the mean over all three steps (the
+    # resumed state agrees to 1e-12, the coefficients to rounding of that)
+    np.testing.assert_allclose(c.force_averages(), cf_a, rtol=1e-9, atol=1e-10 * np.abs(md_a).max())
     assert rel(c.WriteRestartSolution(), a.WriteRestartSolution()) < 1e-12
     avg_a, n_a = a.averages()
     avg_c, n_c = c.averages()

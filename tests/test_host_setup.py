@@ -67,6 +67,14 @@ def test_su2_restart_round_trip(tmp_path):
     assert np.array_equal(data[:, :, 3:], cons)
     hdr = np.fromfile(path, dtype=np.int32, count=5)
     assert list(hdr) == [535532, 8, case.n_elems * case.n_dofs, 1, 8]
+    # meta data: zeros by default, the averaged force coefficients when given
+    assert not vio.read_su2_restart(path, case.n_dofs, with_meta=True)[3].any()
+    cf = np.array([0.1, -0.2, 3e-12, 0.04, 5e-3, -6e-4])
+    vio.write_su2_restart(path, case.nodal_coords(), cons, time_step=7, meta=cf)
+    *_, step, meta = vio.read_su2_restart(path, case.n_dofs, with_meta=True)
+    assert step == 7 and np.array_equal(meta[:6], cf) and not meta[6:].any()
+    # the meta data are the last 64 bytes of the file, after the step
+    assert np.array_equal(np.frombuffer(open(path, "rb").read()[-64:], dtype="<f8"), np.r_[cf, 0.0, 0.0])
 
 
 def test_taylor_green_initial_state():

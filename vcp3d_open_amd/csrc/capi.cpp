@@ -703,6 +703,12 @@ int vcp_sync_step(vcp_ctx *c, double md[8])
       if ((rc = flag_error(flags & ~vcp::kFlagNegativeA2))) return rc;
       c->dt_ready = !(flags & vcp::kFlagNegativeA2);
       fill_monitoring(c, md, true, c->host);
+      if (c->dc.avg) {
+        // averaged force coefficients from the step's last monitoring data
+        // (SolverClass.cpp:3858-3869)
+        const double factNew = 1.0 / c->n_avg, factOld = 1.0 - factNew;
+        for (int i = 0; i < 6; ++i) c->cf_avg[i] = factOld * c->cf_avg[i] + factNew * md[i];
+      }
     }
   }
   c->res_valid = true;
@@ -746,6 +752,26 @@ int vcp_set_averages(vcp_ctx *c, const double *avg, int n_steps)
   HIP_TRY(hipMemcpyAsync(c->dc.avg, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->n_avg = n_steps;
+  return 0;
+  VCP_GUARD_END
+}
+
+int vcp_get_force_averages(vcp_ctx *c, double cf[6])
+{
+  VCP_GUARD_BEGIN
+  if (!c || !cf) return fail("null argument");
+  if (!c->dc.avg) return fail("the case was created without compute_average");
+  for (int i = 0; i < 6; ++i) cf[i] = c->cf_avg[i];
+  return 0;
+  VCP_GUARD_END
+}
+
+int vcp_set_force_averages(vcp_ctx *c, const double cf[6])
+{
+  VCP_GUARD_BEGIN
+  if (!c || !cf) return fail("null argument");
+  if (!c->dc.avg) return fail("the case was created without compute_average");
+  for (int i = 0; i < 6; ++i) c->cf_avg[i] = cf[i];
   return 0;
   VCP_GUARD_END
 }

@@ -63,6 +63,9 @@ struct vcp_ctx {
   double dt_sync = 0.0;
   int time_step = 0;
   int n_avg = 0;                  // synchronization steps in the running averages
+  // running averages of the six force coefficients of monitoring[0..5]
+  // (SolverClass::mCfxAve .. mCfzVisAve)
+  double cf_avg[6] = {};
   // RK steps replayed as captured HIP graphs (vcp_set_graph), one per
   // [state buffer that is current][last step of the call], captured on first use
   bool use_graph = false;

@@ -507,12 +507,13 @@ def open_solver(param_path: str, bug_compat: bool = True, **kw):
     if case.sponge_layer or case.cml:
         s.SetDampingState(case.damping_primitive())
     if case.continue_averaging and case.restart:
-        # the averaged columns and the number of averaged steps of the restart
-        # (SolverClass.cpp:3227-3240, :4401-4414)
-        _, data, n = vio.read_su2_restart(case.restart_file, case.n_dofs)
+        # the averaged columns, the number of averaged steps and the averaged
+        # force coefficients of the restart (SolverClass.cpp:3227-3249, :4401-4414)
+        _, data, n, meta = vio.read_su2_restart(case.restart_file, case.n_dofs, with_meta=True)
         if data.shape[2] < 20:
             raise DeckError("restart file without averaged columns for \"Continue averaging from restart\"")
         s.set_averages(data[:, :, 8:20], n)
+        s.set_force_averages(meta[:6])
     return s, case
 
 
@@ -526,11 +527,13 @@ def write_restart(solver, case, path: str, time_step: int) -> None:
     """SolverClass::WriteRestartSolution (SolverClass.cpp:4375-4804): the
     nodal dimensional conservative state with the DOF coordinates as an SU2
     restart file the reference can read back; with running averages their
-    twelve columns follow and the stored step count is the number of
-    averaged steps, as the reference writes it (SolverClass.cpp:4401-4414)."""
-    extra = names = None
+    twelve columns follow, the stored step count is the number of averaged
+    steps and the meta data hold the averaged force coefficients, as the
+    reference writes them (SolverClass.cpp:4401-4414)."""
+    extra = names = meta = None
     if getattr(case, "compute_average", False):
         extra, time_step = solver.averages()
         names = AVERAGE_NAMES
+        meta = solver.force_averages()
     vio.write_su2_restart(path, case.nodal_coords(), solver.WriteRestartSolution(), time_step=time_step,
-                          extra=extra, extra_names=names)
+                          extra=extra, extra_names=names, meta=meta)

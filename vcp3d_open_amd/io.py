@@ -80,7 +80,8 @@ def block_to_elements(xyz: np.ndarray, n_elem, n_poly_grid: int) -> np.ndarray:
 # ------------------------------------------------------------ SU2 restart
 def write_su2_restart(path: str, coords: np.ndarray, cons: np.ndarray,
                       time_step: int = 0, extra: np.ndarray | None = None,
-                      extra_names: list | None = None) -> None:
+                      extra_names: list | None = None,
+                      meta: np.ndarray | None = None) -> None:
     """Write a restart file.
 
     coords: (nElem, nDOFs, 3) nodal coordinates; cons: (nElem, nDOFs, 5)
@@ -88,6 +89,9 @@ def write_su2_restart(path: str, coords: np.ndarray, cons: np.ndarray,
     extra: optional further columns (nElem, nDOFs, n), e.g. the averaged
     primitive variables, velocity products and eddy viscosity of an averaged
     solution file (SolverClass.hpp:748-809 reads columns 8..12).
+    meta: the 8 meta data doubles after the time step, the first six the
+    time-averaged force coefficients (SolverClass.cpp:4407-4409); zeros when
+    not given.
     """
     n_elem, n_dofs, _ = cons.shape
     names = ["x", "y", "z", "Density", "Momentum_x", "Momentum_y",
@@ -103,11 +107,19 @@ def write_su2_restart(path: str, coords: np.ndarray, cons: np.ndarray,
             f.write(nm.encode().ljust(NAME_LEN, b"\0"))
         f.write(buf.tobytes(order="C"))
         f.write(struct.pack("<i", time_step))
-        f.write(np.zeros(8).tobytes())
+        md = np.zeros(8)
+        if meta is not None:
+            meta = np.asarray(meta, dtype=np.float64).ravel()
+            if meta.size > 8:
+                raise ValueError("at most 8 restart meta data doubles")
+            md[:meta.size] = meta
+        f.write(md.tobytes())
 
 
-def read_su2_restart(path: str, n_dofs: int):
-    """Read a restart file; returns (names, data[nElem, nDOFs, nVar], step)."""
+def read_su2_restart(path: str, n_dofs: int, with_meta: bool = False):
+    """Read a restart file; returns (names, data[nElem, nDOFs, nVar], step),
+    and the 8 meta data doubles as a fourth item with with_meta (zeros when
+    the file ends after the time step)."""
     with open(path, "rb") as f:
         data = f.read()
     hdr = struct.unpack_from("<5i", data, 0)
@@ -126,7 +138,14 @@ def read_su2_restart(path: str, n_dofs: int):
     arr = arr.astype(np.float64).reshape(ndof_tot // n_dofs, n_dofs, nvar)
     off += nvar * ndof_tot * prec
     step = struct.unpack_from("<i", data, off)[0]
-    return names, arr, step
+    if not with_meta:
+        return names, arr, step
+    off += 4
+    meta = np.zeros(8)
+    n_meta = min(8, (len(data) - off) // prec)
+    if n_meta > 0:
+        meta[:n_meta] = np.frombuffer(data, dtype=dt, count=n_meta, offset=off)
+    return names, arr, step, meta
 
 
 # --------------------------------------------------------- parameter file

@@ -56,7 +56,7 @@ EXPORTS = ["vcp_create", "vcp_destroy", "vcp_sizes", "vcp_reference", "vcp_set_r
            "vcp_batch_create", "vcp_batch_destroy", "vcp_batch_add", "vcp_batch_remove",
            "vcp_batch_step", "vcp_batch_wait", "vcp_batch_status", "vcp_batch_set_graph",
            "vcp_batch_set_tile", "vcp_batch_groups", "vcp_batch_profile", "vcp_residual_static", "vcp_set_precision", "vcp_device_bytes",
-           "vcp_damping_from_averages",
+           "vcp_damping_from_averages", "vcp_get_force_averages", "vcp_set_force_averages",
            "vcp_last_error"]
 
 _lib = None
@@ -111,6 +111,8 @@ def load_library(path: str = LIB_PATH):
     L.vcp_set_precision.argtypes = [vp, C.c_int]
     L.vcp_device_bytes.argtypes = [vp, C.POINTER(C.c_size_t)]
     L.vcp_damping_from_averages.argtypes = [vp]
+    L.vcp_get_force_averages.argtypes = [vp, dp]
+    L.vcp_set_force_averages.argtypes = [vp, dp]
     L.vcp_last_error.restype = C.c_char_p
     for name in EXPORTS:
         if name != "vcp_last_error":
@@ -252,6 +254,21 @@ class SolverClass:
 
     def set_averages(self, avg: np.ndarray, n_steps: int) -> None:
         self._check(self._L.vcp_set_averages(self.h, np.ascontiguousarray(avg, dtype=np.float64), int(n_steps)))
+
+    def force_averages(self) -> np.ndarray:
+        """Time-averaged force coefficients (mCfxAve, mCfyAve, mCfzAve,
+        mCfxVisAve, mCfyVisAve, mCfzVisAve; SolverClass.cpp:3858-3869)."""
+        out = np.zeros(6)
+        self._check(self._L.vcp_get_force_averages(self.h, out))
+        return out
+
+    def set_force_averages(self, cf) -> None:
+        """Continue the averaged force coefficients from restart meta data
+        (SolverClass.cpp:3236-3249)."""
+        cf = np.ascontiguousarray(cf, dtype=np.float64)
+        if cf.shape != (6,):
+            raise ValueError("six averaged force coefficients expected")
+        self._check(self._L.vcp_set_force_averages(self.h, cf))
 
     def set_graph(self, enable: bool) -> None:
         """Replay each RK step of ExecuteTimeSynchronizationStep as a captured

@@ -482,32 +482,89 @@ __device__ __forceinline__ void face_body(const FaceArgs<K, M, T> &args, const C
   const T *KA = kArgOps ? VCP_FOPS.A : sA, *KD = kArgOps ? VCP_FOPS.D : sD;
   for (int i = tid; i < N2; i += 64) sW2[i] = VCP_FOPS.w[i % M] * VCP_FOPS.w[i / M];
   const double *g = a.fgeo + ((size_t) e * 3 + DIR) * kFaceGeo * N2;
+  // Loads issued here and used after the contractions, so that their latency
+  // passes under them: the two length scales and, when the points fit the
+  // wave in one pass, this lane's point metric (normal, area and, unless the
+  // metric comes from sm.Cm, both sides' inverse metric).
+  const double lenE = a.econst[(size_t) e * kElemConst + DIR], lenL = a.econst[(size_t) L * kElemConst + DIR];
+  constexpr bool kGeoEarly = N2 <= 64;
+  double geoCm[2][9], geoN[4];
+  if constexpr (kGeoEarly) {
+    const int mm = tid < N2 ? tid : 0;
+    if constexpr (!BUG)
+#pragma unroll
+      for (int q = 0; q < 9; ++q) { geoCm[0][q] = g[(13 + q) * N2 + mm]; geoCm[1][q] = g[(4 + q) * N2 + mm]; }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) geoN[q] = g[q * N2 + mm];
+  }
   if constexpr (BUG)
-    for (int i = tid; i < 2 * 5 * 9; i += 64) {
-      const int side = i / 45, l = (i / 9) % 5, q = i % 9;
-      const int pt = l < N2 ? l : 0;
-      sm.Cm[side][l][q] = g[(side ? 4 + q : 13 + q) * N2 + pt];
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+      const int i = tid + 64 * it;
+      if (i < 2 * 5 * 9) {
+        const int side = i / 45, l = (i / 9) % 5, q = i % 9;
+        const int pt = l < N2 ? l : 0;
+        sm.Cm[side][l][q] = g[(side ? 4 + q : 13 + q) * N2 + pt];
+      }
     }
   if constexpr (kMixed)
     if (tid < 10) sm.Mean[tid / 5][tid % 5] = a.U[(size_t) (tid < 5 ? e : L) * 5 * ND + (tid % 5) * ND];
   __syncthreads();
-  // normal contraction straight from HBM: B[side][0|1][v][s][f]
+  // normal contraction straight from HBM: B[side][0|1][v][s][f]; the loads of
+  // both sides are issued before the first is waited for when they fit 40
+  // registers (K <= 6), item by item otherwise. In the float form the mean
+  // (mode 0) is added at the points.
+  constexpr int NIT = (5 * K * K + 63) / 64;
+  if constexpr (2 * NIT * K <= 40) {
+    double xu[2][NIT][K];
 #pragma unroll
-  for (int side = 0; side < 2; ++side) {
-    const double *Us = a.U + (size_t) (side ? L : e) * 5 * ND;
-    const T *Fv = sm.F[side], *dFv = sm.F[2 + side];
-    for (int pc = tid; pc < 5 * K * K; pc += 64) {
-      const int v = pc / (K * K), sl = (pc / K) % K, f = pc % K;
-      const double *u = Us + v * ND + sl * SSL + f * SFA;
-      T b0 = 0, b1 = 0;
+    for (int side = 0; side < 2; ++side) {
+      const double *Us = a.U + (size_t) (side ? L : e) * 5 * ND;
 #pragma unroll
-      for (int t = 0; t < K; ++t) {
-        // float form: the mean (mode 0) is added at the points
-        const T x = (kMixed && t == 0 && sl == 0 && f == 0) ? T(0) : T(u[t * SN]);
-        b0 += Fv[t] * x; b1 += dFv[t] * x;
+      for (int it = 0; it < NIT; ++it) {
+        const int pc0 = tid + 64 * it, pc = pc0 < 5 * K * K ? pc0 : 0;
+        const int v = pc / (K * K), sl = (pc / K) % K, f = pc % K;
+        const double *u = Us + v * ND + sl * SSL + f * SFA;
+#pragma unroll
+        for (int t = 0; t < K; ++t) xu[side][it][t] = u[t * SN];
       }
-      B[(side * 2 + 0) * 5 * K * K + pc] = b0;
-      B[(side * 2 + 1) * 5 * K * K + pc] = b1;
+    }
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const T *Fv = sm.F[side], *dFv = sm.F[2 + side];
+#pragma unroll
+      for (int it = 0; it < NIT; ++it) {
+        const int pc = tid + 64 * it;
+        if (pc < 5 * K * K) {
+          const int sl = (pc / K) % K, f = pc % K;
+          T b0 = 0, b1 = 0;
+#pragma unroll
+          for (int t = 0; t < K; ++t) {
+            const T x = (kMixed && t == 0 && sl == 0 && f == 0) ? T(0) : T(xu[side][it][t]);
+            b0 += Fv[t] * x; b1 += dFv[t] * x;
+          }
+          B[(side * 2 + 0) * 5 * K * K + pc] = b0;
+          B[(side * 2 + 1) * 5 * K * K + pc] = b1;
+        }
+      }
+    }
+  } else {
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const double *Us = a.U + (size_t) (side ? L : e) * 5 * ND;
+      const T *Fv = sm.F[side], *dFv = sm.F[2 + side];
+      for (int pc = tid; pc < 5 * K * K; pc += 64) {
+        const int v = pc / (K * K), sl = (pc / K) % K, f = pc % K;
+        const double *u = Us + v * ND + sl * SSL + f * SFA;
+        T b0 = 0, b1 = 0;
+#pragma unroll
+        for (int t = 0; t < K; ++t) {
+          const T x = (kMixed && t == 0 && sl == 0 && f == 0) ? T(0) : T(u[t * SN]);
+          b0 += Fv[t] * x; b1 += dFv[t] * x;
+        }
+        B[(side * 2 + 0) * 5 * K * K + pc] = b0;
+        B[(side * 2 + 1) * 5 * K * K + pc] = b1;
+      }
     }
   }
   __syncthreads();
@@ -536,7 +593,7 @@ __device__ __forceinline__ void face_body(const FaceArgs<K, M, T> &args, const C
   // per point: traces, averaged Cartesian gradients of the entropy variables
   // and the fluxes (Ismail-Roe, viscous, penalty, symmetrising); the outputs
   // overwrite T in place when single-pass, else the dead B
-  const double lenScale = fmin(a.econst[(size_t) e * kElemConst + DIR], a.econst[(size_t) L * kElemConst + DIR]);
+  const double lenScale = fmin(lenE, lenL);
   const double tau = a.tauFact / lenScale;
   // LES length scale of the face: the mean of the two elements'
   // (ElementClass.cpp:2655)
@@ -558,7 +615,7 @@ __device__ __forceinline__ void face_body(const FaceArgs<K, M, T> &args, const C
       double cm[9];
       if constexpr (!BUG) {
 #pragma unroll
-        for (int q = 0; q < 9; ++q) cm[q] = g[(side ? 4 + q : 13 + q) * N2 + m];
+        for (int q = 0; q < 9; ++q) cm[q] = kGeoEarly ? geoCm[side][q] : g[(side ? 4 + q : 13 + q) * N2 + m];
       }
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
@@ -586,8 +643,9 @@ __device__ __forceinline__ void face_body(const FaceArgs<K, M, T> &args, const C
     // their computation into the flux and keeps the LDS operands live
 #pragma unroll
     for (int v = 0; v < 5; ++v) asm volatile("" : "+v"(Vs[0][v]), "+v"(Vs[1][v]), "+v"(gx[v]), "+v"(gy[v]), "+v"(gz[v]));
-    const double nn[3] = {g[0 * N2 + m], g[1 * N2 + m], g[2 * N2 + m]};
-    const double wA = sW2[m] * g[3 * N2 + m];
+    const double nn[3] = {kGeoEarly ? geoN[0] : g[0 * N2 + m], kGeoEarly ? geoN[1] : g[1 * N2 + m],
+                          kGeoEarly ? geoN[2] : g[2 * N2 + m]};
+    const double wA = sW2[m] * (kGeoEarly ? geoN[3] : g[3 * N2 + m]);
     double fl[5], FX[5], FY[5], FZ[5], ev = 0.0;
     if constexpr (SGS) {
       ev = face_eddy(a.sgs, lenLES, Vs[1], Vs[0], gx, gy, gz);
@@ -901,17 +959,43 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
   const int sp = a.spIdx ? a.spIdx[e] : -1;
   for (int i = tid; i < M * K; i += NT) { sA[i] = A[i]; sD[i] = D[i]; }
   for (int i = tid; i < 4 * K; i += NT) sFv[i] = args.fvec[i / K][i % K];
-  for (int i = tid; i < 5 * ND; i += NT) {
-    if constexpr (kMixed) {
-      const double u = Ue[i];
-      if (i % ND == 0) sMean[i / ND] = u;
-      sU[(i / ND) * SUV + i % ND] = i % ND == 0 ? T(0) : T(u);
-    } else {
-      sU[(i / ND) * SUV + i % ND] = Ue[i];
+  {
+    // the modal state: every load issued before the first is waited for
+    constexpr int NIT = (5 * ND + NT - 1) / NT;
+    double uu[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int i = tid + it * NT;
+      uu[it] = Ue[i < 5 * ND ? i : 0];
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int i = tid + it * NT;
+      if (i < 5 * ND) {
+        if constexpr (kMixed) {
+          if (i % ND == 0) sMean[i / ND] = uu[it];
+          sU[(i / ND) * SUV + i % ND] = i % ND == 0 ? T(0) : T(uu[it]);
+        } else {
+          sU[(i / ND) * SUV + i % ND] = uu[it];
+        }
+      }
     }
   }
   if (args.bug)
     for (int i = tid; i < 45; i += NT) sCm[i / 9][i % 9] = vg[(1 + i % 9) * NI + i / 9];
+  // loaded here, used much later, so that their latency passes under the
+  // first contractions: the element's Jacobian constant and (K <= 5, where
+  // the registers allow it) the metric of this thread's point in the first
+  // point pass
+  const double elemJac = a.econst[(size_t) e * kElemConst + 3];
+  constexpr bool kGeoEarly = K <= 5;
+  double geoJ0 = 0.0, geoW0 = 0.0, geoCm0[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (kGeoEarly && tid < P0) {
+    geoJ0 = vg[tid];
+    geoW0 = a.tables[args.to.w3 + tid];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) geoCm0[t] = vg[(1 + t) * NI + tid];
+  }
   __syncthreads();
 
   // ---------------- step 1: solution and parametric gradients at the points
@@ -961,7 +1045,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
     if (lane == 0) {
       double rho, u, v, w, p, eKin;
       ent2prim(Vc, rho, u, v, w, p, eKin);
-      const double JacInv = 8.0 / a.econst[(size_t) e * kElemConst + 3];
+      const double JacInv = 8.0 / elemJac;
       const double pInv = 1.0 / p, a1 = kGm1 * rho * pInv * pInv, a2 = eKin * a1;
       sDv[0] = T(JacInv * (kGam / kGm1 / rho + a2 * eKin)); sDv[1] = T(-JacInv * u * a2); sDv[2] = T(-JacInv * v * a2);
       sDv[3] = T(-JacInv * w * a2); sDv[4] = T(JacInv * (a2 - pInv)); sDv[5] = T(JacInv * (a1 * u * u + pInv));
@@ -1013,6 +1097,14 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
     const int m = q == 0 ? tid : P0 + tid;
     const bool act = q == 0 ? tid < P0 : tid < P1;
     double Vp[5], Gr[5], Gs[5], Gt[5];
+    // the second pass's metric, loaded before its x stage
+    double geoJ1 = 0.0, geoW1 = 0.0, geoCm1[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (kGeoEarly && q == 1 && act) {
+      geoJ1 = vg[m];
+      geoW1 = a.tables[args.to.w3 + m];
+#pragma unroll
+      for (int t = 0; t < 9; ++t) geoCm1[t] = vg[(1 + t) * NI + m];
+    }
     if (act) {
       const int i = m % M, jk = m / M;
       T ai[K], di[K];
@@ -1038,11 +1130,11 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
     if (!act) continue;
     T *Fb = q == 0 ? sBig + m : sBig + SF0 + (m - P0);
     const int fstride = q == 0 ? P0 : P1;
-    const double J = vg[m];
-    const double wJ = a.tables[args.to.w3 + m] * J;
+    const double J = !kGeoEarly ? vg[m] : q == 0 ? geoJ0 : geoJ1;
+    const double wJ = (!kGeoEarly ? a.tables[args.to.w3 + m] : q == 0 ? geoW0 : geoW1) * J;
     double cm[9];                                      // dr/dx .. dt/dz at the point
 #pragma unroll
-    for (int t = 0; t < 9; ++t) cm[t] = vg[(1 + t) * NI + m];
+    for (int t = 0; t < 9; ++t) cm[t] = !kGeoEarly ? vg[(1 + t) * NI + m] : q == 0 ? geoCm0[t] : geoCm1[t];
     double gx[5], gy[5], gz[5];
     if (args.bug) {
 #pragma unroll
@@ -1285,7 +1377,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
   }
 
   // ---------------- entropy-variable mass matrix: flexible PCG
-  const double JacInv = 8.0 / a.econst[(size_t) e * kElemConst + 3];
+  const double JacInv = 8.0 / elemJac;
   const bool own = tid < ND;
   T x[5], z[5], pv[5];
   int half = 0;

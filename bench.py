@@ -12,7 +12,8 @@ value = 5 * nElem * (P+1)^3 * 3 stages * steps / seconds / 1e9
         (GDOF-updates/s per RK stage), inputs resident in HBM.
 
 Also reported: the roofline position of the dominant kernel (its mean launch
-time measured with HIP events on the library's stream) and a CPU baseline:
+time over the timed steps, from HIP events recorded around each launch on the
+library's stream) and a CPU baseline:
 one RK step of the same case by the reference itself, built from its sources
 (oracle/_ref), on the host cores, with the C oracle's rate beside it (the
 oracle alone where the reference build is absent or cannot run). Beside the
@@ -48,11 +49,14 @@ def volume_kernel_flops(K: int, M: int, cg_iters: float) -> float:
     return 2.0 * (fwd + bwd + gather + cg_iters * cg) + 350.0 * M**3
 
 
-def volume_kernel_bytes(K: int, M: int) -> float:
+def volume_kernel_bytes(K: int, M: int, update_share: float = 0.0) -> float:
     """Algorithmic HBM bytes per element of a residual launch: U, geometry
     (J + 9 metric terms per point), 16 element constants and the six face
-    slots (value and normal-derivative planes, 10 K^2 each) read; R written."""
-    return 8.0 * (5 * K**3 + 60 * K**2 + 10 * M**3 + 16 + 5 * K**3)
+    slots (value and normal-derivative planes, 10 K^2 each) read; R written.
+    A launch with the fused RK-stage update also reads U0 (and writes U in
+    place of R); update_share is the fraction of such launches averaged over
+    (2/3 in an SSP-RK3 step)."""
+    return 8.0 * (5 * K**3 + 60 * K**2 + 10 * M**3 + 16 + 5 * K**3 + update_share * 5 * K**3)
 
 
 def measured_traffic(kernel: str):
@@ -486,27 +490,35 @@ def main() -> int:
         for _ in range(args.warmup):
             s_.ExecuteTimeSynchronizationStep()
         print(f"[bench] {label}: warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
+        # per-launch durations of the timed steps' face and volume kernels
+        # (HIP events on the library stream, around each launch)
+        s_.set_kernel_timing(True)
         t0 = time.perf_counter()
         for _ in range(args.steps):
             md_ = s_.ExecuteTimeSynchronizationStep()
         secs_ = time.perf_counter() - t0
+        kt_ = s_.kernel_timing()
+        s_.set_kernel_timing(False)
         print(f"[bench] {label}: timed {args.steps} steps in {secs_:.3f} s", file=sys.stderr, flush=True)
-        return secs_, md_
+        return secs_, md_, kt_
 
     solver = make(args.precision)
-    secs, md = timed(solver, args.precision)
+    secs, md, ktimes = timed(solver, args.precision)
     ndof = 5 * case.n_elems * case.n_dofs
     value = ndof * 3 * args.steps / secs / 1e9
 
-    # dominant kernel: per-launch times from HIP events on the library stream
+    # dominant kernel: mean per-launch times of the timed steps' launches
+    # (HIP events on the library stream), with residual-only launches of the
+    # same kernels beside them
     cg = solver.cg_iterations()
     cg_mean = float(cg.mean())
-    ms_face, ms_vol = solver.profile_residual(args.profile_reps)
+    ms_face, ms_vol, n_timed = ktimes
+    rs_face, rs_vol = solver.profile_residual(args.profile_reps)
     K, M = case.p + 1, 7
     if ms_vol >= ms_face:
         name, ms = "volume_kernel<5, 7, 256>", ms_vol
         flops = volume_kernel_flops(K, M, cg_mean) * case.n_elems
-        bytes_ = volume_kernel_bytes(K, M) * case.n_elems
+        bytes_ = volume_kernel_bytes(K, M, 2.0 / 3.0) * case.n_elems
     else:
         name, ms = "face_kernel<5, 7, BUG> (3 directions)", ms_face
         flops = face_kernel_flops(K, M) * 3 * case.n_elems
@@ -525,7 +537,9 @@ def main() -> int:
                 "intensity_flop_per_byte": intensity, "attainable_tflops": attainable,
                 "frac_of_attainable": achieved / attainable,
                 "hbm_gbs_algorithmic": hbm_gbs, "hbm_frac": hbm_gbs / PEAK_HBM_GBS,
-                "ms_face_kernel": ms_face, "ms_volume_kernel": ms_vol, "cg_iters_mean": cg_mean}
+                "ms_face_kernel": ms_face, "ms_volume_kernel": ms_vol, "timed_stages": n_timed,
+                "ms_face_kernel_residual_only": rs_face, "ms_volume_kernel_residual_only": rs_vol,
+                "cg_iters_mean": cg_mean}
 
     out = {
         "metric": METRIC, "value": value, "unit": "GDOF/s", "n_gpus": 1,
@@ -546,9 +560,9 @@ def main() -> int:
         from vcp3d_open_amd.diagnostics import flow_integrals
         from vcp3d_open_amd.solver import host_setup
         mixed = make("mixed")
-        msecs, _ = timed(mixed, "mixed")
+        msecs, _, mkt = timed(mixed, "mixed")
         mvalue = ndof * 3 * args.steps / msecs / 1e9
-        mf_face, mf_vol = mixed.profile_residual(args.profile_reps)
+        mf_face, mf_vol, _ = mkt
         out["mixed"] = {"value": mvalue, "unit": "GDOF/s", "ms_per_step": msecs / args.steps * 1e3,
                         "speedup_vs_fp64": mvalue / value, "dtype": "f32-contractions/f64",
                         "ms_face_kernel": mf_face, "ms_volume_kernel": mf_vol,

@@ -207,6 +207,15 @@ int vcp_sync_step(vcp_ctx *ctx, double monitoring[8]);
    default). */
 int vcp_set_graph(vcp_ctx *ctx, int enable);
 
+/* No counterpart in the reference (measurement): durations of the face and
+   volume launches inside vcp_sync_step, from HIP events recorded before,
+   between and after them on the context's stream. enable = 1 starts
+   recording and clears the sums, 0 stops it; the graph path (vcp_set_graph)
+   is not timed. Get: mean milliseconds per face launch (three directions)
+   and per volume launch, and the number of timed stages. */
+int vcp_set_kernel_timing(vcp_ctx *ctx, int enable);
+int vcp_get_kernel_timing(vcp_ctx *ctx, double ms[2], int *stages);
+
 /* The running averages (ElementClass::mAvePrim, mAveVelProd, mAveEddyVis;
    SolverClass::UpdateAverageData, SolverClass.cpp:3800-3850,
    ElementClass.cpp:3741-3804) of a case created with compute_average:

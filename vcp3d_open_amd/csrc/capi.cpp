@@ -56,7 +56,41 @@ int check_flags(vcp_ctx *c)
 
 int residual_pass(vcp_ctx *c, const vcp::StageCtl &ctl)
 {
-  return vcp::residual_pass(c, ctl, c->stream);
+  if (!c->kt.on || c->use_graph) return vcp::residual_pass(c, ctl, c->stream);
+  // timed: the same launches with an event before, between and after the
+  // face and volume launches
+  if (c->prm.sponge_layer && !c->damping_set)
+    return fail("sponge layer damping state not set (vcp_set_damping_state)");
+  vcp_ctx::KernelTiming &kt = c->kt;
+  while (kt.ev.size() < kt.used + 3) {
+    hipEvent_t e = nullptr;
+    HIP_TRY(hipEventCreate(&e));
+    kt.ev.push_back(e);
+  }
+  hipEvent_t *ev = kt.ev.data() + kt.used;
+  kt.used += 3;
+  HIP_TRY(vcp::launch_boundary(c->dc, c->stream, ctl.monitor != 0));
+  HIP_TRY(hipEventRecord(ev[0], c->stream));
+  HIP_TRY(vcp::launch_faces(c->dc, c->stream));
+  HIP_TRY(hipEventRecord(ev[1], c->stream));
+  HIP_TRY(vcp::launch_volume(c->dc, ctl, c->stream));
+  HIP_TRY(hipEventRecord(ev[2], c->stream));
+  return 0;
+}
+
+// adds the recorded launches' durations to the sums (after a synchronisation
+// of the stream they were recorded on)
+int collect_timing(vcp_ctx *c)
+{
+  vcp_ctx::KernelTiming &kt = c->kt;
+  for (size_t i = 0; i + 3 <= kt.used; i += 3) {
+    float f = 0.f, v = 0.f;
+    HIP_TRY(hipEventElapsedTime(&f, kt.ev[i], kt.ev[i + 1]));
+    HIP_TRY(hipEventElapsedTime(&v, kt.ev[i + 1], kt.ev[i + 2]));
+    kt.face_ms += f; kt.vol_ms += v; ++kt.launches;
+  }
+  kt.used = 0;
+  return 0;
 }
 
 // a context that a batch steps is driven by the batch only
@@ -697,6 +731,7 @@ int vcp_sync_step(vcp_ctx *c, double md[8])
       HIP_TRY(hipMemcpyAsync(c->host, c->dc.scal, kScalRead * sizeof(double), hipMemcpyDeviceToHost, c->stream));
       HIP_TRY(hipMemcpyAsync(c->host + kHostFlags, c->dc.flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
       HIP_TRY(hipStreamSynchronize(c->stream));
+      if (c->kt.on && (rc = collect_timing(c))) return rc;
       int flags = 0;
       std::memcpy(&flags, c->host + kHostFlags, sizeof(int));
       // a failure of the next step's time step is reported by the next call
@@ -752,6 +787,34 @@ int vcp_set_averages(vcp_ctx *c, const double *avg, int n_steps)
   HIP_TRY(hipMemcpyAsync(c->dc.avg, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->n_avg = n_steps;
+  return 0;
+  VCP_GUARD_END
+}
+
+int vcp_set_kernel_timing(vcp_ctx *c, int enable)
+{
+  VCP_GUARD_BEGIN
+  if (!c) return fail("null context");
+  if (int rc_ = not_in_batch(c)) return rc_;
+  vcp_ctx::KernelTiming &kt = c->kt;
+  kt.on = enable != 0;
+  kt.used = 0;
+  if (kt.on) {
+    kt.face_ms = kt.vol_ms = 0.0;
+    kt.launches = 0;
+  }
+  return 0;
+  VCP_GUARD_END
+}
+
+int vcp_get_kernel_timing(vcp_ctx *c, double ms[2], int *launches)
+{
+  VCP_GUARD_BEGIN
+  if (!c || !ms || !launches) return fail("null argument");
+  const vcp_ctx::KernelTiming &kt = c->kt;
+  *launches = kt.launches;
+  ms[0] = kt.launches ? kt.face_ms / kt.launches : 0.0;
+  ms[1] = kt.launches ? kt.vol_ms / kt.launches : 0.0;
   return 0;
   VCP_GUARD_END
 }

@@ -73,6 +73,16 @@ struct vcp_ctx {
   hipGraphExec_t graph[2][2] = {};
   // the batch that steps this case (vcp_batch_add), or null
   vcp_batch *batch = nullptr;
+  // launch timing inside vcp_sync_step (vcp_set_kernel_timing): three events
+  // per RK stage (before the face launch, between it and the volume launch,
+  // after the volume launch), read back after each call's last synchronisation
+  struct KernelTiming {
+    bool on = false;
+    std::vector<hipEvent_t> ev;
+    size_t used = 0;
+    double face_ms = 0.0, vol_ms = 0.0;
+    int launches = 0;
+  } kt;
 
   template <class T>
   hipError_t alloc(T **p, size_t n)
@@ -92,6 +102,7 @@ struct vcp_ctx {
   ~vcp_ctx()
   {
     drop_graphs();
+    for (hipEvent_t e : kt.ev) (void) hipEventDestroy(e);
     for (void *p : allocs) (void) hipFree(p);
     if (host) (void) hipHostFree(host);
     if (evState) (void) hipEventDestroy(evState);

@@ -57,6 +57,7 @@ EXPORTS = ["vcp_create", "vcp_destroy", "vcp_sizes", "vcp_reference", "vcp_set_r
            "vcp_batch_step", "vcp_batch_wait", "vcp_batch_status", "vcp_batch_set_graph",
            "vcp_batch_set_tile", "vcp_batch_groups", "vcp_batch_profile", "vcp_residual_static", "vcp_set_precision", "vcp_device_bytes",
            "vcp_damping_from_averages", "vcp_get_force_averages", "vcp_set_force_averages",
+           "vcp_set_kernel_timing", "vcp_get_kernel_timing",
            "vcp_last_error"]
 
 _lib = None
@@ -113,6 +114,8 @@ def load_library(path: str = LIB_PATH):
     L.vcp_damping_from_averages.argtypes = [vp]
     L.vcp_get_force_averages.argtypes = [vp, dp]
     L.vcp_set_force_averages.argtypes = [vp, dp]
+    L.vcp_set_kernel_timing.argtypes = [vp, C.c_int]
+    L.vcp_get_kernel_timing.argtypes = [vp, dp, C.POINTER(C.c_int)]
     L.vcp_last_error.restype = C.c_char_p
     for name in EXPORTS:
         if name != "vcp_last_error":
@@ -312,6 +315,18 @@ class SolverClass:
         ms = np.zeros(2)
         self._check(self._L.vcp_profile_residual(self.h, reps, ms))
         return float(ms[0]), float(ms[1])
+
+    def set_kernel_timing(self, enable: bool) -> None:
+        """Time the face and volume launches of ExecuteTimeSynchronizationStep
+        with HIP events on the library stream (enable clears the sums)."""
+        self._check(self._L.vcp_set_kernel_timing(self.h, 1 if enable else 0))
+
+    def kernel_timing(self) -> tuple:
+        """(mean ms per face launch, mean ms per volume launch, timed stages)."""
+        ms = np.zeros(2)
+        n = C.c_int()
+        self._check(self._L.vcp_get_kernel_timing(self.h, ms, C.byref(n)))
+        return float(ms[0]), float(ms[1]), int(n.value)
 
     def device_bytes(self) -> int:
         """Device memory held by the context (bytes of its allocations)."""

@@ -925,6 +925,10 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
   constexpr int ND = S::ND, NI = S::NI, NR = S::NR, R0 = S::R0, P0 = S::P0, P1 = S::P1;
   constexpr int SZ = S::SZ, SF0 = S::SF0, SI = S::SI, BIG = S::BIG;
   static_assert(P1 <= NT, "two point passes");
+  // the second point pass runs on the last waves (from thread T2), which own
+  // no DOFs, leaving the first waves free
+  constexpr int T2 = NT - ((P1 + 63) / 64) * 64;
+  constexpr bool kPrecondLate = P1 > 0 && T2 >= 64;
   // rot() below wraps thread numbers with & (NT - 1)
   static_assert((NT & (NT - 1)) == 0, "NT must be a power of two");
   // the first pass's fluxes must leave the y-stage rows of the second pass intact
@@ -1016,10 +1020,12 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
       zD[(v * M + n) * K * K + ji] = sd;
     }
   }
-  if (tid >= NT - 64) {
-    // element-centre entropy variables -> preconditioner dV/dU / Jbar; the
-    // last wave does this while the others run the first contraction (it has
-    // no pencil there when 5 K^2 <= NT - 64)
+  // element-centre entropy variables -> preconditioner dV/dU / Jbar in sDv,
+  // by one whole wave (lane 0 the last part), in a stage where that wave has
+  // no other work: the second point pass when it leaves the first wave idle,
+  // else the first contraction, beside the pencils (none in the last wave when
+  // 5 K^2 <= NT - 64)
+  auto precond = [&]() {
     const int lane = tid & 63;
     double Vc[5] = {0, 0, 0, 0, 0};
     for (int i = lane; i < ND; i += 64) {
@@ -1053,7 +1059,8 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
       sDv[9] = T(JacInv * (a1 * v * v + pInv)); sDv[10] = T(JacInv * a1 * v * w); sDv[11] = T(-JacInv * a1 * v);
       sDv[12] = T(JacInv * (a1 * w * w + pInv)); sDv[13] = T(-JacInv * a1 * w); sDv[14] = T(JacInv * a1);
     }
-  }
+  };
+  if (!kPrecondLate && tid >= NT - 64) precond();
   __syncthreads();
   for (int pc = tid; pc < 5 * M * K; pc += NT) {      // pencil (v, k', i) along j
     const int vk = pc / K, i = pc - vk * K;
@@ -1094,8 +1101,8 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
   }
 #pragma unroll
   for (int q = 0; q < (P1 > 0 ? 2 : 1); ++q) {
-    const int m = q == 0 ? tid : P0 + tid;
-    const bool act = q == 0 ? tid < P0 : tid < P1;
+    const int m = q == 0 ? tid : P0 + tid - T2;
+    const bool act = q == 0 ? tid < P0 : (tid >= T2 && tid - T2 < P1);
     double Vp[5], Gr[5], Gs[5], Gt[5];
     // the second pass's metric, loaded before its x stage
     double geoJ1 = 0.0, geoW1 = 0.0, geoCm1[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -1127,6 +1134,7 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
       }
     }
     __syncthreads();
+    if (kPrecondLate && q == 1 && tid < 64) precond();
     if (!act) continue;
     T *Fb = q == 0 ? sBig + m : sBig + SF0 + (m - P0);
     const int fstride = q == 0 ? P0 : P1;
@@ -1471,8 +1479,8 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
       __syncthreads();
 #pragma unroll
       for (int q = 0; q < (P1 > 0 ? 2 : 1); ++q) {
-        const int m = q == 0 ? tid : P0 + tid;
-        if (!(q == 0 ? tid < P0 : tid < P1)) continue;
+        const int m = q == 0 ? tid : P0 + tid - T2;
+        if (!(q == 0 ? tid < P0 : (tid >= T2 && tid - T2 < P1))) continue;
         T pp[5];
 #pragma unroll
         for (int l = 0; l < 5; ++l) pp[l] = pI[l * NI + m];

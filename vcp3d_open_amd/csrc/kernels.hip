@@ -2233,10 +2233,23 @@ __device__ __forceinline__ void nodal_state(const double *Ue, const double *sL, 
                                             int lane)
 {
   constexpr int ND = K * K * K, NQ = GRAD ? 4 : 1;
-  for (int i = lane; i < 5 * ND; i += 64) {
-    const double u = Ue[i];
+  // the modal state in chunks of up to 16 loads per lane, each chunk issued
+  // before its first load is waited for
+  constexpr int NL = 5 * ND, NIT = (NL + 63) / 64, CH = NIT < 16 ? NIT : 16;
+  for (int c0 = 0; c0 < NIT; c0 += CH) {
+    double u[CH];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) sV[q * 5 * ND + i] = u;
+    for (int t = 0; t < CH; ++t) {
+      const int i = lane + 64 * (c0 + t);
+      u[t] = Ue[i < NL ? i : 0];
+    }
+#pragma unroll
+    for (int t = 0; t < CH; ++t) {
+      const int i = lane + 64 * (c0 + t);
+      if (i < NL)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) sV[q * NL + i] = u[t];
+    }
   }
   __syncthreads();
 #pragma unroll

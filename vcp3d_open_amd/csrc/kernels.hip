@@ -957,11 +957,17 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
   }
   const CaseDev &a = *ap;
   const int tid = threadIdx.x;
-  const T *A = args.op.A, *D = args.op.D;
+  // Operators of the pencil stages: the kernel arguments (scalar registers)
+  // up to 800 bytes; above that (K = 7 in FP64, 1008 bytes) the compiler
+  // parks them in VGPR lanes and reads them back with v_readlane, 28 % of the
+  // kernel's instructions, so those stages read the LDS copies instead
+  // (26 % faster there; slower at K <= 6, where the arguments fit).
+  constexpr bool kArgOps = sizeof(args.op) <= 800;
+  const T *A = kArgOps ? args.op.A : sA, *D = kArgOps ? args.op.D : sD;
   const double *Ue = a.U + (size_t) e * 5 * ND;
   const double *vg = a.vgeo + (size_t) e * kVolGeo * NI;
   const int sp = a.spIdx ? a.spIdx[e] : -1;
-  for (int i = tid; i < M * K; i += NT) { sA[i] = A[i]; sD[i] = D[i]; }
+  for (int i = tid; i < M * K; i += NT) { sA[i] = args.op.A[i]; sD[i] = args.op.D[i]; }
   for (int i = tid; i < 4 * K; i += NT) sFv[i] = args.fvec[i / K][i % K];
   {
     // the modal state: every load issued before the first is waited for

@@ -958,12 +958,13 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
   const CaseDev &a = *ap;
   const int tid = threadIdx.x;
   // Operators of the pencil stages: the kernel arguments (scalar registers)
-  // up to 800 bytes; above that (K = 7 in FP64, 1008 bytes) the compiler
-  // parks them in VGPR lanes and reads them back with v_readlane, 28 % of the
-  // kernel's instructions, so those stages read the LDS copies instead
-  // (26 % faster there; slower at K <= 6, where the arguments fit).
-  constexpr bool kArgOps = sizeof(args.op) <= 800;
-  const T *A = kArgOps ? args.op.A : sA, *D = kArgOps ? args.op.D : sD;
+  // while they fit; beyond that the compiler parks them in VGPR lanes and
+  // reads them back with v_readlane (28 % of the instructions at K = 7 in
+  // FP64), so larger operators come from the LDS copies: both above 800 bytes
+  // (K = 7 in FP64: 26 % faster), D alone above 600 (K = 6 in FP64: 3 %
+  // faster; both from LDS there, or either at K <= 5, measured slower).
+  constexpr bool kArgOpsA = sizeof(args.op) <= 800, kArgOpsD = sizeof(args.op) <= 600;
+  const T *A = kArgOpsA ? args.op.A : sA, *D = kArgOpsD ? args.op.D : sD;
   const double *Ue = a.U + (size_t) e * 5 * ND;
   const double *vg = a.vgeo + (size_t) e * kVolGeo * NI;
   const int sp = a.spIdx ? a.spIdx[e] : -1;

@@ -511,26 +511,29 @@ __device__ __forceinline__ void face_body(const FaceArgs<K, M, T> &args, const C
     if (tid < 10) sm.Mean[tid / 5][tid % 5] = a.U[(size_t) (tid < 5 ? e : L) * 5 * ND + (tid % 5) * ND];
   __syncthreads();
   // normal contraction straight from HBM: B[side][0|1][v][s][f]; the loads of
-  // both sides are issued before the first is waited for when they fit 40
-  // registers (K <= 6), item by item otherwise. In the float form the mean
-  // (mode 0) is added at the points.
+  // both sides (K <= 6, at most 40 registers) or of one side at a time (K = 7)
+  // are issued before the first of them is waited for. In the float form the
+  // mean (mode 0) is added at the points.
   constexpr int NIT = (5 * K * K + 63) / 64;
-  if constexpr (2 * NIT * K <= 40) {
-    double xu[2][NIT][K];
+  constexpr int SB = 2 * NIT * K <= 40 ? 2 : 1;
 #pragma unroll
-    for (int side = 0; side < 2; ++side) {
-      const double *Us = a.U + (size_t) (side ? L : e) * 5 * ND;
+  for (int s0 = 0; s0 < 2; s0 += SB) {
+    double xu[SB][NIT][K];
+#pragma unroll
+    for (int sb = 0; sb < SB; ++sb) {
+      const double *Us = a.U + (size_t) (s0 + sb ? L : e) * 5 * ND;
 #pragma unroll
       for (int it = 0; it < NIT; ++it) {
         const int pc0 = tid + 64 * it, pc = pc0 < 5 * K * K ? pc0 : 0;
         const int v = pc / (K * K), sl = (pc / K) % K, f = pc % K;
         const double *u = Us + v * ND + sl * SSL + f * SFA;
 #pragma unroll
-        for (int t = 0; t < K; ++t) xu[side][it][t] = u[t * SN];
+        for (int t = 0; t < K; ++t) xu[sb][it][t] = u[t * SN];
       }
     }
 #pragma unroll
-    for (int side = 0; side < 2; ++side) {
+    for (int sb = 0; sb < SB; ++sb) {
+      const int side = s0 + sb;
       const T *Fv = sm.F[side], *dFv = sm.F[2 + side];
 #pragma unroll
       for (int it = 0; it < NIT; ++it) {
@@ -540,30 +543,12 @@ __device__ __forceinline__ void face_body(const FaceArgs<K, M, T> &args, const C
           T b0 = 0, b1 = 0;
 #pragma unroll
           for (int t = 0; t < K; ++t) {
-            const T x = (kMixed && t == 0 && sl == 0 && f == 0) ? T(0) : T(xu[side][it][t]);
+            const T x = (kMixed && t == 0 && sl == 0 && f == 0) ? T(0) : T(xu[sb][it][t]);
             b0 += Fv[t] * x; b1 += dFv[t] * x;
           }
           B[(side * 2 + 0) * 5 * K * K + pc] = b0;
           B[(side * 2 + 1) * 5 * K * K + pc] = b1;
         }
-      }
-    }
-  } else {
-#pragma unroll
-    for (int side = 0; side < 2; ++side) {
-      const double *Us = a.U + (size_t) (side ? L : e) * 5 * ND;
-      const T *Fv = sm.F[side], *dFv = sm.F[2 + side];
-      for (int pc = tid; pc < 5 * K * K; pc += 64) {
-        const int v = pc / (K * K), sl = (pc / K) % K, f = pc % K;
-        const double *u = Us + v * ND + sl * SSL + f * SFA;
-        T b0 = 0, b1 = 0;
-#pragma unroll
-        for (int t = 0; t < K; ++t) {
-          const T x = (kMixed && t == 0 && sl == 0 && f == 0) ? T(0) : T(u[t * SN]);
-          b0 += Fv[t] * x; b1 += dFv[t] * x;
-        }
-        B[(side * 2 + 0) * 5 * K * K + pc] = b0;
-        B[(side * 2 + 1) * 5 * K * K + pc] = b1;
       }
     }
   }

@@ -68,13 +68,13 @@ int residual_pass(vcp_ctx *c, const vcp::StageCtl &ctl)
     kt.ev.push_back(e);
   }
   hipEvent_t *ev = kt.ev.data() + kt.used;
-  kt.used += 3;
   HIP_TRY(vcp::launch_boundary(c->dc, c->stream, ctl.monitor != 0));
   HIP_TRY(hipEventRecord(ev[0], c->stream));
   HIP_TRY(vcp::launch_faces(c->dc, c->stream));
   HIP_TRY(hipEventRecord(ev[1], c->stream));
   HIP_TRY(vcp::launch_volume(c->dc, ctl, c->stream));
   HIP_TRY(hipEventRecord(ev[2], c->stream));
+  kt.used += 3;   // counted once all three are recorded
   return 0;
 }
 

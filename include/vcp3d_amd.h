@@ -110,6 +110,12 @@ typedef struct vcp_case_desc {
      terms, the time step, monitoring[7] and the averaged eddy viscosity of
      vcp_get_averages */
   int    sgs_model;
+  /* "Body force per unit volume (x,y,z)" (InputParamClass.cpp:331-334),
+     dimensional; body_force_set = 0 when the keyword is absent. The volume
+     residual gains -w J (0, f, u.f) / p_ref at the integration points
+     (ElementClass.cpp:4492-4533). */
+  double body_force[3];
+  int    body_force_set;
 } vcp_case_desc;
 
 enum { VCP_SGS_NONE = 0, VCP_SGS_WALE = 1, VCP_SGS_VREMAN = 2 };

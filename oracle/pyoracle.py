@@ -41,7 +41,7 @@ class OrcParams(C.Structure):
         ("damping_constant", C.c_double * 6), ("damping_exponent", C.c_double * 6),
         ("monitor_forces", C.c_int), ("reference_area", C.c_double),
         ("compute_average", C.c_int), ("free_stream_dir", C.c_double * 3),
-        ("sgs_model", C.c_int),
+        ("sgs_model", C.c_int), ("body_force", C.c_double * 3), ("body_force_set", C.c_int),
     ]
 
 

@@ -709,8 +709,8 @@ static void cml_source(const double *V, const double *gx, const double *gy, cons
   }
 }
 
-/* ElementClass::VolumeResidual, steps 1-8 (ElementClass.cpp:3809-4489) for
-   entropy variables, with or without an SGS model, without body force. */
+/* ElementClass::VolumeResidual, steps 1-9 (ElementClass.cpp:3809-4533) for
+   entropy variables, with or without an SGS model and a body force. */
 static void volume_residual(orc_case *c, int e, int mon, double *mach2max, double *eddymax)
 {
   const int K = c->K, M = c->M, nInt = c->nInt, nD = c->nDOFs;
@@ -744,7 +744,10 @@ static void volume_residual(orc_case *c, int e, int mon, double *mach2max, doubl
      :4450-4489) */
   const int sp = c->spongeIdx ? c->spongeIdx[e] : -1;
   const int useCml = sp >= 0 && c->prm.cml;
-  double *Scml = Wb + 20 * (size_t) nInt, *Ssp = Wb + 25 * (size_t) nInt;
+  double *Scml = Wb + 20 * (size_t) nInt, *Ssp = Wb + 25 * (size_t) nInt, *Sbf = Wb + 30 * (size_t) nInt;
+  /* non-dimensional body force (ElementClass.cpp:4498-4502) */
+  const double pRefInv = 1.0 / c->pRef;
+  const double fb[3] = {pRefInv * c->prm.body_force[0], pRefInv * c->prm.body_force[1], pRefInv * c->prm.body_force[2]};
   const double *dfn = sp >= 0 ? c->dampFn + (size_t) sp * 3 * nInt : NULL;
   const double *dst = sp >= 0 ? c->dampState + (size_t) sp * 5 * nInt : NULL;
   double ovs[3] = {0.0, 0.0, 0.0}, nsg[3] = {1.0, 1.0, 1.0};
@@ -809,6 +812,14 @@ static void volume_residual(orc_case *c, int e, int mon, double *mach2max, doubl
     }
     /* fluxes (ElementClass.cpp:4314-4416) */
     const double weight = -vm[m] * c->w3[m];
+    if (c->prm.body_force_set) {
+      /* body force source -w J (f, u.f) on the momentum and energy equations
+         (ElementClass.cpp:4504-4523) */
+      Sbf[m] = weight * fb[0];
+      Sbf[nInt + m] = weight * fb[1];
+      Sbf[2 * nInt + m] = weight * fb[2];
+      Sbf[3 * nInt + m] = weight * (u * fb[0] + v * fb[1] + ww * fb[2]);
+    }
     const double divV = LAMBDA_OVER_MU * (dudx + dvdy + dwdz);
     const double txx = muT * (2.0 * dudx + divV), tyy = muT * (2.0 * dvdy + divV), tzz = muT * (2.0 * dwdz + divV);
     const double txy = muT * (dudy + dvdx), txz = muT * (dudz + dwdx), tyz = muT * (dvdz + dwdy);
@@ -832,6 +843,8 @@ static void volume_residual(orc_case *c, int e, int mon, double *mach2max, doubl
   }
   if (sp >= 0)
     for (int l = 0; l < NVAR; ++l) tp3(M, M, M, K, K, K, c->AT, c->AT, c->AT, Ssp + l * nInt, res + l * nD, 1, w);
+  if (c->prm.body_force_set)
+    for (int l = 0; l < 4; ++l) tp3(M, M, M, K, K, K, c->AT, c->AT, c->AT, Sbf + l * nInt, res + (l + 1) * nD, 1, w);
 }
 
 /* Trace of sol (5 x K^3) and its parametric gradients on face f of an

@@ -61,6 +61,10 @@ typedef struct {
   int    compute_average;  /* "Compute average solution" */
   double free_stream_dir[3]; /* "Free stream velocity direction" */
   int    sgs_model;        /* "Subgrid scale model": ORC_SGS_* */
+  /* "Body force per unit volume (x,y,z)", dimensional; body_force_set = 0:
+     the keyword absent (InputParamClass.cpp:331-334) */
+  double body_force[3];
+  int    body_force_set;
 } orc_params;
 
 enum { ORC_SGS_NONE = 0, ORC_SGS_WALE = 1, ORC_SGS_VREMAN = 2 };

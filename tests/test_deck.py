@@ -198,3 +198,21 @@ def test_oracle_on_reference_duct_deck(oracle_lib):
     assert rel(avg[g["elements"]], g["sol_3"][..., 5:17]) < 1e-12
     # per-variable norms as one vector (the transverse momenta are ~1e-6 of it)
     assert rel(np.sqrt((sol ** 2).sum(axis=(0, 1))), g["sol_3_norms"][:5]) < 1e-12
+
+
+def test_body_force_keyword(tmp_path):
+    """"Body force per unit volume (x,y,z)" (InputParamClass.cpp:331-334):
+    present -> body_force_set with its three values, even when all are zero;
+    absent -> unset."""
+    text = open(STATIC).read()
+    (tmp_path / "grid").mkdir()
+    (tmp_path / "grid" / "Box_nPoly3_8x4x4.xyz").write_bytes(
+        open(os.path.join(GOLDEN, "ref_n3_8x4x4", "grid", "Box_nPoly3_8x4x4.xyz"), "rb").read())
+    assert "body force" not in text.lower()
+    d0 = DeckCase(STATIC).description()
+    assert d0["body_force_set"] == 0
+    for line, vals in (("Body force per unit volume (x,y,z): 12.5 -3.0 0.25", (12.5, -3.0, 0.25)),
+                       ("body force per unit volume (x,y,z):  0.0 0.0 0.0", (0.0, 0.0, 0.0))):
+        (tmp_path / "p.cfg").write_text(line + "\n" + text)
+        d = DeckCase(str(tmp_path / "p.cfg")).description()
+        assert d["body_force_set"] == 1 and tuple(d["body_force"]) == vals

@@ -99,6 +99,8 @@ class TGVCase:
     rel_penalty: float = 1.0
     name: str = "tgv"
     sgs: str = "none"             # "Subgrid scale model": none, wale or vreman
+    # "Body force per unit volume (x,y,z)", dimensional (None: keyword absent)
+    body_force: tuple | None = None
 
     def __post_init__(self):
         if self.rule is None:
@@ -214,6 +216,8 @@ class TGVCase:
             "Synchronization time step": self.dt_synchr,
             "CFL number": self.cfl,
             "Compute average solution": "no",
+            **({"Body force per unit volume (x,y,z)": [float(f) for f in self.body_force]}
+               if self.body_force is not None else {}),
         }
 
     def subfaces(self):
@@ -226,7 +230,9 @@ class TGVCase:
                     mach=self.mach, reynolds=self.reynolds, reynolds_length=1.0,
                     temp_free_stream=self.temp, theta_sym=self.theta_sym,
                     rel_penalty=self.rel_penalty, cfl=self.cfl, dt_synchr=self.dt_synchr,
-                    sgs_model=SGS_CODE[self.sgs])
+                    sgs_model=SGS_CODE[self.sgs],
+                    body_force=tuple(self.body_force) if self.body_force is not None else (0.0, 0.0, 0.0),
+                    body_force_set=int(self.body_force is not None))
 
     def write_inputs(self, directory: str) -> str:
         """Write grid, restart and parameter file; returns the parameter path."""
@@ -299,6 +305,8 @@ class OpenBoxCase:
     compute_average: bool = False    # running averages of the primitive variables
     name: str = "openbox"
     sgs: str = "none"                # "Subgrid scale model": none, wale or vreman
+    # "Body force per unit volume (x,y,z)", dimensional (None: keyword absent)
+    body_force: tuple | None = None
 
     def __post_init__(self):
         if self.rule is None:
@@ -433,7 +441,9 @@ class OpenBoxCase:
                  outlet_one_minus_beta=int(self.outlet_one_minus_beta), outlet_average_local=0,
                  inlet_len=self.inlet_len, inlet_sigma=self.inlet_sigma,
                  inlet_incoming=int(self.inlet_incoming), compute_average=int(self.compute_average),
-                 sgs_model=SGS_CODE[self.sgs])
+                 sgs_model=SGS_CODE[self.sgs],
+                 body_force=tuple(self.body_force) if self.body_force is not None else (0.0, 0.0, 0.0),
+                 body_force_set=int(self.body_force is not None))
         if self.sponge_elems > 0:
             d.update(sponge_layer=1, cml=int(self.cml),
                      sponge_elems=[0, self.sponge_elems, 0, 0, 0, 0],

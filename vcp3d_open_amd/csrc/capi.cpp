@@ -182,6 +182,8 @@ void params_from_desc(const vcp_case_desc *d, vcp::CaseParams &p)
   p.inlet_len = d->inlet_len; p.inlet_sigma = d->inlet_sigma; p.inlet_incoming = d->inlet_incoming;
   p.sponge_layer = d->sponge_layer; p.cml = d->cml;
   p.sgs_model = d->sgs_model;
+  p.body_force_set = d->body_force_set;
+  for (int i = 0; i < 3; ++i) p.body_force[i] = d->body_force_set ? d->body_force[i] : 0.0;
 }
 
 // Monitoring data of SolverClass::Residual (SolverClass.cpp:3754-3783) from
@@ -301,6 +303,10 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
   dc.bug_compat = p.bug_compat; dc.mu = c->rc.mu; dc.theta_sym = p.theta_sym; dc.rel_penalty = p.rel_penalty;
   dc.cfl = p.cfl; dc.u_ref = c->rc.u_ref; dc.p_ref = c->rc.p_ref; dc.rho_ref = c->rc.rho_ref;
   dc.sgs = p.sgs_model; dc.nAvg = dc.sgs ? 12 : 11;
+  // non-dimensional body force (ElementClass.cpp:4498-4502)
+  dc.fbodyOn = p.body_force_set != 0;
+  const double pRefInv = 1.0 / dc.p_ref;
+  for (int i = 0; i < 3; ++i) dc.fbody[i] = pRefInv * p.body_force[i];
 
   // table block
   const vcp::Tables &t = c->tab;
@@ -403,6 +409,7 @@ int vcp_create(const vcp_case_desc *d, const double *elem_grid, vcp_ctx **out)
     if (he == hipSuccess) he = hipMemcpy(dc.dampFn, sp.fn.data(), sp.fn.size() * sizeof(double), hipMemcpyHostToDevice);
     if (he == hipSuccess) he = hipMemset(dc.dampState, 0, ns5 * sizeof(double));
   }
+  if (he == hipSuccess && dc.fbodyOn) he = c->alloc(&dc.bfBuf, (size_t) nElem * 4 * nInt);
   if (he == hipSuccess) he = hipMemcpy(dc.tables, tb.data(), tb.size() * sizeof(double), hipMemcpyHostToDevice);
   if (he == hipSuccess) he = hipMemcpy(dc.nbr, g.nbr.data(), g.nbr.size() * sizeof(int), hipMemcpyHostToDevice);
   if (he != hipSuccess) return fail(std::string("device allocation/upload: ") + hipGetErrorString(he), -3);

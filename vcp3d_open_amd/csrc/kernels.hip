@@ -1210,8 +1210,28 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
 #pragma unroll
       for (int l = 0; l < 5; ++l) sb[l * NI + m] = src[l];
     }
+
   }
   __syncthreads();
+
+  if (a.bfBuf) {
+    // body force per unit volume (ElementClass.cpp:4492-4523): -w J f on the
+    // momentum and -w J u.f on the energy equation at this thread's points,
+    // with the velocities kept for the solve; projected with the sponge source
+    const double fx = a.fbody[0], fy = a.fbody[1], fz = a.fbody[2];
+    double *bb = a.bfBuf + (size_t) e * 4 * NI;
+#pragma unroll
+    for (int q = 0; q < (P1 > 0 ? 2 : 1); ++q) {
+      const int m = q == 0 ? tid : P0 + tid - T2;
+      if (!(q == 0 ? tid < P0 : (tid >= T2 && tid - T2 < P1))) continue;
+      const double weight = -(a.tables[args.to.w3 + m] * vg[m]);
+      const double u = prim[q][1], v = prim[q][2], w = prim[q][3];
+      bb[m] = weight * fx;
+      bb[NI + m] = weight * fy;
+      bb[2 * NI + m] = weight * fz;
+      bb[3 * NI + m] = weight * (u * fx + v * fy + w * fz);
+    }
+  }
 
   // the six face contributions at this thread's DOF (DOF-major: thread
   // d < ND owns DOF d of all five variables): each face slot holds the value
@@ -1306,13 +1326,22 @@ __global__ __launch_bounds__(NT, (VolShape<K, M, NT, T>::WAVES)) void volume_ker
     }
   }
   __syncthreads();
-  if (sp >= 0) {
-    // sponge element: source terms projected with the basis functions
-    const double *sb = a.srcBuf + (size_t) sp * 5 * NI;
+  if (sp >= 0 || a.bfBuf) {
+    // the sponge element's damping and CML sources and the body force
+    // (ElementClass.cpp:4492-4533, on the momentum and energy equations),
+    // both from the point pass, summed at the points and projected with the
+    // basis functions: rows along i, pencils along j, then along k,
+    // accumulated into the residual
+    const double *sb = sp >= 0 ? a.srcBuf + (size_t) sp * 5 * NI : nullptr;
+    const double *bb = a.bfBuf ? a.bfBuf + (size_t) e * 4 * NI : nullptr;
     for (int r = tid; r < 5 * M * M; r += NT) {       // rows along i
       T xx[M];
 #pragma unroll
-      for (int q = 0; q < M; ++q) xx[q] = T(sb[r * M + q]);
+      for (int q = 0; q < M; ++q) {
+        double x = sb ? sb[r * M + q] : 0.0;
+        if (bb && r >= M * M) x += bb[r * M + q - NI];
+        xx[q] = T(x);
+      }
 #pragma unroll
       for (int n = 0; n < K; ++n) {
         T s = 0;
@@ -2789,6 +2818,8 @@ CaseDev case_dev(const DeviceCase &dc)
   c.muDim = dc.mu * dc.p_ref / dc.u_ref;
   c.nElem = dc.nElem; c.cml = dc.cml;
   c.sgs = dc.sgs; c.vgeoN = dc.vgeoN; c.eddyPart = dc.sgs ? dc.eddyPart : nullptr;
+  c.bfBuf = dc.fbodyOn ? dc.bfBuf : nullptr;
+  for (int i = 0; i < 3; ++i) c.fbody[i] = dc.fbody[i];
   return c;
 }
 

@@ -50,6 +50,9 @@ struct DeviceCase {
   int mixed = 0;                  // 1: FP32 contractions and solve (vcp_set_precision)
   int sgs = kSgsNone;             // subgrid scale model
   int nAvg = 11;                  // averaged quantities per DOF (12 with an SGS model)
+  int fbodyOn = 0;                // body force per unit volume set
+  double fbody[3] = {0, 0, 0};    // its non-dimensional value
+  double *bfBuf = nullptr;        // [nElem][4][nInt] its point sources (fbodyOn)
   double mu = 0, theta_sym = 1, rel_penalty = 1;
   double cfl = 1, u_ref = 1, p_ref = 1, rho_ref = 1;
   TableOffsets to{};
@@ -123,6 +126,8 @@ struct CaseDev {
   double *srcBuf;
   double *eddyPart;               // eddy-viscosity maxima (null: no SGS model)
   double mu, tauFact, thetaHalf, cfl, u_ref, p_ref, rho_ref, muDim;
+  double *bfBuf;                  // body-force point sources [nElem][4][nInt] (null: none)
+  double fbody[3];                // non-dimensional body force
   int nElem, cml, sgs;
 };
 

@@ -50,6 +50,8 @@ struct CaseParams {
   int compute_average = 0;        // "Compute average solution"
   double reference_area = 1.0;    // "Reference area force coefficients"
   int sgs_model = 0;              // "Subgrid scale model": 0 None, 1 WALE, 2 Vreman
+  double body_force[3] = {0.0, 0.0, 0.0};   // "Body force per unit volume (x,y,z)", dimensional
+  int body_force_set = 0;
 };
 
 // Subgrid scale models (InputParamClass.cpp:241-250).

@@ -30,7 +30,7 @@ ReadRestartSolution replaces it when "Restart: yes", and AddFluctuations adds
 the pressure pulse while the time step is 0 (ElementClass.cpp:515-665).
 
 Only what the GPU path supports is accepted (entropy variables, Ismail-Roe, no
-SGS or wall model, no body force; block faces periodic, PC-NSCBC in/outflow,
+wall model; block faces periodic, PC-NSCBC in/outflow,
 subsonic total-condition inflow, subsonic outflow, supersonic inflow and
 outflow, heat-flux, isothermal or inviscid walls, farfield, one subface each);
 anything else raises DeckError naming the setting.
@@ -235,6 +235,9 @@ class DeckCase:
         self.monitor_forces = prm.yes("monitor forces", True)
         self.compute_average = prm.yes("compute average solution", False)
         self.sgs_model = SGS_CODES[prm.text("subgrid scale model", "none").lower()]
+        # "Body force per unit volume (x,y,z)" (InputParamClass.cpp:331-334):
+        # the keyword's presence switches the source term on
+        self.body_force = prm.vector("body force per unit volume (x,y,z)")
         # InputParamClass.cpp:599: no continuation without averaging
         self.continue_averaging = self.compute_average and prm.yes("continue averaging from restart", False)
         self.reference_area = prm.number("reference area force coefficients", 1.0)
@@ -265,9 +268,6 @@ class DeckCase:
             raise DeckError('Unknown "Subgrid scale model" specified (None, WALE or Vreman)')
         if prm.text("wall modeling", "no wall model").lower() != "no wall model":
             raise DeckError('"Wall modeling": only "No wall model" is supported on this path')
-        fb = prm.vector("body force per unit volume (x,y,z)", (0.0, 0.0, 0.0))
-        if any(v != 0.0 for v in fb):
-            raise DeckError('"Body force per unit volume (x,y,z)" is not supported on this path')
         if prm.yes("outlet use local (element) averaged mach number", False):
             raise DeckError('"Outlet use local (element) averaged Mach number: yes" is not supported')
 
@@ -365,6 +365,8 @@ class DeckCase:
                  dt_synchr=self.dt_synchr, bc=self.bc, bc_data=self.bc_data,
                  monitor_forces=int(self.monitor_forces), reference_area=self.reference_area,
                  compute_average=int(self.compute_average), sgs_model=self.sgs_model,
+                 body_force=tuple(self.body_force) if self.body_force is not None else (0.0, 0.0, 0.0),
+                 body_force_set=int(self.body_force is not None),
                  free_stream_dir=self.vel_dir if self.vel_dir is not None else (1.0, 0.0, 0.0), **self.nscbc)
         if self.sponge_layer or self.cml:
             d.update(sponge_layer=int(self.sponge_layer), cml=int(self.cml),

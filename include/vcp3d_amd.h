@@ -217,7 +217,7 @@ int vcp_set_graph(vcp_ctx *ctx, int enable);
    volume launches inside vcp_sync_step, from HIP events recorded before,
    between and after them on the context's stream. enable = 1 starts
    recording and clears the sums, 0 stops it; the graph path (vcp_set_graph)
-   is not timed. Get: mean milliseconds per face launch (three directions)
+   is not timed, nor are stages past the 1024th of one vcp_sync_step call. Get: mean milliseconds per face launch (three directions)
    and per volume launch, and the number of timed stages. */
 int vcp_set_kernel_timing(vcp_ctx *ctx, int enable);
 int vcp_get_kernel_timing(vcp_ctx *ctx, double ms[2], int *stages);

@@ -56,9 +56,10 @@ int check_flags(vcp_ctx *c)
 
 int residual_pass(vcp_ctx *c, const vcp::StageCtl &ctl)
 {
-  if (!c->kt.on || c->use_graph) return vcp::residual_pass(c, ctl, c->stream);
   // timed: the same launches with an event before, between and after the
-  // face and volume launches
+  // face and volume launches, for at most 1024 stages per call (the events
+  // are read back at the end of the call)
+  if (!c->kt.on || c->use_graph || c->kt.used >= 3 * 1024) return vcp::residual_pass(c, ctl, c->stream);
   if (c->prm.sponge_layer && !c->damping_set)
     return fail("sponge layer damping state not set (vcp_set_damping_state)");
   vcp_ctx::KernelTiming &kt = c->kt;
